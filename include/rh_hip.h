@@ -1,0 +1,87 @@
+/*
+ * rh_hip.h — the thin C ABI between the C++ host layer and the gfx950 HIP
+ * kernels (libray_hip.so).  Plain pointers and sizes only; all buffers use the
+ * layouts of rh_types.h.  Every function returns 0 on success or a negative
+ * status (the HIP error code, negated, or -1000-n for argument errors);
+ * rh_last_error() gives the message.
+ *
+ * What each entry point replaces in the reference's GPU backend
+ * (src/Ray/internal/RendererOCL.cpp, OpenCL):
+ *   rh_create / rh_destroy     <- Ocl::Renderer::Renderer device/program setup   RendererOCL.cpp:57-349
+ *   rh_resize                  <- Ocl::Renderer::Resize buffer allocation         RendererOCL.cpp:351-414
+ *   rh_clear                   <- Ocl::Renderer::Clear                            RendererOCL.h:172-178
+ *   rh_upload_scene            <- Ocl::Scene growable cl::Buffer uploads          SceneOCL.cpp:8-536, VectorOCL.h:71-76
+ *   rh_set_halton              <- Halton upload inside RenderScene                RendererOCL.cpp:449-454
+ *   rh_render_pass             <- Ocl::Renderer::RenderScene                      RendererOCL.cpp:429-640
+ *   rh_read_pixels             <- framebuffer readback                            RendererOCL.cpp:634
+ *   rh_get_stats/rh_reset_stats<- RendererBase::GetStats/ResetStats               RendererBase.h:93-102
+ * Parity entry points, one stage on caller-supplied rays (host buffers); the
+ * reference functions each one reproduces:
+ *   rh_gen_primary_rays        <- Ref::GeneratePrimaryRays                        CoreRef.cpp:427-505
+ *   rh_trace_rays              <- Ref::Traverse_MacroTree_WithStack_{ClosestHit,AnyHit}
+ *                                                                                 CoreRef.cpp:1091-1131, 1224-1268
+ *   rh_ray_hashes              <- get_ray_hash                                    CoreRef.cpp:316-333
+ *   rh_sort_rays               <- Ref::SortRays_CPU                               CoreRef.cpp:615-657
+ *   rh_shade_rays              <- Ref::ShadeSurface                               CoreRef.cpp:2055-2391
+ */
+#ifndef RH_HIP_H
+#define RH_HIP_H
+
+#include "rh_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rh_context rh_context;
+
+/* one RenderScene pass over a region (RendererRef.cpp:19-325) */
+typedef struct rh_pass_desc {
+    rh_camera cam;
+    int32_t rect[4];      /* x, y, w, h; must lie inside the framebuffer */
+    int32_t iteration;    /* region iteration after the increment (>= 1) */
+    int32_t sort_rays;    /* 1 = sort secondary rays by hash before each bounce trace */
+    float root_min[3];    /* macro-tree root box min (sort hash grid origin) */
+    float cell_size[3];   /* (root max - root min) / 255 */
+} rh_pass_desc;
+
+const char *rh_last_error(void);
+
+int rh_create(int device, rh_context **out);
+void rh_destroy(rh_context *ctx);
+int rh_resize(rh_context *ctx, int w, int h);
+int rh_clear(rh_context *ctx, const float rgba[4]);
+int rh_upload_scene(rh_context *ctx, const rh_scene_view *scene);
+int rh_set_halton(rh_context *ctx, const float *seq, size_t count);
+int rh_render_pass(rh_context *ctx, const rh_pass_desc *pass);
+int rh_read_pixels(rh_context *ctx, float *rgba, size_t count);
+int rh_synchronize(rh_context *ctx);
+int rh_get_stats(rh_context *ctx, rh_stats *out);
+int rh_reset_stats(rh_context *ctx);
+int rh_get_ray_counts(rh_context *ctx, rh_ray_counts *out);
+/* Knobs: "sort_rays" (0/1), "stage_timing" (0/1: hipEvent timers per stage),
+ * "trace_block" (threads per block of the trace kernel). */
+int rh_set_option(rh_context *ctx, const char *name, int value);
+int rh_get_option(rh_context *ctx, const char *name, int *value);
+
+/* ---- parity entry points (host buffers in, host buffers out) ---- */
+int rh_gen_primary_rays(rh_context *ctx, const rh_camera *cam, const int32_t rect[4], int w, int h, int iteration,
+                        const float *halton, rh_ray_packet *out);
+int rh_trace_rays(rh_context *ctx, const rh_ray_packet *rays, size_t n, int any_hit, const float *tmax, rh_hit *out);
+int rh_ray_hashes(rh_context *ctx, const rh_ray_packet *rays, size_t n, const float root_min[3],
+                  const float cell_size[3], uint32_t *out);
+int rh_sort_rays(rh_context *ctx, rh_ray_packet *rays, size_t n, const float root_min[3], const float cell_size[3]);
+/* Shades n (ray, hit) pairs, each with its own pass info; secondary rays are
+ * appended in arbitrary order, out_src[k] naming the input ray of out_secondary[k]. */
+int rh_shade_rays(rh_context *ctx, const rh_pass_info *pi, const rh_ray_packet *rays, const rh_hit *hits, size_t n,
+                  const float *halton, float *out_rgba, rh_ray_packet *out_secondary, int *out_count, int *out_src);
+
+/* ---- device-resident entry point for benchmarking the trace kernel ----
+ * rays/hits are device pointers in the split layout (rh_ray / rh_hit). */
+int rh_trace_device(rh_context *ctx, const rh_ray *d_rays, int n, int any_hit, rh_hit *d_hits, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RH_HIP_H */

@@ -1,0 +1,598 @@
+// Device-side building blocks of the HIP backend (gfx950).
+//
+// Every routine here reproduces the arithmetic of the Ray library's CPU Ref
+// backend (src/Ray/internal/CoreRef.cpp) operation for operation: the file is
+// compiled with -ffp-contract=off, so each float multiply and add rounds
+// separately in source order, divisions and square roots are IEEE correctly
+// rounded, and sums over vector components run left to right as the
+// reference's generic simd_vec does (simd_vec.h:319-327, 559-563).  Cosine,
+// sine, power and arc-cosine are evaluated in double precision and rounded to
+// float, which agrees with the host C library's float functions except in
+// rare last-bit cases.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rh_types.h"
+
+namespace rh {
+
+// ---------------------------------------------------------------- vectors
+struct f3 {
+    float x, y, z;
+};
+struct f2 {
+    float x, y;
+};
+struct f4 {
+    float x, y, z, w;
+};
+
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 ld3(const float *p) { return f3{p[0], p[1], p[2]}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator*(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return f3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) {
+    float r = 0.0f;
+    r += a.x * b.x;
+    r += a.y * b.y;
+    r += a.z * b.z;
+    return r;
+}
+__device__ __forceinline__ float length(f3 a) { return __builtin_sqrtf(dot(a, a)); }
+__device__ __forceinline__ f3 normalize(f3 a) { return a / length(a); }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+    return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float comp(f3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+// std::min / std::max argument order
+__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// std::modf fractional part for the finite values used here
+__device__ __forceinline__ float fract_modf(float x) { return x - __builtin_truncf(x); }
+// simd_vec floor (simd_vec.h:408-412): int truncation minus one for negatives
+__device__ __forceinline__ float simd_floor(float v) { return (float)((int)v - (v < 0.0f)); }
+
+__device__ __forceinline__ float rcos(float x) { return (float)cos((double)x); }
+__device__ __forceinline__ float rsin(float x) { return (float)sin((double)x); }
+__device__ __forceinline__ float racos(float x) { return (float)acos((double)x); }
+__device__ __forceinline__ float rpow(float x, float y) { return (float)pow((double)x, (double)y); }
+
+// hash (CoreRef.cpp:264-270)
+__device__ __forceinline__ int hash(int x) {
+    unsigned ret = (unsigned)x;
+    ret = ((ret >> 16) ^ ret) * 0x45d9f3bu;
+    ret = ((ret >> 16) ^ ret) * 0x45d9f3bu;
+    ret = (ret >> 16) ^ ret;
+    return (int)ret;
+}
+// construct_float (CoreRef.cpp:359-368)
+__device__ __forceinline__ float construct_float(uint32_t m) {
+    m &= 0x007FFFFFu;
+    m |= 0x3F800000u;
+    return __uint_as_float(m) - 1.0f;
+}
+// to_norm_float (Core.h:249-252)
+__device__ __forceinline__ float to_norm_float(uint32_t v) {
+    const uint32_t val = 0x3f800000u + v * 0x8080u + (v + 1u) / 2u;
+    return __uint_as_float(val) - 1.0f;
+}
+
+// ---------------------------------------------------------------- scene
+struct Scene {
+    const rh_bvh_node *nodes;
+    const rh_tri_accel *tris;
+    const uint32_t *tri_indices;
+    const rh_mesh *meshes;
+    const rh_mesh_instance *mesh_instances;
+    const uint32_t *mi_indices;
+    const rh_transform *transforms;
+    const rh_vertex *vertices;
+    const uint32_t *vtx_indices;
+    const rh_material *materials;
+    const rh_texture *textures;
+    const rh_tex_mip *tex_mips;
+    const uint32_t *texels;
+    const rh_light *lights;
+    const uint32_t *li_indices;
+    rh_environment env;
+    uint32_t macro_root;
+    uint32_t light_root;
+};
+
+struct Hit {
+    int mask, obj, prim;
+    float t, u, v;
+};
+
+// ---------------------------------------------------------------- traversal
+// bbox_test (CoreRef.cpp:64-86): slab test, tmax widened by 1.00000024
+__device__ __forceinline__ bool bbox_test(f3 o, f3 inv_d, float t, const float *bmin, const float *bmax) {
+    float lo_x = inv_d.x * (bmin[0] - o.x);
+    float hi_x = inv_d.x * (bmax[0] - o.x);
+    if (lo_x > hi_x) {
+        const float tmp = lo_x;
+        lo_x = hi_x;
+        hi_x = tmp;
+    }
+    float lo_y = inv_d.y * (bmin[1] - o.y);
+    float hi_y = inv_d.y * (bmax[1] - o.y);
+    if (lo_y > hi_y) {
+        const float tmp = lo_y;
+        lo_y = hi_y;
+        hi_y = tmp;
+    }
+    float lo_z = inv_d.z * (bmin[2] - o.z);
+    float hi_z = inv_d.z * (bmax[2] - o.z);
+    if (lo_z > hi_z) {
+        const float tmp = lo_z;
+        lo_z = hi_z;
+        hi_z = tmp;
+    }
+    float tmin = lo_x > lo_y ? lo_x : lo_y;
+    if (lo_z > tmin) tmin = lo_z;
+    float tmax = hi_x < hi_y ? hi_x : hi_y;
+    if (hi_z < tmax) tmax = hi_z;
+    tmax *= 1.00000024f;
+    return tmin <= tmax && tmin <= t && tmax > 0;
+}
+
+// safe_invert (CoreRef.cpp:272-294)
+__device__ __forceinline__ f3 safe_invert(f3 v) {
+    f3 o = f3{1.0f / v.x, 1.0f / v.y, 1.0f / v.z};
+    const float big = 3.402823466e+38f;
+    if (v.x <= RH_FLT_EPS && v.x >= 0) o.x = big;
+    else if (v.x >= -RH_FLT_EPS && v.x < 0) o.x = -big;
+    if (v.y <= RH_FLT_EPS && v.y >= 0) o.y = big;
+    else if (v.y >= -RH_FLT_EPS && v.y < 0) o.y = -big;
+    if (v.z <= RH_FLT_EPS && v.z >= 0) o.z = big;
+    else if (v.z >= -RH_FLT_EPS && v.z < 0) o.z = -big;
+    return o;
+}
+
+// _IntersectTri (CoreRef.cpp:11-42): projected-plane test with HIT_EPS edge tolerance
+__device__ __forceinline__ bool intersect_tri(f3 o, f3 d, const rh_tri_accel &tri, float cur_t, float &t_out,
+                                              float &u_out, float &v_out) {
+    const int iw = tri.ci & RH_TRI_W_BITS;
+    const int iu = iw == 0 ? 1 : 0;
+    const int iv = iw == 2 ? 1 : 2;
+    const float dw = comp(d, iw), du = comp(d, iu), dv = comp(d, iv);
+    const float ow = comp(o, iw), ou = comp(o, iu), ov = comp(o, iv);
+
+    const float det = du * tri.nu + dv * tri.nv + dw;
+    const float dett = tri.np - (ou * tri.nu + ov * tri.nv + ow);
+    const float Du = du * dett - (tri.pu - ou) * det;
+    const float Dv = dv * dett - (tri.pv - ov) * det;
+    const float detu = tri.e1v * Du - tri.e1u * Dv;
+    const float detv = tri.e0u * Dv - tri.e0v * Du;
+
+    const float tmpdet0 = det - detu - detv;
+    if ((tmpdet0 > -RH_HIT_EPS && detu > -RH_HIT_EPS && detv > -RH_HIT_EPS) ||
+        (tmpdet0 < RH_HIT_EPS && detu < RH_HIT_EPS && detv < RH_HIT_EPS)) {
+        const float rdet = 1.0f / det;
+        const float t = dett * rdet;
+        if (t > 0 && t < cur_t) {
+            t_out = t;
+            u_out = detu * rdet;
+            v_out = detv * rdet;
+            return true;
+        }
+    }
+    return false;
+}
+
+__device__ __forceinline__ rh_tri_accel load_tri(const rh_tri_accel *tris, uint32_t i) {
+    const float4 *p = reinterpret_cast<const float4 *>(tris + i);
+    const float4 a = p[0], b = p[1], c = p[2];
+    rh_tri_accel t;
+    t.nu = a.x;
+    t.nv = a.y;
+    t.np = a.z;
+    t.pu = a.w;
+    t.pv = b.x;
+    t.ci = __float_as_int(b.y);
+    t.e0u = b.z;
+    t.e0v = b.w;
+    t.e1u = c.x;
+    t.e1v = c.y;
+    t.mi = __float_as_uint(c.z);
+    t.back_mi = __float_as_uint(c.w);
+    return t;
+}
+
+struct NodeBox {
+    float bmin[3], bmax[3];
+    uint32_t a, b; // prim_index/left_child, prim_count/right_child
+};
+__device__ __forceinline__ NodeBox load_node(const rh_bvh_node *nodes, uint32_t i) {
+    const float4 *p = reinterpret_cast<const float4 *>(nodes + i);
+    const float4 lo = p[0], hi = p[1];
+    NodeBox n;
+    n.bmin[0] = lo.x;
+    n.bmin[1] = lo.y;
+    n.bmin[2] = lo.z;
+    n.a = __float_as_uint(lo.w);
+    n.bmax[0] = hi.x;
+    n.bmax[1] = hi.y;
+    n.bmax[2] = hi.z;
+    n.b = __float_as_uint(hi.w);
+    return n;
+}
+
+// Per-lane traversal stack: the first LDS_DEPTH entries live in LDS (lane-minor,
+// so the 64 lanes of a wave hit distinct banks), deeper entries spill to a
+// per-thread slice of global memory; pushes past LDS_DEPTH + SPILL_DEPTH are
+// dropped and counted (the reference's fixed 32-entry stacks, CoreRef.cpp:1099-1111,
+// have no overflow check at all).
+struct Stack {
+    uint32_t *lds;      // base of this block's LDS stack, [depth][blockDim]
+    uint32_t *spill;    // base of this thread's spill slice, stride = spill_stride
+    uint32_t spill_stride;
+    uint32_t lds_depth;
+    uint32_t spill_depth;
+    uint32_t lane;
+    uint32_t block;
+    uint32_t sp;
+    uint32_t overflow;
+
+    __device__ __forceinline__ void push(uint32_t v) {
+        if (sp < lds_depth) {
+            lds[sp * block + lane] = v;
+        } else if (sp < lds_depth + spill_depth) {
+            spill[(size_t)(sp - lds_depth) * spill_stride] = v;
+        } else {
+            overflow++;
+            return;
+        }
+        sp++;
+    }
+    __device__ __forceinline__ uint32_t pop() {
+        --sp;
+        if (sp < lds_depth) return lds[sp * block + lane];
+        return spill[(size_t)(sp - lds_depth) * spill_stride];
+    }
+};
+
+struct TraceCounters {
+    uint32_t nodes, leaf_tris, instances;
+};
+
+// Two-level closest-hit / any-hit traversal in one loop.  Visit order is the
+// reference's (far child pushed first, near child popped first; macro-leaf
+// instances in index order, each instance's micro tree walked to completion
+// before the next), so ties between equal-t triangles resolve identically:
+//   Traverse_MacroTree_WithStack_ClosestHit  CoreRef.cpp:1091-1131
+//   Traverse_MicroTree_WithStack_ClosestHit  CoreRef.cpp:1370-1393
+//   Traverse_MacroTree_WithStack_AnyHit      CoreRef.cpp:1224-1268
+//   Traverse_MicroTree_WithStack_AnyHit      CoreRef.cpp:1470-1498
+//   IntersectTris_ClosestHit / _AnyHit       CoreRef.cpp:760-825
+template <bool AnyHit>
+__device__ __forceinline__ void traverse(const Scene &sc, f3 o, f3 d, Hit &hit, Stack &st, TraceCounters *cnt) {
+    const f3 inv_d = safe_invert(d);
+    uint32_t leaf_cur = 0, leaf_end = 0;
+    bool in_micro = false;
+    uint32_t micro_base = 0;
+    f3 mo = o, md = d, minv = inv_d;
+    int mobj = -1;
+
+    st.sp = 0;
+    st.push(sc.macro_root);
+
+    while (true) {
+        if (in_micro) {
+            if (st.sp > micro_base) {
+                const uint32_t cur = st.pop();
+                const NodeBox n = load_node(sc.nodes, cur);
+                if (cnt) cnt->nodes++;
+                if (!bbox_test(mo, minv, hit.t, n.bmin, n.bmax)) continue;
+                if (!(n.a & RH_LEAF_NODE_BIT)) {
+                    const uint32_t axis = n.b >> 30;
+                    const uint32_t right = n.b & RH_RIGHT_CHILD_BITS, left = n.a;
+                    const bool neg = comp(md, axis) < 0;
+                    st.push(neg ? left : right); // far
+                    st.push(neg ? right : left); // near
+                } else {
+                    const uint32_t first = n.a & RH_PRIM_INDEX_BITS, count = n.b;
+                    bool leaf_hit = false;
+                    for (uint32_t k = first; k < first + count; k++) {
+                        const uint32_t ti = sc.tri_indices[k];
+                        const rh_tri_accel tri = load_tri(sc.tris, ti);
+                        if (cnt) cnt->leaf_tris++;
+                        float t, u, v;
+                        if (intersect_tri(mo, md, tri, hit.t, t, u, v)) {
+                            hit.mask = -1;
+                            hit.prim = (int)ti;
+                            hit.obj = mobj;
+                            hit.t = t;
+                            hit.u = u;
+                            hit.v = v;
+                            leaf_hit = true;
+                        }
+                        if (AnyHit && leaf_hit && (tri.ci & RH_TRI_SOLID_BIT)) break;
+                    }
+                    if (AnyHit && leaf_hit) {
+                        const rh_tri_accel ht = load_tri(sc.tris, (uint32_t)hit.prim);
+                        if (ht.ci & RH_TRI_SOLID_BIT) return; // solid occluder: done
+                    }
+                }
+                continue;
+            }
+            in_micro = false;
+            leaf_cur++;
+            continue;
+        }
+        if (leaf_cur < leaf_end) {
+            const uint32_t mi_index = sc.mi_indices[leaf_cur];
+            const rh_mesh_instance mi = sc.mesh_instances[mi_index];
+            if (cnt) cnt->instances++;
+            if (!bbox_test(o, inv_d, hit.t, mi.bbox_min, mi.bbox_max)) {
+                leaf_cur++;
+                continue;
+            }
+            const float *x = sc.transforms[mi.tr_index].inv_xform;
+            // TransformRay (CoreRef.cpp:1605-1617)
+            mo = f3{x[0] * o.x + x[4] * o.y + x[8] * o.z + x[12], x[1] * o.x + x[5] * o.y + x[9] * o.z + x[13],
+                    x[2] * o.x + x[6] * o.y + x[10] * o.z + x[14]};
+            md = f3{x[0] * d.x + x[4] * d.y + x[8] * d.z, x[1] * d.x + x[5] * d.y + x[9] * d.z,
+                    x[2] * d.x + x[6] * d.y + x[10] * d.z};
+            minv = f3{1.0f / md.x, 1.0f / md.y, 1.0f / md.z};
+            mobj = (int)mi_index;
+            micro_base = st.sp;
+            st.push(sc.meshes[mi.mesh_index].node_index);
+            in_micro = true;
+            continue;
+        }
+        if (st.sp == 0) break;
+        const uint32_t cur = st.pop();
+        const NodeBox n = load_node(sc.nodes, cur);
+        if (cnt) cnt->nodes++;
+        if (!bbox_test(o, inv_d, hit.t, n.bmin, n.bmax)) continue;
+        if (!(n.a & RH_LEAF_NODE_BIT)) {
+            const uint32_t axis = n.b >> 30;
+            const uint32_t right = n.b & RH_RIGHT_CHILD_BITS, left = n.a;
+            const bool neg = comp(d, axis) < 0;
+            st.push(neg ? left : right);
+            st.push(neg ? right : left);
+        } else {
+            leaf_cur = n.a & RH_PRIM_INDEX_BITS;
+            leaf_end = leaf_cur + n.b;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- textures
+// Texel fetch at reference atlas coordinates (x, y) on the atlas page where the
+// texture's mip block sits: the block's own compact storage holds it.
+__device__ __forceinline__ uint32_t atlas_get(const Scene &sc, uint32_t tex, int lod, const rh_texture &t, int x,
+                                              int y) {
+    const rh_tex_mip m = sc.tex_mips[tex * RH_NUM_MIP_LEVELS + lod];
+    int lx = x - (int)t.pos[lod][0], ly = y - (int)t.pos[lod][1];
+    lx = clampi(lx, 0, (int)m.bw - 1);
+    ly = clampi(ly, 0, (int)m.bh - 1);
+    return sc.texels[m.offset + (uint32_t)ly * m.bw + (uint32_t)lx];
+}
+
+__device__ __forceinline__ float texel_ch(uint32_t p, int c) { return (float)((p >> (8 * c)) & 0xffu); }
+
+// SampleBilinear (CoreRef.cpp:1652-1678) with TransformUV (CoreRef.cpp:1625-1632)
+__device__ __forceinline__ f4 sample_bilinear(const Scene &sc, uint32_t tex, f2 uvs, int lod) {
+    const rh_texture t = sc.textures[tex];
+    const float atlas = (float)RH_TEXTURE_ATLAS_SIZE;
+    const float posx = (float)t.pos[lod][0], posy = (float)t.pos[lod][1];
+    const float sizex = (float)(((uint32_t)t.width & RH_TEXTURE_WIDTH_BITS) >> lod);
+    const float sizey = (float)((uint32_t)t.height >> lod);
+    const float ux = uvs.x - simd_floor(uvs.x), uy = uvs.y - simd_floor(uvs.y);
+    float rx = posx + ux * sizex + 1.0f, ry = posy + uy * sizey + 1.0f;
+    rx /= atlas;
+    ry /= atlas;
+    rx = rx * atlas - 0.5f;
+    ry = ry * atlas - 0.5f;
+    const int ix = (int)rx, iy = (int)ry;
+    const uint32_t p00 = atlas_get(sc, tex, lod, t, ix + 0, iy + 0);
+    const uint32_t p01 = atlas_get(sc, tex, lod, t, ix + 1, iy + 0);
+    const uint32_t p10 = atlas_get(sc, tex, lod, t, ix + 0, iy + 1);
+    const uint32_t p11 = atlas_get(sc, tex, lod, t, ix + 1, iy + 1);
+    const float kx = rx - __builtin_floorf(rx), ky = ry - __builtin_floorf(ry);
+    float r[4];
+    const float k = 1.0f / 255.0f;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const float p0 = texel_ch(p01, c) * kx + texel_ch(p00, c) * (1 - kx);
+        const float p1 = texel_ch(p11, c) * kx + texel_ch(p10, c) * (1 - kx);
+        r[c] = (p1 * ky + p0 * (1.0f - ky)) * k;
+    }
+    return f4{r[0], r[1], r[2], r[3]};
+}
+
+// fast_log2 (CoreRef.cpp:391-400)
+__device__ __forceinline__ float fast_log2(float val) {
+    int32_t x = __float_as_int(val);
+    float log_2 = (float)(((x >> 23) & 255) - 128);
+    x &= ~(255 << 23);
+    x += 127 << 23;
+    const float v = __int_as_float(x);
+    log_2 += ((-0.34484843f) * v + 2.02466578f) * v - 0.67487759f;
+    return log_2;
+}
+
+// get_texture_lod (CoreRef.cpp:402-415)
+__device__ __forceinline__ float get_texture_lod(const rh_texture &t, f2 duv_dx, f2 duv_dy) {
+    const float szx = (float)((uint32_t)t.width & RH_TEXTURE_WIDTH_BITS), szy = (float)t.height;
+    const f2 dx = f2{duv_dx.x * szx, duv_dx.y * szy}, dy = f2{duv_dy.x * szx, duv_dy.y * szy};
+    const f2 dg = f2{dx.x + dy.x, dx.y + dy.y};
+    float l_dx = 0.0f, l_dy = 0.0f, l_dg = 0.0f;
+    l_dx += dx.x * dx.x;
+    l_dx += dx.y * dx.y;
+    l_dy += dy.x * dy.x;
+    l_dy += dy.y * dy.y;
+    l_dg += dg.x * dg.x;
+    l_dg += dg.y * dg.y;
+    const float min_length2 = smin(smin(l_dx, l_dy), l_dg);
+    float lod = fast_log2(min_length2);
+    lod = clampf(0.5f * lod - 1.0f, 0.0f, (float)RH_MAX_MIP_LEVEL);
+    return lod;
+}
+
+__device__ __forceinline__ f4 srgb_to_rgb(f4 c) {
+    float v[3] = {c.x, c.y, c.z};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (v[i] > 0.04045f) {
+            v[i] = rpow((v[i] + 0.055f) / 1.055f, 2.4f);
+        } else {
+            v[i] = v[i] / 12.92f;
+        }
+    }
+    return f4{v[0], v[1], v[2], c.w};
+}
+
+// SampleLatlong_RGBE (CoreRef.cpp:1772-1799)
+__device__ __forceinline__ f4 sample_latlong_rgbe(const Scene &sc, uint32_t tex, f3 dir) {
+    const rh_texture t = sc.textures[tex];
+    const float theta = racos(clampf(dir.y, -1.0f, 1.0f)) / RH_PI;
+    const float r = __builtin_sqrtf(dir.x * dir.x + dir.z * dir.z);
+    float u = 0.5f * racos(r > RH_FLT_EPS ? clampf(dir.x / r, -1.0f, 1.0f) : 0.0f) / RH_PI;
+    if (dir.z < 0.0f) u = 1.0f - u;
+    const float posx = (float)t.pos[0][0], posy = (float)t.pos[0][1];
+    const float sizex = (float)((uint32_t)t.width & RH_TEXTURE_WIDTH_BITS), sizey = (float)t.height;
+    const float ux = posx + u * sizex + 1.0f, uy = posy + theta * sizey + 1.0f;
+    const uint32_t p00 = atlas_get(sc, tex, 0, t, (int)(ux + 0), (int)(uy + 0));
+    const uint32_t p01 = atlas_get(sc, tex, 0, t, (int)(ux + 1), (int)(uy + 0));
+    const uint32_t p10 = atlas_get(sc, tex, 0, t, (int)(ux + 0), (int)(uy + 1));
+    const uint32_t p11 = atlas_get(sc, tex, 0, t, (int)(ux + 1), (int)(uy + 1));
+    const float kx = ux - simd_floor(ux), ky = uy - simd_floor(uy);
+    auto rgbe = [](uint32_t p, float *o) {
+        const float f = exp2f((float)((p >> 24) & 0xffu) - 128.0f);
+        o[0] = to_norm_float(p & 0xffu) * f;
+        o[1] = to_norm_float((p >> 8) & 0xffu) * f;
+        o[2] = to_norm_float((p >> 16) & 0xffu) * f;
+        o[3] = 1.0f;
+    };
+    float a00[4], a01[4], a10[4], a11[4], res[4];
+    rgbe(p00, a00);
+    rgbe(p01, a01);
+    rgbe(p10, a10);
+    rgbe(p11, a11);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const float p0 = a01[c] * kx + a00[c] * (1 - kx);
+        const float p1 = a11[c] * kx + a10[c] * (1 - kx);
+        res[c] = p1 * ky + p0 * (1 - ky);
+    }
+    return f4{res[0], res[1], res[2], res[3]};
+}
+
+// ---------------------------------------------------------------- shading
+// ExtractPlaneNormal (Core.cpp:216-228); note the reference scales by the
+// length rather than dividing by it
+__device__ __forceinline__ f3 extract_plane_normal(const rh_tri_accel &tri) {
+    float n[3];
+    const int iw = tri.ci & RH_TRI_W_BITS;
+    const int iu = iw == 0 ? 1 : 0, iv = iw == 2 ? 1 : 2;
+    n[iw] = 1.0f;
+    n[iu] = tri.nu;
+    n[iv] = tri.nv;
+    float inv_l = __builtin_sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (tri.ci & RH_TRI_INV_NORMAL_BIT) inv_l = -inv_l;
+    return f3{n[0] * inv_l, n[1] * inv_l, n[2] * inv_l};
+}
+
+// TransformNormal (CoreRef.cpp:1625-1629)
+__device__ __forceinline__ f3 transform_normal(f3 n, const float *inv) {
+    return f3{inv[0] * n.x + inv[1] * n.y + inv[2] * n.z, inv[4] * n.x + inv[5] * n.y + inv[6] * n.z,
+              inv[8] * n.x + inv[9] * n.y + inv[10] * n.z};
+}
+
+// BRDF_OrenNayar (CoreRef.cpp:1579-1603)
+__device__ __forceinline__ float brdf_oren_nayar(f3 L, f3 I, f3 N, f3 T, float sigma) {
+    const float sigma_sqr = sigma * sigma;
+    const float A = 1.0f - (sigma_sqr / (2.0f * (sigma_sqr + 0.33f)));
+    const float B = (0.45f * sigma_sqr) / (sigma_sqr + 0.09f);
+    const float cos_theta_I = dot(L, N);
+    const float cos_theta_O = -dot(I, N);
+    const float sin_theta_I = __builtin_sqrtf(smax(0.0f, 1.0f - (cos_theta_I * cos_theta_I)));
+    const float sin_theta_O = __builtin_sqrtf(smax(0.0f, 1.0f - (cos_theta_O * cos_theta_O)));
+    const float cos_phi_I = dot(L, T);
+    const float cos_phi_O = -dot(I, T);
+    const float sin_phi_I = __builtin_sqrtf(smax(0.0f, 1.0f - (cos_phi_I * cos_phi_I)));
+    const float sin_phi_O = __builtin_sqrtf(smax(0.0f, 1.0f - (cos_phi_O * cos_phi_O)));
+    const float temp = smax(0.0f, (cos_phi_I * cos_phi_O) + (sin_phi_I * sin_phi_O));
+    float sin_alpha, tan_beta;
+    if (cos_theta_I > cos_theta_O) {
+        sin_alpha = sin_theta_O;
+        tan_beta = (sin_theta_I / cos_theta_I);
+    } else {
+        sin_alpha = sin_theta_I;
+        tan_beta = (sin_theta_O / cos_theta_O);
+    }
+    const float result = (A + (B * temp * sin_alpha * tan_beta));
+    return clampf(result, 0.0f, 1.0f);
+}
+
+struct Derivatives {
+    f3 do_dx, do_dy, dd_dx, dd_dy;
+    f2 duv_dx, duv_dy;
+    f3 dndx, dndy;
+    float ddn_dx, ddn_dy;
+};
+
+// ComputeDerivatives (CoreRef.cpp:1993-2052)
+__device__ __forceinline__ void compute_derivatives(f3 I, float t, f3 do_dx, f3 do_dy, f3 dd_dx, f3 dd_dy,
+                                                    const rh_vertex &v1, const rh_vertex &v2, const rh_vertex &v3,
+                                                    f3 plane_N, Derivatives &out) {
+    const float dot_I_N = -dot(I, plane_N);
+    const float inv_dot = fabsf(dot_I_N) < RH_FLT_EPS ? 0.0f : 1.0f / dot_I_N;
+    const float dt_dx = dot(do_dx + t * dd_dx, plane_N) * inv_dot;
+    const float dt_dy = dot(do_dy + t * dd_dy, plane_N) * inv_dot;
+
+    out.do_dx = (do_dx + t * dd_dx) + dt_dx * I;
+    out.do_dy = (do_dy + t * dd_dy) + dt_dy * I;
+    out.dd_dx = dd_dx;
+    out.dd_dy = dd_dy;
+
+    const f2 duv13 = f2{v1.t[0][0] - v3.t[0][0], v1.t[0][1] - v3.t[0][1]};
+    const f2 duv23 = f2{v2.t[0][0] - v3.t[0][0], v2.t[0][1] - v3.t[0][1]};
+    const f3 dp13 = ld3(v1.p) - ld3(v3.p), dp23 = ld3(v2.p) - ld3(v3.p);
+
+    const float det_uv = duv13.x * duv23.y - duv13.y * duv23.x;
+    const float inv_det_uv = fabsf(det_uv) < RH_FLT_EPS ? 0 : 1.0f / det_uv;
+    const f3 dpdu = (duv23.y * dp13 - duv13.y * dp23) * inv_det_uv;
+    const f3 dpdv = ((-duv23.x) * dp13 + duv13.x * dp23) * inv_det_uv;
+
+    f2 A0 = f2{dpdu.x, dpdv.x}, A1 = f2{dpdu.y, dpdv.y};
+    f2 Bx = f2{out.do_dx.x, out.do_dx.y};
+    f2 By = f2{out.do_dy.x, out.do_dy.y};
+    if (fabsf(plane_N.x) > fabsf(plane_N.y) && fabsf(plane_N.x) > fabsf(plane_N.z)) {
+        A0 = f2{dpdu.y, dpdv.y};
+        A1 = f2{dpdu.z, dpdv.z};
+        Bx = f2{out.do_dx.y, out.do_dx.z};
+        By = f2{out.do_dy.y, out.do_dy.z};
+    } else if (fabsf(plane_N.y) > fabsf(plane_N.z)) {
+        A1 = f2{dpdu.z, dpdv.z};
+        Bx = f2{out.do_dx.x, out.do_dx.z};
+        By = f2{out.do_dy.x, out.do_dy.z};
+    }
+    const float det = A0.x * A1.y - A0.y * A1.x;
+    const float inv_det = fabsf(det) < RH_FLT_EPS ? 0 : 1.0f / det;
+    out.duv_dx = f2{(A1.y * Bx.x - A0.y * Bx.y) * inv_det, (A0.x * Bx.y - A1.x * Bx.x) * inv_det};
+    out.duv_dy = f2{(A1.y * By.x - A0.y * By.y) * inv_det, (A0.x * By.y - A1.x * By.x) * inv_det};
+
+    const f3 dn1 = ld3(v1.n) - ld3(v3.n), dn2 = ld3(v2.n) - ld3(v3.n);
+    const f3 dndu = (duv23.y * dn1 - duv13.y * dn2) * inv_det_uv;
+    const f3 dndv = ((-duv23.x) * dn1 + duv13.x * dn2) * inv_det_uv;
+    out.dndx = dndu * out.duv_dx.x + dndv * out.duv_dx.y;
+    out.dndy = dndu * out.duv_dy.x + dndv * out.duv_dy.y;
+    out.ddn_dx = dot(dd_dx, plane_N) + dot(I, out.dndx);
+    out.ddn_dy = dot(dd_dy, plane_N) + dot(I, out.dndy);
+}
+
+} // namespace rh

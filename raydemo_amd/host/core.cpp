@@ -1,0 +1,678 @@
+// Host-side scene preprocessing; see core.h.  Each routine follows the
+// reference routine cited above it step for step, because the BVH node arrays,
+// triangle records and index orders must come out identical to the
+// reference's: the GPU walks them in the reference's order and must report the
+// same hit primitive on identical rays.  Compiled with -ffp-contract=off.
+#include "core.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <functional>
+#include <limits>
+#include <random>
+
+namespace RayHip {
+
+namespace {
+const float axis_aligned_normal_eps = 0.000001f; // Core.cpp:13
+
+inline vec3 cross3(const vec3 &a, const vec3 &b) {
+    return {{a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]}};
+}
+// sequential sums, as the generic simd_vec<float, 3> (simd_vec.h:319-327, 559-563)
+inline float dot3(const vec3 &a, const vec3 &b) {
+    float r = 0;
+    r += a[0] * b[0];
+    r += a[1] * b[1];
+    r += a[2] * b[2];
+    return r;
+}
+inline float length3(const vec3 &a) { return std::sqrt(dot3(a, a)); }
+inline vec3 div3(const vec3 &a, float s) { return {{a[0] / s, a[1] / s, a[2] / s}}; }
+inline vec3 normalize3(const vec3 &a) { return div3(a, length3(a)); }
+inline vec3 load3(const float *p) { return {{p[0], p[1], p[2]}}; }
+
+// bbox_t (BVHSplit.cpp:8-23): "surface area" is 2 * (dx + dy + dz)
+struct bbox_t {
+    vec3 min = {{std::numeric_limits<float>::max(), std::numeric_limits<float>::max(), std::numeric_limits<float>::max()}};
+    vec3 max = {{std::numeric_limits<float>::lowest(), std::numeric_limits<float>::lowest(),
+                 std::numeric_limits<float>::lowest()}};
+    bbox_t() = default;
+    bbox_t(const vec3 &a, const vec3 &b) : min(a), max(b) {}
+    float surface_area() const { return surface_area(min, max); }
+    static float surface_area(const vec3 &mn, const vec3 &mx) {
+        float d0 = mx[0] - mn[0], d1 = mx[1] - mn[1], d2 = mx[2] - mn[2];
+        return 2 * (d0 + d1 + d2);
+    }
+};
+
+struct split_data_t {
+    std::vector<uint32_t> left_indices, right_indices;
+    bbox_t left_bounds, right_bounds;
+};
+
+// SplitPrimitives_SAH (BVHSplit.cpp:126-390), object splits: full sorted sweep
+// along each axis, primitives ordered by bbox max with std::sort.
+split_data_t SplitPrimitives_SAH(const prim_t *primitives, const std::vector<uint32_t> &prim_indices,
+                                 const vec3 &bbox_min, const vec3 &bbox_max, const bvh_settings_t &s) {
+    const size_t num_tris = prim_indices.size();
+    const bbox_t whole_box(bbox_min, bbox_max);
+
+    std::vector<uint32_t> axis_lists[3];
+    for (int axis = 0; axis < 3; axis++) {
+        axis_lists[axis].resize(num_tris);
+        for (uint32_t i = 0; i < (uint32_t)num_tris; i++) axis_lists[axis][i] = i;
+    }
+
+    std::vector<bbox_t> right_bounds(num_tris);
+
+    float res_sah = s.oversplit_threshold * whole_box.surface_area() * num_tris;
+    int div_axis = -1;
+    uint32_t div_index = 0;
+    bbox_t res_left_bounds, res_right_bounds;
+
+    for (int axis = 0; axis < 3; axis++) {
+        std::vector<uint32_t> &list = axis_lists[axis];
+        std::sort(list.begin(), list.end(), [axis, primitives, &prim_indices](uint32_t p1, uint32_t p2) -> bool {
+            return primitives[prim_indices[p1]].bbox_max[axis] < primitives[prim_indices[p2]].bbox_max[axis];
+        });
+
+        bbox_t cur_right_bounds;
+        for (size_t i = list.size() - 1; i > 0; i--) {
+            const prim_t &p = primitives[prim_indices[list[i]]];
+            cur_right_bounds.min = vmin(cur_right_bounds.min, p.bbox_min);
+            cur_right_bounds.max = vmax(cur_right_bounds.max, p.bbox_max);
+            right_bounds[i - 1] = cur_right_bounds;
+        }
+
+        bbox_t left_bounds;
+        for (size_t i = 1; i < list.size(); i++) {
+            const prim_t &p = primitives[prim_indices[list[i - 1]]];
+            left_bounds.min = vmin(left_bounds.min, p.bbox_min);
+            left_bounds.max = vmax(left_bounds.max, p.bbox_max);
+
+            const float sah = s.node_traversal_cost * whole_box.surface_area() + left_bounds.surface_area() * i +
+                              right_bounds[i - 1].surface_area() * (list.size() - i);
+            if (sah < res_sah) {
+                res_sah = sah;
+                div_axis = axis;
+                div_index = (uint32_t)i;
+                res_left_bounds = left_bounds;
+                res_right_bounds = right_bounds[i - 1];
+            }
+        }
+    }
+
+    split_data_t out;
+    if (div_axis != -1) {
+        out.left_indices.reserve(div_index);
+        out.right_indices.reserve(prim_indices.size() - div_index);
+        for (size_t i = 0; i < div_index; i++) out.left_indices.push_back(prim_indices[axis_lists[div_axis][i]]);
+        for (size_t i = div_index; i < axis_lists[div_axis].size(); i++)
+            out.right_indices.push_back(prim_indices[axis_lists[div_axis][i]]);
+    } else {
+        out.left_indices = prim_indices;
+        res_left_bounds = whole_box;
+    }
+    out.left_bounds = res_left_bounds;
+    out.right_bounds = res_right_bounds;
+    return out;
+}
+} // namespace
+
+bool PreprocessTri(const float *p, int stride, rh_tri_accel *acc) {
+    if (!stride) stride = 3;
+
+    float e0[3] = {p[stride] - p[0], p[stride + 1] - p[1], p[stride + 2] - p[2]},
+          e1[3] = {p[stride * 2] - p[0], p[stride * 2 + 1] - p[1], p[stride * 2 + 2] - p[2]};
+
+    float n[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2], e0[0] * e1[1] - e0[1] * e1[0]};
+
+    int w = 2, u = 0, v = 1;
+    if (std::abs(n[0]) >= std::abs(n[1]) && std::abs(n[0]) >= std::abs(n[2])) {
+        w = 0;
+        u = 1;
+        v = 2;
+    } else if (std::abs(n[1]) >= std::abs(n[0]) && std::abs(n[1]) >= std::abs(n[2])) {
+        w = 1;
+        u = 0;
+        v = 2;
+    }
+
+    if (std::abs(n[w]) > RH_FLT_EPS) {
+        acc->nu = n[u] / n[w];
+        acc->nv = n[v] / n[w];
+        acc->pu = p[u];
+        acc->pv = p[v];
+        acc->np = acc->nu * acc->pu + acc->nv * acc->pv + p[w];
+
+        const float sign = (w == 1) ? -1.0f : 1.0f;
+        acc->e0u = sign * e0[u] / n[w];
+        acc->e0v = sign * e0[v] / n[w];
+        acc->e1u = sign * e1[u] / n[w];
+        acc->e1v = sign * e1[v] / n[w];
+
+        acc->ci = w;
+        if (std::abs(acc->nu) < axis_aligned_normal_eps && std::abs(acc->nv) < axis_aligned_normal_eps) {
+            acc->ci |= RH_TRI_AXIS_ALIGNED_BIT;
+        }
+        if (n[w] < 0) acc->ci |= RH_TRI_INV_NORMAL_BIT;
+        return true;
+    }
+    // InvalidTriangle (Core.cpp:95-103)
+    const float qnan = std::numeric_limits<float>::quiet_NaN();
+    acc->nu = acc->nv = acc->np = acc->pu = acc->pv = qnan;
+    acc->ci = 0;
+    acc->e0u = acc->e0v = acc->e1u = acc->e1v = qnan;
+    acc->mi = acc->back_mi = 0xffffffffu;
+    return false;
+}
+
+void ExtractPlaneNormal(const rh_tri_accel &tri, float *out_normal) {
+    static const int next_u[] = {1, 0, 0}, next_v[] = {2, 2, 1};
+    const int iw = tri.ci & RH_TRI_W_BITS;
+    out_normal[iw] = 1.0f;
+    out_normal[next_u[iw]] = tri.nu;
+    out_normal[next_v[iw]] = tri.nv;
+    float inv_l = std::sqrt(out_normal[0] * out_normal[0] + out_normal[1] * out_normal[1] + out_normal[2] * out_normal[2]);
+    if (tri.ci & RH_TRI_INV_NORMAL_BIT) inv_l = -inv_l;
+    out_normal[0] *= inv_l;
+    out_normal[1] *= inv_l;
+    out_normal[2] *= inv_l;
+}
+
+uint32_t PreprocessMesh(const float *attrs, const uint32_t *vtx_indices, size_t vtx_indices_count, int layout,
+                        int base_vertex, const bvh_settings_t &s, std::vector<rh_bvh_node> &out_nodes,
+                        std::vector<rh_tri_accel> &out_tris, std::vector<uint32_t> &out_tri_indices) {
+    static const size_t attr_strides[] = {8, 10, 11, 13};
+    std::vector<prim_t> primitives;
+    primitives.reserve(vtx_indices_count / 3);
+
+    const size_t tris_start = out_tris.size();
+    const size_t tris_count = vtx_indices_count / 3;
+    out_tris.resize(tris_start + tris_count);
+
+    const float *positions = attrs;
+    const size_t attr_stride = attr_strides[layout];
+
+    for (size_t j = 0; j < vtx_indices_count; j += 3) {
+        float p[9];
+        const uint32_t i0 = vtx_indices[j + 0] + base_vertex, i1 = vtx_indices[j + 1] + base_vertex,
+                       i2 = vtx_indices[j + 2] + base_vertex;
+        memcpy(&p[0], &positions[i0 * attr_stride], 3 * sizeof(float));
+        memcpy(&p[3], &positions[i1 * attr_stride], 3 * sizeof(float));
+        memcpy(&p[6], &positions[i2 * attr_stride], 3 * sizeof(float));
+
+        PreprocessTri(&p[0], 0, &out_tris[tris_start + j / 3]);
+
+        const vec3 p0 = load3(&p[0]), p1 = load3(&p[3]), p2 = load3(&p[6]);
+        primitives.push_back({i0, i1, i2, vmin(p0, vmin(p1, p2)), vmax(p0, vmax(p1, p2))});
+    }
+
+    const size_t indices_start = out_tri_indices.size();
+    const uint32_t num_out_nodes =
+        PreprocessPrims_SAH(primitives.data(), primitives.size(), positions, attr_stride, s, out_nodes, out_tri_indices);
+    for (size_t i = indices_start; i < out_tri_indices.size(); i++) out_tri_indices[i] += (uint32_t)tris_start;
+    return num_out_nodes;
+}
+
+uint32_t PreprocessPrims_SAH(const prim_t *prims, size_t prims_count, const float * /*positions*/, size_t /*stride*/,
+                             const bvh_settings_t &s, std::vector<rh_bvh_node> &out_nodes,
+                             std::vector<uint32_t> &out_indices) {
+    struct prims_coll_t {
+        std::vector<uint32_t> indices;
+        vec3 min = {{std::numeric_limits<float>::max(), std::numeric_limits<float>::max(),
+                     std::numeric_limits<float>::max()}};
+        vec3 max = {{std::numeric_limits<float>::lowest(), std::numeric_limits<float>::lowest(),
+                     std::numeric_limits<float>::lowest()}};
+        prims_coll_t() = default;
+        prims_coll_t(std::vector<uint32_t> &&ind, const vec3 &mn, const vec3 &mx)
+            : indices(std::move(ind)), min(mn), max(mx) {}
+    };
+
+    // breadth-first: new lists go to the front, the oldest is taken from the back
+    std::deque<prims_coll_t> prim_lists;
+    prim_lists.emplace_back();
+
+    size_t num_nodes = out_nodes.size();
+    const uint32_t root_node_index = (uint32_t)num_nodes;
+
+    for (uint32_t j = 0; j < (uint32_t)prims_count; j++) {
+        prim_lists.back().indices.push_back(j);
+        prim_lists.back().min = vmin(prim_lists.back().min, prims[j].bbox_min);
+        prim_lists.back().max = vmax(prim_lists.back().max, prims[j].bbox_max);
+    }
+
+    while (!prim_lists.empty()) {
+        split_data_t split =
+            SplitPrimitives_SAH(prims, prim_lists.back().indices, prim_lists.back().min, prim_lists.back().max, s);
+        prim_lists.pop_back();
+
+        if (split.right_indices.empty()) {
+            rh_bvh_node n;
+            n.prim_index = RH_LEAF_NODE_BIT + (uint32_t)out_indices.size();
+            n.prim_count = (uint32_t)split.left_indices.size();
+            memcpy(n.bbox_min, split.left_bounds.min.v, 3 * sizeof(float));
+            memcpy(n.bbox_max, split.left_bounds.max.v, 3 * sizeof(float));
+            out_nodes.push_back(n);
+            out_indices.insert(out_indices.end(), split.left_indices.begin(), split.left_indices.end());
+        } else {
+            const uint32_t index = (uint32_t)num_nodes;
+
+            uint32_t space_axis = 0;
+            const vec3 &lb0 = split.left_bounds.min, &lb1 = split.left_bounds.max;
+            const vec3 &rb0 = split.right_bounds.min, &rb1 = split.right_bounds.max;
+            const vec3 c_left = {{(lb0[0] + lb1[0]) / 2, (lb0[1] + lb1[1]) / 2, (lb0[2] + lb1[2]) / 2}};
+            const vec3 c_right = {{(rb0[0] + rb1[0]) / 2, (rb0[1] + rb1[1]) / 2, (rb0[2] + rb1[2]) / 2}};
+            const vec3 dist = {{std::abs(c_left[0] - c_right[0]), std::abs(c_left[1] - c_right[1]),
+                                std::abs(c_left[2] - c_right[2])}};
+            if (dist[0] > dist[1] && dist[0] > dist[2]) {
+                space_axis = 0;
+            } else if (dist[1] > dist[0] && dist[1] > dist[2]) {
+                space_axis = 1;
+            } else {
+                space_axis = 2;
+            }
+
+            const vec3 bmin = vmin(lb0, rb0), bmax = vmax(lb1, rb1);
+            rh_bvh_node n;
+            n.prim_index = index + 1;
+            n.prim_count = (space_axis << 30) + index + 2;
+            memcpy(n.bbox_min, bmin.v, 3 * sizeof(float));
+            memcpy(n.bbox_max, bmax.v, 3 * sizeof(float));
+            out_nodes.push_back(n);
+
+            prim_lists.emplace_front(std::move(split.left_indices), lb0, lb1);
+            prim_lists.emplace_front(std::move(split.right_indices), rb0, rb1);
+            num_nodes += 2;
+        }
+    }
+    return (uint32_t)(out_nodes.size() - root_node_index);
+}
+
+void ConstructCamera(int type, int filter, int dtype, const float origin[3], const float fwd[3], const float up[3],
+                     float fov, float gamma, float focus_distance, float focus_factor, rh_camera *cam) {
+    if (type == RH_CAM_PERSP) {
+        vec3 o = load3(origin), f = load3(fwd), u = load3(up);
+        if (dot3(u, u) < RH_FLT_EPS) {
+            if (std::abs(f[1]) >= 0.999f) {
+                u = {{1.0f, 0.0f, 0.0f}};
+            } else {
+                u = {{0.0f, 1.0f, 0.0f}};
+            }
+        }
+        const vec3 s = normalize3(cross3(f, u));
+        u = cross3(s, f);
+
+        cam->type = type;
+        cam->filter = filter;
+        cam->dtype = dtype;
+        cam->fov = fov;
+        cam->gamma = gamma;
+        cam->focus_distance = focus_distance;
+        cam->focus_factor = focus_factor;
+        memcpy(cam->origin, o.v, 3 * sizeof(float));
+        memcpy(cam->fwd, f.v, 3 * sizeof(float));
+        memcpy(cam->side, s.v, 3 * sizeof(float));
+        memcpy(cam->up, u.v, 3 * sizeof(float));
+    }
+}
+
+void TransformBoundingBox(const float bbox_min[3], const float bbox_max[3], const float *xform, float out_min[3],
+                          float out_max[3]) {
+    out_min[0] = out_max[0] = xform[12];
+    out_min[1] = out_max[1] = xform[13];
+    out_min[2] = out_max[2] = xform[14];
+    for (int j = 0; j < 3; j++) {
+        for (int i = 0; i < 3; i++) {
+            const float a = xform[i * 4 + j] * bbox_min[i];
+            const float b = xform[i * 4 + j] * bbox_max[i];
+            if (a < b) {
+                out_min[j] += a;
+                out_max[j] += b;
+            } else {
+                out_min[j] += b;
+                out_max[j] += a;
+            }
+        }
+    }
+}
+
+// Cofactor expansion with the reference's term order (Core.cpp:909-950): the
+// inverse feeds the object-space rays, so it must round identically.
+void InverseMatrix(const float m[16], float o[16]) {
+    const float A2323 = m[10] * m[15] - m[11] * m[14];
+    const float A1323 = m[9] * m[15] - m[11] * m[13];
+    const float A1223 = m[9] * m[14] - m[10] * m[13];
+    const float A0323 = m[8] * m[15] - m[11] * m[12];
+    const float A0223 = m[8] * m[14] - m[10] * m[12];
+    const float A0123 = m[8] * m[13] - m[9] * m[12];
+    const float A2313 = m[6] * m[15] - m[7] * m[14];
+    const float A1313 = m[5] * m[15] - m[7] * m[13];
+    const float A1213 = m[5] * m[14] - m[6] * m[13];
+    const float A2312 = m[6] * m[11] - m[7] * m[10];
+    const float A1312 = m[5] * m[11] - m[7] * m[9];
+    const float A1212 = m[5] * m[10] - m[6] * m[9];
+    const float A0313 = m[4] * m[15] - m[7] * m[12];
+    const float A0213 = m[4] * m[14] - m[6] * m[12];
+    const float A0312 = m[4] * m[11] - m[7] * m[8];
+    const float A0212 = m[4] * m[10] - m[6] * m[8];
+    const float A0113 = m[4] * m[13] - m[5] * m[12];
+    const float A0112 = m[4] * m[9] - m[5] * m[8];
+
+    const float inv_det = 1.0f / (m[0] * (m[5] * A2323 - m[6] * A1323 + m[7] * A1223) -
+                                  m[1] * (m[4] * A2323 - m[6] * A0323 + m[7] * A0223) +
+                                  m[2] * (m[4] * A1323 - m[5] * A0323 + m[7] * A0123) -
+                                  m[3] * (m[4] * A1223 - m[5] * A0223 + m[6] * A0123));
+
+    o[0] = inv_det * (m[5] * A2323 - m[6] * A1323 + m[7] * A1223);
+    o[1] = inv_det * -(m[1] * A2323 - m[2] * A1323 + m[3] * A1223);
+    o[2] = inv_det * (m[1] * A2313 - m[2] * A1313 + m[3] * A1213);
+    o[3] = inv_det * -(m[1] * A2312 - m[2] * A1312 + m[3] * A1212);
+    o[4] = inv_det * -(m[4] * A2323 - m[6] * A0323 + m[7] * A0223);
+    o[5] = inv_det * (m[0] * A2323 - m[2] * A0323 + m[3] * A0223);
+    o[6] = inv_det * -(m[0] * A2313 - m[2] * A0313 + m[3] * A0213);
+    o[7] = inv_det * (m[0] * A2312 - m[2] * A0312 + m[3] * A0212);
+    o[8] = inv_det * (m[4] * A1323 - m[5] * A0323 + m[7] * A0123);
+    o[9] = inv_det * -(m[0] * A1323 - m[1] * A0323 + m[3] * A0123);
+    o[10] = inv_det * (m[0] * A1313 - m[1] * A0313 + m[3] * A0113);
+    o[11] = inv_det * -(m[0] * A1312 - m[1] * A0312 + m[3] * A0112);
+    o[12] = inv_det * -(m[4] * A1223 - m[5] * A0223 + m[6] * A0123);
+    o[13] = inv_det * (m[0] * A1223 - m[1] * A0223 + m[2] * A0123);
+    o[14] = inv_det * -(m[0] * A1213 - m[1] * A0213 + m[2] * A0113);
+    o[15] = inv_det * (m[0] * A1212 - m[1] * A0212 + m[2] * A0112);
+}
+
+void ComputeTangentBasis(size_t vtx_offset, size_t vtx_start, std::vector<rh_vertex> &vertices,
+                         std::vector<uint32_t> &new_vtx_indices, const uint32_t *indices, size_t indices_count) {
+    // per-vertex "twin" copies by binormal-sign class, and accumulated binormals
+    std::vector<std::array<uint32_t, 3>> twin_verts(vertices.size(), {{0, 0, 0}});
+    std::vector<vec3> binormals(vertices.size(), vec3{{0.0f, 0.0f, 0.0f}});
+
+    for (size_t i = 0; i < indices_count; i += 3) {
+        rh_vertex *v0 = &vertices[indices[i + 0]];
+        rh_vertex *v1 = &vertices[indices[i + 1]];
+        rh_vertex *v2 = &vertices[indices[i + 2]];
+
+        vec3 &b0 = binormals[indices[i + 0]];
+        vec3 &b1 = binormals[indices[i + 1]];
+        vec3 &b2 = binormals[indices[i + 2]];
+
+        const vec3 dp1 = {{v1->p[0] - v0->p[0], v1->p[1] - v0->p[1], v1->p[2] - v0->p[2]}};
+        const vec3 dp2 = {{v2->p[0] - v0->p[0], v2->p[1] - v0->p[1], v2->p[2] - v0->p[2]}};
+        const float dt1[2] = {v1->t[0][0] - v0->t[0][0], v1->t[0][1] - v0->t[0][1]};
+        const float dt2[2] = {v2->t[0][0] - v0->t[0][0], v2->t[0][1] - v0->t[0][1]};
+
+        vec3 tangent, binormal;
+        const float det = dt1[0] * dt2[1] - dt1[1] * dt2[0];
+        if (std::abs(det) > RH_FLT_EPS) {
+            const float inv_det = 1.0f / det;
+            for (int k = 0; k < 3; k++) {
+                tangent[k] = (dp1[k] * dt2[1] - dp2[k] * dt1[1]) * inv_det;
+                binormal[k] = (dp2[k] * dt1[0] - dp1[k] * dt2[0]) * inv_det;
+            }
+        } else {
+            const vec3 plane_N = cross3(dp1, dp2);
+            int w = 2;
+            tangent = {{0.0f, 1.0f, 0.0f}};
+            if (std::abs(plane_N[0]) <= std::abs(plane_N[1]) && std::abs(plane_N[0]) <= std::abs(plane_N[2])) {
+                tangent = {{1.0f, 0.0f, 0.0f}};
+                w = 1;
+            } else if (std::abs(plane_N[2]) <= std::abs(plane_N[0]) && std::abs(plane_N[2]) <= std::abs(plane_N[1])) {
+                tangent = {{0.0f, 0.0f, 1.0f}};
+                w = 0;
+            }
+            if (std::abs(plane_N[w]) > RH_FLT_EPS) {
+                binormal = normalize3(cross3(plane_N, tangent));
+                tangent = normalize3(cross3(plane_N, binormal));
+            } else {
+                binormal = {{0.0f, 0.0f, 0.0f}};
+                tangent = {{0.0f, 0.0f, 0.0f}};
+            }
+        }
+
+        // corner 0
+        {
+            const int i1 = (v0->b[0] * tangent[0] + v0->b[1] * tangent[1] + v0->b[2] * tangent[2]) < 0;
+            const int i2 = 2 * (b0[0] * binormal[0] + b0[1] * binormal[1] + b0[2] * binormal[2] < 0);
+            if (i1 || i2) {
+                uint32_t index = twin_verts[indices[i + 0]][i1 + i2 - 1];
+                if (index == 0) {
+                    index = (uint32_t)(vtx_offset + vertices.size());
+                    vertices.push_back(*v0);
+                    memset(&vertices.back().b[0], 0, 3 * sizeof(float));
+                    twin_verts[indices[i + 0]][i1 + i2 - 1] = index;
+                    v1 = &vertices[indices[i + 1]];
+                    v2 = &vertices[indices[i + 2]];
+                }
+                new_vtx_indices[i] = index;
+                v0 = &vertices[index - vtx_offset];
+            } else {
+                b0 = binormal;
+            }
+            v0->b[0] += tangent[0];
+            v0->b[1] += tangent[1];
+            v0->b[2] += tangent[2];
+        }
+        // corner 1
+        {
+            const int i1 = v1->b[0] * tangent[0] + v1->b[1] * tangent[1] + v1->b[2] * tangent[2] < 0;
+            const int i2 = 2 * (b1[0] * binormal[0] + b1[1] * binormal[1] + b1[2] * binormal[2] < 0);
+            if (i1 || i2) {
+                uint32_t index = twin_verts[indices[i + 1]][i1 + i2 - 1];
+                if (index == 0) {
+                    index = (uint32_t)(vtx_offset + vertices.size());
+                    vertices.push_back(*v1);
+                    memset(&vertices.back().b[0], 0, 3 * sizeof(float));
+                    twin_verts[indices[i + 1]][i1 + i2 - 1] = index;
+                    v0 = &vertices[indices[i + 0]];
+                    v2 = &vertices[indices[i + 2]];
+                }
+                new_vtx_indices[i + 1] = index;
+                v1 = &vertices[index - vtx_offset];
+            } else {
+                b1 = binormal;
+            }
+            v1->b[0] += tangent[0];
+            v1->b[1] += tangent[1];
+            v1->b[2] += tangent[2];
+        }
+        // corner 2
+        {
+            const int i1 = v2->b[0] * tangent[0] + v2->b[1] * tangent[1] + v2->b[2] * tangent[2] < 0;
+            const int i2 = 2 * (b2[0] * binormal[0] + b2[1] * binormal[1] + b2[2] * binormal[2] < 0);
+            if (i1 || i2) {
+                uint32_t index = twin_verts[indices[i + 2]][i1 + i2 - 1];
+                if (index == 0) {
+                    index = (uint32_t)(vtx_offset + vertices.size());
+                    vertices.push_back(*v2);
+                    memset(&vertices.back().b[0], 0, 3 * sizeof(float));
+                    twin_verts[indices[i + 2]][i1 + i2 - 1] = index;
+                    v0 = &vertices[indices[i + 0]];
+                    v1 = &vertices[indices[i + 1]];
+                }
+                new_vtx_indices[i + 2] = index;
+                v2 = &vertices[index - vtx_offset];
+            } else {
+                b2 = binormal;
+            }
+            v2->b[0] += tangent[0];
+            v2->b[1] += tangent[1];
+            v2->b[2] += tangent[2];
+        }
+    }
+
+    for (size_t i = vtx_start; i < vertices.size(); i++) {
+        rh_vertex &v = vertices[i];
+        if (std::abs(v.b[0]) > RH_FLT_EPS || std::abs(v.b[1]) > RH_FLT_EPS || std::abs(v.b[2]) > RH_FLT_EPS) {
+            const vec3 tangent = load3(v.b);
+            vec3 binormal = cross3(load3(v.n), tangent);
+            const float l = length3(binormal);
+            if (l > RH_FLT_EPS) {
+                binormal = div3(binormal, l);
+                memcpy(&v.b[0], binormal.v, 3 * sizeof(float));
+            }
+        }
+    }
+}
+
+std::vector<uint32_t> DownsampleTexture(const std::vector<uint32_t> &tex, const int res[2]) {
+    if (res[0] == 1 || res[1] == 1) return tex;
+    std::vector<uint32_t> ret;
+    ret.reserve((size_t)res[0] * res[1] / 4);
+    auto ch = [](uint32_t p, int c) { return (int)((p >> (8 * c)) & 0xff); };
+    for (int j = 0; j < res[1]; j += 2) {
+        for (int i = 0; i < res[0]; i += 2) {
+            const uint32_t a = tex[(size_t)(j + 0) * res[0] + i], b = tex[(size_t)(j + 0) * res[0] + i + 1],
+                           c = tex[(size_t)(j + 1) * res[0] + i], d = tex[(size_t)(j + 1) * res[0] + i + 1];
+            uint32_t out = 0;
+            for (int k = 0; k < 4; k++) {
+                const int s = ch(a, k) + ch(b, k) + ch(c, k) + ch(d, k);
+                out |= (uint32_t)(uint8_t)(s / 4) << (8 * k);
+            }
+            ret.push_back(out);
+        }
+    }
+    return ret;
+}
+
+std::vector<uint16_t> ComputeRadicalInversePermutations() {
+    static const int primes[RH_HALTON_COUNT] = {2,  3,  5,  7,  11, 13, 17, 19, 23, 29, 31, 37,
+                                                41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89};
+    auto rand_func = std::bind(std::uniform_int_distribution<int>(), std::mt19937(0));
+    int total = 0;
+    for (int p : primes) total += p;
+    std::vector<uint16_t> perms((size_t)total);
+    uint16_t *p = perms.data();
+    for (int k = 0; k < RH_HALTON_COUNT; k++) {
+        const int count = primes[k];
+        for (int j = 0; j < count; j++) p[j] = (uint16_t)j;
+        // rejection-sampled shuffle (Halton.h:6-20)
+        for (int i = 0; i < count; i++) {
+            int other;
+            const uint32_t thres = (~((uint32_t)count) + 1u) % count;
+            while (true) {
+                const uint32_t r = rand_func();
+                if (r >= thres) {
+                    other = r % count;
+                    break;
+                }
+            }
+            std::swap(p[i], p[other]);
+        }
+        p += count;
+    }
+    return perms;
+}
+
+namespace {
+template <int base> float ScrambledRadicalInverse(const uint16_t *perm, uint64_t a) {
+    const float inv_base = 1.0f / base;
+    uint64_t reversed_digits = 0;
+    float inv_base_n = 1;
+    while (a) {
+        const uint64_t next = a / base;
+        const uint64_t digit = a - next * base;
+        reversed_digits = reversed_digits * base + perm[digit];
+        inv_base_n *= inv_base;
+        a = next;
+    }
+    return std::min(inv_base_n * (reversed_digits + inv_base * perm[0] / (1 - inv_base)),
+                    1.0f - std::numeric_limits<float>::epsilon());
+}
+} // namespace
+
+void UpdateHaltonSequence(const std::vector<uint16_t> &perms, int iteration, float *seq) {
+    const uint16_t *p = perms.data();
+    for (int i = 0; i < RH_HALTON_SEQ_LEN; i++) {
+        const uint64_t a = (uint64_t)(iteration + i);
+        float *row = seq + i * RH_HALTON_COUNT;
+        row[0] = ScrambledRadicalInverse<2>(p + 0, a);
+        row[1] = ScrambledRadicalInverse<3>(p + 2, a);
+        row[2] = ScrambledRadicalInverse<5>(p + 5, a);
+        row[3] = ScrambledRadicalInverse<7>(p + 10, a);
+        row[4] = ScrambledRadicalInverse<11>(p + 17, a);
+        row[5] = ScrambledRadicalInverse<13>(p + 28, a);
+        row[6] = ScrambledRadicalInverse<17>(p + 41, a);
+        row[7] = ScrambledRadicalInverse<19>(p + 58, a);
+        row[8] = ScrambledRadicalInverse<23>(p + 77, a);
+        row[9] = ScrambledRadicalInverse<29>(p + 100, a);
+        row[10] = ScrambledRadicalInverse<31>(p + 129, a);
+        row[11] = ScrambledRadicalInverse<37>(p + 160, a);
+        row[12] = ScrambledRadicalInverse<41>(p + 197, a);
+        row[13] = ScrambledRadicalInverse<43>(p + 238, a);
+        row[14] = ScrambledRadicalInverse<47>(p + 281, a);
+        row[15] = ScrambledRadicalInverse<53>(p + 328, a);
+        row[16] = ScrambledRadicalInverse<59>(p + 381, a);
+        row[17] = ScrambledRadicalInverse<61>(p + 440, a);
+        row[18] = ScrambledRadicalInverse<67>(p + 501, a);
+        row[19] = ScrambledRadicalInverse<71>(p + 568, a);
+        row[20] = ScrambledRadicalInverse<73>(p + 639, a);
+        row[21] = ScrambledRadicalInverse<79>(p + 712, a);
+        row[22] = ScrambledRadicalInverse<83>(p + 791, a);
+        row[23] = ScrambledRadicalInverse<89>(p + 874, a);
+    }
+}
+
+TextureSplitter::TextureSplitter(const int res[2]) {
+    nodes_.emplace_back();
+    nodes_.back().size[0] = res[0];
+    nodes_.back().size[1] = res[1];
+}
+
+int TextureSplitter::Allocate(const int res[2], int pos[2]) {
+    const int i = Insert_Recursive(0, res);
+    if (i != -1) {
+        pos[0] = nodes_[i].pos[0];
+        pos[1] = nodes_[i].pos[1];
+    }
+    return i;
+}
+
+int TextureSplitter::Insert_Recursive(int i, const int res[2]) {
+    if (!nodes_[i].is_free || res[0] > nodes_[i].size[0] || res[1] > nodes_[i].size[1]) return -1;
+
+    int ch0 = nodes_[i].child[0], ch1 = nodes_[i].child[1];
+    if (ch0 != -1) {
+        const int new_node = Insert_Recursive(ch0, res);
+        if (new_node != -1) return new_node;
+        return Insert_Recursive(ch1, res);
+    }
+    if (res[0] == nodes_[i].size[0] && res[1] == nodes_[i].size[1]) {
+        nodes_[i].is_free = false;
+        return i;
+    }
+    nodes_[i].child[0] = ch0 = (int)nodes_.size();
+    nodes_.emplace_back();
+    nodes_[i].child[1] = ch1 = (int)nodes_.size();
+    nodes_.emplace_back();
+
+    const node_t &n = nodes_[i];
+    const int dw = n.size[0] - res[0], dh = n.size[1] - res[1];
+    if (dw > dh) {
+        nodes_[ch0].pos[0] = n.pos[0];
+        nodes_[ch0].pos[1] = n.pos[1];
+        nodes_[ch0].size[0] = res[0];
+        nodes_[ch0].size[1] = n.size[1];
+        nodes_[ch1].pos[0] = n.pos[0] + res[0];
+        nodes_[ch1].pos[1] = n.pos[1];
+        nodes_[ch1].size[0] = n.size[0] - res[0];
+        nodes_[ch1].size[1] = n.size[1];
+    } else {
+        nodes_[ch0].pos[0] = n.pos[0];
+        nodes_[ch0].pos[1] = n.pos[1];
+        nodes_[ch0].size[0] = n.size[0];
+        nodes_[ch0].size[1] = res[1];
+        nodes_[ch1].pos[0] = n.pos[0];
+        nodes_[ch1].pos[1] = n.pos[1] + res[1];
+        nodes_[ch1].size[0] = n.size[0];
+        nodes_[ch1].size[1] = n.size[1] - res[1];
+    }
+    nodes_[ch0].parent = nodes_[ch1].parent = i;
+    return Insert_Recursive(ch0, res);
+}
+
+} // namespace RayHip

@@ -1,0 +1,97 @@
+// Host-side scene preprocessing for the HIP backend: the parts of the Ray
+// library's core that run on the CPU at scene-build time.  Every function
+// restates a reference routine (cited beside it) so that a scene built here
+// has the same triangle records, BVH node arrays and index orders as one
+// built by the reference's CPU backends.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "rh_types.h"
+
+namespace RayHip {
+
+struct vec3 {
+    float v[3];
+    float &operator[](int i) { return v[i]; }
+    float operator[](int i) const { return v[i]; }
+};
+
+// component-wise min/max with the argument order of std::min/std::max
+inline float fmin_std(float a, float b) { return (b < a) ? b : a; }
+inline float fmax_std(float a, float b) { return (a < b) ? b : a; }
+inline vec3 vmin(const vec3 &a, const vec3 &b) { return {{fmin_std(a[0], b[0]), fmin_std(a[1], b[1]), fmin_std(a[2], b[2])}}; }
+inline vec3 vmax(const vec3 &a, const vec3 &b) { return {{fmax_std(a[0], b[0]), fmax_std(a[1], b[1]), fmax_std(a[2], b[2])}}; }
+
+// prim_t (BVHSplit.h:9-12)
+struct prim_t {
+    uint32_t i0, i1, i2;
+    vec3 bbox_min, bbox_max;
+};
+
+// bvh_settings_t (Core.h:142-147)
+struct bvh_settings_t {
+    float oversplit_threshold = 0.95f;
+    float node_traversal_cost = 0.025f;
+    bool allow_spatial_splits = false;
+};
+
+// PreprocessTri (Core.cpp:162-214)
+bool PreprocessTri(const float *p, int stride, rh_tri_accel *acc);
+// ExtractPlaneNormal (Core.cpp:216-228)
+void ExtractPlaneNormal(const rh_tri_accel &tri, float *out_normal);
+
+// PreprocessMesh (Core.cpp:230-275), SAH path only
+uint32_t PreprocessMesh(const float *attrs, const uint32_t *vtx_indices, size_t vtx_indices_count, int layout,
+                        int base_vertex, const bvh_settings_t &s, std::vector<rh_bvh_node> &out_nodes,
+                        std::vector<rh_tri_accel> &out_tris, std::vector<uint32_t> &out_tri_indices);
+
+// PreprocessPrims_SAH (Core.cpp:431-530) + SplitPrimitives_SAH (BVHSplit.cpp:126-390)
+uint32_t PreprocessPrims_SAH(const prim_t *prims, size_t prims_count, const float *positions, size_t stride,
+                             const bvh_settings_t &s, std::vector<rh_bvh_node> &out_nodes,
+                             std::vector<uint32_t> &out_indices);
+
+// ConstructCamera (Core.cpp:856-886)
+void ConstructCamera(int type, int filter, int dtype, const float origin[3], const float fwd[3], const float up[3],
+                     float fov, float gamma, float focus_distance, float focus_factor, rh_camera *cam);
+
+// TransformBoundingBox (Core.cpp:888-907)
+void TransformBoundingBox(const float bbox_min[3], const float bbox_max[3], const float *xform, float out_min[3],
+                          float out_max[3]);
+
+// InverseMatrix (Core.cpp:909-950)
+void InverseMatrix(const float mat[16], float out_mat[16]);
+
+// ComputeTangentBasis (TextureUtilsRef.cpp:40-178)
+void ComputeTangentBasis(size_t vtx_offset, size_t vtx_start, std::vector<rh_vertex> &vertices,
+                         std::vector<uint32_t> &new_vtx_indices, const uint32_t *indices, size_t indices_count);
+
+// DownsampleTexture (TextureUtilsRef.cpp:9-38); RGBA8 packed as uint32 (r in bits 0-7)
+std::vector<uint32_t> DownsampleTexture(const std::vector<uint32_t> &tex, const int res[2]);
+
+// Radical-inverse permutations (Halton.h:6-43) seeded as RendererRef.cpp:11-12
+std::vector<uint16_t> ComputeRadicalInversePermutations();
+// One 256 x 24 Halton table (RendererRef.cpp:327-358)
+void UpdateHaltonSequence(const std::vector<uint16_t> &perms, int iteration, float *seq);
+
+// Guillotine rectangle packer restating TextureSplitter::Allocate
+// (TextureSplitter.cpp:3-10, 53-101): only placement is needed here, so the
+// atlas position of every mip level equals the reference's.
+class TextureSplitter {
+    struct node_t {
+        int parent = -1;
+        int child[2] = {-1, -1};
+        int pos[2] = {0, 0}, size[2] = {0, 0};
+        bool is_free = true;
+    };
+    std::vector<node_t> nodes_;
+    int Insert_Recursive(int i, const int res[2]);
+
+public:
+    explicit TextureSplitter(const int res[2]);
+    int Allocate(const int res[2], int pos[2]);
+};
+
+} // namespace RayHip

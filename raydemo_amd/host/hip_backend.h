@@ -1,0 +1,129 @@
+// Ray::Hip::Renderer and Ray::Hip::Scene — the HIP backend behind the Ray
+// renderer/scene API.  They take the place of Ray::Ocl::Renderer/Scene
+// (src/Ray/internal/RendererOCL.h:20-190, SceneOCL.h) and follow the CPU Ref
+// backend's semantics (src/Ray/internal/RendererRef.cpp, SceneRef.cpp).
+// Host-side scene preprocessing happens here in C++; the device is reached
+// only through the C ABI of include/rh_hip.h.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "Ray/RendererBase.h"
+#include "core.h"
+#include "rh_hip.h"
+
+namespace Ray {
+namespace Hip {
+
+class Renderer;
+
+class Scene : public SceneBase {
+    friend class Renderer;
+
+    std::vector<rh_bvh_node> nodes_;
+    std::vector<rh_tri_accel> tris_;
+    std::vector<uint32_t> tri_indices_;
+    std::vector<rh_transform> transforms_;
+    std::vector<rh_mesh> meshes_;
+    std::vector<rh_mesh_instance> mesh_instances_;
+    std::vector<uint32_t> mi_indices_;
+    std::vector<rh_vertex> vertices_;
+    std::vector<uint32_t> vtx_indices_;
+    std::vector<rh_material> materials_;
+    std::vector<rh_texture> textures_;
+    std::vector<rh_tex_mip> tex_mips_;
+    std::vector<uint32_t> texels_;
+    std::vector<rh_light> lights_;
+    std::vector<uint32_t> li_indices_;
+    rh_environment env_ = {};
+
+    uint32_t macro_nodes_root_ = 0xffffffff, macro_nodes_count_ = 0;
+    uint32_t light_nodes_root_ = 0xffffffff, light_nodes_count_ = 0;
+    uint32_t default_normals_texture_ = 0xffffffff, default_env_texture_ = 0xffffffff;
+
+    // texture atlas placement (TextureAtlasTiled, TextureAtlasRef.cpp:178-266)
+    std::vector<RayHip::TextureSplitter> splitters_;
+
+    // the macro tree is rebuilt lazily (once before use) instead of after every
+    // instance change; the light tree likewise
+    bool macro_dirty_ = false, light_dirty_ = false;
+    uint64_t version_ = 1;
+
+    void RemoveNodes(uint32_t node_index, uint32_t node_count);
+    void RebuildMacroBVH();
+    void RebuildLightBVH();
+    int AllocateInAtlas(const int res[2], int pos[2]);
+
+public:
+    Scene();
+
+    void GetEnvironment(environment_desc_t &env) override;
+    void SetEnvironment(const environment_desc_t &env) override;
+
+    uint32_t AddTexture(const tex_desc_t &t) override;
+    void RemoveTexture(uint32_t) override {}
+
+    uint32_t AddMaterial(const mat_desc_t &m) override;
+    void RemoveMaterial(uint32_t) override {}
+
+    uint32_t AddMesh(const mesh_desc_t &m) override;
+    void RemoveMesh(uint32_t i) override;
+
+    uint32_t AddLight(const light_desc_t &l) override;
+    void RemoveLight(uint32_t i) override;
+
+    uint32_t AddMeshInstance(uint32_t m_index, const float *xform) override;
+    void SetMeshInstanceTransform(uint32_t mi_index, const float *xform) override;
+    void RemoveMeshInstance(uint32_t mi_index) override;
+
+    uint32_t triangle_count() override { return (uint32_t)tris_.size(); }
+    uint32_t node_count() override;
+
+    // Brings the lazily rebuilt trees up to date and describes the arrays.
+    void Finalize();
+    rh_scene_view View();
+    uint64_t version() const { return version_; }
+    const camera_t &camera(uint32_t i) const { return cams_[i].cam; }
+};
+
+class Renderer : public RendererBase {
+    rh_context *ctx_ = nullptr;
+    int w_ = 0, h_ = 0;
+    bool sort_rays_ = true;
+
+    std::vector<uint16_t> permutations_;
+    mutable std::vector<pixel_color_t> host_pixels_;
+    mutable bool host_dirty_ = true;
+    const Scene *uploaded_scene_ = nullptr;
+    uint64_t uploaded_version_ = 0;
+    const float *uploaded_halton_ = nullptr;
+    int uploaded_halton_epoch_ = -1;
+    stats_t stats_ = {};
+
+public:
+    Renderer(const settings_t &s, int device_index);
+    ~Renderer() override;
+
+    eRendererType type() const override { return RendererHIP; }
+    std::pair<int, int> size() const override { return std::make_pair(w_, h_); }
+    const pixel_color_t *get_pixels_ref() const override;
+    const shl1_data_t *get_sh_data_ref() const override { return nullptr; }
+    void Resize(int w, int h) override;
+    void Clear(const pixel_color_t &c) override;
+    std::shared_ptr<SceneBase> CreateScene() override;
+    void RenderScene(const std::shared_ptr<SceneBase> &s, RegionContext &region) override;
+    void GetStats(stats_t &st) override;
+    void ResetStats() override;
+
+    // HIP-specific
+    rh_context *context() { return ctx_; }
+    void set_sort_rays(bool on) { sort_rays_ = on; }
+    std::string last_error;
+};
+
+} // namespace Hip
+} // namespace Ray

@@ -1,0 +1,144 @@
+// Ray::Hip::Renderer: drives one RenderScene pass on the device through the
+// C ABI of include/rh_hip.h.  Pass set-up follows the Ref renderer
+// (src/Ray/internal/RendererRef.cpp:19-325): region iteration and Halton
+// refresh, depth clamp, hash-grid box from the macro-tree root; the stages
+// themselves (generate, trace, sort, shade, accumulate) run as HIP kernels.
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "Ray/RendererFactory.h"
+#include "hip_backend.h"
+
+namespace Ray {
+namespace Hip {
+
+Renderer::Renderer(const settings_t &s, int device_index) {
+    const int rc = rh_create(device_index, &ctx_);
+    if (rc != 0) {
+        throw std::runtime_error(std::string("HIP backend: ") + rh_last_error());
+    }
+    permutations_ = RayHip::ComputeRadicalInversePermutations();
+    Resize(s.w, s.h);
+}
+
+Renderer::~Renderer() {
+    if (ctx_) rh_destroy(ctx_);
+}
+
+void Renderer::Resize(int w, int h) {
+    if (w == w_ && h == h_) return;
+    if (rh_resize(ctx_, w, h) != 0) throw std::runtime_error(std::string("HIP backend: ") + rh_last_error());
+    w_ = w;
+    h_ = h;
+    host_pixels_.assign((size_t)w * h, pixel_color_t{0, 0, 0, 0});
+    host_dirty_ = true;
+}
+
+void Renderer::Clear(const pixel_color_t &c) {
+    const float rgba[4] = {c.r, c.g, c.b, c.a};
+    if (rh_clear(ctx_, rgba) != 0) last_error = rh_last_error();
+    host_dirty_ = true;
+}
+
+const pixel_color_t *Renderer::get_pixels_ref() const {
+    if (host_dirty_) {
+        if (rh_read_pixels(ctx_, reinterpret_cast<float *>(host_pixels_.data()), host_pixels_.size() * 4) == 0) {
+            host_dirty_ = false;
+        }
+    }
+    return host_pixels_.data();
+}
+
+std::shared_ptr<SceneBase> Renderer::CreateScene() { return std::make_shared<Scene>(); }
+
+void Renderer::RenderScene(const std::shared_ptr<SceneBase> &_s, RegionContext &region) {
+    const auto s = std::dynamic_pointer_cast<Scene>(_s);
+    if (!s) return; // scene of another backend: no-op, as RendererRef.cpp:20-21
+    if (s->current_cam() == 0xffffffff) return;
+
+    s->Finalize();
+    if (uploaded_scene_ != s.get() || uploaded_version_ != s->version()) {
+        const rh_scene_view v = s->View();
+        if (rh_upload_scene(ctx_, &v) != 0) {
+            last_error = rh_last_error();
+            return;
+        }
+        uploaded_scene_ = s.get();
+        uploaded_version_ = s->version();
+    }
+
+    rh_pass_desc pass;
+    memset(&pass, 0, sizeof(pass));
+    memcpy(&pass.cam, &s->camera(s->current_cam()), sizeof(rh_camera));
+
+    if (s->macro_nodes_root_ != 0xffffffff) {
+        const rh_bvh_node &root = s->nodes_[s->macro_nodes_root_];
+        for (int i = 0; i < 3; i++) {
+            pass.root_min[i] = root.bbox_min[i];
+            pass.cell_size[i] = (root.bbox_max[i] - root.bbox_min[i]) / 255;
+        }
+    }
+
+    rect_t rect = region.rect();
+    if (rect.w == 0 || rect.h == 0) rect = {0, 0, w_, h_};
+    if (rect.x < 0 || rect.y < 0 || rect.x + rect.w > w_ || rect.y + rect.h > h_) {
+        last_error = "region outside the framebuffer";
+        return;
+    }
+
+    region.iteration++;
+    if (!region.halton_seq || region.iteration % RH_HALTON_SEQ_LEN == 0) {
+        if (!region.halton_seq) region.halton_seq.reset(new float[RH_HALTON_COUNT * RH_HALTON_SEQ_LEN]);
+        RayHip::UpdateHaltonSequence(permutations_, region.iteration, region.halton_seq.get());
+        uploaded_halton_ = nullptr;
+    }
+    const int epoch = region.iteration / RH_HALTON_SEQ_LEN;
+    if (uploaded_halton_ != region.halton_seq.get() || uploaded_halton_epoch_ != epoch) {
+        if (rh_set_halton(ctx_, region.halton_seq.get(), RH_HALTON_COUNT * RH_HALTON_SEQ_LEN) != 0) {
+            last_error = rh_last_error();
+            return;
+        }
+        uploaded_halton_ = region.halton_seq.get();
+        uploaded_halton_epoch_ = epoch;
+    }
+
+    pass.rect[0] = rect.x;
+    pass.rect[1] = rect.y;
+    pass.rect[2] = rect.w;
+    pass.rect[3] = rect.h;
+    pass.iteration = region.iteration;
+    pass.sort_rays = sort_rays_ ? 1 : 0;
+    pass.cam.pass_settings.max_total_depth = std::min<uint8_t>(pass.cam.pass_settings.max_total_depth, RH_MAX_BOUNCES);
+
+    if (rh_render_pass(ctx_, &pass) != 0) {
+        last_error = rh_last_error();
+        return;
+    }
+    host_dirty_ = true;
+}
+
+void Renderer::GetStats(stats_t &st) {
+    rh_stats rs;
+    if (rh_get_stats(ctx_, &rs) == 0) {
+        static_assert(sizeof(rs) == sizeof(stats_t), "stats layout");
+        memcpy(&st, &rs, sizeof(st));
+    } else {
+        st = stats_;
+    }
+}
+
+void Renderer::ResetStats() { rh_reset_stats(ctx_); }
+
+} // namespace Hip
+
+std::shared_ptr<RendererBase> CreateRenderer(const settings_t &s, uint32_t flags, std::ostream &log_stream) {
+    if (flags & RendererHIP) {
+        log_stream << "Ray: Creating HIP renderer " << s.w << "x" << s.h << std::endl;
+        return std::make_shared<Hip::Renderer>(s, s.device_index);
+    }
+    throw std::runtime_error("this library provides only the HIP backend (RendererHIP)");
+}
+
+} // namespace Ray

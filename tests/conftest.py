@@ -1,0 +1,77 @@
+"""Shared fixtures.  GPU tests are marked ``gpu``; CPU tests cover the host
+scene build, the CPU oracle against the golden vectors and the reference
+library, and that the C-ABI libraries load and export every declared symbol."""
+import os
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+sys.path.insert(0, str(ROOT))
+
+GOLDEN = ROOT / "tests" / "golden"
+REF_LIB = ROOT / "oracle" / "_ref" / "libray_ref.so"
+ORACLE_LIB = ROOT / "oracle" / "liboracle.so"
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
+
+
+@pytest.fixture(scope="session")
+def ref_lib():
+    """The reference Ray library (CPU Ref backend) built from its own sources."""
+    if not REF_LIB.exists():
+        pytest.skip("reference library oracle/_ref/libray_ref.so not built")
+    from raydemo_amd.capi import RayLib
+    lib = RayLib(str(REF_LIB))
+    import ctypes as C
+    L = lib.lib
+    P = C.c_void_p
+    for name, res, args in [
+        ("ref_halton_table", C.c_int, [C.c_int, P]),
+        ("ref_halton_perms", C.c_int, [P, C.c_size_t]),
+        ("ref_gen_primary", C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P]),
+        ("ref_trace", C.c_int, [P, P, C.c_size_t, C.c_int, P, P]),
+        ("ref_ray_hashes", C.c_int, [P, P, C.c_size_t, P]),
+        ("ref_sort_rays", C.c_int, [P, P, C.c_size_t]),
+        ("ref_shade", C.c_int, [P, P, P, P, C.c_size_t, P, P, P, P, P]),
+        ("ref_sample_bilinear", C.c_int, [P, C.c_uint32, P, C.c_int, C.c_size_t, P]),
+    ]:
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+@pytest.fixture(scope="session")
+def host_lib():
+    from raydemo_amd import host_lib as hl
+    return hl()
+
+
+@pytest.fixture(scope="session")
+def dev_lib():
+    from raydemo_amd import device_lib
+    return device_lib()
+
+
+@pytest.fixture(scope="session")
+def golden_scene1():
+    d = np.load(GOLDEN / "scene1.npz")
+    return {k: d[k] for k in d.files}
+
+
+class _NoRenderer:
+    """Stands in for a renderer so a scene can be built without a device."""
+
+    def __init__(self, lib):
+        self.lib = lib
+        self.h = None
+
+
+def standalone_scene(lib):
+    from raydemo_amd.capi import Scene
+    return Scene(_NoRenderer(lib))

@@ -1,0 +1,132 @@
+"""Test helpers: driving the reference library's per-ray entry points and
+building the same scene in two libraries."""
+import ctypes as C
+
+import numpy as np
+
+from raydemo_amd.capi import (HALTON_COUNT, HALTON_SEQ_LEN, HIT_DTYPE, PASS_INFO_DTYPE, RAY_PACKET_DTYPE, Camera)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def ref_halton(ref_lib, iteration):
+    out = np.zeros(HALTON_COUNT * HALTON_SEQ_LEN, np.float32)
+    assert ref_lib.lib.ref_halton_table(iteration, _p(out)) == 0
+    return out
+
+
+def ref_perms(ref_lib):
+    out = np.zeros(1024, np.uint16)
+    n = ref_lib.lib.ref_halton_perms(_p(out), out.size)
+    return out[:n]
+
+
+def ref_gen_primary(ref_lib, cam: Camera, rect, w, h, iteration, halton):
+    out = np.zeros(rect[2] * rect[3], RAY_PACKET_DTYPE)
+    halton = np.ascontiguousarray(halton, np.float32)
+    n = ref_lib.lib.ref_gen_primary(C.byref(cam), rect[0], rect[1], rect[2], rect[3], w, h, iteration, _p(halton),
+                                    _p(out))
+    assert n == len(out)
+    return out
+
+
+def ref_trace(ref_lib, scene, rays, any_hit=False, tmax=None):
+    rays = np.ascontiguousarray(rays, RAY_PACKET_DTYPE)
+    out = np.zeros(len(rays), HIT_DTYPE)
+    tm = None if tmax is None else np.ascontiguousarray(tmax, np.float32)
+    assert ref_lib.lib.ref_trace(scene.h, _p(rays), len(rays), int(any_hit), _p(tm), _p(out)) == 0, ref_lib.error()
+    return out
+
+
+def ref_ray_hashes(ref_lib, scene, rays):
+    rays = np.ascontiguousarray(rays, RAY_PACKET_DTYPE)
+    out = np.zeros(len(rays), np.uint32)
+    assert ref_lib.lib.ref_ray_hashes(scene.h, _p(rays), len(rays), _p(out)) == 0
+    return out
+
+
+def ref_sort_rays(ref_lib, scene, rays):
+    rays = np.array(rays, RAY_PACKET_DTYPE, copy=True)
+    assert ref_lib.lib.ref_sort_rays(scene.h, _p(rays), len(rays)) == 0
+    return rays
+
+
+def ref_shade(ref_lib, scene, pass_infos, rays, hits, halton):
+    rays = np.ascontiguousarray(rays, RAY_PACKET_DTYPE)
+    hits = np.ascontiguousarray(hits, HIT_DTYPE)
+    pis = np.ascontiguousarray(pass_infos, PASS_INFO_DTYPE)
+    halton = np.ascontiguousarray(halton, np.float32)
+    n = len(rays)
+    rgba = np.zeros((n, 4), np.float32)
+    sec = np.zeros(max(n, 1), RAY_PACKET_DTYPE)
+    src = np.full(max(n, 1), -1, np.int32)
+    cnt = np.zeros(1, np.int32)
+    assert ref_lib.lib.ref_shade(scene.h, _p(pis), _p(rays), _p(hits), n, _p(halton), _p(rgba), _p(sec), _p(cnt),
+                                 _p(src)) == 0
+    return rgba, sec[:cnt[0]], src[:cnt[0]]
+
+
+def camera_record(lib, **kw) -> Camera:
+    """ConstructCamera through a throwaway scene of `lib`."""
+    from conftest import standalone_scene
+    s = standalone_scene(lib)
+    i = s.add_camera(**kw)
+    return s.camera_record(i)
+
+
+def pass_infos_for(rays, w, iteration, bounce, settings: dict, coherent=False):
+    pis = np.zeros(len(rays), PASS_INFO_DTYPE)
+    x = (rays["xy"] >> 16) & 0xFFFF
+    y = rays["xy"] & 0xFFFF
+    pis["index"] = y * w + x
+    pis["rand_index"] = pis["index"]
+    pis["iteration"] = iteration
+    pis["bounce"] = bounce
+    for k, v in settings.items():
+        pis[k] = v
+    return pis
+
+
+def secondary_rays_from_hits(rays, hits, rng, n_dirs=1):
+    """Rays leaving each hit point in random directions (identical sets for both sides)."""
+    m = hits["mask"] != 0
+    o = rays["o"][m] + rays["d"][m] * hits["t"][m][:, None]
+    o = np.repeat(o, n_dirs, axis=0)
+    d = rng.standard_normal((len(o), 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    out = np.zeros(len(o), RAY_PACKET_DTYPE)
+    out["o"] = (o + 1e-3 * d).astype(np.float32)
+    out["d"] = d.astype(np.float32)
+    out["c"] = 1.0
+    out["ior"] = 1.0
+    out["xy"] = np.repeat(rays["xy"][m], n_dirs)
+    return out
+
+
+def moller_trumbore_f64(view, rays, hits):
+    """FP64 ray/triangle intersection of each reported hit: returns (t, u, v) in
+    double precision, computed in object space of the hit instance."""
+    tri_idx = hits["prim"]
+    obj = hits["obj"]
+    verts = view["vertices"]["p"].astype(np.float64)
+    vi = view["vtx_indices"]
+    inv = view["transforms"]["inv_xform"][view["mesh_instances"]["tr_index"][obj]].astype(np.float64).reshape(-1, 4, 4)
+    o = rays["o"].astype(np.float64)
+    d = rays["d"].astype(np.float64)
+    # row-vector convention of TransformRay: o' = o . M[:3,:3] + M[3,:3]
+    oo = np.einsum("ni,nij->nj", o, inv[:, :3, :3]) + inv[:, 3, :3]
+    dd = np.einsum("ni,nij->nj", d, inv[:, :3, :3])
+    p0 = verts[vi[tri_idx * 3 + 0]]
+    p1 = verts[vi[tri_idx * 3 + 1]]
+    p2 = verts[vi[tri_idx * 3 + 2]]
+    e1, e2 = p1 - p0, p2 - p0
+    pv = np.cross(dd, e2)
+    det = np.einsum("ni,ni->n", e1, pv)
+    tv = oo - p0
+    u = np.einsum("ni,ni->n", tv, pv) / det
+    qv = np.cross(tv, e1)
+    v = np.einsum("ni,ni->n", dd, qv) / det
+    t = np.einsum("ni,ni->n", e2, qv) / det
+    return t, u, v
