@@ -60,9 +60,21 @@ int rh_get_stats(rh_context *ctx, rh_stats *out);
 int rh_reset_stats(rh_context *ctx);
 int rh_get_ray_counts(rh_context *ctx, rh_ray_counts *out);
 /* Knobs: "sort_rays" (0/1), "stage_timing" (0/1: hipEvent timers per stage),
- * "trace_block" (threads per block of the trace kernel). */
+ * "kernel_timing" (0/1: events around each kernel launch, see rh_get_kernel_profile),
+ * "count_visits" (0/1: instrumented trace kernels count node/triangle/instance visits),
+ * "lds_stack_depth" (4..64 LDS entries of the per-lane traversal stack). */
 int rh_set_option(rh_context *ctx, const char *name, int value);
 int rh_get_option(rh_context *ctx, const char *name, int *value);
+
+/* Per-kernel profile collected while "kernel_timing" is on (HIP events around
+ * every trace/shade/sort launch of rh_render_pass) and, while "count_visits"
+ * is on, the traversal work of the trace kernels.  Kinds: 0 trace, 1 shade,
+ * 2 sort (rh_kernel_profile in rh_types.h).  No counterpart in the reference
+ * beyond stats_t. */
+int rh_get_kernel_profile(rh_context *ctx, rh_kernel_profile *out);
+int rh_reset_kernel_profile(rh_context *ctx);
+/* hipDeviceSynchronize on the context's device */
+int rh_device_synchronize(rh_context *ctx);
 
 /* ---- parity entry points (host buffers in, host buffers out) ---- */
 int rh_gen_primary_rays(rh_context *ctx, const rh_camera *cam, const int32_t rect[4], int w, int h, int iteration,

@@ -122,6 +122,11 @@ int rhx_get_stats(rhx_renderer *r, rh_stats *out);
 int rhx_reset_stats(rhx_renderer *r);
 int rhx_get_ray_counts(rhx_renderer *r, rh_ray_counts *out);
 int rhx_set_option(rhx_renderer *r, const char *name, int value);
+/* Device-wide synchronisation, and the per-kernel profile of the device layer
+ * (HIP backend only; see rh_get_kernel_profile in rh_hip.h). */
+int rhx_synchronize(rhx_renderer *r);
+int rhx_get_kernel_profile(rhx_renderer *r, rh_kernel_profile *out);
+int rhx_reset_kernel_profile(rhx_renderer *r);
 
 int rhx_create_scene(rhx_renderer *r, rhx_scene **out);
 void rhx_destroy_scene(rhx_scene *s);

@@ -1,0 +1,55 @@
+/*
+ * ray_oracle.h — TEST INFRASTRUCTURE: the CPU oracle.
+ *
+ * A scalar C++ restatement of the Ray library's CPU "Ref" backend for the
+ * wavefront path (generate -> trace -> sort -> shade/NEE -> accumulate), used
+ * only by the tests, the smoke check and bench.py's CPU-baseline leg as the
+ * checker of the HIP kernels.  It is pinned against the compiled reference
+ * (oracle/_ref/libray_ref.so, built from the reference's own sources) and the
+ * reference's golden vectors; see tests/test_oracle.py.
+ *
+ * Scene data comes in as an rh_scene_view (include/rh_types.h).
+ */
+#ifndef RAY_ORACLE_H
+#define RAY_ORACLE_H
+
+#include "rh_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* visit counters of the traversal: node visits, leaf triangle tests, instance tests */
+typedef struct orc_counters {
+    unsigned long long nodes, tris, instances;
+} orc_counters;
+
+int orc_halton_perms(uint16_t *out, size_t count);
+int orc_halton_table(int iteration, float *out);
+
+int orc_gen_primary(const rh_camera *cam, const int32_t rect[4], int w, int h, int iteration, const float *halton,
+                    rh_ray_packet *out);
+
+int orc_trace(const rh_scene_view *sc, const rh_ray_packet *rays, size_t n, int any_hit, const float *tmax,
+              rh_hit *out, orc_counters *counters);
+
+int orc_ray_hashes(const rh_ray_packet *rays, size_t n, const float root_min[3], const float cell_size[3],
+                   uint32_t *out);
+int orc_sort_rays(rh_ray_packet *rays, size_t n, const float root_min[3], const float cell_size[3]);
+
+int orc_shade(const rh_scene_view *sc, const rh_pass_info *pi, const rh_ray_packet *rays, const rh_hit *hits,
+              size_t n, const float *halton, float *out_rgba, rh_ray_packet *out_secondary, int *out_count,
+              int *out_src);
+
+/* Runs `passes` RenderScene passes over `rect` of a w x h image, continuing
+ * from region iteration *iteration (updated), accumulating into clean (w*h*4
+ * floats, caller-owned, zero to start) and writing the tonemapped result to
+ * final_rgba.  ray_counts: primary, secondary, shadow. */
+int orc_render(const rh_scene_view *sc, const rh_camera *cam, int w, int h, const int32_t rect[4], int passes,
+               int *iteration, float *clean, float *final_rgba, unsigned long long ray_counts[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

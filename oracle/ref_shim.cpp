@@ -131,6 +131,12 @@ int rhx_get_ray_counts(rhx_renderer *, rh_ray_counts *out) {
     return -1;
 }
 int rhx_set_option(rhx_renderer *, const char *, int) { return -1; }
+int rhx_synchronize(rhx_renderer *) { return 0; }
+int rhx_get_kernel_profile(rhx_renderer *, rh_kernel_profile *out) {
+    memset(out, 0, sizeof(*out));
+    return -1;
+}
+int rhx_reset_kernel_profile(rhx_renderer *) { return 0; }
 
 int rhx_create_scene(rhx_renderer *r, rhx_scene **out) {
     auto *s = new rhx_scene;

@@ -77,6 +77,11 @@ class RayCounts(Structure):
     _fields_ = [("primary", c_uint64), ("secondary", c_uint64), ("shadow", c_uint64), ("sorted", c_uint64)]
 
 
+class KernelProfile(Structure):
+    _fields_ = [("ms", C.c_double * 3), ("launches", c_uint64 * 3), ("node_visits", c_uint64),
+                ("tri_tests", c_uint64), ("instance_tests", c_uint64)]
+
+
 class PassSettings(Structure):
     _fields_ = [("max_diff_depth", c_uint8), ("max_glossy_depth", c_uint8), ("max_refr_depth", c_uint8),
                 ("max_transp_depth", c_uint8), ("max_total_depth", c_uint8), ("termination_start_depth", c_uint8),
@@ -176,6 +181,9 @@ class RayLib:
             "rhx_reset_stats": (c_int, [P]),
             "rhx_get_ray_counts": (c_int, [P, POINTER(RayCounts)]),
             "rhx_set_option": (c_int, [P, C.c_char_p, c_int]),
+            "rhx_synchronize": (c_int, [P]),
+            "rhx_get_kernel_profile": (c_int, [P, POINTER(KernelProfile)]),
+            "rhx_reset_kernel_profile": (c_int, [P]),
             "rhx_create_scene": (c_int, [P, POINTER(P)]),
             "rhx_destroy_scene": (None, [P]),
             "rhx_region_create": (c_int, [c_int, c_int, c_int, c_int, POINTER(P)]),
@@ -269,6 +277,21 @@ class Renderer:
 
     def set_option(self, name: str, value: int):
         self.lib.check(self.lib.lib.rhx_set_option(self.h, name.encode(), int(value)), f"rhx_set_option({name})")
+
+    def synchronize(self):
+        self.lib.check(self.lib.lib.rhx_synchronize(self.h), "rhx_synchronize")
+
+    def kernel_profile(self) -> dict:
+        p = KernelProfile()
+        self.lib.check(self.lib.lib.rhx_get_kernel_profile(self.h, C.byref(p)), "rhx_get_kernel_profile")
+        kinds = ("trace", "shade", "sort")
+        d = {f"{k}_ms": p.ms[i] for i, k in enumerate(kinds)}
+        d.update({f"{k}_launches": p.launches[i] for i, k in enumerate(kinds)})
+        d.update(node_visits=p.node_visits, tri_tests=p.tri_tests, instance_tests=p.instance_tests)
+        return d
+
+    def reset_kernel_profile(self):
+        self.lib.check(self.lib.lib.rhx_reset_kernel_profile(self.h), "rhx_reset_kernel_profile")
 
     def create_scene(self) -> "Scene":
         return Scene(self)

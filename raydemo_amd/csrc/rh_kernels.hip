@@ -222,9 +222,10 @@ struct TraceArgs {
     uint32_t lds_depth;
     unsigned long long *ray_counter; // optional: rays traced
     unsigned long long *overflow;    // stack overflow events
+    unsigned long long *visits;      // Count mode: node visits, leaf triangle tests, instance tests
 };
 
-template <bool AnyHit> __global__ void __launch_bounds__(kBlock) k_trace(TraceArgs a) {
+template <bool AnyHit, bool Count> __global__ void __launch_bounds__(kBlock) k_trace(TraceArgs a) {
     extern __shared__ uint32_t lds_stack[];
     const int n = a.count ? *a.count : a.n;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -251,9 +252,15 @@ template <bool AnyHit> __global__ void __launch_bounds__(kBlock) k_trace(TraceAr
     st.block = blockDim.x;
     st.sp = 0;
     st.overflow = 0;
+    TraceCounters tc = {0, 0, 0};
     if (a.sc.macro_root != RH_INVALID)
-        traverse<AnyHit>(a.sc, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, hit, st, nullptr);
+        traverse<AnyHit>(a.sc, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, hit, st, Count ? &tc : nullptr);
     if (st.overflow && a.overflow) atomicAdd(a.overflow, (unsigned long long)st.overflow);
+    if (Count) {
+        atomicAdd(a.visits + 0, (unsigned long long)tc.nodes);
+        atomicAdd(a.visits + 1, (unsigned long long)tc.leaf_tris);
+        atomicAdd(a.visits + 2, (unsigned long long)tc.instances);
+    }
 
     float4 *h = reinterpret_cast<float4 *>(a.hits + i);
     h[0] = make_float4(__int_as_float(hit.mask), __int_as_float(hit.obj), __int_as_float(hit.prim), hit.t);
@@ -562,7 +569,15 @@ struct rh_context {
     unsigned long long *stats_dev = nullptr; // [0] primary [1] secondary [2] shadow [3] sorted [4] overflow
     float *halton = nullptr;
 
+    // profiling: event pairs around trace / shade / sort launches, read back on demand
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, std::pair<size_t, size_t>>> ev_marks; // kind, (start, stop)
+    unsigned long long *visits = nullptr;                             // [nodes, tris, instances]
+
     // options
+    int kernel_timing = 0;
+    int count_visits = 0;
     int sort_rays = 1;
     int stage_timing = 0;
     int lds_depth = 24;
@@ -674,15 +689,50 @@ TraceArgs trace_args(rh_context *c, const rh_ray *rays, const int *count, int n,
     a.lds_depth = (uint32_t)c->lds_depth;
     a.ray_counter = ray_counter;
     a.overflow = c->stats_dev + 4;
+    a.visits = c->visits;
     return a;
 }
 
+size_t ev_next(rh_context *c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return (size_t)-1;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_used++;
+}
+
+// kind: 0 trace, 1 shade, 2 sort
+struct KernelMark {
+    rh_context *c;
+    int kind;
+    size_t a = (size_t)-1;
+    KernelMark(rh_context *ctx, int k) : c(ctx), kind(k) {
+        if (c->kernel_timing) {
+            a = ev_next(c);
+            if (a != (size_t)-1) (void)hipEventRecord(c->ev_pool[a], c->stream);
+        }
+    }
+    ~KernelMark() {
+        if (c->kernel_timing && a != (size_t)-1) {
+            const size_t b = ev_next(c);
+            if (b == (size_t)-1) return;
+            (void)hipEventRecord(c->ev_pool[b], c->stream);
+            c->ev_marks.push_back({kind, {a, b}});
+        }
+    }
+};
+
 int launch_trace(rh_context *c, const TraceArgs &a, int grid_rays, bool any_hit) {
     const int blocks = std::max(1, (grid_rays + kBlock - 1) / kBlock);
+    KernelMark km(c, 0);
+    const bool count = c->count_visits && a.visits;
     if (any_hit) {
-        hipLaunchKernelGGL(k_trace<true>, dim3(blocks), dim3(kBlock), lds_bytes(c), c->stream, a);
+        if (count) hipLaunchKernelGGL((k_trace<true, true>), dim3(blocks), dim3(kBlock), lds_bytes(c), c->stream, a);
+        else hipLaunchKernelGGL((k_trace<true, false>), dim3(blocks), dim3(kBlock), lds_bytes(c), c->stream, a);
     } else {
-        hipLaunchKernelGGL(k_trace<false>, dim3(blocks), dim3(kBlock), lds_bytes(c), c->stream, a);
+        if (count) hipLaunchKernelGGL((k_trace<false, true>), dim3(blocks), dim3(kBlock), lds_bytes(c), c->stream, a);
+        else hipLaunchKernelGGL((k_trace<false, false>), dim3(blocks), dim3(kBlock), lds_bytes(c), c->stream, a);
     }
     RH_CHECK(hipGetLastError());
     return 0;
@@ -702,6 +752,7 @@ ShadeArgs shade_args(rh_context *c) {
 
 int launch_shade(rh_context *c, const ShadeArgs &a, int grid_rays) {
     const int blocks = std::max(1, (grid_rays + kBlock - 1) / kBlock);
+    KernelMark km(c, 1);
     hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(kBlock), lds_bytes(c), c->stream, a);
     RH_CHECK(hipGetLastError());
     return 0;
@@ -711,6 +762,7 @@ int launch_shade(rh_context *c, const ShadeArgs &a, int grid_rays) {
 // ends in keys[0]/vals[0] after the four 8-bit passes.
 int radix_sort(rh_context *c, const int *count, int max_n) {
     const int tiles = std::max(1, (max_n + kSortTile - 1) / kSortTile);
+    KernelMark km(c, 2);
     int src = 0;
     for (int shift = 0; shift < 32; shift += 8) {
         hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kBlock), 0, c->stream, c->keys[src], count, max_n, shift,
@@ -793,6 +845,8 @@ int rh_create(int device, rh_context **out) {
     RH_CHECK(hipMalloc(&c->stats_dev, 8 * sizeof(unsigned long long)));
     RH_CHECK(hipMemsetAsync(c->counters, 0, 16 * sizeof(int), c->stream));
     RH_CHECK(hipMemsetAsync(c->stats_dev, 0, 8 * sizeof(unsigned long long), c->stream));
+    RH_CHECK(hipMalloc(&c->visits, 4 * sizeof(unsigned long long)));
+    RH_CHECK(hipMemsetAsync(c->visits, 0, 4 * sizeof(unsigned long long), c->stream));
     RH_CHECK(hipMalloc(&c->halton, RH_HALTON_COUNT * RH_HALTON_SEQ_LEN * sizeof(float)));
     RH_CHECK(hipMemsetAsync(c->halton, 0, RH_HALTON_COUNT * RH_HALTON_SEQ_LEN * sizeof(float), c->stream));
     *out = c;
@@ -819,6 +873,8 @@ void rh_destroy(rh_context *c) {
     free_ptr(c->counters);
     free_ptr(c->stats_dev);
     free_ptr(c->halton);
+    free_ptr(c->visits);
+    for (auto &e : c->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1032,6 +1088,8 @@ int rh_set_option(rh_context *c, const char *name, int value) {
     const std::string n = name;
     if (n == "sort_rays") c->sort_rays = value != 0;
     else if (n == "stage_timing") c->stage_timing = value != 0;
+    else if (n == "kernel_timing") c->kernel_timing = value != 0;
+    else if (n == "count_visits") c->count_visits = value != 0;
     else if (n == "lds_stack_depth") {
         if (value < 4 || value > 64) return arg_error("lds_stack_depth must be 4..64");
         c->lds_depth = value;
@@ -1050,6 +1108,38 @@ int rh_get_option(rh_context *c, const char *name, int *value) {
         RH_CHECK(hipStreamSynchronize(c->stream));
         *value = (int)v[4];
     } else return arg_error("unknown option");
+    return 0;
+}
+
+int rh_device_synchronize(rh_context *c) {
+    RH_CHECK(hipSetDevice(c->device));
+    RH_CHECK(hipDeviceSynchronize());
+    return 0;
+}
+
+int rh_get_kernel_profile(rh_context *c, rh_kernel_profile *out) {
+    memset(out, 0, sizeof(*out));
+    RH_CHECK(hipStreamSynchronize(c->stream));
+    for (const auto &m : c->ev_marks) {
+        float ms = 0;
+        RH_CHECK(hipEventElapsedTime(&ms, c->ev_pool[m.second.first], c->ev_pool[m.second.second]));
+        out->ms[m.first] += ms;
+        out->launches[m.first]++;
+    }
+    unsigned long long v[4];
+    RH_CHECK(hipMemcpy(v, c->visits, sizeof(v), hipMemcpyDeviceToHost));
+    out->node_visits = v[0];
+    out->tri_tests = v[1];
+    out->instance_tests = v[2];
+    return 0;
+}
+
+int rh_reset_kernel_profile(rh_context *c) {
+    RH_CHECK(hipStreamSynchronize(c->stream));
+    c->ev_marks.clear();
+    c->ev_used = 0;
+    RH_CHECK(hipMemsetAsync(c->visits, 0, 4 * sizeof(unsigned long long), c->stream));
+    RH_CHECK(hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -1153,9 +1243,9 @@ int rh_trace_device(rh_context *c, const rh_ray *d_rays, int n, int any_hit, rh_
     const int blocks = (n + kBlock - 1) / kBlock;
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     if (any_hit) {
-        hipLaunchKernelGGL(k_trace<true>, dim3(blocks), dim3(kBlock), lds_bytes(c), s, a);
+        hipLaunchKernelGGL((k_trace<true, false>), dim3(blocks), dim3(kBlock), lds_bytes(c), s, a);
     } else {
-        hipLaunchKernelGGL(k_trace<false>, dim3(blocks), dim3(kBlock), lds_bytes(c), s, a);
+        hipLaunchKernelGGL((k_trace<false, false>), dim3(blocks), dim3(kBlock), lds_bytes(c), s, a);
     }
     RH_CHECK(hipGetLastError());
     return 0;

@@ -112,6 +112,24 @@ int rhx_set_option(rhx_renderer *r, const char *name, int value) {
     return rc;
 }
 
+int rhx_synchronize(rhx_renderer *r) {
+    const int rc = rh_device_synchronize(r->r->context());
+    if (rc) g_err = rh_last_error();
+    return rc;
+}
+
+int rhx_get_kernel_profile(rhx_renderer *r, rh_kernel_profile *out) {
+    const int rc = rh_get_kernel_profile(r->r->context(), out);
+    if (rc) g_err = rh_last_error();
+    return rc;
+}
+
+int rhx_reset_kernel_profile(rhx_renderer *r) {
+    const int rc = rh_reset_kernel_profile(r->r->context());
+    if (rc) g_err = rh_last_error();
+    return rc;
+}
+
 int rhx_create_scene(rhx_renderer *r, rhx_scene **out) {
     return guarded([&] {
         // a scene without a renderer is allowed: host-side preprocessing only

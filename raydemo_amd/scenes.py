@@ -39,10 +39,13 @@ def _white(scene):
 
 
 def _quad(p0, p1, p2, p3, n):
-    """Two triangles (p0,p1,p2), (p0,p2,p3) with PxyzNxyzTuv attributes."""
+    """Two triangles with PxyzNxyzTuv attributes, wound so that the geometric
+    normal (e0 x e1, the front face of PreprocessTri) agrees with n."""
     uv = [(0, 0), (1, 0), (1, 1), (0, 1)]
     attrs = [list(p) + list(n) + list(t) for p, t in zip((p0, p1, p2, p3), uv)]
-    return np.array(attrs, np.float32), np.array([0, 1, 2, 0, 2, 3], np.uint32)
+    geo = np.cross(np.subtract(p1, p0), np.subtract(p2, p0))
+    idx = [0, 1, 2, 0, 2, 3] if np.dot(geo, n) > 0 else [0, 2, 1, 0, 3, 2]
+    return np.array(attrs, np.float32), np.array(idx, np.uint32)
 
 
 def _box(center, size, angle):

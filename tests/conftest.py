@@ -75,3 +75,11 @@ class _NoRenderer:
 def standalone_scene(lib):
     from raydemo_amd.capi import Scene
     return Scene(_NoRenderer(lib))
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    if not ORACLE_LIB.exists():
+        pytest.skip("CPU oracle oracle/liboracle.so not built")
+    from helpers import Oracle
+    return Oracle(ORACLE_LIB)

@@ -130,3 +130,83 @@ def moller_trumbore_f64(view, rays, hits):
     v = np.einsum("ni,ni->n", dd, qv) / det
     t = np.einsum("ni,ni->n", e2, qv) / det
     return t, u, v
+
+
+# ---------------------------------------------------------------- CPU oracle (oracle/liboracle.so)
+class Oracle:
+    """The CPU restatement of the Ref path (oracle/ray_oracle.cpp)."""
+
+    def __init__(self, path):
+        self.lib = C.CDLL(str(path), mode=C.RTLD_LOCAL)
+        L = self.lib
+        P = C.c_void_p
+        for name, args in [
+            ("orc_halton_perms", [P, C.c_size_t]),
+            ("orc_halton_table", [C.c_int, P]),
+            ("orc_gen_primary", [P, P, C.c_int, C.c_int, C.c_int, P, P]),
+            ("orc_trace", [P, P, C.c_size_t, C.c_int, P, P, P]),
+            ("orc_ray_hashes", [P, C.c_size_t, P, P, P]),
+            ("orc_sort_rays", [P, C.c_size_t, P, P]),
+            ("orc_shade", [P, P, P, P, C.c_size_t, P, P, P, P, P]),
+            ("orc_render", [P, P, C.c_int, C.c_int, P, C.c_int, P, P, P, P]),
+        ]:
+            f = getattr(L, name)
+            f.restype = C.c_int
+            f.argtypes = args
+
+    def perms(self):
+        out = np.zeros(1024, np.uint16)
+        n = self.lib.orc_halton_perms(_p(out), out.size)
+        return out[:n]
+
+    def halton(self, it):
+        out = np.zeros(HALTON_COUNT * HALTON_SEQ_LEN, np.float32)
+        self.lib.orc_halton_table(it, _p(out))
+        return out
+
+    def gen_primary(self, cam, rect, w, h, it, halton):
+        out = np.zeros(rect[2] * rect[3], RAY_PACKET_DTYPE)
+        r = np.array(rect, np.int32)
+        self.lib.orc_gen_primary(C.byref(cam), _p(r), w, h, it, _p(np.ascontiguousarray(halton, np.float32)), _p(out))
+        return out
+
+    def trace(self, view, rays, any_hit=False, tmax=None, counters=False):
+        rays = np.ascontiguousarray(rays, RAY_PACKET_DTYPE)
+        out = np.zeros(len(rays), HIT_DTYPE)
+        cnt = np.zeros(3, np.uint64)
+        tm = None if tmax is None else np.ascontiguousarray(tmax, np.float32)
+        self.lib.orc_trace(C.byref(view), _p(rays), len(rays), int(any_hit), _p(tm), _p(out), _p(cnt))
+        return (out, cnt) if counters else out
+
+    def ray_hashes(self, rays, root_min, cell):
+        rays = np.ascontiguousarray(rays, RAY_PACKET_DTYPE)
+        out = np.zeros(len(rays), np.uint32)
+        self.lib.orc_ray_hashes(_p(rays), len(rays), _p(np.array(root_min, np.float32)), _p(np.array(cell, np.float32)),
+                                _p(out))
+        return out
+
+    def sort_rays(self, rays, root_min, cell):
+        rays = np.array(rays, RAY_PACKET_DTYPE, copy=True)
+        self.lib.orc_sort_rays(_p(rays), len(rays), _p(np.array(root_min, np.float32)), _p(np.array(cell, np.float32)))
+        return rays
+
+    def shade(self, view, pis, rays, hits, halton):
+        n = len(rays)
+        rgba = np.zeros((n, 4), np.float32)
+        sec = np.zeros(max(n, 1), RAY_PACKET_DTYPE)
+        src = np.zeros(max(n, 1), np.int32)
+        cnt = np.zeros(1, np.int32)
+        self.lib.orc_shade(C.byref(view), _p(np.ascontiguousarray(pis, PASS_INFO_DTYPE)),
+                           _p(np.ascontiguousarray(rays, RAY_PACKET_DTYPE)), _p(np.ascontiguousarray(hits, HIT_DTYPE)),
+                           n, _p(np.ascontiguousarray(halton, np.float32)), _p(rgba), _p(sec), _p(cnt), _p(src))
+        return rgba, sec[:cnt[0]], src[:cnt[0]]
+
+    def render(self, view, cam, w, h, passes, rect=None):
+        rect = np.array(rect or (0, 0, w, h), np.int32)
+        clean = np.zeros((h, w, 4), np.float32)
+        final = np.zeros((h, w, 4), np.float32)
+        it = np.zeros(1, np.int32)
+        counts = np.zeros(3, np.uint64)
+        self.lib.orc_render(C.byref(view), C.byref(cam), w, h, _p(rect), passes, _p(it), _p(clean), _p(final),
+                            _p(counts))
+        return final, counts
