@@ -104,6 +104,8 @@ struct Scene {
     const uint32_t *texels;
     const rh_light *lights;
     const uint32_t *li_indices;
+    const rh_bvh_node *light_leaves; // light-tree leaves in depth-first visit order
+    uint32_t light_leaf_count;
     rh_environment env;
     uint32_t macro_root;
     uint32_t light_root;
@@ -266,106 +268,136 @@ struct TraceCounters {
     uint32_t nodes, leaf_tris, instances;
 };
 
-// Two-level closest-hit / any-hit traversal in one loop.  Visit order is the
-// reference's (far child pushed first, near child popped first; macro-leaf
-// instances in index order, each instance's micro tree walked to completion
-// before the next), so ties between equal-t triangles resolve identically:
-//   Traverse_MacroTree_WithStack_ClosestHit  CoreRef.cpp:1091-1131
-//   Traverse_MicroTree_WithStack_ClosestHit  CoreRef.cpp:1370-1393
-//   Traverse_MacroTree_WithStack_AnyHit      CoreRef.cpp:1224-1268
-//   Traverse_MicroTree_WithStack_AnyHit      CoreRef.cpp:1470-1498
-//   IntersectTris_ClosestHit / _AnyHit       CoreRef.cpp:760-825
+// Leaf triangle loop (IntersectTris_ClosestHit / _AnyHit, CoreRef.cpp:760-825).
+// Returns true when an any-hit search may stop (a solid triangle was hit).
 template <bool AnyHit>
-__device__ __forceinline__ void traverse(const Scene &sc, f3 o, f3 d, Hit &hit, Stack &st, TraceCounters *cnt) {
+__device__ __forceinline__ bool intersect_leaf(const Scene &sc, const NodeBox &n, f3 o, f3 d, int obj, Hit &hit,
+                                               TraceCounters *cnt) {
+    const uint32_t first = n.a & RH_PRIM_INDEX_BITS, count = n.b;
+    bool leaf_hit = false;
+    for (uint32_t k = first; k < first + count; k++) {
+        const uint32_t ti = sc.tri_indices[k];
+        const rh_tri_accel tri = load_tri(sc.tris, ti);
+        if (cnt) cnt->leaf_tris++;
+        float t, u, v;
+        if (intersect_tri(o, d, tri, hit.t, t, u, v)) {
+            hit.mask = -1;
+            hit.prim = (int)ti;
+            hit.obj = obj;
+            hit.t = t;
+            hit.u = u;
+            hit.v = v;
+            leaf_hit = true;
+        }
+        if (AnyHit && leaf_hit && (tri.ci & RH_TRI_SOLID_BIT)) break;
+    }
+    if (AnyHit && leaf_hit) {
+        const rh_tri_accel ht = load_tri(sc.tris, (uint32_t)hit.prim);
+        return (ht.ci & RH_TRI_SOLID_BIT) != 0;
+    }
+    return false;
+}
+
+// Interior-node step: both children's boxes are tested here (siblings are
+// adjacent in the node array, so this is one 64-byte pair load), the lane
+// descends into the near child and pushes the far one only when both boxes
+// pass.  This visits exactly the nodes the reference visits, in its order:
+// the reference pushes far then near and box-tests each node when popped; the
+// near child is popped at once (same t), and a pushed far child is re-tested
+// against the current t when popped here too (boxes only get culled as t
+// shrinks).  Returns false when neither child is entered.
+__device__ __forceinline__ bool enter_children(const Scene &sc, const NodeBox &n, f3 o, f3 d, f3 inv_d, float t,
+                                               Stack &st, NodeBox &next, TraceCounters *cnt) {
+    const uint32_t axis = n.b >> 30;
+    const uint32_t right = n.b & RH_RIGHT_CHILD_BITS, left = n.a;
+    const bool neg = comp(d, axis) < 0;
+    const uint32_t near_i = neg ? right : left, far_i = neg ? left : right;
+    const NodeBox nb = load_node(sc.nodes, near_i), fb = load_node(sc.nodes, far_i);
+    if (cnt) cnt->nodes += 2;
+    const bool hn = bbox_test(o, inv_d, t, nb.bmin, nb.bmax);
+    const bool hf = bbox_test(o, inv_d, t, fb.bmin, fb.bmax);
+    if (hn) {
+        if (hf) st.push(far_i);
+        next = nb;
+        return true;
+    }
+    if (hf) {
+        next = fb;
+        return true;
+    }
+    return false;
+}
+
+// Pops until a node passes its box test against the current t.
+__device__ __forceinline__ bool pop_entered(const Scene &sc, uint32_t base, f3 o, f3 inv_d, float t, Stack &st,
+                                            NodeBox &next, TraceCounters *cnt) {
+    while (st.sp > base) {
+        const NodeBox n = load_node(sc.nodes, st.pop());
+        if (cnt) cnt->nodes++;
+        if (bbox_test(o, inv_d, t, n.bmin, n.bmax)) {
+            next = n;
+            return true;
+        }
+    }
+    return false;
+}
+
+// Two-level traversal, entered-node form (see enter_children): the lane holds
+// the node it is in; the stack holds far siblings still to be re-tested.
+template <bool AnyHit>
+__device__ __forceinline__ void traverse_fast(const Scene &sc, f3 o, f3 d, Hit &hit, Stack &st, TraceCounters *cnt) {
     const f3 inv_d = safe_invert(d);
-    uint32_t leaf_cur = 0, leaf_end = 0;
-    bool in_micro = false;
-    uint32_t micro_base = 0;
-    f3 mo = o, md = d, minv = inv_d;
-    int mobj = -1;
-
     st.sp = 0;
-    st.push(sc.macro_root);
+    NodeBox cur = load_node(sc.nodes, sc.macro_root);
+    if (cnt) cnt->nodes++;
+    if (!bbox_test(o, inv_d, hit.t, cur.bmin, cur.bmax)) return;
 
+    uint32_t leaf_cur = 0, leaf_end = 0;
+    bool have = true; // macro phase: cur is an entered node
     while (true) {
-        if (in_micro) {
-            if (st.sp > micro_base) {
-                const uint32_t cur = st.pop();
-                const NodeBox n = load_node(sc.nodes, cur);
-                if (cnt) cnt->nodes++;
-                if (!bbox_test(mo, minv, hit.t, n.bmin, n.bmax)) continue;
-                if (!(n.a & RH_LEAF_NODE_BIT)) {
-                    const uint32_t axis = n.b >> 30;
-                    const uint32_t right = n.b & RH_RIGHT_CHILD_BITS, left = n.a;
-                    const bool neg = comp(md, axis) < 0;
-                    st.push(neg ? left : right); // far
-                    st.push(neg ? right : left); // near
-                } else {
-                    const uint32_t first = n.a & RH_PRIM_INDEX_BITS, count = n.b;
-                    bool leaf_hit = false;
-                    for (uint32_t k = first; k < first + count; k++) {
-                        const uint32_t ti = sc.tri_indices[k];
-                        const rh_tri_accel tri = load_tri(sc.tris, ti);
-                        if (cnt) cnt->leaf_tris++;
-                        float t, u, v;
-                        if (intersect_tri(mo, md, tri, hit.t, t, u, v)) {
-                            hit.mask = -1;
-                            hit.prim = (int)ti;
-                            hit.obj = mobj;
-                            hit.t = t;
-                            hit.u = u;
-                            hit.v = v;
-                            leaf_hit = true;
-                        }
-                        if (AnyHit && leaf_hit && (tri.ci & RH_TRI_SOLID_BIT)) break;
-                    }
-                    if (AnyHit && leaf_hit) {
-                        const rh_tri_accel ht = load_tri(sc.tris, (uint32_t)hit.prim);
-                        if (ht.ci & RH_TRI_SOLID_BIT) return; // solid occluder: done
-                    }
-                }
+        if (have) {
+            if (!(cur.a & RH_LEAF_NODE_BIT)) {
+                have = enter_children(sc, cur, o, d, inv_d, hit.t, st, cur, cnt);
                 continue;
             }
-            in_micro = false;
-            leaf_cur++;
-            continue;
+            leaf_cur = cur.a & RH_PRIM_INDEX_BITS;
+            leaf_end = leaf_cur + cur.b;
+            have = false;
         }
-        if (leaf_cur < leaf_end) {
-            const uint32_t mi_index = sc.mi_indices[leaf_cur];
+        // instances of the current macro leaf, in index order
+        while (leaf_cur < leaf_end) {
+            const uint32_t mi_index = sc.mi_indices[leaf_cur++];
             const rh_mesh_instance mi = sc.mesh_instances[mi_index];
             if (cnt) cnt->instances++;
-            if (!bbox_test(o, inv_d, hit.t, mi.bbox_min, mi.bbox_max)) {
-                leaf_cur++;
-                continue;
-            }
+            if (!bbox_test(o, inv_d, hit.t, mi.bbox_min, mi.bbox_max)) continue;
             const float *x = sc.transforms[mi.tr_index].inv_xform;
             // TransformRay (CoreRef.cpp:1605-1617)
-            mo = f3{x[0] * o.x + x[4] * o.y + x[8] * o.z + x[12], x[1] * o.x + x[5] * o.y + x[9] * o.z + x[13],
-                    x[2] * o.x + x[6] * o.y + x[10] * o.z + x[14]};
-            md = f3{x[0] * d.x + x[4] * d.y + x[8] * d.z, x[1] * d.x + x[5] * d.y + x[9] * d.z,
-                    x[2] * d.x + x[6] * d.y + x[10] * d.z};
-            minv = f3{1.0f / md.x, 1.0f / md.y, 1.0f / md.z};
-            mobj = (int)mi_index;
-            micro_base = st.sp;
-            st.push(sc.meshes[mi.mesh_index].node_index);
-            in_micro = true;
-            continue;
+            const f3 mo = f3{x[0] * o.x + x[4] * o.y + x[8] * o.z + x[12], x[1] * o.x + x[5] * o.y + x[9] * o.z + x[13],
+                             x[2] * o.x + x[6] * o.y + x[10] * o.z + x[14]};
+            const f3 md = f3{x[0] * d.x + x[4] * d.y + x[8] * d.z, x[1] * d.x + x[5] * d.y + x[9] * d.z,
+                             x[2] * d.x + x[6] * d.y + x[10] * d.z};
+            const f3 minv = f3{1.0f / md.x, 1.0f / md.y, 1.0f / md.z};
+            const uint32_t base = st.sp;
+            NodeBox mc = load_node(sc.nodes, sc.meshes[mi.mesh_index].node_index);
+            if (cnt) cnt->nodes++;
+            bool mhave = bbox_test(mo, minv, hit.t, mc.bmin, mc.bmax);
+            while (true) {
+                if (mhave) {
+                    if (!(mc.a & RH_LEAF_NODE_BIT)) {
+                        mhave = enter_children(sc, mc, mo, md, minv, hit.t, st, mc, cnt);
+                        continue;
+                    }
+                    if (intersect_leaf<AnyHit>(sc, mc, mo, md, (int)mi_index, hit, cnt)) {
+                        st.sp = 0;
+                        return; // solid occluder found
+                    }
+                }
+                mhave = pop_entered(sc, base, mo, minv, hit.t, st, mc, cnt);
+                if (!mhave) break;
+            }
         }
-        if (st.sp == 0) break;
-        const uint32_t cur = st.pop();
-        const NodeBox n = load_node(sc.nodes, cur);
-        if (cnt) cnt->nodes++;
-        if (!bbox_test(o, inv_d, hit.t, n.bmin, n.bmax)) continue;
-        if (!(n.a & RH_LEAF_NODE_BIT)) {
-            const uint32_t axis = n.b >> 30;
-            const uint32_t right = n.b & RH_RIGHT_CHILD_BITS, left = n.a;
-            const bool neg = comp(d, axis) < 0;
-            st.push(neg ? left : right);
-            st.push(neg ? right : left);
-        } else {
-            leaf_cur = n.a & RH_PRIM_INDEX_BITS;
-            leaf_end = leaf_cur + n.b;
-        }
+        have = pop_entered(sc, 0, o, inv_d, hit.t, st, cur, cnt);
+        if (!have) break;
     }
 }
 
@@ -495,15 +527,14 @@ __device__ __forceinline__ f4 sample_latlong_rgbe(const Scene &sc, uint32_t tex,
 // ExtractPlaneNormal (Core.cpp:216-228); note the reference scales by the
 // length rather than dividing by it
 __device__ __forceinline__ f3 extract_plane_normal(const rh_tri_accel &tri) {
-    float n[3];
+    // component w is 1, (u, v) = the next two axes in (1,2), (0,2), (0,1) order
     const int iw = tri.ci & RH_TRI_W_BITS;
-    const int iu = iw == 0 ? 1 : 0, iv = iw == 2 ? 1 : 2;
-    n[iw] = 1.0f;
-    n[iu] = tri.nu;
-    n[iv] = tri.nv;
-    float inv_l = __builtin_sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    const float n0 = iw == 0 ? 1.0f : tri.nu;
+    const float n1 = iw == 1 ? 1.0f : (iw == 0 ? tri.nu : tri.nv);
+    const float n2 = iw == 2 ? 1.0f : tri.nv;
+    float inv_l = __builtin_sqrtf(n0 * n0 + n1 * n1 + n2 * n2);
     if (tri.ci & RH_TRI_INV_NORMAL_BIT) inv_l = -inv_l;
-    return f3{n[0] * inv_l, n[1] * inv_l, n[2] * inv_l};
+    return f3{n0 * inv_l, n1 * inv_l, n2 * inv_l};
 }
 
 // TransformNormal (CoreRef.cpp:1625-1629)

@@ -254,7 +254,7 @@ template <bool AnyHit, bool Count> __global__ void __launch_bounds__(kBlock) k_t
     st.overflow = 0;
     TraceCounters tc = {0, 0, 0};
     if (a.sc.macro_root != RH_INVALID)
-        traverse<AnyHit>(a.sc, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, hit, st, Count ? &tc : nullptr);
+        traverse_fast<AnyHit>(a.sc, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, hit, st, Count ? &tc : nullptr);
     if (st.overflow && a.overflow) atomicAdd(a.overflow, (unsigned long long)st.overflow);
     if (Count) {
         atomicAdd(a.visits + 0, (unsigned long long)tc.nodes);
@@ -922,6 +922,26 @@ int rh_upload_scene(rh_context *c, const rh_scene_view *v) {
     if (v->env) s.env = *v->env;
     s.macro_root = v->macro_nodes_root;
     s.light_root = v->light_nodes_root;
+    // light-tree leaves in the reference's depth-first order (right child first)
+    std::vector<rh_bvh_node> leaves;
+    if (s.light_root != RH_INVALID && v->nodes && s.light_root < v->nodes_count) {
+        std::vector<uint32_t> st{s.light_root};
+        while (!st.empty()) {
+            const uint32_t cur = st.back();
+            st.pop_back();
+            if (cur >= v->nodes_count) return arg_error("light tree node outside the node array");
+            const rh_bvh_node &n = v->nodes[cur];
+            if (n.prim_index & RH_LEAF_NODE_BIT) {
+                leaves.push_back(n);
+            } else {
+                st.push_back(n.prim_index);
+                st.push_back(n.prim_count & RH_RIGHT_CHILD_BITS);
+            }
+        }
+    }
+    rc = upload(c, leaves.data(), leaves.size(), &s.light_leaves);
+    if (rc) return rc;
+    s.light_leaf_count = (uint32_t)leaves.size();
     if (s.macro_root != RH_INVALID && (!v->nodes || s.macro_root >= v->nodes_count))
         return arg_error("macro tree root outside the node array");
     c->sc = s;

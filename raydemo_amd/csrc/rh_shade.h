@@ -37,7 +37,7 @@ __device__ __forceinline__ float compute_visibility(const Scene &sc, f3 p1, f3 p
         sh.t = dist;
         sh.u = sh.v = 0.0f;
         cnt.shadow_rays++;
-        if (sc.macro_root != RH_INVALID) traverse<true>(sc, ro, dir, sh, st, nullptr);
+        if (sc.macro_root != RH_INVALID) traverse_fast<true>(sc, ro, dir, sh, st, nullptr);
         if (!sh.mask) break;
 
         const rh_tri_accel tri = load_tri(sc.tris, (uint32_t)sh.prim);
@@ -137,29 +137,23 @@ __device__ __forceinline__ f3 compute_direct_lighting(const Scene &sc, f3 I, f3 
                                                       float rand_offset, float rand_offset2, Stack &st,
                                                       ShadeCounters &cnt) {
     f3 col = f3{0.0f, 0.0f, 0.0f};
-    if (sc.light_root == RH_INVALID) return col;
-    // the light tree is small; its stack lives in registers
-    uint32_t lstack[RH_MAX_STACK_SIZE];
-    uint32_t lsp = 0;
-    lstack[lsp++] = sc.light_root;
-    while (lsp) {
-        const uint32_t cur = lstack[--lsp];
-        const NodeBox n = load_node(sc.nodes, cur);
+    // The reference walks the light tree depth first (right child popped
+    // first) and enters a node only when P lies strictly inside its box.  A
+    // parent box contains its children's boxes, so P strictly inside a leaf box
+    // implies it is inside every ancestor: visiting the leaves in that
+    // depth-first order and testing only the leaf boxes reaches the same lights
+    // in the same order, without a per-lane stack.  The leaf list is built
+    // once at scene upload.
+    for (uint32_t k = 0; k < sc.light_leaf_count; k++) {
+        const NodeBox n = load_node(sc.light_leaves, k);
         if (!(P.x > n.bmin[0] && P.x < n.bmax[0] && P.y > n.bmin[1] && P.y < n.bmax[1] && P.z > n.bmin[2] &&
               P.z < n.bmax[2]))
             continue;
-        if (!(n.a & RH_LEAF_NODE_BIT)) {
-            if (lsp + 2 <= RH_MAX_STACK_SIZE) {
-                lstack[lsp++] = n.a;
-                lstack[lsp++] = n.b & RH_RIGHT_CHILD_BITS;
-            }
-        } else {
-            const uint32_t first = n.a & RH_PRIM_INDEX_BITS;
-            for (uint32_t i = first; i < first + n.b; i++) {
-                const rh_light l = sc.lights[sc.li_indices[i]];
-                accumulate_light(sc, l, I, P, N, B, plane_N, sigma, halton, hi, rand_hash2, rand_offset, rand_offset2,
-                                 col, st, cnt);
-            }
+        const uint32_t first = n.a & RH_PRIM_INDEX_BITS;
+        for (uint32_t i = first; i < first + n.b; i++) {
+            const rh_light l = sc.lights[sc.li_indices[i]];
+            accumulate_light(sc, l, I, P, N, B, plane_N, sigma, halton, hi, rand_hash2, rand_offset, rand_offset2, col,
+                             st, cnt);
         }
     }
     return col;
