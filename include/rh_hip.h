@@ -62,14 +62,15 @@ int rh_get_ray_counts(rh_context *ctx, rh_ray_counts *out);
 /* Knobs: "sort_rays" (0/1), "stage_timing" (0/1: hipEvent timers per stage),
  * "kernel_timing" (0/1: events around each kernel launch, see rh_get_kernel_profile),
  * "count_visits" (0/1: instrumented trace kernels count node/triangle/instance visits),
- * "lds_stack_depth" (4..64 LDS entries of the per-lane traversal stack). */
+ * "lds_stack_depth" (4..64 LDS entries of the per-lane traversal stack),
+ * "defer_nee" (0/1: shadow rays of next-event estimation as their own stage). */
 int rh_set_option(rh_context *ctx, const char *name, int value);
 int rh_get_option(rh_context *ctx, const char *name, int *value);
 
 /* Per-kernel profile collected while "kernel_timing" is on (HIP events around
  * every trace/shade/sort launch of rh_render_pass) and, while "count_visits"
  * is on, the traversal work of the trace kernels.  Kinds: 0 trace, 1 shade,
- * 2 sort (rh_kernel_profile in rh_types.h).  No counterpart in the reference
+ * 2 sort, 3 shadow (rh_kernel_profile in rh_types.h).  No counterpart in the reference
  * beyond stats_t. */
 int rh_get_kernel_profile(rh_context *ctx, rh_kernel_profile *out);
 int rh_reset_kernel_profile(rh_context *ctx);

@@ -288,10 +288,10 @@ typedef struct rh_ray_counts {
     unsigned long long sorted;
 } rh_ray_counts;
 
-/* per-kernel profile: kinds 0 trace, 1 shade, 2 sort */
+/* per-kernel profile: kinds 0 trace, 1 shade, 2 sort, 3 shadow (deferred NEE visibility) */
 typedef struct rh_kernel_profile {
-    double ms[3];
-    unsigned long long launches[3];
+    double ms[4];
+    unsigned long long launches[4];
     unsigned long long node_visits, tri_tests, instance_tests;
 } rh_kernel_profile;
 

@@ -78,7 +78,7 @@ class RayCounts(Structure):
 
 
 class KernelProfile(Structure):
-    _fields_ = [("ms", C.c_double * 3), ("launches", c_uint64 * 3), ("node_visits", c_uint64),
+    _fields_ = [("ms", C.c_double * 4), ("launches", c_uint64 * 4), ("node_visits", c_uint64),
                 ("tri_tests", c_uint64), ("instance_tests", c_uint64)]
 
 
@@ -284,7 +284,7 @@ class Renderer:
     def kernel_profile(self) -> dict:
         p = KernelProfile()
         self.lib.check(self.lib.lib.rhx_get_kernel_profile(self.h, C.byref(p)), "rhx_get_kernel_profile")
-        kinds = ("trace", "shade", "sort")
+        kinds = ("trace", "shade", "sort", "shadow")
         d = {f"{k}_ms": p.ms[i] for i, k in enumerate(kinds)}
         d.update({f"{k}_launches": p.launches[i] for i, k in enumerate(kinds)})
         d.update(node_visits=p.node_visits, tri_tests=p.tri_tests, instance_tests=p.instance_tests)

@@ -28,6 +28,7 @@ constexpr int kBlock = 256;        // threads per block of the ray kernels (4 wa
 constexpr int kSpillDepth = 48;    // per-ray global stack entries beyond the LDS part
 constexpr int kSortItems = 16;     // keys per thread in the radix passes
 constexpr int kSortTile = kBlock * kSortItems;
+constexpr uint32_t kMaxDeferredLights = 8; // lights per scene for the deferred NEE stage
 
 // ------------------------------------------------------------------ tables
 // Ray-hash tables of the reference (Core.cpp:106-150): sphere-sector lookup
@@ -290,9 +291,18 @@ struct ShadeArgs {
     uint32_t lds_depth;
     unsigned long long *shadow_counter;
     unsigned long long *overflow;
+    // deferred next-event estimation (null nee_ray: visibility traced inline)
+    float4 *nee_ray;
+    int2 *nee_aux;
+    float4 *nee_term;
+    uint32_t nee_slots; // slots per shaded ray (= light count)
+    float4 *pend_c;     // ray throughput xyz, pixel index bits
+    float4 *pend_alb;   // albedo xyz, flags bits (1: multiply by albedo, 2: primary)
+    int *pend_count;    // deferred terms of the ray (0: nothing pending)
 };
 
-__global__ void __launch_bounds__(kBlock) k_shade(ShadeArgs a) {
+// Defer: direct light goes to the NEE stage (no traversal in this kernel).
+template <bool Defer> __global__ void __launch_bounds__(kBlock) k_shade(ShadeArgs a) {
     extern __shared__ uint32_t lds_stack[];
     const int n = a.count ? *a.count : a.n;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -338,7 +348,24 @@ __global__ void __launch_bounds__(kBlock) k_shade(ShadeArgs a) {
             pi.rand_index = pi.index;
             if (pi.settings.flags & RH_USE_COHERENT_SAMPLING) pi.rand_index = c_sampling_pattern[(py % 8) * 8 + (px % 8)];
         }
-        shade_surface(a.sc, pi, hit, ray, a.halton, st, cnt, so);
+        NeeSink sink;
+        NeeSink *sinkp = nullptr;
+        if (Defer) {
+            sink = NeeSink{a.nee_ray, a.nee_aux, a.nee_term, i * a.nee_slots, 0};
+            sinkp = &sink;
+        }
+        shade_surface(a.sc, pi, hit, ray, a.halton, st, cnt, so, sinkp);
+        if (Defer) {
+            // the colour written below carries a zero direct term; the combine
+            // stage adds ray.c * (sum of visible terms [* albedo]) on top
+            const bool pending = so.nee_pending && sink.count > 0;
+            a.pend_count[i] = pending ? (int)sink.count : 0;
+            if (pending) {
+                a.pend_c[i] = make_float4(ray.c.x, ray.c.y, ray.c.z, __int_as_float(py * a.w + px));
+                a.pend_alb[i] = make_float4(so.albedo.x, so.albedo.y, so.albedo.z,
+                                            __int_as_float(so.consider_albedo ? 1 : 0));
+            }
+        }
 
         if (a.temp) {
             float4 &dst = a.temp[(size_t)py * a.w + px];
@@ -362,6 +389,88 @@ __global__ void __launch_bounds__(kBlock) k_shade(ShadeArgs a) {
     }
     if (a.shadow_counter) wave_add_u64(a.shadow_counter, cnt.shadow_rays);
     if (st.overflow && a.overflow) atomicAdd(a.overflow, (unsigned long long)st.overflow);
+}
+
+struct ShadowArgs {
+    Scene sc;
+    const int *count; // shaded rays of this stage (device); nullptr -> n
+    int n;
+    uint32_t slots;
+    const int *pend_count;
+    const float4 *nee_ray;
+    const int2 *nee_aux;
+    uint8_t *nee_vis;
+    const float *halton;
+    uint32_t *spill;
+    uint32_t spill_stride;
+    uint32_t lds_depth;
+    unsigned long long *shadow_counter;
+    unsigned long long *overflow;
+};
+
+// Visibility of each deferred light sample: ComputeVisibility
+// (CoreRef.cpp:1801-1896), any-hit traversal stepping through transparent
+// surfaces.  One lane per slot; empty slots exit at once.
+__global__ void __launch_bounds__(kBlock) k_shadow(ShadowArgs a) {
+    extern __shared__ uint32_t lds_stack[];
+    const int n = a.count ? *a.count : a.n;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = k / a.slots, j = k % a.slots;
+    ShadeCounters cnt;
+    cnt.shadow_rays = 0;
+    const bool active = (int)i < n && (int)j < a.pend_count[(int)i < n ? i : 0];
+    if (active) {
+        const float4 r = a.nee_ray[k];
+        const int2 x = a.nee_aux[k];
+        const rh_light &l = a.sc.lights[__float_as_uint(r.w)];
+        Stack st;
+        st.lds = lds_stack;
+        st.spill = a.spill + k;
+        st.spill_stride = a.spill_stride;
+        st.lds_depth = a.lds_depth;
+        st.spill_depth = kSpillDepth;
+        st.lane = threadIdx.x;
+        st.block = blockDim.x;
+        st.sp = 0;
+        st.overflow = 0;
+        const float vis = compute_visibility(a.sc, f3{r.x, r.y, r.z}, ld3(l.pos), a.halton, x.x, x.y, st, cnt);
+        a.nee_vis[k] = vis != 0.0f ? 1 : 0;
+        if (st.overflow && a.overflow) atomicAdd(a.overflow, (unsigned long long)st.overflow);
+    }
+    if (a.shadow_counter) wave_add_u64(a.shadow_counter, cnt.shadow_rays);
+}
+
+// Adds each shaded ray's deferred direct light: ray.c * (sum of visible terms
+// in light order [* albedo]), to its pixel (or to the per-ray output).
+__global__ void __launch_bounds__(kBlock) k_nee_combine(const int *count, int n_static, uint32_t slots,
+                                                        const int *__restrict__ pend_count,
+                                                        const float4 *__restrict__ pend_c,
+                                                        const float4 *__restrict__ pend_alb,
+                                                        const float4 *__restrict__ nee_term,
+                                                        const uint8_t *__restrict__ nee_vis, float4 *temp,
+                                                        float4 *out_rgba) {
+    const int n = count ? *count : n_static;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int m = pend_count[i];
+    if (!m) return;
+    f3 col = f3{0.0f, 0.0f, 0.0f};
+    const uint32_t base = (uint32_t)i * slots;
+    for (int j = 0; j < m; j++) {
+        if (nee_vis[base + j]) {
+            const float4 t = nee_term[base + j];
+            col = col + f3{t.x, t.y, t.z};
+        }
+    }
+    const float4 pc = pend_c[i], pa = pend_alb[i];
+    if (__float_as_int(pa.w) & 1) col = col * f3{pa.x, pa.y, pa.z};
+    const f3 add = f3{pc.x * col.x, pc.y * col.y, pc.z * col.z};
+    float4 *dst = temp ? &temp[__float_as_int(pc.w)] : &out_rgba[i];
+    float4 v = *dst;
+    v.x += add.x;
+    v.y += add.y;
+    v.z += add.z;
+    *dst = v;
 }
 
 // MixWith + tonemap (FramebufferRef.cpp:80-86, RendererRef.cpp:301-324)
@@ -565,6 +674,16 @@ struct rh_context {
     uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
     uint32_t *hist = nullptr;
     uint32_t *spill = nullptr;
+    size_t spill_cap = 0; // lanes with a spill slice (stride of the spill array)
+    // deferred next-event estimation: slots per shaded ray = light count (0: inline)
+    uint32_t nee_slots = 0;
+    size_t nee_cap = 0;
+    float4 *nee_ray = nullptr, *nee_term = nullptr;
+    int2 *nee_aux = nullptr;
+    uint8_t *nee_vis = nullptr;
+    float4 *pend_c = nullptr, *pend_alb = nullptr;
+    int *pend_count = nullptr;
+    int defer_nee = 1;
     int *counters = nullptr; // [0], [1]: queue counts
     unsigned long long *stats_dev = nullptr; // [0] primary [1] secondary [2] shadow [3] sorted [4] overflow
     float *halton = nullptr;
@@ -636,8 +755,10 @@ int alloc_frame(rh_context *c, int w, int h) {
     return 0;
 }
 
+int ensure_nee(rh_context *c);
+
 int ensure_rays(rh_context *c, size_t n) {
-    if (n <= c->max_rays && c->rays[0]) return 0;
+    if (n <= c->max_rays && c->rays[0]) return ensure_nee(c);
     c->max_rays = std::max(n, c->max_rays);
     const size_t m = c->max_rays;
     for (int k = 0; k < 2; k++) {
@@ -652,13 +773,46 @@ int ensure_rays(rh_context *c, size_t n) {
     }
     free_ptr(c->hits);
     free_ptr(c->hist);
-    free_ptr(c->spill);
     RH_CHECK(hipMalloc(&c->hits, m * sizeof(rh_hit)));
     const size_t tiles = (m + kSortTile - 1) / kSortTile;
     RH_CHECK(hipMalloc(&c->hist, 256 * tiles * sizeof(uint32_t)));
-    // spill slices are indexed by ray index: stride = max rays
-    RH_CHECK(hipMalloc(&c->spill, (size_t)kSpillDepth * m * sizeof(uint32_t)));
+    return ensure_nee(c);
+}
+
+// Spill slices are indexed by lane (ray or shadow slot): the array holds
+// kSpillDepth entries for each of spill_cap lanes, stride spill_cap.
+int ensure_spill(rh_context *c, size_t lanes) {
+    if (lanes <= c->spill_cap && c->spill) return 0;
+    free_ptr(c->spill);
+    c->spill = nullptr;
+    c->spill_cap = std::max(lanes, c->spill_cap);
+    RH_CHECK(hipMalloc(&c->spill, (size_t)kSpillDepth * c->spill_cap * sizeof(uint32_t)));
     return 0;
+}
+
+// Buffers of the deferred NEE stage for the current scene and frame size.
+int ensure_nee(rh_context *c) {
+    const size_t m = c->max_rays;
+    if (!c->nee_slots) return ensure_spill(c, m);
+    const size_t cap = m * c->nee_slots;
+    if (cap > c->nee_cap || !c->nee_ray) {
+        free_ptr(c->nee_ray);
+        free_ptr(c->nee_term);
+        free_ptr(c->nee_aux);
+        free_ptr(c->nee_vis);
+        free_ptr(c->pend_c);
+        free_ptr(c->pend_alb);
+        free_ptr(c->pend_count);
+        c->nee_cap = cap;
+        RH_CHECK(hipMalloc(&c->nee_ray, cap * sizeof(float4)));
+        RH_CHECK(hipMalloc(&c->nee_term, cap * sizeof(float4)));
+        RH_CHECK(hipMalloc(&c->nee_aux, cap * sizeof(int2)));
+        RH_CHECK(hipMalloc(&c->nee_vis, cap * sizeof(uint8_t)));
+        RH_CHECK(hipMalloc(&c->pend_c, m * sizeof(float4)));
+        RH_CHECK(hipMalloc(&c->pend_alb, m * sizeof(float4)));
+        RH_CHECK(hipMalloc(&c->pend_count, m * sizeof(int)));
+    }
+    return ensure_spill(c, std::max(m, cap));
 }
 
 template <typename T> int upload(rh_context *c, const T *src, size_t count, const T **dst) {
@@ -685,7 +839,7 @@ TraceArgs trace_args(rh_context *c, const rh_ray *rays, const int *count, int n,
     a.tmax = tmax;
     a.hits = hits;
     a.spill = c->spill;
-    a.spill_stride = (uint32_t)c->max_rays;
+    a.spill_stride = (uint32_t)c->spill_cap;
     a.lds_depth = (uint32_t)c->lds_depth;
     a.ray_counter = ray_counter;
     a.overflow = c->stats_dev + 4;
@@ -743,17 +897,60 @@ ShadeArgs shade_args(rh_context *c) {
     memset(&a, 0, sizeof(a));
     a.sc = c->sc;
     a.spill = c->spill;
-    a.spill_stride = (uint32_t)c->max_rays;
+    a.spill_stride = (uint32_t)c->spill_cap;
     a.lds_depth = (uint32_t)c->lds_depth;
     a.shadow_counter = c->stats_dev + 2;
     a.overflow = c->stats_dev + 4;
+    if (c->defer_nee && c->nee_slots) {
+        a.nee_ray = c->nee_ray;
+        a.nee_aux = c->nee_aux;
+        a.nee_term = c->nee_term;
+        a.nee_slots = c->nee_slots;
+        a.pend_c = c->pend_c;
+        a.pend_alb = c->pend_alb;
+        a.pend_count = c->pend_count;
+    }
     return a;
 }
 
+// shade, then (deferred NEE) visibility of the recorded light samples and
+// their combination into the pixels / per-ray colours
 int launch_shade(rh_context *c, const ShadeArgs &a, int grid_rays) {
     const int blocks = std::max(1, (grid_rays + kBlock - 1) / kBlock);
-    KernelMark km(c, 1);
-    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(kBlock), lds_bytes(c), c->stream, a);
+    {
+        KernelMark km(c, 1);
+        if (a.nee_ray) {
+            hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(kBlock), 0, c->stream, a);
+        } else {
+            hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(kBlock), lds_bytes(c), c->stream, a);
+        }
+        RH_CHECK(hipGetLastError());
+    }
+    if (!a.nee_ray) return 0;
+    ShadowArgs s;
+    s.sc = c->sc;
+    s.count = a.count;
+    s.n = a.n;
+    s.slots = a.nee_slots;
+    s.pend_count = a.pend_count;
+    s.nee_ray = a.nee_ray;
+    s.nee_aux = a.nee_aux;
+    s.nee_vis = c->nee_vis;
+    s.halton = a.halton;
+    s.spill = c->spill;
+    s.spill_stride = (uint32_t)c->spill_cap;
+    s.lds_depth = (uint32_t)c->lds_depth;
+    s.shadow_counter = c->stats_dev + 2;
+    s.overflow = c->stats_dev + 4;
+    const size_t lanes = (size_t)grid_rays * a.nee_slots;
+    {
+        KernelMark km(c, 3);
+        hipLaunchKernelGGL(k_shadow, dim3((unsigned)std::max<size_t>(1, (lanes + kBlock - 1) / kBlock)), dim3(kBlock),
+                           lds_bytes(c), c->stream, s);
+        RH_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_nee_combine, dim3(blocks), dim3(kBlock), 0, c->stream, a.count, a.n, a.nee_slots,
+                       a.pend_count, a.pend_c, a.pend_alb, a.nee_term, c->nee_vis, a.temp, a.out_rgba);
     RH_CHECK(hipGetLastError());
     return 0;
 }
@@ -870,6 +1067,13 @@ void rh_destroy(rh_context *c) {
     free_ptr(c->hits);
     free_ptr(c->hist);
     free_ptr(c->spill);
+    free_ptr(c->nee_ray);
+    free_ptr(c->nee_term);
+    free_ptr(c->nee_aux);
+    free_ptr(c->nee_vis);
+    free_ptr(c->pend_c);
+    free_ptr(c->pend_alb);
+    free_ptr(c->pend_count);
     free_ptr(c->counters);
     free_ptr(c->stats_dev);
     free_ptr(c->halton);
@@ -942,6 +1146,9 @@ int rh_upload_scene(rh_context *c, const rh_scene_view *v) {
     rc = upload(c, leaves.data(), leaves.size(), &s.light_leaves);
     if (rc) return rc;
     s.light_leaf_count = (uint32_t)leaves.size();
+    // deferred NEE keeps one slot per light for every shaded ray; scenes with
+    // many lights trace visibility inline instead
+    c->nee_slots = (v->lights_count >= 1 && v->lights_count <= kMaxDeferredLights) ? v->lights_count : 0;
     if (s.macro_root != RH_INVALID && (!v->nodes || s.macro_root >= v->nodes_count))
         return arg_error("macro tree root outside the node array");
     c->sc = s;
@@ -1110,6 +1317,7 @@ int rh_set_option(rh_context *c, const char *name, int value) {
     else if (n == "stage_timing") c->stage_timing = value != 0;
     else if (n == "kernel_timing") c->kernel_timing = value != 0;
     else if (n == "count_visits") c->count_visits = value != 0;
+    else if (n == "defer_nee") c->defer_nee = value != 0;
     else if (n == "lds_stack_depth") {
         if (value < 4 || value > 64) return arg_error("lds_stack_depth must be 4..64");
         c->lds_depth = value;

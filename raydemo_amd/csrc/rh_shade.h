@@ -159,19 +159,89 @@ __device__ __forceinline__ f3 compute_direct_lighting(const Scene &sc, f3 I, f3 
     return col;
 }
 
+// Deferred next-event estimation.  ComputeVisibility only ever returns 0 or 1,
+// and the reference adds each light's term ((l.col*dot1)*vis*atten)*brdf to
+// the running colour in light order.  So shading can record, per light, the
+// shadow-ray end points and the term with vis = 1 (x*1 == x exactly); a
+// separate any-hit stage resolves each visibility, and a combine stage adds
+// the visible terms in the same order: bit-identical to the inline form.
+// Slots are per shaded ray: [ray * max_lights, ray * max_lights + count).
+struct NeeSink {
+    float4 *ray;  // p1.xyz, light index (bits)
+    int2 *aux;    // halton index hi, rand_hash2
+    float4 *term; // the light's term with visibility 1
+    uint32_t base;
+    uint32_t count;
+};
+
+__device__ __forceinline__ void defer_light(const Scene &sc, uint32_t light_index, const rh_light &l, f3 I, f3 P, f3 N,
+                                            f3 B, f3 plane_N, float sigma, const float *halton, int hi, int rand_hash2,
+                                            float rand_offset, float rand_offset2, NeeSink &sink) {
+    const f3 lpos = ld3(l.pos);
+    f3 L = P - lpos;
+    const float distance = length(L);
+    const float d = smax(distance - l.radius, 0.0f);
+    L = L / distance;
+    const float z = fract_modf(halton[hi + 0] + rand_offset);
+    const float dir = __builtin_sqrtf(z);
+    const float phi = 2 * RH_PI * fract_modf(halton[hi + 1] + rand_offset2);
+    const float cos_phi = rcos(phi), sin_phi = rsin(phi);
+    const f3 TT = cross(L, B);
+    const f3 BB = cross(L, TT);
+    const f3 V = (dir * sin_phi) * BB + __builtin_sqrtf(1.0f - dir) * L + (dir * cos_phi) * TT;
+    L = normalize(lpos + V * l.radius - P);
+    const float denom = d / l.radius + 1.0f;
+    float atten = 1.0f / (denom * denom);
+    atten = (atten - RH_LIGHT_ATTEN_CUTOFF / l.brightness) / (1.0f - RH_LIGHT_ATTEN_CUTOFF);
+    atten = smax(atten, 0.0f);
+    const float _dot1 = smax(dot(L, N), 0.0f);
+    const float _dot2 = dot(L, ld3(l.dir));
+    if (_dot1 > RH_FLT_EPS && _dot2 > l.spot && (l.brightness * atten) > RH_FLT_EPS) {
+        const f3 p1 = P + RH_HIT_BIAS * plane_N;
+        const f3 term = ((ld3(l.col) * _dot1) * atten) * brdf_oren_nayar(L, I, N, B, sigma);
+        const uint32_t k = sink.base + sink.count++;
+        sink.ray[k] = make_float4(p1.x, p1.y, p1.z, __uint_as_float(light_index));
+        sink.aux[k] = make_int2(hi, rand_hash2);
+        sink.term[k] = make_float4(term.x, term.y, term.z, 0.0f);
+    }
+}
+
+__device__ __forceinline__ void defer_direct_lighting(const Scene &sc, f3 I, f3 P, f3 N, f3 B, f3 plane_N, float sigma,
+                                                      const float *halton, int hi, int rand_hash2, float rand_offset,
+                                                      float rand_offset2, NeeSink &sink) {
+    for (uint32_t k = 0; k < sc.light_leaf_count; k++) {
+        const NodeBox n = load_node(sc.light_leaves, k);
+        if (!(P.x > n.bmin[0] && P.x < n.bmax[0] && P.y > n.bmin[1] && P.y < n.bmax[1] && P.z > n.bmin[2] &&
+              P.z < n.bmax[2]))
+            continue;
+        const uint32_t first = n.a & RH_PRIM_INDEX_BITS;
+        for (uint32_t i = first; i < first + n.b; i++) {
+            const uint32_t li = sc.li_indices[i];
+            defer_light(sc, li, sc.lights[li], I, P, N, B, plane_N, sigma, halton, hi, rand_hash2, rand_offset,
+                        rand_offset2, sink);
+        }
+    }
+}
+
 __device__ __forceinline__ float pow5(float v) { return (v * v) * (v * v) * v; }
 
 struct ShadeOut {
     f4 col;
     bool emit;
+    bool nee_pending; // diffuse hit whose direct light waits for the shadow stage
+    bool consider_albedo;
+    f3 albedo;
     RayFull next;
 };
 
 // ShadeSurface (CoreRef.cpp:2055-2391) for one (ray, hit) pair.
+// With a sink, a diffuse hit's direct light is deferred (see NeeSink) and
+// out.nee_pending is set; without one, visibility is traced inline.
 __device__ __forceinline__ void shade_surface(const Scene &sc, const rh_pass_info &pi, const Hit &inter,
                                               const RayFull &ray, const float *halton, Stack &st, ShadeCounters &cnt,
-                                              ShadeOut &out) {
+                                              ShadeOut &out, NeeSink *sink = nullptr) {
     out.emit = false;
+    out.nee_pending = false;
     const uint32_t flags = pi.settings.flags;
     const bool add_env = !(flags & RH_NO_BACKGROUND) || pi.bounce > 2;
     const bool add_direct = !(flags & RH_SKIP_DIRECT_LIGHT) || pi.bounce > 2;
@@ -299,9 +369,17 @@ __device__ __forceinline__ void shade_surface(const Scene &sc, const rh_pass_inf
 
     if (mat->type == RH_MAT_DIFFUSE) {
         if (add_direct) {
-            col = compute_direct_lighting(sc, I, P, N, B, plane_N, mat->roughness, halton, hi, rand_hash2,
-                                          rand_offset, rand_offset2, st, cnt);
-            if (consider_albedo) col = col * alb;
+            if (sink) {
+                defer_direct_lighting(sc, I, P, N, B, plane_N, mat->roughness, halton, hi, rand_hash2, rand_offset,
+                                      rand_offset2, *sink);
+                out.nee_pending = true;
+                out.consider_albedo = consider_albedo;
+                out.albedo = alb;
+            } else {
+                col = compute_direct_lighting(sc, I, P, N, B, plane_N, mat->roughness, halton, hi, rand_hash2,
+                                              rand_offset, rand_offset2, st, cnt);
+                if (consider_albedo) col = col * alb;
+            }
         }
         if (diff_depth < pi.settings.max_diff_depth && total_depth < pi.settings.max_total_depth) {
             const float u1 = fract_modf(halton[hi + 0] + rand_offset);
