@@ -55,7 +55,7 @@ def run_gpu(args, lib, A, scenes):
     results = {}
     for sort_on in (1, 0):
         r.set_option("sort_rays", sort_on)
-        r.set_option("kernel_timing", 1 if sort_on else 0)
+        r.set_option("kernel_timing", 1)
         r.clear()
         reg = A.Region(lib, 0, 0, W, H)
         for _ in range(args.warmup):
@@ -224,6 +224,7 @@ def main():
                    "triangles": spec.triangles, "width": args.width, "height": args.height, "spp": args.steps,
                    "depth": args.depth, "parallelism": "single GPU"},
         "modes": {"sort_on_mrays": round(on["mrays"], 2), "sort_off_mrays": round(off["mrays"], 2),
+                  "sort_off_kernel_ms": {k: round(v, 3) for k, v in off["prof"].items() if k.endswith("_ms")},
                   "rays_per_pass": int(on["rays"] / args.steps), "ray_counts": on["counts"]},
         "kernel_ms": {k: round(v, 3) for k, v in prof.items() if k.endswith("_ms")},
         "roofline": roofline,

@@ -550,29 +550,56 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const uint32_t *__restric
     hist[threadIdx.x * num_tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Radix pass 2/3: exclusive scan of the digit-major histogram (one block).
-__global__ void __launch_bounds__(1024) k_radix_scan(uint32_t *__restrict__ hist, int total) {
-    __shared__ uint32_t partial[1024];
-    const int t = threadIdx.x;
-    const int per = (total + 1023) / 1024;
-    const int b = t * per, e = min(b + per, total);
-    uint32_t s = 0;
-    for (int i = b; i < e; i++) s += hist[i];
-    partial[t] = s;
+// Wave-level inclusive scan (64 lanes) with cross-lane shuffles.
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(v, off);
+        if (lane >= (uint32_t)off) v += u;
+    }
+    return v;
+}
+
+// Radix pass 2a/3: one block per digit scans that digit's row of per-tile
+// counts (exclusive, in place) and records the digit's total.
+__global__ void __launch_bounds__(kBlock) k_radix_scan_rows(uint32_t *__restrict__ hist, int num_tiles,
+                                                            uint32_t *__restrict__ totals) {
+    __shared__ uint32_t wsum[kBlock / 64];
+    __shared__ uint32_t carry;
+    uint32_t *row = hist + (size_t)blockIdx.x * num_tiles;
+    const int t = threadIdx.x, wave = t >> 6;
+    if (t == 0) carry = 0;
     __syncthreads();
-    // Hillis-Steele over the 1024 partial sums
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = t >= off ? partial[t - off] : 0u;
+    for (int base = 0; base < num_tiles; base += kBlock) {
+        const int i = base + t;
+        const uint32_t v = i < num_tiles ? row[i] : 0u;
+        const uint32_t inc = wave_inclusive_scan(v);
+        if ((t & 63) == 63) wsum[wave] = inc;
         __syncthreads();
-        partial[t] += v;
+        uint32_t before = carry;
+        for (int w = 0; w < wave; w++) before += wsum[w];
+        if (i < num_tiles) row[i] = before + inc - v;
+        __syncthreads();
+        if (t == kBlock - 1) carry = before + inc;
         __syncthreads();
     }
-    uint32_t run = t ? partial[t - 1] : 0u;
-    for (int i = b; i < e; i++) {
-        const uint32_t v = hist[i];
-        hist[i] = run;
-        run += v;
+    if (t == 0) totals[blockIdx.x] = carry;
+}
+
+// Radix pass 2b/3: adds each digit's base (sum of the smaller digits' totals).
+__global__ void __launch_bounds__(kBlock) k_radix_scan_bases(uint32_t *__restrict__ hist, int num_tiles,
+                                                             const uint32_t *__restrict__ totals) {
+    __shared__ uint32_t base;
+    const int d = blockIdx.x;
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int k = 0; k < d; k++) s += totals[k];
+        base = s;
     }
+    __syncthreads();
+    uint32_t *row = hist + (size_t)d * num_tiles;
+    for (int i = threadIdx.x; i < num_tiles; i += kBlock) row[i] += base;
 }
 
 // Radix pass 3/3: stable scatter.  Within a round of 256 keys, each wave finds
@@ -775,7 +802,8 @@ int ensure_rays(rh_context *c, size_t n) {
     free_ptr(c->hist);
     RH_CHECK(hipMalloc(&c->hits, m * sizeof(rh_hit)));
     const size_t tiles = (m + kSortTile - 1) / kSortTile;
-    RH_CHECK(hipMalloc(&c->hist, 256 * tiles * sizeof(uint32_t)));
+    // per-tile digit counts (digit-major) followed by the 256 digit totals
+    RH_CHECK(hipMalloc(&c->hist, (256 * tiles + 256) * sizeof(uint32_t)));
     return ensure_nee(c);
 }
 
@@ -964,7 +992,10 @@ int radix_sort(rh_context *c, const int *count, int max_n) {
     for (int shift = 0; shift < 32; shift += 8) {
         hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kBlock), 0, c->stream, c->keys[src], count, max_n, shift,
                            c->hist, tiles);
-        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, c->stream, c->hist, 256 * tiles);
+        hipLaunchKernelGGL(k_radix_scan_rows, dim3(256), dim3(kBlock), 0, c->stream, c->hist, tiles,
+                           c->hist + (size_t)256 * tiles);
+        hipLaunchKernelGGL(k_radix_scan_bases, dim3(256), dim3(kBlock), 0, c->stream, c->hist, tiles,
+                           c->hist + (size_t)256 * tiles);
         hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kBlock), 0, c->stream, c->keys[src], c->vals[src],
                            c->keys[src ^ 1], c->vals[src ^ 1], count, max_n, shift, c->hist, tiles);
         src ^= 1;
