@@ -7,15 +7,23 @@ region is exactly K passes bracketed by device synchronisation; rays counted
 on the device (primary + secondary + shadow) divided by the wall time give
 Mrays/s.  Scene build and upload happen before the timed region.
 
+The launch knobs (LDS share of the traversal stack, ray sort on/off and the
+number of key bits it orders by) are chosen first by raydemo_amd.autotune on
+a few passes per candidate; every candidate renders the identical image.  The
+headline value is the tuned setting; sort-off (coherent order) and full 32-bit
+sort modes are timed the same way and reported beside it.
+
 Also reported on the same JSON line:
   roofline      the trace kernel's achieved algorithmic bandwidth (bytes per
                 traced ray from the node/triangle/instance visit counts of an
                 instrumented replay of the same passes) over its HIP-event time
-                inside the timed region, against 8 TB/s
-  cpu_baseline  the reference Ray library's Ref backend (or, where that is not
-                built, the CPU oracle) on the host cores, 128x128 buckets on a
-                thread pool, over a bounded sample of the same workload
-  modes         Mrays/s with secondary-ray sorting on and off
+                inside the timed region, against 8 TB/s; traffic = HBM bytes
+                per launch from the copy-calibrated FETCH_SIZE/WRITE_SIZE
+                counters committed in profiles/trace_traffic.json
+  cpu_baseline  the reference Ray library's Ref backend on the host cores,
+                128x128 buckets on a thread pool, over a bounded sample of the
+                same workload
+  modes         Mrays/s and per-kernel ms of each timed mode
 
 Usage: python bench.py [--gpus 1] [--steps 64] [--warmup 2]
 """
@@ -44,7 +52,34 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def timed_passes(r, s, A, lib, W, H, steps, warmup, opts):
+    """K progressive passes under opts, bracketed by device synchronisation."""
+    for k, v in opts.items():
+        r.set_option(k, v)
+    r.set_option("kernel_timing", 1)
+    r.clear()
+    reg = A.Region(lib, 0, 0, W, H)
+    for _ in range(warmup):
+        r.render(s, reg)
+    r.synchronize()
+    r.reset_stats()
+    r.reset_kernel_profile()
+    r.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        r.render(s, reg)
+    r.synchronize()
+    dt = time.perf_counter() - t1
+    rc = r.ray_counts()
+    rays = rc["primary"] + rc["secondary"] + rc["shadow"]
+    prof = r.kernel_profile()
+    r.set_option("kernel_timing", 0)
+    return dict(opts=dict(opts), seconds=dt, rays=rays, counts=rc, prof=prof, mrays=rays / dt / 1e6)
+
+
 def run_gpu(args, lib, A, scenes):
+    from raydemo_amd import autotune
+
     W, H = args.width, args.height
     r = A.Renderer(lib, W, H, A.RENDERER_HIP, use_wide_bvh=False)
     s = r.create_scene()
@@ -52,32 +87,30 @@ def run_gpu(args, lib, A, scenes):
     spec = scenes.triangle_soup(s, width=W, height=H, spp=args.steps, depth=args.depth)
     log(f"scene: {spec.triangles} triangles, host build {time.time() - t0:.1f}s")
 
-    results = {}
-    for sort_on in (1, 0):
-        r.set_option("sort_rays", sort_on)
-        r.set_option("kernel_timing", 1)
-        r.clear()
-        reg = A.Region(lib, 0, 0, W, H)
-        for _ in range(args.warmup):
-            r.render(s, reg)
-        r.synchronize()
-        r.reset_stats()
-        r.reset_kernel_profile()
-        r.synchronize()
-        t1 = time.perf_counter()
-        for _ in range(args.steps):
-            r.render(s, reg)
-        r.synchronize()
-        dt = time.perf_counter() - t1
-        rc = r.ray_counts()
-        rays = rc["primary"] + rc["secondary"] + rc["shadow"]
-        prof = r.kernel_profile()
-        results[sort_on] = dict(seconds=dt, rays=rays, counts=rc, prof=prof, mrays=rays / dt / 1e6)
-        log(f"sort={sort_on}: {args.steps} passes in {dt:.3f}s, {rays / 1e6:.1f} Mrays -> {rays / dt / 1e6:.1f} Mrays/s")
+    # launch knobs (LDS stack share, sort on/off and key bits, path tail): all give the same image
+    if args.knobs:
+        fixed = dict(zip(autotune.KNOBS, (int(v) for v in args.knobs.split(","))))
+        tuned = autotune.TuneResult(best=fixed)
+        log(f"knobs given: {fixed}")
+    else:
+        tuned = autotune.tune(r, s, lambda: A.Region(lib, 0, 0, W, H), passes=args.tune_passes)
+        log(f"autotune ({tuned.seconds:.1f}s): best {tuned.best}")
+    depth = tuned.best["lds_stack_depth"]
 
-    # instrumented replay of the same passes (sorting on) for the visit counts
-    r.set_option("sort_rays", 1)
-    r.set_option("kernel_timing", 0)
+    results = {}
+    modes = {"tuned": dict(tuned.best),
+             "sort_off": {"lds_stack_depth": depth, "sort_rays": 0, "sort_bits": 32, "path_tail": -1},
+             "sort_on": {"lds_stack_depth": depth, "sort_rays": 1, "sort_bits": 32, "path_tail": -1}}
+    if args.only_tuned:
+        modes = {"tuned": modes["tuned"]}
+    for name, opts in modes.items():
+        res = timed_passes(r, s, A, lib, W, H, args.steps, args.warmup, opts)
+        results[name] = res
+        log(f"{name} {opts}: {args.steps} passes in {res['seconds']:.3f}s, {res['rays'] / 1e6:.1f} Mrays -> "
+            f"{res['mrays']:.1f} Mrays/s")
+
+    # instrumented replay of the tuned passes for the traversal visit counts
+    autotune.apply(r, tuned.best)
     r.set_option("count_visits", 1)
     r.clear()
     reg = A.Region(lib, 0, 0, W, H)
@@ -92,8 +125,8 @@ def run_gpu(args, lib, A, scenes):
     vis = r.kernel_profile()
     rc2 = r.ray_counts()
     r.set_option("count_visits", 0)
-    traced = rc2["primary"] + rc2["secondary"]
-    return spec, results, vis, traced
+    traced = vis["counted_rays"]  # rays of the counting k_trace launches (the path tail is not counted)
+    return spec, results, vis, traced, tuned
 
 
 def run_cpu_baseline(args, A, scenes, spp_sample):
@@ -148,8 +181,11 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--depth", type=int, default=8)
-    ap.add_argument("--cpu-spp", type=int, default=8, help="full-frame passes of the CPU baseline sample")
+    ap.add_argument("--cpu-spp", type=int, default=64, help="full-frame passes of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune-passes", type=int, default=6, help="passes per autotuner candidate")
+    ap.add_argument("--knobs", default="", help="skip the autotuner: lds_stack_depth,sort_rays,sort_bits,path_tail")
+    ap.add_argument("--only-tuned", action="store_true", help="time only the tuned (or given) knobs")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -157,15 +193,14 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from raydemo_amd import capi as A, host_lib, scenes
+    from raydemo_amd import autotune, capi as A, host_lib, scenes
 
     lib = host_lib()
-    spec, res, vis, traced = run_gpu(args, lib, A, scenes)
-    on, off = res[1], res[0]
-    best = on
-    prof = on["prof"]
+    spec, res, vis, traced, tuned = run_gpu(args, lib, A, scenes)
+    best = res["tuned"]
+    prof = best["prof"]
 
-    # roofline of the trace kernel (closest hit), sorting on
+    # roofline of the trace kernel (closest hit), tuned mode
     trace_ms = prof["trace_ms"]
     bytes_alg = (traced * (B_RAY + B_HIT) + vis["node_visits"] * B_NODE + vis["instance_tests"] * B_INST +
                  vis["tri_tests"] * B_TRI)
@@ -220,12 +255,16 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"triangle_soup {spec.triangles} tris, {args.width}x{args.height}, "
-                               f"{args.steps} spp, depth {args.depth}, sorted secondary rays",
+                               f"{args.steps} spp, depth {args.depth}, autotuned launch knobs",
                    "triangles": spec.triangles, "width": args.width, "height": args.height, "spp": args.steps,
-                   "depth": args.depth, "parallelism": "single GPU"},
-        "modes": {"sort_on_mrays": round(on["mrays"], 2), "sort_off_mrays": round(off["mrays"], 2),
-                  "sort_off_kernel_ms": {k: round(v, 3) for k, v in off["prof"].items() if k.endswith("_ms")},
-                  "rays_per_pass": int(on["rays"] / args.steps), "ray_counts": on["counts"]},
+                   "depth": args.depth, "knobs": tuned.best, "parallelism": "single GPU"},
+        "modes": {name: {"mrays": round(m["mrays"], 2), "opts": m["opts"],
+                         "kernel_ms": {k: round(v, 3) for k, v in m["prof"].items() if k.endswith("_ms")}}
+                  for name, m in res.items()},
+        "rays_per_pass": int(best["rays"] / args.steps), "ray_counts": best["counts"],
+        "autotune": {"seconds": round(tuned.seconds, 2), "passes_per_candidate": args.tune_passes,
+                     "columns": list(autotune.KNOBS) + ["mrays"],
+                     "table": [[t[k] for k in autotune.KNOBS] + [t["mrays"]] for t in tuned.table]},
         "kernel_ms": {k: round(v, 3) for k, v in prof.items() if k.endswith("_ms")},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -63,6 +63,8 @@ int rh_get_ray_counts(rh_context *ctx, rh_ray_counts *out);
  * "kernel_timing" (0/1: events around each kernel launch, see rh_get_kernel_profile),
  * "count_visits" (0/1: instrumented trace kernels count node/triangle/instance visits),
  * "lds_stack_depth" (4..64 LDS entries of the per-lane traversal stack),
+ * "sort_bits" (8/16/24/32: the in-pass ray sort orders by the top bits of the key),
+ * "path_tail" (-1: every bounce staged; b >= 0: bounces b.. run as one path-per-lane launch),
  * "defer_nee" (0/1: shadow rays of next-event estimation as their own stage). */
 int rh_set_option(rh_context *ctx, const char *name, int value);
 int rh_get_option(rh_context *ctx, const char *name, int *value);
@@ -70,7 +72,7 @@ int rh_get_option(rh_context *ctx, const char *name, int *value);
 /* Per-kernel profile collected while "kernel_timing" is on (HIP events around
  * every trace/shade/sort launch of rh_render_pass) and, while "count_visits"
  * is on, the traversal work of the trace kernels.  Kinds: 0 trace, 1 shade,
- * 2 sort, 3 shadow (rh_kernel_profile in rh_types.h).  No counterpart in the reference
+ * 2 sort, 3 shadow, 4 path tail (rh_kernel_profile in rh_types.h).  No counterpart in the reference
  * beyond stats_t. */
 int rh_get_kernel_profile(rh_context *ctx, rh_kernel_profile *out);
 int rh_reset_kernel_profile(rh_context *ctx);

@@ -288,11 +288,14 @@ typedef struct rh_ray_counts {
     unsigned long long sorted;
 } rh_ray_counts;
 
-/* per-kernel profile: kinds 0 trace, 1 shade, 2 sort, 3 shadow (deferred NEE visibility) */
+/* per-kernel profile: kinds 0 trace, 1 shade, 2 sort, 3 shadow (deferred NEE visibility),
+ * 4 path tail (the last bounces traced and shaded in one launch) */
+#define RH_KERNEL_KINDS 5
 typedef struct rh_kernel_profile {
-    double ms[4];
-    unsigned long long launches[4];
+    double ms[RH_KERNEL_KINDS];
+    unsigned long long launches[RH_KERNEL_KINDS];
     unsigned long long node_visits, tri_tests, instance_tests;
+    unsigned long long counted_rays; /* rays traced by the counting trace kernels */
 } rh_kernel_profile;
 
 #ifdef __cplusplus

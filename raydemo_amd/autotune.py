@@ -1,0 +1,105 @@
+"""Autotuner for the HIP backend's launch knobs.
+
+Every knob here changes only how the work is laid out on the device, never
+what is computed: the traversal stack's LDS share (the rest spills to a
+global slice), whether bounce rays are sorted by their origin/direction hash
+before tracing, how many top bits of the 32-bit key the sort orders by, and
+the bounce from which the remaining bounces run as one path-per-lane launch
+instead of one trace/shade/shadow stage each.
+The image is bit-identical for every setting (tests/test_gpu_sort.py), so the
+tuner may pick whichever is fastest on the scene at hand.
+
+The sweep renders a few progressive passes per candidate through the normal
+renderer API (device-side timing with hipEvents around each pass loop) and
+restores the renderer's accumulation afterwards by clearing it.  Results are
+cached per (device, scene signature, frame size) in a JSON file.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+import time
+from dataclasses import dataclass, field
+from pathlib import Path
+
+LDS_DEPTHS = (8, 16, 24, 32)
+# (sort_rays, sort_bits): off, then the 8-bit radix passes over the top 8..32 key bits
+SORT_MODES = ((0, 32), (1, 8), (1, 32))
+# first bounce of the path-tail launch (-1: every bounce staged)
+PATH_TAILS = (-1, 0, 1, 2, 3, 4)
+KNOBS = ("lds_stack_depth", "sort_rays", "sort_bits", "path_tail")
+
+
+@dataclass
+class TuneResult:
+    best: dict
+    table: list = field(default_factory=list)  # [{<knobs>, ms_per_pass, mrays}]
+    seconds: float = 0.0
+
+    def to_json(self) -> dict:
+        return {"best": self.best, "table": self.table, "seconds": round(self.seconds, 3)}
+
+
+def candidates(depths=LDS_DEPTHS, sort_modes=SORT_MODES, tails=PATH_TAILS):
+    for d, (so, sb), pt in itertools.product(depths, sort_modes, tails):
+        yield {"lds_stack_depth": d, "sort_rays": so, "sort_bits": sb, "path_tail": pt}
+
+
+def apply(renderer, cfg: dict):
+    for k, v in cfg.items():
+        renderer.set_option(k, int(v))
+
+
+def measure(renderer, scene, region_factory, cfg: dict, passes: int, warmup: int = 1) -> dict:
+    """Mrays/s of `passes` progressive passes under cfg (after `warmup` passes)."""
+    apply(renderer, cfg)
+    renderer.clear()
+    reg = region_factory()
+    for _ in range(warmup):
+        renderer.render(scene, reg)
+    renderer.synchronize()
+    renderer.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        renderer.render(scene, reg)
+    renderer.synchronize()
+    dt = time.perf_counter() - t0
+    rc = renderer.ray_counts()
+    rays = rc["primary"] + rc["secondary"] + rc["shadow"]
+    return dict(cfg, ms_per_pass=round(dt / passes * 1e3, 4), mrays=round(rays / dt / 1e6, 2))
+
+
+def tune(renderer, scene, region_factory, passes: int = 6, depths=LDS_DEPTHS, sort_modes=SORT_MODES,
+         tails=PATH_TAILS, cache: str | os.PathLike | None = None, key: str | None = None) -> TuneResult:
+    """Sweep the candidates and leave the renderer set to the fastest one."""
+    if cache and key:
+        p = Path(cache)
+        if p.exists():
+            try:
+                hit = json.loads(p.read_text()).get(key)
+            except (OSError, ValueError):
+                hit = None
+            if hit:
+                apply(renderer, hit["best"])
+                renderer.clear()
+                return TuneResult(best=hit["best"], table=hit.get("table", []), seconds=0.0)
+    t0 = time.perf_counter()
+    table = [measure(renderer, scene, region_factory, cfg, passes) for cfg in candidates(depths, sort_modes, tails)]
+    best_row = max(table, key=lambda r: r["mrays"])
+    best = {k: best_row[k] for k in KNOBS}
+    apply(renderer, best)
+    renderer.clear()
+    res = TuneResult(best=best, table=table, seconds=time.perf_counter() - t0)
+    if cache and key:
+        p = Path(cache)
+        data = {}
+        if p.exists():
+            try:
+                data = json.loads(p.read_text())
+            except (OSError, ValueError):
+                data = {}
+        data[key] = res.to_json()
+        p.parent.mkdir(parents=True, exist_ok=True)
+        p.write_text(json.dumps(data, indent=1))
+    return res

@@ -78,8 +78,8 @@ class RayCounts(Structure):
 
 
 class KernelProfile(Structure):
-    _fields_ = [("ms", C.c_double * 4), ("launches", c_uint64 * 4), ("node_visits", c_uint64),
-                ("tri_tests", c_uint64), ("instance_tests", c_uint64)]
+    _fields_ = [("ms", C.c_double * 5), ("launches", c_uint64 * 5), ("node_visits", c_uint64),
+                ("tri_tests", c_uint64), ("instance_tests", c_uint64), ("counted_rays", c_uint64)]
 
 
 class PassSettings(Structure):
@@ -284,10 +284,11 @@ class Renderer:
     def kernel_profile(self) -> dict:
         p = KernelProfile()
         self.lib.check(self.lib.lib.rhx_get_kernel_profile(self.h, C.byref(p)), "rhx_get_kernel_profile")
-        kinds = ("trace", "shade", "sort", "shadow")
+        kinds = ("trace", "shade", "sort", "shadow", "tail")
         d = {f"{k}_ms": p.ms[i] for i, k in enumerate(kinds)}
         d.update({f"{k}_launches": p.launches[i] for i, k in enumerate(kinds)})
-        d.update(node_visits=p.node_visits, tri_tests=p.tri_tests, instance_tests=p.instance_tests)
+        d.update(node_visits=p.node_visits, tri_tests=p.tri_tests, instance_tests=p.instance_tests,
+                 counted_rays=p.counted_rays)
         return d
 
     def reset_kernel_profile(self):

@@ -404,10 +404,10 @@ __device__ __forceinline__ void traverse_fast(const Scene &sc, f3 o, f3 d, Hit &
 // ---------------------------------------------------------------- textures
 // Texel fetch at reference atlas coordinates (x, y) on the atlas page where the
 // texture's mip block sits: the block's own compact storage holds it.
-__device__ __forceinline__ uint32_t atlas_get(const Scene &sc, uint32_t tex, int lod, const rh_texture &t, int x,
-                                              int y) {
+// (px, py) is the block's atlas position, the texture's pos[lod].
+__device__ __forceinline__ uint32_t atlas_get(const Scene &sc, uint32_t tex, int lod, int px, int py, int x, int y) {
     const rh_tex_mip m = sc.tex_mips[tex * RH_NUM_MIP_LEVELS + lod];
-    int lx = x - (int)t.pos[lod][0], ly = y - (int)t.pos[lod][1];
+    int lx = x - px, ly = y - py;
     lx = clampi(lx, 0, (int)m.bw - 1);
     ly = clampi(ly, 0, (int)m.bh - 1);
     return sc.texels[m.offset + (uint32_t)ly * m.bw + (uint32_t)lx];
@@ -417,11 +417,14 @@ __device__ __forceinline__ float texel_ch(uint32_t p, int c) { return (float)((p
 
 // SampleBilinear (CoreRef.cpp:1652-1678) with TransformUV (CoreRef.cpp:1625-1632)
 __device__ __forceinline__ f4 sample_bilinear(const Scene &sc, uint32_t tex, f2 uvs, int lod) {
-    const rh_texture t = sc.textures[tex];
+    // fields read in place: copying the record and indexing its arrays by lod
+    // would put it in scratch memory
+    const rh_texture *t = sc.textures + tex;
+    const int ipx = (int)t->pos[lod][0], ipy = (int)t->pos[lod][1];
     const float atlas = (float)RH_TEXTURE_ATLAS_SIZE;
-    const float posx = (float)t.pos[lod][0], posy = (float)t.pos[lod][1];
-    const float sizex = (float)(((uint32_t)t.width & RH_TEXTURE_WIDTH_BITS) >> lod);
-    const float sizey = (float)((uint32_t)t.height >> lod);
+    const float posx = (float)ipx, posy = (float)ipy;
+    const float sizex = (float)(((uint32_t)t->width & RH_TEXTURE_WIDTH_BITS) >> lod);
+    const float sizey = (float)((uint32_t)t->height >> lod);
     const float ux = uvs.x - simd_floor(uvs.x), uy = uvs.y - simd_floor(uvs.y);
     float rx = posx + ux * sizex + 1.0f, ry = posy + uy * sizey + 1.0f;
     rx /= atlas;
@@ -429,10 +432,10 @@ __device__ __forceinline__ f4 sample_bilinear(const Scene &sc, uint32_t tex, f2 
     rx = rx * atlas - 0.5f;
     ry = ry * atlas - 0.5f;
     const int ix = (int)rx, iy = (int)ry;
-    const uint32_t p00 = atlas_get(sc, tex, lod, t, ix + 0, iy + 0);
-    const uint32_t p01 = atlas_get(sc, tex, lod, t, ix + 1, iy + 0);
-    const uint32_t p10 = atlas_get(sc, tex, lod, t, ix + 0, iy + 1);
-    const uint32_t p11 = atlas_get(sc, tex, lod, t, ix + 1, iy + 1);
+    const uint32_t p00 = atlas_get(sc, tex, lod, ipx, ipy, ix + 0, iy + 0);
+    const uint32_t p01 = atlas_get(sc, tex, lod, ipx, ipy, ix + 1, iy + 0);
+    const uint32_t p10 = atlas_get(sc, tex, lod, ipx, ipy, ix + 0, iy + 1);
+    const uint32_t p11 = atlas_get(sc, tex, lod, ipx, ipy, ix + 1, iy + 1);
     const float kx = rx - __builtin_floorf(rx), ky = ry - __builtin_floorf(ry);
     float r[4];
     const float k = 1.0f / 255.0f;
@@ -497,10 +500,11 @@ __device__ __forceinline__ f4 sample_latlong_rgbe(const Scene &sc, uint32_t tex,
     const float posx = (float)t.pos[0][0], posy = (float)t.pos[0][1];
     const float sizex = (float)((uint32_t)t.width & RH_TEXTURE_WIDTH_BITS), sizey = (float)t.height;
     const float ux = posx + u * sizex + 1.0f, uy = posy + theta * sizey + 1.0f;
-    const uint32_t p00 = atlas_get(sc, tex, 0, t, (int)(ux + 0), (int)(uy + 0));
-    const uint32_t p01 = atlas_get(sc, tex, 0, t, (int)(ux + 1), (int)(uy + 0));
-    const uint32_t p10 = atlas_get(sc, tex, 0, t, (int)(ux + 0), (int)(uy + 1));
-    const uint32_t p11 = atlas_get(sc, tex, 0, t, (int)(ux + 1), (int)(uy + 1));
+    const int ipx = (int)t.pos[0][0], ipy = (int)t.pos[0][1];
+    const uint32_t p00 = atlas_get(sc, tex, 0, ipx, ipy, (int)(ux + 0), (int)(uy + 0));
+    const uint32_t p01 = atlas_get(sc, tex, 0, ipx, ipy, (int)(ux + 1), (int)(uy + 0));
+    const uint32_t p10 = atlas_get(sc, tex, 0, ipx, ipy, (int)(ux + 0), (int)(uy + 1));
+    const uint32_t p11 = atlas_get(sc, tex, 0, ipx, ipy, (int)(ux + 1), (int)(uy + 1));
     const float kx = ux - simd_floor(ux), ky = uy - simd_floor(uy);
     auto rgbe = [](uint32_t p, float *o) {
         const float f = exp2f((float)((p >> 24) & 0xffu) - 128.0f);

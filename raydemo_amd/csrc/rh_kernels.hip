@@ -134,11 +134,50 @@ __device__ __forceinline__ int wave_append(bool pred, int *counter) {
     return pred ? base + __popcll(mask & lt) : -1;
 }
 
+// Block-aggregated append of per-lane predicates to `slots` queues laid out
+// back to back in one array: entries of queue j come before those of queue
+// j+1 within the block's range, lanes in order.  One atomic per block.  Every
+// thread of the block must call it; s needs slots * waves + 1 ints of LDS.
+// Returns the first index of the block's range; lane (j) goes to
+// base + s[j * waves + wave] + (lanes below it in the wave with pred j).
+__device__ __forceinline__ int block_append_base(int m, uint32_t slots, int *counter, int *s) {
+    const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (uint32_t j = 0; j < slots; j++) {
+        const uint64_t mask = __ballot((int)j < m);
+        if (lane_id() == 0) s[j * nw + wave] = __popcll(mask);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int k = 0; k < (int)slots * nw; k++) {
+            const int c = s[k];
+            s[k] = tot;
+            tot += c;
+        }
+        s[slots * nw] = tot ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
+    return s[slots * nw];
+}
+
+__device__ __forceinline__ int lanes_below(uint64_t mask) {
+    const uint32_t lane = lane_id();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    return __popcll(mask & lt);
+}
+
 __device__ __forceinline__ void wave_add_u64(unsigned long long *ctr, uint32_t v) {
     // reduce across the wave, one atomic
     uint32_t s = v;
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     if (lane_id() == 0 && s) atomicAdd(ctr, (unsigned long long)s);
+}
+
+__device__ __forceinline__ void wave_add_i64(unsigned long long *ctr, int v) {
+    // signed variant: the counter wraps modulo 2^64, so adding -k subtracts k
+    int s = v;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane_id() == 0 && s) atomicAdd(ctr, (unsigned long long)(long long)s);
 }
 
 struct PassConst {
@@ -223,7 +262,7 @@ struct TraceArgs {
     uint32_t lds_depth;
     unsigned long long *ray_counter; // optional: rays traced
     unsigned long long *overflow;    // stack overflow events
-    unsigned long long *visits;      // Count mode: node visits, leaf triangle tests, instance tests
+    unsigned long long *visits;      // Count mode: node visits, leaf triangle tests, instance tests, rays
 };
 
 template <bool AnyHit, bool Count> __global__ void __launch_bounds__(kBlock) k_trace(TraceArgs a) {
@@ -231,7 +270,7 @@ template <bool AnyHit, bool Count> __global__ void __launch_bounds__(kBlock) k_t
     const int n = a.count ? *a.count : a.n;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = (int)i < n;
-    if (a.ray_counter) wave_add_u64(a.ray_counter, active ? 1u : 0u);
+    if (i == 0 && a.ray_counter) atomicAdd(a.ray_counter, (unsigned long long)n);
     if (!active) return;
 
     const float4 *p = reinterpret_cast<const float4 *>(a.rays + i);
@@ -261,6 +300,7 @@ template <bool AnyHit, bool Count> __global__ void __launch_bounds__(kBlock) k_t
         atomicAdd(a.visits + 0, (unsigned long long)tc.nodes);
         atomicAdd(a.visits + 1, (unsigned long long)tc.leaf_tris);
         atomicAdd(a.visits + 2, (unsigned long long)tc.instances);
+        atomicAdd(a.visits + 3, 1ull);
     }
 
     float4 *h = reinterpret_cast<float4 *>(a.hits + i);
@@ -291,6 +331,7 @@ struct ShadeArgs {
     uint32_t lds_depth;
     unsigned long long *shadow_counter;
     unsigned long long *overflow;
+    unsigned long long *ray_counter; // path tail: secondary rays traced
     // deferred next-event estimation (null nee_ray: visibility traced inline)
     float4 *nee_ray;
     int2 *nee_aux;
@@ -299,6 +340,8 @@ struct ShadeArgs {
     float4 *pend_c;     // ray throughput xyz, pixel index bits
     float4 *pend_alb;   // albedo xyz, flags bits (1: multiply by albedo, 2: primary)
     int *pend_count;    // deferred terms of the ray (0: nothing pending)
+    uint32_t *shadow_queue; // compacted slot indices of the recorded light samples
+    int *shadow_count;      // entries appended to shadow_queue
 };
 
 // Defer: direct light goes to the NEE stage (no traversal in this kernel).
@@ -324,6 +367,7 @@ template <bool Defer> __global__ void __launch_bounds__(kBlock) k_shade(ShadeArg
     st.sp = 0;
     st.overflow = 0;
     int px = 0, py = 0;
+    int nee_m = 0; // light samples recorded for this ray (deferred NEE)
 
     if (active) {
         const RayFull ray = load_ray(a.rays, a.aux, i);
@@ -359,7 +403,8 @@ template <bool Defer> __global__ void __launch_bounds__(kBlock) k_shade(ShadeArg
             // the colour written below carries a zero direct term; the combine
             // stage adds ray.c * (sum of visible terms [* albedo]) on top
             const bool pending = so.nee_pending && sink.count > 0;
-            a.pend_count[i] = pending ? (int)sink.count : 0;
+            nee_m = pending ? (int)sink.count : 0;
+            a.pend_count[i] = nee_m;
             if (pending) {
                 a.pend_c[i] = make_float4(ray.c.x, ray.c.y, ray.c.z, __int_as_float(py * a.w + px));
                 a.pend_alb[i] = make_float4(so.albedo.x, so.albedo.y, so.albedo.z,
@@ -382,21 +427,101 @@ template <bool Defer> __global__ void __launch_bounds__(kBlock) k_shade(ShadeArg
         }
         if (a.out_rgba) a.out_rgba[i] = make_float4(so.col.x, so.col.y, so.col.z, so.col.w);
     }
-    const int slot = wave_append(active && so.emit, a.out_count);
-    if (slot >= 0) {
-        store_ray(a.out_rays, a.out_aux, (uint32_t)slot, so.next);
-        if (a.out_src) a.out_src[slot] = (int)i;
+    __shared__ int s_app[kMaxDeferredLights * (kBlock / 64) + 1];
+    const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (Defer) {
+        // One shadow work item per recorded light sample.  Sample j of every
+        // lane of the block goes into one run, so lanes that aim at the same
+        // light sit next to each other in the queue.
+        const int base = block_append_base(nee_m, a.nee_slots, a.shadow_count, s_app);
+        for (uint32_t j = 0; j < a.nee_slots; j++) {
+            const uint64_t mask = __ballot((int)j < nee_m);
+            if ((int)j < nee_m) a.shadow_queue[base + s_app[j * nw + wave] + lanes_below(mask)] = i * a.nee_slots + j;
+        }
+        __syncthreads(); // s_app is reused below
     }
+    {
+        const bool emit = active && so.emit;
+        const int base = block_append_base(emit ? 1 : 0, 1, a.out_count, s_app);
+        const uint64_t mask = __ballot(emit);
+        if (emit) {
+            const int slot = base + s_app[wave] + lanes_below(mask);
+            store_ray(a.out_rays, a.out_aux, (uint32_t)slot, so.next);
+            if (a.out_src) a.out_src[slot] = (int)i;
+        }
+    }
+    if (a.shadow_counter) wave_add_u64(a.shadow_counter, cnt.shadow_rays);
+    if (st.overflow && a.overflow) atomicAdd(a.overflow, (unsigned long long)st.overflow);
+}
+
+// Path tail: the remaining bounces of each ray in one launch, one lane per
+// path (trace, then ShadeSurface with visibility traced inline, then the
+// next ray), from host bounce `first` up to `last` (exclusive).  Late bounces
+// hold few rays whose cost is the latency of their longest traversals; here
+// those latencies overlap across bounces instead of adding up launch after
+// launch.  Every pixel still receives its path's contributions in bounce
+// order, and each contribution is computed exactly as in the staged form, so
+// the image is identical.
+__global__ void __launch_bounds__(kBlock) k_path_tail(ShadeArgs a, int first, int last) {
+    extern __shared__ uint32_t lds_stack[];
+    const int n = *a.count;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int)(blockIdx.x * blockDim.x) >= n) return; // whole block idle
+    const bool active = (int)i < n;
+    ShadeCounters cnt;
+    cnt.shadow_rays = 0;
+    uint32_t traced = 0;
+    Stack st;
+    st.lds = lds_stack;
+    st.spill = a.spill + i;
+    st.spill_stride = a.spill_stride;
+    st.lds_depth = a.lds_depth;
+    st.spill_depth = kSpillDepth;
+    st.lane = threadIdx.x;
+    st.block = blockDim.x;
+    st.sp = 0;
+    st.overflow = 0;
+    if (active) {
+        RayFull ray = load_ray(a.rays, a.aux, i);
+        const int px = (ray.xy >> 16) & 0xffff, py = ray.xy & 0xffff;
+        rh_pass_info pi = a.pi;
+        pi.index = py * a.w + px;
+        pi.rand_index = pi.index;
+        if (pi.settings.flags & RH_USE_COHERENT_SAMPLING) pi.rand_index = c_sampling_pattern[(py % 8) * 8 + (px % 8)];
+        float4 &dst = a.temp[(size_t)py * a.w + px];
+        for (int b = first; b < last; b++) {
+            Hit hit;
+            hit.mask = 0;
+            hit.obj = -1;
+            hit.prim = -1;
+            hit.t = RH_MAX_DIST;
+            hit.u = hit.v = 0.0f;
+            st.sp = 0;
+            if (a.sc.macro_root != RH_INVALID) traverse_fast<false>(a.sc, ray.o, ray.d, hit, st, nullptr);
+            traced++;
+            pi.bounce = b + 3;
+            ShadeOut so;
+            so.emit = false;
+            shade_surface(a.sc, pi, hit, ray, a.halton, st, cnt, so, nullptr);
+            float4 v = dst;
+            v.x += so.col.x;
+            v.y += so.col.y;
+            v.z += so.col.z;
+            v.w += 0.0f;
+            dst = v;
+            if (!so.emit) break;
+            ray = so.next;
+        }
+    }
+    if (a.ray_counter) wave_add_u64(a.ray_counter, traced);
     if (a.shadow_counter) wave_add_u64(a.shadow_counter, cnt.shadow_rays);
     if (st.overflow && a.overflow) atomicAdd(a.overflow, (unsigned long long)st.overflow);
 }
 
 struct ShadowArgs {
     Scene sc;
-    const int *count; // shaded rays of this stage (device); nullptr -> n
-    int n;
-    uint32_t slots;
-    const int *pend_count;
+    const uint32_t *queue; // slot indices of the recorded light samples
+    const int *count;      // entries in queue (device)
     const float4 *nee_ray;
     const int2 *nee_aux;
     uint8_t *nee_vis;
@@ -410,22 +535,24 @@ struct ShadowArgs {
 
 // Visibility of each deferred light sample: ComputeVisibility
 // (CoreRef.cpp:1801-1896), any-hit traversal stepping through transparent
-// surfaces.  One lane per slot; empty slots exit at once.
+// surfaces.  One lane per queue entry; the grid covers every slot and lanes
+// past the device count exit at once.
 __global__ void __launch_bounds__(kBlock) k_shadow(ShadowArgs a) {
     extern __shared__ uint32_t lds_stack[];
-    const int n = a.count ? *a.count : a.n;
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t i = k / a.slots, j = k % a.slots;
+    const int n = *a.count;
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int)(blockIdx.x * blockDim.x) >= n) return; // whole block idle
+    if (q == 0 && a.shadow_counter) atomicAdd(a.shadow_counter, (unsigned long long)n);
     ShadeCounters cnt;
     cnt.shadow_rays = 0;
-    const bool active = (int)i < n && (int)j < a.pend_count[(int)i < n ? i : 0];
-    if (active) {
+    if ((int)q < n) {
+        const uint32_t k = a.queue[q];
         const float4 r = a.nee_ray[k];
         const int2 x = a.nee_aux[k];
         const rh_light &l = a.sc.lights[__float_as_uint(r.w)];
         Stack st;
         st.lds = lds_stack;
-        st.spill = a.spill + k;
+        st.spill = a.spill + q;
         st.spill_stride = a.spill_stride;
         st.lds_depth = a.lds_depth;
         st.spill_depth = kSpillDepth;
@@ -437,7 +564,11 @@ __global__ void __launch_bounds__(kBlock) k_shadow(ShadowArgs a) {
         a.nee_vis[k] = vis != 0.0f ? 1 : 0;
         if (st.overflow && a.overflow) atomicAdd(a.overflow, (unsigned long long)st.overflow);
     }
-    if (a.shadow_counter) wave_add_u64(a.shadow_counter, cnt.shadow_rays);
+    // one segment per entry is counted above; add the difference (segments
+    // past the first through transparent surfaces, or none at all when the
+    // light is within HIT_EPS)
+    const int extra = (int)q < n ? (int)cnt.shadow_rays - 1 : 0;
+    if (a.shadow_counter && __ballot(extra != 0)) wave_add_i64(a.shadow_counter, extra);
 }
 
 // Adds each shaded ray's deferred direct light: ray.c * (sum of visible terms
@@ -710,8 +841,9 @@ struct rh_context {
     uint8_t *nee_vis = nullptr;
     float4 *pend_c = nullptr, *pend_alb = nullptr;
     int *pend_count = nullptr;
+    uint32_t *shadow_queue = nullptr; // compacted shadow work: slot indices
     int defer_nee = 1;
-    int *counters = nullptr; // [0], [1]: queue counts
+    int *counters = nullptr; // [0], [1]: ray queue counts, [2]: shadow queue count
     unsigned long long *stats_dev = nullptr; // [0] primary [1] secondary [2] shadow [3] sorted [4] overflow
     float *halton = nullptr;
 
@@ -719,12 +851,14 @@ struct rh_context {
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<std::pair<int, std::pair<size_t, size_t>>> ev_marks; // kind, (start, stop)
-    unsigned long long *visits = nullptr;                             // [nodes, tris, instances]
+    unsigned long long *visits = nullptr;                             // [nodes, tris, instances, rays]
 
     // options
     int kernel_timing = 0;
     int count_visits = 0;
     int sort_rays = 1;
+    int sort_bits = 32; // key bits the in-pass ray sort orders by (top bits)
+    int path_tail = 2;  // first bounce run by the path-tail kernel (-1: none)
     int stage_timing = 0;
     int lds_depth = 24;
     rh_stats stats = {};
@@ -831,6 +965,7 @@ int ensure_nee(rh_context *c) {
         free_ptr(c->pend_c);
         free_ptr(c->pend_alb);
         free_ptr(c->pend_count);
+        free_ptr(c->shadow_queue);
         c->nee_cap = cap;
         RH_CHECK(hipMalloc(&c->nee_ray, cap * sizeof(float4)));
         RH_CHECK(hipMalloc(&c->nee_term, cap * sizeof(float4)));
@@ -839,6 +974,7 @@ int ensure_nee(rh_context *c) {
         RH_CHECK(hipMalloc(&c->pend_c, m * sizeof(float4)));
         RH_CHECK(hipMalloc(&c->pend_alb, m * sizeof(float4)));
         RH_CHECK(hipMalloc(&c->pend_count, m * sizeof(int)));
+        RH_CHECK(hipMalloc(&c->shadow_queue, cap * sizeof(uint32_t)));
     }
     return ensure_spill(c, std::max(m, cap));
 }
@@ -937,6 +1073,8 @@ ShadeArgs shade_args(rh_context *c) {
         a.pend_c = c->pend_c;
         a.pend_alb = c->pend_alb;
         a.pend_count = c->pend_count;
+        a.shadow_queue = c->shadow_queue;
+        a.shadow_count = c->counters + 2;
     }
     return a;
 }
@@ -948,6 +1086,7 @@ int launch_shade(rh_context *c, const ShadeArgs &a, int grid_rays) {
     {
         KernelMark km(c, 1);
         if (a.nee_ray) {
+            hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, a.shadow_count, 0);
             hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(kBlock), 0, c->stream, a);
         } else {
             hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(kBlock), lds_bytes(c), c->stream, a);
@@ -957,10 +1096,8 @@ int launch_shade(rh_context *c, const ShadeArgs &a, int grid_rays) {
     if (!a.nee_ray) return 0;
     ShadowArgs s;
     s.sc = c->sc;
-    s.count = a.count;
-    s.n = a.n;
-    s.slots = a.nee_slots;
-    s.pend_count = a.pend_count;
+    s.queue = a.shadow_queue;
+    s.count = a.shadow_count;
     s.nee_ray = a.nee_ray;
     s.nee_aux = a.nee_aux;
     s.nee_vis = c->nee_vis;
@@ -983,13 +1120,14 @@ int launch_shade(rh_context *c, const ShadeArgs &a, int grid_rays) {
     return 0;
 }
 
-// Stable LSB radix sort of (key, val) pairs in keys[0]/vals[0]; the result
-// ends in keys[0]/vals[0] after the four 8-bit passes.
-int radix_sort(rh_context *c, const int *count, int max_n) {
+// Stable LSB radix sort of (key, val) pairs in keys[0]/vals[0] by the top
+// `bits` bits of the key (8, 16, 24 or 32: one 8-bit pass each).  Returns the
+// buffer index (0 or 1) that holds the sorted pairs.
+int radix_sort(rh_context *c, const int *count, int max_n, int bits, int *result) {
     const int tiles = std::max(1, (max_n + kSortTile - 1) / kSortTile);
     KernelMark km(c, 2);
     int src = 0;
-    for (int shift = 0; shift < 32; shift += 8) {
+    for (int shift = 32 - bits; shift < 32; shift += 8) {
         hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kBlock), 0, c->stream, c->keys[src], count, max_n, shift,
                            c->hist, tiles);
         hipLaunchKernelGGL(k_radix_scan_rows, dim3(256), dim3(kBlock), 0, c->stream, c->hist, tiles,
@@ -1001,6 +1139,7 @@ int radix_sort(rh_context *c, const int *count, int max_n) {
         src ^= 1;
     }
     RH_CHECK(hipGetLastError());
+    *result = src;
     return 0;
 }
 
@@ -1105,6 +1244,7 @@ void rh_destroy(rh_context *c) {
     free_ptr(c->pend_c);
     free_ptr(c->pend_alb);
     free_ptr(c->pend_count);
+    free_ptr(c->shadow_queue);
     free_ptr(c->counters);
     free_ptr(c->stats_dev);
     free_ptr(c->halton);
@@ -1249,15 +1389,33 @@ int rh_render_pass(rh_context *c, const rh_pass_desc *p) {
     const bool skip_indirect = (pi.settings.flags & RH_SKIP_INDIRECT_LIGHT) != 0;
     const int sort_on = c->sort_rays && p->sort_rays;
     for (int bounce = 0; bounce < pi.settings.max_total_depth && !skip_indirect; bounce++) {
+        if (c->path_tail >= 0 && bounce >= c->path_tail) {
+            // the remaining bounces, one lane per path
+            ShadeArgs tb = sa;
+            tb.rays = c->rays[q];
+            tb.aux = c->aux[q];
+            tb.count = cnt[q];
+            tb.primary = 0;
+            tb.nee_ray = nullptr;
+            tb.nee_aux = nullptr;
+            tb.nee_term = nullptr;
+            tb.ray_counter = c->stats_dev + 1;
+            KernelMark km(c, 4);
+            hipLaunchKernelGGL(k_path_tail, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), lds_bytes(c), c->stream, tb,
+                               bounce, (int)pi.settings.max_total_depth);
+            RH_CHECK(hipGetLastError());
+            break;
+        }
         StageTimer bt(c);
         bt.mark();
         if (sort_on) {
             hipLaunchKernelGGL(k_ray_keys, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->rays[q],
                                cnt[q], n, pc, c->keys[0], c->vals[0]);
-            rc = radix_sort(c, cnt[q], n);
+            int sorted = 0;
+            rc = radix_sort(c, cnt[q], n, c->sort_bits, &sorted);
             if (rc) return rc;
             hipLaunchKernelGGL(k_gather, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->rays[q],
-                               c->aux[q], c->vals[0], cnt[q], n, c->rays[q ^ 1], c->aux[q ^ 1]);
+                               c->aux[q], c->vals[sorted], cnt[q], n, c->rays[q ^ 1], c->aux[q ^ 1]);
             RH_CHECK(hipGetLastError());
             // the sorted rays now sit in the other buffer; move the count along
             hipLaunchKernelGGL(k_copy_int, dim3(1), dim3(1), 0, c->stream, cnt[q ^ 1], cnt[q]);
@@ -1352,6 +1510,12 @@ int rh_set_option(rh_context *c, const char *name, int value) {
     else if (n == "lds_stack_depth") {
         if (value < 4 || value > 64) return arg_error("lds_stack_depth must be 4..64");
         c->lds_depth = value;
+    } else if (n == "path_tail") {
+        if (value < -1 || value > 255) return arg_error("path_tail must be -1..255");
+        c->path_tail = value;
+    } else if (n == "sort_bits") {
+        if (value != 8 && value != 16 && value != 24 && value != 32) return arg_error("sort_bits must be 8, 16, 24 or 32");
+        c->sort_bits = value;
     } else return arg_error("unknown option");
     return 0;
 }
@@ -1361,6 +1525,9 @@ int rh_get_option(rh_context *c, const char *name, int *value) {
     if (n == "sort_rays") *value = c->sort_rays;
     else if (n == "stage_timing") *value = c->stage_timing;
     else if (n == "lds_stack_depth") *value = c->lds_depth;
+    else if (n == "sort_bits") *value = c->sort_bits;
+    else if (n == "path_tail") *value = c->path_tail;
+    else if (n == "defer_nee") *value = c->defer_nee;
     else if (n == "stack_overflows") {
         unsigned long long v[8];
         RH_CHECK(hipMemcpyAsync(v, c->stats_dev, sizeof(v), hipMemcpyDeviceToHost, c->stream));
@@ -1388,6 +1555,7 @@ int rh_get_kernel_profile(rh_context *c, rh_kernel_profile *out) {
     unsigned long long v[4];
     RH_CHECK(hipMemcpy(v, c->visits, sizeof(v), hipMemcpyDeviceToHost));
     out->node_visits = v[0];
+    out->counted_rays = v[3];
     out->tri_tests = v[1];
     out->instance_tests = v[2];
     return 0;
@@ -1541,10 +1709,11 @@ int rh_sort_rays(rh_context *c, rh_ray_packet *rays, size_t n, const float root_
     memcpy(pc.cell_size, cell_size, sizeof(pc.cell_size));
     hipLaunchKernelGGL(k_ray_keys, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, c->rays[0],
                        nullptr, (int)n, pc, c->keys[0], c->vals[0]);
-    rc = radix_sort(c, nullptr, (int)n);
+    int sorted = 0;
+    rc = radix_sort(c, nullptr, (int)n, 32, &sorted); // the full key, as SortRays_CPU
     if (rc) return rc;
     hipLaunchKernelGGL(k_gather, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, c->rays[0],
-                       c->aux[0], c->vals[0], nullptr, (int)n, c->rays[1], c->aux[1]);
+                       c->aux[0], c->vals[sorted], nullptr, (int)n, c->rays[1], c->aux[1]);
     RH_CHECK(hipGetLastError());
     return download_packets(c, rays, n, 1);
 }

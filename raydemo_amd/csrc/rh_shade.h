@@ -346,7 +346,7 @@ __device__ __forceinline__ void shade_surface(const Scene &sc, const rh_pass_inf
 
     // albedo (CoreRef.cpp:2172-2181)
     const uint32_t main_tex = mat->textures[RH_MAIN_TEXTURE];
-    const rh_texture mt = sc.textures[main_tex];
+    const rh_texture &mt = sc.textures[main_tex];
     const float albedo_lod = get_texture_lod(mt, der.duv_dx, der.duv_dy);
     f4 albedo = sample_bilinear(sc, main_tex, uvs, (int)albedo_lod);
     if (mt.width & RH_TEXTURE_SRGB_BIT) albedo = srgb_to_rgb(albedo);

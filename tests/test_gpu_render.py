@@ -7,6 +7,8 @@ import pytest
 
 from raydemo_amd import capi as A, scenes
 
+from conftest import standalone_scene
+
 pytestmark = pytest.mark.gpu
 
 
@@ -68,3 +70,19 @@ def test_soup_matches_reference_small(ref_lib, host_lib):
     e = rmse(img_g, img_r)
     print(f"soup 8 spp: RMSE {e:.3e}; rays {rg.ray_counts()}")
     assert e <= 1e-3
+
+
+@pytest.mark.parametrize("opts", [{}, {"sort_rays": 0, "lds_stack_depth": 8, "path_tail": -1}, {"defer_nee": 0},
+                                  {"path_tail": 0}])
+def test_ray_counts_match_oracle(oracle, host_lib, opts):
+    """The device's primary/secondary/shadow counters (the numerator of Mrays/s)
+    equal the CPU oracle's counts over the same passes."""
+    w, h, spp = 48, 32, 3
+    build = lambda s: scenes.triangle_soup(s, n_spheres=24, n_fine=2)
+    img, r = render(host_lib, A.RENDERER_HIP, build, w, h, spp, **opts)
+    hs = standalone_scene(host_lib)
+    build(hs)
+    ref_img, counts = oracle.render(hs.raw_view(), hs.camera_record(0), w, h, spp)
+    rc = r.ray_counts()
+    assert (rc["primary"], rc["secondary"], rc["shadow"]) == tuple(int(c) for c in counts)
+    assert rmse(img, ref_img) <= 1e-3

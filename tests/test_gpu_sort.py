@@ -52,14 +52,25 @@ def test_sort_matches_reference(ref_lib, host_lib, dev_lib, n, seed, deg):
     assert (np.diff(s_gpu["xy"])[eq] > 0).all()
 
 
-def test_sort_is_invisible_in_the_image(host_lib):
+@pytest.mark.parametrize("lds_depth", [24, 8])
+def test_sort_is_invisible_in_the_image(host_lib, lds_depth):
+    """Sorting (by the full key or only its top bits), the LDS share of the
+    traversal stack, deferred or inline shadow rays and the path-tail launch
+    change only the order of the work: the image is identical."""
     imgs = []
-    for sort_on in (0, 1):
-        r = A.Renderer(host_lib, 160, 90, A.RENDERER_HIP, sort_rays=bool(sort_on))
+    for opts in ({"sort_rays": 0, "path_tail": -1}, {"sort_rays": 1, "path_tail": -1},
+                 {"sort_rays": 1, "sort_bits": 8}, {"sort_rays": 1, "sort_bits": 16},
+                 {"sort_rays": 0, "defer_nee": 0, "path_tail": -1}, {"path_tail": 0}, {"path_tail": 1, "sort_rays": 1},
+                 {"path_tail": 3, "defer_nee": 0}):
+        r = A.Renderer(host_lib, 160, 90, A.RENDERER_HIP)
+        r.set_option("lds_stack_depth", lds_depth)
+        for k, v in opts.items():
+            r.set_option(k, v)
         s = r.create_scene()
         scenes.triangle_soup(s, n_spheres=64, n_fine=4)
         reg = A.Region(host_lib, 0, 0, 160, 90)
         for _ in range(4):
             r.render(s, reg)
         imgs.append(r.pixels())
-    assert imgs[0].tobytes() == imgs[1].tobytes()
+    for im in imgs[1:]:
+        assert im.tobytes() == imgs[0].tobytes()
