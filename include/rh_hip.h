@@ -91,6 +91,24 @@ int rh_sort_rays(rh_context *ctx, rh_ray_packet *rays, size_t n, const float roo
 int rh_shade_rays(rh_context *ctx, const rh_pass_info *pi, const rh_ray_packet *rays, const rh_hit *hits, size_t n,
                   const float *halton, float *out_rgba, rh_ray_packet *out_secondary, int *out_count, int *out_src);
 
+/* ---- differentiation (no counterpart in the reference: the adjoint extension) ----
+ * rh_adjoint_pass replays one pass (same camera, rect, iteration and Halton
+ * table as the rh_render_pass it differentiates) and adds, for every material,
+ *   scale * sum_pixels adj[pixel] . d(pass colour of pixel) / d main_color
+ * to d_grad (4 floats per material: r, g, b, unused).  d_adj_rgba is the
+ * adjoint of the pass image (w*h RGBA floats, alpha unused).  Both are device
+ * pointers; the call is asynchronous on the context's stream (rh_synchronize).
+ * Paths are treated with their sampling decisions fixed: Russian roulette
+ * weights are not differentiated.
+ * rh_set_material_colors overwrites main_color of materials [first, first+n)
+ * of the uploaded scene from packed rgb triples (host or device pointer).
+ * rh_read_accum copies the linear accumulated image (the Ref renderer's
+ * "clean" framebuffer, FramebufferRef.cpp:80-86, before tonemapping; w*h RGBA
+ * floats) to dst (host or device pointer). */
+int rh_adjoint_pass(rh_context *ctx, const rh_pass_desc *pass, const float *d_adj_rgba, float *d_grad, float scale);
+int rh_set_material_colors(rh_context *ctx, uint32_t first, uint32_t n, const float *rgb);
+int rh_read_accum(rh_context *ctx, float *dst, size_t count);
+
 /* ---- device-resident entry point for benchmarking the trace kernel ----
  * rays/hits are device pointers in the split layout (rh_ray / rh_hit). */
 int rh_trace_device(rh_context *ctx, const rh_ray *d_rays, int n, int any_hit, rh_hit *d_hits, void *stream);

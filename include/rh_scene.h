@@ -138,6 +138,18 @@ int rhx_region_clear(rhx_region *reg);
 
 int rhx_render_scene(rhx_renderer *r, rhx_scene *s, rhx_region *reg);
 
+/* Differentiation (HIP backend only; no counterpart in the reference, see
+ * rh_adjoint_pass in rh_hip.h).  rhx_adjoint_scene advances reg exactly as
+ * rhx_render_scene would and replays that pass, adding scale * d loss /
+ * d main_color (4 floats per material) to d_grad for the adjoint image
+ * d_adj_rgba (device pointers).  rhx_set_material_colors sets main_color of
+ * materials [first, first+n) from host rgb triples, in the scene and in its
+ * device copy.  rhx_read_accum copies the linear accumulated image (w*h RGBA). */
+int rhx_adjoint_scene(rhx_renderer *r, rhx_scene *s, rhx_region *reg, const float *d_adj_rgba, float *d_grad,
+                      float scale);
+int rhx_set_material_colors(rhx_renderer *r, rhx_scene *s, uint32_t first, uint32_t n, const float *rgb);
+int rhx_read_accum(rhx_renderer *r, float *dst, size_t count);
+
 void rhx_set_environment(rhx_scene *s, const float env_col[3], float env_clamp, uint32_t env_map);
 uint32_t rhx_add_texture(rhx_scene *s, const uint8_t *rgba, int w, int h, int is_srgb, int generate_mipmaps);
 uint32_t rhx_add_material(rhx_scene *s, const rhx_mat_desc *m);

@@ -286,6 +286,8 @@ typedef struct rh_ray_counts {
     unsigned long long secondary;
     unsigned long long shadow;
     unsigned long long sorted;
+    unsigned long long adjoint;        /* path segments traced by the adjoint replay */
+    unsigned long long adjoint_shadow; /* visibility segments of the adjoint replay */
 } rh_ray_counts;
 
 /* per-kernel profile: kinds 0 trace, 1 shade, 2 sort, 3 shadow (deferred NEE visibility),

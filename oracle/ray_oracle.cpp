@@ -548,11 +548,15 @@ V3 direct_light(Ctx &c, V3 I, V3 P, V3 N, V3 B, V3 pN, float sigma, int hi, int 
 inline float pow5(float v) { return (v * v) * (v * v) * v; }
 
 // ShadeSurface (CoreRef.cpp:2055-2391)
+// alb_mat (optional): the material whose main_color scales this vertex's
+// colour and the rest of the path once (emissive; diffuse and glossy when the
+// albedo is considered), or -1
 void shade(Ctx &c, const rh_pass_info &pi, const rh_hit &inter, const rh_ray_packet &ray, float out[4],
-           rh_ray_packet *sec, bool &emitted) {
+           rh_ray_packet *sec, bool &emitted, int *alb_mat = nullptr) {
     const rh_scene_view &sc = c.sc;
     const float *halton = c.halton;
     emitted = false;
+    if (alb_mat) *alb_mat = -1;
     const uint32_t fl = pi.settings.flags;
     if (!inter.mask) {
         float env[4] = {0, 0, 0, 0};
@@ -647,6 +651,9 @@ void shade(Ctx &c, const rh_pass_info &pi, const rh_hit &inter, const rh_ray_pac
               td = (ray.ray_depth >> 24) & 0xff, tot = dd + gd + rd + td;
     const bool cant_terminate = tot < pi.settings.termination_start_depth;
     const bool consider_albedo = !(fl & RH_LIGHTING_ONLY) || pi.bounce > 2;
+    if (alb_mat && (mat->type == RH_MAT_EMISSIVE ||
+                    (consider_albedo && (mat->type == RH_MAT_DIFFUSE || mat->type == RH_MAT_GLOSSY))))
+        *alb_mat = (int)(mat - sc.materials);
     rh_ray_packet r;
     memset(&r, 0, sizeof(r));
     bool emit = false;
@@ -997,6 +1004,84 @@ int orc_render(const rh_scene_view *sc, const rh_camera *cam, int w, int h, cons
                 if (cam->pass_settings.flags & RH_CLAMP)
                     for (float &x_ : v) x_ = clampf(x_, 0.0f, 1.0f);
                 memcpy(&final_rgba[p], v, 16);
+            }
+        }
+    }
+    return 0;
+}
+
+// FP64 reference gradient of a loss through the rendered mean image, by path
+// replay: passes 1..passes are replayed as orc_render renders them; every
+// shaded vertex j of a pixel's path records its colour C_j (the value added to
+// the pass image) and the material whose main_color scales C_j and all later
+// contributions once.  d(pass value)/d main_color[m] = S_m / main_color[m] with
+// S_m the suffix sum of the path's colours from the vertex on, accumulated here
+// in double: grad[3 m + c] += scale * adj[pixel][c] * S_m[c] / main_color[m][c].
+int orc_adjoint(const rh_scene_view *sc, const rh_camera *cam, int w, int h, const int32_t rect[4], int passes,
+                const float *adj_rgba, double scale, double *grad) {
+    static const std::vector<uint16_t> pm = perms();
+    std::vector<float> halton(RH_HALTON_COUNT * RH_HALTON_SEQ_LEN);
+    int have_halton = -1;
+    const size_t n = (size_t)rect[2] * rect[3];
+    std::vector<rh_ray_packet> rays(n), sec(n);
+    std::vector<rh_hit> hits(n);
+    struct Vtx {
+        int mat;
+        float c[3];
+    };
+    std::vector<std::vector<Vtx>> path((size_t)w * h);
+    for (int it = 1; it <= passes; it++) {
+        if (have_halton < 0 || it % RH_HALTON_SEQ_LEN == 0) {
+            halton_table(pm, it, halton.data());
+            have_halton = it;
+        }
+        rh_pass_info pi;
+        memset(&pi, 0, sizeof(pi));
+        pi.iteration = it;
+        pi.bounce = 2;
+        pi.settings = cam->pass_settings;
+        pi.settings.max_total_depth = std::min<uint8_t>(pi.settings.max_total_depth, RH_MAX_BOUNCES);
+        Ctx c{*sc, halton.data()};
+        for (auto &p : path) p.clear();
+
+        orc_gen_primary(cam, rect, w, h, it, halton.data(), rays.data());
+        orc_trace(sc, rays.data(), n, 0, nullptr, hits.data(), nullptr);
+        int count = 0;
+        auto shade_all = [&](int rc) {
+            count = 0;
+            for (int i = 0; i < rc; i++) {
+                const int x = (rays[i].xy >> 16) & 0xffff, y = rays[i].xy & 0xffff;
+                pi.index = y * w + x;
+                pi.rand_index = pi.index;
+                bool e;
+                float col[4];
+                int m;
+                shade(c, pi, hits[i], rays[i], col, &sec[count], e, &m);
+                if (e) count++;
+                path[(size_t)pi.index].push_back(Vtx{m, {col[0], col[1], col[2]}});
+            }
+        };
+        shade_all((int)n);
+        for (int b = 0; b < pi.settings.max_total_depth && count && !(pi.settings.flags & RH_SKIP_INDIRECT_LIGHT); b++) {
+            orc_trace(sc, sec.data(), (size_t)count, 0, nullptr, hits.data(), nullptr);
+            std::swap(rays, sec);
+            pi.bounce = b + 3;
+            shade_all(count);
+        }
+        for (int y = rect[1]; y < rect[1] + rect[3]; y++) {
+            for (int x = rect[0]; x < rect[0] + rect[2]; x++) {
+                const size_t p = (size_t)y * w + x;
+                const auto &v = path[p];
+                double S[3] = {0, 0, 0};
+                for (size_t k = v.size(); k-- > 0;) {
+                    for (int ch = 0; ch < 3; ch++) S[ch] += (double)v[k].c[ch];
+                    const int m = v[k].mat;
+                    if (m < 0) continue;
+                    for (int ch = 0; ch < 3; ch++) {
+                        const double mc = (double)sc->materials[m].main_color[ch];
+                        if (mc != 0.0) grad[3 * (size_t)m + ch] += scale * (double)adj_rgba[4 * p + ch] * S[ch] / mc;
+                    }
+                }
             }
         }
     }

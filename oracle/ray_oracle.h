@@ -45,6 +45,12 @@ int orc_shade(const rh_scene_view *sc, const rh_pass_info *pi, const rh_ray_pack
  * from region iteration *iteration (updated), accumulating into clean (w*h*4
  * floats, caller-owned, zero to start) and writing the tonemapped result to
  * final_rgba.  ray_counts: primary, secondary, shadow. */
+/* FP64 gradient of sum_pixels adj . (mean image of passes 1..passes) with
+ * respect to every material's main_color (3 doubles per material, added to
+ * grad), by path replay of the same passes; scale multiplies each pass's term
+ * (1/passes for the mean). */
+int orc_adjoint(const rh_scene_view *sc, const rh_camera *cam, int w, int h, const int32_t rect[4], int passes,
+                const float *adj_rgba, double scale, double *grad);
 int orc_render(const rh_scene_view *sc, const rh_camera *cam, int w, int h, const int32_t rect[4], int passes,
                int *iteration, float *clean, float *final_rgba, unsigned long long ray_counts[3]);
 

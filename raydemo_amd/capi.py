@@ -74,7 +74,8 @@ class Stats(Structure):
 
 
 class RayCounts(Structure):
-    _fields_ = [("primary", c_uint64), ("secondary", c_uint64), ("shadow", c_uint64), ("sorted", c_uint64)]
+    _fields_ = [("primary", c_uint64), ("secondary", c_uint64), ("shadow", c_uint64), ("sorted", c_uint64),
+                ("adjoint", c_uint64), ("adjoint_shadow", c_uint64)]
 
 
 class KernelProfile(Structure):
@@ -213,6 +214,18 @@ class RayLib:
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
+        # differentiation entry points (HIP backend only)
+        optional = {
+            "rhx_adjoint_scene": (c_int, [P, P, P, P, P, c_float]),
+            "rhx_set_material_colors": (c_int, [P, P, c_uint32, c_uint32, P]),
+            "rhx_read_accum": (c_int, [P, P, c_size_t]),
+        }
+        self.has_adjoint = all(hasattr(L, n) for n in optional)
+        for name, (res, args) in optional.items():
+            if hasattr(L, name):
+                f = getattr(L, name)
+                f.restype = res
+                f.argtypes = args
 
     def error(self) -> str:
         e = self.lib.rhx_last_error()
@@ -299,6 +312,30 @@ class Renderer:
 
     def render(self, scene: "Scene", region: "Region"):
         self.lib.check(self.lib.lib.rhx_render_scene(self.h, scene.h, region.h), "rhx_render_scene")
+
+    # ---- differentiation (HIP backend)
+    def adjoint(self, scene: "Scene", region: "Region", d_adj_rgba: int, d_grad: int, scale: float):
+        """Replays the pass rhx_render_scene would render next for region and adds
+        scale * d loss / d main_color to the device array d_grad (4 floats per
+        material) for the device adjoint image d_adj_rgba (w*h RGBA floats)."""
+        self.lib.check(self.lib.lib.rhx_adjoint_scene(self.h, scene.h, region.h, c_void_p(d_adj_rgba),
+                                                      c_void_p(d_grad), float(scale)), "rhx_adjoint_scene")
+
+    def set_material_colors(self, scene: "Scene", first: int, rgb: np.ndarray):
+        rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+        self.lib.check(self.lib.lib.rhx_set_material_colors(self.h, scene.h, first, rgb.shape[0], _ptr(rgb)),
+                       "rhx_set_material_colors")
+
+    def read_accum(self, dst: int | None = None) -> np.ndarray | None:
+        """The linear accumulated image (h, w, 4); into the device or host
+        address dst when given."""
+        w, h = self.w, self.hgt
+        if dst is not None:
+            self.lib.check(self.lib.lib.rhx_read_accum(self.h, c_void_p(dst), w * h * 4), "rhx_read_accum")
+            return None
+        out = np.zeros((h, w, 4), np.float32)
+        self.lib.check(self.lib.lib.rhx_read_accum(self.h, _ptr(out), out.size), "rhx_read_accum")
+        return out
 
 
 class Region:

@@ -192,15 +192,8 @@ struct PassConst {
 
 // ------------------------------------------------------------------ kernels
 
-// GeneratePrimaryRays (CoreRef.cpp:427-505): one lane per pixel of the region
-__global__ void __launch_bounds__(kBlock) k_gen_primary(PassConst pc, const float *__restrict__ halton,
-                                                        rh_ray *__restrict__ rays, rh_ray_aux *__restrict__ aux,
-                                                        int *__restrict__ count) {
-    const int rw = pc.rect[2], rh_ = pc.rect[3];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *count = rw * rh_;
-    if (i >= rw * rh_) return;
-    const int x = pc.rect[0] + i % rw, y = pc.rect[1] + i / rw;
+// GeneratePrimaryRays (CoreRef.cpp:427-505) for the pixel (x, y)
+__device__ __forceinline__ RayFull gen_primary_ray(const PassConst &pc, const float *__restrict__ halton, int x, int y) {
     const rh_camera &cam = pc.cam;
     const f3 cam_origin = ld3(cam.origin), fwd = ld3(cam.fwd), side = ld3(cam.side), up = ld3(cam.up);
     const float focus_distance = cam.focus_distance;
@@ -247,7 +240,19 @@ __global__ void __launch_bounds__(kBlock) k_gen_primary(PassConst pc, const floa
     r.dd_dy = dy - d;
     r.xy = (x << 16) | y;
     r.depth = 0;
-    store_ray(rays, aux, (uint32_t)i, r);
+    return r;
+}
+
+// one lane per pixel of the region
+__global__ void __launch_bounds__(kBlock) k_gen_primary(PassConst pc, const float *__restrict__ halton,
+                                                        rh_ray *__restrict__ rays, rh_ray_aux *__restrict__ aux,
+                                                        int *__restrict__ count) {
+    const int rw = pc.rect[2], rh_ = pc.rect[3];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *count = rw * rh_;
+    if (i >= rw * rh_) return;
+    const int x = pc.rect[0] + i % rw, y = pc.rect[1] + i / rw;
+    store_ray(rays, aux, (uint32_t)i, gen_primary_ray(pc, halton, x, y));
 }
 
 struct TraceArgs {
@@ -516,6 +521,118 @@ __global__ void __launch_bounds__(kBlock) k_path_tail(ShadeArgs a, int first, in
     if (a.ray_counter) wave_add_u64(a.ray_counter, traced);
     if (a.shadow_counter) wave_add_u64(a.shadow_counter, cnt.shadow_rays);
     if (st.overflow && a.overflow) atomicAdd(a.overflow, (unsigned long long)st.overflow);
+}
+
+// ---------------------------------------------------------------- adjoint
+
+struct AdjointArgs {
+    Scene sc;
+    PassConst pc;
+    rh_pass_info pi;    // pass-level; index/rand_index derived per pixel
+    const float *halton;
+    const float4 *adj;  // dLoss/d(pass image) per pixel of the frame (rgb, w unused)
+    float scale;        // weight of this pass in the image (1 / passes for the mean)
+    float *grad;        // 4 floats per material: d loss / d main_color (rgb, pad)
+    float4 *rec;        // per-lane vertex records, max_vertices x lanes, lane-minor
+    uint32_t lanes;
+    uint32_t *spill;
+    uint32_t spill_stride;
+    uint32_t lds_depth;
+    unsigned long long *ray_counter;
+    unsigned long long *shadow_counter;
+    unsigned long long *overflow;
+};
+
+// Path replay for one pass: each lane regenerates its pixel's primary ray with
+// the pass's iteration and Halton table, follows the same path as the forward
+// pass (trace, ShadeSurface with visibility traced inline, next ray) and
+// records each vertex's colour C_j and the material whose main_color scales it.
+// A vertex's main_color m multiplies C_j for every j >= its own index exactly
+// once (the throughput carries it; diffuse NEE and emission carry it at the
+// vertex itself), so with S_m = sum_{j >= m} C_j:
+//     d(pixel) / d main_color[c] = S_m[c] / main_color[c].
+// The suffix sums are formed backwards over the records; the gradient adds
+// adj[pixel] * scale * S_m / main_color with float atomics (their order is the
+// only run-to-run difference).
+__global__ void __launch_bounds__(kBlock) k_adjoint(AdjointArgs a) {
+    extern __shared__ uint32_t lds_stack[];
+    const int rw = a.pc.rect[2], rh_ = a.pc.rect[3];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = (int)i < rw * rh_;
+    ShadeCounters cnt;
+    cnt.shadow_rays = 0;
+    uint32_t traced = 0;
+    Stack st;
+    st.lds = lds_stack;
+    st.spill = a.spill + i;
+    st.spill_stride = a.spill_stride;
+    st.lds_depth = a.lds_depth;
+    st.spill_depth = kSpillDepth;
+    st.lane = threadIdx.x;
+    st.block = blockDim.x;
+    st.sp = 0;
+    st.overflow = 0;
+    if (active) {
+        const int x = a.pc.rect[0] + (int)i % rw, y = a.pc.rect[1] + (int)i / rw;
+        const int p = y * a.pc.w + x;
+        const float4 dl = a.adj[p];
+        if (dl.x != 0.0f || dl.y != 0.0f || dl.z != 0.0f) {
+            RayFull ray = gen_primary_ray(a.pc, a.halton, x, y);
+            rh_pass_info pi = a.pi;
+            pi.index = p;
+            pi.rand_index = p;
+            if (pi.settings.flags & RH_USE_COHERENT_SAMPLING) pi.rand_index = c_sampling_pattern[(y % 8) * 8 + (x % 8)];
+            const int last = pi.settings.max_total_depth;
+            const bool skip_indirect = (pi.settings.flags & RH_SKIP_INDIRECT_LIGHT) != 0;
+            int nv = 0;
+            for (int b = -1;;) {
+                Hit hit;
+                hit.mask = 0;
+                hit.obj = -1;
+                hit.prim = -1;
+                hit.t = RH_MAX_DIST;
+                hit.u = hit.v = 0.0f;
+                st.sp = 0;
+                if (a.sc.macro_root != RH_INVALID) traverse_fast<false>(a.sc, ray.o, ray.d, hit, st, nullptr);
+                traced++;
+                pi.bounce = b < 0 ? 2 : b + 3;
+                ShadeOut so;
+                so.emit = false;
+                shade_surface(a.sc, pi, hit, ray, a.halton, st, cnt, so, nullptr);
+                a.rec[(size_t)nv * a.lanes + i] = make_float4(so.col.x, so.col.y, so.col.z, __int_as_float(so.alb_mat));
+                nv++;
+                b++;
+                if (!so.emit || skip_indirect || b >= last) break;
+                ray = so.next;
+            }
+            f3 S = f3{0.0f, 0.0f, 0.0f};
+            for (int k = nv - 1; k >= 0; k--) {
+                const float4 r = a.rec[(size_t)k * a.lanes + i];
+                S = S + f3{r.x, r.y, r.z};
+                const int m = __float_as_int(r.w);
+                if (m < 0) continue;
+                const float *mc = a.sc.materials[m].main_color;
+                const float g[3] = {dl.x * S.x, dl.y * S.y, dl.z * S.z};
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++) {
+                    if (mc[ch] != 0.0f && g[ch] != 0.0f) atomicAdd(&a.grad[4 * m + ch], a.scale * (g[ch] / mc[ch]));
+                }
+            }
+        }
+    }
+    if (a.ray_counter) wave_add_u64(a.ray_counter, traced);
+    if (a.shadow_counter) wave_add_u64(a.shadow_counter, cnt.shadow_rays);
+    if (st.overflow && a.overflow) atomicAdd(a.overflow, (unsigned long long)st.overflow);
+}
+
+// main_color of materials [first, first + n) from packed rgb triples
+__global__ void k_set_material_colors(rh_material *mats, const float *__restrict__ rgb, uint32_t first, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    rh_material &m = mats[first + k];
+    m.main_color[0] = rgb[3 * k + 0];
+    m.main_color[1] = rgb[3 * k + 1];
+    m.main_color[2] = rgb[3 * k + 2];
 }
 
 struct ShadowArgs {
@@ -843,6 +960,12 @@ struct rh_context {
     int *pend_count = nullptr;
     uint32_t *shadow_queue = nullptr; // compacted shadow work: slot indices
     int defer_nee = 1;
+    // adjoint replay: per-lane vertex records (max vertices x lanes)
+    float4 *adj_rec = nullptr;
+    size_t adj_rec_cap = 0; // lanes
+    float *color_stage = nullptr;
+    size_t color_stage_cap = 0; // floats
+    uint32_t n_materials = 0;
     int *counters = nullptr; // [0], [1]: ray queue counts, [2]: shadow queue count
     unsigned long long *stats_dev = nullptr; // [0] primary [1] secondary [2] shadow [3] sorted [4] overflow
     float *halton = nullptr;
@@ -1323,6 +1446,7 @@ int rh_upload_scene(rh_context *c, const rh_scene_view *v) {
     if (s.macro_root != RH_INVALID && (!v->nodes || s.macro_root >= v->nodes_count))
         return arg_error("macro tree root outside the node array");
     c->sc = s;
+    c->n_materials = (uint32_t)v->materials_count;
     c->have_scene = true;
     RH_CHECK(hipStreamSynchronize(c->stream));
     return 0;
@@ -1494,6 +1618,8 @@ int rh_get_ray_counts(rh_context *c, rh_ray_counts *out) {
     out->secondary = v[1];
     out->shadow = v[2];
     out->sorted = v[3];
+    out->adjoint = v[5];
+    out->adjoint_shadow = v[6];
     if (v[4]) {
         g_err = "traversal stack overflow: " + std::to_string(v[4]) + " pushes dropped";
     }
@@ -1767,6 +1893,83 @@ int rh_shade_rays(rh_context *c, const rh_pass_info *pi, const rh_ray_packet *ra
     free_ptr(d_pi);
     free_ptr(d_rgba);
     free_ptr(d_src);
+    return 0;
+}
+
+
+// ---- differentiation: path-replay adjoint of one pass, material colours, accumulated image
+
+int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgba, float *d_grad, float scale) {
+    if (!c->have_scene) return arg_error("no scene uploaded");
+    const int rx = p->rect[0], ry = p->rect[1], rw = p->rect[2], rhh = p->rect[3];
+    if (rw <= 0 || rhh <= 0 || rx < 0 || ry < 0 || rx + rw > c->w || ry + rhh > c->h)
+        return arg_error("region outside the framebuffer");
+    if (p->iteration < 1) return arg_error("iteration must be >= 1");
+    if (!d_adj_rgba || !d_grad) return arg_error("adjoint image and gradient buffers are required");
+    const int depth = std::min<int>(p->cam.pass_settings.max_total_depth, RH_MAX_BOUNCES);
+    const size_t lanes = (size_t)rw * rhh;
+    const size_t max_vertices = (size_t)depth + 1;
+    if (lanes > c->adj_rec_cap || !c->adj_rec) {
+        free_ptr(c->adj_rec);
+        c->adj_rec = nullptr;
+        c->adj_rec_cap = lanes;
+        RH_CHECK(hipMalloc(&c->adj_rec, (RH_MAX_BOUNCES + 1) * lanes * sizeof(float4)));
+    }
+    (void)max_vertices;
+    int rc = ensure_spill(c, lanes);
+    if (rc) return rc;
+    AdjointArgs a;
+    memset(&a, 0, sizeof(a));
+    a.sc = c->sc;
+    a.pc = make_pass_const(c, p);
+    a.pi.iteration = p->iteration;
+    a.pi.bounce = 2;
+    a.pi.settings = p->cam.pass_settings;
+    a.pi.settings.max_total_depth = (uint8_t)depth;
+    a.halton = c->halton;
+    a.adj = reinterpret_cast<const float4 *>(d_adj_rgba);
+    a.scale = scale;
+    a.grad = d_grad;
+    a.rec = c->adj_rec;
+    a.lanes = (uint32_t)c->adj_rec_cap;
+    a.spill = c->spill;
+    a.spill_stride = (uint32_t)c->spill_cap;
+    a.lds_depth = (uint32_t)c->lds_depth;
+    a.ray_counter = c->stats_dev + 5;
+    a.shadow_counter = c->stats_dev + 6;
+    a.overflow = c->stats_dev + 4;
+    KernelMark km(c, 4);
+    hipLaunchKernelGGL(k_adjoint, dim3((unsigned)((lanes + kBlock - 1) / kBlock)), dim3(kBlock), lds_bytes(c),
+                       c->stream, a);
+    RH_CHECK(hipGetLastError());
+    return 0;
+}
+
+int rh_set_material_colors(rh_context *c, uint32_t first, uint32_t n, const float *rgb) {
+    if (!c->have_scene) return arg_error("no scene uploaded");
+    if ((size_t)first + n > c->n_materials) return arg_error("material range outside the scene");
+    if (!n) return 0;
+    const size_t floats = (size_t)3 * n;
+    if (floats > c->color_stage_cap || !c->color_stage) {
+        free_ptr(c->color_stage);
+        c->color_stage = nullptr;
+        c->color_stage_cap = floats;
+        RH_CHECK(hipMalloc(&c->color_stage, floats * sizeof(float)));
+    }
+    RH_CHECK(hipMemcpyAsync(c->color_stage, rgb, floats * sizeof(float), hipMemcpyDefault, c->stream));
+    hipLaunchKernelGGL(k_set_material_colors, dim3((n + 255) / 256), dim3(256), 0, c->stream,
+                       const_cast<rh_material *>(c->sc.materials), c->color_stage, first, n);
+    RH_CHECK(hipGetLastError());
+    // the caller's array may change right after this call returns
+    RH_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int rh_read_accum(rh_context *c, float *dst, size_t count) {
+    const size_t n = (size_t)c->w * c->h * 4;
+    if (count < n) return arg_error("output buffer too small");
+    RH_CHECK(hipMemcpyAsync(dst, c->clean, n * sizeof(float), hipMemcpyDefault, c->stream));
+    RH_CHECK(hipStreamSynchronize(c->stream));
     return 0;
 }
 

@@ -232,6 +232,10 @@ struct ShadeOut {
     bool consider_albedo;
     f3 albedo;
     RayFull next;
+    // material whose main_color scales this vertex's colour and every later
+    // contribution of the path exactly once (emissive; diffuse and glossy
+    // when the albedo is considered), or -1
+    int alb_mat;
 };
 
 // ShadeSurface (CoreRef.cpp:2055-2391) for one (ray, hit) pair.
@@ -242,6 +246,7 @@ __device__ __forceinline__ void shade_surface(const Scene &sc, const rh_pass_inf
                                               ShadeOut &out, NeeSink *sink = nullptr) {
     out.emit = false;
     out.nee_pending = false;
+    out.alb_mat = -1;
     const uint32_t flags = pi.settings.flags;
     const bool add_env = !(flags & RH_NO_BACKGROUND) || pi.bounce > 2;
     const bool add_direct = !(flags & RH_SKIP_DIRECT_LIGHT) || pi.bounce > 2;
@@ -356,6 +361,8 @@ __device__ __forceinline__ void shade_surface(const Scene &sc, const rh_pass_inf
     const f3 alb = f3{albedo.x, albedo.y, albedo.z};
 
     f3 col = f3{0.0f, 0.0f, 0.0f};
+    if (mat->type == RH_MAT_EMISSIVE || (consider_albedo && (mat->type == RH_MAT_DIFFUSE || mat->type == RH_MAT_GLOSSY)))
+        out.alb_mat = (int)(mat - sc.materials);
 
     const int diff_depth = ray.depth & 0xff;
     const int gloss_depth = (ray.depth >> 8) & 0xff;

@@ -168,6 +168,38 @@ int rhx_render_scene(rhx_renderer *r, rhx_scene *s, rhx_region *reg) {
     });
 }
 
+int rhx_adjoint_scene(rhx_renderer *r, rhx_scene *s, rhx_region *reg, const float *d_adj_rgba, float *d_grad,
+                      float scale) {
+    return guarded([&] {
+        r->r->last_error.clear();
+        if (!r->r->AdjointScene(s->s, reg->ctx, d_adj_rgba, d_grad, scale)) {
+            g_err = r->r->last_error.empty() ? "adjoint pass not run" : r->r->last_error;
+            return -1;
+        }
+        return 0;
+    });
+}
+
+int rhx_set_material_colors(rhx_renderer *r, rhx_scene *s, uint32_t first, uint32_t n, const float *rgb) {
+    return guarded([&] {
+        if (!r->r->SetMaterialColors(s->s, first, n, rgb)) {
+            g_err = r->r->last_error;
+            return -1;
+        }
+        return 0;
+    });
+}
+
+int rhx_read_accum(rhx_renderer *r, float *dst, size_t count) {
+    return guarded([&] {
+        if (!r->r->ReadAccumulated(dst, count)) {
+            g_err = r->r->last_error;
+            return -1;
+        }
+        return 0;
+    });
+}
+
 void rhx_set_environment(rhx_scene *s, const float env_col[3], float env_clamp, uint32_t env_map) {
     Ray::environment_desc_t e;
     memcpy(e.env_col, env_col, 3 * sizeof(float));

@@ -104,6 +104,8 @@ class Renderer : public RendererBase {
     int uploaded_halton_epoch_ = -1;
     stats_t stats_ = {};
 
+    bool PreparePass(Scene &s, RegionContext &region, rh_pass_desc &pass);
+
 public:
     Renderer(const settings_t &s, int device_index);
     ~Renderer() override;
@@ -121,6 +123,15 @@ public:
 
     // HIP-specific
     rh_context *context() { return ctx_; }
+    // Differentiation.  AdjointScene replays the pass a RenderScene call with
+    // the same region state would render and accumulates d loss / d
+    // main_color (see rh_adjoint_pass); SetMaterialColors patches main_color
+    // (host rgb triples) in the scene and in its device copy;
+    // ReadAccumulated copies the linear accumulated image.
+    bool AdjointScene(const std::shared_ptr<SceneBase> &s, RegionContext &region, const float *d_adj_rgba,
+                      float *d_grad, float scale);
+    bool SetMaterialColors(const std::shared_ptr<SceneBase> &s, uint32_t first, uint32_t n, const float *rgb);
+    bool ReadAccumulated(float *dst, size_t count);
     void set_sort_rays(bool on) { sort_rays_ = on; }
     std::string last_error;
 };

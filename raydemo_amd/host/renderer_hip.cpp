@@ -53,28 +53,28 @@ const pixel_color_t *Renderer::get_pixels_ref() const {
 
 std::shared_ptr<SceneBase> Renderer::CreateScene() { return std::make_shared<Scene>(); }
 
-void Renderer::RenderScene(const std::shared_ptr<SceneBase> &_s, RegionContext &region) {
-    const auto s = std::dynamic_pointer_cast<Scene>(_s);
-    if (!s) return; // scene of another backend: no-op, as RendererRef.cpp:20-21
-    if (s->current_cam() == 0xffffffff) return;
+// Uploads the scene if it changed, advances the region's iteration and
+// refreshes/uploads its Halton table exactly as the Ref renderer does
+// (RendererRef.cpp:19-96), and describes the pass.  false: nothing to render.
+bool Renderer::PreparePass(Scene &s, RegionContext &region, rh_pass_desc &pass) {
+    if (s.current_cam() == 0xffffffff) return false;
 
-    s->Finalize();
-    if (uploaded_scene_ != s.get() || uploaded_version_ != s->version()) {
-        const rh_scene_view v = s->View();
+    s.Finalize();
+    if (uploaded_scene_ != &s || uploaded_version_ != s.version()) {
+        const rh_scene_view v = s.View();
         if (rh_upload_scene(ctx_, &v) != 0) {
             last_error = rh_last_error();
-            return;
+            return false;
         }
-        uploaded_scene_ = s.get();
-        uploaded_version_ = s->version();
+        uploaded_scene_ = &s;
+        uploaded_version_ = s.version();
     }
 
-    rh_pass_desc pass;
     memset(&pass, 0, sizeof(pass));
-    memcpy(&pass.cam, &s->camera(s->current_cam()), sizeof(rh_camera));
+    memcpy(&pass.cam, &s.camera(s.current_cam()), sizeof(rh_camera));
 
-    if (s->macro_nodes_root_ != 0xffffffff) {
-        const rh_bvh_node &root = s->nodes_[s->macro_nodes_root_];
+    if (s.macro_nodes_root_ != 0xffffffff) {
+        const rh_bvh_node &root = s.nodes_[s.macro_nodes_root_];
         for (int i = 0; i < 3; i++) {
             pass.root_min[i] = root.bbox_min[i];
             pass.cell_size[i] = (root.bbox_max[i] - root.bbox_min[i]) / 255;
@@ -85,7 +85,7 @@ void Renderer::RenderScene(const std::shared_ptr<SceneBase> &_s, RegionContext &
     if (rect.w == 0 || rect.h == 0) rect = {0, 0, w_, h_};
     if (rect.x < 0 || rect.y < 0 || rect.x + rect.w > w_ || rect.y + rect.h > h_) {
         last_error = "region outside the framebuffer";
-        return;
+        return false;
     }
 
     region.iteration++;
@@ -98,7 +98,7 @@ void Renderer::RenderScene(const std::shared_ptr<SceneBase> &_s, RegionContext &
     if (uploaded_halton_ != region.halton_seq.get() || uploaded_halton_epoch_ != epoch) {
         if (rh_set_halton(ctx_, region.halton_seq.get(), RH_HALTON_COUNT * RH_HALTON_SEQ_LEN) != 0) {
             last_error = rh_last_error();
-            return;
+            return false;
         }
         uploaded_halton_ = region.halton_seq.get();
         uploaded_halton_epoch_ = epoch;
@@ -111,12 +111,60 @@ void Renderer::RenderScene(const std::shared_ptr<SceneBase> &_s, RegionContext &
     pass.iteration = region.iteration;
     pass.sort_rays = sort_rays_ ? 1 : 0;
     pass.cam.pass_settings.max_total_depth = std::min<uint8_t>(pass.cam.pass_settings.max_total_depth, RH_MAX_BOUNCES);
+    return true;
+}
 
+void Renderer::RenderScene(const std::shared_ptr<SceneBase> &_s, RegionContext &region) {
+    const auto s = std::dynamic_pointer_cast<Scene>(_s);
+    if (!s) return; // scene of another backend: no-op, as RendererRef.cpp:20-21
+    rh_pass_desc pass;
+    if (!PreparePass(*s, region, pass)) return;
     if (rh_render_pass(ctx_, &pass) != 0) {
         last_error = rh_last_error();
         return;
     }
     host_dirty_ = true;
+}
+
+bool Renderer::AdjointScene(const std::shared_ptr<SceneBase> &_s, RegionContext &region, const float *d_adj_rgba,
+                            float *d_grad, float scale) {
+    const auto s = std::dynamic_pointer_cast<Scene>(_s);
+    if (!s) return false;
+    rh_pass_desc pass;
+    if (!PreparePass(*s, region, pass)) return false;
+    if (rh_adjoint_pass(ctx_, &pass, d_adj_rgba, d_grad, scale) != 0) {
+        last_error = rh_last_error();
+        return false;
+    }
+    return true;
+}
+
+bool Renderer::SetMaterialColors(const std::shared_ptr<SceneBase> &_s, uint32_t first, uint32_t n, const float *rgb) {
+    const auto s = std::dynamic_pointer_cast<Scene>(_s);
+    if (!s) return false;
+    if ((size_t)first + n > s->materials_.size()) {
+        last_error = "material range outside the scene";
+        return false;
+    }
+    for (uint32_t k = 0; k < n; k++)
+        for (int ch = 0; ch < 3; ch++) s->materials_[first + k].main_color[ch] = rgb[3 * k + ch];
+    // the device copy is patched in place when this scene is the uploaded one;
+    // otherwise the next pass uploads the updated host arrays
+    if (uploaded_scene_ == s.get() && uploaded_version_ == s->version()) {
+        if (rh_set_material_colors(ctx_, first, n, rgb) != 0) {
+            last_error = rh_last_error();
+            return false;
+        }
+    }
+    return true;
+}
+
+bool Renderer::ReadAccumulated(float *dst, size_t count) {
+    if (rh_read_accum(ctx_, dst, count) != 0) {
+        last_error = rh_last_error();
+        return false;
+    }
+    return true;
 }
 
 void Renderer::GetStats(stats_t &st) {

@@ -10,6 +10,12 @@ Workloads (seed = 0x5EED0000 + k for workload k):
                      lights plus a constant environment, 50/50 diffuse / glossy
                      (roughness U[0.05, 1]); 1920x1080, 64 spp, depth 8.
   mesh_lights      — the reference's own mesh-lights test scene (test_scene1.h).
+  cornell_refined  — the Cornell box with every wall and box face split into a
+                     k x k grid (k = 17: 9,826 triangles + the 2-triangle light),
+                     one diffuse material per triangle with albedo U[0.1, 0.9]
+                     (seed 0x5EED0004; the hidden target uses 0x5EED0005), one
+                     emissive quad; 512x512, 32 spp, depth 6, Russian roulette off
+                     (the estimator is then smooth in the albedos).
 """
 from __future__ import annotations
 
@@ -121,6 +127,93 @@ def cornell_box(scene: "A.Scene", seed: int = SEED_BASE + 1, width=256, height=2
                            max_transp_depth=depth, max_total_depth=depth)
     scene.set_current_cam(cam)
     return SceneSpec("cornell_box", width, height, spp, depth, triangles=len(idx) // 3)
+
+
+def _quad_grid(p0, p1, p2, p3, n, k):
+    """A planar quad split into k x k cells, two triangles each, wound like _quad."""
+    p0, p1, p3 = (np.asarray(p, np.float64) for p in (p0, p1, p3))
+    ex, ey = p1 - p0, p3 - p0
+    geo = np.cross(np.subtract(p1, p0), np.subtract(p2, p0))
+    flip = np.dot(geo, n) <= 0
+    verts = []
+    for j in range(k + 1):
+        for i in range(k + 1):
+            p = p0 + ex * (i / k) + ey * (j / k)
+            verts.append(list(p) + list(n) + [i / k, j / k])
+    idx = []
+    for j in range(k):
+        for i in range(k):
+            a, b, c, d = j * (k + 1) + i, j * (k + 1) + i + 1, (j + 1) * (k + 1) + i + 1, (j + 1) * (k + 1) + i
+            idx += [a, c, b, a, d, c] if flip else [a, b, c, a, c, d]
+    return np.array(verts, np.float32), np.array(idx, np.uint32)
+
+
+def cornell_refined_geometry(k: int = 17, seed: int = SEED_BASE + 1):
+    """Cornell box faces as k x k grids: (attrs, indices, n_emissive_tris) with the
+    two emissive triangles last."""
+    rng = np.random.default_rng(seed)
+    L = 5.0
+    quads = [
+        ((-L, 0, -L), (L, 0, -L), (L, 0, L), (-L, 0, L), (0, 1, 0)),
+        ((-L, 2 * L, L), (L, 2 * L, L), (L, 2 * L, -L), (-L, 2 * L, -L), (0, -1, 0)),
+        ((-L, 0, L), (L, 0, L), (L, 2 * L, L), (-L, 2 * L, L), (0, 0, -1)),
+        ((-L, 0, -L), (-L, 0, L), (-L, 2 * L, L), (-L, 2 * L, -L), (1, 0, 0)),
+        ((L, 0, L), (L, 0, -L), (L, 2 * L, -L), (L, 2 * L, L), (-1, 0, 0)),
+    ]
+    a1, a2 = rng.uniform(0.0, math.pi / 2, size=2)
+    for center, size, angle in (((-1.8, 1.5, 1.0), (2.6, 3.0, 2.6), a1), ((1.8, 3.0, -0.8), (2.6, 6.0, 2.6), a2)):
+        cx, cy, cz = center
+        sx, sy, sz = size
+        c, s = math.cos(angle), math.sin(angle)
+        tr = lambda x, y, z: (cx + c * x + s * z, cy + y, cz - s * x + c * z)
+        rn = lambda x, y, z: (c * x + s * z, y, -s * x + c * z)
+        hx, hy, hz = sx / 2, sy / 2, sz / 2
+        for f in (((-hx, -hy, hz), (hx, -hy, hz), (hx, hy, hz), (-hx, hy, hz), (0, 0, 1)),
+                  ((hx, -hy, -hz), (-hx, -hy, -hz), (-hx, hy, -hz), (hx, hy, -hz), (0, 0, -1)),
+                  ((hx, -hy, hz), (hx, -hy, -hz), (hx, hy, -hz), (hx, hy, hz), (1, 0, 0)),
+                  ((-hx, -hy, -hz), (-hx, -hy, hz), (-hx, hy, hz), (-hx, hy, -hz), (-1, 0, 0)),
+                  ((-hx, hy, hz), (hx, hy, hz), (hx, hy, -hz), (-hx, hy, -hz), (0, 1, 0)),
+                  ((-hx, -hy, -hz), (hx, -hy, -hz), (hx, -hy, hz), (-hx, -hy, hz), (0, -1, 0))):
+            quads.append((tr(*f[0]), tr(*f[1]), tr(*f[2]), tr(*f[3]), rn(*f[4])))
+    attrs, idx, nv = [], [], 0
+    for q in quads:
+        a, i = _quad_grid(*q, k)
+        attrs.append(a)
+        idx.append(i + nv)
+        nv += len(a)
+    a, i = _quad((-1.2, 2 * L - 0.01, 1.2), (1.2, 2 * L - 0.01, 1.2), (1.2, 2 * L - 0.01, -1.2),
+                 (-1.2, 2 * L - 0.01, -1.2), (0, -1, 0))
+    attrs.append(a)
+    idx.append(i + nv)
+    return np.concatenate(attrs).astype(np.float32), np.concatenate(idx).astype(np.uint32), 2
+
+
+def refined_albedos(n_tris: int, seed: int = SEED_BASE + 4) -> np.ndarray:
+    """Per-triangle RGB albedo U[0.1, 0.9] (float32, n x 3)."""
+    return np.random.default_rng(seed).uniform(0.1, 0.9, size=(n_tris, 3)).astype(np.float32)
+
+
+def cornell_refined(scene: "A.Scene", k: int = 17, albedo_seed: int = SEED_BASE + 4, width=512, height=512, spp=32,
+                    depth=6, albedos=None) -> SceneSpec:
+    """Materials 0..n-1 are the diffuse triangles in index order, material n the
+    emissive light (strength 10, main_color (1, 1, 1))."""
+    attrs, idx, n_light = cornell_refined_geometry(k)
+    n_tris = len(idx) // 3 - n_light
+    alb = refined_albedos(n_tris, albedo_seed) if albedos is None else np.asarray(albedos, np.float32)
+    tex = _white(scene)
+    mats = [scene.add_material(A.DiffuseMaterial, tex, main_color=(float(a[0]), float(a[1]), float(a[2])))
+            for a in alb]
+    light = scene.add_material(A.EmissiveMaterial, tex, main_color=(1.0, 1.0, 1.0), strength=10.0)
+    shapes = [(mats[t], 3 * t, 3) for t in range(n_tris)] + [(light, 3 * n_tris, 3 * n_light)]
+    mesh = scene.add_mesh(A.PxyzNxyzTuv, attrs, idx, shapes)
+    scene.add_mesh_instance(mesh)
+    scene.set_environment((0.0, 0.0, 0.0))
+    cam = scene.add_camera(origin=(0.0, 5.0, -14.0), fwd=(0.0, 0.0, 1.0), fov=45.0, filter=A.Box,
+                           dtype=A.DeviceNone, max_diff_depth=depth, max_glossy_depth=depth, max_refr_depth=depth,
+                           max_transp_depth=depth, max_total_depth=depth, termination_start_depth=16)
+    scene.set_current_cam(cam)
+    return SceneSpec("cornell_refined", width, height, spp, depth, triangles=len(idx) // 3,
+                     info={"materials": n_tris + 1, "diffuse": n_tris, "light_material": n_tris})
 
 
 def icosphere(subdiv: int):

@@ -149,6 +149,7 @@ class Oracle:
             ("orc_sort_rays", [P, C.c_size_t, P, P]),
             ("orc_shade", [P, P, P, P, C.c_size_t, P, P, P, P, P]),
             ("orc_render", [P, P, C.c_int, C.c_int, P, C.c_int, P, P, P, P]),
+            ("orc_adjoint", [P, P, C.c_int, C.c_int, P, C.c_int, P, C.c_double, P]),
         ]:
             f = getattr(L, name)
             f.restype = C.c_int
@@ -200,6 +201,23 @@ class Oracle:
                            _p(np.ascontiguousarray(rays, RAY_PACKET_DTYPE)), _p(np.ascontiguousarray(hits, HIT_DTYPE)),
                            n, _p(np.ascontiguousarray(halton, np.float32)), _p(rgba), _p(sec), _p(cnt), _p(src))
         return rgba, sec[:cnt[0]], src[:cnt[0]]
+
+    def render_linear(self, view, cam, w, h, passes, rect=None):
+        """The mean (linear, before tonemapping) image of passes 1..passes."""
+        rect = np.array(rect or (0, 0, w, h), np.int32)
+        clean = np.zeros((h, w, 4), np.float32)
+        final = np.zeros((h, w, 4), np.float32)
+        it = np.zeros(1, np.int32)
+        self.lib.orc_render(C.byref(view), C.byref(cam), w, h, _p(rect), passes, _p(it), _p(clean), _p(final), None)
+        return clean
+
+    def adjoint(self, view, cam, w, h, passes, adj, rect=None):
+        """FP64 d(sum adj . mean image)/d main_color, (materials, 3)."""
+        rect = np.array(rect or (0, 0, w, h), np.int32)
+        adj = np.ascontiguousarray(adj, np.float32).reshape(h, w, 4)
+        grad = np.zeros((view.materials_count, 3), np.float64)
+        self.lib.orc_adjoint(C.byref(view), C.byref(cam), w, h, _p(rect), passes, _p(adj), 1.0 / passes, _p(grad))
+        return grad
 
     def render(self, view, cam, w, h, passes, rect=None):
         rect = np.array(rect or (0, 0, w, h), np.int32)

@@ -1,0 +1,93 @@
+"""Path-replay gradients on the GPU (rh_adjoint_pass through the flat ABI):
+against the CPU oracle's FP64 gradient on the same passes, against central
+finite differences (h = 1e-3) of the GPU renderer itself, and an Adam descent
+that must lower the loss.  Material colours patched on the device must render
+exactly like a scene built with them."""
+import numpy as np
+import pytest
+
+from conftest import standalone_scene
+from raydemo_amd import capi as A, scenes
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+W, H, SPP, DEPTH, K = 32, 24, 3, 3, 2
+
+
+def _small(host_lib, albedo_seed=None):
+    from raydemo_amd import adjoint
+    return adjoint.refined_cornell(host_lib, k=K, width=W, height=H, spp=SPP, depth=DEPTH, albedo_seed=albedo_seed)
+
+
+def test_patched_colors_render_like_a_rebuilt_scene(host_lib):
+    dr, colors, spec = _small(host_lib)
+    other = scenes.refined_albedos(spec.info["diffuse"], scenes.SEED_BASE + 5)
+    new = colors.copy()
+    new[:len(other)] = other
+    img_a = dr.render(torch.tensor(new)).cpu().numpy()          # patched in place on the device
+    dr2, _, _ = _small(host_lib, albedo_seed=scenes.SEED_BASE + 5)  # built with those albedos
+    img_b = dr2.render().cpu().numpy()
+    assert img_a.tobytes() == img_b.tobytes()
+
+
+def test_gradient_matches_oracle_fp64(host_lib, oracle):
+    dr, colors, spec = _small(host_lib)
+    rng = np.random.default_rng(11)
+    wts = rng.uniform(-1, 1, size=(H, W, 4)).astype(np.float32)
+    wts[..., 3] = 0
+    img = dr.render(torch.tensor(colors))
+    g = dr.adjoint(torch.tensor(wts, device="cuda")).cpu().numpy().astype(np.float64)
+    hs = standalone_scene(host_lib)
+    scenes.cornell_refined(hs, k=K, width=W, height=H, spp=SPP, depth=DEPTH)
+    ref = oracle.adjoint(hs.raw_view(), hs.camera_record(0), W, H, SPP, wts)
+    # the forward images agree bit for bit, so the paths are the same
+    ref_img = oracle.render_linear(hs.raw_view(), hs.camera_record(0), W, H, SPP)
+    assert img.cpu().numpy().tobytes() == ref_img.tobytes()
+    scale = np.abs(ref).max()
+    err = np.abs(g - ref)
+    print(f"adjoint vs FP64: max abs err {err.max():.3e} (max |g| {scale:.3e})")
+    assert err.max() <= 1e-5 * scale
+    assert np.all(err <= 1e-4 * np.abs(ref) + 1e-6 * scale)
+
+
+def test_gradient_matches_finite_differences(host_lib):
+    dr, colors, spec = _small(host_lib)
+    rng = np.random.default_rng(5)
+    target = torch.tensor(rng.uniform(0, 0.5, size=(H, W, 3)).astype(np.float32), device="cuda")
+    base = torch.tensor(colors, device="cuda")
+
+    def loss_of(c):
+        img = dr.render(c)[..., :3]
+        return float(torch.sum((img.double() - target.double()) ** 2))
+
+    img = dr.render(base)[..., :3]
+    grad = dr.adjoint(2 * (img - target)).cpu().numpy()
+    hstep = 1e-3
+    order = np.argsort(-np.abs(grad).sum(axis=1))
+    picks = list(order[:8]) + [spec.info["light_material"]]
+    for m in picks:
+        for ch in range(3):
+            cp, cm = base.clone(), base.clone()
+            cp[m, ch] += hstep
+            cm[m, ch] -= hstep
+            fd = (loss_of(cp) - loss_of(cm)) / (2 * hstep)
+            g = float(grad[m, ch])
+            # the loss is quadratic in the image: central differences carry an
+            # O(h^2) term besides FP32 rounding of the rendered images
+            assert abs(fd - g) <= 2e-3 * abs(g) + 1e-4 * np.abs(grad).max(), (m, ch, fd, g)
+
+
+def test_adam_descent_lowers_the_loss(host_lib):
+    from raydemo_amd import adjoint
+    dr, init, spec = _small(host_lib)
+    n = spec.info["diffuse"]
+    hidden = init.copy()
+    hidden[:n] = scenes.refined_albedos(n, scenes.SEED_BASE + 5)
+    target = dr.render(torch.tensor(hidden))[..., :3].clone()
+    res = adjoint.descend(dr, init, target, hidden, steps=30, lr=0.03, optimize=slice(0, n))
+    print(res)
+    assert res.loss_last < 0.5 * res.loss_first
+    rc = dr.r.ray_counts()
+    assert rc["adjoint"] > 0
