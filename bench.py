@@ -173,6 +173,51 @@ def run_cpu_baseline(args, A, scenes, spp_sample):
     return dict(kind=kind, seconds=dt, cores=cores, spp=spp_sample)
 
 
+def run_adjoint(args, lib, A, scenes):
+    """Refined Cornell box (9,828 triangles, one material per triangle), 512x512,
+    32 spp, depth 6: Adam on the 9,826 albedos against a target rendered from a
+    hidden albedo set (seed 0x5EED0005).  One iteration = forward (32 passes) +
+    adjoint replay (32 passes) + Adam update."""
+    import torch
+    from raydemo_amd import adjoint
+
+    dr, init, spec = adjoint.refined_cornell(lib, width=args.adj_size, height=args.adj_size, spp=args.adj_spp,
+                                            depth=6)
+    n = spec.info["diffuse"]
+    hidden = init.copy()
+    hidden[:n] = scenes.refined_albedos(n, scenes.SEED_BASE + 5)
+    target = dr.render(torch.tensor(hidden))[..., :3].clone()
+    # stage timing of one forward and one backward (after a warm-up)
+    x = torch.tensor(init, device="cuda")
+    for _ in range(2):
+        dr.r.reset_stats()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        img = dr.render(x)[..., :3]
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        g = dr.adjoint(2 * (img - target) / img.numel())
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+    rc = dr.r.ray_counts()
+    fwd_rays = rc["primary"] + rc["secondary"] + rc["shadow"]
+    adj_rays = rc["adjoint"] + rc["adjoint_shadow"]
+    res = adjoint.descend(dr, init, target, hidden, steps=args.adj_steps, lr=0.02, optimize=slice(0, n))
+    out = {
+        "metric": "adjoint s/iter (forward + path-replay adjoint + Adam)",
+        "s_per_iter": round(res.s_per_iter, 5), "steps": res.steps, "seconds": round(res.seconds, 3),
+        "forward_ms": round((t1 - t0) * 1e3, 3), "adjoint_ms": round((t2 - t1) * 1e3, 3),
+        "forward_mrays": round(fwd_rays / (t1 - t0) / 1e6, 2), "adjoint_mrays": round(adj_rays / (t2 - t1) / 1e6, 2),
+        "loss_first": res.loss_first, "loss_last": res.loss_last,
+        "albedo_rmse_first": round(res.albedo_rmse_first, 5), "albedo_rmse_last": round(res.albedo_rmse_last, 5),
+        "config": {"workload": f"cornell_refined {spec.triangles} tris, {n} albedo parameters x 3, "
+                               f"{args.adj_size}x{args.adj_size}, {args.adj_spp} spp, depth 6, roulette off",
+                   "gradient": "d L2(image, target) / d main_color, all materials"},
+    }
+    log(f"adjoint: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +231,9 @@ def main():
     ap.add_argument("--tune-passes", type=int, default=6, help="passes per autotuner candidate")
     ap.add_argument("--knobs", default="", help="skip the autotuner: lds_stack_depth,sort_rays,sort_bits,path_tail")
     ap.add_argument("--only-tuned", action="store_true", help="time only the tuned (or given) knobs")
+    ap.add_argument("--adj-steps", type=int, default=200, help="Adam steps of the differentiation workload (0: skip)")
+    ap.add_argument("--adj-size", type=int, default=512)
+    ap.add_argument("--adj-spp", type=int, default=32)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -193,6 +241,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.adj_steps > 0:
+        import torch  # noqa: F401  (loaded before the HIP libraries: one HIP runtime in the process)
     from raydemo_amd import autotune, capi as A, host_lib, scenes
 
     lib = host_lib()
@@ -241,6 +291,8 @@ def main():
                              f"128x128 buckets on {cpu_run['cores']} threads, {cpu_run['seconds']:.1f}s"}
             log(f"cpu baseline: {cpu}")
 
+    adj = run_adjoint(args, lib, A, scenes) if args.adj_steps > 0 else None
+
     line = {
         "metric": "Mrays/s at 1080p depth-8 (triangle soup, primary+secondary+shadow rays)",
         "value": round(best["mrays"], 2),
@@ -268,6 +320,7 @@ def main():
         "kernel_ms": {k: round(v, 3) for k, v in prof.items() if k.endswith("_ms")},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "adjoint": adj,
     }
     print(json.dumps(line), flush=True)
 

@@ -109,6 +109,33 @@ int rh_adjoint_pass(rh_context *ctx, const rh_pass_desc *pass, const float *d_ad
 int rh_set_material_colors(rh_context *ctx, uint32_t first, uint32_t n, const float *rgb);
 int rh_read_accum(rh_context *ctx, float *dst, size_t count);
 
+/* ---- batching (no counterpart in the reference: the many-scene extension) ----
+ * A batch renders many scenes of different sizes, depths and sample counts in
+ * one set of persistent kernels per pass; every scene's image is identical
+ * to what rh_render_pass renders for it alone, pass for pass.
+ * rh_batch_add_scene copies a scene (its arrays, and the camera, full-frame
+ * rect and hash-grid box of desc) and returns its index (>= 0) or a negative
+ * status; desc->iteration is ignored.  Pass k (1-based) renders iteration k of
+ * every scene with spp >= k; rh_batch_passes gives the largest spp.  The
+ * caller uploads the Halton table of iteration k with rh_batch_set_halton
+ * before pass k when k == 1 or k % 256 == 0, as RenderScene refreshes it.
+ * A pass's launches are captured once as a hipGraph and replayed ("use_graph",
+ * default 1).  Knobs: "lds_stack_depth", "path_tail" (as rh_set_option),
+ * "use_graph", "blocks_per_cu" (persistent grid size).  Stats: rays counted
+ * as by rh_get_ray_counts (primary, secondary, shadow). */
+typedef struct rh_batch rh_batch;
+int rh_batch_create(int device, rh_batch **out);
+void rh_batch_destroy(rh_batch *batch);
+int rh_batch_add_scene(rh_batch *batch, const rh_scene_view *scene, const rh_pass_desc *desc, int w, int h, int spp);
+int rh_batch_set_halton(rh_batch *batch, const float *seq, size_t count);
+int rh_batch_passes(const rh_batch *batch);
+int rh_batch_render_pass(rh_batch *batch);
+int rh_batch_synchronize(rh_batch *batch);
+int rh_batch_read_pixels(rh_batch *batch, int scene, float *rgba, size_t count);
+int rh_batch_reset_stats(rh_batch *batch);
+int rh_batch_get_ray_counts(rh_batch *batch, rh_ray_counts *out);
+int rh_batch_set_option(rh_batch *batch, const char *name, int value);
+
 /* ---- device-resident entry point for benchmarking the trace kernel ----
  * rays/hits are device pointers in the split layout (rh_ray / rh_hit). */
 int rh_trace_device(rh_context *ctx, const rh_ray *d_rays, int n, int any_hit, rh_hit *d_hits, void *stream);

@@ -150,6 +150,26 @@ int rhx_adjoint_scene(rhx_renderer *r, rhx_scene *s, rhx_region *reg, const floa
 int rhx_set_material_colors(rhx_renderer *r, rhx_scene *s, uint32_t first, uint32_t n, const float *rgb);
 int rhx_read_accum(rhx_renderer *r, float *dst, size_t count);
 
+/* Batching (HIP backend only; no counterpart in the reference, see
+ * rh_batch_* in rh_hip.h): Ray::Hip::BatchRenderer.  rhx_batch_add copies a
+ * scene with its current camera, to be rendered at w x h for spp passes, and
+ * returns its index (< 0 on error); scenes are added before the first pass.
+ * rhx_batch_render_pass renders one progressive pass of every scene with
+ * samples left; rhx_batch_get_pixels reads scene i's tonemapped image (RGBA
+ * w*h), equal to what rhx_render_scene gives for that scene after as many
+ * passes. */
+typedef struct rhx_batch rhx_batch;
+int rhx_batch_create(int device_index, rhx_batch **out);
+void rhx_batch_destroy(rhx_batch *b);
+int rhx_batch_add(rhx_batch *b, rhx_scene *s, int w, int h, int spp);
+int rhx_batch_passes(const rhx_batch *b);
+int rhx_batch_render_pass(rhx_batch *b);
+int rhx_batch_get_pixels(rhx_batch *b, int scene, float *out, size_t count);
+int rhx_batch_set_option(rhx_batch *b, const char *name, int value);
+int rhx_batch_get_ray_counts(rhx_batch *b, rh_ray_counts *out);
+int rhx_batch_reset_stats(rhx_batch *b);
+int rhx_batch_synchronize(rhx_batch *b);
+
 void rhx_set_environment(rhx_scene *s, const float env_col[3], float env_clamp, uint32_t env_map);
 uint32_t rhx_add_texture(rhx_scene *s, const uint8_t *rgba, int w, int h, int is_srgb, int generate_mipmaps);
 uint32_t rhx_add_material(rhx_scene *s, const rhx_mat_desc *m);

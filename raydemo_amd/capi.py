@@ -220,7 +220,20 @@ class RayLib:
             "rhx_set_material_colors": (c_int, [P, P, c_uint32, c_uint32, P]),
             "rhx_read_accum": (c_int, [P, P, c_size_t]),
         }
-        self.has_adjoint = all(hasattr(L, n) for n in optional)
+        optional.update({
+            "rhx_batch_create": (c_int, [c_int, POINTER(P)]),
+            "rhx_batch_destroy": (None, [P]),
+            "rhx_batch_add": (c_int, [P, P, c_int, c_int, c_int]),
+            "rhx_batch_passes": (c_int, [P]),
+            "rhx_batch_render_pass": (c_int, [P]),
+            "rhx_batch_get_pixels": (c_int, [P, c_int, P, c_size_t]),
+            "rhx_batch_set_option": (c_int, [P, C.c_char_p, c_int]),
+            "rhx_batch_get_ray_counts": (c_int, [P, POINTER(RayCounts)]),
+            "rhx_batch_reset_stats": (c_int, [P]),
+            "rhx_batch_synchronize": (c_int, [P]),
+        })
+        self.has_adjoint = all(hasattr(L, n) for n in ("rhx_adjoint_scene", "rhx_set_material_colors", "rhx_read_accum"))
+        self.has_batch = hasattr(L, "rhx_batch_create")
         for name, (res, args) in optional.items():
             if hasattr(L, name):
                 f = getattr(L, name)
@@ -336,6 +349,68 @@ class Renderer:
         out = np.zeros((h, w, 4), np.float32)
         self.lib.check(self.lib.lib.rhx_read_accum(self.h, _ptr(out), out.size), "rhx_read_accum")
         return out
+
+
+class Batch:
+    """Many scenes per pass (HIP backend): Ray::Hip::BatchRenderer."""
+
+    def __init__(self, lib: RayLib, device_index: int = -1):
+        if not lib.has_batch:
+            raise RuntimeError(f"{lib.path} has no batch entry points (HIP backend required)")
+        self.lib = lib
+        self.h = c_void_p()
+        lib.check(lib.lib.rhx_batch_create(device_index, C.byref(self.h)), "rhx_batch_create")
+        self.sizes = []
+
+    def close(self):
+        if self.h:
+            self.lib.lib.rhx_batch_destroy(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, scene: "Scene", w: int, h: int, spp: int) -> int:
+        i = self.lib.lib.rhx_batch_add(self.h, scene.h, w, h, spp)
+        if i < 0:
+            raise RuntimeError(f"rhx_batch_add: {self.lib.error()}")
+        self.sizes.append((w, h))
+        return i
+
+    @property
+    def passes(self) -> int:
+        return self.lib.lib.rhx_batch_passes(self.h)
+
+    def render_pass(self):
+        self.lib.check(self.lib.lib.rhx_batch_render_pass(self.h), "rhx_batch_render_pass")
+
+    def render_all(self):
+        for _ in range(self.passes):
+            self.render_pass()
+
+    def pixels(self, i: int) -> np.ndarray:
+        w, h = self.sizes[i]
+        out = np.zeros((h, w, 4), np.float32)
+        self.lib.check(self.lib.lib.rhx_batch_get_pixels(self.h, i, _ptr(out), out.size), "rhx_batch_get_pixels")
+        return out
+
+    def set_option(self, name: str, value: int):
+        self.lib.check(self.lib.lib.rhx_batch_set_option(self.h, name.encode(), int(value)),
+                       f"rhx_batch_set_option({name})")
+
+    def ray_counts(self) -> dict:
+        rc = RayCounts()
+        self.lib.check(self.lib.lib.rhx_batch_get_ray_counts(self.h, C.byref(rc)), "rhx_batch_get_ray_counts")
+        return {f: getattr(rc, f) for f, _ in RayCounts._fields_}
+
+    def reset_stats(self):
+        self.lib.check(self.lib.lib.rhx_batch_reset_stats(self.h), "rhx_batch_reset_stats")
+
+    def synchronize(self):
+        self.lib.check(self.lib.lib.rhx_batch_synchronize(self.h), "rhx_batch_synchronize")
 
 
 class Region:

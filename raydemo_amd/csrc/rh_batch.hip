@@ -1,0 +1,695 @@
+// Many scenes in one wavefront: the batch pipeline of the HIP backend.
+//
+// Every scene of a batch owns a contiguous segment of the shared queues
+// (rays, hits, framebuffers, next-event-estimation slots) and its own queue
+// counters.  Each stage is one persistent kernel over the whole batch: its
+// blocks take (scene, 256-ray tile) work items from an atomic counter, the
+// tiles of scene s following those of scene s-1, and run the same stage body
+// as the single-scene pipeline (rh_wavefront.h) with the scene's arguments, so
+// every pixel of every scene is computed exactly as a lone RenderScene pass
+// computes it.  Small and large scenes share the GPU instead of leaving it
+// idle in turn.  Per-pass state (iteration, active scenes, counters) lives on
+// the device and is set by the first kernel of the pass, so the launch
+// sequence of a pass is fixed and is captured once as a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rh_runtime.h"
+
+using rh_rt::arg_error;
+using rh_rt::DevBuf;
+using rh_rt::fail;
+using rh_rt::free_ptr;
+
+namespace {
+
+constexpr int kMaxBatchScenes = 256;
+// tile counters of one pass: generate, five per staged bounce, tail, mix
+constexpr int kMaxStages = 5 * (RH_MAX_BOUNCES + 1) + 4;
+
+// one scene of the batch (device table)
+struct BatchEntry {
+    Scene sc;
+    PassConst pc;      // camera and frame; iteration and mix_k are set per pass on the device
+    rh_pass_info pi;   // pass settings (bounce set per stage)
+    uint32_t offset;   // first pixel / ray of the scene in the batch arrays
+    uint32_t pixels;   // w * h
+    uint32_t nee_slots;  // light samples per shaded ray (0: visibility inline)
+    uint32_t nee_offset; // first slot of the scene in the NEE arrays
+    int spp;             // passes the scene takes part in
+    int depth;           // max_total_depth (clamped)
+};
+
+struct BatchDev {
+    const BatchEntry *__restrict__ table;
+    int n_scenes;
+    int *pass;          // passes started so far (device)
+    int *counts;        // [(RH_MAX_BOUNCES + 2) * n_scenes]: rays entering stage b (b = 0: primary)
+    int *shadow_counts; // [(RH_MAX_BOUNCES + 2) * n_scenes]
+    int *work;          // [kMaxStages] tile counters
+    rh_ray *rays[2];
+    rh_ray_aux *aux[2];
+    rh_hit *hits;
+    float4 *temp, *clean, *final_;
+    float4 *nee_ray, *nee_term;
+    int2 *nee_aux;
+    uint8_t *nee_vis;
+    float4 *pend_c, *pend_alb;
+    int *pend_count;
+    uint32_t *shadow_queue;
+    uint32_t *spill;
+    uint32_t spill_stride;
+    uint32_t lds_depth;
+    const float *halton;
+    unsigned long long *stats; // [0] primary [1] secondary [2] shadow [4] overflow
+};
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Table entry s (s uniform) read through the constant address space: the
+// loads become scalar and the scene's pointers stay in SGPRs, as the kernel
+// arguments of the single-scene kernels do.
+typedef const __attribute__((address_space(4))) BatchEntry ConstEntry;
+__device__ __forceinline__ BatchEntry entry(const BatchDev &b, int s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((ConstEntry *)b.table)[s];
+#else
+    return b.table[s]; // the host pass only parses device functions
+#endif
+}
+
+// Runs body(scene, tile base) for every 256-ray tile of every scene's queue,
+// count_s = counts[s] rays (0 for scenes that skip this stage).  Persistent:
+// blocks take tiles from *work until none is left.
+template <typename Active, typename Body>
+__device__ __forceinline__ void for_each_tile(const BatchDev &b, const int *counts, int *work, Active active,
+                                              Body body) {
+    __shared__ int s_pre[kMaxBatchScenes + 1];
+    __shared__ int s_t;
+    const int S = b.n_scenes;
+    // tiles per scene, then an exclusive scan (S <= 256: one scene per thread)
+    int tiles = 0;
+    if ((int)threadIdx.x < S) {
+        const int s = threadIdx.x;
+        if (active(s)) tiles = (counts[s] + kBlock - 1) / kBlock;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int incl = tiles;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off);
+        if (lane >= off) incl += v;
+    }
+    if (lane == 63) s_pre[wave] = incl; // wave totals (temporary)
+    __syncthreads();
+    int wave_base = 0;
+    for (int w = 0; w < wave; w++) wave_base += s_pre[w];
+    __syncthreads();
+    if ((int)threadIdx.x < S) s_pre[threadIdx.x] = wave_base + incl - tiles;
+    if ((int)threadIdx.x == S - 1) s_pre[S] = wave_base + incl;
+    __syncthreads();
+    const int total = s_pre[S];
+    for (;;) {
+        if (threadIdx.x == 0) s_t = atomicAdd(work, 1);
+        __syncthreads();
+        const int t = uniform(s_t);
+        __syncthreads();
+        if (t >= total) break;
+        int lo = 0, hi = S; // s_pre[lo] <= t < s_pre[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_pre[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const int s = uniform(lo);
+        body(s, (uint32_t)(t - s_pre[s]) * kBlock);
+        __syncthreads(); // the body's LDS is reused by the next tile
+    }
+}
+
+// First kernel of a pass: advances the pass counter, sets every scene's
+// iteration and mixing weight, clears the queue and tile counters, and
+// activates the primary queue of scenes still rendering.
+__global__ void k_b_begin_pass(BatchDev b, BatchEntry *table, int n_stages) {
+    const int pass = *b.pass + 1; // every thread reads the old value
+    __syncthreads();
+    const int S = b.n_scenes;
+    const int nc = (RH_MAX_BOUNCES + 2) * S;
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+        b.counts[k] = (k < S && pass <= table[k].spp) ? (int)table[k].pixels : 0;
+        b.shadow_counts[k] = 0;
+    }
+    for (int k = threadIdx.x; k < n_stages; k += blockDim.x) b.work[k] = 0;
+    for (int s = threadIdx.x; s < S; s += blockDim.x) {
+        table[s].pc.iteration = pass;
+        table[s].pi.iteration = pass;
+        table[s].pc.mix_k = 1.0f / pass;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *b.pass = pass;
+}
+
+__global__ void __launch_bounds__(kBlock) k_b_gen(BatchDev b, int work) {
+    for_each_tile(b, b.counts, b.work + work, [](int) { return true; }, [&](int s, uint32_t base) {
+        const BatchEntry e = entry(b, s);
+        const int n = b.counts[s];
+        const int i = (int)(base + threadIdx.x);
+        if (i >= n) return;
+        const int w = e.pc.w;
+        if (i == 0) atomicAdd(b.stats + 0, (unsigned long long)n);
+        store_ray(b.rays[0] + e.offset, b.aux[0] + e.offset, (uint32_t)i, gen_primary_ray(e.pc, b.halton, i % w, i / w));
+    });
+}
+
+// stage b: 0 = primary, 1 + k = bounce k
+__device__ __forceinline__ bool stage_active(const BatchEntry &e, int stage) {
+    if (stage == 0) return true;
+    if (e.pi.settings.flags & RH_SKIP_INDIRECT_LIGHT) return false;
+    return stage - 1 < e.depth;
+}
+
+__global__ void __launch_bounds__(kBlock) k_b_trace(BatchDev b, int stage, int q, int work) {
+    extern __shared__ uint32_t lds_stack[];
+    const int *counts = b.counts + stage * b.n_scenes;
+    for_each_tile(b, counts, b.work + work, [&](int s) { return stage_active(b.table[s], stage); },
+                  [&](int s, uint32_t base) {
+                      const BatchEntry e = entry(b, s);
+                      TraceArgs a;
+                      a.sc = e.sc;
+                      a.rays = b.rays[q] + e.offset;
+                      a.count = counts + s;
+                      a.n = 0;
+                      a.tmax = nullptr;
+                      a.hits = b.hits + e.offset;
+                      a.spill = b.spill + e.offset;
+                      a.spill_stride = b.spill_stride;
+                      a.lds_depth = b.lds_depth;
+                      a.ray_counter = b.stats + (stage == 0 ? 0 : 1);
+                      a.overflow = b.stats + 4;
+                      a.visits = nullptr;
+                      if (stage == 0) a.ray_counter = nullptr; // primary rays are counted at generation
+                      trace_body<false, false>(a, base, lds_stack);
+                  });
+}
+
+__device__ __forceinline__ ShadeArgs batch_shade_args(const BatchDev &b, const BatchEntry &e, int s, int stage, int q,
+                                                      bool defer) {
+    ShadeArgs a = {};
+    a.sc = e.sc;
+    a.rays = b.rays[q] + e.offset;
+    a.aux = b.aux[q] + e.offset;
+    a.hits = b.hits + e.offset;
+    a.count = b.counts + stage * b.n_scenes + s;
+    a.pi = e.pi;
+    a.pi.bounce = stage == 0 ? 2 : stage + 2;
+    a.w = e.pc.w;
+    a.halton = b.halton;
+    a.temp = b.temp + e.offset;
+    a.primary = stage == 0 ? 1 : 0;
+    a.out_rays = b.rays[q ^ 1] + e.offset;
+    a.out_aux = b.aux[q ^ 1] + e.offset;
+    a.out_count = b.counts + (stage + 1) * b.n_scenes + s;
+    a.spill = b.spill + e.offset;
+    a.spill_stride = b.spill_stride;
+    a.lds_depth = b.lds_depth;
+    a.shadow_counter = b.stats + 2;
+    a.overflow = b.stats + 4;
+    a.ray_counter = b.stats + 1;
+    if (defer && e.nee_slots) {
+        a.nee_ray = b.nee_ray + e.nee_offset;
+        a.nee_aux = b.nee_aux + e.nee_offset;
+        a.nee_term = b.nee_term + e.nee_offset;
+        a.nee_slots = e.nee_slots;
+        a.pend_c = b.pend_c + e.offset;
+        a.pend_alb = b.pend_alb + e.offset;
+        a.pend_count = b.pend_count + e.offset;
+        a.shadow_queue = b.shadow_queue + e.nee_offset;
+        a.shadow_count = b.shadow_counts + stage * b.n_scenes + s;
+    }
+    return a;
+}
+
+// Shading with next-event estimation deferred to the shadow stage for scenes
+// with 1..8 lights; scenes with more trace visibility inline (Defer = false).
+template <bool Defer> __global__ void __launch_bounds__(kBlock) k_b_shade(BatchDev b, int stage, int q, int work) {
+    extern __shared__ uint32_t lds_stack[];
+    __shared__ int s_app[kMaxDeferredLights * (kBlock / 64) + 1];
+    const int *counts = b.counts + stage * b.n_scenes;
+    for_each_tile(b, counts, b.work + work,
+                  [&](int s) { return stage_active(b.table[s], stage) && ((b.table[s].nee_slots != 0) == Defer); },
+                  [&](int s, uint32_t base) {
+                      const BatchEntry e = entry(b, s);
+                      const ShadeArgs a = batch_shade_args(b, e, s, stage, q, Defer);
+                      shade_body<Defer>(a, base, lds_stack, s_app);
+                  });
+}
+
+__global__ void __launch_bounds__(kBlock) k_b_shadow(BatchDev b, int stage, int work) {
+    extern __shared__ uint32_t lds_stack[];
+    const int *counts = b.shadow_counts + stage * b.n_scenes;
+    for_each_tile(b, counts, b.work + work, [&](int s) { return b.table[s].nee_slots != 0; },
+                  [&](int s, uint32_t base) {
+                      const BatchEntry e = entry(b, s);
+                      ShadowArgs a;
+                      a.sc = e.sc;
+                      a.queue = b.shadow_queue + e.nee_offset;
+                      a.count = counts + s;
+                      a.nee_ray = b.nee_ray + e.nee_offset;
+                      a.nee_aux = b.nee_aux + e.nee_offset;
+                      a.nee_vis = b.nee_vis + e.nee_offset;
+                      a.halton = b.halton;
+                      a.spill = b.spill + e.nee_offset;
+                      a.spill_stride = b.spill_stride;
+                      a.lds_depth = b.lds_depth;
+                      a.shadow_counter = b.stats + 2;
+                      a.overflow = b.stats + 4;
+                      shadow_body(a, base, lds_stack);
+                  });
+}
+
+__global__ void __launch_bounds__(kBlock) k_b_combine(BatchDev b, int stage, int work) {
+    const int *counts = b.counts + stage * b.n_scenes;
+    for_each_tile(b, counts, b.work + work,
+                  [&](int s) { return stage_active(b.table[s], stage) && b.table[s].nee_slots != 0; },
+                  [&](int s, uint32_t base) {
+                      const BatchEntry e = entry(b, s);
+                      CombineArgs a;
+                      a.count = counts + s;
+                      a.n = 0;
+                      a.slots = e.nee_slots;
+                      a.pend_count = b.pend_count + e.offset;
+                      a.pend_c = b.pend_c + e.offset;
+                      a.pend_alb = b.pend_alb + e.offset;
+                      a.nee_term = b.nee_term + e.nee_offset;
+                      a.nee_vis = b.nee_vis + e.nee_offset;
+                      a.temp = b.temp + e.offset;
+                      // not read while temp is set; a real address keeps the
+                      // per-ray form's address arithmetic off a null pointer
+                      a.out_rgba = a.temp;
+                      combine_body(a, base);
+                  });
+}
+
+// the rays entering `stage` follow their paths to each scene's depth in one lane
+__global__ void __launch_bounds__(kBlock) k_b_tail(BatchDev b, int stage, int q, int work) {
+    extern __shared__ uint32_t lds_stack[];
+    const int *counts = b.counts + stage * b.n_scenes;
+    for_each_tile(b, counts, b.work + work, [&](int s) { return stage_active(b.table[s], stage); },
+                  [&](int s, uint32_t base) {
+                      const BatchEntry e = entry(b, s);
+                      ShadeArgs a = batch_shade_args(b, e, s, stage, q, false);
+                      path_tail_body(a, stage - 1, e.depth, base, lds_stack);
+                  });
+}
+
+__global__ void __launch_bounds__(kBlock) k_b_mix(BatchDev b, int work) {
+    for_each_tile(b, b.counts, b.work + work, [](int) { return true; }, [&](int s, uint32_t base) {
+        const BatchEntry e = entry(b, s);
+        const int i = (int)(base + threadIdx.x);
+        if (i >= (int)e.pixels || b.counts[s] == 0) return;
+        mix_tonemap_pixel(b.temp + e.offset, b.clean + e.offset, b.final_ + e.offset, (size_t)i, e.pc.mix_k,
+                          e.pc.cam.dtype, e.pc.cam.gamma, e.pc.cam.pass_settings.flags);
+    });
+}
+
+__global__ void k_b_fill(float4 *buf, size_t n, float4 v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) buf[i] = v;
+}
+
+} // namespace
+
+struct rh_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 0;
+
+    std::vector<std::vector<DevBuf>> scene_bufs;
+    std::vector<BatchEntry> entries; // host copy of the table
+    bool built = false;
+    int max_depth = 0;
+    int max_spp = 0;
+    size_t total_pixels = 0, total_slots = 0;
+
+    BatchEntry *table = nullptr;
+    int *pass = nullptr;
+    int *counts = nullptr, *shadow_counts = nullptr, *work = nullptr;
+    rh_ray *rays[2] = {nullptr, nullptr};
+    rh_ray_aux *aux[2] = {nullptr, nullptr};
+    rh_hit *hits = nullptr;
+    float4 *temp = nullptr, *clean = nullptr, *final_ = nullptr;
+    float4 *nee_ray = nullptr, *nee_term = nullptr;
+    int2 *nee_aux = nullptr;
+    uint8_t *nee_vis = nullptr;
+    float4 *pend_c = nullptr, *pend_alb = nullptr;
+    int *pend_count = nullptr;
+    uint32_t *shadow_queue = nullptr;
+    uint32_t *spill = nullptr;
+    size_t spill_cap = 0;
+    float *halton = nullptr;
+    unsigned long long *stats = nullptr;
+    int passes_done = 0;
+
+    // options
+    int lds_depth = 16;
+    int path_tail = 2;
+    int use_graph = 1;
+    int blocks_per_cu = 8;
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_key[3] = {-1, -1, -1};
+};
+
+namespace {
+
+void drop_graph(rh_batch *b) {
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+}
+
+void free_arrays(rh_batch *b) {
+    void *ps[] = {b->table, b->pass, b->counts, b->shadow_counts, b->work, b->rays[0], b->rays[1], b->aux[0],
+                  b->aux[1], b->hits, b->temp, b->clean, b->final_, b->nee_ray, b->nee_term, b->nee_aux, b->nee_vis,
+                  b->pend_c, b->pend_alb, b->pend_count, b->shadow_queue, b->spill};
+    for (void *p : ps) free_ptr(p);
+    b->table = nullptr;
+    b->pass = b->counts = b->shadow_counts = b->work = nullptr;
+    b->rays[0] = b->rays[1] = nullptr;
+    b->aux[0] = b->aux[1] = nullptr;
+    b->hits = nullptr;
+    b->temp = b->clean = b->final_ = nullptr;
+    b->nee_ray = b->nee_term = nullptr;
+    b->nee_aux = nullptr;
+    b->nee_vis = nullptr;
+    b->pend_c = b->pend_alb = nullptr;
+    b->pend_count = nullptr;
+    b->shadow_queue = nullptr;
+    b->spill = nullptr;
+    b->built = false;
+}
+
+template <typename T> int dev_alloc(T **p, size_t n) {
+    RH_CHECK(hipMalloc(p, std::max<size_t>(n, 1) * sizeof(T)));
+    return 0;
+}
+
+// Allocates the shared queues for the scenes added so far and uploads the table.
+int build(rh_batch *b) {
+    if (b->built) return 0;
+    if (b->entries.empty()) return arg_error("batch has no scenes");
+    free_arrays(b);
+    drop_graph(b);
+    const size_t P = b->total_pixels, NS = b->total_slots, S = b->entries.size();
+    int rc = 0;
+    rc |= dev_alloc(&b->table, S);
+    rc |= dev_alloc(&b->pass, 1);
+    rc |= dev_alloc(&b->counts, (RH_MAX_BOUNCES + 2) * S);
+    rc |= dev_alloc(&b->shadow_counts, (RH_MAX_BOUNCES + 2) * S);
+    rc |= dev_alloc(&b->work, (size_t)kMaxStages);
+    for (int k = 0; k < 2; k++) {
+        rc |= dev_alloc(&b->rays[k], P);
+        rc |= dev_alloc(&b->aux[k], P);
+    }
+    rc |= dev_alloc(&b->hits, P);
+    rc |= dev_alloc(&b->temp, P);
+    rc |= dev_alloc(&b->clean, P);
+    rc |= dev_alloc(&b->final_, P);
+    rc |= dev_alloc(&b->nee_ray, NS);
+    rc |= dev_alloc(&b->nee_term, NS);
+    rc |= dev_alloc(&b->nee_aux, NS);
+    rc |= dev_alloc(&b->nee_vis, NS);
+    rc |= dev_alloc(&b->shadow_queue, NS);
+    rc |= dev_alloc(&b->pend_c, P);
+    rc |= dev_alloc(&b->pend_alb, P);
+    rc |= dev_alloc(&b->pend_count, P);
+    b->spill_cap = std::max(P, NS);
+    rc |= dev_alloc(&b->spill, (size_t)kSpillDepth * b->spill_cap);
+    if (rc) return rc;
+    RH_CHECK(hipMemcpyAsync(b->table, b->entries.data(), S * sizeof(BatchEntry), hipMemcpyHostToDevice, b->stream));
+    RH_CHECK(hipMemsetAsync(b->pass, 0, sizeof(int), b->stream));
+    const size_t n4 = P;
+    hipLaunchKernelGGL(k_b_fill, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, b->stream, b->clean, n4,
+                       make_float4(0, 0, 0, 0));
+    hipLaunchKernelGGL(k_b_fill, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, b->stream, b->final_, n4,
+                       make_float4(0, 0, 0, 0));
+    RH_CHECK(hipGetLastError());
+    RH_CHECK(hipStreamSynchronize(b->stream));
+    b->passes_done = 0;
+    b->built = true;
+    return 0;
+}
+
+BatchDev dev_view(rh_batch *b) {
+    BatchDev d;
+    memset(&d, 0, sizeof(d));
+    d.table = b->table;
+    d.n_scenes = (int)b->entries.size();
+    d.pass = b->pass;
+    d.counts = b->counts;
+    d.shadow_counts = b->shadow_counts;
+    d.work = b->work;
+    d.rays[0] = b->rays[0];
+    d.rays[1] = b->rays[1];
+    d.aux[0] = b->aux[0];
+    d.aux[1] = b->aux[1];
+    d.hits = b->hits;
+    d.temp = b->temp;
+    d.clean = b->clean;
+    d.final_ = b->final_;
+    d.nee_ray = b->nee_ray;
+    d.nee_term = b->nee_term;
+    d.nee_aux = b->nee_aux;
+    d.nee_vis = b->nee_vis;
+    d.pend_c = b->pend_c;
+    d.pend_alb = b->pend_alb;
+    d.pend_count = b->pend_count;
+    d.shadow_queue = b->shadow_queue;
+    d.spill = b->spill;
+    d.spill_stride = (uint32_t)b->spill_cap;
+    d.lds_depth = (uint32_t)b->lds_depth;
+    d.halton = b->halton;
+    d.stats = b->stats;
+    return d;
+}
+
+// The launch sequence of one pass (fixed for given options): begin, generate,
+// then per stage trace / shade / shadow / combine, the path tail, mix.
+int enqueue_pass(rh_batch *b) {
+    // first stage run by the tail kernel (bounce path_tail)
+    const int tail = b->path_tail < 0 ? b->max_depth + 1 : std::min(b->path_tail + 1, b->max_depth + 1);
+    const int launches = 2 + 5 * tail + (tail <= b->max_depth ? 1 : 0);
+    if (launches > kMaxStages) return arg_error("batch pass needs more stage counters than reserved");
+    const BatchDev d = dev_view(b);
+    const size_t lds = (size_t)b->lds_depth * kBlock * sizeof(uint32_t);
+    const dim3 grid((unsigned)(b->cus * b->blocks_per_cu)), blk(kBlock);
+    int work = 0;
+    hipLaunchKernelGGL(k_b_begin_pass, dim3(1), dim3(1024), 0, b->stream, d, b->table, (int)kMaxStages);
+    hipLaunchKernelGGL(k_b_gen, grid, blk, 0, b->stream, d, work++);
+    int q = 0;
+    // stage 0 is the primary hit; stage k >= 1 traces bounce k - 1.  Bounces
+    // from path_tail on run in the tail kernel.
+    for (int stage = 0; stage <= b->max_depth; stage++) {
+        if (stage >= tail) {
+            hipLaunchKernelGGL(k_b_tail, grid, blk, lds, b->stream, d, stage, q, work++);
+            break;
+        }
+        hipLaunchKernelGGL(k_b_trace, grid, blk, lds, b->stream, d, stage, q, work++);
+        hipLaunchKernelGGL(k_b_shade<true>, grid, blk, 0, b->stream, d, stage, q, work++);
+        hipLaunchKernelGGL(k_b_shade<false>, grid, blk, lds, b->stream, d, stage, q, work++);
+        hipLaunchKernelGGL(k_b_shadow, grid, blk, lds, b->stream, d, stage, work++);
+        hipLaunchKernelGGL(k_b_combine, grid, blk, 0, b->stream, d, stage, work++);
+        q ^= 1;
+    }
+    hipLaunchKernelGGL(k_b_mix, grid, blk, 0, b->stream, d, work++);
+    RH_CHECK(hipGetLastError());
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int rh_batch_create(int device, rh_batch **out) {
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        rh_rt::err() = "no HIP device available";
+        return -1002;
+    }
+    if (device < 0) RH_CHECK(hipGetDevice(&device));
+    if (device >= ndev) return arg_error("device index out of range");
+    RH_CHECK(hipSetDevice(device));
+    auto *b = new rh_batch;
+    b->device = device;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        rh_rt::err() = "hipStreamCreate failed";
+        return -1003;
+    }
+    hipDeviceProp_t prop;
+    RH_CHECK(hipGetDeviceProperties(&prop, device));
+    b->cus = prop.multiProcessorCount;
+    RH_CHECK(hipMalloc(&b->halton, RH_HALTON_COUNT * RH_HALTON_SEQ_LEN * sizeof(float)));
+    RH_CHECK(hipMalloc(&b->stats, 8 * sizeof(unsigned long long)));
+    RH_CHECK(hipMemsetAsync(b->stats, 0, 8 * sizeof(unsigned long long), b->stream));
+    RH_CHECK(hipStreamSynchronize(b->stream));
+    *out = b;
+    return 0;
+}
+
+void rh_batch_destroy(rh_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    free_arrays(b);
+    for (auto &v : b->scene_bufs)
+        for (auto &x : v) free_ptr(x.p);
+    free_ptr(b->halton);
+    free_ptr(b->stats);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int rh_batch_add_scene(rh_batch *b, const rh_scene_view *v, const rh_pass_desc *desc, int w, int h, int spp) {
+    if ((int)b->entries.size() >= kMaxBatchScenes) return arg_error("batch is full (256 scenes)");
+    if (w <= 0 || h <= 0 || w > 65535 || h > 65535) return arg_error("framebuffer size must be 1..65535 per side");
+    if (spp < 1) return arg_error("spp must be >= 1");
+    if (desc->rect[0] != 0 || desc->rect[1] != 0 || desc->rect[2] != w || desc->rect[3] != h)
+        return arg_error("batched scenes render their whole frame");
+    if (b->total_pixels + (size_t)w * h > 0x7fffffffu) return arg_error("batch exceeds 2^31 pixels");
+    RH_CHECK(hipSetDevice(b->device));
+    std::vector<DevBuf> bufs;
+    BatchEntry e;
+    memset(&e, 0, sizeof(e));
+    const int rc = build_device_scene(v, b->stream, bufs, e.sc);
+    if (rc) {
+        for (auto &x : bufs) free_ptr(x.p);
+        return rc;
+    }
+    RH_CHECK(hipStreamSynchronize(b->stream));
+    rh_pass_desc p = *desc;
+    p.iteration = 1;
+    e.pc = make_pass_const(w, h, &p);
+    e.pi.iteration = 1;
+    e.pi.bounce = 2;
+    e.pi.settings = desc->cam.pass_settings;
+    e.depth = std::min<int>(desc->cam.pass_settings.max_total_depth, RH_MAX_BOUNCES);
+    e.pi.settings.max_total_depth = (uint8_t)e.depth;
+    e.pc.cam.pass_settings.max_total_depth = (uint8_t)e.depth;
+    e.offset = (uint32_t)b->total_pixels;
+    e.pixels = (uint32_t)((size_t)w * h);
+    e.nee_slots = (v->lights_count >= 1 && v->lights_count <= kMaxDeferredLights) ? v->lights_count : 0;
+    e.nee_offset = (uint32_t)b->total_slots;
+    e.spp = spp;
+    b->total_pixels += e.pixels;
+    b->total_slots += (size_t)e.pixels * e.nee_slots;
+    b->max_depth = std::max(b->max_depth, e.depth);
+    b->max_spp = std::max(b->max_spp, spp);
+    b->entries.push_back(e);
+    b->scene_bufs.push_back(std::move(bufs));
+    b->built = false;
+    return (int)b->entries.size() - 1;
+}
+
+int rh_batch_set_halton(rh_batch *b, const float *seq, size_t count) {
+    if (count < (size_t)RH_HALTON_COUNT * RH_HALTON_SEQ_LEN) return arg_error("Halton table needs 256 x 24 floats");
+    RH_CHECK(hipMemcpyAsync(b->halton, seq, RH_HALTON_COUNT * RH_HALTON_SEQ_LEN * sizeof(float),
+                            hipMemcpyHostToDevice, b->stream));
+    RH_CHECK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int rh_batch_passes(const rh_batch *b) { return b->max_spp; }
+
+int rh_batch_render_pass(rh_batch *b) {
+    RH_CHECK(hipSetDevice(b->device));
+    int rc = build(b);
+    if (rc) return rc;
+    if (b->passes_done >= b->max_spp) return arg_error("every scene of the batch has rendered its passes");
+    const int key[3] = {b->lds_depth, b->path_tail, b->blocks_per_cu};
+    if (b->use_graph) {
+        if (!b->graph_exec || memcmp(key, b->graph_key, sizeof(key)) != 0) {
+            drop_graph(b);
+            RH_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_pass(b);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(e, "hipStreamEndCapture");
+            b->graph = g;
+            RH_CHECK(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+            memcpy(b->graph_key, key, sizeof(key));
+        }
+        RH_CHECK(hipGraphLaunch(b->graph_exec, b->stream));
+    } else {
+        rc = enqueue_pass(b);
+        if (rc) return rc;
+    }
+    b->passes_done++;
+    return 0;
+}
+
+int rh_batch_synchronize(rh_batch *b) {
+    RH_CHECK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int rh_batch_read_pixels(rh_batch *b, int scene, float *rgba, size_t count) {
+    if (scene < 0 || scene >= (int)b->entries.size()) return arg_error("scene index out of range");
+    if (!b->built) return arg_error("batch has not rendered");
+    const BatchEntry &e = b->entries[scene];
+    if (count < (size_t)e.pixels * 4) return arg_error("output buffer too small");
+    RH_CHECK(hipMemcpyAsync(rgba, b->final_ + e.offset, (size_t)e.pixels * sizeof(float4), hipMemcpyDefault,
+                            b->stream));
+    RH_CHECK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int rh_batch_reset_stats(rh_batch *b) {
+    RH_CHECK(hipMemsetAsync(b->stats, 0, 8 * sizeof(unsigned long long), b->stream));
+    RH_CHECK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int rh_batch_get_ray_counts(rh_batch *b, rh_ray_counts *out) {
+    unsigned long long v[8];
+    RH_CHECK(hipMemcpyAsync(v, b->stats, sizeof(v), hipMemcpyDeviceToHost, b->stream));
+    RH_CHECK(hipStreamSynchronize(b->stream));
+    memset(out, 0, sizeof(*out));
+    out->primary = v[0];
+    out->secondary = v[1];
+    out->shadow = v[2];
+    if (v[4]) rh_rt::err() = "traversal stack overflow: " + std::to_string(v[4]) + " pushes dropped";
+    return 0;
+}
+
+int rh_batch_set_option(rh_batch *b, const char *name, int value) {
+    const std::string n = name;
+    if (n == "lds_stack_depth") {
+        if (value < 4 || value > 64) return arg_error("lds_stack_depth must be 4..64");
+        b->lds_depth = value;
+    } else if (n == "path_tail") {
+        if (value < -1 || value > 255) return arg_error("path_tail must be -1..255");
+        b->path_tail = value;
+    } else if (n == "use_graph") {
+        b->use_graph = value != 0;
+    } else if (n == "blocks_per_cu") {
+        if (value < 1 || value > 64) return arg_error("blocks_per_cu must be 1..64");
+        b->blocks_per_cu = value;
+    } else {
+        return arg_error("unknown option");
+    }
+    return 0;
+}
+
+} // extern "C"

@@ -28,6 +28,9 @@ struct rhx_renderer {
 struct rhx_scene {
     std::shared_ptr<Ray::Hip::Scene> s;
 };
+struct rhx_batch {
+    std::unique_ptr<Ray::Hip::BatchRenderer> b;
+};
 struct rhx_region {
     Ray::RegionContext ctx;
     explicit rhx_region(const Ray::rect_t &rc) : ctx(rc) {}
@@ -318,5 +321,73 @@ int rhx_scene_view(rhx_scene *s, rh_scene_view *out) {
     *out = s->s->View();
     return 0;
 }
+
+
+int rhx_batch_create(int device_index, rhx_batch **out) {
+    return guarded([&] {
+        *out = new rhx_batch{std::unique_ptr<Ray::Hip::BatchRenderer>(new Ray::Hip::BatchRenderer(device_index))};
+        return 0;
+    });
+}
+
+void rhx_batch_destroy(rhx_batch *b) { delete b; }
+
+int rhx_batch_add(rhx_batch *b, rhx_scene *s, int w, int h, int spp) {
+    return guarded([&] {
+        const int i = b->b->AddScene(s->s, w, h, spp);
+        if (i < 0) g_err = b->b->last_error;
+        return i;
+    });
+}
+
+int rhx_batch_passes(const rhx_batch *b) { return b->b->passes(); }
+
+int rhx_batch_render_pass(rhx_batch *b) {
+    return guarded([&] {
+        if (!b->b->RenderPass()) {
+            g_err = b->b->last_error;
+            return -1;
+        }
+        return 0;
+    });
+}
+
+int rhx_batch_get_pixels(rhx_batch *b, int scene, float *out, size_t count) {
+    return guarded([&] {
+        if (scene < 0 || scene >= b->b->scenes()) {
+            g_err = "scene index out of range";
+            return -1;
+        }
+        const auto sz = b->b->size(scene);
+        const size_t n = (size_t)sz.first * sz.second * 4;
+        if (count < n) {
+            g_err = "output buffer too small";
+            return -1;
+        }
+        const Ray::pixel_color_t *p = b->b->get_pixels_ref(scene);
+        if (!p) {
+            g_err = b->b->last_error;
+            return -1;
+        }
+        memcpy(out, p, n * sizeof(float));
+        return 0;
+    });
+}
+
+int rhx_batch_set_option(rhx_batch *b, const char *name, int value) {
+    const int rc = rh_batch_set_option(b->b->handle(), name, value);
+    if (rc != 0) g_err = rh_last_error();
+    return rc;
+}
+
+int rhx_batch_get_ray_counts(rhx_batch *b, rh_ray_counts *out) {
+    const int rc = rh_batch_get_ray_counts(b->b->handle(), out);
+    if (rc != 0) g_err = rh_last_error();
+    return rc;
+}
+
+int rhx_batch_reset_stats(rhx_batch *b) { return rh_batch_reset_stats(b->b->handle()); }
+
+int rhx_batch_synchronize(rhx_batch *b) { return rh_batch_synchronize(b->b->handle()); }
 
 } // extern "C"

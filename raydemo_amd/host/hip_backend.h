@@ -86,6 +86,8 @@ public:
     // Brings the lazily rebuilt trees up to date and describes the arrays.
     void Finalize();
     rh_scene_view View();
+    // The current camera and the sort hash grid (macro-tree root box) of a pass.
+    void PassCamera(rh_pass_desc &pass);
     uint64_t version() const { return version_; }
     const camera_t &camera(uint32_t i) const { return cams_[i].cam; }
 };
@@ -133,6 +135,39 @@ public:
     bool SetMaterialColors(const std::shared_ptr<SceneBase> &s, uint32_t first, uint32_t n, const float *rgb);
     bool ReadAccumulated(float *dst, size_t count);
     void set_sort_rays(bool on) { sort_rays_ = on; }
+    std::string last_error;
+};
+
+// Many scenes rendered together: each pass renders one progressive pass of
+// every scene that still has samples to take, in one set of persistent
+// kernels (include/rh_hip.h, rh_batch_*).  Each scene's image equals what
+// Renderer::RenderScene renders for it with a full-frame region, pass for
+// pass.  Scenes are copied when added; later changes to them are not seen.
+class BatchRenderer {
+    rh_batch *b_ = nullptr;
+    std::vector<uint16_t> permutations_;
+    std::vector<float> halton_;
+    std::vector<std::pair<int, int>> sizes_;
+    std::vector<std::vector<pixel_color_t>> pixels_;
+    int pass_ = 0;
+
+public:
+    explicit BatchRenderer(int device_index);
+    ~BatchRenderer();
+    BatchRenderer(const BatchRenderer &) = delete;
+    BatchRenderer &operator=(const BatchRenderer &) = delete;
+
+    // Adds a scene of this backend (its current camera) at w x h for spp
+    // passes; returns its index, or -1 (see last_error).
+    int AddScene(const std::shared_ptr<SceneBase> &s, int w, int h, int spp);
+    int passes() const { return rh_batch_passes(b_); }
+    int passes_done() const { return pass_; }
+    int scenes() const { return (int)sizes_.size(); }
+    // One pass over the batch; false on error or when every scene is done.
+    bool RenderPass();
+    const pixel_color_t *get_pixels_ref(int scene);
+    std::pair<int, int> size(int scene) const { return sizes_[scene]; }
+    rh_batch *handle() { return b_; }
     std::string last_error;
 };
 

@@ -71,15 +71,7 @@ bool Renderer::PreparePass(Scene &s, RegionContext &region, rh_pass_desc &pass) 
     }
 
     memset(&pass, 0, sizeof(pass));
-    memcpy(&pass.cam, &s.camera(s.current_cam()), sizeof(rh_camera));
-
-    if (s.macro_nodes_root_ != 0xffffffff) {
-        const rh_bvh_node &root = s.nodes_[s.macro_nodes_root_];
-        for (int i = 0; i < 3; i++) {
-            pass.root_min[i] = root.bbox_min[i];
-            pass.cell_size[i] = (root.bbox_max[i] - root.bbox_min[i]) / 255;
-        }
-    }
+    s.PassCamera(pass);
 
     rect_t rect = region.rect();
     if (rect.w == 0 || rect.h == 0) rect = {0, 0, w_, h_};

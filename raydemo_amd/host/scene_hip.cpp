@@ -546,6 +546,19 @@ uint32_t Scene::node_count() {
     return (uint32_t)nodes_.size();
 }
 
+// camera record of the current camera; the ray-sort hash grid spans the
+// macro-tree root box in 255 cells per axis (RendererRef.cpp:66-77)
+void Scene::PassCamera(rh_pass_desc &pass) {
+    memcpy(&pass.cam, &camera(current_cam()), sizeof(rh_camera));
+    if (macro_nodes_root_ != 0xffffffff) {
+        const rh_bvh_node &root = nodes_[macro_nodes_root_];
+        for (int i = 0; i < 3; i++) {
+            pass.root_min[i] = root.bbox_min[i];
+            pass.cell_size[i] = (root.bbox_max[i] - root.bbox_min[i]) / 255;
+        }
+    }
+}
+
 rh_scene_view Scene::View() {
     Finalize();
     rh_scene_view v;

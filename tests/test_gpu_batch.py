@@ -1,0 +1,75 @@
+"""The batch pipeline (rh_batch_* through Ray::Hip::BatchRenderer): scenes of
+different sizes, depths, sample counts and light setups rendered together
+must give, scene by scene, the same image as rendering each alone with the
+single-scene renderer (bit for bit), and the same ray counts in total,
+whatever the launch knobs."""
+import numpy as np
+import pytest
+
+from conftest import standalone_scene
+from raydemo_amd import capi as A, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases(golden):
+    return [
+        ("cornell", lambda s: scenes.cornell_box(s), 64, 48, 4),
+        ("soup", lambda s: scenes.triangle_soup(s, n_spheres=24, n_fine=2), 96, 64, 3),
+        ("mesh_lights", lambda s: scenes.mesh_lights(s, golden), 40, 40, 5),
+        ("refined", lambda s: scenes.cornell_refined(s, k=2, width=48, height=40, spp=2, depth=3), 48, 40, 2),
+        ("soup_wide", lambda s: scenes.triangle_soup(s, n_spheres=40, n_fine=3, seed=scenes.SEED_BASE + 9), 160, 24, 1),
+    ]
+
+
+def _single(lib, build, w, h, spp):
+    r = A.Renderer(lib, w, h, A.RENDERER_HIP)
+    s = r.create_scene()
+    build(s)
+    reg = A.Region(lib, 0, 0, w, h)
+    for _ in range(spp):
+        r.render(s, reg)
+    img = r.pixels()
+    rc = r.ray_counts()
+    return img, rc
+
+
+@pytest.mark.parametrize("opts", [{}, {"use_graph": 0}, {"path_tail": -1}, {"path_tail": 0, "lds_stack_depth": 8},
+                                  {"path_tail": 1, "blocks_per_cu": 1}])
+def test_batch_matches_single_renders(host_lib, golden_scene1, opts):
+    cases = _cases(golden_scene1)
+    singles = [_single(host_lib, b, w, h, spp) for _, b, w, h, spp in cases]
+    batch = A.Batch(host_lib)
+    for k, v in opts.items():
+        batch.set_option(k, v)
+    keep = []
+    for _, build, w, h, spp in cases:
+        s = standalone_scene(host_lib)
+        build(s)
+        keep.append(s)
+        batch.add(s, w, h, spp)
+    assert batch.passes == max(c[4] for c in cases)
+    batch.render_all()
+    for i, (name, _, w, h, spp) in enumerate(cases):
+        assert batch.pixels(i).tobytes() == singles[i][0].tobytes(), name
+    rc = batch.ray_counts()
+    for key in ("primary", "secondary", "shadow"):
+        assert rc[key] == sum(s[1][key] for s in singles), key
+
+
+def test_batch_images_after_each_pass(host_lib):
+    """Scenes with fewer samples stop while the others go on; every scene's
+    image after pass k equals its single render after min(k, spp) passes."""
+    cases = [(lambda s: scenes.cornell_box(s), 32, 32, 1), (lambda s: scenes.cornell_box(s, seed=scenes.SEED_BASE + 7), 64, 32, 3)]
+    batch = A.Batch(host_lib)
+    keep = []
+    for build, w, h, spp in cases:
+        s = standalone_scene(host_lib)
+        build(s)
+        keep.append(s)
+        batch.add(s, w, h, spp)
+    for k in range(1, 4):
+        batch.render_pass()
+        for i, (build, w, h, spp) in enumerate(cases):
+            img, _ = _single(host_lib, build, w, h, min(k, spp))
+            assert batch.pixels(i).tobytes() == img.tobytes(), (k, i)
