@@ -173,7 +173,7 @@ def run_cpu_baseline(args, A, scenes, spp_sample):
     return dict(kind=kind, seconds=dt, cores=cores, spp=spp_sample)
 
 
-def run_adjoint(args, lib, A, scenes):
+def run_adjoint(args, lib, A, scenes, knobs):
     """Refined Cornell box (9,828 triangles, one material per triangle), 512x512,
     32 spp, depth 6: Adam on the 9,826 albedos against a target rendered from a
     hidden albedo set (seed 0x5EED0005).  One iteration = forward (32 passes) +
@@ -183,6 +183,8 @@ def run_adjoint(args, lib, A, scenes):
 
     dr, init, spec = adjoint.refined_cornell(lib, width=args.adj_size, height=args.adj_size, spp=args.adj_spp,
                                             depth=6)
+    for k, v in knobs.items():
+        dr.r.set_option(k, v)
     n = spec.info["diffuse"]
     hidden = init.copy()
     hidden[:n] = scenes.refined_albedos(n, scenes.SEED_BASE + 5)
@@ -210,11 +212,113 @@ def run_adjoint(args, lib, A, scenes):
         "forward_mrays": round(fwd_rays / (t1 - t0) / 1e6, 2), "adjoint_mrays": round(adj_rays / (t2 - t1) / 1e6, 2),
         "loss_first": res.loss_first, "loss_last": res.loss_last,
         "albedo_rmse_first": round(res.albedo_rmse_first, 5), "albedo_rmse_last": round(res.albedo_rmse_last, 5),
+        "knobs": knobs,
         "config": {"workload": f"cornell_refined {spec.triangles} tris, {n} albedo parameters x 3, "
                                f"{args.adj_size}x{args.adj_size}, {args.adj_spp} spp, depth 6, roulette off",
                    "gradient": "d L2(image, target) / d main_color, all materials"},
     }
     log(f"adjoint: {out}")
+    return out
+
+
+def run_batch(args, lib, A, scenes, knobs):
+    """The 64-scene workload: rendered scene after scene by single-scene
+    renderers (the plain wavefront path, each scene's passes back to back),
+    then all together by the batch renderer.  Same passes, same rays (the
+    images are identical); Mrays/s of each."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    items = scenes.batch_workload(args.batch_scenes)
+    t0 = time.time()
+
+    def build(it):
+        s = A.standalone_scene(lib)
+        scenes.batch_scene(s, it)
+        s.triangle_count()
+        return s
+
+    with ThreadPoolExecutor(max_workers=16) as ex:
+        ss = list(ex.map(build, items))
+    log(f"batch: {len(items)} scenes, {sum(s.triangle_count() for s in ss)} triangles, host build {time.time() - t0:.1f}s")
+
+    def total(rc):
+        return rc["primary"] + rc["secondary"] + rc["shadow"]
+
+    # plain: one renderer per scene (scene uploaded by a warm-up pass), scenes rendered in turn
+    rs = []
+    for it, s in zip(items, ss):
+        r = A.Renderer(lib, it.width, it.height, A.RENDERER_HIP)
+        for k, v in knobs.items():
+            r.set_option(k, v)
+        r.render(s, A.Region(lib, 0, 0, it.width, it.height))
+        r.synchronize()
+        r.clear()
+        r.reset_stats()
+        rs.append(r)
+    t1 = time.perf_counter()
+    for r, s, it in zip(rs, ss, items):
+        reg = A.Region(lib, 0, 0, it.width, it.height)
+        for _ in range(it.spp):
+            r.render(s, reg)
+    for r in rs:
+        r.synchronize()
+    t_plain = time.perf_counter() - t1
+    rays_plain = sum(total(r.ray_counts()) for r in rs)
+    for r in rs:
+        r.close()
+    del rs
+
+    b = A.Batch(lib)
+    for it, s in zip(items, ss):
+        b.add(s, it.width, it.height, it.spp)
+    # knobs of the batch: path tail, LDS stack share, persistent grid size (3 passes each)
+    table = []
+    for pt in (-1, 1, 2, 3):
+        for lds in (8, 16):
+            for bpc in (4, 8, 16):
+                b.set_option("path_tail", pt)
+                b.set_option("lds_stack_depth", lds)
+                b.set_option("blocks_per_cu", bpc)
+                b.restart()
+                b.render_pass()  # graph capture and warm-up
+                b.synchronize()
+                b.restart()
+                b.reset_stats()
+                t2 = time.perf_counter()
+                for _ in range(3):
+                    b.render_pass()
+                b.synchronize()
+                dt = time.perf_counter() - t2
+                table.append([pt, lds, bpc, round(total(b.ray_counts()) / dt / 1e6, 2)])
+    best = max(table, key=lambda r: r[-1])
+    b.set_option("path_tail", best[0])
+    b.set_option("lds_stack_depth", best[1])
+    b.set_option("blocks_per_cu", best[2])
+    b.restart()
+    b.render_pass()
+    b.synchronize()
+    b.restart()
+    b.reset_stats()
+    t3 = time.perf_counter()
+    b.render_all()
+    b.synchronize()
+    t_batch = time.perf_counter() - t3
+    rays_batch = total(b.ray_counts())
+    out = {
+        "metric": "Mrays/s, 64 scenes: batched vs rendered one after another",
+        "batched_mrays": round(rays_batch / t_batch / 1e6, 2), "plain_mrays": round(rays_plain / t_plain / 1e6, 2),
+        "speedup": round(t_plain / t_batch, 3), "batched_s": round(t_batch, 4), "plain_s": round(t_plain, 4),
+        "rays": rays_batch, "rays_plain": rays_plain, "passes": b.passes,
+        "knobs": {"path_tail": best[0], "lds_stack_depth": best[1], "blocks_per_cu": best[2], "use_graph": 1},
+        "plain_knobs": knobs,
+        "autotune": {"columns": ["path_tail", "lds_stack_depth", "blocks_per_cu", "mrays_first_3_passes"],
+                     "table": table},
+        "config": {"workload": f"{len(items)} icosphere soups, {sum(s.triangle_count() for s in ss)} triangles, "
+                               f"{sum(i.width * i.height for i in items)} pixels, 4..32 spp, depth 4..8",
+                   "scenes": len(items)},
+    }
+    log(f"batch: {out}")
+    b.close()
     return out
 
 
@@ -234,6 +338,7 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=200, help="Adam steps of the differentiation workload (0: skip)")
     ap.add_argument("--adj-size", type=int, default=512)
     ap.add_argument("--adj-spp", type=int, default=32)
+    ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batch workload (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -291,7 +396,9 @@ def main():
                              f"128x128 buckets on {cpu_run['cores']} threads, {cpu_run['seconds']:.1f}s"}
             log(f"cpu baseline: {cpu}")
 
-    adj = run_adjoint(args, lib, A, scenes) if args.adj_steps > 0 else None
+    plain_knobs = {k: v for k, v in tuned.best.items() if k in ("lds_stack_depth", "sort_rays", "sort_bits", "path_tail")}
+    adj = run_adjoint(args, lib, A, scenes, plain_knobs) if args.adj_steps > 0 else None
+    batch = run_batch(args, lib, A, scenes, plain_knobs) if args.batch_scenes > 0 else None
 
     line = {
         "metric": "Mrays/s at 1080p depth-8 (triangle soup, primary+secondary+shadow rays)",
@@ -321,6 +428,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "adjoint": adj,
+        "batch": batch,
     }
     print(json.dumps(line), flush=True)
 

@@ -131,6 +131,8 @@ int rh_batch_set_halton(rh_batch *batch, const float *seq, size_t count);
 int rh_batch_passes(const rh_batch *batch);
 int rh_batch_render_pass(rh_batch *batch);
 int rh_batch_synchronize(rh_batch *batch);
+/* back to pass 0 with cleared images (the scenes stay uploaded) */
+int rh_batch_restart(rh_batch *batch);
 int rh_batch_read_pixels(rh_batch *batch, int scene, float *rgba, size_t count);
 int rh_batch_reset_stats(rh_batch *batch);
 int rh_batch_get_ray_counts(rh_batch *batch, rh_ray_counts *out);

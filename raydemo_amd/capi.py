@@ -231,6 +231,7 @@ class RayLib:
             "rhx_batch_get_ray_counts": (c_int, [P, POINTER(RayCounts)]),
             "rhx_batch_reset_stats": (c_int, [P]),
             "rhx_batch_synchronize": (c_int, [P]),
+            "rhx_batch_restart": (c_int, [P]),
         })
         self.has_adjoint = all(hasattr(L, n) for n in ("rhx_adjoint_scene", "rhx_set_material_colors", "rhx_read_accum"))
         self.has_batch = hasattr(L, "rhx_batch_create")
@@ -351,6 +352,18 @@ class Renderer:
         return out
 
 
+class _NoRenderer:
+    def __init__(self, lib):
+        self.lib = lib
+        self.h = None
+
+
+def standalone_scene(lib: RayLib) -> "Scene":
+    """A scene built on the host without a renderer; any renderer (or batch)
+    of the same library can render it."""
+    return Scene(_NoRenderer(lib))
+
+
 class Batch:
     """Many scenes per pass (HIP backend): Ray::Hip::BatchRenderer."""
 
@@ -411,6 +424,9 @@ class Batch:
 
     def synchronize(self):
         self.lib.check(self.lib.lib.rhx_batch_synchronize(self.h), "rhx_batch_synchronize")
+
+    def restart(self):
+        self.lib.check(self.lib.lib.rhx_batch_restart(self.h), "rhx_batch_restart")
 
 
 class Region:

@@ -639,6 +639,24 @@ int rh_batch_render_pass(rh_batch *b) {
     return 0;
 }
 
+int rh_batch_restart(rh_batch *b) {
+    RH_CHECK(hipSetDevice(b->device));
+    if (!b->built) return 0;
+    RH_CHECK(hipStreamSynchronize(b->stream));
+    RH_CHECK(hipMemcpyAsync(b->table, b->entries.data(), b->entries.size() * sizeof(BatchEntry), hipMemcpyHostToDevice,
+                            b->stream));
+    RH_CHECK(hipMemsetAsync(b->pass, 0, sizeof(int), b->stream));
+    const size_t n4 = b->total_pixels;
+    hipLaunchKernelGGL(k_b_fill, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, b->stream, b->clean, n4,
+                       make_float4(0, 0, 0, 0));
+    hipLaunchKernelGGL(k_b_fill, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, b->stream, b->final_, n4,
+                       make_float4(0, 0, 0, 0));
+    RH_CHECK(hipGetLastError());
+    RH_CHECK(hipStreamSynchronize(b->stream));
+    b->passes_done = 0;
+    return 0;
+}
+
 int rh_batch_synchronize(rh_batch *b) {
     RH_CHECK(hipStreamSynchronize(b->stream));
     return 0;

@@ -74,6 +74,15 @@ bool BatchRenderer::RenderPass() {
     return true;
 }
 
+bool BatchRenderer::Restart() {
+    if (rh_batch_restart(b_) != 0) {
+        last_error = rh_last_error();
+        return false;
+    }
+    pass_ = 0;
+    return true;
+}
+
 const pixel_color_t *BatchRenderer::get_pixels_ref(int scene) {
     if (scene < 0 || scene >= (int)sizes_.size()) return nullptr;
     auto &px = pixels_[scene];

@@ -390,4 +390,12 @@ int rhx_batch_reset_stats(rhx_batch *b) { return rh_batch_reset_stats(b->b->hand
 
 int rhx_batch_synchronize(rhx_batch *b) { return rh_batch_synchronize(b->b->handle()); }
 
+int rhx_batch_restart(rhx_batch *b) {
+    if (!b->b->Restart()) {
+        g_err = b->b->last_error;
+        return -1;
+    }
+    return 0;
+}
+
 } // extern "C"

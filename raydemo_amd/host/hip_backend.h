@@ -165,6 +165,8 @@ public:
     int scenes() const { return (int)sizes_.size(); }
     // One pass over the batch; false on error or when every scene is done.
     bool RenderPass();
+    // Back to pass 0 with cleared images; the scenes stay on the device.
+    bool Restart();
     const pixel_color_t *get_pixels_ref(int scene);
     std::pair<int, int> size(int scene) const { return sizes_[scene]; }
     rh_batch *handle() { return b_; }

@@ -10,6 +10,10 @@ Workloads (seed = 0x5EED0000 + k for workload k):
                      lights plus a constant environment, 50/50 diffuse / glossy
                      (roughness U[0.05, 1]); 1920x1080, 64 spp, depth 8.
   mesh_lights      — the reference's own mesh-lights test scene (test_scene1.h).
+  batch_workload   — 64 icosphere soups (seed 0x5EED0003): triangle counts
+                     log-uniform 1k..500k, frames drawn from {64, 128, 256, 512,
+                     1024}^2, 4..32 spp, depth 4..8, each rendered alone or all
+                     together by the batch renderer.
   cornell_refined  — the Cornell box with every wall and box face split into a
                      k x k grid (k = 17: 9,826 triangles + the 2-triangle light),
                      one diffuse material per triangle with albedo U[0.1, 0.9]
@@ -319,3 +323,35 @@ def mesh_lights(scene: "A.Scene", data) -> SceneSpec:
                           [(m1, g[0], g[1]), (m2, g[2], g[3]), (m3, g[4], g[5]), (m4, g[6], g[7])])
     scene.add_mesh_instance(mesh)
     return SceneSpec("mesh_lights", 64, 64, 2048, 4)
+
+
+@dataclass
+class BatchItem:
+    seed: int
+    triangles: int   # target
+    n_spheres: int
+    n_fine: int
+    width: int
+    height: int
+    spp: int
+    depth: int
+
+
+def batch_workload(n: int = 64, seed: int = SEED_BASE + 3) -> list:
+    """Scene list of the many-scene workload (see the module docstring)."""
+    rng = np.random.default_rng(seed)
+    items = []
+    for k in range(n):
+        tris = int(round(math.exp(rng.uniform(math.log(1e3), math.log(5e5)))))
+        n_spheres = max(2, int(round((tris - 2) / 391.0)))
+        n_fine = int(round(n_spheres * 38 / 512))
+        res = int(rng.choice([64, 128, 256, 512, 1024]))
+        spp = int(rng.integers(4, 33))
+        depth = int(rng.integers(4, 9))
+        items.append(BatchItem(seed + 1000 + k, tris, n_spheres, n_fine, res, res, spp, depth))
+    return items
+
+
+def batch_scene(scene: "A.Scene", item: BatchItem) -> SceneSpec:
+    return triangle_soup(scene, seed=item.seed, width=item.width, height=item.height, spp=item.spp, depth=item.depth,
+                         n_spheres=item.n_spheres, n_fine=item.n_fine)
