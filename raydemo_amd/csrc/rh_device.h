@@ -148,6 +148,28 @@ __device__ __forceinline__ bool bbox_test(f3 o, f3 inv_d, float t, const float *
     return tmin <= tmax && tmin <= t && tmax > 0;
 }
 
+// The same slab test for rays whose inv_d has no infinite component: no slab
+// value can then be NaN (finite * finite), and for non-NaN operands the
+// reference's swaps and selects are exactly min and max (only the sign of a
+// zero may differ, which no comparison sees), so hardware min/max (min3/max3)
+// give the same verdict with fewer instructions.
+__device__ __forceinline__ bool bbox_test_finite(f3 o, f3 inv_d, float t, const float *bmin, const float *bmax) {
+    const float lx = inv_d.x * (bmin[0] - o.x), hx = inv_d.x * (bmax[0] - o.x);
+    const float ly = inv_d.y * (bmin[1] - o.y), hy = inv_d.y * (bmax[1] - o.y);
+    const float lz = inv_d.z * (bmin[2] - o.z), hz = inv_d.z * (bmax[2] - o.z);
+    const float tmin = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(lx, hx), __builtin_fminf(ly, hy)),
+                                       __builtin_fminf(lz, hz));
+    const float tmax = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(lx, hx), __builtin_fmaxf(ly, hy)),
+                                       __builtin_fmaxf(lz, hz)) *
+                       1.00000024f;
+    return tmin <= tmax && tmin <= t && tmax > 0;
+}
+
+template <bool Finite>
+__device__ __forceinline__ bool box_hit(f3 o, f3 inv_d, float t, const float *bmin, const float *bmax) {
+    return Finite ? bbox_test_finite(o, inv_d, t, bmin, bmax) : bbox_test(o, inv_d, t, bmin, bmax);
+}
+
 // safe_invert (CoreRef.cpp:272-294)
 __device__ __forceinline__ f3 safe_invert(f3 v) {
     f3 o = f3{1.0f / v.x, 1.0f / v.y, 1.0f / v.z};
@@ -306,6 +328,7 @@ __device__ __forceinline__ bool intersect_leaf(const Scene &sc, const NodeBox &n
 // near child is popped at once (same t), and a pushed far child is re-tested
 // against the current t when popped here too (boxes only get culled as t
 // shrinks).  Returns false when neither child is entered.
+template <bool Finite>
 __device__ __forceinline__ bool enter_children(const Scene &sc, const NodeBox &n, f3 o, f3 d, f3 inv_d, float t,
                                                Stack &st, NodeBox &next, TraceCounters *cnt) {
     const uint32_t axis = n.b >> 30;
@@ -314,8 +337,8 @@ __device__ __forceinline__ bool enter_children(const Scene &sc, const NodeBox &n
     const uint32_t near_i = neg ? right : left, far_i = neg ? left : right;
     const NodeBox nb = load_node(sc.nodes, near_i), fb = load_node(sc.nodes, far_i);
     if (cnt) cnt->nodes += 2;
-    const bool hn = bbox_test(o, inv_d, t, nb.bmin, nb.bmax);
-    const bool hf = bbox_test(o, inv_d, t, fb.bmin, fb.bmax);
+    const bool hn = box_hit<Finite>(o, inv_d, t, nb.bmin, nb.bmax);
+    const bool hf = box_hit<Finite>(o, inv_d, t, fb.bmin, fb.bmax);
     if (hn) {
         if (hf) st.push(far_i);
         next = nb;
@@ -329,12 +352,13 @@ __device__ __forceinline__ bool enter_children(const Scene &sc, const NodeBox &n
 }
 
 // Pops until a node passes its box test against the current t.
+template <bool Finite>
 __device__ __forceinline__ bool pop_entered(const Scene &sc, uint32_t base, f3 o, f3 inv_d, float t, Stack &st,
                                             NodeBox &next, TraceCounters *cnt) {
     while (st.sp > base) {
         const NodeBox n = load_node(sc.nodes, st.pop());
         if (cnt) cnt->nodes++;
-        if (bbox_test(o, inv_d, t, n.bmin, n.bmax)) {
+        if (box_hit<Finite>(o, inv_d, t, n.bmin, n.bmax)) {
             next = n;
             return true;
         }
@@ -342,22 +366,44 @@ __device__ __forceinline__ bool pop_entered(const Scene &sc, uint32_t base, f3 o
     return false;
 }
 
+// One instance's triangle tree, from its root node mc.  Returns true when an
+// any-hit query found a solid occluder.
+template <bool AnyHit, bool Finite>
+__device__ __forceinline__ bool walk_micro(const Scene &sc, NodeBox mc, f3 mo, f3 md, f3 minv, int mi_index, Hit &hit,
+                                           Stack &st, TraceCounters *cnt) {
+    const uint32_t base = st.sp;
+    bool mhave = box_hit<Finite>(mo, minv, hit.t, mc.bmin, mc.bmax);
+    while (true) {
+        if (mhave) {
+            if (!(mc.a & RH_LEAF_NODE_BIT)) {
+                mhave = enter_children<Finite>(sc, mc, mo, md, minv, hit.t, st, mc, cnt);
+                continue;
+            }
+            if (intersect_leaf<AnyHit>(sc, mc, mo, md, mi_index, hit, cnt)) return true;
+        }
+        mhave = pop_entered<Finite>(sc, base, mo, minv, hit.t, st, mc, cnt);
+        if (!mhave) return false;
+    }
+}
+
 // Two-level traversal, entered-node form (see enter_children): the lane holds
-// the node it is in; the stack holds far siblings still to be re-tested.
+// the node it is in; the stack holds far siblings still to be re-tested.  The
+// macro tree uses safe_invert's finite inverse; an instance's object-space
+// ray takes the exact NaN-preserving box test only when 1/d overflowed.
 template <bool AnyHit>
 __device__ __forceinline__ void traverse_fast(const Scene &sc, f3 o, f3 d, Hit &hit, Stack &st, TraceCounters *cnt) {
     const f3 inv_d = safe_invert(d);
     st.sp = 0;
     NodeBox cur = load_node(sc.nodes, sc.macro_root);
     if (cnt) cnt->nodes++;
-    if (!bbox_test(o, inv_d, hit.t, cur.bmin, cur.bmax)) return;
+    if (!box_hit<true>(o, inv_d, hit.t, cur.bmin, cur.bmax)) return;
 
     uint32_t leaf_cur = 0, leaf_end = 0;
     bool have = true; // macro phase: cur is an entered node
     while (true) {
         if (have) {
             if (!(cur.a & RH_LEAF_NODE_BIT)) {
-                have = enter_children(sc, cur, o, d, inv_d, hit.t, st, cur, cnt);
+                have = enter_children<true>(sc, cur, o, d, inv_d, hit.t, st, cur, cnt);
                 continue;
             }
             leaf_cur = cur.a & RH_PRIM_INDEX_BITS;
@@ -369,7 +415,7 @@ __device__ __forceinline__ void traverse_fast(const Scene &sc, f3 o, f3 d, Hit &
             const uint32_t mi_index = sc.mi_indices[leaf_cur++];
             const rh_mesh_instance mi = sc.mesh_instances[mi_index];
             if (cnt) cnt->instances++;
-            if (!bbox_test(o, inv_d, hit.t, mi.bbox_min, mi.bbox_max)) continue;
+            if (!box_hit<true>(o, inv_d, hit.t, mi.bbox_min, mi.bbox_max)) continue;
             const float *x = sc.transforms[mi.tr_index].inv_xform;
             // TransformRay (CoreRef.cpp:1605-1617)
             const f3 mo = f3{x[0] * o.x + x[4] * o.y + x[8] * o.z + x[12], x[1] * o.x + x[5] * o.y + x[9] * o.z + x[13],
@@ -377,26 +423,17 @@ __device__ __forceinline__ void traverse_fast(const Scene &sc, f3 o, f3 d, Hit &
             const f3 md = f3{x[0] * d.x + x[4] * d.y + x[8] * d.z, x[1] * d.x + x[5] * d.y + x[9] * d.z,
                              x[2] * d.x + x[6] * d.y + x[10] * d.z};
             const f3 minv = f3{1.0f / md.x, 1.0f / md.y, 1.0f / md.z};
-            const uint32_t base = st.sp;
-            NodeBox mc = load_node(sc.nodes, sc.meshes[mi.mesh_index].node_index);
+            const NodeBox mc = load_node(sc.nodes, sc.meshes[mi.mesh_index].node_index);
             if (cnt) cnt->nodes++;
-            bool mhave = bbox_test(mo, minv, hit.t, mc.bmin, mc.bmax);
-            while (true) {
-                if (mhave) {
-                    if (!(mc.a & RH_LEAF_NODE_BIT)) {
-                        mhave = enter_children(sc, mc, mo, md, minv, hit.t, st, mc, cnt);
-                        continue;
-                    }
-                    if (intersect_leaf<AnyHit>(sc, mc, mo, md, (int)mi_index, hit, cnt)) {
-                        st.sp = 0;
-                        return; // solid occluder found
-                    }
-                }
-                mhave = pop_entered(sc, base, mo, minv, hit.t, st, mc, cnt);
-                if (!mhave) break;
+            const bool finite = __builtin_isfinite(minv.x) && __builtin_isfinite(minv.y) && __builtin_isfinite(minv.z);
+            const bool solid = finite ? walk_micro<AnyHit, true>(sc, mc, mo, md, minv, (int)mi_index, hit, st, cnt)
+                                      : walk_micro<AnyHit, false>(sc, mc, mo, md, minv, (int)mi_index, hit, st, cnt);
+            if (solid) {
+                st.sp = 0;
+                return; // solid occluder found
             }
         }
-        have = pop_entered(sc, 0, o, inv_d, hit.t, st, cur, cnt);
+        have = pop_entered<true>(sc, 0, o, inv_d, hit.t, st, cur, cnt);
         if (!have) break;
     }
 }
