@@ -333,7 +333,8 @@ def main():
     ap.add_argument("--cpu-spp", type=int, default=64, help="full-frame passes of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune-passes", type=int, default=6, help="passes per autotuner candidate")
-    ap.add_argument("--knobs", default="", help="skip the autotuner: lds_stack_depth,sort_rays,sort_bits,path_tail")
+    ap.add_argument("--knobs", default="",
+                    help="skip the autotuner: lds_stack_depth,sort_rays,sort_bits,path_tail[,tail_occupancy]")
     ap.add_argument("--only-tuned", action="store_true", help="time only the tuned (or given) knobs")
     ap.add_argument("--adj-steps", type=int, default=200, help="Adam steps of the differentiation workload (0: skip)")
     ap.add_argument("--adj-size", type=int, default=512)
@@ -396,7 +397,7 @@ def main():
                              f"128x128 buckets on {cpu_run['cores']} threads, {cpu_run['seconds']:.1f}s"}
             log(f"cpu baseline: {cpu}")
 
-    plain_knobs = {k: v for k, v in tuned.best.items() if k in ("lds_stack_depth", "sort_rays", "sort_bits", "path_tail")}
+    plain_knobs = {k: v for k, v in tuned.best.items() if k in autotune.KNOBS}
     adj = run_adjoint(args, lib, A, scenes, plain_knobs) if args.adj_steps > 0 else None
     batch = run_batch(args, lib, A, scenes, plain_knobs) if args.batch_scenes > 0 else None
 

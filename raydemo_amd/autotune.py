@@ -28,7 +28,8 @@ LDS_DEPTHS = (8, 16, 24, 32)
 SORT_MODES = ((0, 32), (1, 8), (1, 32))
 # first bounce of the path-tail launch (-1: every bounce staged)
 PATH_TAILS = (-1, 0, 1, 2, 3, 4)
-KNOBS = ("lds_stack_depth", "sort_rays", "sort_bits", "path_tail")
+TAIL_OCCUPANCY = (2, 3)
+KNOBS = ("lds_stack_depth", "sort_rays", "sort_bits", "path_tail", "tail_occupancy")
 
 
 @dataclass
@@ -41,9 +42,11 @@ class TuneResult:
         return {"best": self.best, "table": self.table, "seconds": round(self.seconds, 3)}
 
 
-def candidates(depths=LDS_DEPTHS, sort_modes=SORT_MODES, tails=PATH_TAILS):
-    for d, (so, sb), pt in itertools.product(depths, sort_modes, tails):
-        yield {"lds_stack_depth": d, "sort_rays": so, "sort_bits": sb, "path_tail": pt}
+def candidates(depths=LDS_DEPTHS, sort_modes=SORT_MODES, tails=PATH_TAILS, occupancies=TAIL_OCCUPANCY):
+    for d, (so, sb), pt, occ in itertools.product(depths, sort_modes, tails, occupancies):
+        if pt < 0 and occ != occupancies[0]:
+            continue  # no tail launch: occupancy is moot
+        yield {"lds_stack_depth": d, "sort_rays": so, "sort_bits": sb, "path_tail": pt, "tail_occupancy": occ}
 
 
 def apply(renderer, cfg: dict):

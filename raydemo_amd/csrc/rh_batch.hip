@@ -28,7 +28,7 @@ using rh_rt::free_ptr;
 namespace {
 
 constexpr int kMaxBatchScenes = 256;
-// tile counters of one pass: generate, five per staged bounce, tail, mix
+// tile counters of one pass: generate, five per staged bounce, two tails, mix
 constexpr int kMaxStages = 5 * (RH_MAX_BOUNCES + 1) + 4;
 
 // one scene of the batch (device table)
@@ -293,15 +293,17 @@ __global__ void __launch_bounds__(kBlock) k_b_combine(BatchDev b, int stage, int
                   });
 }
 
-// the rays entering `stage` follow their paths to each scene's depth in one lane
-__global__ void __launch_bounds__(kBlock) k_b_tail(BatchDev b, int stage, int q, int work) {
+// the rays entering `stage` follow their paths to each scene's depth in one
+// lane; Defer: scenes whose light samples are resolved after shading (1..8 lights)
+template <bool Defer> __global__ void __launch_bounds__(kBlock) k_b_tail(BatchDev b, int stage, int q, int work) {
     extern __shared__ uint32_t lds_stack[];
     const int *counts = b.counts + stage * b.n_scenes;
-    for_each_tile(b, counts, b.work + work, [&](int s) { return stage_active(b.table[s], stage); },
+    for_each_tile(b, counts, b.work + work,
+                  [&](int s) { return stage_active(b.table[s], stage) && ((b.table[s].nee_slots != 0) == Defer); },
                   [&](int s, uint32_t base) {
                       const BatchEntry e = entry(b, s);
-                      ShadeArgs a = batch_shade_args(b, e, s, stage, q, false);
-                      path_tail_body(a, stage - 1, e.depth, base, lds_stack);
+                      const ShadeArgs a = batch_shade_args(b, e, s, stage, q, Defer);
+                      path_tail_body<Defer>(a, stage - 1, e.depth, base, lds_stack);
                   });
 }
 
@@ -483,7 +485,7 @@ BatchDev dev_view(rh_batch *b) {
 int enqueue_pass(rh_batch *b) {
     // first stage run by the tail kernel (bounce path_tail)
     const int tail = b->path_tail < 0 ? b->max_depth + 1 : std::min(b->path_tail + 1, b->max_depth + 1);
-    const int launches = 2 + 5 * tail + (tail <= b->max_depth ? 1 : 0);
+    const int launches = 2 + 5 * tail + (tail <= b->max_depth ? 2 : 0);
     if (launches > kMaxStages) return arg_error("batch pass needs more stage counters than reserved");
     const BatchDev d = dev_view(b);
     const size_t lds = (size_t)b->lds_depth * kBlock * sizeof(uint32_t);
@@ -496,7 +498,8 @@ int enqueue_pass(rh_batch *b) {
     // from path_tail on run in the tail kernel.
     for (int stage = 0; stage <= b->max_depth; stage++) {
         if (stage >= tail) {
-            hipLaunchKernelGGL(k_b_tail, grid, blk, lds, b->stream, d, stage, q, work++);
+            hipLaunchKernelGGL(k_b_tail<true>, grid, blk, lds, b->stream, d, stage, q, work++);
+            hipLaunchKernelGGL(k_b_tail<false>, grid, blk, lds, b->stream, d, stage, q, work++);
             break;
         }
         hipLaunchKernelGGL(k_b_trace, grid, blk, lds, b->stream, d, stage, q, work++);

@@ -48,10 +48,14 @@ template <bool Defer> __global__ void __launch_bounds__(kBlock) k_shade(ShadeArg
     shade_body<Defer>(a, blockIdx.x * blockDim.x, lds_stack, s_app);
 }
 
-// the bounces first..last-1 of each queued ray in one lane (path_tail_body)
-__global__ void __launch_bounds__(kBlock) k_path_tail(ShadeArgs a, int first, int last) {
+// the bounces first..last-1 of each queued ray in one lane (path_tail_body).
+// MinBlocks 3 asks the compiler for 3 resident blocks (waves per SIMD) per CU,
+// trading register spills for occupancy; 1 lets it use all the registers it
+// wants (2 waves per SIMD).  Which is faster is measured (tail_occupancy).
+template <bool Defer, int MinBlocks>
+__global__ void __launch_bounds__(kBlock, MinBlocks) k_path_tail(ShadeArgs a, int first, int last) {
     extern __shared__ uint32_t lds_stack[];
-    path_tail_body(a, first, last, blockIdx.x * blockDim.x, lds_stack);
+    path_tail_body<Defer>(a, first, last, blockIdx.x * blockDim.x, lds_stack);
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -406,6 +410,7 @@ struct rh_context {
     int sort_rays = 1;
     int sort_bits = 32; // key bits the in-pass ray sort orders by (top bits)
     int path_tail = 2;  // first bounce run by the path-tail kernel (-1: none)
+    int tail_occupancy = 2; // waves per SIMD the path-tail kernel is compiled for (2 or 3)
     int stage_timing = 0;
     int lds_depth = 24;
     rh_stats stats = {};
@@ -869,13 +874,23 @@ int rh_render_pass(rh_context *c, const rh_pass_desc *p) {
             tb.aux = c->aux[q];
             tb.count = cnt[q];
             tb.primary = 0;
-            tb.nee_ray = nullptr;
-            tb.nee_aux = nullptr;
-            tb.nee_term = nullptr;
             tb.ray_counter = c->stats_dev + 1;
             KernelMark km(c, 4);
-            hipLaunchKernelGGL(k_path_tail, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), lds_bytes(c), c->stream, tb,
-                               bounce, (int)pi.settings.max_total_depth);
+            // with deferred NEE on, the lane resolves its light samples itself
+            // after shading (the slots of ray i are free during the tail)
+            const dim3 grid((n + kBlock - 1) / kBlock), blk(kBlock);
+            const int last = pi.settings.max_total_depth;
+            if (tb.nee_ray) {
+                if (c->tail_occupancy == 3)
+                    hipLaunchKernelGGL((k_path_tail<true, 3>), grid, blk, lds_bytes(c), c->stream, tb, bounce, last);
+                else
+                    hipLaunchKernelGGL((k_path_tail<true, 1>), grid, blk, lds_bytes(c), c->stream, tb, bounce, last);
+            } else {
+                if (c->tail_occupancy == 3)
+                    hipLaunchKernelGGL((k_path_tail<false, 3>), grid, blk, lds_bytes(c), c->stream, tb, bounce, last);
+                else
+                    hipLaunchKernelGGL((k_path_tail<false, 1>), grid, blk, lds_bytes(c), c->stream, tb, bounce, last);
+            }
             RH_CHECK(hipGetLastError());
             break;
         }
@@ -988,6 +1003,9 @@ int rh_set_option(rh_context *c, const char *name, int value) {
     } else if (n == "path_tail") {
         if (value < -1 || value > 255) return arg_error("path_tail must be -1..255");
         c->path_tail = value;
+    } else if (n == "tail_occupancy") {
+        if (value != 2 && value != 3) return arg_error("tail_occupancy must be 2 or 3");
+        c->tail_occupancy = value;
     } else if (n == "sort_bits") {
         if (value != 8 && value != 16 && value != 24 && value != 32) return arg_error("sort_bits must be 8, 16, 24 or 32");
         c->sort_bits = value;
@@ -1002,6 +1020,7 @@ int rh_get_option(rh_context *c, const char *name, int *value) {
     else if (n == "lds_stack_depth") *value = c->lds_depth;
     else if (n == "sort_bits") *value = c->sort_bits;
     else if (n == "path_tail") *value = c->path_tail;
+    else if (n == "tail_occupancy") *value = c->tail_occupancy;
     else if (n == "defer_nee") *value = c->defer_nee;
     else if (n == "stack_overflows") {
         unsigned long long v[8];

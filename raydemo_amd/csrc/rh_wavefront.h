@@ -447,6 +447,12 @@ __device__ __forceinline__ void shade_body(const ShadeArgs &a, uint32_t base, ui
 // launch.  Every pixel still receives its path's contributions in bounce
 // order, and each contribution is computed exactly as in the staged form, so
 // the image is identical.
+// Defer: the direct light is recorded in the lane's NEE slots as in the staged
+// deferred form and resolved after ShadeSurface returns (visibility, visible
+// terms summed in light order, albedo, throughput: the combine stage's
+// arithmetic), so the shading state is dead while the shadow rays traverse
+// and fewer registers are live at the peak.
+template <bool Defer>
 __device__ __forceinline__ void path_tail_body(const ShadeArgs &a, int first, int last, uint32_t base,
                                                uint32_t *lds_stack) {
     const int n = *a.count;
@@ -487,13 +493,36 @@ __device__ __forceinline__ void path_tail_body(const ShadeArgs &a, int first, in
             pi.bounce = b + 3;
             ShadeOut so;
             so.emit = false;
-            shade_surface(a.sc, pi, hit, ray, a.halton, st, cnt, so, nullptr);
+            NeeSink sink;
+            if (Defer) sink = NeeSink{a.nee_ray, a.nee_aux, a.nee_term, i * a.nee_slots, 0};
+            shade_surface(a.sc, pi, hit, ray, a.halton, st, cnt, so, Defer ? &sink : nullptr);
             float4 v = dst;
             v.x += so.col.x;
             v.y += so.col.y;
             v.z += so.col.z;
             v.w += 0.0f;
             dst = v;
+            if (Defer && so.nee_pending && sink.count > 0) {
+                f3 col = f3{0.0f, 0.0f, 0.0f};
+                for (uint32_t j = 0; j < sink.count; j++) {
+                    const uint32_t k = sink.base + j;
+                    const float4 r = a.nee_ray[k];
+                    const int2 x = a.nee_aux[k];
+                    const rh_light &l = a.sc.lights[__float_as_uint(r.w)];
+                    st.sp = 0;
+                    if (compute_visibility(a.sc, f3{r.x, r.y, r.z}, ld3(l.pos), a.halton, x.x, x.y, st, cnt) != 0.0f) {
+                        const float4 t = a.nee_term[k];
+                        col = col + f3{t.x, t.y, t.z};
+                    }
+                }
+                if (so.consider_albedo) col = col * so.albedo;
+                const f3 add = f3{ray.c.x * col.x, ray.c.y * col.y, ray.c.z * col.z};
+                float4 w = dst;
+                w.x += add.x;
+                w.y += add.y;
+                w.z += add.z;
+                dst = w;
+            }
             if (!so.emit) break;
             ray = so.next;
         }
