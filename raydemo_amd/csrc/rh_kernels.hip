@@ -76,6 +76,11 @@ struct AdjointArgs {
     unsigned long long *ray_counter;
     unsigned long long *shadow_counter;
     unsigned long long *overflow;
+    // light samples recorded by shading and resolved in the lane (Defer)
+    float4 *nee_ray;
+    int2 *nee_aux;
+    float4 *nee_term;
+    uint32_t nee_slots;
 };
 
 // Path replay for one pass: each lane regenerates its pixel's primary ray with
@@ -89,7 +94,7 @@ struct AdjointArgs {
 // The suffix sums are formed backwards over the records; the gradient adds
 // adj[pixel] * scale * S_m / main_color with float atomics (their order is the
 // only run-to-run difference).
-__global__ void __launch_bounds__(kBlock) k_adjoint(AdjointArgs a) {
+template <bool Defer, int MinBlocks> __global__ void __launch_bounds__(kBlock, MinBlocks) k_adjoint(AdjointArgs a) {
     extern __shared__ uint32_t lds_stack[];
     const int rw = a.pc.rect[2], rh_ = a.pc.rect[3];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -133,8 +138,13 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(AdjointArgs a) {
                 pi.bounce = b < 0 ? 2 : b + 3;
                 ShadeOut so;
                 so.emit = false;
-                shade_surface(a.sc, pi, hit, ray, a.halton, st, cnt, so, nullptr);
-                a.rec[(size_t)nv * a.lanes + i] = make_float4(so.col.x, so.col.y, so.col.z, __int_as_float(so.alb_mat));
+                NeeSink sink;
+                if (Defer) sink = NeeSink{a.nee_ray, a.nee_aux, a.nee_term, i * a.nee_slots, 0};
+                shade_surface(a.sc, pi, hit, ray, a.halton, st, cnt, so, Defer ? &sink : nullptr);
+                f3 cj = f3{so.col.x, so.col.y, so.col.z};
+                if (Defer && so.nee_pending && sink.count > 0)
+                    cj = cj + resolve_light_samples(a.sc, a.halton, sink, so, ray.c, st, cnt);
+                a.rec[(size_t)nv * a.lanes + i] = make_float4(cj.x, cj.y, cj.z, __int_as_float(so.alb_mat));
                 nv++;
                 b++;
                 if (!so.emit || skip_indirect || b >= last) break;
@@ -410,7 +420,8 @@ struct rh_context {
     int sort_rays = 1;
     int sort_bits = 32; // key bits the in-pass ray sort orders by (top bits)
     int path_tail = 2;  // first bounce run by the path-tail kernel (-1: none)
-    int tail_occupancy = 2; // waves per SIMD the path-tail kernel is compiled for (2 or 3)
+    int tail_occupancy = 2;    // waves per SIMD the path-tail kernel is compiled for (2 or 3)
+    int adjoint_occupancy = 2; // the same for the adjoint replay kernel
     int stage_timing = 0;
     int lds_depth = 24;
     rh_stats stats = {};
@@ -1006,6 +1017,9 @@ int rh_set_option(rh_context *c, const char *name, int value) {
     } else if (n == "tail_occupancy") {
         if (value != 2 && value != 3) return arg_error("tail_occupancy must be 2 or 3");
         c->tail_occupancy = value;
+    } else if (n == "adjoint_occupancy") {
+        if (value != 2 && value != 3) return arg_error("adjoint_occupancy must be 2 or 3");
+        c->adjoint_occupancy = value;
     } else if (n == "sort_bits") {
         if (value != 8 && value != 16 && value != 24 && value != 32) return arg_error("sort_bits must be 8, 16, 24 or 32");
         c->sort_bits = value;
@@ -1021,6 +1035,7 @@ int rh_get_option(rh_context *c, const char *name, int *value) {
     else if (n == "sort_bits") *value = c->sort_bits;
     else if (n == "path_tail") *value = c->path_tail;
     else if (n == "tail_occupancy") *value = c->tail_occupancy;
+    else if (n == "adjoint_occupancy") *value = c->adjoint_occupancy;
     else if (n == "defer_nee") *value = c->defer_nee;
     else if (n == "stack_overflows") {
         unsigned long long v[8];
@@ -1276,15 +1291,19 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
     if (!d_adj_rgba || !d_grad) return arg_error("adjoint image and gradient buffers are required");
     const int depth = std::min<int>(p->cam.pass_settings.max_total_depth, RH_MAX_BOUNCES);
     const size_t lanes = (size_t)rw * rhh;
-    const size_t max_vertices = (size_t)depth + 1;
+    // vertex records: at most depth + 1 <= RH_MAX_BOUNCES + 1 per lane
     if (lanes > c->adj_rec_cap || !c->adj_rec) {
         free_ptr(c->adj_rec);
         c->adj_rec = nullptr;
         c->adj_rec_cap = lanes;
         RH_CHECK(hipMalloc(&c->adj_rec, (RH_MAX_BOUNCES + 1) * lanes * sizeof(float4)));
     }
-    (void)max_vertices;
-    int rc = ensure_spill(c, lanes);
+    const bool defer = c->defer_nee && c->nee_slots;
+    // NEE slots for every lane when deferred (this may grow the spill array,
+    // so it comes before any buffer pointer is taken)
+    int rc = defer ? ensure_rays(c, lanes) : 0;
+    if (rc) return rc;
+    rc = ensure_spill(c, lanes);
     if (rc) return rc;
     AdjointArgs a;
     memset(&a, 0, sizeof(a));
@@ -1306,9 +1325,21 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
     a.ray_counter = c->stats_dev + 5;
     a.shadow_counter = c->stats_dev + 6;
     a.overflow = c->stats_dev + 4;
+    if (defer) {
+        a.nee_ray = c->nee_ray;
+        a.nee_aux = c->nee_aux;
+        a.nee_term = c->nee_term;
+        a.nee_slots = c->nee_slots;
+    }
     KernelMark km(c, 4);
-    hipLaunchKernelGGL(k_adjoint, dim3((unsigned)((lanes + kBlock - 1) / kBlock)), dim3(kBlock), lds_bytes(c),
-                       c->stream, a);
+    const dim3 grid((unsigned)((lanes + kBlock - 1) / kBlock)), blk(kBlock);
+    if (defer) {
+        if (c->adjoint_occupancy == 3) hipLaunchKernelGGL((k_adjoint<true, 3>), grid, blk, lds_bytes(c), c->stream, a);
+        else hipLaunchKernelGGL((k_adjoint<true, 1>), grid, blk, lds_bytes(c), c->stream, a);
+    } else {
+        if (c->adjoint_occupancy == 3) hipLaunchKernelGGL((k_adjoint<false, 3>), grid, blk, lds_bytes(c), c->stream, a);
+        else hipLaunchKernelGGL((k_adjoint<false, 1>), grid, blk, lds_bytes(c), c->stream, a);
+    }
     RH_CHECK(hipGetLastError());
     return 0;
 }

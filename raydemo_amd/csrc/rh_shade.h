@@ -518,4 +518,28 @@ __device__ __forceinline__ void shade_surface(const Scene &sc, const rh_pass_inf
     out.col = f4{ray.c.x * col.x, ray.c.y * col.y, ray.c.z * col.z, 1.0f};
 }
 
+// The direct light a deferred sink recorded for one shaded ray, resolved in
+// the same lane: visibility of each sample (ComputeVisibility), the visible
+// terms summed in light order, times the albedo when considered, times the
+// ray's throughput -- the arithmetic of the separate shadow and combine
+// stages, so adding the result after out.col gives the same pixel value.
+__device__ __forceinline__ f3 resolve_light_samples(const Scene &sc, const float *halton, const NeeSink &sink,
+                                                    const ShadeOut &out, f3 throughput, Stack &st,
+                                                    ShadeCounters &cnt) {
+    f3 col = f3{0.0f, 0.0f, 0.0f};
+    for (uint32_t j = 0; j < sink.count; j++) {
+        const uint32_t k = sink.base + j;
+        const float4 r = sink.ray[k];
+        const int2 x = sink.aux[k];
+        const rh_light &l = sc.lights[__float_as_uint(r.w)];
+        st.sp = 0;
+        if (compute_visibility(sc, f3{r.x, r.y, r.z}, ld3(l.pos), halton, x.x, x.y, st, cnt) != 0.0f) {
+            const float4 t = sink.term[k];
+            col = col + f3{t.x, t.y, t.z};
+        }
+    }
+    if (out.consider_albedo) col = col * out.albedo;
+    return f3{throughput.x * col.x, throughput.y * col.y, throughput.z * col.z};
+}
+
 } // namespace rh

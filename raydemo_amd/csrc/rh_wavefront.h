@@ -503,20 +503,7 @@ __device__ __forceinline__ void path_tail_body(const ShadeArgs &a, int first, in
             v.w += 0.0f;
             dst = v;
             if (Defer && so.nee_pending && sink.count > 0) {
-                f3 col = f3{0.0f, 0.0f, 0.0f};
-                for (uint32_t j = 0; j < sink.count; j++) {
-                    const uint32_t k = sink.base + j;
-                    const float4 r = a.nee_ray[k];
-                    const int2 x = a.nee_aux[k];
-                    const rh_light &l = a.sc.lights[__float_as_uint(r.w)];
-                    st.sp = 0;
-                    if (compute_visibility(a.sc, f3{r.x, r.y, r.z}, ld3(l.pos), a.halton, x.x, x.y, st, cnt) != 0.0f) {
-                        const float4 t = a.nee_term[k];
-                        col = col + f3{t.x, t.y, t.z};
-                    }
-                }
-                if (so.consider_albedo) col = col * so.albedo;
-                const f3 add = f3{ray.c.x * col.x, ray.c.y * col.y, ray.c.z * col.z};
+                const f3 add = resolve_light_samples(a.sc, a.halton, sink, so, ray.c, st, cnt);
                 float4 w = dst;
                 w.x += add.x;
                 w.y += add.y;
