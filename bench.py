@@ -332,7 +332,7 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--cpu-spp", type=int, default=64, help="full-frame passes of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tune-passes", type=int, default=6, help="passes per autotuner candidate")
+    ap.add_argument("--tune-passes", type=int, default=4, help="passes per autotuner candidate")
     ap.add_argument("--knobs", default="",
                     help="skip the autotuner: lds_stack_depth,sort_rays,sort_bits,path_tail[,tail_occupancy]")
     ap.add_argument("--only-tuned", action="store_true", help="time only the tuned (or given) knobs")

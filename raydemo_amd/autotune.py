@@ -29,7 +29,8 @@ SORT_MODES = ((0, 32), (1, 8), (1, 32))
 # first bounce of the path-tail launch (-1: every bounce staged)
 PATH_TAILS = (-1, 0, 1, 2, 3, 4)
 TAIL_OCCUPANCY = (2, 3)
-KNOBS = ("lds_stack_depth", "sort_rays", "sort_bits", "path_tail", "tail_occupancy")
+REFILL = (0, 1)
+KNOBS = ("lds_stack_depth", "sort_rays", "sort_bits", "path_tail", "tail_occupancy", "refill")
 
 
 @dataclass
@@ -46,7 +47,8 @@ def candidates(depths=LDS_DEPTHS, sort_modes=SORT_MODES, tails=PATH_TAILS, occup
     for d, (so, sb), pt, occ in itertools.product(depths, sort_modes, tails, occupancies):
         if pt < 0 and occ != occupancies[0]:
             continue  # no tail launch: occupancy is moot
-        yield {"lds_stack_depth": d, "sort_rays": so, "sort_bits": sb, "path_tail": pt, "tail_occupancy": occ}
+        yield {"lds_stack_depth": d, "sort_rays": so, "sort_bits": sb, "path_tail": pt, "tail_occupancy": occ,
+               "refill": 0}
 
 
 def apply(renderer, cfg: dict):
@@ -89,6 +91,12 @@ def tune(renderer, scene, region_factory, passes: int = 6, depths=LDS_DEPTHS, so
                 return TuneResult(best=hit["best"], table=hit.get("table", []), seconds=0.0)
     t0 = time.perf_counter()
     table = [measure(renderer, scene, region_factory, cfg, passes) for cfg in candidates(depths, sort_modes, tails)]
+    # the refill trace only as a second stage, on the best setting so far
+    first = max(table, key=lambda r: r["mrays"])
+    for rf in REFILL[1:]:
+        cfg = {k: first[k] for k in KNOBS}
+        cfg["refill"] = rf
+        table.append(measure(renderer, scene, region_factory, cfg, passes))
     best_row = max(table, key=lambda r: r["mrays"])
     best = {k: best_row[k] for k in KNOBS}
     apply(renderer, best)

@@ -41,6 +41,12 @@ template <bool AnyHit, bool Count> __global__ void __launch_bounds__(kBlock) k_t
     trace_body<AnyHit, Count>(a, blockIdx.x * blockDim.x, lds_stack);
 }
 
+// persistent closest-hit trace with lane refill (trace_refill_body)
+template <bool Count> __global__ void __launch_bounds__(kBlock) k_trace_refill(TraceArgs a, int *next_ray) {
+    extern __shared__ uint32_t lds_stack[];
+    trace_refill_body<Count>(a, next_ray, lds_stack);
+}
+
 // Defer: direct light goes to the NEE stage (no traversal in this kernel).
 template <bool Defer> __global__ void __launch_bounds__(kBlock) k_shade(ShadeArgs a) {
     extern __shared__ uint32_t lds_stack[];
@@ -404,7 +410,7 @@ struct rh_context {
     float *color_stage = nullptr;
     size_t color_stage_cap = 0; // floats
     uint32_t n_materials = 0;
-    int *counters = nullptr; // [0], [1]: ray queue counts, [2]: shadow queue count
+    int *counters = nullptr; // [0], [1]: ray queue counts, [2]: shadow queue count, [3]: refill fetch
     unsigned long long *stats_dev = nullptr; // [0] primary [1] secondary [2] shadow [3] sorted [4] overflow
     float *halton = nullptr;
 
@@ -421,6 +427,9 @@ struct rh_context {
     int sort_bits = 32; // key bits the in-pass ray sort orders by (top bits)
     int path_tail = 2;  // first bounce run by the path-tail kernel (-1: none)
     int tail_occupancy = 2;    // waves per SIMD the path-tail kernel is compiled for (2 or 3)
+    int refill = 0;            // closest-hit trace: persistent kernel with lane refill
+    int cus = 0;               // compute units of the device
+    int refill_blocks = 0, refill_blocks_lds = -1;
     int adjoint_occupancy = 2; // the same for the adjoint replay kernel
     int stage_timing = 0;
     int lds_depth = 24;
@@ -488,6 +497,7 @@ int ensure_rays(rh_context *c, size_t n) {
 // Spill slices are indexed by lane (ray or shadow slot): the array holds
 // kSpillDepth entries for each of spill_cap lanes, stride spill_cap.
 int ensure_spill(rh_context *c, size_t lanes) {
+    lanes = (lanes + kBlock - 1) / kBlock * kBlock; // whole blocks: persistent lanes index slices too
     if (lanes <= c->spill_cap && c->spill) return 0;
     free_ptr(c->spill);
     c->spill = nullptr;
@@ -576,8 +586,29 @@ struct KernelMark {
 
 int launch_trace(rh_context *c, const TraceArgs &a, int grid_rays, bool any_hit) {
     const int blocks = std::max(1, (grid_rays + kBlock - 1) / kBlock);
-    KernelMark km(c, 0);
     const bool count = c->count_visits && a.visits;
+    if (!any_hit && c->refill) {
+        // persistent grid: as many blocks as fit on the device at once (and no
+        // more than the rays need); spill slices exist for every lane of them
+        if (c->refill_blocks_lds != c->lds_depth) {
+            int per_cu = 0;
+            RH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_refill<false>, kBlock, lds_bytes(c)));
+            c->refill_blocks = std::max(1, per_cu) * c->cus;
+            c->refill_blocks_lds = c->lds_depth;
+        }
+        const int grid = std::min(blocks, c->refill_blocks);
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, c->counters + 3, 0);
+        KernelMark km(c, 0);
+        if (count)
+            hipLaunchKernelGGL((k_trace_refill<true>), dim3(grid), dim3(kBlock), lds_bytes(c), c->stream, a,
+                               c->counters + 3);
+        else
+            hipLaunchKernelGGL((k_trace_refill<false>), dim3(grid), dim3(kBlock), lds_bytes(c), c->stream, a,
+                               c->counters + 3);
+        RH_CHECK(hipGetLastError());
+        return 0;
+    }
+    KernelMark km(c, 0);
     if (any_hit) {
         if (count) hipLaunchKernelGGL((k_trace<true, true>), dim3(blocks), dim3(kBlock), lds_bytes(c), c->stream, a);
         else hipLaunchKernelGGL((k_trace<true, false>), dim3(blocks), dim3(kBlock), lds_bytes(c), c->stream, a);
@@ -736,6 +767,9 @@ int rh_create(int device, rh_context **out) {
         rh_rt::err() = "hipStreamCreate failed";
         return -1003;
     }
+    hipDeviceProp_t prop;
+    RH_CHECK(hipGetDeviceProperties(&prop, device));
+    c->cus = prop.multiProcessorCount;
     RH_CHECK(hipMalloc(&c->counters, 16 * sizeof(int)));
     RH_CHECK(hipMalloc(&c->stats_dev, 8 * sizeof(unsigned long long)));
     RH_CHECK(hipMemsetAsync(c->counters, 0, 16 * sizeof(int), c->stream));
@@ -1017,6 +1051,8 @@ int rh_set_option(rh_context *c, const char *name, int value) {
     } else if (n == "tail_occupancy") {
         if (value != 2 && value != 3) return arg_error("tail_occupancy must be 2 or 3");
         c->tail_occupancy = value;
+    } else if (n == "refill") {
+        c->refill = value != 0;
     } else if (n == "adjoint_occupancy") {
         if (value != 2 && value != 3) return arg_error("adjoint_occupancy must be 2 or 3");
         c->adjoint_occupancy = value;
@@ -1035,6 +1071,7 @@ int rh_get_option(rh_context *c, const char *name, int *value) {
     else if (n == "sort_bits") *value = c->sort_bits;
     else if (n == "path_tail") *value = c->path_tail;
     else if (n == "tail_occupancy") *value = c->tail_occupancy;
+    else if (n == "refill") *value = c->refill;
     else if (n == "adjoint_occupancy") *value = c->adjoint_occupancy;
     else if (n == "defer_nee") *value = c->defer_nee;
     else if (n == "stack_overflows") {

@@ -294,6 +294,166 @@ __device__ __forceinline__ void trace_body(const TraceArgs &a, uint32_t base, ui
     h[1] = make_float4(hit.u, hit.v, r0.w, 0.0f);
 }
 
+// Closest-hit trace with lane refill: a persistent grid walks the queue and a
+// lane whose ray is finished takes the next one while the rest of its wave is
+// still traversing, so the wave does not idle until its slowest ray ends.
+// The traversal of every ray is traverse_fast's, cut into single steps (one
+// interior-node pair, one instance, one leaf or one pop sequence per loop
+// iteration), so each ray visits the same nodes in the same order and gets
+// the same hit.  Lanes fetch new rays in wave batches (one atomic per batch)
+// once a quarter of the wave is idle, or when every lane is.
+enum : uint32_t { kStepMacro = 0, kStepInstances = 1, kStepMicro = 2, kStepIdle = 3 };
+
+template <bool Count>
+__device__ __forceinline__ void trace_refill_body(const TraceArgs &a, int *next_ray, uint32_t *lds_stack) {
+    const int n = a.count ? *a.count : a.n;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.ray_counter) atomicAdd(a.ray_counter, (unsigned long long)n);
+    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x; // persistent lane: owns spill slice gl
+    Stack st;
+    st.lds = lds_stack;
+    st.spill = a.spill + gl;
+    st.spill_stride = a.spill_stride;
+    st.lds_depth = a.lds_depth;
+    st.spill_depth = kSpillDepth;
+    st.lane = threadIdx.x;
+    st.block = blockDim.x;
+    st.sp = 0;
+    st.overflow = 0;
+    TraceCounters tc = {0, 0, 0};
+    TraceCounters *cnt = Count ? &tc : nullptr;
+
+    int r = -1;                  // ray of this lane
+    uint32_t step = kStepIdle;
+    f3 o = f3{0, 0, 0}, d = o, inv_d = o;
+    float xy = 0.0f;
+    Hit hit;
+    hit.mask = 0;
+    hit.obj = hit.prim = -1;
+    hit.t = 0.0f;
+    hit.u = hit.v = 0.0f;
+    NodeBox node;                // entered node (macro or micro)
+    bool have = false;           // micro: node is entered (else pop next)
+    uint32_t leaf_cur = 0, leaf_end = 0, base = 0;
+    f3 mo = o, md = o, minv = o;
+    int mi_index = 0;
+    bool finite = true;
+    bool exhausted = false;      // the queue has no rays left for this wave
+
+    for (;;) {
+        // ---- refill
+        const uint64_t idle = __ballot(step == kStepIdle);
+        const uint64_t live = __ballot(true);
+        if (!exhausted && idle && (__popcll(idle) >= 16 || idle == live)) {
+            int first = 0;
+            const int want = __popcll(idle);
+            if (lane_id() == (uint32_t)(__ffsll((unsigned long long)idle) - 1)) first = atomicAdd(next_ray, want);
+            first = __shfl(first, __ffsll((unsigned long long)idle) - 1);
+            if (first + want >= n) exhausted = true;
+            if (step == kStepIdle) {
+                r = first + lanes_below(idle);
+                if (r < n) {
+                    const float4 *p = reinterpret_cast<const float4 *>(a.rays + r);
+                    const float4 r0 = p[0], r1 = p[1];
+                    o = f3{r0.x, r0.y, r0.z};
+                    d = f3{r1.x, r1.y, r1.z};
+                    xy = r0.w;
+                    inv_d = safe_invert(d);
+                    hit.mask = 0;
+                    hit.obj = -1;
+                    hit.prim = -1;
+                    hit.t = a.tmax ? a.tmax[r] : RH_MAX_DIST;
+                    hit.u = hit.v = 0.0f;
+                    st.sp = 0;
+                    step = kStepMacro;
+                    if (a.sc.macro_root == RH_INVALID) {
+                        step = kStepIdle; // empty scene: the miss is written below
+                    } else {
+                        node = load_node(a.sc.nodes, a.sc.macro_root);
+                        if (cnt) cnt->nodes++;
+                        if (!box_hit<true>(o, inv_d, hit.t, node.bmin, node.bmax)) step = kStepIdle;
+                    }
+                    if (step == kStepIdle) {
+                        float4 *h = reinterpret_cast<float4 *>(a.hits + r);
+                        h[0] = make_float4(__int_as_float(hit.mask), __int_as_float(hit.obj), __int_as_float(hit.prim),
+                                           hit.t);
+                        h[1] = make_float4(hit.u, hit.v, xy, 0.0f);
+                        if (Count) atomicAdd(a.visits + 3, 1ull);
+                    }
+                }
+            }
+        }
+        if (!__ballot(step != kStepIdle)) {
+            if (exhausted) break;
+            continue;
+        }
+        // ---- one traversal step
+        bool finished = false;
+        if (step == kStepMacro) {
+            if (!(node.a & RH_LEAF_NODE_BIT)) {
+                if (!enter_children<true>(a.sc, node, o, d, inv_d, hit.t, st, node, cnt) &&
+                    !pop_entered<true>(a.sc, 0, o, inv_d, hit.t, st, node, cnt))
+                    finished = true;
+            } else {
+                leaf_cur = node.a & RH_PRIM_INDEX_BITS;
+                leaf_end = leaf_cur + node.b;
+                step = kStepInstances;
+            }
+        } else if (step == kStepInstances) {
+            if (leaf_cur < leaf_end) {
+                mi_index = (int)a.sc.mi_indices[leaf_cur++];
+                const rh_mesh_instance mi = a.sc.mesh_instances[mi_index];
+                if (cnt) cnt->instances++;
+                if (box_hit<true>(o, inv_d, hit.t, mi.bbox_min, mi.bbox_max)) {
+                    const float *x = a.sc.transforms[mi.tr_index].inv_xform;
+                    mo = f3{x[0] * o.x + x[4] * o.y + x[8] * o.z + x[12], x[1] * o.x + x[5] * o.y + x[9] * o.z + x[13],
+                            x[2] * o.x + x[6] * o.y + x[10] * o.z + x[14]};
+                    md = f3{x[0] * d.x + x[4] * d.y + x[8] * d.z, x[1] * d.x + x[5] * d.y + x[9] * d.z,
+                            x[2] * d.x + x[6] * d.y + x[10] * d.z};
+                    minv = f3{1.0f / md.x, 1.0f / md.y, 1.0f / md.z};
+                    finite = __builtin_isfinite(minv.x) && __builtin_isfinite(minv.y) && __builtin_isfinite(minv.z);
+                    base = st.sp;
+                    node = load_node(a.sc.nodes, a.sc.meshes[mi.mesh_index].node_index);
+                    if (cnt) cnt->nodes++;
+                    have = finite ? box_hit<true>(mo, minv, hit.t, node.bmin, node.bmax)
+                                  : box_hit<false>(mo, minv, hit.t, node.bmin, node.bmax);
+                    step = kStepMicro;
+                }
+            } else if (pop_entered<true>(a.sc, 0, o, inv_d, hit.t, st, node, cnt)) {
+                step = kStepMacro;
+            } else {
+                finished = true;
+            }
+        } else if (step == kStepMicro) {
+            if (have) {
+                if (!(node.a & RH_LEAF_NODE_BIT)) {
+                    have = finite ? enter_children<true>(a.sc, node, mo, md, minv, hit.t, st, node, cnt)
+                                  : enter_children<false>(a.sc, node, mo, md, minv, hit.t, st, node, cnt);
+                } else {
+                    intersect_leaf<false>(a.sc, node, mo, md, mi_index, hit, cnt);
+                    have = false;
+                }
+            } else {
+                have = finite ? pop_entered<true>(a.sc, base, mo, minv, hit.t, st, node, cnt)
+                              : pop_entered<false>(a.sc, base, mo, minv, hit.t, st, node, cnt);
+                if (!have) step = kStepInstances;
+            }
+        }
+        if (finished) {
+            float4 *h = reinterpret_cast<float4 *>(a.hits + r);
+            h[0] = make_float4(__int_as_float(hit.mask), __int_as_float(hit.obj), __int_as_float(hit.prim), hit.t);
+            h[1] = make_float4(hit.u, hit.v, xy, 0.0f);
+            if (Count) atomicAdd(a.visits + 3, 1ull);
+            step = kStepIdle;
+        }
+    }
+    if (st.overflow && a.overflow) atomicAdd(a.overflow, (unsigned long long)st.overflow);
+    if (Count) {
+        atomicAdd(a.visits + 0, (unsigned long long)tc.nodes);
+        atomicAdd(a.visits + 1, (unsigned long long)tc.leaf_tris);
+        atomicAdd(a.visits + 2, (unsigned long long)tc.instances);
+    }
+}
+
 struct ShadeArgs {
     Scene sc;
     const rh_ray *rays;

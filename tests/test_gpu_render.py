@@ -73,7 +73,7 @@ def test_soup_matches_reference_small(ref_lib, host_lib):
 
 
 @pytest.mark.parametrize("opts", [{}, {"sort_rays": 0, "lds_stack_depth": 8, "path_tail": -1}, {"defer_nee": 0},
-                                  {"path_tail": 0}, {"path_tail": 1, "tail_occupancy": 3}])
+                                  {"path_tail": 0}, {"path_tail": 1, "tail_occupancy": 3}, {"refill": 1}])
 def test_ray_counts_match_oracle(oracle, host_lib, opts):
     """The device's primary/secondary/shadow counters (the numerator of Mrays/s)
     equal the CPU oracle's counts over the same passes."""

@@ -37,8 +37,9 @@ def _check_hits(h_gpu, h_ref, what):
     return m.sum()
 
 
+@pytest.mark.parametrize("refill", [0, 1])
 @pytest.mark.parametrize("scene_name", ["mesh_lights", "cornell_box", "soup_small"])
-def test_trace_primary_and_secondary_bit_exact(ref_lib, host_lib, dev_lib, golden_scene1, scene_name):
+def test_trace_primary_and_secondary_bit_exact(ref_lib, host_lib, dev_lib, golden_scene1, scene_name, refill):
     build = {
         "mesh_lights": lambda s: scenes.mesh_lights(s, golden_scene1),
         "cornell_box": lambda s: scenes.cornell_box(s),
@@ -51,6 +52,7 @@ def test_trace_primary_and_secondary_bit_exact(ref_lib, host_lib, dev_lib, golde
     rays = ref_gen_primary(ref_lib, cam, (0, 0, w, h), w, h, 1, halton)
 
     ctx = Context(dev_lib, w=w, h=h)
+    ctx.set_option("refill", refill)
     ctx.upload(hs.raw_view())
     hg = ctx.trace(rays)
     hr = ref_trace(ref_lib, rs, rays)
