@@ -98,9 +98,13 @@ int rh_shade_rays(rh_context *ctx, const rh_pass_info *pi, const rh_ray_packet *
  * rh_adjoint_pass replays one pass (same camera, rect, iteration and Halton
  * table as the rh_render_pass it differentiates) and adds, for every material,
  *   scale * sum_pixels adj[pixel] . d(pass colour of pixel) / d main_color
- * to d_grad (4 floats per material: r, g, b, unused).  d_adj_rgba is the
- * adjoint of the pass image (w*h RGBA floats, alpha unused).  Both are device
- * pointers; the call is asynchronous on the context's stream (rh_synchronize).
+ * to d_grad (4 floats per material: r, g, b, unused), and, when d_texel_grad is
+ * not null, the same for every stored texel of the albedo textures (4 floats
+ * per texel of the scene's texel array, r, g, b per channel value in [0, 1],
+ * alpha unused; a block's wrap border and the mip levels are separate stored
+ * texels: folding them onto the level-0 texels is the caller's).  d_adj_rgba
+ * is the adjoint of the pass image (w*h RGBA floats, alpha unused).  All are
+ * device pointers; the call is asynchronous on the context's stream.
  * Paths are treated with their sampling decisions fixed: Russian roulette
  * weights are not differentiated.
  * rh_set_material_colors overwrites main_color of materials [first, first+n)
@@ -108,8 +112,11 @@ int rh_shade_rays(rh_context *ctx, const rh_pass_info *pi, const rh_ray_packet *
  * rh_read_accum copies the linear accumulated image (the Ref renderer's
  * "clean" framebuffer, FramebufferRef.cpp:80-86, before tonemapping; w*h RGBA
  * floats) to dst (host or device pointer). */
-int rh_adjoint_pass(rh_context *ctx, const rh_pass_desc *pass, const float *d_adj_rgba, float *d_grad, float scale);
+int rh_adjoint_pass(rh_context *ctx, const rh_pass_desc *pass, const float *d_adj_rgba, float *d_grad,
+                    float *d_texel_grad, float scale);
 int rh_set_material_colors(rh_context *ctx, uint32_t first, uint32_t n, const float *rgb);
+/* overwrites stored texels [first, first+n) of the uploaded scene (RGBA8 words) */
+int rh_set_texels(rh_context *ctx, uint32_t first, uint32_t n, const uint32_t *rgba);
 int rh_read_accum(rh_context *ctx, float *dst, size_t count);
 
 /* ---- batching (no counterpart in the reference: the many-scene extension) ----

@@ -141,13 +141,17 @@ int rhx_render_scene(rhx_renderer *r, rhx_scene *s, rhx_region *reg);
 /* Differentiation (HIP backend only; no counterpart in the reference, see
  * rh_adjoint_pass in rh_hip.h).  rhx_adjoint_scene advances reg exactly as
  * rhx_render_scene would and replays that pass, adding scale * d loss /
- * d main_color (4 floats per material) to d_grad for the adjoint image
- * d_adj_rgba (device pointers).  rhx_set_material_colors sets main_color of
- * materials [first, first+n) from host rgb triples, in the scene and in its
- * device copy.  rhx_read_accum copies the linear accumulated image (w*h RGBA). */
+ * d main_color (4 floats per material) to d_grad, and d loss / d stored
+ * texel to d_texel_grad when not null (4 floats per texel), for the adjoint
+ * image d_adj_rgba (device pointers).  rhx_set_material_colors sets
+ * main_color of materials [first, first+n) from host rgb triples, and
+ * rhx_set_texture_data a texture's level-0 RGBA8 texels (its bordered blocks
+ * and mips rebuilt), in the scene and in its device copy.  rhx_read_accum
+ * copies the linear accumulated image (w*h RGBA). */
 int rhx_adjoint_scene(rhx_renderer *r, rhx_scene *s, rhx_region *reg, const float *d_adj_rgba, float *d_grad,
-                      float scale);
+                      float *d_texel_grad, float scale);
 int rhx_set_material_colors(rhx_renderer *r, rhx_scene *s, uint32_t first, uint32_t n, const float *rgb);
+int rhx_set_texture_data(rhx_renderer *r, rhx_scene *s, uint32_t tex, const uint8_t *rgba);
 int rhx_read_accum(rhx_renderer *r, float *dst, size_t count);
 
 /* Batching (HIP backend only; no counterpart in the reference, see

@@ -258,18 +258,27 @@ void macro(const rh_scene_view &sc, const float o[3], const float d[3], bool any
 }
 
 // ------------------------------------------------------------------ textures
-uint32_t texel(const rh_scene_view &sc, uint32_t tex, int lod, int x, int y) {
+uint32_t texel_index(const rh_scene_view &sc, uint32_t tex, int lod, int x, int y) {
     const rh_texture &t = sc.textures[tex];
     const rh_tex_mip &m = sc.tex_mips[tex * RH_NUM_MIP_LEVELS + lod];
     const int lx = clampi(x - (int)t.pos[lod][0], 0, m.bw - 1), ly = clampi(y - (int)t.pos[lod][1], 0, m.bh - 1);
-    return sc.texels[m.offset + (uint32_t)ly * m.bw + (uint32_t)lx];
+    return m.offset + (uint32_t)ly * m.bw + (uint32_t)lx;
 }
+uint32_t texel(const rh_scene_view &sc, uint32_t tex, int lod, int x, int y) {
+    return sc.texels[texel_index(sc, tex, lod, x, y)];
+}
+
+// the stored texels a bilinear sample read (00, 01, 10, 11) and its weights
+struct Foot {
+    uint32_t idx[4];
+    float kx, ky;
+};
 inline float ch(uint32_t p, int c) { return (float)((p >> (8 * c)) & 0xff); }
 // generic simd_vec floor (simd_vec.h:408-412)
 inline float vfloor(float v) { return (float)((int)v - (v < 0.0f)); }
 
 // SampleBilinear + TransformUV (CoreRef.cpp:1625-1678)
-void bilinear(const rh_scene_view &sc, uint32_t tex, float u, float v, int lod, float out[4]) {
+void bilinear(const rh_scene_view &sc, uint32_t tex, float u, float v, int lod, float out[4], Foot *foot = nullptr) {
     const rh_texture &t = sc.textures[tex];
     const float A = (float)RH_TEXTURE_ATLAS_SIZE;
     const float sx = (float)((t.width & RH_TEXTURE_WIDTH_BITS) >> lod), sy = (float)(t.height >> lod);
@@ -281,6 +290,11 @@ void bilinear(const rh_scene_view &sc, uint32_t tex, float u, float v, int lod, 
     const uint32_t p00 = texel(sc, tex, lod, (int)x, (int)y), p01 = texel(sc, tex, lod, (int)x + 1, (int)y);
     const uint32_t p10 = texel(sc, tex, lod, (int)x, (int)y + 1), p11 = texel(sc, tex, lod, (int)x + 1, (int)y + 1);
     const float kx = x - std::floor(x), ky = y - std::floor(y);
+    if (foot) {
+        *foot = Foot{{texel_index(sc, tex, lod, (int)x, (int)y), texel_index(sc, tex, lod, (int)x + 1, (int)y),
+                      texel_index(sc, tex, lod, (int)x, (int)y + 1), texel_index(sc, tex, lod, (int)x + 1, (int)y + 1)},
+                     kx, ky};
+    }
     for (int c = 0; c < 4; c++) {
         const float a = ch(p01, c) * kx + ch(p00, c) * (1 - kx), b = ch(p11, c) * kx + ch(p10, c) * (1 - kx);
         out[c] = (b * ky + a * (1.0f - ky)) * (1.0f / 255.0f);
@@ -551,8 +565,14 @@ inline float pow5(float v) { return (v * v) * (v * v) * v; }
 // alb_mat (optional): the material whose main_color scales this vertex's
 // colour and the rest of the path once (emissive; diffuse and glossy when the
 // albedo is considered), or -1
+struct AlbedoSample {
+    Foot foot;
+    float raw[3]; // before the sRGB conversion
+    bool srgb;
+};
+
 void shade(Ctx &c, const rh_pass_info &pi, const rh_hit &inter, const rh_ray_packet &ray, float out[4],
-           rh_ray_packet *sec, bool &emitted, int *alb_mat = nullptr) {
+           rh_ray_packet *sec, bool &emitted, int *alb_mat = nullptr, AlbedoSample *alb_sample = nullptr) {
     const rh_scene_view &sc = c.sc;
     const float *halton = c.halton;
     emitted = false;
@@ -639,7 +659,11 @@ void shade(Ctx &c, const rh_pass_info &pi, const rh_hit &inter, const rh_ray_pac
     const uint32_t mt = mat->textures[RH_MAIN_TEXTURE];
     const float lod = texture_lod(sc.textures[mt], der.duv_dx, der.duv_dy);
     float alb[4];
-    bilinear(sc, mt, uvs[0], uvs[1], (int)lod, alb);
+    bilinear(sc, mt, uvs[0], uvs[1], (int)lod, alb, alb_sample ? &alb_sample->foot : nullptr);
+    if (alb_sample) {
+        for (int i = 0; i < 3; i++) alb_sample->raw[i] = alb[i];
+        alb_sample->srgb = (sc.textures[mt].width & RH_TEXTURE_SRGB_BIT) != 0;
+    }
     if (sc.textures[mt].width & RH_TEXTURE_SRGB_BIT) {
         for (int i = 0; i < 3; i++) alb[i] = alb[i] > 0.04045f ? std::pow((alb[i] + 0.055f) / 1.055f, 2.4f) : alb[i] / 12.92f;
     }
@@ -1018,7 +1042,7 @@ int orc_render(const rh_scene_view *sc, const rh_camera *cam, int w, int h, cons
 // S_m the suffix sum of the path's colours from the vertex on, accumulated here
 // in double: grad[3 m + c] += scale * adj[pixel][c] * S_m[c] / main_color[m][c].
 int orc_adjoint(const rh_scene_view *sc, const rh_camera *cam, int w, int h, const int32_t rect[4], int passes,
-                const float *adj_rgba, double scale, double *grad) {
+                const float *adj_rgba, double scale, double *grad, double *texel_grad) {
     static const std::vector<uint16_t> pm = perms();
     std::vector<float> halton(RH_HALTON_COUNT * RH_HALTON_SEQ_LEN);
     int have_halton = -1;
@@ -1028,6 +1052,7 @@ int orc_adjoint(const rh_scene_view *sc, const rh_camera *cam, int w, int h, con
     struct Vtx {
         int mat;
         float c[3];
+        AlbedoSample as;
     };
     std::vector<std::vector<Vtx>> path((size_t)w * h);
     for (int it = 1; it <= passes; it++) {
@@ -1055,10 +1080,11 @@ int orc_adjoint(const rh_scene_view *sc, const rh_camera *cam, int w, int h, con
                 pi.rand_index = pi.index;
                 bool e;
                 float col[4];
-                int m;
-                shade(c, pi, hits[i], rays[i], col, &sec[count], e, &m);
+                Vtx vx;
+                shade(c, pi, hits[i], rays[i], col, &sec[count], e, &vx.mat, &vx.as);
                 if (e) count++;
-                path[(size_t)pi.index].push_back(Vtx{m, {col[0], col[1], col[2]}});
+                for (int k = 0; k < 3; k++) vx.c[k] = col[k];
+                path[(size_t)pi.index].push_back(vx);
             }
         };
         shade_all((int)n);
@@ -1080,6 +1106,28 @@ int orc_adjoint(const rh_scene_view *sc, const rh_camera *cam, int w, int h, con
                     for (int ch = 0; ch < 3; ch++) {
                         const double mc = (double)sc->materials[m].main_color[ch];
                         if (mc != 0.0) grad[3 * (size_t)m + ch] += scale * (double)adj_rgba[4 * p + ch] * S[ch] / mc;
+                    }
+                    if (texel_grad) {
+                        // albedo = lin(raw) * main_color, raw = sum of texel * weight
+                        const AlbedoSample &as = v[k].as;
+                        const double kx = as.foot.kx, ky = as.foot.ky;
+                        const double wq[4] = {(1 - kx) * (1 - ky), kx * (1 - ky), (1 - kx) * ky, kx * ky};
+                        for (int ch = 0; ch < 3; ch++) {
+                            const double r = as.raw[ch];
+                            double lin = r, dlin = 1.0;
+                            if (as.srgb) {
+                                if (r > 0.04045) {
+                                    lin = std::pow((r + 0.055) / 1.055, 2.4);
+                                    dlin = 2.4 / 1.055 * std::pow((r + 0.055) / 1.055, 1.4);
+                                } else {
+                                    lin = r / 12.92;
+                                    dlin = 1.0 / 12.92;
+                                }
+                            }
+                            if (lin == 0.0) continue;
+                            const double g = scale * (double)adj_rgba[4 * p + ch] * S[ch] / lin * dlin;
+                            for (int q = 0; q < 4; q++) texel_grad[4 * (size_t)as.foot.idx[q] + ch] += g * wq[q];
+                        }
                     }
                 }
             }

@@ -47,10 +47,12 @@ int orc_shade(const rh_scene_view *sc, const rh_pass_info *pi, const rh_ray_pack
  * final_rgba.  ray_counts: primary, secondary, shadow. */
 /* FP64 gradient of sum_pixels adj . (mean image of passes 1..passes) with
  * respect to every material's main_color (3 doubles per material, added to
- * grad), by path replay of the same passes; scale multiplies each pass's term
- * (1/passes for the mean). */
+ * grad) and, when texel_grad is not null, every stored texel's value in
+ * [0, 1] (4 doubles per texel of the scene's texel array, alpha unused), by
+ * path replay of the same passes; scale multiplies each pass's term (1/passes
+ * for the mean). */
 int orc_adjoint(const rh_scene_view *sc, const rh_camera *cam, int w, int h, const int32_t rect[4], int passes,
-                const float *adj_rgba, double scale, double *grad);
+                const float *adj_rgba, double scale, double *grad, double *texel_grad);
 int orc_render(const rh_scene_view *sc, const rh_camera *cam, int w, int h, const int32_t rect[4], int passes,
                int *iteration, float *clean, float *final_rgba, unsigned long long ray_counts[3]);
 

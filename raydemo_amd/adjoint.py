@@ -56,19 +56,61 @@ class DiffRenderer:
         self.r.read_accum(img.data_ptr())
         return img
 
-    def adjoint(self, grad_img: torch.Tensor) -> torch.Tensor:
+    def adjoint(self, grad_img: torch.Tensor, texels: bool = False):
         """d loss / d main_color (n_materials, 3) for d loss / d image grad_img
-        ((h, w, 3) or (h, w, 4)), for the colours of the last render."""
+        ((h, w, 3) or (h, w, 4)), for the colours of the last render; with
+        texels, also d loss / d stored texel (n_texels, 4) on the device."""
         adj = torch.zeros((self.h, self.w, 4), dtype=torch.float32, device=self.device)
         adj[..., :grad_img.shape[-1]] = grad_img.to(torch.float32)
         adj[..., 3] = 0.0
         g = torch.zeros((self.n_materials, 4), dtype=torch.float32, device=self.device)
+        tg = None
+        if texels:
+            tg = torch.zeros((self.s.view()["texels_count"], 4), dtype=torch.float32, device=self.device)
         torch.cuda.synchronize()
         reg = A.Region(self.lib, 0, 0, self.w, self.h)
         for _ in range(self.spp):
-            self.r.adjoint(self.s, reg, adj.data_ptr(), g.data_ptr(), 1.0 / self.spp)
+            self.r.adjoint(self.s, reg, adj.data_ptr(), g.data_ptr(), 1.0 / self.spp,
+                           tg.data_ptr() if tg is not None else None)
         self.r.synchronize()
+        if texels:
+            return g[:, :3].contiguous(), tg
         return g[:, :3].contiguous()
+
+    def set_texture(self, tex: int, rgb: torch.Tensor):
+        """Level-0 texels of texture tex from values in [0, 1] ((h, w, 3),
+        alpha 255), quantised to 8 bits as the texture stores them."""
+        q = torch.clamp(torch.round(rgb.detach().to(torch.float32) * 255.0), 0, 255).to(torch.uint8).cpu().numpy()
+        rgba = np.concatenate([q, np.full(q.shape[:2] + (1,), 255, np.uint8)], axis=2)
+        self.r.set_texture_data(self.s, tex, rgba)
+
+
+def fold_texel_grad(view: dict, tex: int, tg: np.ndarray) -> np.ndarray:
+    """Gradient of the stored texels of texture tex (its bordered blocks, one
+    per mip level) onto its level-0 texels, (h, w, 4): a block texel (x, y)
+    is level texel ((x-1) mod w, (y-1) mod h), and a level-k texel is the box
+    average of 4 level-(k-1) texels, so its gradient spreads as 1/4 to each
+    (the 8-bit rounding of the mips is taken as the identity)."""
+    t = view["textures"][tex]
+    w0, h0 = int(t["width"]) & 0x7FFF, int(t["height"])
+    mips = view["tex_mips"][tex]
+    levels = []
+    for lev in range(len(mips)):
+        if lev > 0 and mips[lev]["offset"] == mips[lev - 1]["offset"]:
+            break
+        bw, bh, off = int(mips[lev]["bw"]), int(mips[lev]["bh"]), int(mips[lev]["offset"])
+        lw, lh = bw - 2, bh - 2
+        block = tg[off:off + bw * bh].reshape(bh, bw, 4)
+        g = np.zeros((lh, lw, 4), np.float64)
+        ys = (np.arange(bh) - 1) % lh
+        xs = (np.arange(bw) - 1) % lw
+        np.add.at(g, (ys[:, None], xs[None, :]), block)
+        levels.append(g)
+    for lev in range(len(levels) - 1, 0, -1):
+        up = np.repeat(np.repeat(levels[lev], 2, axis=0), 2, axis=1) / 4.0
+        levels[lev - 1] += up
+    assert levels[0].shape[:2] == (h0, w0)
+    return levels[0]
 
 
 class RenderFunction(torch.autograd.Function):
@@ -88,6 +130,32 @@ def render(colors: torch.Tensor, dr: DiffRenderer) -> torch.Tensor:
     return RenderFunction.apply(colors, dr)
 
 
+class TextureRenderFunction(torch.autograd.Function):
+    """image = render(level-0 texels of the given textures); backward by path
+    replay, texel gradients folded onto level 0 (straight through the 8-bit
+    quantisation of the texels)."""
+
+    @staticmethod
+    def forward(ctx, texels: torch.Tensor, dr: DiffRenderer, tex_ids: tuple):
+        ctx.dr, ctx.tex_ids, ctx.param_device = dr, tex_ids, texels.device
+        for k, t in enumerate(tex_ids):
+            dr.set_texture(t, texels[k])
+        return dr.render()[..., :3].contiguous()
+
+    @staticmethod
+    def backward(ctx, grad_out: torch.Tensor):
+        dr = ctx.dr
+        _, tg = dr.adjoint(grad_out.contiguous(), texels=True)
+        tg = tg.cpu().numpy().astype(np.float64)
+        view = dr.s.view()
+        g = np.stack([fold_texel_grad(view, t, tg)[..., :3] for t in ctx.tex_ids])
+        return torch.tensor(g, dtype=torch.float32, device=ctx.param_device), None, None
+
+
+def render_textures(texels: torch.Tensor, dr: DiffRenderer, tex_ids: tuple) -> torch.Tensor:
+    return TextureRenderFunction.apply(texels, dr, tex_ids)
+
+
 def refined_cornell(lib: A.RayLib, k=17, width=512, height=512, spp=32, depth=6, albedo_seed=None, renderer=None):
     """DiffRenderer over the refined Cornell box: (dr, initial colours (n, 3) numpy)."""
     from . import scenes
@@ -99,6 +167,18 @@ def refined_cornell(lib: A.RayLib, k=17, width=512, height=512, spp=32, depth=6,
     view = s.view()
     colors = np.array(view["materials"]["main_color"], np.float32)
     return DiffRenderer(lib, r, s, width, height, spp, n), colors, spec
+
+
+def textured_cornell(lib: A.RayLib, width=128, height=128, spp=8, depth=4, tex_size=64, mipmaps=True, textures=None,
+                     renderer=None):
+    """DiffRenderer over the textured Cornell box: (dr, spec); spec.info["textures"]
+    are the four differentiable textures."""
+    from . import scenes
+    r = renderer or A.Renderer(lib, width, height, A.RENDERER_HIP)
+    s = r.create_scene()
+    spec = scenes.textured_cornell(s, tex_size=tex_size, width=width, height=height, spp=spp, depth=depth,
+                                   mipmaps=mipmaps, textures=textures)
+    return DiffRenderer(lib, r, s, width, height, spp, len(s.view()["materials"])), spec
 
 
 @dataclass

@@ -135,6 +135,7 @@ MI_DTYPE = np.dtype([("bbox_min", "<f4", 3), ("tr_index", "<u4"), ("bbox_max", "
 MATERIAL_DTYPE = np.dtype([("textures", "<u4", 4), ("main_color", "<f4", 3), ("pad", "<f4"), ("type", "<u4"),
                            ("roughness", "<f4"), ("int_ior", "<f4"), ("ext_ior", "<f4")])
 TEXTURE_DTYPE = np.dtype([("width", "<u2"), ("height", "<u2"), ("page", "u1", 14), ("pos", "<u2", (14, 2))])
+TEX_MIP_DTYPE = np.dtype([("offset", "<u4"), ("bw", "<u2"), ("bh", "<u2")])
 LIGHT_DTYPE = np.dtype([("pos", "<f4", 3), ("radius", "<f4"), ("col", "<f4", 3), ("brightness", "<f4"),
                         ("dir", "<f4", 3), ("spot", "<f4")])
 RAY_PACKET_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("c", "<f4", 3), ("ior", "<f4"), ("do_dx", "<f4", 3),
@@ -216,7 +217,8 @@ class RayLib:
             f.argtypes = args
         # differentiation entry points (HIP backend only)
         optional = {
-            "rhx_adjoint_scene": (c_int, [P, P, P, P, P, c_float]),
+            "rhx_adjoint_scene": (c_int, [P, P, P, P, P, P, c_float]),
+            "rhx_set_texture_data": (c_int, [P, P, c_uint32, P]),
             "rhx_set_material_colors": (c_int, [P, P, c_uint32, c_uint32, P]),
             "rhx_read_accum": (c_int, [P, P, c_size_t]),
         }
@@ -233,7 +235,8 @@ class RayLib:
             "rhx_batch_synchronize": (c_int, [P]),
             "rhx_batch_restart": (c_int, [P]),
         })
-        self.has_adjoint = all(hasattr(L, n) for n in ("rhx_adjoint_scene", "rhx_set_material_colors", "rhx_read_accum"))
+        self.has_adjoint = all(hasattr(L, n) for n in ("rhx_adjoint_scene", "rhx_set_material_colors", "rhx_read_accum",
+                                                       "rhx_set_texture_data"))
         self.has_batch = hasattr(L, "rhx_batch_create")
         for name, (res, args) in optional.items():
             if hasattr(L, name):
@@ -328,12 +331,19 @@ class Renderer:
         self.lib.check(self.lib.lib.rhx_render_scene(self.h, scene.h, region.h), "rhx_render_scene")
 
     # ---- differentiation (HIP backend)
-    def adjoint(self, scene: "Scene", region: "Region", d_adj_rgba: int, d_grad: int, scale: float):
+    def adjoint(self, scene: "Scene", region: "Region", d_adj_rgba: int, d_grad: int, scale: float,
+                d_texel_grad: int | None = None):
         """Replays the pass rhx_render_scene would render next for region and adds
         scale * d loss / d main_color to the device array d_grad (4 floats per
-        material) for the device adjoint image d_adj_rgba (w*h RGBA floats)."""
+        material), and d loss / d stored texel to d_texel_grad when given (4
+        floats per texel), for the device adjoint image d_adj_rgba (w*h RGBA)."""
         self.lib.check(self.lib.lib.rhx_adjoint_scene(self.h, scene.h, region.h, c_void_p(d_adj_rgba),
-                                                      c_void_p(d_grad), float(scale)), "rhx_adjoint_scene")
+                                                      c_void_p(d_grad), c_void_p(d_texel_grad or 0), float(scale)),
+                       "rhx_adjoint_scene")
+
+    def set_texture_data(self, scene: "Scene", tex: int, rgba: np.ndarray):
+        rgba = np.ascontiguousarray(rgba, np.uint8)
+        self.lib.check(self.lib.lib.rhx_set_texture_data(self.h, scene.h, tex, _ptr(rgba)), "rhx_set_texture_data")
 
     def set_material_colors(self, scene: "Scene", first: int, rgb: np.ndarray):
         rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
@@ -584,6 +594,8 @@ class Scene:
             "tri_indices": _arr(v.tri_indices, v.tri_indices_count, np.uint32),
             "materials": _arr(v.materials, v.materials_count, MATERIAL_DTYPE),
             "textures": _arr(v.textures, v.textures_count, TEXTURE_DTYPE),
+            "tex_mips": _arr(v.tex_mips, v.textures_count * 14, TEX_MIP_DTYPE).reshape(-1, 14),
+            "texels_count": v.texels_count,
             "lights": _arr(v.lights, v.lights_count, LIGHT_DTYPE),
             "li_indices": _arr(v.li_indices, v.li_indices_count, np.uint32),
             "macro_nodes_root": v.macro_nodes_root,

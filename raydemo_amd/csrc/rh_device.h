@@ -442,18 +442,29 @@ __device__ __forceinline__ void traverse_fast(const Scene &sc, f3 o, f3 d, Hit &
 // Texel fetch at reference atlas coordinates (x, y) on the atlas page where the
 // texture's mip block sits: the block's own compact storage holds it.
 // (px, py) is the block's atlas position, the texture's pos[lod].
-__device__ __forceinline__ uint32_t atlas_get(const Scene &sc, uint32_t tex, int lod, int px, int py, int x, int y) {
+__device__ __forceinline__ uint32_t atlas_index(const Scene &sc, uint32_t tex, int lod, int px, int py, int x, int y) {
     const rh_tex_mip m = sc.tex_mips[tex * RH_NUM_MIP_LEVELS + lod];
     int lx = x - px, ly = y - py;
     lx = clampi(lx, 0, (int)m.bw - 1);
     ly = clampi(ly, 0, (int)m.bh - 1);
-    return sc.texels[m.offset + (uint32_t)ly * m.bw + (uint32_t)lx];
+    return m.offset + (uint32_t)ly * m.bw + (uint32_t)lx;
 }
+__device__ __forceinline__ uint32_t atlas_get(const Scene &sc, uint32_t tex, int lod, int px, int py, int x, int y) {
+    return sc.texels[atlas_index(sc, tex, lod, px, py, x, y)];
+}
+
+// The texels a bilinear sample read (indices into the texel array: 00, 01,
+// 10, 11) and its weights: value = sum of texel * weight, w00 = (1-kx)(1-ky),
+// w01 = kx(1-ky), w10 = (1-kx)ky, w11 = kx ky.
+struct TexFoot {
+    uint32_t idx[4];
+    float kx, ky;
+};
 
 __device__ __forceinline__ float texel_ch(uint32_t p, int c) { return (float)((p >> (8 * c)) & 0xffu); }
 
 // SampleBilinear (CoreRef.cpp:1652-1678) with TransformUV (CoreRef.cpp:1625-1632)
-__device__ __forceinline__ f4 sample_bilinear(const Scene &sc, uint32_t tex, f2 uvs, int lod) {
+__device__ __forceinline__ f4 sample_bilinear(const Scene &sc, uint32_t tex, f2 uvs, int lod, TexFoot *foot = nullptr) {
     // fields read in place: copying the record and indexing its arrays by lod
     // would put it in scratch memory
     const rh_texture *t = sc.textures + tex;
@@ -469,11 +480,13 @@ __device__ __forceinline__ f4 sample_bilinear(const Scene &sc, uint32_t tex, f2 
     rx = rx * atlas - 0.5f;
     ry = ry * atlas - 0.5f;
     const int ix = (int)rx, iy = (int)ry;
-    const uint32_t p00 = atlas_get(sc, tex, lod, ipx, ipy, ix + 0, iy + 0);
-    const uint32_t p01 = atlas_get(sc, tex, lod, ipx, ipy, ix + 1, iy + 0);
-    const uint32_t p10 = atlas_get(sc, tex, lod, ipx, ipy, ix + 0, iy + 1);
-    const uint32_t p11 = atlas_get(sc, tex, lod, ipx, ipy, ix + 1, iy + 1);
+    const uint32_t i00 = atlas_index(sc, tex, lod, ipx, ipy, ix + 0, iy + 0);
+    const uint32_t i01 = atlas_index(sc, tex, lod, ipx, ipy, ix + 1, iy + 0);
+    const uint32_t i10 = atlas_index(sc, tex, lod, ipx, ipy, ix + 0, iy + 1);
+    const uint32_t i11 = atlas_index(sc, tex, lod, ipx, ipy, ix + 1, iy + 1);
+    const uint32_t p00 = sc.texels[i00], p01 = sc.texels[i01], p10 = sc.texels[i10], p11 = sc.texels[i11];
     const float kx = rx - __builtin_floorf(rx), ky = ry - __builtin_floorf(ry);
+    if (foot) *foot = TexFoot{{i00, i01, i10, i11}, kx, ky};
     float r[4];
     const float k = 1.0f / 255.0f;
 #pragma unroll

@@ -74,7 +74,9 @@ struct AdjointArgs {
     const float4 *adj;  // dLoss/d(pass image) per pixel of the frame (rgb, w unused)
     float scale;        // weight of this pass in the image (1 / passes for the mean)
     float *grad;        // 4 floats per material: d loss / d main_color (rgb, pad)
+    float *tex_grad;    // optional: 4 floats per stored texel, d loss / d texel value (rgb, a unused)
     float4 *rec;        // per-lane vertex records, max_vertices x lanes, lane-minor
+    float4 *rec_tex;    // with tex_grad: 3 records per vertex (texels, weights + raw value, raw + sRGB flag)
     uint32_t lanes;
     uint32_t *spill;
     uint32_t spill_stride;
@@ -151,6 +153,13 @@ template <bool Defer, int MinBlocks> __global__ void __launch_bounds__(kBlock, M
                 if (Defer && so.nee_pending && sink.count > 0)
                     cj = cj + resolve_light_samples(a.sc, a.halton, sink, so, ray.c, st, cnt);
                 a.rec[(size_t)nv * a.lanes + i] = make_float4(cj.x, cj.y, cj.z, __int_as_float(so.alb_mat));
+                if (a.tex_grad && so.alb_mat >= 0) {
+                    float4 *rt = a.rec_tex + (size_t)nv * 3 * a.lanes + i;
+                    rt[0] = make_float4(__uint_as_float(so.alb_foot.idx[0]), __uint_as_float(so.alb_foot.idx[1]),
+                                        __uint_as_float(so.alb_foot.idx[2]), __uint_as_float(so.alb_foot.idx[3]));
+                    rt[a.lanes] = make_float4(so.alb_foot.kx, so.alb_foot.ky, so.alb_raw.x, so.alb_raw.y);
+                    rt[2 * (size_t)a.lanes] = make_float4(so.alb_raw.z, so.alb_srgb ? 1.0f : 0.0f, 0.0f, 0.0f);
+                }
                 nv++;
                 b++;
                 if (!so.emit || skip_indirect || b >= last) break;
@@ -167,6 +176,38 @@ template <bool Defer, int MinBlocks> __global__ void __launch_bounds__(kBlock, M
 #pragma unroll
                 for (int ch = 0; ch < 3; ch++) {
                     if (mc[ch] != 0.0f && g[ch] != 0.0f) atomicAdd(&a.grad[4 * m + ch], a.scale * (g[ch] / mc[ch]));
+                }
+                if (a.tex_grad) {
+                    // albedo = lin(raw) * main_color with raw the bilinear sum; S is
+                    // linear in the albedo, so d pixel / d raw = S / lin(raw) * lin'(raw)
+                    const float4 *rt = a.rec_tex + (size_t)k * 3 * a.lanes + i;
+                    const float4 t0 = rt[0], t1 = rt[a.lanes], t2 = rt[2 * (size_t)a.lanes];
+                    const float raw[3] = {t1.z, t1.w, t2.x};
+                    const bool srgb = t2.y != 0.0f;
+                    const float kx = t1.x, ky = t1.y;
+                    const float w[4] = {(1.0f - kx) * (1.0f - ky), kx * (1.0f - ky), (1.0f - kx) * ky, kx * ky};
+                    const uint32_t idx[4] = {__float_as_uint(t0.x), __float_as_uint(t0.y), __float_as_uint(t0.z),
+                                             __float_as_uint(t0.w)};
+#pragma unroll
+                    for (int ch = 0; ch < 3; ch++) {
+                        float lin, dlin;
+                        if (!srgb) {
+                            lin = raw[ch];
+                            dlin = 1.0f;
+                        } else if (raw[ch] > 0.04045f) {
+                            const float b = (raw[ch] + 0.055f) / 1.055f;
+                            lin = rpow(b, 2.4f);
+                            dlin = 2.4f / 1.055f * rpow(b, 1.4f);
+                        } else {
+                            lin = raw[ch] / 12.92f;
+                            dlin = 1.0f / 12.92f;
+                        }
+                        if (lin == 0.0f || g[ch] == 0.0f) continue;
+                        const float gr = a.scale * (g[ch] / lin) * dlin;
+#pragma unroll
+                        for (int q = 0; q < 4; q++)
+                            if (w[q] != 0.0f) atomicAdd(&a.tex_grad[4 * (size_t)idx[q] + ch], gr * w[q]);
+                    }
                 }
             }
         }
@@ -407,8 +448,11 @@ struct rh_context {
     // adjoint replay: per-lane vertex records (max vertices x lanes)
     float4 *adj_rec = nullptr;
     size_t adj_rec_cap = 0; // lanes
+    float4 *adj_rec_tex = nullptr; // texture footprints of the vertices (texel gradients)
+    size_t adj_rec_tex_cap = 0;
     float *color_stage = nullptr;
     size_t color_stage_cap = 0; // floats
+    uint32_t n_texels = 0;
     uint32_t n_materials = 0;
     int *counters = nullptr; // [0], [1]: ray queue counts, [2]: shadow queue count, [3]: refill fetch
     unsigned long long *stats_dev = nullptr; // [0] primary [1] secondary [2] shadow [3] sorted [4] overflow
@@ -811,6 +855,9 @@ void rh_destroy(rh_context *c) {
     free_ptr(c->stats_dev);
     free_ptr(c->halton);
     free_ptr(c->visits);
+    free_ptr(c->adj_rec);
+    free_ptr(c->adj_rec_tex);
+    free_ptr(c->color_stage);
     for (auto &e : c->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -846,6 +893,7 @@ int rh_upload_scene(rh_context *c, const rh_scene_view *v) {
     c->nee_slots = (v->lights_count >= 1 && v->lights_count <= kMaxDeferredLights) ? v->lights_count : 0;
     c->sc = s;
     c->n_materials = (uint32_t)v->materials_count;
+    c->n_texels = (uint32_t)v->texels_count;
     c->have_scene = true;
     RH_CHECK(hipStreamSynchronize(c->stream));
     return 0;
@@ -1319,7 +1367,8 @@ int rh_shade_rays(rh_context *c, const rh_pass_info *pi, const rh_ray_packet *ra
 
 // ---- differentiation: path-replay adjoint of one pass, material colours, accumulated image
 
-int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgba, float *d_grad, float scale) {
+int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgba, float *d_grad, float *d_texel_grad,
+                    float scale) {
     if (!c->have_scene) return arg_error("no scene uploaded");
     const int rx = p->rect[0], ry = p->rect[1], rw = p->rect[2], rhh = p->rect[3];
     if (rw <= 0 || rhh <= 0 || rx < 0 || ry < 0 || rx + rw > c->w || ry + rhh > c->h)
@@ -1334,6 +1383,12 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
         c->adj_rec = nullptr;
         c->adj_rec_cap = lanes;
         RH_CHECK(hipMalloc(&c->adj_rec, (RH_MAX_BOUNCES + 1) * lanes * sizeof(float4)));
+    }
+    if (d_texel_grad && (lanes > c->adj_rec_tex_cap || !c->adj_rec_tex)) {
+        free_ptr(c->adj_rec_tex);
+        c->adj_rec_tex = nullptr;
+        c->adj_rec_tex_cap = lanes;
+        RH_CHECK(hipMalloc(&c->adj_rec_tex, 3 * (RH_MAX_BOUNCES + 1) * lanes * sizeof(float4)));
     }
     const bool defer = c->defer_nee && c->nee_slots;
     // NEE slots for every lane when deferred (this may grow the spill array,
@@ -1355,7 +1410,9 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
     a.scale = scale;
     a.grad = d_grad;
     a.rec = c->adj_rec;
-    a.lanes = (uint32_t)c->adj_rec_cap;
+    a.lanes = (uint32_t)lanes;
+    a.tex_grad = d_texel_grad;
+    a.rec_tex = d_texel_grad ? c->adj_rec_tex : nullptr;
     a.spill = c->spill;
     a.spill_stride = (uint32_t)c->spill_cap;
     a.lds_depth = (uint32_t)c->lds_depth;
@@ -1397,6 +1454,16 @@ int rh_set_material_colors(rh_context *c, uint32_t first, uint32_t n, const floa
                        const_cast<rh_material *>(c->sc.materials), c->color_stage, first, n);
     RH_CHECK(hipGetLastError());
     // the caller's array may change right after this call returns
+    RH_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int rh_set_texels(rh_context *c, uint32_t first, uint32_t n, const uint32_t *rgba) {
+    if (!c->have_scene) return arg_error("no scene uploaded");
+    if ((size_t)first + n > c->n_texels) return arg_error("texel range outside the scene");
+    if (!n) return 0;
+    RH_CHECK(hipMemcpyAsync(const_cast<uint32_t *>(c->sc.texels) + first, rgba, (size_t)n * sizeof(uint32_t),
+                            hipMemcpyDefault, c->stream));
     RH_CHECK(hipStreamSynchronize(c->stream));
     return 0;
 }

@@ -236,6 +236,11 @@ struct ShadeOut {
     // contribution of the path exactly once (emissive; diffuse and glossy
     // when the albedo is considered), or -1
     int alb_mat;
+    // the albedo texture sample of this vertex: texels and weights, the
+    // sampled value before the sRGB conversion, and whether it was applied
+    TexFoot alb_foot;
+    f3 alb_raw;
+    bool alb_srgb;
 };
 
 // ShadeSurface (CoreRef.cpp:2055-2391) for one (ray, hit) pair.
@@ -353,7 +358,9 @@ __device__ __forceinline__ void shade_surface(const Scene &sc, const rh_pass_inf
     const uint32_t main_tex = mat->textures[RH_MAIN_TEXTURE];
     const rh_texture &mt = sc.textures[main_tex];
     const float albedo_lod = get_texture_lod(mt, der.duv_dx, der.duv_dy);
-    f4 albedo = sample_bilinear(sc, main_tex, uvs, (int)albedo_lod);
+    f4 albedo = sample_bilinear(sc, main_tex, uvs, (int)albedo_lod, &out.alb_foot);
+    out.alb_raw = f3{albedo.x, albedo.y, albedo.z};
+    out.alb_srgb = (mt.width & RH_TEXTURE_SRGB_BIT) != 0;
     if (mt.width & RH_TEXTURE_SRGB_BIT) albedo = srgb_to_rgb(albedo);
     albedo.x *= mat->main_color[0];
     albedo.y *= mat->main_color[1];

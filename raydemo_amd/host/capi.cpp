@@ -172,10 +172,10 @@ int rhx_render_scene(rhx_renderer *r, rhx_scene *s, rhx_region *reg) {
 }
 
 int rhx_adjoint_scene(rhx_renderer *r, rhx_scene *s, rhx_region *reg, const float *d_adj_rgba, float *d_grad,
-                      float scale) {
+                      float *d_texel_grad, float scale) {
     return guarded([&] {
         r->r->last_error.clear();
-        if (!r->r->AdjointScene(s->s, reg->ctx, d_adj_rgba, d_grad, scale)) {
+        if (!r->r->AdjointScene(s->s, reg->ctx, d_adj_rgba, d_grad, d_texel_grad, scale)) {
             g_err = r->r->last_error.empty() ? "adjoint pass not run" : r->r->last_error;
             return -1;
         }
@@ -186,6 +186,16 @@ int rhx_adjoint_scene(rhx_renderer *r, rhx_scene *s, rhx_region *reg, const floa
 int rhx_set_material_colors(rhx_renderer *r, rhx_scene *s, uint32_t first, uint32_t n, const float *rgb) {
     return guarded([&] {
         if (!r->r->SetMaterialColors(s->s, first, n, rgb)) {
+            g_err = r->r->last_error;
+            return -1;
+        }
+        return 0;
+    });
+}
+
+int rhx_set_texture_data(rhx_renderer *r, rhx_scene *s, uint32_t tex, const uint8_t *rgba) {
+    return guarded([&] {
+        if (!r->r->SetTextureData(s->s, tex, rgba)) {
             g_err = r->r->last_error;
             return -1;
         }

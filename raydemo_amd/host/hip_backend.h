@@ -54,6 +54,7 @@ class Scene : public SceneBase {
     uint64_t version_ = 1;
 
     void RemoveNodes(uint32_t node_index, uint32_t node_count);
+    void WriteBlock(const rh_tex_mip &m, const std::vector<uint32_t> &data, const int res[2]);
     void RebuildMacroBVH();
     void RebuildLightBVH();
     int AllocateInAtlas(const int res[2], int pos[2]);
@@ -88,6 +89,10 @@ public:
     rh_scene_view View();
     // The current camera and the sort hash grid (macro-tree root box) of a pass.
     void PassCamera(rh_pass_desc &pass);
+    // New level-0 texels (RGBA8, the texture's size) of texture tex: its
+    // stored blocks (borders, and mips when it has them) are rebuilt in place;
+    // [first, first + count) is the range of the texel array they occupy.
+    bool SetTextureData(uint32_t tex, const uint32_t *rgba, uint32_t &first, uint32_t &count);
     uint64_t version() const { return version_; }
     const camera_t &camera(uint32_t i) const { return cams_[i].cam; }
 };
@@ -131,8 +136,9 @@ public:
     // (host rgb triples) in the scene and in its device copy;
     // ReadAccumulated copies the linear accumulated image.
     bool AdjointScene(const std::shared_ptr<SceneBase> &s, RegionContext &region, const float *d_adj_rgba,
-                      float *d_grad, float scale);
+                      float *d_grad, float *d_texel_grad, float scale);
     bool SetMaterialColors(const std::shared_ptr<SceneBase> &s, uint32_t first, uint32_t n, const float *rgb);
+    bool SetTextureData(const std::shared_ptr<SceneBase> &s, uint32_t tex, const uint8_t *rgba);
     bool ReadAccumulated(float *dst, size_t count);
     void set_sort_rays(bool on) { sort_rays_ = on; }
     std::string last_error;

@@ -119,12 +119,12 @@ void Renderer::RenderScene(const std::shared_ptr<SceneBase> &_s, RegionContext &
 }
 
 bool Renderer::AdjointScene(const std::shared_ptr<SceneBase> &_s, RegionContext &region, const float *d_adj_rgba,
-                            float *d_grad, float scale) {
+                            float *d_grad, float *d_texel_grad, float scale) {
     const auto s = std::dynamic_pointer_cast<Scene>(_s);
     if (!s) return false;
     rh_pass_desc pass;
     if (!PreparePass(*s, region, pass)) return false;
-    if (rh_adjoint_pass(ctx_, &pass, d_adj_rgba, d_grad, scale) != 0) {
+    if (rh_adjoint_pass(ctx_, &pass, d_adj_rgba, d_grad, d_texel_grad, scale) != 0) {
         last_error = rh_last_error();
         return false;
     }
@@ -144,6 +144,23 @@ bool Renderer::SetMaterialColors(const std::shared_ptr<SceneBase> &_s, uint32_t 
     // otherwise the next pass uploads the updated host arrays
     if (uploaded_scene_ == s.get() && uploaded_version_ == s->version()) {
         if (rh_set_material_colors(ctx_, first, n, rgb) != 0) {
+            last_error = rh_last_error();
+            return false;
+        }
+    }
+    return true;
+}
+
+bool Renderer::SetTextureData(const std::shared_ptr<SceneBase> &_s, uint32_t tex, const uint8_t *rgba) {
+    const auto s = std::dynamic_pointer_cast<Scene>(_s);
+    if (!s) return false;
+    uint32_t first = 0, count = 0;
+    if (!s->SetTextureData(tex, reinterpret_cast<const uint32_t *>(rgba), first, count)) {
+        last_error = "texture index out of range";
+        return false;
+    }
+    if (uploaded_scene_ == s.get() && uploaded_version_ == s->version()) {
+        if (rh_set_texels(ctx_, first, count, &s->texels_[first]) != 0) {
             last_error = rh_last_error();
             return false;
         }

@@ -149,6 +149,41 @@ int Scene::AllocateInAtlas(const int res[2], int pos[2]) {
     }
 }
 
+// bordered block: texel (x, y) of the block is data[(y-1) mod h][(x-1) mod w],
+// the one-texel wrap border the reference writes around every block
+void Scene::WriteBlock(const rh_tex_mip &m, const std::vector<uint32_t> &data, const int res[2]) {
+    uint32_t *dst = &texels_[m.offset];
+    for (int y = 0; y < m.bh; y++) {
+        const int sy = (y - 1 + res[1]) % res[1];
+        for (int x = 0; x < m.bw; x++) {
+            const int sx = (x - 1 + res[0]) % res[0];
+            dst[(size_t)y * m.bw + x] = data[(size_t)sy * res[0] + sx];
+        }
+    }
+}
+
+bool Scene::SetTextureData(uint32_t tex, const uint32_t *rgba, uint32_t &first, uint32_t &count) {
+    if (tex >= textures_.size()) return false;
+    const rh_texture &t = textures_[tex];
+    int res[2] = {t.width & RH_TEXTURE_WIDTH_BITS, t.height};
+    std::vector<uint32_t> data(rgba, rgba + (size_t)res[0] * res[1]);
+    const rh_tex_mip *mips = &tex_mips_[(size_t)tex * RH_NUM_MIP_LEVELS];
+    first = mips[0].offset;
+    uint32_t end = first;
+    for (int lev = 0; lev < RH_NUM_MIP_LEVELS; lev++) {
+        if (lev > 0 && mips[lev].offset == mips[lev - 1].offset) break; // levels past the last repeat it
+        if (lev > 0) {
+            data = RayHip::DownsampleTexture(data, res);
+            res[0] /= 2;
+            res[1] /= 2;
+        }
+        WriteBlock(mips[lev], data, res);
+        end = mips[lev].offset + (uint32_t)mips[lev].bw * mips[lev].bh;
+    }
+    count = end - first;
+    return true;
+}
+
 uint32_t Scene::AddTexture(const tex_desc_t &_t) {
     const uint32_t tex_index = (uint32_t)textures_.size();
     rh_texture t;
@@ -171,21 +206,12 @@ uint32_t Scene::AddTexture(const tex_desc_t &_t) {
         t.pos[mip][0] = (uint16_t)pos[0];
         t.pos[mip][1] = (uint16_t)pos[1];
 
-        // bordered block: texel (x, y) of the block is data[(y-1) mod h][(x-1) mod w],
-        // the one-texel wrap border the reference writes around every block
         const int bw = res[0] + 2, bh = res[1] + 2;
         mips[mip].offset = (uint32_t)texels_.size();
         mips[mip].bw = (uint16_t)bw;
         mips[mip].bh = (uint16_t)bh;
         texels_.resize(texels_.size() + (size_t)bw * bh);
-        uint32_t *dst = &texels_[mips[mip].offset];
-        for (int y = 0; y < bh; y++) {
-            const int sy = (y - 1 + res[1]) % res[1];
-            for (int x = 0; x < bw; x++) {
-                const int sx = (x - 1 + res[0]) % res[0];
-                dst[(size_t)y * bw + x] = data[(size_t)sy * res[0] + sx];
-            }
-        }
+        WriteBlock(mips[mip], data, res);
 
         mip++;
         if (_t.generate_mipmaps) {

@@ -355,3 +355,45 @@ def batch_workload(n: int = 64, seed: int = SEED_BASE + 3) -> list:
 def batch_scene(scene: "A.Scene", item: BatchItem) -> SceneSpec:
     return triangle_soup(scene, seed=item.seed, width=item.width, height=item.height, spp=item.spp, depth=item.depth,
                          n_spheres=item.n_spheres, n_fine=item.n_fine)
+
+
+def textured_cornell(scene: "A.Scene", k: int = 8, tex_size: int = 64, seed: int = SEED_BASE + 6, width=128,
+                     height=128, spp=8, depth=4, mipmaps: bool = True, textures=None) -> SceneSpec:
+    """Refined Cornell box whose floor, back, left and right walls carry their
+    own sRGB texture (tex_size^2, seeded noise around a base colour, or the
+    given (4, n, n, 3) uint8 textures); ceiling and boxes diffuse, one
+    emissive quad, Russian roulette off.  info["textures"] are the texture ids."""
+    attrs, idx, n_light = cornell_refined_geometry(k)
+    rng = np.random.default_rng(seed)
+    per_quad = 2 * k * k
+    white = _white(scene)
+    tex_ids, mats = [], []
+    for q in range(4):
+        if textures is None:
+            base = rng.uniform(0.25, 0.8, size=3)
+            img = np.clip(base + 0.15 * rng.standard_normal((tex_size, tex_size, 3)), 0.0, 1.0)
+            rgb = np.round(img * 255).astype(np.uint8)
+        else:
+            rgb = np.asarray(textures[q], np.uint8)
+        rgba = np.concatenate([rgb, np.full(rgb.shape[:2] + (1,), 255, np.uint8)], axis=2)
+        t = scene.add_texture(rgba, is_srgb=True, generate_mipmaps=mipmaps)
+        tex_ids.append(t)
+        mats.append(scene.add_material(A.DiffuseMaterial, t, main_color=(1.0, 1.0, 1.0)))
+    plain = scene.add_material(A.DiffuseMaterial, white, main_color=(0.7, 0.7, 0.7))
+    light = scene.add_material(A.EmissiveMaterial, white, main_color=(1.0, 1.0, 1.0), strength=10.0)
+    # quads: floor, ceiling, back, left, right, then 12 box faces
+    quad_mat = {0: mats[0], 2: mats[1], 3: mats[2], 4: mats[3]}
+    shapes = []
+    n_quads = (len(idx) // 3 - n_light) // per_quad
+    for qi in range(n_quads):
+        shapes.append((quad_mat.get(qi, plain), 3 * qi * per_quad, 3 * per_quad))
+    shapes.append((light, 3 * n_quads * per_quad, 3 * n_light))
+    mesh = scene.add_mesh(A.PxyzNxyzTuv, attrs, idx, shapes)
+    scene.add_mesh_instance(mesh)
+    scene.set_environment((0.0, 0.0, 0.0))
+    cam = scene.add_camera(origin=(0.0, 5.0, -14.0), fwd=(0.0, 0.0, 1.0), fov=45.0, filter=A.Box,
+                           dtype=A.DeviceNone, max_diff_depth=depth, max_glossy_depth=depth, max_refr_depth=depth,
+                           max_transp_depth=depth, max_total_depth=depth, termination_start_depth=16)
+    scene.set_current_cam(cam)
+    return SceneSpec("textured_cornell", width, height, spp, depth, triangles=len(idx) // 3,
+                     info={"textures": tex_ids, "tex_size": tex_size})

@@ -149,7 +149,7 @@ class Oracle:
             ("orc_sort_rays", [P, C.c_size_t, P, P]),
             ("orc_shade", [P, P, P, P, C.c_size_t, P, P, P, P, P]),
             ("orc_render", [P, P, C.c_int, C.c_int, P, C.c_int, P, P, P, P]),
-            ("orc_adjoint", [P, P, C.c_int, C.c_int, P, C.c_int, P, C.c_double, P]),
+            ("orc_adjoint", [P, P, C.c_int, C.c_int, P, C.c_int, P, C.c_double, P, P]),
         ]:
             f = getattr(L, name)
             f.restype = C.c_int
@@ -211,13 +211,16 @@ class Oracle:
         self.lib.orc_render(C.byref(view), C.byref(cam), w, h, _p(rect), passes, _p(it), _p(clean), _p(final), None)
         return clean
 
-    def adjoint(self, view, cam, w, h, passes, adj, rect=None):
-        """FP64 d(sum adj . mean image)/d main_color, (materials, 3)."""
+    def adjoint(self, view, cam, w, h, passes, adj, rect=None, texels=False):
+        """FP64 d(sum adj . mean image)/d main_color, (materials, 3); with
+        texels, also d/d stored texel value, (texels, 4)."""
         rect = np.array(rect or (0, 0, w, h), np.int32)
         adj = np.ascontiguousarray(adj, np.float32).reshape(h, w, 4)
         grad = np.zeros((view.materials_count, 3), np.float64)
-        self.lib.orc_adjoint(C.byref(view), C.byref(cam), w, h, _p(rect), passes, _p(adj), 1.0 / passes, _p(grad))
-        return grad
+        tg = np.zeros((view.texels_count, 4), np.float64) if texels else None
+        self.lib.orc_adjoint(C.byref(view), C.byref(cam), w, h, _p(rect), passes, _p(adj), 1.0 / passes, _p(grad),
+                             _p(tg) if texels else None)
+        return (grad, tg) if texels else grad
 
     def render(self, view, cam, w, h, passes, rect=None):
         rect = np.array(rect or (0, 0, w, h), np.int32)
