@@ -16,7 +16,9 @@ sort modes are timed the same way and reported beside it.
 Also reported on the same JSON line:
   roofline      the trace kernel's achieved algorithmic bandwidth (bytes per
                 traced ray from the node/triangle/instance visit counts of an
-                instrumented replay of the same passes) over its HIP-event time
+                instrumented replay of the same passes, which count the
+                reference traversal's visits: equal to the oracle's counters on
+                the same rays, tests/test_gpu_trace.py) over its HIP-event time
                 inside the timed region, against 8 TB/s; traffic = HBM bytes
                 per launch from the copy-calibrated FETCH_SIZE/WRITE_SIZE
                 counters committed in profiles/trace_traffic.json

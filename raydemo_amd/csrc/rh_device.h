@@ -351,13 +351,14 @@ __device__ __forceinline__ bool enter_children(const Scene &sc, const NodeBox &n
     return false;
 }
 
-// Pops until a node passes its box test against the current t.
+// Pops until a node passes its box test against the current t.  A popped node
+// was counted when its parent's children were tested: the reference visits
+// (box-tests) each child once, at its pop, and so does this count.
 template <bool Finite>
 __device__ __forceinline__ bool pop_entered(const Scene &sc, uint32_t base, f3 o, f3 inv_d, float t, Stack &st,
-                                            NodeBox &next, TraceCounters *cnt) {
+                                            NodeBox &next, TraceCounters *) {
     while (st.sp > base) {
         const NodeBox n = load_node(sc.nodes, st.pop());
-        if (cnt) cnt->nodes++;
         if (box_hit<Finite>(o, inv_d, t, n.bmin, n.bmax)) {
             next = n;
             return true;

@@ -8,7 +8,8 @@ from ctypes import POINTER, Structure, c_float, c_int, c_int32, c_size_t, c_uint
 
 import numpy as np
 
-from .capi import (HALTON_COUNT, HALTON_SEQ_LEN, HIT_DTYPE, PASS_INFO_DTYPE, RAY_PACKET_DTYPE, Camera, SceneView)
+from .capi import (HALTON_COUNT, HALTON_SEQ_LEN, HIT_DTYPE, PASS_INFO_DTYPE, RAY_PACKET_DTYPE, Camera, KernelProfile,
+                   SceneView)
 
 
 class PassDesc(Structure):
@@ -43,6 +44,8 @@ class DeviceLib:
             "rh_ray_hashes": (c_int, [P, P, c_size_t, P, P, P]),
             "rh_sort_rays": (c_int, [P, P, c_size_t, P, P]),
             "rh_shade_rays": (c_int, [P, P, P, P, c_size_t, P, P, P, POINTER(c_int), P]),
+            "rh_get_kernel_profile": (c_int, [P, POINTER(KernelProfile)]),
+            "rh_reset_kernel_profile": (c_int, [P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -89,6 +92,16 @@ class Context:
         v = c_int()
         self.dev.check(self.dev.lib.rh_get_option(self.h, name.encode(), C.byref(v)), f"rh_get_option({name})")
         return v.value
+
+    def visit_counts(self) -> tuple:
+        """(node visits, triangle tests, instance tests, rays) counted by the
+        trace launches since the last reset_visit_counts, with "count_visits" on."""
+        p = KernelProfile()
+        self.dev.check(self.dev.lib.rh_get_kernel_profile(self.h, C.byref(p)), "rh_get_kernel_profile")
+        return int(p.node_visits), int(p.tri_tests), int(p.instance_tests), int(p.counted_rays)
+
+    def reset_visit_counts(self):
+        self.dev.check(self.dev.lib.rh_reset_kernel_profile(self.h), "rh_reset_kernel_profile")
 
     def gen_primary(self, cam: Camera, rect, w, h, iteration, halton) -> np.ndarray:
         n = rect[2] * rect[3]

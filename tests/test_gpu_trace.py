@@ -95,25 +95,28 @@ def test_trace_any_hit_bit_exact(ref_lib, host_lib, dev_lib):
     assert 0 < (hr["mask"] != 0).mean() < 1
 
 
+def _instanced(s):
+    """Many instances of the Cornell geometry with seeded affine transforms."""
+    attrs, idx, shapes, alb = scenes.cornell_geometry()
+    tex = s.add_texture(np.full((1, 1, 4), 255, np.uint8))
+    m = s.add_material(A.DiffuseMaterial, tex)
+    mesh = s.add_mesh(A.PxyzNxyzTuv, attrs, idx, [(m, 0, len(idx))])
+    rng = np.random.default_rng(3)
+    for k in range(40):
+        a = rng.uniform(0, 2 * np.pi)
+        sc = rng.uniform(0.2, 0.6)
+        x = np.eye(4, dtype=np.float32)
+        x[0, 0], x[0, 2], x[2, 0], x[2, 2] = sc * np.cos(a), -sc * np.sin(a), sc * np.sin(a), sc * np.cos(a)
+        x[1, 1] = sc
+        x[3, :3] = rng.uniform(-15, 15, 3)
+        s.add_mesh_instance(mesh, x)
+    c = s.add_camera(origin=(0, 5, -40), fwd=(0, -0.1, 1), fov=60.0, filter=A.Box, dtype=A.DeviceNone)
+    s.set_current_cam(c)
+
+
 def test_trace_instanced_scene(ref_lib, host_lib, dev_lib):
     """Two-level traversal: many instances with seeded affine transforms."""
-    def build(s):
-        attrs, idx, shapes, alb = scenes.cornell_geometry()
-        tex = s.add_texture(np.full((1, 1, 4), 255, np.uint8))
-        m = s.add_material(A.DiffuseMaterial, tex)
-        mesh = s.add_mesh(A.PxyzNxyzTuv, attrs, idx, [(m, 0, len(idx))])
-        rng = np.random.default_rng(3)
-        for k in range(40):
-            a = rng.uniform(0, 2 * np.pi)
-            sc = rng.uniform(0.2, 0.6)
-            x = np.eye(4, dtype=np.float32)
-            x[0, 0], x[0, 2], x[2, 0], x[2, 2] = sc * np.cos(a), -sc * np.sin(a), sc * np.sin(a), sc * np.cos(a)
-            x[1, 1] = sc
-            x[3, :3] = rng.uniform(-15, 15, 3)
-            s.add_mesh_instance(mesh, x)
-        c = s.add_camera(origin=(0, 5, -40), fwd=(0, -0.1, 1), fov=60.0, filter=A.Box, dtype=A.DeviceNone)
-        s.set_current_cam(c)
-    rs, hs, _ = _pair(ref_lib, host_lib, build)
+    rs, hs, _ = _pair(ref_lib, host_lib, _instanced)
     cam = rs.camera_record(0)
     rays = ref_gen_primary(ref_lib, cam, (0, 0, 160, 120), 160, 120, 1, ref_halton(ref_lib, 1))
     ctx = Context(dev_lib, w=160, h=120)
@@ -121,3 +124,42 @@ def test_trace_instanced_scene(ref_lib, host_lib, dev_lib):
     hg, hr = ctx.trace(rays), ref_trace(ref_lib, rs, rays)
     _check_hits(hg, hr, "instanced")
     assert len(np.unique(hr["obj"][hr["mask"] != 0])) > 5
+
+
+@pytest.mark.parametrize("refill", [0, 1])
+@pytest.mark.parametrize("scene_name", ["soup_small", "instanced"])
+def test_visit_counts_match_oracle(ref_lib, host_lib, dev_lib, oracle, scene_name, refill):
+    """The instrumented trace kernel ("count_visits") counts the node visits,
+    leaf triangle tests and instance tests of the reference traversal, which
+    the roofline's algorithmic bytes are priced from: equal to the oracle's
+    instrumented Traverse_MacroTree_WithStack_ClosestHit on the same rays.
+    Any-hit: triangle and instance tests equal; nodes at least the oracle's
+    (far children tested before an early exit are counted, the reference
+    leaves them on its stack)."""
+    build = {"soup_small": lambda s: scenes.triangle_soup(s, n_spheres=48, n_fine=4), "instanced": _instanced}
+    rs, hs, spec = _pair(ref_lib, host_lib, build[scene_name])
+    w, h = 128, 96
+    rays = ref_gen_primary(ref_lib, rs.camera_record(0), (0, 0, w, h), w, h, 1, ref_halton(ref_lib, 1))
+    hr = ref_trace(ref_lib, rs, rays)
+    sec = secondary_rays_from_hits(rays, hr, np.random.default_rng(9), n_dirs=2)
+    view = hs.raw_view()
+    ctx = Context(dev_lib, w=w, h=h)
+    ctx.set_option("refill", refill)
+    ctx.set_option("count_visits", 1)
+    ctx.upload(view)
+    for what, rs_ in (("primary", rays), ("secondary", sec)):
+        ctx.reset_visit_counts()
+        hg = ctx.trace(rs_)
+        nodes, tris, inst, counted = ctx.visit_counts()
+        ho, cnt = oracle.trace(view, rs_, counters=True)
+        _check_hits(hg, ho, what)
+        assert counted == len(rs_)
+        assert (nodes, tris, inst) == tuple(int(c) for c in cnt), f"{what}: {(nodes, tris, inst)} vs {cnt}"
+    tmax = np.full(len(sec), 6.0, np.float32)
+    ctx.reset_visit_counts()
+    hg = ctx.trace(sec, any_hit=True, tmax=tmax)
+    nodes, tris, inst, _ = ctx.visit_counts()
+    ho, cnt = oracle.trace(view, sec, True, tmax, counters=True)
+    _check_hits(hg, ho, "any-hit")
+    assert (tris, inst) == (int(cnt[1]), int(cnt[2]))
+    assert nodes >= int(cnt[0])
