@@ -139,7 +139,9 @@ def run_cpu_baseline(args, A, scenes, spp_sample):
         return None
     lib = A.RayLib(str(ref_path))
     W, H = args.width, args.height
-    r = A.Renderer(lib, W, H, A.RENDERER_REF, use_wide_bvh=False)
+    # the reference's default settings (settings_t::use_wide_bvh = true, RendererBase.h:29):
+    # its 8-wide BVH is the faster of its two trees on the host
+    r = A.Renderer(lib, W, H, A.RENDERER_REF, use_wide_bvh=True)
     s = r.create_scene()
     scenes.triangle_soup(s, width=W, height=H, depth=args.depth)
     try:
@@ -396,7 +398,8 @@ def main():
             cpu = {"value": round(rays / cpu_run["seconds"] / 1e6, 3), "unit": "Mrays/s", "cores": cpu_run["cores"],
                    "kind": cpu_run["kind"],
                    "sample": f"{cpu_run['spp']} full-frame 1920x1080 passes (spp 1..{cpu_run['spp']}) of the same scene, "
-                             f"128x128 buckets on {cpu_run['cores']} threads, {cpu_run['seconds']:.1f}s"}
+                             f"128x128 buckets on {cpu_run['cores']} threads, 8-wide BVH (the reference default), "
+                             f"{cpu_run['seconds']:.1f}s"}
             log(f"cpu baseline: {cpu}")
 
     plain_knobs = {k: v for k, v in tuned.best.items() if k in autotune.KNOBS}
