@@ -129,6 +129,11 @@ int rhx_get_kernel_profile(rhx_renderer *r, rh_kernel_profile *out);
 int rhx_reset_kernel_profile(rhx_renderer *r);
 
 int rhx_create_scene(rhx_renderer *r, rhx_scene **out);
+/* A scene built on the host only (no renderer, no device), with binary or
+ * 8-wide trees (settings use_wide_bvh); any renderer or batch of the library
+ * can render it.  No counterpart in the reference, whose scenes come from
+ * RendererBase::CreateScene. */
+int rhx_create_scene_standalone(int use_wide_bvh, rhx_scene **out);
 void rhx_destroy_scene(rhx_scene *s);
 
 int rhx_region_create(int x, int y, int w, int h, rhx_region **out);

@@ -112,6 +112,17 @@ typedef struct rh_bvh_node {
     uint32_t prim_count; /* == right_child for interior nodes; bits 30-31 hold the split axis */
 } rh_bvh_node;
 
+/* mbvh_node_t (Core.h:87-92): the 8-wide node of the reference's default
+ * tree (settings use_wide_bvh).  Interior: child[i] is a node index, or
+ * RH_MBVH_EMPTY with NaN bounds for an unused slot; leaf: child[0] = prim
+ * index | RH_LEAF_NODE_BIT, child[1] = prim count, bounds in slot 0. */
+#define RH_MBVH_EMPTY 0x7fffffffu
+typedef struct rh_mbvh_node {
+    float bbox_min[3][8];
+    float bbox_max[3][8];
+    uint32_t child[8];
+} rh_mbvh_node;
+
 typedef struct rh_texture {
     uint16_t width; /* bit 15 = sRGB flag */
     uint16_t height;
@@ -225,6 +236,10 @@ typedef struct rh_scene_view {
     uint32_t li_indices_count;
     uint32_t macro_nodes_root;
     uint32_t light_nodes_root;
+    /* 8-wide trees (use_wide_bvh): when mnodes is set, nodes is empty and
+     * meshes[].node_index, macro_nodes_root and light_nodes_root index mnodes */
+    const rh_mbvh_node *mnodes;
+    uint32_t mnodes_count;
 } rh_scene_view;
 
 /* ---- wavefront records (this backend's own) ---- */
@@ -305,6 +320,7 @@ typedef struct rh_kernel_profile {
 
 static_assert(sizeof(rh_tri_accel) == 48, "tri_accel_t is 48 bytes");
 static_assert(sizeof(rh_bvh_node) == 32, "bvh_node_t is 32 bytes");
+static_assert(sizeof(rh_mbvh_node) == 224, "mbvh_node_t is 224 bytes");
 static_assert(sizeof(rh_texture) == 74, "texture_t is 74 bytes");
 static_assert(sizeof(rh_material) == 48, "material_t is 48 bytes");
 static_assert(sizeof(rh_light) == 48, "light_t is 48 bytes");

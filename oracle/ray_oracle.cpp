@@ -218,7 +218,7 @@ void safe_invert(const float v[3], float o[3]) {
 }
 
 // Traverse_MacroTree_WithStack_{ClosestHit,AnyHit} (CoreRef.cpp:1091-1131, 1224-1268)
-void macro(const rh_scene_view &sc, const float o[3], const float d[3], bool any, Hit &h, orc_counters *cnt) {
+void macro_binary(const rh_scene_view &sc, const float o[3], const float d[3], bool any, Hit &h, orc_counters *cnt) {
     if (sc.macro_nodes_root == RH_INVALID) return;
     float inv_d[3];
     safe_invert(d, inv_d);
@@ -254,6 +254,241 @@ void macro(const rh_scene_view &sc, const float o[3], const float d[3], bool any
             const bool found = micro(sc, oo, dd, inv2, sc.meshes[mi.mesh_index].node_index, (int)mii, any, h, cnt);
             if (any && found && (sc.tris[h.prim].ci & RH_TRI_SOLID_BIT)) return;
         }
+    }
+}
+
+// ------------------------------------------------------------------ 8-wide trees
+// bbox_test_oct (CoreRef.cpp:133-161): the slab test of the 8 slots, each
+// slot's entry distance in dist; bit i of the result = slot i hit (an unused
+// slot has NaN bounds and never passes)
+int bbox_test_oct(const float o[3], const float inv_d[3], float t, const rh_mbvh_node &n, float dist[8]) {
+    int mask = 0;
+    for (int i = 0; i < 8; i++) {
+        float lo_x = inv_d[0] * (n.bbox_min[0][i] - o[0]), hi_x = inv_d[0] * (n.bbox_max[0][i] - o[0]);
+        if (lo_x > hi_x) std::swap(lo_x, hi_x);
+        float lo_y = inv_d[1] * (n.bbox_min[1][i] - o[1]), hi_y = inv_d[1] * (n.bbox_max[1][i] - o[1]);
+        if (lo_y > hi_y) std::swap(lo_y, hi_y);
+        float lo_z = inv_d[2] * (n.bbox_min[2][i] - o[2]), hi_z = inv_d[2] * (n.bbox_max[2][i] - o[2]);
+        if (lo_z > hi_z) std::swap(lo_z, hi_z);
+        float tmin = lo_x > lo_y ? lo_x : lo_y;
+        if (lo_z > tmin) tmin = lo_z;
+        float tmax = hi_x < hi_y ? hi_x : hi_y;
+        if (hi_z < tmax) tmax = hi_z;
+        tmax *= 1.00000024f;
+        dist[i] = tmin;
+        if (tmin <= tmax && tmin <= t && tmax > 0) mask |= 1 << i;
+    }
+    return mask;
+}
+
+// TraversalStack (CoreRef.cpp:164-262): (node, entry distance) entries; after
+// pushing the hit children of a node in slot order, its top entries are
+// ordered by decreasing distance (nearest on top) by the reference's own
+// exchange sequences, whose handling of equal distances this keeps.
+struct WEntry {
+    uint32_t index;
+    float dist;
+};
+struct WStack {
+    std::vector<WEntry> e;
+    void push(uint32_t i, float d) { e.push_back({i, d}); }
+    WEntry pop() {
+        const WEntry x = e.back();
+        e.pop_back();
+        return x;
+    }
+    void sort_top3() {
+        WEntry *s = &e[e.size() - 3];
+        if (s[0].dist > s[1].dist) {
+            if (s[1].dist > s[2].dist) return;
+            if (s[0].dist > s[2].dist) {
+                std::swap(s[1], s[2]);
+            } else { // s2 >= s0 > s1
+                const WEntry a = s[0];
+                s[0] = s[2];
+                s[2] = s[1];
+                s[1] = a;
+            }
+        } else {
+            if (s[0].dist > s[2].dist) {
+                std::swap(s[0], s[1]);
+            } else if (s[2].dist > s[1].dist) {
+                std::swap(s[0], s[2]);
+            } else { // s1 >= s2 >= s0
+                const WEntry a = s[0];
+                s[0] = s[1];
+                s[1] = s[2];
+                s[2] = a;
+            }
+        }
+    }
+    void sort_top4() {
+        WEntry *s = &e[e.size() - 4];
+        const int pairs[5][2] = {{0, 1}, {2, 3}, {0, 2}, {1, 3}, {1, 2}};
+        for (const auto &p : pairs)
+            if (s[p[0]].dist < s[p[1]].dist) std::swap(s[p[0]], s[p[1]]);
+    }
+    void sort_top(int n) { // insertion sort, stable: equal distances keep push order
+        const size_t start = e.size() - n;
+        for (size_t i = start + 1; i < e.size(); i++) {
+            const WEntry key = e[i];
+            size_t j = i;
+            while (j > start && e[j - 1].dist < key.dist) {
+                e[j] = e[j - 1];
+                j--;
+            }
+            e[j] = key;
+        }
+    }
+};
+
+// The common walk of the four wide traversals (CoreRef.cpp:1133-1222,
+// 1270-1368, 1395-1468, 1500-1568): a node is entered without a test of its
+// own bounds; of its hit slots the nearest is entered next and the others
+// are stacked nearest-on-top; a stacked entry farther than the current t is
+// skipped when popped.  leaf(first, count) handles a leaf and returns true to
+// end the walk.  Counts interior node visits.
+template <class Leaf>
+void wide_walk(const rh_scene_view &sc, uint32_t root, const float o[3], const float inv_d[3], Hit &h, orc_counters *cnt,
+               Leaf leaf) {
+    WStack st;
+    st.push(root, 0.0f);
+    while (!st.e.empty()) {
+        WEntry cur = st.pop();
+        if (cur.dist > h.t) continue;
+        while (true) {
+            const rh_mbvh_node &n = sc.mnodes[cur.index];
+            if (n.child[0] & RH_LEAF_NODE_BIT) {
+                if (leaf(n.child[0] & RH_PRIM_INDEX_BITS, n.child[1])) return;
+                break;
+            }
+            if (cnt) cnt->nodes++;
+            float dist[8];
+            int mask = bbox_test_oct(o, inv_d, h.t, n, dist);
+            if (!mask) break;
+            int hit[8], k = 0;
+            while (mask) {
+                const int i = __builtin_ctz(mask);
+                mask &= mask - 1;
+                hit[k++] = i;
+            }
+            if (k == 1) {
+                cur.index = n.child[hit[0]];
+                continue;
+            }
+            if (k == 2) {
+                const int a = hit[0], b = hit[1];
+                if (dist[a] < dist[b]) {
+                    st.push(n.child[b], dist[b]);
+                    cur.index = n.child[a];
+                } else {
+                    st.push(n.child[a], dist[a]);
+                    cur.index = n.child[b];
+                }
+                continue;
+            }
+            for (int j = 0; j < k; j++) st.push(n.child[hit[j]], dist[hit[j]]);
+            if (k == 3) st.sort_top3();
+            else if (k == 4) st.sort_top4();
+            else st.sort_top(k);
+            cur.index = st.pop().index;
+        }
+    }
+}
+
+// Traverse_MicroTree_WithStack_{ClosestHit,AnyHit}, 8-wide (CoreRef.cpp:1395-1568);
+// the reference's wide walks call only the closest-hit form
+bool micro_wide(const rh_scene_view &sc, const float o[3], const float d[3], const float inv_d[3], uint32_t root,
+                int obj, bool any, Hit &h, orc_counters *cnt) {
+    bool res = false;
+    wide_walk(sc, root, o, inv_d, h, cnt, [&](uint32_t first, uint32_t count) {
+        // IntersectTris_ClosestHit / _AnyHit (CoreRef.cpp:760-825)
+        bool leaf_hit = false;
+        for (uint32_t k = first; k < first + count; k++) {
+            const uint32_t ti = sc.tri_indices[k];
+            if (cnt) cnt->tris++;
+            if (intersect(o, d, sc.tris[ti], ti, h)) {
+                h.obj = obj;
+                leaf_hit = true;
+            }
+            if (any && leaf_hit && (sc.tris[ti].ci & RH_TRI_SOLID_BIT)) break;
+        }
+        res |= leaf_hit;
+        return any && leaf_hit && (sc.tris[h.prim].ci & RH_TRI_SOLID_BIT);
+    });
+    return res;
+}
+
+// Traverse_MacroTree_WithStack_{ClosestHit,AnyHit}, 8-wide (CoreRef.cpp:1133-1222, 1270-1368)
+void macro_wide(const rh_scene_view &sc, const float o[3], const float d[3], bool any, Hit &h, orc_counters *cnt) {
+    if (sc.macro_nodes_root == RH_INVALID) return;
+    float inv_d[3];
+    safe_invert(d, inv_d);
+    wide_walk(sc, sc.macro_nodes_root, o, inv_d, h, cnt, [&](uint32_t first, uint32_t count) {
+        for (uint32_t i = first; i < first + count; i++) {
+            const uint32_t mii = sc.mi_indices[i];
+            const rh_mesh_instance &mi = sc.mesh_instances[mii];
+            if (cnt) cnt->instances++;
+            if (!bbox_test(o, inv_d, h.t, mi.bbox_min, mi.bbox_max)) continue;
+            const float *x = sc.transforms[mi.tr_index].inv_xform;
+            const float oo[3] = {x[0] * o[0] + x[4] * o[1] + x[8] * o[2] + x[12],
+                                 x[1] * o[0] + x[5] * o[1] + x[9] * o[2] + x[13],
+                                 x[2] * o[0] + x[6] * o[1] + x[10] * o[2] + x[14]};
+            const float dd[3] = {x[0] * d[0] + x[4] * d[1] + x[8] * d[2], x[1] * d[0] + x[5] * d[1] + x[9] * d[2],
+                                 x[2] * d[0] + x[6] * d[1] + x[10] * d[2]};
+            const float inv2[3] = {1.0f / dd[0], 1.0f / dd[1], 1.0f / dd[2]};
+            // the wide any-hit walk also takes the closest hit inside each
+            // instance (CoreRef.cpp:1357), then stops at a solid one
+            const bool found =
+                micro_wide(sc, oo, dd, inv2, sc.meshes[mi.mesh_index].node_index, (int)mii, false, h, cnt);
+            if (any && found && (sc.tris[h.prim].ci & RH_TRI_SOLID_BIT)) return true;
+        }
+        return false;
+    });
+}
+
+// the scene's trees: 8-wide when it has them
+void macro(const rh_scene_view &sc, const float o[3], const float d[3], bool any, Hit &h, orc_counters *cnt) {
+    if (sc.mnodes) macro_wide(sc, o, d, any, h, cnt);
+    else macro_binary(sc, o, d, any, h, cnt);
+}
+
+// The light-tree leaves (first, count) whose bounds hold P strictly, in the
+// order ComputeDirectLighting visits them (CoreRef.cpp:1935-1991): the binary
+// tree tests every popped node and pushes left then right; the wide tree tests
+// the slots of an interior node (not the root's own bounds) and pushes the
+// passing ones in slot order.
+void light_leaves(const rh_scene_view &sc, V3 P, std::vector<std::pair<uint32_t, uint32_t>> &out) {
+    out.clear();
+    if (sc.light_nodes_root == RH_INVALID) return;
+    auto inside = [&](const float *mn, const float *mx) {
+        return P.x > mn[0] && P.x < mx[0] && P.y > mn[1] && P.y < mx[1] && P.z > mn[2] && P.z < mx[2];
+    };
+    std::vector<uint32_t> st{sc.light_nodes_root};
+    while (!st.empty()) {
+        const uint32_t cur = st.back();
+        st.pop_back();
+        if (sc.mnodes) {
+            const rh_mbvh_node &n = sc.mnodes[cur];
+            if (n.child[0] & RH_LEAF_NODE_BIT) {
+                out.push_back({n.child[0] & RH_PRIM_INDEX_BITS, n.child[1]});
+                continue;
+            }
+            for (int i = 0; i < 8; i++) {
+                const float mn[3] = {n.bbox_min[0][i], n.bbox_min[1][i], n.bbox_min[2][i]};
+                const float mx[3] = {n.bbox_max[0][i], n.bbox_max[1][i], n.bbox_max[2][i]};
+                if (inside(mn, mx)) st.push_back(n.child[i]);
+            }
+            continue;
+        }
+        const rh_bvh_node &n = sc.nodes[cur];
+        if (!inside(n.bbox_min, n.bbox_max)) continue;
+        if (!(n.prim_index & RH_LEAF_NODE_BIT)) {
+            st.push_back(n.prim_index);
+            st.push_back(n.prim_count & RH_RIGHT_CHILD_BITS);
+            continue;
+        }
+        out.push_back({n.prim_index & RH_PRIM_INDEX_BITS, n.prim_count});
     }
 }
 
@@ -517,22 +752,11 @@ float visibility(Ctx &c, V3 p1, V3 p2, int hi, int rand_hash2) {
 V3 direct_light(Ctx &c, V3 I, V3 P, V3 N, V3 B, V3 pN, float sigma, int hi, int rh2, float ro1, float ro2) {
     const rh_scene_view &sc = c.sc;
     V3 col = {0, 0, 0};
-    if (sc.light_nodes_root == RH_INVALID) return col;
-    std::vector<uint32_t> st{sc.light_nodes_root};
-    while (!st.empty()) {
-        const uint32_t cur = st.back();
-        st.pop_back();
-        const rh_bvh_node &n = sc.nodes[cur];
-        if (!(P.x > n.bbox_min[0] && P.x < n.bbox_max[0] && P.y > n.bbox_min[1] && P.y < n.bbox_max[1] &&
-              P.z > n.bbox_min[2] && P.z < n.bbox_max[2]))
-            continue;
-        if (!(n.prim_index & RH_LEAF_NODE_BIT)) {
-            st.push_back(n.prim_index);
-            st.push_back(n.prim_count & RH_RIGHT_CHILD_BITS);
-            continue;
-        }
-        const uint32_t first = n.prim_index & RH_PRIM_INDEX_BITS;
-        for (uint32_t i = first; i < first + n.prim_count; i++) {
+    std::vector<std::pair<uint32_t, uint32_t>> leaves;
+    light_leaves(sc, P, leaves);
+    for (const auto &leaf : leaves) {
+        const uint32_t first = leaf.first;
+        for (uint32_t i = first; i < first + leaf.second; i++) {
             const rh_light &l = sc.lights[sc.li_indices[i]];
             V3 L = sub(P, v3(l.pos));
             const float distance = len(L);
@@ -951,7 +1175,21 @@ int orc_render(const rh_scene_view *sc, const rh_camera *cam, int w, int h, cons
     std::vector<float> halton(RH_HALTON_COUNT * RH_HALTON_SEQ_LEN);
     int have_halton = -1;
     float rmin[3] = {0, 0, 0}, cell[3] = {1, 1, 1};
-    if (sc->macro_nodes_root != RH_INVALID) {
+    if (sc->macro_nodes_root != RH_INVALID && sc->mnodes) {
+        // the wide root as RendererRef.cpp:52-66 reads it: slot 0 of a leaf,
+        // else the last used slot
+        const rh_mbvh_node &root = sc->mnodes[sc->macro_nodes_root];
+        const bool leaf = (root.child[0] & RH_LEAF_NODE_BIT) != 0;
+        float rmax[3] = {0, 0, 0};
+        for (int j = 0; j < (leaf ? 1 : 8); j++) {
+            if (!leaf && root.child[j] == RH_MBVH_EMPTY) continue;
+            for (int i = 0; i < 3; i++) {
+                rmin[i] = root.bbox_min[i][j];
+                rmax[i] = root.bbox_max[i][j];
+            }
+        }
+        for (int i = 0; i < 3; i++) cell[i] = (rmax[i] - rmin[i]) / 255;
+    } else if (sc->macro_nodes_root != RH_INVALID) {
         const rh_bvh_node &root = sc->nodes[sc->macro_nodes_root];
         for (int i = 0; i < 3; i++) {
             rmin[i] = root.bbox_min[i];

@@ -40,6 +40,7 @@ struct SceneSpy : public Ray::Ref::Scene {
     using Ray::Ref::Scene::meshes_;
     using Ray::Ref::Scene::mi_indices_;
     using Ray::Ref::Scene::nodes_;
+    using Ray::Ref::Scene::mnodes_;
     using Ray::Ref::Scene::texture_atlas_;
     using Ray::Ref::Scene::textures_;
     using Ray::Ref::Scene::transforms_;
@@ -142,6 +143,10 @@ int rhx_create_scene(rhx_renderer *r, rhx_scene **out) {
     auto *s = new rhx_scene;
     s->s = std::make_shared<SceneSpy>(r ? r->wide : false);
     *out = s;
+    return 0;
+}
+int rhx_create_scene_standalone(int use_wide_bvh, rhx_scene **out) {
+    *out = new rhx_scene{std::make_shared<SceneSpy>(use_wide_bvh != 0)};
     return 0;
 }
 void rhx_destroy_scene(rhx_scene *s) { delete s; }
@@ -296,6 +301,7 @@ int rhx_scene_view(rhx_scene *s, rh_scene_view *v) {
     RH_VIEW(vtx_indices, sc.vtx_indices_, uint32_t)
     RH_VIEW(vertices, sc.vertices_, rh_vertex)
     RH_VIEW(nodes, sc.nodes_, rh_bvh_node)
+    RH_VIEW(mnodes, sc.mnodes_, rh_mbvh_node)
     RH_VIEW(tris, sc.tris_, rh_tri_accel)
     RH_VIEW(tri_indices, sc.tri_indices_, uint32_t)
     RH_VIEW(materials, sc.materials_, rh_material)
@@ -376,7 +382,7 @@ static Ray::scene_data_t scene_data(SceneSpy &sc) {
     d.vtx_indices = sc.vtx_indices_.empty() ? nullptr : sc.vtx_indices_.data();
     d.vertices = sc.vertices_.empty() ? nullptr : sc.vertices_.data();
     d.nodes = sc.nodes_.empty() ? nullptr : sc.nodes_.data();
-    d.mnodes = nullptr;
+    d.mnodes = sc.mnodes_.empty() ? nullptr : sc.mnodes_.data();
     d.tris = sc.tris_.empty() ? nullptr : sc.tris_.data();
     d.tri_indices = sc.tri_indices_.empty() ? nullptr : sc.tri_indices_.data();
     d.materials = sc.materials_.empty() ? nullptr : sc.materials_.data();
@@ -386,20 +392,29 @@ static Ray::scene_data_t scene_data(SceneSpy &sc) {
     return d;
 }
 
-/* Closest-hit (any_hit == 0) or any-hit traversal of the binary macro/micro trees. */
+/* Closest-hit (any_hit == 0) or any-hit traversal of the macro/micro trees
+ * (binary, or 8-wide when the scene was built with use_wide_bvh). */
 int ref_trace(rhx_scene *s, const rh_ray_packet *rays, size_t n, int any_hit, const float *tmax, rh_hit *out) {
     SceneSpy &sc = *s->s;
-    if (sc.nodes_.empty()) {
-        g_err = "ref_trace needs a binary-BVH scene";
-        return -1;
-    }
+    const bool wide = !sc.mnodes_.empty();
     for (size_t i = 0; i < n; i++) {
         Ray::Ref::ray_packet_t r;
         memcpy(&r, &rays[i], sizeof(r));
         Ray::Ref::hit_data_t inter;
         inter.xy = r.xy;
         if (tmax) inter.t = tmax[i];
-        if (sc.macro_nodes_root_ != 0xffffffff) {
+        if (sc.macro_nodes_root_ != 0xffffffff && wide) {
+            // the 8-wide trees (CoreRef.cpp:1133-1222, 1270-1368)
+            if (!any_hit) {
+                Ray::Ref::Traverse_MacroTree_WithStack_ClosestHit(
+                    r, sc.mnodes_.data(), sc.macro_nodes_root_, sc.mesh_instances_.data(), sc.mi_indices_.data(),
+                    sc.meshes_.data(), sc.transforms_.data(), sc.tris_.data(), sc.tri_indices_.data(), inter);
+            } else {
+                Ray::Ref::Traverse_MacroTree_WithStack_AnyHit(
+                    r, sc.mnodes_.data(), sc.macro_nodes_root_, sc.mesh_instances_.data(), sc.mi_indices_.data(),
+                    sc.meshes_.data(), sc.transforms_.data(), sc.tris_.data(), sc.tri_indices_.data(), inter);
+            }
+        } else if (sc.macro_nodes_root_ != 0xffffffff) {
             if (!any_hit) {
                 Ray::Ref::Traverse_MacroTree_WithStack_ClosestHit(
                     r, sc.nodes_.data(), sc.macro_nodes_root_, sc.mesh_instances_.data(), sc.mi_indices_.data(),
@@ -422,15 +437,43 @@ int ref_trace(rhx_scene *s, const rh_ray_packet *rays, size_t n, int any_hit, co
     return 0;
 }
 
+} // extern "C"
+
+/* The sort hash grid of RendererRef.cpp:48-77: the macro root's box (for the
+ * wide tree, as that code reads it: slot 0 of a leaf, else the last used slot). */
+static void root_grid(SceneSpy &sc, float root_min[3], float cell_size[3]) {
+    float root_max[3];
+    if (!sc.mnodes_.empty()) {
+        const Ray::mbvh_node_t &root = sc.mnodes_[sc.macro_nodes_root_];
+        for (int i = 0; i < 3; i++) {
+            root_min[i] = Ray::MAX_DIST;
+            root_max[i] = -Ray::MAX_DIST;
+        }
+        const bool leaf = (root.child[0] & Ray::LEAF_NODE_BIT) != 0;
+        for (int j = 0; j < (leaf ? 1 : 8); j++) {
+            if (!leaf && root.child[j] == 0x7fffffff) continue;
+            for (int i = 0; i < 3; i++) {
+                root_min[i] = root.bbox_min[i][j];
+                root_max[i] = root.bbox_max[i][j];
+            }
+        }
+    } else {
+        const Ray::bvh_node_t &root = sc.nodes_[sc.macro_nodes_root_];
+        for (int i = 0; i < 3; i++) {
+            root_min[i] = root.bbox_min[i];
+            root_max[i] = root.bbox_max[i];
+        }
+    }
+    for (int i = 0; i < 3; i++) cell_size[i] = (root_max[i] - root_min[i]) / 255;
+}
+
+extern "C" {
+
 /* Ray hash keys of CoreRef.cpp:316-333 over the macro-tree root box. */
 int ref_ray_hashes(rhx_scene *s, const rh_ray_packet *rays, size_t n, uint32_t *out) {
     SceneSpy &sc = *s->s;
-    const Ray::bvh_node_t &root = sc.nodes_[sc.macro_nodes_root_];
     float root_min[3], cell_size[3];
-    for (int i = 0; i < 3; i++) {
-        root_min[i] = root.bbox_min[i];
-        cell_size[i] = (root.bbox_max[i] - root.bbox_min[i]) / 255;
-    }
+    root_grid(sc, root_min, cell_size);
     for (size_t i = 0; i < n; i++) {
         Ray::Ref::ray_packet_t r;
         memcpy(&r, &rays[i], sizeof(r));
@@ -442,12 +485,8 @@ int ref_ray_hashes(rhx_scene *s, const rh_ray_packet *rays, size_t n, uint32_t *
 /* SortRays_CPU (CoreRef.cpp:615-657): reorders rays in place. */
 int ref_sort_rays(rhx_scene *s, rh_ray_packet *rays, size_t n) {
     SceneSpy &sc = *s->s;
-    const Ray::bvh_node_t &root = sc.nodes_[sc.macro_nodes_root_];
     float root_min[3], cell_size[3];
-    for (int i = 0; i < 3; i++) {
-        root_min[i] = root.bbox_min[i];
-        cell_size[i] = (root.bbox_max[i] - root.bbox_min[i]) / 255;
-    }
+    root_grid(sc, root_min, cell_size);
     std::vector<uint32_t> hashes(n), scan(n);
     std::vector<Ray::ray_chunk_t> chunks(n), chunks_tmp(n);
     if (!n) return 0;

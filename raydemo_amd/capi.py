@@ -121,6 +121,7 @@ class SceneView(Structure):
         ("li_indices", c_void_p), ("li_indices_count", c_uint32),
         ("macro_nodes_root", c_uint32),
         ("light_nodes_root", c_uint32),
+        ("mnodes", c_void_p), ("mnodes_count", c_uint32),
     ]
 
 
@@ -128,6 +129,8 @@ class SceneView(Structure):
 TRI_DTYPE = np.dtype([("nu", "<f4"), ("nv", "<f4"), ("np", "<f4"), ("pu", "<f4"), ("pv", "<f4"), ("ci", "<i4"),
                       ("e0u", "<f4"), ("e0v", "<f4"), ("e1u", "<f4"), ("e1v", "<f4"), ("mi", "<u4"), ("back_mi", "<u4")])
 NODE_DTYPE = np.dtype([("bbox_min", "<f4", 3), ("prim_index", "<u4"), ("bbox_max", "<f4", 3), ("prim_count", "<u4")])
+MBVH_DTYPE = np.dtype([("bbox_min", "<f4", (3, 8)), ("bbox_max", "<f4", (3, 8)), ("child", "<u4", 8)])
+MBVH_EMPTY = 0x7FFFFFFF
 VERTEX_DTYPE = np.dtype([("p", "<f4", 3), ("n", "<f4", 3), ("b", "<f4", 3), ("t", "<f4", (2, 2))])
 MESH_DTYPE = np.dtype([("node_index", "<u4"), ("node_count", "<u4"), ("tris_index", "<u4"), ("tris_count", "<u4")])
 TRANSFORM_DTYPE = np.dtype([("xform", "<f4", 16), ("inv_xform", "<f4", 16)])
@@ -210,6 +213,7 @@ class RayLib:
             "rhx_triangle_count": (c_uint32, [P]),
             "rhx_node_count": (c_uint32, [P]),
             "rhx_scene_view": (c_int, [P, POINTER(SceneView)]),
+            "rhx_create_scene_standalone": (c_int, [c_int, POINTER(P)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -363,15 +367,17 @@ class Renderer:
 
 
 class _NoRenderer:
-    def __init__(self, lib):
+    def __init__(self, lib, wide=False):
         self.lib = lib
         self.h = None
+        self.wide = wide
 
 
-def standalone_scene(lib: RayLib) -> "Scene":
-    """A scene built on the host without a renderer; any renderer (or batch)
-    of the same library can render it."""
-    return Scene(_NoRenderer(lib))
+def standalone_scene(lib: RayLib, wide: bool = False) -> "Scene":
+    """A scene built on the host without a renderer, with binary or 8-wide
+    trees (use_wide_bvh); any renderer (or batch) of the same library can
+    render it."""
+    return Scene(_NoRenderer(lib, wide))
 
 
 class Batch:
@@ -474,7 +480,11 @@ class Scene:
         self.r = renderer
         self.lib = renderer.lib
         h_ = c_void_p()
-        self.lib.check(self.lib.lib.rhx_create_scene(renderer.h, C.byref(h_)), "rhx_create_scene")
+        if renderer.h is None:
+            self.lib.check(self.lib.lib.rhx_create_scene_standalone(int(getattr(renderer, "wide", False)),
+                                                                    C.byref(h_)), "rhx_create_scene_standalone")
+        else:
+            self.lib.check(self.lib.lib.rhx_create_scene(renderer.h, C.byref(h_)), "rhx_create_scene")
         self.h = h_
         self._keep = []
 
@@ -590,6 +600,7 @@ class Scene:
             "vtx_indices": _arr(v.vtx_indices, v.vtx_indices_count, np.uint32),
             "vertices": _arr(v.vertices, v.vertices_count, VERTEX_DTYPE),
             "nodes": _arr(v.nodes, v.nodes_count, NODE_DTYPE),
+            "mnodes": _arr(v.mnodes, v.mnodes_count, MBVH_DTYPE),
             "tris": _arr(v.tris, v.tris_count, TRI_DTYPE),
             "tri_indices": _arr(v.tri_indices, v.tri_indices_count, np.uint32),
             "materials": _arr(v.materials, v.materials_count, MATERIAL_DTYPE),

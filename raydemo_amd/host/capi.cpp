@@ -50,7 +50,7 @@ int rhx_create_renderer(const rhx_settings *s, uint32_t flags, rhx_renderer **ou
         st.w = s->w;
         st.h = s->h;
         st.device_index = s->device_index;
-        st.use_wide_bvh = false;
+        st.use_wide_bvh = s->use_wide_bvh != 0;
         std::ostringstream log;
         auto r = std::dynamic_pointer_cast<Ray::Hip::Renderer>(Ray::CreateRenderer(st, Ray::RendererHIP, log));
         r->set_sort_rays(s->sort_rays != 0);
@@ -138,6 +138,12 @@ int rhx_create_scene(rhx_renderer *r, rhx_scene **out) {
         // a scene without a renderer is allowed: host-side preprocessing only
         *out = new rhx_scene{r ? std::dynamic_pointer_cast<Ray::Hip::Scene>(r->r->CreateScene())
                                : std::make_shared<Ray::Hip::Scene>()};
+        return 0;
+    });
+}
+int rhx_create_scene_standalone(int use_wide_bvh, rhx_scene **out) {
+    return guarded([&] {
+        *out = new rhx_scene{std::make_shared<Ray::Hip::Scene>(use_wide_bvh != 0)};
         return 0;
     });
 }

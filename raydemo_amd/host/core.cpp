@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <deque>
@@ -673,6 +674,118 @@ int TextureSplitter::Insert_Recursive(int i, const int res[2]) {
     }
     nodes_[ch0].parent = nodes_[ch1].parent = i;
     return Insert_Recursive(ch0, res);
+}
+
+// ---- 8-wide trees ----
+namespace {
+bool is_leaf(const rh_bvh_node &n) { return (n.prim_index & RH_LEAF_NODE_BIT) != 0; }
+uint32_t right_of(const rh_bvh_node &n) { return n.prim_count & RH_RIGHT_CHILD_BITS; }
+
+// the (up to 8) binary nodes that become the children of the wide node made
+// for interior node n, in left-to-right order
+int wide_children(const rh_bvh_node *nodes, const rh_bvh_node &n, uint32_t out[8]) {
+    int count = 0;
+    const uint32_t halves[2] = {n.prim_index, right_of(n)};
+    for (uint32_t h : halves) {
+        const rh_bvh_node &c = nodes[h];
+        if (is_leaf(c)) {
+            out[count++] = h;
+            continue;
+        }
+        const uint32_t quarters[2] = {c.prim_index, right_of(c)};
+        for (uint32_t q : quarters) {
+            const rh_bvh_node &g = nodes[q];
+            if (is_leaf(g)) {
+                out[count++] = q;
+            } else {
+                out[count++] = g.prim_index;
+                out[count++] = right_of(g);
+            }
+        }
+    }
+    return count;
+}
+} // namespace
+
+uint32_t FlattenBVH(const rh_bvh_node *nodes, uint32_t root, std::vector<rh_mbvh_node> &out) {
+    const rh_bvh_node &n = nodes[root];
+    const uint32_t self = (uint32_t)out.size();
+    out.emplace_back();
+    if (is_leaf(n)) {
+        rh_mbvh_node &w = out[self];
+        memset(&w, 0, sizeof(w));
+        for (int a = 0; a < 3; a++) {
+            w.bbox_min[a][0] = n.bbox_min[a];
+            w.bbox_max[a][0] = n.bbox_max[a];
+        }
+        w.child[0] = n.prim_index;
+        w.child[1] = n.prim_count;
+        return self;
+    }
+    uint32_t kids[8];
+    const int count = wide_children(nodes, n, kids);
+
+    // slot = Morton code of the child's box centre on a 2x2x2 grid over the
+    // centres' bounding box (top widened by 0.001)
+    vec3 centre[8];
+    vec3 lo = {{FLT_MAX, FLT_MAX, FLT_MAX}}, hi = {{-FLT_MAX, -FLT_MAX, -FLT_MAX}};
+    for (int i = 0; i < count; i++) {
+        const rh_bvh_node &k = nodes[kids[i]];
+        for (int a = 0; a < 3; a++) centre[i][a] = 0.5f * (k.bbox_min[a] + k.bbox_max[a]);
+        lo = vmin(lo, centre[i]);
+        hi = vmax(hi, centre[i]);
+    }
+    float scale[3];
+    for (int a = 0; a < 3; a++) scale[a] = 2.0f / ((hi[a] + 0.001f) - lo[a]);
+    uint32_t slot_of[8];
+    for (int s = 0; s < 8; s++) slot_of[s] = 0xffffffffu;
+    for (int i = 0; i < count; i++) {
+        uint32_t code = 0;
+        for (int a = 0; a < 3; a++) code |= (uint32_t)((centre[i][a] - lo[a]) * scale[a]) << a;
+        code %= 8; // a coordinate of 2 (rounding at huge extents) stays in the table
+        while (slot_of[code] != 0xffffffffu) code = (code + 1) % 8;
+        slot_of[code] = kids[i];
+    }
+    uint32_t child[8];
+    for (int s = 0; s < 8; s++) {
+        child[s] = slot_of[s] != 0xffffffffu ? FlattenBVH(nodes, slot_of[s], out) : RH_MBVH_EMPTY;
+    }
+    rh_mbvh_node &w = out[self]; // out may have grown: take the reference now
+    for (int s = 0; s < 8; s++) {
+        w.child[s] = child[s];
+        for (int a = 0; a < 3; a++) {
+            if (child[s] != RH_MBVH_EMPTY) {
+                w.bbox_min[a][s] = nodes[slot_of[s]].bbox_min[a];
+                w.bbox_max[a][s] = nodes[slot_of[s]].bbox_max[a];
+            } else {
+                w.bbox_min[a][s] = w.bbox_max[a][s] = NAN;
+            }
+        }
+    }
+    return self;
+}
+
+void WideRootBounds(const rh_bvh_node *nodes, uint32_t root, float bmin[3], float bmax[3]) {
+    const rh_bvh_node &n = nodes[root];
+    if (is_leaf(n)) {
+        for (int a = 0; a < 3; a++) {
+            bmin[a] = n.bbox_min[a];
+            bmax[a] = n.bbox_max[a];
+        }
+        return;
+    }
+    uint32_t kids[8];
+    const int count = wide_children(nodes, n, kids);
+    for (int a = 0; a < 3; a++) {
+        bmin[a] = RH_MAX_DIST;
+        bmax[a] = -RH_MAX_DIST;
+    }
+    for (int i = 0; i < count; i++) {
+        for (int a = 0; a < 3; a++) {
+            bmin[a] = std::min(bmin[a], nodes[kids[i]].bbox_min[a]);
+            bmax[a] = std::max(bmax[a], nodes[kids[i]].bbox_max[a]);
+        }
+    }
 }
 
 } // namespace RayHip

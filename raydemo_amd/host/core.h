@@ -53,6 +53,17 @@ uint32_t PreprocessPrims_SAH(const prim_t *prims, size_t prims_count, const floa
                              const bvh_settings_t &s, std::vector<rh_bvh_node> &out_nodes,
                              std::vector<uint32_t> &out_indices);
 
+// FlattenBVH_Recursive (Core.cpp:683-819): appends the 8-wide form of the
+// binary tree rooted at nodes[root] to out (depth-first, a node before its
+// children) and returns the index of its root.  An interior node takes the
+// binary nodes two levels below it (a leaf child stands for itself), placed
+// in the slot of their centre's Morton code within the children's centre box,
+// the next free slot on a collision.
+uint32_t FlattenBVH(const rh_bvh_node *nodes, uint32_t root, std::vector<rh_mbvh_node> &out);
+// Bounds of the wide node FlattenBVH makes for nodes[root]: slot 0 of a leaf,
+// else the union of its children's boxes (SceneRef.cpp:383-406, 52-66)
+void WideRootBounds(const rh_bvh_node *nodes, uint32_t root, float bmin[3], float bmax[3]);
+
 // ConstructCamera (Core.cpp:856-886)
 void ConstructCamera(int type, int filter, int dtype, const float origin[3], const float fwd[3], const float up[3],
                      float fov, float gamma, float focus_distance, float focus_factor, rh_camera *cam);

@@ -53,6 +53,17 @@ class Scene : public SceneBase {
     bool macro_dirty_ = false, light_dirty_ = false;
     uint64_t version_ = 1;
 
+    // 8-wide trees (settings use_wide_bvh, the reference's default): the
+    // binary trees above stay the build's working form, and the wide arrays
+    // are flattened from them before use, mesh trees first, then the macro
+    // and light trees (SceneRef.cpp:172-181, 506-515, 577-586)
+    bool wide_ = false;
+    bool wide_dirty_ = true;
+    std::vector<rh_mbvh_node> mnodes_;
+    std::vector<rh_mesh> wide_meshes_;
+    uint32_t wide_macro_root_ = 0xffffffff, wide_light_root_ = 0xffffffff;
+    void RebuildWide();
+
     void RemoveNodes(uint32_t node_index, uint32_t node_count);
     void WriteBlock(const rh_tex_mip &m, const std::vector<uint32_t> &data, const int res[2]);
     void RebuildMacroBVH();
@@ -60,7 +71,8 @@ class Scene : public SceneBase {
     int AllocateInAtlas(const int res[2], int pos[2]);
 
 public:
-    Scene();
+    explicit Scene(bool use_wide_bvh = false);
+    bool wide() const { return wide_; }
 
     void GetEnvironment(environment_desc_t &env) override;
     void SetEnvironment(const environment_desc_t &env) override;
@@ -101,6 +113,7 @@ class Renderer : public RendererBase {
     rh_context *ctx_ = nullptr;
     int w_ = 0, h_ = 0;
     bool sort_rays_ = true;
+    bool use_wide_bvh_ = false;
 
     std::vector<uint16_t> permutations_;
     mutable std::vector<pixel_color_t> host_pixels_;

@@ -20,6 +20,7 @@ Renderer::Renderer(const settings_t &s, int device_index) {
         throw std::runtime_error(std::string("HIP backend: ") + rh_last_error());
     }
     permutations_ = RayHip::ComputeRadicalInversePermutations();
+    use_wide_bvh_ = s.use_wide_bvh;
     Resize(s.w, s.h);
 }
 
@@ -51,7 +52,7 @@ const pixel_color_t *Renderer::get_pixels_ref() const {
     return host_pixels_.data();
 }
 
-std::shared_ptr<SceneBase> Renderer::CreateScene() { return std::make_shared<Scene>(); }
+std::shared_ptr<SceneBase> Renderer::CreateScene() { return std::make_shared<Scene>(use_wide_bvh_); }
 
 // Uploads the scene if it changed, advances the region's iteration and
 // refreshes/uploads its Halton table exactly as the Ref renderer does

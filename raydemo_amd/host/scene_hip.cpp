@@ -98,7 +98,7 @@ namespace Hip {
 
 static_assert(sizeof(camera_t) == sizeof(rh_camera), "camera_t layout");
 
-Scene::Scene() {
+Scene::Scene(bool use_wide_bvh) : wide_(use_wide_bvh) {
     const int atlas_res[2] = {RH_TEXTURE_ATLAS_SIZE, RH_TEXTURE_ATLAS_SIZE};
     splitters_.emplace_back(atlas_res);
 
@@ -341,6 +341,7 @@ uint32_t Scene::AddMesh(const mesh_desc_t &_m) {
                                     new_vtx_indices.size());
     }
     vtx_indices_.insert(vtx_indices_.end(), new_vtx_indices.begin(), new_vtx_indices.end());
+    wide_dirty_ = true;
     version_++;
     return (uint32_t)(meshes_.size() - 1);
 }
@@ -451,15 +452,22 @@ uint32_t Scene::AddMeshInstance(uint32_t mesh_index, const float *xform) {
     return mi_index;
 }
 
-// SceneRef::SetMeshInstanceTransform (SceneRef.cpp:370-414), binary BVH
+// SceneRef::SetMeshInstanceTransform (SceneRef.cpp:370-414)
 void Scene::SetMeshInstanceTransform(uint32_t mi_index, const float *xform) {
     rh_mesh_instance &mi = mesh_instances_[mi_index];
     rh_transform &tr = transforms_[mi.tr_index];
     memcpy(tr.xform, xform, 16 * sizeof(float));
     RayHip::InverseMatrix(tr.xform, tr.inv_xform);
     const rh_mesh &m = meshes_[mi.mesh_index];
-    const rh_bvh_node &n = nodes_[m.node_index];
-    RayHip::TransformBoundingBox(n.bbox_min, n.bbox_max, xform, mi.bbox_min, mi.bbox_max);
+    if (wide_) {
+        // the bounds of the mesh's wide root (SceneRef.cpp:383-406)
+        float bmin[3], bmax[3];
+        RayHip::WideRootBounds(nodes_.data(), m.node_index, bmin, bmax);
+        RayHip::TransformBoundingBox(bmin, bmax, xform, mi.bbox_min, mi.bbox_max);
+    } else {
+        const rh_bvh_node &n = nodes_[m.node_index];
+        RayHip::TransformBoundingBox(n.bbox_min, n.bbox_max, xform, mi.bbox_min, mi.bbox_max);
+    }
     macro_dirty_ = true;
     version_++;
 }
@@ -560,23 +568,62 @@ void Scene::Finalize() {
     if (light_dirty_) {
         RebuildLightBVH();
         light_dirty_ = false;
+        wide_dirty_ = true;
     }
     if (macro_dirty_) {
         RebuildMacroBVH();
         macro_dirty_ = false;
+        wide_dirty_ = true;
     }
+    if (wide_ && wide_dirty_) RebuildWide();
+}
+
+// FlattenBVH of every tree: the mesh trees in mesh order, then the macro and
+// light trees.  Node indices differ from the reference's array (which the
+// reference edits in place as meshes, instances and lights come and go), the
+// trees themselves do not.
+void Scene::RebuildWide() {
+    mnodes_.clear();
+    wide_meshes_ = meshes_;
+    for (rh_mesh &m : wide_meshes_) {
+        const uint32_t before = (uint32_t)mnodes_.size();
+        m.node_index = RayHip::FlattenBVH(nodes_.data(), m.node_index, mnodes_);
+        m.node_count = (uint32_t)mnodes_.size() - before;
+    }
+    wide_macro_root_ = macro_nodes_root_ != 0xffffffff ? RayHip::FlattenBVH(nodes_.data(), macro_nodes_root_, mnodes_)
+                                                       : 0xffffffff;
+    wide_light_root_ = light_nodes_root_ != 0xffffffff ? RayHip::FlattenBVH(nodes_.data(), light_nodes_root_, mnodes_)
+                                                       : 0xffffffff;
+    wide_dirty_ = false;
 }
 
 uint32_t Scene::node_count() {
     Finalize();
-    return (uint32_t)nodes_.size();
+    return (uint32_t)(wide_ ? mnodes_.size() : nodes_.size());
 }
 
 // camera record of the current camera; the ray-sort hash grid spans the
 // macro-tree root box in 255 cells per axis (RendererRef.cpp:66-77)
 void Scene::PassCamera(rh_pass_desc &pass) {
     memcpy(&pass.cam, &camera(current_cam()), sizeof(rh_camera));
-    if (macro_nodes_root_ != 0xffffffff) {
+    if (wide_ && wide_macro_root_ != 0xffffffff) {
+        // the wide root's box as the reference's wide path reads it: slot 0 of
+        // a leaf, else the box of the last used slot (RendererRef.cpp:52-66)
+        const rh_mbvh_node &root = mnodes_[wide_macro_root_];
+        float lo[3] = {RH_MAX_DIST, RH_MAX_DIST, RH_MAX_DIST}, hi[3] = {-RH_MAX_DIST, -RH_MAX_DIST, -RH_MAX_DIST};
+        for (int j = 0; j < 8; j++) {
+            if (!(root.child[0] & RH_LEAF_NODE_BIT) && root.child[j] == RH_MBVH_EMPTY) continue;
+            for (int i = 0; i < 3; i++) {
+                lo[i] = root.bbox_min[i][j];
+                hi[i] = root.bbox_max[i][j];
+            }
+            if (root.child[0] & RH_LEAF_NODE_BIT) break;
+        }
+        for (int i = 0; i < 3; i++) {
+            pass.root_min[i] = lo[i];
+            pass.cell_size[i] = (hi[i] - lo[i]) / 255;
+        }
+    } else if (macro_nodes_root_ != 0xffffffff) {
         const rh_bvh_node &root = nodes_[macro_nodes_root_];
         for (int i = 0; i < 3; i++) {
             pass.root_min[i] = root.bbox_min[i];
@@ -599,7 +646,12 @@ rh_scene_view Scene::View() {
     RH_VIEW(transforms, transforms_)
     RH_VIEW(vtx_indices, vtx_indices_)
     RH_VIEW(vertices, vertices_)
-    RH_VIEW(nodes, nodes_)
+    if (wide_) {
+        RH_VIEW(meshes, wide_meshes_)
+        RH_VIEW(mnodes, mnodes_)
+    } else {
+        RH_VIEW(nodes, nodes_)
+    }
     RH_VIEW(tris, tris_)
     RH_VIEW(tri_indices, tri_indices_)
     RH_VIEW(materials, materials_)
@@ -609,8 +661,8 @@ rh_scene_view Scene::View() {
     RH_VIEW(li_indices, li_indices_)
 #undef RH_VIEW
     v.tex_mips = tex_mips_.empty() ? nullptr : tex_mips_.data();
-    v.macro_nodes_root = macro_nodes_root_;
-    v.light_nodes_root = light_nodes_root_;
+    v.macro_nodes_root = wide_ ? wide_macro_root_ : macro_nodes_root_;
+    v.light_nodes_root = wide_ ? wide_light_root_ : light_nodes_root_;
     return v;
 }
 

@@ -64,17 +64,9 @@ def golden_scene1():
     return {k: d[k] for k in d.files}
 
 
-class _NoRenderer:
-    """Stands in for a renderer so a scene can be built without a device."""
-
-    def __init__(self, lib):
-        self.lib = lib
-        self.h = None
-
-
-def standalone_scene(lib):
-    from raydemo_amd.capi import Scene
-    return Scene(_NoRenderer(lib))
+def standalone_scene(lib, wide=False):
+    from raydemo_amd.capi import standalone_scene as make
+    return make(lib, wide)
 
 
 @pytest.fixture(scope="session")
