@@ -171,7 +171,8 @@ __device__ __forceinline__ bool stage_active(const BatchEntry &e, int stage) {
     return stage - 1 < e.depth;
 }
 
-__global__ void __launch_bounds__(kBlock) k_b_trace(BatchDev b, int stage, int q, int work) {
+// Tree: the trees of the batch's scenes (kTreeAny when they are mixed)
+template <int Tree> __global__ void __launch_bounds__(kBlock) k_b_trace(BatchDev b, int stage, int q, int work) {
     extern __shared__ uint32_t lds_stack[];
     const int *counts = b.counts + stage * b.n_scenes;
     for_each_tile(b, counts, b.work + work, [&](int s) { return stage_active(b.table[s], stage); },
@@ -191,7 +192,7 @@ __global__ void __launch_bounds__(kBlock) k_b_trace(BatchDev b, int stage, int q
                       a.overflow = b.stats + 4;
                       a.visits = nullptr;
                       if (stage == 0) a.ray_counter = nullptr; // primary rays are counted at generation
-                      trace_body<false, false>(a, base, lds_stack);
+                      trace_body<false, false, Tree>(a, base, lds_stack);
                   });
 }
 
@@ -247,7 +248,7 @@ template <bool Defer> __global__ void __launch_bounds__(kBlock) k_b_shade(BatchD
                   });
 }
 
-__global__ void __launch_bounds__(kBlock) k_b_shadow(BatchDev b, int stage, int work) {
+template <int Tree> __global__ void __launch_bounds__(kBlock) k_b_shadow(BatchDev b, int stage, int work) {
     extern __shared__ uint32_t lds_stack[];
     const int *counts = b.shadow_counts + stage * b.n_scenes;
     for_each_tile(b, counts, b.work + work, [&](int s) { return b.table[s].nee_slots != 0; },
@@ -266,7 +267,7 @@ __global__ void __launch_bounds__(kBlock) k_b_shadow(BatchDev b, int stage, int 
                       a.lds_depth = b.lds_depth;
                       a.shadow_counter = b.stats + 2;
                       a.overflow = b.stats + 4;
-                      shadow_body(a, base, lds_stack);
+                      shadow_body<Tree>(a, base, lds_stack);
                   });
 }
 
@@ -295,7 +296,8 @@ __global__ void __launch_bounds__(kBlock) k_b_combine(BatchDev b, int stage, int
 
 // the rays entering `stage` follow their paths to each scene's depth in one
 // lane; Defer: scenes whose light samples are resolved after shading (1..8 lights)
-template <bool Defer> __global__ void __launch_bounds__(kBlock) k_b_tail(BatchDev b, int stage, int q, int work) {
+template <bool Defer, int Tree>
+__global__ void __launch_bounds__(kBlock) k_b_tail(BatchDev b, int stage, int q, int work) {
     extern __shared__ uint32_t lds_stack[];
     const int *counts = b.counts + stage * b.n_scenes;
     for_each_tile(b, counts, b.work + work,
@@ -303,7 +305,7 @@ template <bool Defer> __global__ void __launch_bounds__(kBlock) k_b_tail(BatchDe
                   [&](int s, uint32_t base) {
                       const BatchEntry e = entry(b, s);
                       const ShadeArgs a = batch_shade_args(b, e, s, stage, q, Defer);
-                      path_tail_body<Defer>(a, stage - 1, e.depth, base, lds_stack);
+                      path_tail_body<Defer, Tree>(a, stage - 1, e.depth, base, lds_stack);
                   });
 }
 
@@ -401,6 +403,38 @@ template <typename T> int dev_alloc(T **p, size_t n) {
     return 0;
 }
 
+bool any_wide(const rh_batch *b) {
+    for (const BatchEntry &e : b->entries)
+        if (e.sc.wnodes) return true;
+    return false;
+}
+
+int batch_tree(const rh_batch *b) {
+    int wide = 0;
+    for (const BatchEntry &e : b->entries) wide += e.sc.wnodes ? 1 : 0;
+    return wide == 0 ? kTreeBinary : (wide == (int)b->entries.size() ? kTreeWide : kTreeAny);
+}
+
+// The per-bounce launches of a pass: stage 0 is the primary hit; stage k >= 1
+// traces bounce k - 1.  Bounces from path_tail on run in the tail kernel.
+template <int Tree>
+void launch_stages(rh_batch *b, const BatchDev &d, size_t lds, dim3 grid, dim3 blk, int tail, int &work) {
+    int q = 0;
+    for (int stage = 0; stage <= b->max_depth; stage++) {
+        if (stage >= tail) {
+            hipLaunchKernelGGL((k_b_tail<true, Tree>), grid, blk, lds, b->stream, d, stage, q, work++);
+            hipLaunchKernelGGL((k_b_tail<false, Tree>), grid, blk, lds, b->stream, d, stage, q, work++);
+            break;
+        }
+        hipLaunchKernelGGL(k_b_trace<Tree>, grid, blk, lds, b->stream, d, stage, q, work++);
+        hipLaunchKernelGGL(k_b_shade<true>, grid, blk, 0, b->stream, d, stage, q, work++);
+        hipLaunchKernelGGL(k_b_shade<false>, grid, blk, lds, b->stream, d, stage, q, work++);
+        hipLaunchKernelGGL(k_b_shadow<Tree>, grid, blk, lds, b->stream, d, stage, work++);
+        hipLaunchKernelGGL(k_b_combine, grid, blk, 0, b->stream, d, stage, work++);
+        q ^= 1;
+    }
+}
+
 // Allocates the shared queues for the scenes added so far and uploads the table.
 int build(rh_batch *b) {
     if (b->built) return 0;
@@ -431,7 +465,8 @@ int build(rh_batch *b) {
     rc |= dev_alloc(&b->pend_alb, P);
     rc |= dev_alloc(&b->pend_count, P);
     b->spill_cap = std::max(P, NS);
-    rc |= dev_alloc(&b->spill, (size_t)kSpillDepth * b->spill_cap);
+    // entries, and their distances when a scene has 8-wide trees
+    rc |= dev_alloc(&b->spill, (size_t)kSpillDepth * (any_wide(b) ? 2 : 1) * b->spill_cap);
     if (rc) return rc;
     RH_CHECK(hipMemcpyAsync(b->table, b->entries.data(), S * sizeof(BatchEntry), hipMemcpyHostToDevice, b->stream));
     RH_CHECK(hipMemsetAsync(b->pass, 0, sizeof(int), b->stream));
@@ -488,27 +523,15 @@ int enqueue_pass(rh_batch *b) {
     const int launches = 2 + 5 * tail + (tail <= b->max_depth ? 2 : 0);
     if (launches > kMaxStages) return arg_error("batch pass needs more stage counters than reserved");
     const BatchDev d = dev_view(b);
-    const size_t lds = (size_t)b->lds_depth * kBlock * sizeof(uint32_t);
+    const size_t lds = (size_t)b->lds_depth * kBlock * sizeof(uint32_t) * (any_wide(b) ? 2 : 1);
     const dim3 grid((unsigned)(b->cus * b->blocks_per_cu)), blk(kBlock);
     int work = 0;
     hipLaunchKernelGGL(k_b_begin_pass, dim3(1), dim3(1024), 0, b->stream, d, b->table, (int)kMaxStages);
     hipLaunchKernelGGL(k_b_gen, grid, blk, 0, b->stream, d, work++);
-    int q = 0;
-    // stage 0 is the primary hit; stage k >= 1 traces bounce k - 1.  Bounces
-    // from path_tail on run in the tail kernel.
-    for (int stage = 0; stage <= b->max_depth; stage++) {
-        if (stage >= tail) {
-            hipLaunchKernelGGL(k_b_tail<true>, grid, blk, lds, b->stream, d, stage, q, work++);
-            hipLaunchKernelGGL(k_b_tail<false>, grid, blk, lds, b->stream, d, stage, q, work++);
-            break;
-        }
-        hipLaunchKernelGGL(k_b_trace, grid, blk, lds, b->stream, d, stage, q, work++);
-        hipLaunchKernelGGL(k_b_shade<true>, grid, blk, 0, b->stream, d, stage, q, work++);
-        hipLaunchKernelGGL(k_b_shade<false>, grid, blk, lds, b->stream, d, stage, q, work++);
-        hipLaunchKernelGGL(k_b_shadow, grid, blk, lds, b->stream, d, stage, work++);
-        hipLaunchKernelGGL(k_b_combine, grid, blk, 0, b->stream, d, stage, work++);
-        q ^= 1;
-    }
+    const int tree = batch_tree(b);
+    if (tree == kTreeBinary) launch_stages<kTreeBinary>(b, d, lds, grid, blk, tail, work);
+    else if (tree == kTreeWide) launch_stages<kTreeWide>(b, d, lds, grid, blk, tail, work);
+    else launch_stages<kTreeAny>(b, d, lds, grid, blk, tail, work);
     hipLaunchKernelGGL(k_b_mix, grid, blk, 0, b->stream, d, work++);
     RH_CHECK(hipGetLastError());
     return 0;

@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "rh_types.h"
 
 namespace rh {
@@ -109,6 +111,11 @@ struct Scene {
     rh_environment env;
     uint32_t macro_root;
     uint32_t light_root;
+    // 8-wide trees (see the wide traversal below); null for binary scenes
+    const float4 *wnodes;
+    // the wide light tree is a single leaf: its lights are taken for every
+    // point, with no bounds test (the wide walk tests only child slots)
+    uint32_t light_untested;
 };
 
 struct Hit {
@@ -267,6 +274,55 @@ struct Stack {
     uint32_t block;
     uint32_t sp;
     uint32_t overflow;
+    // the 8-wide walk keeps an entry distance beside each entry: the LDS
+    // stack's second half and the spill slice's second half
+    float *lds_d;
+    float *spill_d;
+
+    __device__ __forceinline__ void init(uint32_t *lds_base, uint32_t *spill_base, uint32_t stride, uint32_t depth,
+                                         uint32_t spill_entries) {
+        lds = lds_base;
+        spill = spill_base;
+        spill_stride = stride;
+        lds_depth = depth;
+        spill_depth = spill_entries;
+        lane = threadIdx.x;
+        block = blockDim.x;
+        sp = 0;
+        overflow = 0;
+        lds_d = reinterpret_cast<float *>(lds_base + (size_t)depth * blockDim.x);
+        spill_d = reinterpret_cast<float *>(spill_base + (size_t)spill_entries * stride);
+    }
+    // entry k (any position below capacity) with its distance
+    __device__ __forceinline__ void put(uint32_t k, uint32_t v, float d) {
+        if (k < lds_depth) {
+            lds[k * block + lane] = v;
+            lds_d[k * block + lane] = d;
+        } else if (k < lds_depth + spill_depth) {
+            spill[(size_t)(k - lds_depth) * spill_stride] = v;
+            spill_d[(size_t)(k - lds_depth) * spill_stride] = d;
+        } else {
+            overflow++;
+        }
+    }
+    __device__ __forceinline__ void push2(uint32_t v, float d) {
+        if (sp < lds_depth + spill_depth) {
+            put(sp, v, d);
+            sp++;
+        } else {
+            overflow++;
+        }
+    }
+    __device__ __forceinline__ void pop2(uint32_t &v, float &d) {
+        --sp;
+        if (sp < lds_depth) {
+            v = lds[sp * block + lane];
+            d = lds_d[sp * block + lane];
+        } else {
+            v = spill[(size_t)(sp - lds_depth) * spill_stride];
+            d = spill_d[(size_t)(sp - lds_depth) * spill_stride];
+        }
+    }
 
     __device__ __forceinline__ void push(uint32_t v) {
         if (sp < lds_depth) {
@@ -437,6 +493,303 @@ __device__ __forceinline__ void traverse_fast(const Scene &sc, f3 o, f3 d, Hit &
         have = pop_entered<true>(sc, 0, o, inv_d, hit.t, st, cur, cnt);
         if (!have) break;
     }
+}
+
+// ---------------------------------------------------------------- 8-wide trees
+// The reference's default trees (use_wide_bvh): FlattenBVH's 8-wide nodes,
+// walked as Traverse_{Macro,Micro}Tree_WithStack_ClosestHit (CoreRef.cpp:
+// 1133-1222, 1395-1468) walk them.  Device form of node i: 8 slots of two
+// float4 at wnodes[16 i + 2 s]: a = (bbox_min, word), b = (bbox_max, count),
+// word = the child node's index, or prim index | RH_LEAF_NODE_BIT for a leaf
+// child (count = its prim count: a leaf is entered without loading its
+// node), or RH_MBVH_EMPTY.  A leaf node, reachable only as a tree root, holds
+// itself in slot 0.  Stacked leaf children are RH_LEAF_NODE_BIT | (8 node + s),
+// a reference to the slot that describes them.
+constexpr uint32_t kWideSlotRef = RH_LEAF_NODE_BIT;
+
+// slab test of one slot with its entry distance (bbox_test_oct, CoreRef.cpp:133-161)
+template <bool Finite>
+__device__ __forceinline__ bool slot_hit(f3 o, f3 inv_d, float t, float4 a, float4 b, float &tmin) {
+    if (Finite) {
+        const float lx = inv_d.x * (a.x - o.x), hx = inv_d.x * (b.x - o.x);
+        const float ly = inv_d.y * (a.y - o.y), hy = inv_d.y * (b.y - o.y);
+        const float lz = inv_d.z * (a.z - o.z), hz = inv_d.z * (b.z - o.z);
+        tmin = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(lx, hx), __builtin_fminf(ly, hy)),
+                               __builtin_fminf(lz, hz));
+        const float tmax =
+            __builtin_fminf(__builtin_fminf(__builtin_fmaxf(lx, hx), __builtin_fmaxf(ly, hy)), __builtin_fmaxf(lz, hz)) *
+            1.00000024f;
+        return tmin <= tmax && tmin <= t && tmax > 0;
+    }
+    float lo_x = inv_d.x * (a.x - o.x), hi_x = inv_d.x * (b.x - o.x);
+    if (lo_x > hi_x) {
+        const float tmp = lo_x;
+        lo_x = hi_x;
+        hi_x = tmp;
+    }
+    float lo_y = inv_d.y * (a.y - o.y), hi_y = inv_d.y * (b.y - o.y);
+    if (lo_y > hi_y) {
+        const float tmp = lo_y;
+        lo_y = hi_y;
+        hi_y = tmp;
+    }
+    float lo_z = inv_d.z * (a.z - o.z), hi_z = inv_d.z * (b.z - o.z);
+    if (lo_z > hi_z) {
+        const float tmp = lo_z;
+        lo_z = hi_z;
+        hi_z = tmp;
+    }
+    tmin = lo_x > lo_y ? lo_x : lo_y;
+    if (lo_z > tmin) tmin = lo_z;
+    float tmax = hi_x < hi_y ? hi_x : hi_y;
+    if (hi_z < tmax) tmax = hi_z;
+    tmax *= 1.00000024f;
+    return tmin <= tmax && tmin <= t && tmax > 0;
+}
+
+// f(integral_constant<int, i>) for i = 0..N-1, unrolled by instantiation: the
+// array indices below are constants from the start, so the arrays live in
+// registers (a loop unrolled later leaves indexed loads that put them in
+// scratch memory)
+template <int N, int I = 0, class F> __device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<N, I + 1>(f);
+    }
+}
+
+// v[i] for a per-lane i as a tree of selects on the bits of i
+template <class T> __device__ __forceinline__ T pick8(const T (&v)[8], uint32_t i) {
+    const bool b0 = i & 1u, b1 = i & 2u, b2 = i & 4u;
+    const T a0 = b0 ? v[1] : v[0], a1 = b0 ? v[3] : v[2], a2 = b0 ? v[5] : v[4], a3 = b0 ? v[7] : v[6];
+    const T c0 = b1 ? a1 : a0, c1 = b1 ? a3 : a2;
+    return b2 ? c1 : c0;
+}
+
+// The wide node the walk is in: an interior node's index, or a leaf's range.
+struct WideCur {
+    uint32_t word, count;
+};
+
+// Tests the slots of interior node `node` and picks the next node as the
+// reference does: one hit slot is entered; of two, the nearer is entered and
+// the other stacked; of three or four, all are stacked in slot order, the top
+// ordered by TraversalStack::sort_top3/sort_top4 (CoreRef.cpp:192-237, their
+// handling of equal distances included) and the nearest popped; of five or
+// more, the same with the stable insertion sort of sort_topN, done here by
+// writing each entry straight to its sorted place.  false: no slot hit.
+template <bool Finite>
+__device__ __forceinline__ bool wide_step(const float4 *wn, uint32_t node, f3 o, f3 inv_d, float t, Stack &st,
+                                          WideCur &next) {
+    const float4 *p = wn + (size_t)node * 16;
+    float dist[8];
+    uint32_t word[8], count[8];
+    uint32_t mask = 0;
+    static_for<8>([&](auto S) {
+        constexpr int s = S;
+        const float4 a = p[2 * s], b = p[2 * s + 1];
+        word[s] = __float_as_uint(a.w);
+        count[s] = __float_as_uint(b.w);
+        const bool h = slot_hit<Finite>(o, inv_d, t, a, b, dist[s]);
+        if (h && word[s] != RH_MBVH_EMPTY) mask |= 1u << s;
+    });
+    if (!mask) return false;
+    const uint32_t k = __builtin_popcount(mask);
+    uint32_t top; // slot entered next
+    if (k == 1) {
+        top = __builtin_ctz(mask);
+    } else if (k <= 4) {
+        // slots in push (slot) order
+        uint32_t e[4];
+        float d[4];
+        uint32_t m = mask;
+        static_for<4>([&](auto C) {
+            constexpr int c = C;
+            e[c] = __builtin_ctz(m | 0x100u); // 8 once the hit slots are used up
+            m &= m - 1;
+            d[c] = pick8(dist, e[c]);
+        });
+        auto swap = [&](int x, int y) {
+            const uint32_t te = e[x];
+            e[x] = e[y];
+            e[y] = te;
+            const float td = d[x];
+            d[x] = d[y];
+            d[y] = td;
+        };
+        if (k == 2) {
+            // nearer entered, the other stacked (dist[i] < dist[i2] picks i)
+            if (!(d[0] < d[1])) swap(0, 1);
+            swap(0, 1); // e[1] = entered, e[0] = stacked
+        } else if (k == 3) {
+            if (d[0] > d[1]) {
+                if (!(d[1] > d[2])) {
+                    if (d[0] > d[2]) {
+                        swap(1, 2);
+                    } else { // [2, 0, 1]
+                        swap(0, 2);
+                        swap(1, 2);
+                    }
+                }
+            } else {
+                if (d[0] > d[2]) {
+                    swap(0, 1);
+                } else if (d[2] > d[1]) {
+                    swap(0, 2);
+                } else { // [1, 2, 0]
+                    swap(0, 1);
+                    swap(1, 2);
+                }
+            }
+        } else {
+            if (d[0] < d[1]) swap(0, 1);
+            if (d[2] < d[3]) swap(2, 3);
+            if (d[0] < d[2]) swap(0, 2);
+            if (d[1] < d[3]) swap(1, 3);
+            if (d[1] < d[2]) swap(1, 2);
+        }
+        // stack e[0..k-2] bottom to top, enter e[k-1]
+        static_for<3>([&](auto C) {
+            constexpr uint32_t c = C;
+            if (c + 1 < k) {
+                const uint32_t w = pick8(word, e[c]);
+                st.push2((w & RH_LEAF_NODE_BIT) ? (kWideSlotRef | (node * 8 + e[c])) : w, d[c]);
+            }
+        });
+        top = k == 2 ? e[1] : (k == 3 ? e[2] : e[3]);
+    } else {
+        // stable order by decreasing distance (ties keep slot order): slot s
+        // goes to sp + #{hit t: d_t > d_s} + #{hit t < s: d_t == d_s}
+        top = 0;
+        static_for<8>([&](auto S) {
+            constexpr uint32_t s = S;
+            if (!((mask >> s) & 1u)) return;
+            uint32_t pos = 0;
+            static_for<8>([&](auto U) {
+                constexpr uint32_t u = U;
+                if (u != s && ((mask >> u) & 1u))
+                    pos += (dist[u] > dist[s] || (dist[u] == dist[s] && u < s)) ? 1u : 0u;
+            });
+            if (pos == k - 1) {
+                top = s;
+            } else {
+                st.put(st.sp + pos, (word[s] & RH_LEAF_NODE_BIT) ? (kWideSlotRef | (node * 8 + s)) : word[s], dist[s]);
+            }
+        });
+        if (st.sp + k - 1 <= st.lds_depth + st.spill_depth) st.sp += k - 1;
+        else st.sp = st.lds_depth + st.spill_depth;
+    }
+    next.word = pick8(word, top);
+    next.count = pick8(count, top);
+    return true;
+}
+
+// Pops until an entry nearer than t: false when the stack is back at base.
+__device__ __forceinline__ bool wide_pop(const float4 *wn, uint32_t base, float t, Stack &st, WideCur &next) {
+    while (st.sp > base) {
+        uint32_t w;
+        float d;
+        st.pop2(w, d);
+        if (d > t) continue;
+        if (w & kWideSlotRef) {
+            const float *slot = reinterpret_cast<const float *>(wn + (size_t)(w & RH_PRIM_INDEX_BITS) * 2);
+            next.word = __float_as_uint(slot[3]);
+            next.count = __float_as_uint(slot[7]);
+        } else {
+            next.word = w;
+            next.count = 0;
+        }
+        return true;
+    }
+    return false;
+}
+
+// a tree's root: a leaf root describes itself in its slot 0
+__device__ __forceinline__ WideCur wide_root(const float4 *wn, uint32_t root) {
+    const float4 a = wn[(size_t)root * 16], b = wn[(size_t)root * 16 + 1];
+    const uint32_t w = __float_as_uint(a.w);
+    return (w & RH_LEAF_NODE_BIT) ? WideCur{w, __float_as_uint(b.w)} : WideCur{root, 0};
+}
+
+// Traverse_MicroTree_WithStack_ClosestHit, 8-wide (CoreRef.cpp:1395-1468).
+// The reference's wide any-hit walk takes the closest hit inside each
+// instance too (CoreRef.cpp:1357), so this is the only micro walk.
+template <bool Finite>
+__device__ __forceinline__ bool walk_micro_wide(const Scene &sc, uint32_t root, f3 mo, f3 md, f3 minv, int mi_index,
+                                                Hit &hit, Stack &st, TraceCounters *cnt) {
+    const uint32_t base = st.sp;
+    WideCur cur = wide_root(sc.wnodes, root);
+    bool res = false;
+    while (true) {
+        if (cur.word & RH_LEAF_NODE_BIT) {
+            NodeBox leaf;
+            leaf.a = cur.word;
+            leaf.b = cur.count;
+            const int before = hit.prim;
+            const float t_before = hit.t;
+            intersect_leaf<false>(sc, leaf, mo, md, mi_index, hit, cnt);
+            res |= hit.t != t_before || hit.prim != before;
+        } else {
+            if (cnt) cnt->nodes++;
+            if (wide_step<Finite>(sc.wnodes, cur.word, mo, minv, hit.t, st, cur)) continue;
+        }
+        if (!wide_pop(sc.wnodes, base, hit.t, st, cur)) return res;
+    }
+}
+
+// Traverse_MacroTree_WithStack_{ClosestHit,AnyHit}, 8-wide (CoreRef.cpp:
+// 1133-1222, 1270-1368): an any-hit walk ends at the first instance whose
+// closest hit is on a solid triangle.
+template <bool AnyHit>
+__device__ __forceinline__ void traverse_wide(const Scene &sc, f3 o, f3 d, Hit &hit, Stack &st, TraceCounters *cnt) {
+    const f3 inv_d = safe_invert(d);
+    st.sp = 0;
+    WideCur cur = wide_root(sc.wnodes, sc.macro_root);
+    while (true) {
+        if (cur.word & RH_LEAF_NODE_BIT) {
+            const uint32_t first = cur.word & RH_PRIM_INDEX_BITS, end = first + cur.count;
+            for (uint32_t i = first; i < end; i++) {
+                const uint32_t mi_index = sc.mi_indices[i];
+                const rh_mesh_instance mi = sc.mesh_instances[mi_index];
+                if (cnt) cnt->instances++;
+                if (!box_hit<true>(o, inv_d, hit.t, mi.bbox_min, mi.bbox_max)) continue;
+                const float *x = sc.transforms[mi.tr_index].inv_xform;
+                // TransformRay (CoreRef.cpp:1605-1617)
+                const f3 mo = f3{x[0] * o.x + x[4] * o.y + x[8] * o.z + x[12],
+                                 x[1] * o.x + x[5] * o.y + x[9] * o.z + x[13],
+                                 x[2] * o.x + x[6] * o.y + x[10] * o.z + x[14]};
+                const f3 md = f3{x[0] * d.x + x[4] * d.y + x[8] * d.z, x[1] * d.x + x[5] * d.y + x[9] * d.z,
+                                 x[2] * d.x + x[6] * d.y + x[10] * d.z};
+                const f3 minv = f3{1.0f / md.x, 1.0f / md.y, 1.0f / md.z};
+                const uint32_t root = sc.meshes[mi.mesh_index].node_index;
+                const bool finite =
+                    __builtin_isfinite(minv.x) && __builtin_isfinite(minv.y) && __builtin_isfinite(minv.z);
+                const bool found = finite ? walk_micro_wide<true>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt)
+                                          : walk_micro_wide<false>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt);
+                if (AnyHit && found) {
+                    const rh_tri_accel ht = load_tri(sc.tris, (uint32_t)hit.prim);
+                    if (ht.ci & RH_TRI_SOLID_BIT) {
+                        st.sp = 0;
+                        return;
+                    }
+                }
+            }
+        } else {
+            if (cnt) cnt->nodes++;
+            if (wide_step<true>(sc.wnodes, cur.word, o, inv_d, hit.t, st, cur)) continue;
+        }
+        if (!wide_pop(sc.wnodes, 0, hit.t, st, cur)) return;
+    }
+}
+
+// The scene's trees: Tree = kTreeBinary or kTreeWide compiles one walk (the
+// kernel is launched only for scenes of that kind), kTreeAny both, chosen per
+// scene at run time (at the register cost of the larger).
+constexpr int kTreeBinary = 0, kTreeWide = 1, kTreeAny = 2;
+template <bool AnyHit, int Tree = kTreeAny>
+__device__ __forceinline__ void traverse(const Scene &sc, f3 o, f3 d, Hit &hit, Stack &st, TraceCounters *cnt) {
+    if (Tree == kTreeWide || (Tree == kTreeAny && sc.wnodes)) traverse_wide<AnyHit>(sc, o, d, hit, st, cnt);
+    else traverse_fast<AnyHit>(sc, o, d, hit, st, cnt);
 }
 
 // ---------------------------------------------------------------- textures

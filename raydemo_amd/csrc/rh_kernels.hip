@@ -36,9 +36,10 @@ __global__ void __launch_bounds__(kBlock) k_gen_primary(PassConst pc, const floa
     store_ray(rays, aux, (uint32_t)i, gen_primary_ray(pc, halton, x, y));
 }
 
-template <bool AnyHit, bool Count> __global__ void __launch_bounds__(kBlock) k_trace(TraceArgs a) {
+// Tree: the scene's tree kind (kTreeBinary / kTreeWide), one walk compiled per kernel
+template <bool AnyHit, bool Count, int Tree> __global__ void __launch_bounds__(kBlock) k_trace(TraceArgs a) {
     extern __shared__ uint32_t lds_stack[];
-    trace_body<AnyHit, Count>(a, blockIdx.x * blockDim.x, lds_stack);
+    trace_body<AnyHit, Count, Tree>(a, blockIdx.x * blockDim.x, lds_stack);
 }
 
 // persistent closest-hit trace with lane refill (trace_refill_body)
@@ -58,10 +59,10 @@ template <bool Defer> __global__ void __launch_bounds__(kBlock) k_shade(ShadeArg
 // MinBlocks 3 asks the compiler for 3 resident blocks (waves per SIMD) per CU,
 // trading register spills for occupancy; 1 lets it use all the registers it
 // wants (2 waves per SIMD).  Which is faster is measured (tail_occupancy).
-template <bool Defer, int MinBlocks>
+template <bool Defer, int MinBlocks, int Tree>
 __global__ void __launch_bounds__(kBlock, MinBlocks) k_path_tail(ShadeArgs a, int first, int last) {
     extern __shared__ uint32_t lds_stack[];
-    path_tail_body<Defer>(a, first, last, blockIdx.x * blockDim.x, lds_stack);
+    path_tail_body<Defer, Tree>(a, first, last, blockIdx.x * blockDim.x, lds_stack);
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -102,7 +103,8 @@ struct AdjointArgs {
 // The suffix sums are formed backwards over the records; the gradient adds
 // adj[pixel] * scale * S_m / main_color with float atomics (their order is the
 // only run-to-run difference).
-template <bool Defer, int MinBlocks> __global__ void __launch_bounds__(kBlock, MinBlocks) k_adjoint(AdjointArgs a) {
+template <bool Defer, int MinBlocks, int Tree>
+__global__ void __launch_bounds__(kBlock, MinBlocks) k_adjoint(AdjointArgs a) {
     extern __shared__ uint32_t lds_stack[];
     const int rw = a.pc.rect[2], rh_ = a.pc.rect[3];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -111,15 +113,7 @@ template <bool Defer, int MinBlocks> __global__ void __launch_bounds__(kBlock, M
     cnt.shadow_rays = 0;
     uint32_t traced = 0;
     Stack st;
-    st.lds = lds_stack;
-    st.spill = a.spill + i;
-    st.spill_stride = a.spill_stride;
-    st.lds_depth = a.lds_depth;
-    st.spill_depth = kSpillDepth;
-    st.lane = threadIdx.x;
-    st.block = blockDim.x;
-    st.sp = 0;
-    st.overflow = 0;
+    st.init(lds_stack, a.spill + i, a.spill_stride, a.lds_depth, kSpillDepth);
     if (active) {
         const int x = a.pc.rect[0] + (int)i % rw, y = a.pc.rect[1] + (int)i / rw;
         const int p = y * a.pc.w + x;
@@ -141,17 +135,17 @@ template <bool Defer, int MinBlocks> __global__ void __launch_bounds__(kBlock, M
                 hit.t = RH_MAX_DIST;
                 hit.u = hit.v = 0.0f;
                 st.sp = 0;
-                if (a.sc.macro_root != RH_INVALID) traverse_fast<false>(a.sc, ray.o, ray.d, hit, st, nullptr);
+                if (a.sc.macro_root != RH_INVALID) traverse<false, Tree>(a.sc, ray.o, ray.d, hit, st, nullptr);
                 traced++;
                 pi.bounce = b < 0 ? 2 : b + 3;
                 ShadeOut so;
                 so.emit = false;
                 NeeSink sink;
                 if (Defer) sink = NeeSink{a.nee_ray, a.nee_aux, a.nee_term, i * a.nee_slots, 0};
-                shade_surface(a.sc, pi, hit, ray, a.halton, st, cnt, so, Defer ? &sink : nullptr);
+                shade_surface<Tree>(a.sc, pi, hit, ray, a.halton, st, cnt, so, Defer ? &sink : nullptr);
                 f3 cj = f3{so.col.x, so.col.y, so.col.z};
                 if (Defer && so.nee_pending && sink.count > 0)
-                    cj = cj + resolve_light_samples(a.sc, a.halton, sink, so, ray.c, st, cnt);
+                    cj = cj + resolve_light_samples<Tree>(a.sc, a.halton, sink, so, ray.c, st, cnt);
                 a.rec[(size_t)nv * a.lanes + i] = make_float4(cj.x, cj.y, cj.z, __int_as_float(so.alb_mat));
                 if (a.tex_grad && so.alb_mat >= 0) {
                     float4 *rt = a.rec_tex + (size_t)nv * 3 * a.lanes + i;
@@ -227,9 +221,9 @@ __global__ void k_set_material_colors(rh_material *mats, const float *__restrict
     m.main_color[2] = rgb[3 * k + 2];
 }
 
-__global__ void __launch_bounds__(kBlock) k_shadow(ShadowArgs a) {
+template <int Tree> __global__ void __launch_bounds__(kBlock) k_shadow(ShadowArgs a) {
     extern __shared__ uint32_t lds_stack[];
-    shadow_body(a, blockIdx.x * blockDim.x, lds_stack);
+    shadow_body<Tree>(a, blockIdx.x * blockDim.x, lds_stack);
 }
 
 __global__ void __launch_bounds__(kBlock) k_nee_combine(CombineArgs a) { combine_body(a, blockIdx.x * blockDim.x); }
@@ -435,6 +429,7 @@ struct rh_context {
     uint32_t *hist = nullptr;
     uint32_t *spill = nullptr;
     size_t spill_cap = 0; // lanes with a spill slice (stride of the spill array)
+    size_t spill_words = 0; // words per lane the spill array holds
     // deferred next-event estimation: slots per shaded ray = light count (0: inline)
     uint32_t nee_slots = 0;
     size_t nee_cap = 0;
@@ -539,14 +534,18 @@ int ensure_rays(rh_context *c, size_t n) {
 }
 
 // Spill slices are indexed by lane (ray or shadow slot): the array holds
-// kSpillDepth entries for each of spill_cap lanes, stride spill_cap.
+// kSpillDepth entries for each of spill_cap lanes, stride spill_cap, and for
+// a scene with 8-wide trees as many entry distances after them.
+size_t spill_words(const rh_context *c) { return (size_t)kSpillDepth * (c->sc.wnodes ? 2 : 1); }
+
 int ensure_spill(rh_context *c, size_t lanes) {
     lanes = (lanes + kBlock - 1) / kBlock * kBlock; // whole blocks: persistent lanes index slices too
-    if (lanes <= c->spill_cap && c->spill) return 0;
+    if (lanes <= c->spill_cap && c->spill && c->spill_words >= spill_words(c)) return 0;
     free_ptr(c->spill);
     c->spill = nullptr;
     c->spill_cap = std::max(lanes, c->spill_cap);
-    RH_CHECK(hipMalloc(&c->spill, (size_t)kSpillDepth * c->spill_cap * sizeof(uint32_t)));
+    c->spill_words = spill_words(c);
+    RH_CHECK(hipMalloc(&c->spill, c->spill_words * c->spill_cap * sizeof(uint32_t)));
     return 0;
 }
 
@@ -578,7 +577,10 @@ int ensure_nee(rh_context *c) {
 }
 
 
-size_t lds_bytes(const rh_context *c) { return (size_t)c->lds_depth * kBlock * sizeof(uint32_t); }
+// the LDS part of the traversal stacks: entries, and their distances for 8-wide trees
+size_t lds_bytes(const rh_context *c) {
+    return (size_t)c->lds_depth * kBlock * sizeof(uint32_t) * (c->sc.wnodes ? 2 : 1);
+}
 
 TraceArgs trace_args(rh_context *c, const rh_ray *rays, const int *count, int n, rh_hit *hits, const float *tmax,
                      unsigned long long *ray_counter) {
@@ -631,7 +633,7 @@ struct KernelMark {
 int launch_trace(rh_context *c, const TraceArgs &a, int grid_rays, bool any_hit) {
     const int blocks = std::max(1, (grid_rays + kBlock - 1) / kBlock);
     const bool count = c->count_visits && a.visits;
-    if (!any_hit && c->refill) {
+    if (!any_hit && c->refill && !c->sc.wnodes) {
         // persistent grid: as many blocks as fit on the device at once (and no
         // more than the rays need); spill slices exist for every lane of them
         if (c->refill_blocks_lds != c->lds_depth) {
@@ -653,12 +655,24 @@ int launch_trace(rh_context *c, const TraceArgs &a, int grid_rays, bool any_hit)
         return 0;
     }
     KernelMark km(c, 0);
-    if (any_hit) {
-        if (count) hipLaunchKernelGGL((k_trace<true, true>), dim3(blocks), dim3(kBlock), lds_bytes(c), c->stream, a);
-        else hipLaunchKernelGGL((k_trace<true, false>), dim3(blocks), dim3(kBlock), lds_bytes(c), c->stream, a);
+    const dim3 grid(blocks), blk(kBlock);
+    const size_t lds = lds_bytes(c);
+    if (c->sc.wnodes) {
+        if (any_hit) {
+            if (count) hipLaunchKernelGGL((k_trace<true, true, kTreeWide>), grid, blk, lds, c->stream, a);
+            else hipLaunchKernelGGL((k_trace<true, false, kTreeWide>), grid, blk, lds, c->stream, a);
+        } else {
+            if (count) hipLaunchKernelGGL((k_trace<false, true, kTreeWide>), grid, blk, lds, c->stream, a);
+            else hipLaunchKernelGGL((k_trace<false, false, kTreeWide>), grid, blk, lds, c->stream, a);
+        }
     } else {
-        if (count) hipLaunchKernelGGL((k_trace<false, true>), dim3(blocks), dim3(kBlock), lds_bytes(c), c->stream, a);
-        else hipLaunchKernelGGL((k_trace<false, false>), dim3(blocks), dim3(kBlock), lds_bytes(c), c->stream, a);
+        if (any_hit) {
+            if (count) hipLaunchKernelGGL((k_trace<true, true, kTreeBinary>), grid, blk, lds, c->stream, a);
+            else hipLaunchKernelGGL((k_trace<true, false, kTreeBinary>), grid, blk, lds, c->stream, a);
+        } else {
+            if (count) hipLaunchKernelGGL((k_trace<false, true, kTreeBinary>), grid, blk, lds, c->stream, a);
+            else hipLaunchKernelGGL((k_trace<false, false, kTreeBinary>), grid, blk, lds, c->stream, a);
+        }
     }
     RH_CHECK(hipGetLastError());
     return 0;
@@ -718,8 +732,9 @@ int launch_shade(rh_context *c, const ShadeArgs &a, int grid_rays) {
     const size_t lanes = (size_t)grid_rays * a.nee_slots;
     {
         KernelMark km(c, 3);
-        hipLaunchKernelGGL(k_shadow, dim3((unsigned)std::max<size_t>(1, (lanes + kBlock - 1) / kBlock)), dim3(kBlock),
-                           lds_bytes(c), c->stream, s);
+        const dim3 grid((unsigned)std::max<size_t>(1, (lanes + kBlock - 1) / kBlock)), blk(kBlock);
+        if (c->sc.wnodes) hipLaunchKernelGGL(k_shadow<kTreeWide>, grid, blk, lds_bytes(c), c->stream, s);
+        else hipLaunchKernelGGL(k_shadow<kTreeBinary>, grid, blk, lds_bytes(c), c->stream, s);
         RH_CHECK(hipGetLastError());
     }
     CombineArgs cb;
@@ -895,6 +910,11 @@ int rh_upload_scene(rh_context *c, const rh_scene_view *v) {
     c->n_materials = (uint32_t)v->materials_count;
     c->n_texels = (uint32_t)v->texels_count;
     c->have_scene = true;
+    if (c->max_rays) {
+        // spill slices sized for this scene's trees
+        const int rc2 = ensure_nee(c);
+        if (rc2) return rc2;
+    }
     RH_CHECK(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -973,16 +993,26 @@ int rh_render_pass(rh_context *c, const rh_pass_desc *p) {
             // after shading (the slots of ray i are free during the tail)
             const dim3 grid((n + kBlock - 1) / kBlock), blk(kBlock);
             const int last = pi.settings.max_total_depth;
-            if (tb.nee_ray) {
-                if (c->tail_occupancy == 3)
-                    hipLaunchKernelGGL((k_path_tail<true, 3>), grid, blk, lds_bytes(c), c->stream, tb, bounce, last);
+            const size_t lds = lds_bytes(c);
+            const bool occ3 = c->tail_occupancy == 3;
+            if (c->sc.wnodes) {
+                if (tb.nee_ray && occ3)
+                    hipLaunchKernelGGL((k_path_tail<true, 3, kTreeWide>), grid, blk, lds, c->stream, tb, bounce, last);
+                else if (tb.nee_ray)
+                    hipLaunchKernelGGL((k_path_tail<true, 1, kTreeWide>), grid, blk, lds, c->stream, tb, bounce, last);
+                else if (occ3)
+                    hipLaunchKernelGGL((k_path_tail<false, 3, kTreeWide>), grid, blk, lds, c->stream, tb, bounce, last);
                 else
-                    hipLaunchKernelGGL((k_path_tail<true, 1>), grid, blk, lds_bytes(c), c->stream, tb, bounce, last);
+                    hipLaunchKernelGGL((k_path_tail<false, 1, kTreeWide>), grid, blk, lds, c->stream, tb, bounce, last);
             } else {
-                if (c->tail_occupancy == 3)
-                    hipLaunchKernelGGL((k_path_tail<false, 3>), grid, blk, lds_bytes(c), c->stream, tb, bounce, last);
+                if (tb.nee_ray && occ3)
+                    hipLaunchKernelGGL((k_path_tail<true, 3, kTreeBinary>), grid, blk, lds, c->stream, tb, bounce, last);
+                else if (tb.nee_ray)
+                    hipLaunchKernelGGL((k_path_tail<true, 1, kTreeBinary>), grid, blk, lds, c->stream, tb, bounce, last);
+                else if (occ3)
+                    hipLaunchKernelGGL((k_path_tail<false, 3, kTreeBinary>), grid, blk, lds, c->stream, tb, bounce, last);
                 else
-                    hipLaunchKernelGGL((k_path_tail<false, 1>), grid, blk, lds_bytes(c), c->stream, tb, bounce, last);
+                    hipLaunchKernelGGL((k_path_tail<false, 1, kTreeBinary>), grid, blk, lds, c->stream, tb, bounce, last);
             }
             RH_CHECK(hipGetLastError());
             break;
@@ -1263,10 +1293,13 @@ int rh_trace_device(rh_context *c, const rh_ray *d_rays, int n, int any_hit, rh_
     TraceArgs a = trace_args(c, d_rays, nullptr, n, d_hits, nullptr, c->stats_dev + 1);
     const int blocks = (n + kBlock - 1) / kBlock;
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    const bool wide = c->sc.wnodes != nullptr;
     if (any_hit) {
-        hipLaunchKernelGGL((k_trace<true, false>), dim3(blocks), dim3(kBlock), lds_bytes(c), s, a);
+        if (wide) hipLaunchKernelGGL((k_trace<true, false, kTreeWide>), dim3(blocks), dim3(kBlock), lds_bytes(c), s, a);
+        else hipLaunchKernelGGL((k_trace<true, false, kTreeBinary>), dim3(blocks), dim3(kBlock), lds_bytes(c), s, a);
     } else {
-        hipLaunchKernelGGL((k_trace<false, false>), dim3(blocks), dim3(kBlock), lds_bytes(c), s, a);
+        if (wide) hipLaunchKernelGGL((k_trace<false, false, kTreeWide>), dim3(blocks), dim3(kBlock), lds_bytes(c), s, a);
+        else hipLaunchKernelGGL((k_trace<false, false, kTreeBinary>), dim3(blocks), dim3(kBlock), lds_bytes(c), s, a);
     }
     RH_CHECK(hipGetLastError());
     return 0;
@@ -1427,12 +1460,18 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
     }
     KernelMark km(c, 4);
     const dim3 grid((unsigned)((lanes + kBlock - 1) / kBlock)), blk(kBlock);
-    if (defer) {
-        if (c->adjoint_occupancy == 3) hipLaunchKernelGGL((k_adjoint<true, 3>), grid, blk, lds_bytes(c), c->stream, a);
-        else hipLaunchKernelGGL((k_adjoint<true, 1>), grid, blk, lds_bytes(c), c->stream, a);
+    const size_t lds = lds_bytes(c);
+    const bool occ3 = c->adjoint_occupancy == 3;
+    if (c->sc.wnodes) {
+        if (defer && occ3) hipLaunchKernelGGL((k_adjoint<true, 3, kTreeWide>), grid, blk, lds, c->stream, a);
+        else if (defer) hipLaunchKernelGGL((k_adjoint<true, 1, kTreeWide>), grid, blk, lds, c->stream, a);
+        else if (occ3) hipLaunchKernelGGL((k_adjoint<false, 3, kTreeWide>), grid, blk, lds, c->stream, a);
+        else hipLaunchKernelGGL((k_adjoint<false, 1, kTreeWide>), grid, blk, lds, c->stream, a);
     } else {
-        if (c->adjoint_occupancy == 3) hipLaunchKernelGGL((k_adjoint<false, 3>), grid, blk, lds_bytes(c), c->stream, a);
-        else hipLaunchKernelGGL((k_adjoint<false, 1>), grid, blk, lds_bytes(c), c->stream, a);
+        if (defer && occ3) hipLaunchKernelGGL((k_adjoint<true, 3, kTreeBinary>), grid, blk, lds, c->stream, a);
+        else if (defer) hipLaunchKernelGGL((k_adjoint<true, 1, kTreeBinary>), grid, blk, lds, c->stream, a);
+        else if (occ3) hipLaunchKernelGGL((k_adjoint<false, 3, kTreeBinary>), grid, blk, lds, c->stream, a);
+        else hipLaunchKernelGGL((k_adjoint<false, 1, kTreeBinary>), grid, blk, lds, c->stream, a);
     }
     RH_CHECK(hipGetLastError());
     return 0;

@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -62,6 +63,87 @@ int upload_array(std::vector<rh_rt::DevBuf> &bufs, hipStream_t stream, const T *
     return 0;
 }
 
+// The device form of the 8-wide trees (rh_device.h, "8-wide trees"): each
+// node as 8 slots of two float4, a leaf child's prim range folded into its
+// parent's slot; and the wide light tree's leaves in the order
+// ComputeDirectLighting walks them (CoreRef.cpp:1949-1970: slots pushed in
+// slot order, the last popped first).  A light leaf is reached only if the
+// point is strictly inside its slot box and its ancestors' (which contain
+// it), so testing the leaf boxes alone, in this order, takes the same lights
+// in the same order; a leaf root has no box test at all.
+inline int build_wide(const rh_scene_view *v, hipStream_t stream, std::vector<rh_rt::DevBuf> &bufs, Scene &s,
+                      std::vector<rh_bvh_node> &leaves) {
+    const uint32_t n = v->mnodes_count;
+    if ((uint64_t)n * 8 >= RH_LEAF_NODE_BIT) return rh_rt::arg_error("too many wide nodes (8 x count must fit 31 bits)");
+    std::vector<float4> w((size_t)n * 16);
+    const float nan = std::numeric_limits<float>::quiet_NaN();
+    auto as_float = [](uint32_t u) {
+        float f;
+        memcpy(&f, &u, 4);
+        return f;
+    };
+    for (uint32_t i = 0; i < n; i++) {
+        const rh_mbvh_node &m = v->mnodes[i];
+        const bool leaf = (m.child[0] & RH_LEAF_NODE_BIT) != 0;
+        for (int k = 0; k < 8; k++) {
+            uint32_t word = RH_MBVH_EMPTY, count = 0;
+            if (leaf) {
+                if (k == 0) {
+                    word = m.child[0];
+                    count = m.child[1];
+                }
+            } else if (m.child[k] != RH_MBVH_EMPTY) {
+                if (m.child[k] >= n) return rh_rt::arg_error("wide node child outside the node array");
+                const rh_mbvh_node &c = v->mnodes[m.child[k]];
+                if (c.child[0] & RH_LEAF_NODE_BIT) {
+                    word = c.child[0];
+                    count = c.child[1];
+                } else {
+                    word = m.child[k];
+                }
+            }
+            const bool used = word != RH_MBVH_EMPTY;
+            w[(size_t)i * 16 + 2 * k] = make_float4(used ? m.bbox_min[0][k] : nan, used ? m.bbox_min[1][k] : nan,
+                                                    used ? m.bbox_min[2][k] : nan, as_float(word));
+            w[(size_t)i * 16 + 2 * k + 1] = make_float4(used ? m.bbox_max[0][k] : nan, used ? m.bbox_max[1][k] : nan,
+                                                        used ? m.bbox_max[2][k] : nan, as_float(count));
+        }
+    }
+    for (uint32_t k = 0; k < v->meshes_count; k++)
+        if (v->meshes[k].node_index >= n) return rh_rt::arg_error("mesh tree root outside the wide node array");
+    int rc = upload_array(bufs, stream, w.data(), w.size(), &s.wnodes);
+    if (rc) return rc;
+    if (s.light_root == RH_INVALID) return 0;
+    if (s.light_root >= n) return rh_rt::arg_error("light tree root outside the node array");
+    const rh_mbvh_node &root = v->mnodes[s.light_root];
+    if (root.child[0] & RH_LEAF_NODE_BIT) {
+        s.light_untested = 1;
+        leaves.push_back(rh_bvh_node{{-RH_MAX_DIST, -RH_MAX_DIST, -RH_MAX_DIST}, root.child[0],
+                                     {RH_MAX_DIST, RH_MAX_DIST, RH_MAX_DIST}, root.child[1]});
+        return 0;
+    }
+    std::vector<std::pair<uint32_t, int>> st; // (node, slot) still to visit
+    for (int k = 0; k < 8; k++)
+        if (root.child[k] != RH_MBVH_EMPTY) st.push_back({s.light_root, k});
+    while (!st.empty()) {
+        const auto e = st.back();
+        st.pop_back();
+        const rh_mbvh_node &p = v->mnodes[e.first];
+        const uint32_t ci = p.child[e.second];
+        if (ci >= n) return rh_rt::arg_error("light tree node outside the node array");
+        const rh_mbvh_node &c = v->mnodes[ci];
+        if (c.child[0] & RH_LEAF_NODE_BIT) {
+            const int k = e.second;
+            leaves.push_back(rh_bvh_node{{p.bbox_min[0][k], p.bbox_min[1][k], p.bbox_min[2][k]}, c.child[0],
+                                         {p.bbox_max[0][k], p.bbox_max[1][k], p.bbox_max[2][k]}, c.child[1]});
+        } else {
+            for (int k = 0; k < 8; k++)
+                if (c.child[k] != RH_MBVH_EMPTY) st.push_back({ci, k});
+        }
+    }
+    return 0;
+}
+
 // Device copies of the scene arrays of v (appended to bufs) described by out.
 // The light tree's leaves are listed in the reference's depth-first order
 // (right child first) for the device's stackless light walk.
@@ -89,7 +171,10 @@ inline int build_device_scene(const rh_scene_view *v, hipStream_t stream, std::v
     s.macro_root = v->macro_nodes_root;
     s.light_root = v->light_nodes_root;
     std::vector<rh_bvh_node> leaves;
-    if (s.light_root != RH_INVALID && v->nodes && s.light_root < v->nodes_count) {
+    if (v->mnodes) {
+        rc = build_wide(v, stream, bufs, s, leaves);
+        if (rc) return rc;
+    } else if (s.light_root != RH_INVALID && v->nodes && s.light_root < v->nodes_count) {
         std::vector<uint32_t> st{s.light_root};
         while (!st.empty()) {
             const uint32_t cur = st.back();
@@ -107,7 +192,8 @@ inline int build_device_scene(const rh_scene_view *v, hipStream_t stream, std::v
     rc = upload_array(bufs, stream, leaves.data(), leaves.size(), &s.light_leaves);
     if (rc) return rc;
     s.light_leaf_count = (uint32_t)leaves.size();
-    if (s.macro_root != RH_INVALID && (!v->nodes || s.macro_root >= v->nodes_count))
+    const uint32_t tree_nodes = v->mnodes ? v->mnodes_count : v->nodes_count;
+    if (s.macro_root != RH_INVALID && ((!v->nodes && !v->mnodes) || s.macro_root >= tree_nodes))
         return rh_rt::arg_error("macro tree root outside the node array");
     out = s;
     return 0;

@@ -21,6 +21,7 @@ struct ShadeCounters {
 
 // ComputeVisibility (CoreRef.cpp:1801-1896): steps through transparent
 // occluders, resolving mix materials with the lane's random offset
+template <int Tree = kTreeAny>
 __device__ __forceinline__ float compute_visibility(const Scene &sc, f3 p1, f3 p2, const float *halton, int hi,
                                                     int rand_hash2, Stack &st, ShadeCounters &cnt) {
     f3 dir = p2 - p1;
@@ -37,7 +38,7 @@ __device__ __forceinline__ float compute_visibility(const Scene &sc, f3 p1, f3 p
         sh.t = dist;
         sh.u = sh.v = 0.0f;
         cnt.shadow_rays++;
-        if (sc.macro_root != RH_INVALID) traverse_fast<true>(sc, ro, dir, sh, st, nullptr);
+        if (sc.macro_root != RH_INVALID) traverse<true, Tree>(sc, ro, dir, sh, st, nullptr);
         if (!sh.mask) break;
 
         const rh_tri_accel tri = load_tri(sc.tris, (uint32_t)sh.prim);
@@ -95,6 +96,7 @@ __device__ __forceinline__ float compute_visibility(const Scene &sc, f3 p1, f3 p
 }
 
 // AcumulateLightContribution (CoreRef.cpp:1898-1933): one sample on a sphere light
+template <int Tree = kTreeAny>
 __device__ __forceinline__ void accumulate_light(const Scene &sc, const rh_light &l, f3 I, f3 P, f3 N, f3 B,
                                                  f3 plane_N, float sigma, const float *halton, int hi, int rand_hash2,
                                                  float rand_offset, float rand_offset2, f3 &col, Stack &st,
@@ -126,12 +128,13 @@ __device__ __forceinline__ void accumulate_light(const Scene &sc, const rh_light
 
     if (_dot1 > RH_FLT_EPS && _dot2 > l.spot && (l.brightness * atten) > RH_FLT_EPS) {
         const float visibility =
-            compute_visibility(sc, P + RH_HIT_BIAS * plane_N, lpos, halton, hi, rand_hash2, st, cnt);
+            compute_visibility<Tree>(sc, P + RH_HIT_BIAS * plane_N, lpos, halton, hi, rand_hash2, st, cnt);
         col = col + ((((ld3(l.col) * _dot1) * visibility) * atten) * brdf_oren_nayar(L, I, N, B, sigma));
     }
 }
 
-// ComputeDirectLighting (CoreRef.cpp:1935-1991), binary light tree: point-in-box walk
+// ComputeDirectLighting (CoreRef.cpp:1935-1991): point-in-box walk of the light tree
+template <int Tree = kTreeAny>
 __device__ __forceinline__ f3 compute_direct_lighting(const Scene &sc, f3 I, f3 P, f3 N, f3 B, f3 plane_N,
                                                       float sigma, const float *halton, int hi, int rand_hash2,
                                                       float rand_offset, float rand_offset2, Stack &st,
@@ -146,13 +149,13 @@ __device__ __forceinline__ f3 compute_direct_lighting(const Scene &sc, f3 I, f3 
     // once at scene upload.
     for (uint32_t k = 0; k < sc.light_leaf_count; k++) {
         const NodeBox n = load_node(sc.light_leaves, k);
-        if (!(P.x > n.bmin[0] && P.x < n.bmax[0] && P.y > n.bmin[1] && P.y < n.bmax[1] && P.z > n.bmin[2] &&
-              P.z < n.bmax[2]))
+        if (!sc.light_untested && !(P.x > n.bmin[0] && P.x < n.bmax[0] && P.y > n.bmin[1] && P.y < n.bmax[1] &&
+                                    P.z > n.bmin[2] && P.z < n.bmax[2]))
             continue;
         const uint32_t first = n.a & RH_PRIM_INDEX_BITS;
         for (uint32_t i = first; i < first + n.b; i++) {
             const rh_light l = sc.lights[sc.li_indices[i]];
-            accumulate_light(sc, l, I, P, N, B, plane_N, sigma, halton, hi, rand_hash2, rand_offset, rand_offset2, col,
+            accumulate_light<Tree>(sc, l, I, P, N, B, plane_N, sigma, halton, hi, rand_hash2, rand_offset, rand_offset2, col,
                              st, cnt);
         }
     }
@@ -211,8 +214,8 @@ __device__ __forceinline__ void defer_direct_lighting(const Scene &sc, f3 I, f3 
                                                       float rand_offset2, NeeSink &sink) {
     for (uint32_t k = 0; k < sc.light_leaf_count; k++) {
         const NodeBox n = load_node(sc.light_leaves, k);
-        if (!(P.x > n.bmin[0] && P.x < n.bmax[0] && P.y > n.bmin[1] && P.y < n.bmax[1] && P.z > n.bmin[2] &&
-              P.z < n.bmax[2]))
+        if (!sc.light_untested && !(P.x > n.bmin[0] && P.x < n.bmax[0] && P.y > n.bmin[1] && P.y < n.bmax[1] &&
+                                    P.z > n.bmin[2] && P.z < n.bmax[2]))
             continue;
         const uint32_t first = n.a & RH_PRIM_INDEX_BITS;
         for (uint32_t i = first; i < first + n.b; i++) {
@@ -246,6 +249,7 @@ struct ShadeOut {
 // ShadeSurface (CoreRef.cpp:2055-2391) for one (ray, hit) pair.
 // With a sink, a diffuse hit's direct light is deferred (see NeeSink) and
 // out.nee_pending is set; without one, visibility is traced inline.
+template <int Tree = kTreeAny>
 __device__ __forceinline__ void shade_surface(const Scene &sc, const rh_pass_info &pi, const Hit &inter,
                                               const RayFull &ray, const float *halton, Stack &st, ShadeCounters &cnt,
                                               ShadeOut &out, NeeSink *sink = nullptr) {
@@ -390,7 +394,7 @@ __device__ __forceinline__ void shade_surface(const Scene &sc, const rh_pass_inf
                 out.consider_albedo = consider_albedo;
                 out.albedo = alb;
             } else {
-                col = compute_direct_lighting(sc, I, P, N, B, plane_N, mat->roughness, halton, hi, rand_hash2,
+                col = compute_direct_lighting<Tree>(sc, I, P, N, B, plane_N, mat->roughness, halton, hi, rand_hash2,
                                               rand_offset, rand_offset2, st, cnt);
                 if (consider_albedo) col = col * alb;
             }
@@ -530,6 +534,7 @@ __device__ __forceinline__ void shade_surface(const Scene &sc, const rh_pass_inf
 // terms summed in light order, times the albedo when considered, times the
 // ray's throughput -- the arithmetic of the separate shadow and combine
 // stages, so adding the result after out.col gives the same pixel value.
+template <int Tree = kTreeAny>
 __device__ __forceinline__ f3 resolve_light_samples(const Scene &sc, const float *halton, const NeeSink &sink,
                                                     const ShadeOut &out, f3 throughput, Stack &st,
                                                     ShadeCounters &cnt) {
@@ -540,7 +545,7 @@ __device__ __forceinline__ f3 resolve_light_samples(const Scene &sc, const float
         const int2 x = sink.aux[k];
         const rh_light &l = sc.lights[__float_as_uint(r.w)];
         st.sp = 0;
-        if (compute_visibility(sc, f3{r.x, r.y, r.z}, ld3(l.pos), halton, x.x, x.y, st, cnt) != 0.0f) {
+        if (compute_visibility<Tree>(sc, f3{r.x, r.y, r.z}, ld3(l.pos), halton, x.x, x.y, st, cnt) != 0.0f) {
             const float4 t = sink.term[k];
             col = col + f3{t.x, t.y, t.z};
         }

@@ -251,7 +251,7 @@ struct TraceArgs {
     unsigned long long *visits;      // Count mode: node visits, leaf triangle tests, instance tests, rays
 };
 
-template <bool AnyHit, bool Count>
+template <bool AnyHit, bool Count, int Tree = kTreeAny>
 __device__ __forceinline__ void trace_body(const TraceArgs &a, uint32_t base, uint32_t *lds_stack) {
     const int n = a.count ? *a.count : a.n;
     const uint32_t i = base + threadIdx.x;
@@ -269,18 +269,10 @@ __device__ __forceinline__ void trace_body(const TraceArgs &a, uint32_t base, ui
     hit.u = hit.v = 0.0f;
 
     Stack st;
-    st.lds = lds_stack;
-    st.spill = a.spill + i;
-    st.spill_stride = a.spill_stride;
-    st.lds_depth = a.lds_depth;
-    st.spill_depth = kSpillDepth;
-    st.lane = threadIdx.x;
-    st.block = blockDim.x;
-    st.sp = 0;
-    st.overflow = 0;
+    st.init(lds_stack, a.spill + i, a.spill_stride, a.lds_depth, kSpillDepth);
     TraceCounters tc = {0, 0, 0};
     if (a.sc.macro_root != RH_INVALID)
-        traverse_fast<AnyHit>(a.sc, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, hit, st, Count ? &tc : nullptr);
+        traverse<AnyHit, Tree>(a.sc, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, hit, st, Count ? &tc : nullptr);
     if (st.overflow && a.overflow) atomicAdd(a.overflow, (unsigned long long)st.overflow);
     if (Count) {
         atomicAdd(a.visits + 0, (unsigned long long)tc.nodes);
@@ -310,15 +302,7 @@ __device__ __forceinline__ void trace_refill_body(const TraceArgs &a, int *next_
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.ray_counter) atomicAdd(a.ray_counter, (unsigned long long)n);
     const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x; // persistent lane: owns spill slice gl
     Stack st;
-    st.lds = lds_stack;
-    st.spill = a.spill + gl;
-    st.spill_stride = a.spill_stride;
-    st.lds_depth = a.lds_depth;
-    st.spill_depth = kSpillDepth;
-    st.lane = threadIdx.x;
-    st.block = blockDim.x;
-    st.sp = 0;
-    st.overflow = 0;
+    st.init(lds_stack, a.spill + gl, a.spill_stride, a.lds_depth, kSpillDepth);
     TraceCounters tc = {0, 0, 0};
     TraceCounters *cnt = Count ? &tc : nullptr;
 
@@ -503,15 +487,7 @@ __device__ __forceinline__ void shade_body(const ShadeArgs &a, uint32_t base, ui
     ShadeCounters cnt;
     cnt.shadow_rays = 0;
     Stack st;
-    st.lds = lds_stack;
-    st.spill = a.spill + i;
-    st.spill_stride = a.spill_stride;
-    st.lds_depth = a.lds_depth;
-    st.spill_depth = kSpillDepth;
-    st.lane = threadIdx.x;
-    st.block = blockDim.x;
-    st.sp = 0;
-    st.overflow = 0;
+    st.init(lds_stack, a.spill + i, a.spill_stride, a.lds_depth, kSpillDepth);
     int px = 0, py = 0;
     int nee_m = 0; // light samples recorded for this ray (deferred NEE)
 
@@ -612,7 +588,7 @@ __device__ __forceinline__ void shade_body(const ShadeArgs &a, uint32_t base, ui
 // terms summed in light order, albedo, throughput: the combine stage's
 // arithmetic), so the shading state is dead while the shadow rays traverse
 // and fewer registers are live at the peak.
-template <bool Defer>
+template <bool Defer, int Tree = kTreeAny>
 __device__ __forceinline__ void path_tail_body(const ShadeArgs &a, int first, int last, uint32_t base,
                                                uint32_t *lds_stack) {
     const int n = *a.count;
@@ -623,15 +599,7 @@ __device__ __forceinline__ void path_tail_body(const ShadeArgs &a, int first, in
     cnt.shadow_rays = 0;
     uint32_t traced = 0;
     Stack st;
-    st.lds = lds_stack;
-    st.spill = a.spill + i;
-    st.spill_stride = a.spill_stride;
-    st.lds_depth = a.lds_depth;
-    st.spill_depth = kSpillDepth;
-    st.lane = threadIdx.x;
-    st.block = blockDim.x;
-    st.sp = 0;
-    st.overflow = 0;
+    st.init(lds_stack, a.spill + i, a.spill_stride, a.lds_depth, kSpillDepth);
     if (active) {
         RayFull ray = load_ray(a.rays, a.aux, i);
         const int px = (ray.xy >> 16) & 0xffff, py = ray.xy & 0xffff;
@@ -648,14 +616,14 @@ __device__ __forceinline__ void path_tail_body(const ShadeArgs &a, int first, in
             hit.t = RH_MAX_DIST;
             hit.u = hit.v = 0.0f;
             st.sp = 0;
-            if (a.sc.macro_root != RH_INVALID) traverse_fast<false>(a.sc, ray.o, ray.d, hit, st, nullptr);
+            if (a.sc.macro_root != RH_INVALID) traverse<false, Tree>(a.sc, ray.o, ray.d, hit, st, nullptr);
             traced++;
             pi.bounce = b + 3;
             ShadeOut so;
             so.emit = false;
             NeeSink sink;
             if (Defer) sink = NeeSink{a.nee_ray, a.nee_aux, a.nee_term, i * a.nee_slots, 0};
-            shade_surface(a.sc, pi, hit, ray, a.halton, st, cnt, so, Defer ? &sink : nullptr);
+            shade_surface<Tree>(a.sc, pi, hit, ray, a.halton, st, cnt, so, Defer ? &sink : nullptr);
             float4 v = dst;
             v.x += so.col.x;
             v.y += so.col.y;
@@ -663,7 +631,7 @@ __device__ __forceinline__ void path_tail_body(const ShadeArgs &a, int first, in
             v.w += 0.0f;
             dst = v;
             if (Defer && so.nee_pending && sink.count > 0) {
-                const f3 add = resolve_light_samples(a.sc, a.halton, sink, so, ray.c, st, cnt);
+                const f3 add = resolve_light_samples<Tree>(a.sc, a.halton, sink, so, ray.c, st, cnt);
                 float4 w = dst;
                 w.x += add.x;
                 w.y += add.y;
@@ -698,6 +666,7 @@ struct ShadowArgs {
 // (CoreRef.cpp:1801-1896), any-hit traversal stepping through transparent
 // surfaces.  One lane per queue entry; the grid covers every slot and lanes
 // past the device count exit at once.
+template <int Tree = kTreeAny>
 __device__ __forceinline__ void shadow_body(const ShadowArgs &a, uint32_t base, uint32_t *lds_stack) {
     const int n = *a.count;
     const uint32_t q = base + threadIdx.x;
@@ -711,16 +680,8 @@ __device__ __forceinline__ void shadow_body(const ShadowArgs &a, uint32_t base, 
         const int2 x = a.nee_aux[k];
         const rh_light &l = a.sc.lights[__float_as_uint(r.w)];
         Stack st;
-        st.lds = lds_stack;
-        st.spill = a.spill + q;
-        st.spill_stride = a.spill_stride;
-        st.lds_depth = a.lds_depth;
-        st.spill_depth = kSpillDepth;
-        st.lane = threadIdx.x;
-        st.block = blockDim.x;
-        st.sp = 0;
-        st.overflow = 0;
-        const float vis = compute_visibility(a.sc, f3{r.x, r.y, r.z}, ld3(l.pos), a.halton, x.x, x.y, st, cnt);
+        st.init(lds_stack, a.spill + q, a.spill_stride, a.lds_depth, kSpillDepth);
+        const float vis = compute_visibility<Tree>(a.sc, f3{r.x, r.y, r.z}, ld3(l.pos), a.halton, x.x, x.y, st, cnt);
         a.nee_vis[k] = vis != 0.0f ? 1 : 0;
         if (st.overflow && a.overflow) atomicAdd(a.overflow, (unsigned long long)st.overflow);
     }
