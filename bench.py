@@ -47,7 +47,9 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes (layouts of include/rh_types.h): per traced ray 32 (ray) + 32 (hit);
 # per node visit 32; per instance test 112 (instance 32 + inverse transform 64 + mesh 16);
 # per triangle test 52 (index 4 + triangle record 48)
+# per interior node visit of an 8-wide tree 224 (mbvh_node_t: 8 boxes + 8 children)
 B_RAY, B_HIT, B_NODE, B_INST, B_TRI = 32, 32, 32, 112, 52
+B_WNODE = 224
 
 
 def log(*a):
@@ -79,18 +81,23 @@ def timed_passes(r, s, A, lib, W, H, steps, warmup, opts):
     return dict(opts=dict(opts), seconds=dt, rays=rays, counts=rc, prof=prof, mrays=rays / dt / 1e6)
 
 
-def run_gpu(args, lib, A, scenes):
+def run_gpu(args, lib, A, scenes, wide, staged_modes=True, tuned=None):
+    """Autotune and time the workload on one tree kind (binary or the
+    reference's default 8-wide trees); then the instrumented replay."""
     from raydemo_amd import autotune
 
     W, H = args.width, args.height
-    r = A.Renderer(lib, W, H, A.RENDERER_HIP, use_wide_bvh=False)
+    r = A.Renderer(lib, W, H, A.RENDERER_HIP, use_wide_bvh=wide)
     s = r.create_scene()
     t0 = time.time()
     spec = scenes.triangle_soup(s, width=W, height=H, spp=args.steps, depth=args.depth)
-    log(f"scene: {spec.triangles} triangles, host build {time.time() - t0:.1f}s")
+    log(f"scene ({'8-wide' if wide else 'binary'} trees): {spec.triangles} triangles, "
+        f"host build {time.time() - t0:.1f}s")
 
     # launch knobs (LDS stack share, sort on/off and key bits, path tail): all give the same image
-    if args.knobs:
+    if tuned is not None:
+        pass  # knobs chosen by an earlier sweep of the same scene
+    elif args.knobs:
         fixed = dict(zip(autotune.KNOBS, (int(v) for v in args.knobs.split(","))))
         tuned = autotune.TuneResult(best=fixed)
         log(f"knobs given: {fixed}")
@@ -103,7 +110,7 @@ def run_gpu(args, lib, A, scenes):
     modes = {"tuned": dict(tuned.best),
              "sort_off": {"lds_stack_depth": depth, "sort_rays": 0, "sort_bits": 32, "path_tail": -1},
              "sort_on": {"lds_stack_depth": depth, "sort_rays": 1, "sort_bits": 32, "path_tail": -1}}
-    if args.only_tuned:
+    if args.only_tuned or not staged_modes:
         modes = {"tuned": modes["tuned"]}
     for name, opts in modes.items():
         res = timed_passes(r, s, A, lib, W, H, args.steps, args.warmup, opts)
@@ -128,6 +135,7 @@ def run_gpu(args, lib, A, scenes):
     rc2 = r.ray_counts()
     r.set_option("count_visits", 0)
     traced = vis["counted_rays"]  # rays of the counting k_trace launches (the path tail is not counted)
+    r.close()
     return spec, results, vis, traced, tuned
 
 
@@ -177,7 +185,7 @@ def run_cpu_baseline(args, A, scenes, spp_sample):
     return dict(kind=kind, seconds=dt, cores=cores, spp=spp_sample)
 
 
-def run_adjoint(args, lib, A, scenes, knobs):
+def run_adjoint(args, lib, A, scenes, knobs, wide):
     """Refined Cornell box (9,828 triangles, one material per triangle), 512x512,
     32 spp, depth 6: Adam on the 9,826 albedos against a target rendered from a
     hidden albedo set (seed 0x5EED0005).  One iteration = forward (32 passes) +
@@ -185,8 +193,9 @@ def run_adjoint(args, lib, A, scenes, knobs):
     import torch
     from raydemo_amd import adjoint
 
+    r = A.Renderer(lib, args.adj_size, args.adj_size, A.RENDERER_HIP, use_wide_bvh=wide)
     dr, init, spec = adjoint.refined_cornell(lib, width=args.adj_size, height=args.adj_size, spp=args.adj_spp,
-                                            depth=6)
+                                            depth=6, renderer=r)
     for k, v in knobs.items():
         dr.r.set_option(k, v)
     n = spec.info["diffuse"]
@@ -219,13 +228,14 @@ def run_adjoint(args, lib, A, scenes, knobs):
         "knobs": knobs,
         "config": {"workload": f"cornell_refined {spec.triangles} tris, {n} albedo parameters x 3, "
                                f"{args.adj_size}x{args.adj_size}, {args.adj_spp} spp, depth 6, roulette off",
+                   "trees": "8-wide" if wide else "binary",
                    "gradient": "d L2(image, target) / d main_color, all materials"},
     }
     log(f"adjoint: {out}")
     return out
 
 
-def run_batch(args, lib, A, scenes, knobs):
+def run_batch(args, lib, A, scenes, knobs, wide):
     """The 64-scene workload: rendered scene after scene by single-scene
     renderers (the plain wavefront path, each scene's passes back to back),
     then all together by the batch renderer.  Same passes, same rays (the
@@ -236,7 +246,7 @@ def run_batch(args, lib, A, scenes, knobs):
     t0 = time.time()
 
     def build(it):
-        s = A.standalone_scene(lib)
+        s = A.standalone_scene(lib, wide)
         scenes.batch_scene(s, it)
         s.triangle_count()
         return s
@@ -251,7 +261,7 @@ def run_batch(args, lib, A, scenes, knobs):
     # plain: one renderer per scene (scene uploaded by a warm-up pass), scenes rendered in turn
     rs = []
     for it, s in zip(items, ss):
-        r = A.Renderer(lib, it.width, it.height, A.RENDERER_HIP)
+        r = A.Renderer(lib, it.width, it.height, A.RENDERER_HIP, use_wide_bvh=wide)
         for k, v in knobs.items():
             r.set_option(k, v)
         r.render(s, A.Region(lib, 0, 0, it.width, it.height))
@@ -319,7 +329,7 @@ def run_batch(args, lib, A, scenes, knobs):
                      "table": table},
         "config": {"workload": f"{len(items)} icosphere soups, {sum(s.triangle_count() for s in ss)} triangles, "
                                f"{sum(i.width * i.height for i in items)} pixels, 4..32 spp, depth 4..8",
-                   "scenes": len(items)},
+                   "scenes": len(items), "trees": "8-wide" if wide else "binary"},
     }
     log(f"batch: {out}")
     b.close()
@@ -340,6 +350,9 @@ def main():
     ap.add_argument("--knobs", default="",
                     help="skip the autotuner: lds_stack_depth,sort_rays,sort_bits,path_tail[,tail_occupancy]")
     ap.add_argument("--only-tuned", action="store_true", help="time only the tuned (or given) knobs")
+    ap.add_argument("--trees", default="both", choices=("both", "binary", "wide"),
+                    help="BVH kind(s) to time: binary, the reference's default 8-wide, or both (the faster is the "
+                         "headline)")
     ap.add_argument("--adj-steps", type=int, default=200, help="Adam steps of the differentiation workload (0: skip)")
     ap.add_argument("--adj-size", type=int, default=512)
     ap.add_argument("--adj-spp", type=int, default=32)
@@ -356,26 +369,41 @@ def main():
     from raydemo_amd import autotune, capi as A, host_lib, scenes
 
     lib = host_lib()
-    spec, res, vis, traced, tuned = run_gpu(args, lib, A, scenes)
+    kinds = {"both": (False, True), "binary": (False,), "wide": (True,)}[args.trees]
+    runs = {}
+    for wide in kinds:
+        runs[wide] = run_gpu(args, lib, A, scenes, wide, staged_modes=False)
+    wide = max(runs, key=lambda k: runs[k][1]["tuned"]["mrays"])
+    per_tree = {("8-wide" if k else "binary"): round(v[1]["tuned"]["mrays"], 2) for k, v in runs.items()}
+    log(f"trees: {per_tree}; headline on the {'8-wide' if wide else 'binary'} trees")
+    if not args.only_tuned:
+        # the staged modes (every bounce its own launches; sort off / 32-bit sort) on the faster trees
+        first = runs[wide][4]
+        runs[wide] = run_gpu(args, lib, A, scenes, wide, staged_modes=True, tuned=first)
+    spec, res, vis, traced, tuned = runs[wide]
     best = res["tuned"]
     prof = best["prof"]
 
-    # roofline of the trace kernel (closest hit), tuned mode
+    # roofline of the trace kernel (closest hit), tuned mode: bytes of the
+    # reference structures the reference traversal reads, per visit
     trace_ms = prof["trace_ms"]
-    bytes_alg = (traced * (B_RAY + B_HIT) + vis["node_visits"] * B_NODE + vis["instance_tests"] * B_INST +
+    b_node = B_WNODE if wide else B_NODE
+    bytes_alg = (traced * (B_RAY + B_HIT) + vis["node_visits"] * b_node + vis["instance_tests"] * B_INST +
                  vis["tri_tests"] * B_TRI)
     achieved = bytes_alg / (trace_ms / 1e3) / 1e9 if trace_ms > 0 else 0.0
+    kernel = f"k_trace<false,false,{1 if wide else 0}>"
     traffic_file = ROOT / "profiles" / "trace_traffic.json"
     traffic = None
     if traffic_file.exists():
         try:
-            traffic = json.loads(traffic_file.read_text()).get("bytes_per_launch")
+            rec = json.loads(traffic_file.read_text()).get(kernel) or {}
+            traffic = rec.get("bytes_per_launch")
         except Exception:
             traffic = None
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-        "kernel": "k_trace<false,false>", "launches": prof["trace_launches"],
+        "kernel": kernel, "trees": "8-wide" if wide else "binary", "launches": prof["trace_launches"],
         "avg_launch_ms": round(trace_ms / max(prof["trace_launches"], 1), 4),
         "alg_bytes_per_launch": int(bytes_alg / max(prof["trace_launches"], 1)),
     }
@@ -385,7 +413,7 @@ def main():
         cpu_run = run_cpu_baseline(args, A, scenes, args.cpu_spp)
         if cpu_run is not None:
             # rays of the identical sample, counted on the device (same paths, same seeds)
-            r = A.Renderer(lib, args.width, args.height, A.RENDERER_HIP, use_wide_bvh=False)
+            r = A.Renderer(lib, args.width, args.height, A.RENDERER_HIP, use_wide_bvh=True)
             s = r.create_scene()
             scenes.triangle_soup(s, width=args.width, height=args.height, depth=args.depth)
             reg = A.Region(lib, 0, 0, args.width, args.height)
@@ -403,8 +431,8 @@ def main():
             log(f"cpu baseline: {cpu}")
 
     plain_knobs = {k: v for k, v in tuned.best.items() if k in autotune.KNOBS}
-    adj = run_adjoint(args, lib, A, scenes, plain_knobs) if args.adj_steps > 0 else None
-    batch = run_batch(args, lib, A, scenes, plain_knobs) if args.batch_scenes > 0 else None
+    adj = run_adjoint(args, lib, A, scenes, plain_knobs, wide) if args.adj_steps > 0 else None
+    batch = run_batch(args, lib, A, scenes, plain_knobs, wide) if args.batch_scenes > 0 else None
 
     line = {
         "metric": "Mrays/s at 1080p depth-8 (triangle soup, primary+secondary+shadow rays)",
@@ -422,7 +450,9 @@ def main():
         "config": {"workload": f"triangle_soup {spec.triangles} tris, {args.width}x{args.height}, "
                                f"{args.steps} spp, depth {args.depth}, autotuned launch knobs",
                    "triangles": spec.triangles, "width": args.width, "height": args.height, "spp": args.steps,
-                   "depth": args.depth, "knobs": tuned.best, "parallelism": "single GPU"},
+                   "depth": args.depth, "trees": "8-wide" if wide else "binary", "knobs": tuned.best,
+                   "parallelism": "single GPU"},
+        "trees": per_tree,
         "modes": {name: {"mrays": round(m["mrays"], 2), "opts": m["opts"],
                          "kernel_ms": {k: round(v, 3) for k, v in m["prof"].items() if k.endswith("_ms")}}
                   for name, m in res.items()},

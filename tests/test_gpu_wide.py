@@ -15,6 +15,9 @@ from test_gpu_trace import _check_hits, _instanced
 
 pytestmark = pytest.mark.gpu
 
+# torch joins the process before the backend initialises the device
+torch = pytest.importorskip("torch")
+
 BUILDS = {
     "cornell_box": lambda s, g: scenes.cornell_box(s),
     "soup_small": lambda s, g: scenes.triangle_soup(s, n_spheres=48, n_fine=4),
@@ -115,7 +118,6 @@ def test_batch_of_binary_and_wide_scenes(host_lib, golden_scene1):
 
 
 def test_wide_adjoint_matches_oracle(host_lib, oracle):
-    torch = pytest.importorskip("torch")
     from raydemo_amd import adjoint
     W, H, SPP, DEPTH, K = 32, 24, 3, 3, 2
     r = A.Renderer(host_lib, W, H, A.RENDERER_HIP, use_wide_bvh=True)
