@@ -3,9 +3,10 @@
 Every knob here changes only how the work is laid out on the device, never
 what is computed: the traversal stack's LDS share (the rest spills to a
 global slice), whether bounce rays are sorted by their origin/direction hash
-before tracing, how many top bits of the 32-bit key the sort orders by, and
-the bounce from which the remaining bounces run as one path-per-lane launch
-instead of one trace/shade/shadow stage each.
+before tracing, how many top bits of the 32-bit key the sort orders by, the
+bounce from which the remaining bounces run as one path-per-lane launch
+instead of one trace/shade/shadow stage each, the register budget (waves per
+SIMD) of the path-tail and 8-wide trace kernels, and the lane-refill trace.
 The image is bit-identical for every setting (tests/test_gpu_sort.py), so the
 tuner may pick whichever is fastest on the scene at hand.
 
@@ -30,7 +31,8 @@ SORT_MODES = ((0, 32), (1, 8), (1, 32))
 PATH_TAILS = (-1, 0, 1, 2, 3, 4)
 TAIL_OCCUPANCY = (2, 3)
 REFILL = (0, 1)
-KNOBS = ("lds_stack_depth", "sort_rays", "sort_bits", "path_tail", "tail_occupancy", "refill")
+TRACE_OCCUPANCY = (0, 5)
+KNOBS = ("lds_stack_depth", "sort_rays", "sort_bits", "path_tail", "tail_occupancy", "refill", "trace_occupancy")
 
 
 @dataclass
@@ -48,7 +50,7 @@ def candidates(depths=LDS_DEPTHS, sort_modes=SORT_MODES, tails=PATH_TAILS, occup
         if pt < 0 and occ != occupancies[0]:
             continue  # no tail launch: occupancy is moot
         yield {"lds_stack_depth": d, "sort_rays": so, "sort_bits": sb, "path_tail": pt, "tail_occupancy": occ,
-               "refill": 0}
+               "refill": 0, "trace_occupancy": 0}
 
 
 def apply(renderer, cfg: dict):
@@ -91,12 +93,19 @@ def tune(renderer, scene, region_factory, passes: int = 6, depths=LDS_DEPTHS, so
                 return TuneResult(best=hit["best"], table=hit.get("table", []), seconds=0.0)
     t0 = time.perf_counter()
     table = [measure(renderer, scene, region_factory, cfg, passes) for cfg in candidates(depths, sort_modes, tails)]
-    # the refill trace only as a second stage, on the best setting so far
+    # the refill trace and the capped-register trace only as second stages, on
+    # the best settings so far (both leave the image unchanged)
     first = max(table, key=lambda r: r["mrays"])
     for rf in REFILL[1:]:
         cfg = {k: first[k] for k in KNOBS}
         cfg["refill"] = rf
         table.append(measure(renderer, scene, region_factory, cfg, passes))
+    for occ in TRACE_OCCUPANCY[1:]:
+        for d in sorted({first["lds_stack_depth"], LDS_DEPTHS[0]}):
+            cfg = {k: first[k] for k in KNOBS}
+            cfg["trace_occupancy"] = occ
+            cfg["lds_stack_depth"] = d
+            table.append(measure(renderer, scene, region_factory, cfg, passes))
     best_row = max(table, key=lambda r: r["mrays"])
     best = {k: best_row[k] for k in KNOBS}
     apply(renderer, best)

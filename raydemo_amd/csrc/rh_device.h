@@ -499,13 +499,17 @@ __device__ __forceinline__ void traverse_fast(const Scene &sc, f3 o, f3 d, Hit &
 // The reference's default trees (use_wide_bvh): FlattenBVH's 8-wide nodes,
 // walked as Traverse_{Macro,Micro}Tree_WithStack_ClosestHit (CoreRef.cpp:
 // 1133-1222, 1395-1468) walk them.  Device form of node i: 8 slots of two
-// float4 at wnodes[16 i + 2 s]: a = (bbox_min, word), b = (bbox_max, count),
-// word = the child node's index, or prim index | RH_LEAF_NODE_BIT for a leaf
-// child (count = its prim count: a leaf is entered without loading its
-// node), or RH_MBVH_EMPTY.  A leaf node, reachable only as a tree root, holds
-// itself in slot 0.  Stacked leaf children are RH_LEAF_NODE_BIT | (8 node + s),
-// a reference to the slot that describes them.
+// float4 at wnodes[16 i + 2 s]: a = (bbox_min, word), b = (bbox_max, count).
+// The used slots come first, in the reference's slot order (which is all the
+// walk depends on: unused slots never hit), then RH_MBVH_EMPTY slots.  word =
+// an interior child's node index, or prim index | RH_LEAF_NODE_BIT for a leaf
+// child (count = its prim count: leaves have no device node of their own), or
+// RH_MBVH_EMPTY.  All 8 slots are loaded at once whatever their use: loads
+// issued under per-slot conditions measured 16 % slower.  A tree root that is
+// a leaf holds itself in slot 0 (see wide_root).  Stacked leaf children are
+// RH_LEAF_NODE_BIT | (8 node + s), a reference to the slot that describes them.
 constexpr uint32_t kWideSlotRef = RH_LEAF_NODE_BIT;
+constexpr uint32_t kWideLeafRoot = 0x40000000u;
 
 // slab test of one slot with its entry distance (bbox_test_oct, CoreRef.cpp:133-161)
 template <bool Finite>
@@ -704,11 +708,12 @@ __device__ __forceinline__ bool wide_pop(const float4 *wn, uint32_t base, float 
     return false;
 }
 
-// a tree's root: a leaf root describes itself in its slot 0
-__device__ __forceinline__ WideCur wide_root(const float4 *wn, uint32_t root) {
-    const float4 a = wn[(size_t)root * 16], b = wn[(size_t)root * 16 + 1];
-    const uint32_t w = __float_as_uint(a.w);
-    return (w & RH_LEAF_NODE_BIT) ? WideCur{w, __float_as_uint(b.w)} : WideCur{root, 0};
+// A tree's root word: its node's index, with kWideLeafRoot set when the root
+// is a leaf, whose node then describes it in slot 0.
+__device__ __forceinline__ WideCur wide_root(const float4 *wn, uint32_t root_word) {
+    if (!(root_word & kWideLeafRoot)) return WideCur{root_word, 0};
+    const float *slot = reinterpret_cast<const float *>(wn + (size_t)(root_word & ~kWideLeafRoot) * 16);
+    return WideCur{__float_as_uint(slot[3]), __float_as_uint(slot[7])};
 }
 
 // Traverse_MicroTree_WithStack_ClosestHit, 8-wide (CoreRef.cpp:1395-1468).

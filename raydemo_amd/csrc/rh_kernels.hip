@@ -36,8 +36,11 @@ __global__ void __launch_bounds__(kBlock) k_gen_primary(PassConst pc, const floa
     store_ray(rays, aux, (uint32_t)i, gen_primary_ray(pc, halton, x, y));
 }
 
-// Tree: the scene's tree kind (kTreeBinary / kTreeWide), one walk compiled per kernel
-template <bool AnyHit, bool Count, int Tree> __global__ void __launch_bounds__(kBlock) k_trace(TraceArgs a) {
+// Tree: the scene's tree kind (kTreeBinary / kTreeWide), one walk compiled per
+// kernel.  MinBlocks 5 caps an 8-wide walk's registers at 5 waves per SIMD
+// (some spills) instead of the 4 it would take (knob trace_occupancy).
+template <bool AnyHit, bool Count, int Tree, int MinBlocks = 1>
+__global__ void __launch_bounds__(kBlock, MinBlocks) k_trace(TraceArgs a) {
     extern __shared__ uint32_t lds_stack[];
     trace_body<AnyHit, Count, Tree>(a, blockIdx.x * blockDim.x, lds_stack);
 }
@@ -221,7 +224,7 @@ __global__ void k_set_material_colors(rh_material *mats, const float *__restrict
     m.main_color[2] = rgb[3 * k + 2];
 }
 
-template <int Tree> __global__ void __launch_bounds__(kBlock) k_shadow(ShadowArgs a) {
+template <int Tree, int MinBlocks = 1> __global__ void __launch_bounds__(kBlock, MinBlocks) k_shadow(ShadowArgs a) {
     extern __shared__ uint32_t lds_stack[];
     shadow_body<Tree>(a, blockIdx.x * blockDim.x, lds_stack);
 }
@@ -467,6 +470,7 @@ struct rh_context {
     int path_tail = 2;  // first bounce run by the path-tail kernel (-1: none)
     int tail_occupancy = 2;    // waves per SIMD the path-tail kernel is compiled for (2 or 3)
     int refill = 0;            // closest-hit trace: persistent kernel with lane refill
+    int trace_occupancy = 0;   // 8-wide trace/shadow kernels: 0 = as compiled (4 waves/SIMD), 5 = capped registers
     int cus = 0;               // compute units of the device
     int refill_blocks = 0, refill_blocks_lds = -1;
     int adjoint_occupancy = 2; // the same for the adjoint replay kernel
@@ -657,7 +661,15 @@ int launch_trace(rh_context *c, const TraceArgs &a, int grid_rays, bool any_hit)
     KernelMark km(c, 0);
     const dim3 grid(blocks), blk(kBlock);
     const size_t lds = lds_bytes(c);
-    if (c->sc.wnodes) {
+    if (c->sc.wnodes && c->trace_occupancy == 5) {
+        if (any_hit) {
+            if (count) hipLaunchKernelGGL((k_trace<true, true, kTreeWide, 5>), grid, blk, lds, c->stream, a);
+            else hipLaunchKernelGGL((k_trace<true, false, kTreeWide, 5>), grid, blk, lds, c->stream, a);
+        } else {
+            if (count) hipLaunchKernelGGL((k_trace<false, true, kTreeWide, 5>), grid, blk, lds, c->stream, a);
+            else hipLaunchKernelGGL((k_trace<false, false, kTreeWide, 5>), grid, blk, lds, c->stream, a);
+        }
+    } else if (c->sc.wnodes) {
         if (any_hit) {
             if (count) hipLaunchKernelGGL((k_trace<true, true, kTreeWide>), grid, blk, lds, c->stream, a);
             else hipLaunchKernelGGL((k_trace<true, false, kTreeWide>), grid, blk, lds, c->stream, a);
@@ -733,7 +745,9 @@ int launch_shade(rh_context *c, const ShadeArgs &a, int grid_rays) {
     {
         KernelMark km(c, 3);
         const dim3 grid((unsigned)std::max<size_t>(1, (lanes + kBlock - 1) / kBlock)), blk(kBlock);
-        if (c->sc.wnodes) hipLaunchKernelGGL(k_shadow<kTreeWide>, grid, blk, lds_bytes(c), c->stream, s);
+        if (c->sc.wnodes && c->trace_occupancy == 5)
+            hipLaunchKernelGGL((k_shadow<kTreeWide, 5>), grid, blk, lds_bytes(c), c->stream, s);
+        else if (c->sc.wnodes) hipLaunchKernelGGL(k_shadow<kTreeWide>, grid, blk, lds_bytes(c), c->stream, s);
         else hipLaunchKernelGGL(k_shadow<kTreeBinary>, grid, blk, lds_bytes(c), c->stream, s);
         RH_CHECK(hipGetLastError());
     }
@@ -1131,6 +1145,9 @@ int rh_set_option(rh_context *c, const char *name, int value) {
         c->tail_occupancy = value;
     } else if (n == "refill") {
         c->refill = value != 0;
+    } else if (n == "trace_occupancy") {
+        if (value != 0 && value != 5) return arg_error("trace_occupancy must be 0 or 5");
+        c->trace_occupancy = value;
     } else if (n == "adjoint_occupancy") {
         if (value != 2 && value != 3) return arg_error("adjoint_occupancy must be 2 or 3");
         c->adjoint_occupancy = value;
@@ -1150,6 +1167,7 @@ int rh_get_option(rh_context *c, const char *name, int *value) {
     else if (n == "path_tail") *value = c->path_tail;
     else if (n == "tail_occupancy") *value = c->tail_occupancy;
     else if (n == "refill") *value = c->refill;
+    else if (n == "trace_occupancy") *value = c->trace_occupancy;
     else if (n == "adjoint_occupancy") *value = c->adjoint_occupancy;
     else if (n == "defer_nee") *value = c->defer_nee;
     else if (n == "stack_overflows") {

@@ -63,9 +63,11 @@ int upload_array(std::vector<rh_rt::DevBuf> &bufs, hipStream_t stream, const T *
     return 0;
 }
 
-// The device form of the 8-wide trees (rh_device.h, "8-wide trees"): each
-// node as 8 slots of two float4, a leaf child's prim range folded into its
-// parent's slot; and the wide light tree's leaves in the order
+// The device form of the 8-wide trees (rh_device.h, "8-wide trees"): the
+// interior nodes (and leaf roots) only, each as 8 slots of two float4 with the
+// used slots first, a leaf child's prim range folded into its parent's slot,
+// the meshes' and macro tree's roots renumbered; and the wide light tree's
+// leaves in the order
 // ComputeDirectLighting walks them (CoreRef.cpp:1949-1970: slots pushed in
 // slot order, the last popped first).  A light leaf is reached only if the
 // point is strictly inside its slot box and its ancestors' (which contain
@@ -74,8 +76,24 @@ int upload_array(std::vector<rh_rt::DevBuf> &bufs, hipStream_t stream, const T *
 inline int build_wide(const rh_scene_view *v, hipStream_t stream, std::vector<rh_rt::DevBuf> &bufs, Scene &s,
                       std::vector<rh_bvh_node> &leaves) {
     const uint32_t n = v->mnodes_count;
-    if ((uint64_t)n * 8 >= RH_LEAF_NODE_BIT) return rh_rt::arg_error("too many wide nodes (8 x count must fit 31 bits)");
-    std::vector<float4> w((size_t)n * 16);
+    auto is_leaf = [&](uint32_t i) { return (v->mnodes[i].child[0] & RH_LEAF_NODE_BIT) != 0; };
+    // device nodes: the interior nodes and the tree roots (a leaf child lives in its parent's slot)
+    std::vector<uint8_t> root(n, 0);
+    for (uint32_t k = 0; k < v->meshes_count; k++) {
+        if (v->meshes[k].node_index >= n) return rh_rt::arg_error("mesh tree root outside the wide node array");
+        root[v->meshes[k].node_index] = 1;
+    }
+    if (s.macro_root != RH_INVALID) {
+        if (s.macro_root >= n) return rh_rt::arg_error("macro tree root outside the wide node array");
+        root[s.macro_root] = 1;
+    }
+    std::vector<uint32_t> dev(n, RH_INVALID);
+    uint32_t m = 0;
+    for (uint32_t i = 0; i < n; i++)
+        if (!is_leaf(i) || root[i]) dev[i] = m++;
+    if (m >= (1u << 28)) return rh_rt::arg_error("too many wide nodes for the device form (2^28)");
+
+    std::vector<float4> w((size_t)m * 16);
     const float nan = std::numeric_limits<float>::quiet_NaN();
     auto as_float = [](uint32_t u) {
         float f;
@@ -83,48 +101,58 @@ inline int build_wide(const rh_scene_view *v, hipStream_t stream, std::vector<rh
         return f;
     };
     for (uint32_t i = 0; i < n; i++) {
-        const rh_mbvh_node &m = v->mnodes[i];
-        const bool leaf = (m.child[0] & RH_LEAF_NODE_BIT) != 0;
+        if (dev[i] == RH_INVALID) continue;
+        const rh_mbvh_node &nd = v->mnodes[i];
+        float4 *out = &w[(size_t)dev[i] * 16];
+        int j = 0; // next device slot: used slots first, in slot order
         for (int k = 0; k < 8; k++) {
-            uint32_t word = RH_MBVH_EMPTY, count = 0;
-            if (leaf) {
-                if (k == 0) {
-                    word = m.child[0];
-                    count = m.child[1];
-                }
-            } else if (m.child[k] != RH_MBVH_EMPTY) {
-                if (m.child[k] >= n) return rh_rt::arg_error("wide node child outside the node array");
-                const rh_mbvh_node &c = v->mnodes[m.child[k]];
-                if (c.child[0] & RH_LEAF_NODE_BIT) {
-                    word = c.child[0];
-                    count = c.child[1];
+            uint32_t word, count = 0;
+            if (is_leaf(i)) {
+                if (k > 0) break;
+                word = nd.child[0];
+                count = nd.child[1];
+            } else {
+                if (nd.child[k] == RH_MBVH_EMPTY) continue;
+                const uint32_t c = nd.child[k];
+                if (c >= n) return rh_rt::arg_error("wide node child outside the node array");
+                if (is_leaf(c)) {
+                    word = v->mnodes[c].child[0];
+                    count = v->mnodes[c].child[1];
                 } else {
-                    word = m.child[k];
+                    word = dev[c];
                 }
             }
-            const bool used = word != RH_MBVH_EMPTY;
-            w[(size_t)i * 16 + 2 * k] = make_float4(used ? m.bbox_min[0][k] : nan, used ? m.bbox_min[1][k] : nan,
-                                                    used ? m.bbox_min[2][k] : nan, as_float(word));
-            w[(size_t)i * 16 + 2 * k + 1] = make_float4(used ? m.bbox_max[0][k] : nan, used ? m.bbox_max[1][k] : nan,
-                                                        used ? m.bbox_max[2][k] : nan, as_float(count));
+            out[2 * j] = make_float4(nd.bbox_min[0][k], nd.bbox_min[1][k], nd.bbox_min[2][k], as_float(word));
+            out[2 * j + 1] = make_float4(nd.bbox_max[0][k], nd.bbox_max[1][k], nd.bbox_max[2][k], as_float(count));
+            j++;
+        }
+        for (; j < 8; j++) {
+            out[2 * j] = make_float4(nan, nan, nan, as_float(RH_MBVH_EMPTY));
+            out[2 * j + 1] = make_float4(nan, nan, nan, as_float(0u));
         }
     }
-    for (uint32_t k = 0; k < v->meshes_count; k++)
-        if (v->meshes[k].node_index >= n) return rh_rt::arg_error("mesh tree root outside the wide node array");
     int rc = upload_array(bufs, stream, w.data(), w.size(), &s.wnodes);
     if (rc) return rc;
+    // a tree root is named by its node's index, flagged when it is a leaf
+    auto root_word = [&](uint32_t i) { return dev[i] | (is_leaf(i) ? kWideLeafRoot : 0u); };
+    std::vector<rh_mesh> meshes(v->meshes, v->meshes + v->meshes_count);
+    for (rh_mesh &mesh : meshes) mesh.node_index = root_word(mesh.node_index);
+    rc = upload_array(bufs, stream, meshes.data(), meshes.size(), &s.meshes);
+    if (rc) return rc;
+    if (s.macro_root != RH_INVALID) s.macro_root = root_word(s.macro_root);
+
     if (s.light_root == RH_INVALID) return 0;
     if (s.light_root >= n) return rh_rt::arg_error("light tree root outside the node array");
-    const rh_mbvh_node &root = v->mnodes[s.light_root];
-    if (root.child[0] & RH_LEAF_NODE_BIT) {
+    const rh_mbvh_node &lroot = v->mnodes[s.light_root];
+    if (lroot.child[0] & RH_LEAF_NODE_BIT) {
         s.light_untested = 1;
-        leaves.push_back(rh_bvh_node{{-RH_MAX_DIST, -RH_MAX_DIST, -RH_MAX_DIST}, root.child[0],
-                                     {RH_MAX_DIST, RH_MAX_DIST, RH_MAX_DIST}, root.child[1]});
+        leaves.push_back(rh_bvh_node{{-RH_MAX_DIST, -RH_MAX_DIST, -RH_MAX_DIST}, lroot.child[0],
+                                     {RH_MAX_DIST, RH_MAX_DIST, RH_MAX_DIST}, lroot.child[1]});
         return 0;
     }
     std::vector<std::pair<uint32_t, int>> st; // (node, slot) still to visit
     for (int k = 0; k < 8; k++)
-        if (root.child[k] != RH_MBVH_EMPTY) st.push_back({s.light_root, k});
+        if (lroot.child[k] != RH_MBVH_EMPTY) st.push_back({s.light_root, k});
     while (!st.empty()) {
         const auto e = st.back();
         st.pop_back();
@@ -154,7 +182,7 @@ inline int build_device_scene(const rh_scene_view *v, hipStream_t stream, std::v
     rc |= upload_array(bufs, stream, v->nodes, v->nodes_count, &s.nodes);
     rc |= upload_array(bufs, stream, v->tris, v->tris_count, &s.tris);
     rc |= upload_array(bufs, stream, v->tri_indices, v->tri_indices_count, &s.tri_indices);
-    rc |= upload_array(bufs, stream, v->meshes, v->meshes_count, &s.meshes);
+    if (!v->mnodes) rc |= upload_array(bufs, stream, v->meshes, v->meshes_count, &s.meshes); // wide: renumbered
     rc |= upload_array(bufs, stream, v->mesh_instances, v->mesh_instances_count, &s.mesh_instances);
     rc |= upload_array(bufs, stream, v->mi_indices, v->mi_indices_count, &s.mi_indices);
     rc |= upload_array(bufs, stream, v->transforms, v->transforms_count, &s.transforms);
@@ -192,8 +220,7 @@ inline int build_device_scene(const rh_scene_view *v, hipStream_t stream, std::v
     rc = upload_array(bufs, stream, leaves.data(), leaves.size(), &s.light_leaves);
     if (rc) return rc;
     s.light_leaf_count = (uint32_t)leaves.size();
-    const uint32_t tree_nodes = v->mnodes ? v->mnodes_count : v->nodes_count;
-    if (s.macro_root != RH_INVALID && ((!v->nodes && !v->mnodes) || s.macro_root >= tree_nodes))
+    if (!v->mnodes && s.macro_root != RH_INVALID && (!v->nodes || s.macro_root >= v->nodes_count))
         return rh_rt::arg_error("macro tree root outside the node array");
     out = s;
     return 0;

@@ -69,7 +69,8 @@ def _rmse(a, b):
 
 
 @pytest.mark.parametrize("opts", [{}, {"sort_rays": 1, "lds_stack_depth": 8, "path_tail": -1}, {"defer_nee": 0},
-                                  {"path_tail": 0, "tail_occupancy": 3}, {"refill": 1}])
+                                  {"path_tail": 0, "tail_occupancy": 3}, {"refill": 1},
+                                  {"trace_occupancy": 5, "lds_stack_depth": 8, "path_tail": 1}])
 def test_wide_render_matches_oracle(oracle, host_lib, golden_scene1, opts):
     """Image and ray counts of the wide-tree renderer equal the oracle's wide
     render of the same passes."""
