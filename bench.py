@@ -392,7 +392,8 @@ def main():
     bytes_alg = (traced * (B_RAY + B_HIT) + vis["node_visits"] * b_node + vis["instance_tests"] * B_INST +
                  vis["tri_tests"] * B_TRI)
     achieved = bytes_alg / (trace_ms / 1e3) / 1e9 if trace_ms > 0 else 0.0
-    kernel = f"k_trace<false,false,{1 if wide else 0}>"
+    min_blocks = 5 if (wide and tuned.best.get("trace_occupancy", 0) == 5) else 1
+    kernel = f"k_trace<false,false,{1 if wide else 0},{min_blocks}>"
     traffic_file = ROOT / "profiles" / "trace_traffic.json"
     traffic = None
     if traffic_file.exists():
