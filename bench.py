@@ -336,6 +336,143 @@ def run_batch(args, lib, A, scenes, knobs, wide):
     return out
 
 
+def run_large(args, lib, A, scenes, knobs, wide):
+    """The large two-level workload (scenes.large_instanced: 4097 instances of
+    16 textured meshes, 32 sRGB textures of 2048^2, 3840x2160, 16 spp, depth
+    12 through the max_bounces extension): Mrays/s of its passes; texel-gradient
+    descent on 4 of its textures against a target rendered from them perturbed
+    with Gaussian noise (sigma 0.1), s per iteration; and the scene rendered in
+    one batch with 16 small soups against each rendered alone."""
+    import numpy as np
+    import torch
+    from raydemo_amd import adjoint
+
+    t0 = time.time()
+    r = A.Renderer(lib, args.large_width, args.large_height, A.RENDERER_HIP, use_wide_bvh=wide)
+    for k, v in knobs.items():
+        r.set_option(k, v)
+    r.set_option("max_bounces", 12)
+    s = r.create_scene()
+    spec = scenes.large_instanced(s, width=args.large_width, height=args.large_height, spp=args.large_spp)
+    s.triangle_count()
+    s.view()
+    t_build = time.time() - t0
+    log(f"large: {spec.info['instances']} instances, {spec.info['instanced_triangles']} instanced triangles "
+        f"({spec.info['base_triangles']} in the meshes), host build {t_build:.1f}s")
+    W, H, spp = args.large_width, args.large_height, args.large_spp
+
+    def total(rc):
+        return rc["primary"] + rc["secondary"] + rc["shadow"]
+
+    reg = A.Region(lib, 0, 0, W, H)
+    r.render(s, reg)  # upload + warm-up
+    r.synchronize()
+    r.clear()
+    r.reset_stats()
+    reg = A.Region(lib, 0, 0, W, H)
+    t1 = time.perf_counter()
+    for _ in range(spp):
+        r.render(s, reg)
+    r.synchronize()
+    dt = time.perf_counter() - t1
+    rays = total(r.ray_counts())
+    out = {"mrays": round(rays / dt / 1e6, 2), "seconds": round(dt, 4), "rays": rays,
+           "config": {"workload": f"large_instanced: {spec.info['instances']} instances, "
+                                  f"{spec.info['instanced_triangles']} instanced triangles, 32 sRGB textures "
+                                  f"2048^2 with mips, {W}x{H}, {spp} spp, depth 12",
+                      "trees": "8-wide" if wide else "binary", "host_build_s": round(t_build, 1)}}
+    log(f"large render: {out['mrays']} Mrays/s ({spp} passes in {dt:.3f}s)")
+
+    # texel-gradient descent on 4 textures
+    if args.large_adj_steps > 0:
+        dr = adjoint.DiffRenderer(lib, r, s, W, H, spp, len(s.view()["materials"]))
+        ids = tuple(spec.info["adjoint_textures"])
+        orig = np.stack(spec.info["adjoint_texels"]).astype(np.float32) / 255.0
+        noise = np.random.default_rng(scenes.SEED_BASE + 55).normal(0.0, 0.1, orig.shape).astype(np.float32)
+        target_tex = torch.tensor(np.clip(orig + noise, 0.0, 1.0), device="cuda")
+        target = adjoint.render_textures(target_tex, dr, ids).detach().clone()
+        params = torch.tensor(orig, device="cuda").requires_grad_(True)
+        opt = torch.optim.Adam([params], lr=0.01)
+        losses = []
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for _ in range(args.large_adj_steps):
+            opt.zero_grad(set_to_none=True)
+            img = adjoint.render_textures(params, dr, ids)
+            loss = torch.mean((img - target) ** 2)
+            loss.backward()
+            opt.step()
+            with torch.no_grad():
+                params.clamp_(0.0, 1.0)
+            losses.append(float(loss.detach()))
+        torch.cuda.synchronize()
+        dt2 = time.perf_counter() - t2
+        out["texel_adjoint"] = {
+            "metric": "texel-gradient descent s/iter (forward + path-replay adjoint + fold + Adam)",
+            "s_per_iter": round(dt2 / args.large_adj_steps, 4), "steps": args.large_adj_steps,
+            "parameters": int(params.numel()), "loss_first": losses[0], "loss_last": losses[-1],
+            "textures": list(ids)}
+        log(f"large texel adjoint: {out['texel_adjoint']}")
+        # back to the generator's textures for the batch leg
+        adjoint.render_textures(torch.tensor(orig, device="cuda"), dr, ids)
+    r.close()
+
+    # the scene in one batch with 16 small soups, against each alone
+    if args.large_batch_scenes > 0:
+        items = scenes.batch_workload(args.large_batch_scenes, seed=scenes.SEED_BASE + 3)
+        builds = [(None, W, H, spp)] + [
+            (lambda sc, it=it: scenes.batch_scene(sc, it), it.width, it.height, it.spp) for it in items]
+        ss = [s]  # the large scene itself (its textures are back to the generator's)
+        for build, w, h, n in builds[1:]:
+            sc = A.standalone_scene(lib, wide)
+            build(sc)
+            sc.triangle_count()
+            ss.append(sc)
+        rs = []
+        for sc, (build, w, h, n) in zip(ss, builds):
+            rr = A.Renderer(lib, w, h, A.RENDERER_HIP, use_wide_bvh=wide)
+            for k, v in knobs.items():
+                rr.set_option(k, v)
+            rr.set_option("max_bounces", 12)
+            rr.render(sc, A.Region(lib, 0, 0, w, h))
+            rr.synchronize()
+            rr.clear()
+            rr.reset_stats()
+            rs.append(rr)
+        t3 = time.perf_counter()
+        for rr, sc, (build, w, h, n) in zip(rs, ss, builds):
+            rg = A.Region(lib, 0, 0, w, h)
+            for _ in range(n):
+                rr.render(sc, rg)
+        for rr in rs:
+            rr.synchronize()
+        t_plain = time.perf_counter() - t3
+        rays_plain = sum(total(rr.ray_counts()) for rr in rs)
+        for rr in rs:
+            rr.close()
+        b = A.Batch(lib)
+        b.set_option("max_bounces", 12)
+        for sc, (build, w, h, n) in zip(ss, builds):
+            b.add(sc, w, h, n)
+        for k, v in (args.large_batch_knobs or {}).items():
+            b.set_option(k, v)
+        b.render_pass()
+        b.synchronize()
+        b.restart()
+        b.reset_stats()
+        t4 = time.perf_counter()
+        b.render_all()
+        b.synchronize()
+        t_batch = time.perf_counter() - t4
+        rays_batch = total(b.ray_counts())
+        b.close()
+        out["batch"] = {"scenes": len(builds), "batched_mrays": round(rays_batch / t_batch / 1e6, 2),
+                        "plain_mrays": round(rays_plain / t_plain / 1e6, 2), "speedup": round(t_plain / t_batch, 3),
+                        "rays": rays_batch, "rays_plain": rays_plain}
+        log(f"large batch: {out['batch']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -358,6 +495,12 @@ def main():
     ap.add_argument("--adj-size", type=int, default=512)
     ap.add_argument("--adj-spp", type=int, default=32)
     ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batch workload (0: skip)")
+    ap.add_argument("--large", type=int, default=1, help="the large two-level workload (0: skip)")
+    ap.add_argument("--large-width", type=int, default=3840)
+    ap.add_argument("--large-height", type=int, default=2160)
+    ap.add_argument("--large-spp", type=int, default=16)
+    ap.add_argument("--large-adj-steps", type=int, default=5, help="texel-gradient descent steps (0: skip)")
+    ap.add_argument("--large-batch-scenes", type=int, default=16, help="small scenes batched with it (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -365,7 +508,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    if args.adj_steps > 0:
+    args.large_batch_knobs = None
+    if args.adj_steps > 0 or args.large:
         import torch  # noqa: F401  (loaded before the HIP libraries: one HIP runtime in the process)
     from raydemo_amd import autotune, capi as A, host_lib, scenes
 
@@ -435,6 +579,9 @@ def main():
     plain_knobs = {k: v for k, v in tuned.best.items() if k in autotune.KNOBS}
     adj = run_adjoint(args, lib, A, scenes, plain_knobs, wide) if args.adj_steps > 0 else None
     batch = run_batch(args, lib, A, scenes, plain_knobs, wide) if args.batch_scenes > 0 else None
+    if batch is not None:
+        args.large_batch_knobs = {k: v for k, v in batch["knobs"].items() if k != "use_graph"}
+    large = run_large(args, lib, A, scenes, plain_knobs, wide) if args.large else None
 
     line = {
         "metric": "Mrays/s at 1080p depth-8 (triangle soup, primary+secondary+shadow rays)",
@@ -467,6 +614,7 @@ def main():
         "cpu_baseline": cpu,
         "adjoint": adj,
         "batch": batch,
+        "large": large,
     }
     print(json.dumps(line), flush=True)
 

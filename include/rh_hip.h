@@ -69,6 +69,8 @@ int rh_get_ray_counts(rh_context *ctx, rh_ray_counts *out);
  * "refill" (0/1: closest-hit trace as a persistent kernel whose lanes take new rays as theirs finish;
  *   binary trees only),
  * "trace_occupancy" (0 or 5: the 8-wide trace and shadow kernels as compiled, or capped at 5 waves per SIMD),
+ * "max_bounces" (1..16: the clamp of max_total_depth; 10, the reference's MAX_BOUNCES, by default:
+ *   larger values render deeper paths than the reference can, an extension),
  * "adjoint_occupancy" (the same for the adjoint replay kernel),
  * "defer_nee" (0/1: shadow rays of next-event estimation as their own stage). */
 int rh_set_option(rh_context *ctx, const char *name, int value);
@@ -132,7 +134,8 @@ int rh_read_accum(rh_context *ctx, float *dst, size_t count);
  * caller uploads the Halton table of iteration k with rh_batch_set_halton
  * before pass k when k == 1 or k % 256 == 0, as RenderScene refreshes it.
  * A pass's launches are captured once as a hipGraph and replayed ("use_graph",
- * default 1).  Knobs: "lds_stack_depth", "path_tail" (as rh_set_option),
+ * default 1).  Knobs: "lds_stack_depth", "path_tail", "max_bounces" (as
+ * rh_set_option; max_bounces applies to scenes added after it is set),
  * "use_graph", "blocks_per_cu" (persistent grid size).  Stats: rays counted
  * as by rh_get_ray_counts (primary, secondary, shadow). */
 typedef struct rh_batch rh_batch;

@@ -44,7 +44,8 @@ extern "C" {
 #define RH_FLT_EPS 0.0000001f
 #define RH_PI 3.141592653589793238463f
 #define RH_MAX_DIST 3.402823466e+38F
-#define RH_MAX_BOUNCES 10
+#define RH_MAX_BOUNCES 10     /* MAX_BOUNCES (Core.h): the reference clamps max_total_depth to it */
+#define RH_MAX_BOUNCES_EXT 16 /* the largest depth the "max_bounces" option allows (extension) */
 #define RH_LIGHT_ATTEN_CUTOFF 0.001f
 
 #define RH_LEAF_NODE_BIT 0x80000000u

@@ -1168,6 +1168,15 @@ int orc_shade(const rh_scene_view *sc, const rh_pass_info *pi, const rh_ray_pack
     return 0;
 }
 
+// the max_total_depth clamp: MAX_BOUNCES of the reference, or a larger
+// value for the HIP backend's deeper-path extension ("max_bounces")
+static int g_max_bounces = RH_MAX_BOUNCES;
+int orc_set_max_bounces(int n) {
+    if (n < 1 || n > RH_MAX_BOUNCES_EXT) return -1;
+    g_max_bounces = n;
+    return 0;
+}
+
 // RendererRef::RenderScene (RendererRef.cpp:19-325) over one region
 int orc_render(const rh_scene_view *sc, const rh_camera *cam, int w, int h, const int32_t rect[4], int passes,
                int *iteration, float *clean, float *final_rgba, unsigned long long ray_counts[3]) {
@@ -1211,7 +1220,7 @@ int orc_render(const rh_scene_view *sc, const rh_camera *cam, int w, int h, cons
         pi.iteration = it;
         pi.bounce = 2;
         pi.settings = cam->pass_settings;
-        pi.settings.max_total_depth = std::min<uint8_t>(pi.settings.max_total_depth, RH_MAX_BOUNCES);
+        pi.settings.max_total_depth = (uint8_t)std::min<int>(pi.settings.max_total_depth, g_max_bounces);
         Ctx c{*sc, halton.data()};
 
         orc_gen_primary(cam, rect, w, h, it, halton.data(), rays.data());
@@ -1303,7 +1312,7 @@ int orc_adjoint(const rh_scene_view *sc, const rh_camera *cam, int w, int h, con
         pi.iteration = it;
         pi.bounce = 2;
         pi.settings = cam->pass_settings;
-        pi.settings.max_total_depth = std::min<uint8_t>(pi.settings.max_total_depth, RH_MAX_BOUNCES);
+        pi.settings.max_total_depth = (uint8_t)std::min<int>(pi.settings.max_total_depth, g_max_bounces);
         Ctx c{*sc, halton.data()};
         for (auto &p : path) p.clear();
 

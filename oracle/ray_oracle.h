@@ -53,6 +53,8 @@ int orc_shade(const rh_scene_view *sc, const rh_pass_info *pi, const rh_ray_pack
  * for the mean). */
 int orc_adjoint(const rh_scene_view *sc, const rh_camera *cam, int w, int h, const int32_t rect[4], int passes,
                 const float *adj_rgba, double scale, double *grad, double *texel_grad);
+/* the max_total_depth clamp (RH_MAX_BOUNCES by default, up to RH_MAX_BOUNCES_EXT) */
+int orc_set_max_bounces(int n);
 int orc_render(const rh_scene_view *sc, const rh_camera *cam, int w, int h, const int32_t rect[4], int passes,
                int *iteration, float *clean, float *final_rgba, unsigned long long ray_counts[3]);
 

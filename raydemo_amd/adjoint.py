@@ -38,6 +38,15 @@ class DiffRenderer:
         self.device = device
         self.rays_forward = 0
         self.rays_adjoint = 0
+        self._layout = None
+
+    def tex_layout(self) -> dict:
+        """The scene's texture records, mip blocks and texel count (they do not
+        change when texel values do)."""
+        if self._layout is None:
+            v = self.s.view()
+            self._layout = {k: v[k] for k in ("textures", "tex_mips", "texels_count")}
+        return self._layout
 
     def set_colors(self, colors: torch.Tensor):
         rgb = colors.detach().to(torch.float32).cpu().numpy()
@@ -66,7 +75,7 @@ class DiffRenderer:
         g = torch.zeros((self.n_materials, 4), dtype=torch.float32, device=self.device)
         tg = None
         if texels:
-            tg = torch.zeros((self.s.view()["texels_count"], 4), dtype=torch.float32, device=self.device)
+            tg = torch.zeros((self.tex_layout()["texels_count"], 4), dtype=torch.float32, device=self.device)
         torch.cuda.synchronize()
         reg = A.Region(self.lib, 0, 0, self.w, self.h)
         for _ in range(self.spp):
@@ -113,6 +122,33 @@ def fold_texel_grad(view: dict, tex: int, tg: np.ndarray) -> np.ndarray:
     return levels[0]
 
 
+def fold_texel_grad_torch(view: dict, tex: int, tg: torch.Tensor) -> torch.Tensor:
+    """fold_texel_grad on the device: tg is the (texels, 4) gradient tensor,
+    the result the (h, w, 3) level-0 gradient (float32) on tg's device."""
+    t = view["textures"][tex]
+    w0, h0 = int(t["width"]) & 0x7FFF, int(t["height"])
+    mips = view["tex_mips"][tex]
+    dev = tg.device
+    levels = []
+    for lev in range(len(mips)):
+        if lev > 0 and mips[lev]["offset"] == mips[lev - 1]["offset"]:
+            break
+        bw, bh, off = int(mips[lev]["bw"]), int(mips[lev]["bh"]), int(mips[lev]["offset"])
+        lw, lh = bw - 2, bh - 2
+        block = tg[off:off + bw * bh, :3].reshape(bh * bw, 3)
+        ys = (torch.arange(bh, device=dev) - 1) % lh
+        xs = (torch.arange(bw, device=dev) - 1) % lw
+        dst = (ys[:, None] * lw + xs[None, :]).reshape(-1)
+        g = torch.zeros((lh * lw, 3), dtype=torch.float32, device=dev)
+        g.index_add_(0, dst, block.to(torch.float32))
+        levels.append(g.reshape(lh, lw, 3))
+    for lev in range(len(levels) - 1, 0, -1):
+        up = levels[lev].repeat_interleave(2, 0).repeat_interleave(2, 1) / 4.0
+        levels[lev - 1] += up
+    assert levels[0].shape[:2] == (h0, w0)
+    return levels[0]
+
+
 class RenderFunction(torch.autograd.Function):
     """image = render(colors); backward by path replay."""
 
@@ -146,10 +182,9 @@ class TextureRenderFunction(torch.autograd.Function):
     def backward(ctx, grad_out: torch.Tensor):
         dr = ctx.dr
         _, tg = dr.adjoint(grad_out.contiguous(), texels=True)
-        tg = tg.cpu().numpy().astype(np.float64)
-        view = dr.s.view()
-        g = np.stack([fold_texel_grad(view, t, tg)[..., :3] for t in ctx.tex_ids])
-        return torch.tensor(g, dtype=torch.float32, device=ctx.param_device), None, None
+        layout = dr.tex_layout()
+        g = torch.stack([fold_texel_grad_torch(layout, t, tg) for t in ctx.tex_ids])
+        return g.to(ctx.param_device), None, None
 
 
 def render_textures(texels: torch.Tensor, dr: DiffRenderer, tex_ids: tuple) -> torch.Tensor:

@@ -29,7 +29,7 @@ namespace {
 
 constexpr int kMaxBatchScenes = 256;
 // tile counters of one pass: generate, five per staged bounce, two tails, mix
-constexpr int kMaxStages = 5 * (RH_MAX_BOUNCES + 1) + 4;
+constexpr int kMaxStages = 5 * (RH_MAX_BOUNCES_EXT + 1) + 4;
 
 // one scene of the batch (device table)
 struct BatchEntry {
@@ -48,8 +48,8 @@ struct BatchDev {
     const BatchEntry *__restrict__ table;
     int n_scenes;
     int *pass;          // passes started so far (device)
-    int *counts;        // [(RH_MAX_BOUNCES + 2) * n_scenes]: rays entering stage b (b = 0: primary)
-    int *shadow_counts; // [(RH_MAX_BOUNCES + 2) * n_scenes]
+    int *counts;        // [(RH_MAX_BOUNCES_EXT + 2) * n_scenes]: rays entering stage b (b = 0: primary)
+    int *shadow_counts; // [(RH_MAX_BOUNCES_EXT + 2) * n_scenes]
     int *work;          // [kMaxStages] tile counters
     rh_ray *rays[2];
     rh_ray_aux *aux[2];
@@ -137,7 +137,7 @@ __global__ void k_b_begin_pass(BatchDev b, BatchEntry *table, int n_stages) {
     const int pass = *b.pass + 1; // every thread reads the old value
     __syncthreads();
     const int S = b.n_scenes;
-    const int nc = (RH_MAX_BOUNCES + 2) * S;
+    const int nc = (RH_MAX_BOUNCES_EXT + 2) * S;
     for (int k = threadIdx.x; k < nc; k += blockDim.x) {
         b.counts[k] = (k < S && pass <= table[k].spp) ? (int)table[k].pixels : 0;
         b.shadow_counts[k] = 0;
@@ -362,6 +362,7 @@ struct rh_batch {
     int path_tail = 2;
     int use_graph = 1;
     int blocks_per_cu = 8;
+    int max_bounces = RH_MAX_BOUNCES; // depth clamp of scenes added from now on
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -445,8 +446,8 @@ int build(rh_batch *b) {
     int rc = 0;
     rc |= dev_alloc(&b->table, S);
     rc |= dev_alloc(&b->pass, 1);
-    rc |= dev_alloc(&b->counts, (RH_MAX_BOUNCES + 2) * S);
-    rc |= dev_alloc(&b->shadow_counts, (RH_MAX_BOUNCES + 2) * S);
+    rc |= dev_alloc(&b->counts, (RH_MAX_BOUNCES_EXT + 2) * S);
+    rc |= dev_alloc(&b->shadow_counts, (RH_MAX_BOUNCES_EXT + 2) * S);
     rc |= dev_alloc(&b->work, (size_t)kMaxStages);
     for (int k = 0; k < 2; k++) {
         rc |= dev_alloc(&b->rays[k], P);
@@ -606,7 +607,7 @@ int rh_batch_add_scene(rh_batch *b, const rh_scene_view *v, const rh_pass_desc *
     e.pi.iteration = 1;
     e.pi.bounce = 2;
     e.pi.settings = desc->cam.pass_settings;
-    e.depth = std::min<int>(desc->cam.pass_settings.max_total_depth, RH_MAX_BOUNCES);
+    e.depth = std::min<int>(desc->cam.pass_settings.max_total_depth, b->max_bounces);
     e.pi.settings.max_total_depth = (uint8_t)e.depth;
     e.pc.cam.pass_settings.max_total_depth = (uint8_t)e.depth;
     e.offset = (uint32_t)b->total_pixels;
@@ -730,6 +731,9 @@ int rh_batch_set_option(rh_batch *b, const char *name, int value) {
     } else if (n == "blocks_per_cu") {
         if (value < 1 || value > 64) return arg_error("blocks_per_cu must be 1..64");
         b->blocks_per_cu = value;
+    } else if (n == "max_bounces") {
+        if (value < 1 || value > RH_MAX_BOUNCES_EXT) return arg_error("max_bounces must be 1..16");
+        b->max_bounces = value;
     } else {
         return arg_error("unknown option");
     }

@@ -116,6 +116,9 @@ struct Scene {
     // the wide light tree is a single leaf: its lights are taken for every
     // point, with no bounds test (the wide walk tests only child slots)
     uint32_t light_untested;
+    // every triangle is solid (no transparent material reachable): a
+    // visibility query then only asks whether any triangle lies in between
+    uint32_t all_solid;
 };
 
 struct Hit {
@@ -718,10 +721,13 @@ __device__ __forceinline__ WideCur wide_root(const float4 *wn, uint32_t root_wor
 
 // Traverse_MicroTree_WithStack_ClosestHit, 8-wide (CoreRef.cpp:1395-1468).
 // The reference's wide any-hit walk takes the closest hit inside each
-// instance too (CoreRef.cpp:1357), so this is the only micro walk.
+// instance too (CoreRef.cpp:1357), so this is the only micro walk.  With
+// first_hit the walk ends at the first solid triangle it hits instead: for a
+// scene whose triangles are all solid, whether a segment is blocked does not
+// depend on which blocking triangle is found (visibility queries only).
 template <bool Finite>
 __device__ __forceinline__ bool walk_micro_wide(const Scene &sc, uint32_t root, f3 mo, f3 md, f3 minv, int mi_index,
-                                                Hit &hit, Stack &st, TraceCounters *cnt) {
+                                                Hit &hit, Stack &st, TraceCounters *cnt, bool first_hit) {
     const uint32_t base = st.sp;
     WideCur cur = wide_root(sc.wnodes, root);
     bool res = false;
@@ -732,7 +738,14 @@ __device__ __forceinline__ bool walk_micro_wide(const Scene &sc, uint32_t root, 
             leaf.b = cur.count;
             const int before = hit.prim;
             const float t_before = hit.t;
-            intersect_leaf<false>(sc, leaf, mo, md, mi_index, hit, cnt);
+            if (first_hit) {
+                if (intersect_leaf<true>(sc, leaf, mo, md, mi_index, hit, cnt)) {
+                    st.sp = base;
+                    return true;
+                }
+            } else {
+                intersect_leaf<false>(sc, leaf, mo, md, mi_index, hit, cnt);
+            }
             res |= hit.t != t_before || hit.prim != before;
         } else {
             if (cnt) cnt->nodes++;
@@ -746,7 +759,8 @@ __device__ __forceinline__ bool walk_micro_wide(const Scene &sc, uint32_t root, 
 // 1133-1222, 1270-1368): an any-hit walk ends at the first instance whose
 // closest hit is on a solid triangle.
 template <bool AnyHit>
-__device__ __forceinline__ void traverse_wide(const Scene &sc, f3 o, f3 d, Hit &hit, Stack &st, TraceCounters *cnt) {
+__device__ __forceinline__ void traverse_wide(const Scene &sc, f3 o, f3 d, Hit &hit, Stack &st, TraceCounters *cnt,
+                                              bool first_hit = false) {
     const f3 inv_d = safe_invert(d);
     st.sp = 0;
     WideCur cur = wide_root(sc.wnodes, sc.macro_root);
@@ -769,8 +783,10 @@ __device__ __forceinline__ void traverse_wide(const Scene &sc, f3 o, f3 d, Hit &
                 const uint32_t root = sc.meshes[mi.mesh_index].node_index;
                 const bool finite =
                     __builtin_isfinite(minv.x) && __builtin_isfinite(minv.y) && __builtin_isfinite(minv.z);
-                const bool found = finite ? walk_micro_wide<true>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt)
-                                          : walk_micro_wide<false>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt);
+                const bool fh = AnyHit && first_hit;
+                const bool found =
+                    finite ? walk_micro_wide<true>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt, fh)
+                           : walk_micro_wide<false>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt, fh);
                 if (AnyHit && found) {
                     const rh_tri_accel ht = load_tri(sc.tris, (uint32_t)hit.prim);
                     if (ht.ci & RH_TRI_SOLID_BIT) {
@@ -791,9 +807,12 @@ __device__ __forceinline__ void traverse_wide(const Scene &sc, f3 o, f3 d, Hit &
 // kernel is launched only for scenes of that kind), kTreeAny both, chosen per
 // scene at run time (at the register cost of the larger).
 constexpr int kTreeBinary = 0, kTreeWide = 1, kTreeAny = 2;
+// first_hit: see walk_micro_wide (the binary any-hit walk already stops at
+// the first solid triangle, as the reference's does).
 template <bool AnyHit, int Tree = kTreeAny>
-__device__ __forceinline__ void traverse(const Scene &sc, f3 o, f3 d, Hit &hit, Stack &st, TraceCounters *cnt) {
-    if (Tree == kTreeWide || (Tree == kTreeAny && sc.wnodes)) traverse_wide<AnyHit>(sc, o, d, hit, st, cnt);
+__device__ __forceinline__ void traverse(const Scene &sc, f3 o, f3 d, Hit &hit, Stack &st, TraceCounters *cnt,
+                                         bool first_hit = false) {
+    if (Tree == kTreeWide || (Tree == kTreeAny && sc.wnodes)) traverse_wide<AnyHit>(sc, o, d, hit, st, cnt, first_hit);
     else traverse_fast<AnyHit>(sc, o, d, hit, st, cnt);
 }
 

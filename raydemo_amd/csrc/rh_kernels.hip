@@ -471,6 +471,7 @@ struct rh_context {
     int tail_occupancy = 2;    // waves per SIMD the path-tail kernel is compiled for (2 or 3)
     int refill = 0;            // closest-hit trace: persistent kernel with lane refill
     int trace_occupancy = 0;   // 8-wide trace/shadow kernels: 0 = as compiled (4 waves/SIMD), 5 = capped registers
+    int max_bounces = RH_MAX_BOUNCES; // max_total_depth clamp (the reference's 10; up to RH_MAX_BOUNCES_EXT)
     int cus = 0;               // compute units of the device
     int refill_blocks = 0, refill_blocks_lds = -1;
     int adjoint_occupancy = 2; // the same for the adjoint replay kernel
@@ -958,6 +959,7 @@ int rh_render_pass(rh_context *c, const rh_pass_desc *p) {
     pi.iteration = p->iteration;
     pi.bounce = 2;
     pi.settings = p->cam.pass_settings;
+    pi.settings.max_total_depth = (uint8_t)std::min<int>(pi.settings.max_total_depth, c->max_bounces);
 
     StageTimer tm(c);
     tm.mark(); // 0
@@ -1145,6 +1147,9 @@ int rh_set_option(rh_context *c, const char *name, int value) {
         c->tail_occupancy = value;
     } else if (n == "refill") {
         c->refill = value != 0;
+    } else if (n == "max_bounces") {
+        if (value < 1 || value > RH_MAX_BOUNCES_EXT) return arg_error("max_bounces must be 1..16");
+        c->max_bounces = value;
     } else if (n == "trace_occupancy") {
         if (value != 0 && value != 5) return arg_error("trace_occupancy must be 0 or 5");
         c->trace_occupancy = value;
@@ -1168,6 +1173,7 @@ int rh_get_option(rh_context *c, const char *name, int *value) {
     else if (n == "tail_occupancy") *value = c->tail_occupancy;
     else if (n == "refill") *value = c->refill;
     else if (n == "trace_occupancy") *value = c->trace_occupancy;
+    else if (n == "max_bounces") *value = c->max_bounces;
     else if (n == "adjoint_occupancy") *value = c->adjoint_occupancy;
     else if (n == "defer_nee") *value = c->defer_nee;
     else if (n == "stack_overflows") {
@@ -1426,20 +1432,20 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
         return arg_error("region outside the framebuffer");
     if (p->iteration < 1) return arg_error("iteration must be >= 1");
     if (!d_adj_rgba || !d_grad) return arg_error("adjoint image and gradient buffers are required");
-    const int depth = std::min<int>(p->cam.pass_settings.max_total_depth, RH_MAX_BOUNCES);
+    const int depth = std::min<int>(p->cam.pass_settings.max_total_depth, c->max_bounces);
     const size_t lanes = (size_t)rw * rhh;
-    // vertex records: at most depth + 1 <= RH_MAX_BOUNCES + 1 per lane
+    // vertex records: at most depth + 1 <= RH_MAX_BOUNCES_EXT + 1 per lane
     if (lanes > c->adj_rec_cap || !c->adj_rec) {
         free_ptr(c->adj_rec);
         c->adj_rec = nullptr;
         c->adj_rec_cap = lanes;
-        RH_CHECK(hipMalloc(&c->adj_rec, (RH_MAX_BOUNCES + 1) * lanes * sizeof(float4)));
+        RH_CHECK(hipMalloc(&c->adj_rec, (RH_MAX_BOUNCES_EXT + 1) * lanes * sizeof(float4)));
     }
     if (d_texel_grad && (lanes > c->adj_rec_tex_cap || !c->adj_rec_tex)) {
         free_ptr(c->adj_rec_tex);
         c->adj_rec_tex = nullptr;
         c->adj_rec_tex_cap = lanes;
-        RH_CHECK(hipMalloc(&c->adj_rec_tex, 3 * (RH_MAX_BOUNCES + 1) * lanes * sizeof(float4)));
+        RH_CHECK(hipMalloc(&c->adj_rec_tex, 3 * (RH_MAX_BOUNCES_EXT + 1) * lanes * sizeof(float4)));
     }
     const bool defer = c->defer_nee && c->nee_slots;
     // NEE slots for every lane when deferred (this may grow the spill array,

@@ -196,6 +196,9 @@ inline int build_device_scene(const rh_scene_view *v, hipStream_t stream, std::v
     rc |= upload_array(bufs, stream, v->li_indices, v->li_indices_count, &s.li_indices);
     if (rc) return rc;
     if (v->env) s.env = *v->env;
+    s.all_solid = 1;
+    for (uint32_t i = 0; i < v->tris_count; i++)
+        if (!((uint32_t)v->tris[i].ci & RH_TRI_SOLID_BIT)) s.all_solid = 0;
     s.macro_root = v->macro_nodes_root;
     s.light_root = v->light_nodes_root;
     std::vector<rh_bvh_node> leaves;

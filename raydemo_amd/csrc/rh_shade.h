@@ -38,7 +38,8 @@ __device__ __forceinline__ float compute_visibility(const Scene &sc, f3 p1, f3 p
         sh.t = dist;
         sh.u = sh.v = 0.0f;
         cnt.shadow_rays++;
-        if (sc.macro_root != RH_INVALID) traverse<true, Tree>(sc, ro, dir, sh, st, nullptr);
+        // all-solid scene: the first blocking triangle found decides (visibility 0)
+        if (sc.macro_root != RH_INVALID) traverse<true, Tree>(sc, ro, dir, sh, st, nullptr, sc.all_solid != 0);
         if (!sh.mask) break;
 
         const rh_tri_accel tri = load_tri(sc.tris, (uint32_t)sh.prim);

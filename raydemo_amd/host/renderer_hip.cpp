@@ -103,7 +103,9 @@ bool Renderer::PreparePass(Scene &s, RegionContext &region, rh_pass_desc &pass) 
     pass.rect[3] = rect.h;
     pass.iteration = region.iteration;
     pass.sort_rays = sort_rays_ ? 1 : 0;
-    pass.cam.pass_settings.max_total_depth = std::min<uint8_t>(pass.cam.pass_settings.max_total_depth, RH_MAX_BOUNCES);
+    // the device clamps to its "max_bounces" option (the reference's 10 by default)
+    pass.cam.pass_settings.max_total_depth =
+        std::min<uint8_t>(pass.cam.pass_settings.max_total_depth, RH_MAX_BOUNCES_EXT);
     return true;
 }
 

@@ -14,6 +14,18 @@ Workloads (seed = 0x5EED0000 + k for workload k):
                      log-uniform 1k..500k, frames drawn from {64, 128, 256, 512,
                      1024}^2, 4..32 spp, depth 4..8, each rendered alone or all
                      together by the batch renderer.
+  large_instanced  — two-level scene (seed 0x5EED0005): 16 base meshes of
+                     jittered icosphere clusters, 1.2k..80k triangles each
+                     (log-uniform), 4096 instances with seeded affine transforms
+                     (uniform rotation, log-uniform scale, positions in a 120 x 20
+                     x 120 field; instances spread over the meshes with weight
+                     size^-2.5, ~7.5 M instanced triangles) over a textured ground
+                     quad; 32 RGBA8 sRGB textures of 2048^2 with mips (seeded
+                     checker + noise), each mesh diffuse-textured or a mix of
+                     textured diffuse and glossy (roughness U[0.1, 0.6]); four
+                     sphere lights and a constant environment; 3840x2160, 16 spp,
+                     depth 12 (beyond the reference's cap of 10: the renderer's
+                     "max_bounces" extension).
   cornell_refined  — the Cornell box with every wall and box face split into a
                      k x k grid (k = 17: 9,826 triangles + the 2-triangle light),
                      one diffuse material per triangle with albedo U[0.1, 0.9]
@@ -397,3 +409,119 @@ def textured_cornell(scene: "A.Scene", k: int = 8, tex_size: int = 64, seed: int
     scene.set_current_cam(cam)
     return SceneSpec("textured_cornell", width, height, spp, depth, triangles=len(idx) // 3,
                      info={"textures": tex_ids, "tex_size": tex_size})
+
+
+def _pattern_texture(rng, n: int) -> np.ndarray:
+    """n x n RGBA8: a two-colour checker of seeded period plus a tiled noise."""
+    base, alt = rng.uniform(0.15, 0.85, 3).astype(np.float32), rng.uniform(0.15, 0.85, 3).astype(np.float32)
+    period = int(rng.choice([8, 16, 32, 64, 128]))
+    k = np.arange(n) // period
+    chk = ((k[:, None] + k[None, :]) & 1).astype(bool)
+    img = np.where(chk[..., None], base, alt)
+    t = min(64, n)
+    tile = rng.normal(0.0, 0.06, (t, t, 3)).astype(np.float32)
+    img = img + np.tile(tile, (n // t, n // t, 1))
+    rgb = np.clip(np.round(img * 255.0), 0, 255).astype(np.uint8)
+    return np.concatenate([rgb, np.full((n, n, 1), 255, np.uint8)], axis=2)
+
+
+def _cluster_geometry(rng, target: int):
+    """Jittered subdivision-2 icospheres (320 triangles each) in a unit cube,
+    about `target` triangles: (attrs (n, 8), indices)."""
+    v, f = icosphere(2)
+    m = max(1, int(round(target / 320.0)))
+    centres = rng.uniform(-1.0, 1.0, (m, 3))
+    radii = np.exp(rng.uniform(math.log(0.05), math.log(0.35), m))
+    uv = np.stack([0.5 + np.arctan2(v[:, 2], v[:, 0]) / (2 * math.pi),
+                   0.5 - np.arcsin(np.clip(v[:, 1], -1, 1)) / math.pi], 1)
+    jit = 1.0 + 0.04 * rng.standard_normal((m, len(v)))
+    p = centres[:, None, :] + v[None, :, :] * (radii[:, None] * jit)[..., None]
+    n = np.broadcast_to(v, p.shape)
+    attrs = np.concatenate([p, n, np.broadcast_to(uv, p.shape[:2] + (2,))], axis=2).reshape(-1, 8)
+    idx = (f.reshape(1, -1) + (np.arange(m) * len(v))[:, None]).reshape(-1)
+    return attrs.astype(np.float32), idx.astype(np.uint32)
+
+
+def _random_rotation(rng) -> np.ndarray:
+    q = rng.standard_normal(4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def large_instanced(scene: "A.Scene", seed: int = SEED_BASE + 5, width=3840, height=2160, spp=16, depth=12,
+                    n_meshes=16, n_instances=4096, tri_range=(1200, 80000), n_textures=32, tex_size=2048,
+                    field_size=120.0) -> SceneSpec:
+    """The large two-level workload (see the module docstring); the keyword
+    arguments scale it down for tests.  info["textures"] are the texture ids,
+    info["adjoint_textures"] the 4 differentiated ones (the ground's and those
+    of the three most instanced meshes), info["adjoint_texels"] their level-0
+    RGB texels."""
+    rng = np.random.default_rng(seed)
+    white = _white(scene)
+    tex, texels = [], []
+    for _ in range(n_textures):
+        rgba = _pattern_texture(rng, tex_size)
+        tex.append(scene.add_texture(rgba, is_srgb=True, generate_mipmaps=True))
+        texels.append(rgba)
+    sizes = np.exp(rng.uniform(math.log(tri_range[0]), math.log(tri_range[1]), n_meshes))
+    meshes, mesh_tex = [], []
+    tris_per_mesh = []
+    for k in range(n_meshes):
+        attrs, idx = _cluster_geometry(rng, float(sizes[k]))
+        t0 = tex[(2 * k + 1) % n_textures]
+        t1 = tex[(2 * k + 2) % n_textures]
+        diff0 = scene.add_material(A.DiffuseMaterial, t0, main_color=(1.0, 1.0, 1.0))
+        if rng.random() < 0.5:
+            second = scene.add_material(A.DiffuseMaterial, t1, main_color=(1.0, 1.0, 1.0))
+        else:
+            d1 = scene.add_material(A.DiffuseMaterial, t1, main_color=(1.0, 1.0, 1.0))
+            gl = scene.add_material(A.GlossyMaterial, white, main_color=(0.9, 0.9, 0.9),
+                                    roughness=float(rng.uniform(0.1, 0.6)))
+            second = scene.add_material(A.MixMaterial, white, strength=0.5, mix_materials=(d1, gl), int_ior=1.5)
+        half = (len(idx) // 6) * 3
+        shapes = [(diff0, 0, half), (second, half, len(idx) - half)]
+        meshes.append(scene.add_mesh(A.PxyzNxyzTuv, attrs, idx, shapes))
+        mesh_tex.append((t0, t1))
+        tris_per_mesh.append(len(idx) // 3)
+    tris_per_mesh = np.array(tris_per_mesh, np.float64)
+    # instances per mesh: weight size^-2.5, at least one each
+    w = tris_per_mesh ** -2.5
+    w /= w.sum()
+    counts = np.maximum(1, np.floor(w * n_instances)).astype(int)
+    counts[int(np.argmin(tris_per_mesh))] += n_instances - counts.sum()
+    order = np.repeat(np.arange(n_meshes), counts)
+    rng.shuffle(order)
+    half_f = field_size / 2
+    for k in order:
+        x = np.eye(4, dtype=np.float64)
+        s_ = math.exp(rng.uniform(math.log(0.5), math.log(4.0)))
+        x[:3, :3] = (_random_rotation(rng) * s_).T  # row-vector convention: rows are the basis images
+        x[3, :3] = (rng.uniform(-half_f, half_f), rng.uniform(s_, s_ + 20.0), rng.uniform(-half_f, half_f))
+        scene.add_mesh_instance(meshes[int(k)], x.astype(np.float32))
+    # ground
+    g = field_size * 1.5
+    ga, gi = _quad((-g, 0.0, -g), (-g, 0.0, g), (g, 0.0, g), (g, 0.0, -g), (0, 1, 0))
+    ga = ga.astype(np.float32)
+    ga[:, 6:8] *= 24.0  # tile the ground texture
+    ground_mat = scene.add_material(A.DiffuseMaterial, tex[0], main_color=(1.0, 1.0, 1.0))
+    ground = scene.add_mesh(A.PxyzNxyzTuv, ga, gi, [(ground_mat, 0, 6)])
+    scene.add_mesh_instance(ground)
+    scene.set_environment((0.3, 0.35, 0.4))
+    for lx, lz in ((-0.3, -0.3), (0.3, -0.2), (-0.2, 0.3), (0.25, 0.3)):
+        scene.add_light(A.PointLight, position=(lx * field_size, 60.0, lz * field_size), radius=3.0,
+                        color=(900.0, 880.0, 850.0))
+    cam = scene.add_camera(origin=(0.0, 30.0, -0.85 * field_size), fwd=(0.0, -0.25, 1.0), fov=60.0, filter=A.Box,
+                           dtype=A.DeviceNone, max_diff_depth=depth, max_glossy_depth=depth, max_refr_depth=depth,
+                           max_transp_depth=depth, max_total_depth=depth, termination_start_depth=16)
+    scene.set_current_cam(cam)
+    most = [int(m) for m in np.argsort(-counts)[:3]]
+    adj_tex = [tex[0]] + [mesh_tex[m][0] for m in most]
+    instanced = int((counts * tris_per_mesh).sum())
+    adj_rgb = [texels[tex.index(t)][..., :3] for t in adj_tex]
+    return SceneSpec("large_instanced", width, height, spp, depth, triangles=instanced + 2,
+                     info={"textures": tex, "adjoint_textures": adj_tex, "adjoint_texels": adj_rgb,
+                           "instances": int(counts.sum()) + 1, "base_triangles": int(tris_per_mesh.sum()),
+                           "instanced_triangles": instanced})

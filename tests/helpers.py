@@ -150,10 +150,15 @@ class Oracle:
             ("orc_shade", [P, P, P, P, C.c_size_t, P, P, P, P, P]),
             ("orc_render", [P, P, C.c_int, C.c_int, P, C.c_int, P, P, P, P]),
             ("orc_adjoint", [P, P, C.c_int, C.c_int, P, C.c_int, P, C.c_double, P, P]),
+            ("orc_set_max_bounces", [C.c_int]),
         ]:
             f = getattr(L, name)
             f.restype = C.c_int
             f.argtypes = args
+
+    def set_max_bounces(self, n):
+        """max_total_depth clamp of render/adjoint (the reference's 10 by default)."""
+        assert self.lib.orc_set_max_bounces(int(n)) == 0
 
     def perms(self):
         out = np.zeros(1024, np.uint16)

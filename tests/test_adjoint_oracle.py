@@ -8,6 +8,7 @@ to the derivative (h = 1e-3, FP32 rendering)."""
 import ctypes as C
 
 import numpy as np
+import pytest
 
 from conftest import standalone_scene
 from raydemo_amd import capi as A, scenes
@@ -116,3 +117,20 @@ def test_oracle_texel_gradient_matches_finite_differences(oracle, host_lib):
             assert abs(fd - g[y, x, c]) <= 2e-4 * abs(g[y, x, c]) + 1e-6, (q, y, x, c, fd, g[y, x, c])
             checked += 1
     assert checked == 12
+
+
+def test_device_fold_matches_host_fold(host_lib):
+    """fold_texel_grad_torch (the fold the texture autograd function runs on
+    the device) against the float64 host fold, on random stored-texel
+    gradients of a mipmapped textured scene (borders and 7 mip levels)."""
+    torch = pytest.importorskip("torch")
+    from raydemo_amd.adjoint import fold_texel_grad, fold_texel_grad_torch
+    s = standalone_scene(host_lib)
+    spec = scenes.textured_cornell(s, tex_size=64)
+    v = s.view()
+    tg = np.random.default_rng(1).standard_normal((v["texels_count"], 4)).astype(np.float32)
+    for t in spec.info["textures"]:
+        a = fold_texel_grad(v, t, tg.astype(np.float64))[..., :3]
+        b = fold_texel_grad_torch(v, t, torch.tensor(tg)).numpy()
+        assert a.shape == b.shape == (64, 64, 3)
+        assert np.abs(a - b).max() <= 1e-5 * np.abs(a).max()

@@ -1,0 +1,133 @@
+"""The large two-level workload (scenes.large_instanced), scaled down: many
+instances with seeded affine transforms over textured meshes, depth 12 (the
+"max_bounces" extension past the reference's cap of 10).  The renderer against
+the oracle with the same cap (images, ray counts), texel gradients of the
+differentiated textures against the oracle's FP64 replay, and the scene
+batched together with small soups."""
+import numpy as np
+import pytest
+
+from conftest import standalone_scene
+from raydemo_amd import capi as A, scenes
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+W, H, SPP = 96, 54, 3
+SMALL = dict(width=W, height=H, spp=SPP, n_meshes=6, n_instances=200, tri_range=(300, 3000), n_textures=8,
+             tex_size=64, field_size=40.0)
+
+
+def _build(s):
+    return scenes.large_instanced(s, **SMALL)
+
+
+def _rmse(a, b):
+    return float(np.sqrt(np.mean((a[..., :3].astype(np.float64) - b[..., :3]) ** 2)))
+
+
+@pytest.fixture
+def deep_oracle(oracle):
+    oracle.set_max_bounces(12)
+    yield oracle
+    oracle.set_max_bounces(10)
+
+
+@pytest.mark.parametrize("wide", [True, False])
+def test_large_render_matches_oracle(host_lib, deep_oracle, wide):
+    r = A.Renderer(host_lib, W, H, A.RENDERER_HIP, use_wide_bvh=wide)
+    r.set_option("max_bounces", 12)
+    s = r.create_scene()
+    spec = _build(s)
+    assert spec.depth == 12 and spec.info["instances"] == 201
+    reg = A.Region(host_lib, 0, 0, W, H)
+    for _ in range(SPP):
+        r.render(s, reg)
+    img = r.pixels()
+    hs = standalone_scene(host_lib, wide=wide)
+    _build(hs)
+    ref, counts = deep_oracle.render(hs.raw_view(), hs.camera_record(0), W, H, SPP)
+    rc = r.ray_counts()
+    assert (rc["primary"], rc["secondary"], rc["shadow"]) == tuple(int(c) for c in counts)
+    e = _rmse(img, ref)
+    print(f"large (wide={wide}): RMSE vs oracle {e:.3e}, rays {rc}")
+    assert e <= 1e-3
+
+
+def test_depth_extension_adds_light(host_lib, deep_oracle):
+    """A closed box with roulette off: paths past bounce 10 only add light, and
+    the device renders the extended depth as the oracle does."""
+    w, h, spp, depth = 32, 24, 2, 12
+    imgs = {}
+    for cap in (10, 12):
+        r = A.Renderer(host_lib, w, h, A.RENDERER_HIP, use_wide_bvh=True)
+        r.set_option("max_bounces", cap)
+        s = r.create_scene()
+        scenes.cornell_refined(s, k=2, width=w, height=h, spp=spp, depth=depth)
+        reg = A.Region(host_lib, 0, 0, w, h)
+        for _ in range(spp):
+            r.render(s, reg)
+        acc = np.zeros((h, w, 4), np.float32)
+        r.read_accum(acc.ctypes.data)
+        imgs[cap] = acc
+    assert np.all(imgs[12][..., :3] >= imgs[10][..., :3] - 1e-6)
+    assert imgs[12][..., :3].sum() > imgs[10][..., :3].sum()
+    hs = standalone_scene(host_lib, wide=True)
+    scenes.cornell_refined(hs, k=2, width=w, height=h, spp=spp, depth=depth)
+    ref = deep_oracle.render_linear(hs.raw_view(), hs.camera_record(0), w, h, spp)
+    assert imgs[12].tobytes() == ref.tobytes()
+
+
+def test_large_texel_gradients_match_oracle(host_lib, deep_oracle):
+    from raydemo_amd import adjoint
+    r = A.Renderer(host_lib, W, H, A.RENDERER_HIP, use_wide_bvh=True)
+    r.set_option("max_bounces", 12)
+    s = r.create_scene()
+    spec = _build(s)
+    dr = adjoint.DiffRenderer(host_lib, r, s, W, H, SPP, len(s.view()["materials"]))
+    wts = np.random.default_rng(4).uniform(-1, 1, size=(H, W, 4)).astype(np.float32)
+    wts[..., 3] = 0
+    dr.render()
+    _, tg = dr.adjoint(torch.tensor(wts, device="cuda"), texels=True)
+    layout = dr.tex_layout()
+    hs = standalone_scene(host_lib, wide=True)
+    _build(hs)
+    _, ref_tg = deep_oracle.adjoint(hs.raw_view(), hs.camera_record(0), W, H, SPP, wts, texels=True)
+    v = hs.view()
+    for t in spec.info["adjoint_textures"]:
+        g = adjoint.fold_texel_grad_torch(layout, t, tg).cpu().numpy().astype(np.float64)
+        ref = adjoint.fold_texel_grad(v, t, ref_tg)[..., :3]
+        scale = np.abs(ref).max()
+        assert scale > 0, t
+        err = np.abs(g - ref).max()
+        print(f"texture {t}: max |grad| {scale:.3e}, max err {err:.3e}")
+        assert err <= 1e-4 * scale
+
+
+def test_large_scene_batched_with_small_scenes(host_lib):
+    cases = [(_build, W, H, SPP)] + [
+        (lambda s, it=it: scenes.batch_scene(s, it), 64, 64, it.spp)
+        for it in scenes.batch_workload(3, seed=scenes.SEED_BASE + 77)]
+    cases = [(b, w, h, min(spp, 4)) for b, w, h, spp in cases]
+    singles = []
+    for build, w, h, spp in cases:
+        r = A.Renderer(host_lib, w, h, A.RENDERER_HIP, use_wide_bvh=True)
+        r.set_option("max_bounces", 12)
+        s = r.create_scene()
+        build(s)
+        reg = A.Region(host_lib, 0, 0, w, h)
+        for _ in range(spp):
+            r.render(s, reg)
+        singles.append(r.pixels())
+    batch = A.Batch(host_lib)
+    batch.set_option("max_bounces", 12)
+    keep = []
+    for build, w, h, spp in cases:
+        s = standalone_scene(host_lib, wide=True)
+        build(s)
+        keep.append(s)
+        batch.add(s, w, h, spp)
+    batch.render_all()
+    for i in range(len(cases)):
+        assert batch.pixels(i).tobytes() == singles[i].tobytes(), i
