@@ -71,7 +71,9 @@ int rh_get_ray_counts(rh_context *ctx, rh_ray_counts *out);
  * "trace_occupancy" (0 or 5: the 8-wide trace and shadow kernels as compiled, or capped at 5 waves per SIMD),
  * "max_bounces" (1..16: the clamp of max_total_depth; 10, the reference's MAX_BOUNCES, by default:
  *   larger values render deeper paths than the reference can, an extension),
- * "adjoint_occupancy" (the same for the adjoint replay kernel),
+ * "adjoint_staged" (1, the default: rh_adjoint_pass replays the pass as staged wavefront passes whose
+ *   stages write the vertex records, then one kernel forms the gradient; 0: one lane per path),
+ * "adjoint_occupancy" (2 or 3 waves per SIMD for the one-lane-per-path adjoint kernel),
  * "defer_nee" (0/1: shadow rays of next-event estimation as their own stage). */
 int rh_set_option(rh_context *ctx, const char *name, int value);
 int rh_get_option(rh_context *ctx, const char *name, int *value);

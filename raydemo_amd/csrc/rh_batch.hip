@@ -290,6 +290,7 @@ __global__ void __launch_bounds__(kBlock) k_b_combine(BatchDev b, int stage, int
                       // not read while temp is set; a real address keeps the
                       // per-ray form's address arithmetic off a null pointer
                       a.out_rgba = a.temp;
+                      a.rec = AdjRecords{};
                       combine_body(a, base);
                   });
 }

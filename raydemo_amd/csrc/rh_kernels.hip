@@ -60,8 +60,8 @@ template <bool Defer> __global__ void __launch_bounds__(kBlock) k_shade(ShadeArg
 
 // the bounces first..last-1 of each queued ray in one lane (path_tail_body).
 // MinBlocks 3 asks the compiler for 3 resident blocks (waves per SIMD) per CU,
-// trading register spills for occupancy; 1 lets it use all the registers it
-// wants (2 waves per SIMD).  Which is faster is measured (tail_occupancy).
+// trading register spills for occupancy; 2 gives it 256 registers (2 waves per
+// SIMD).  Which is faster is measured (tail_occupancy).
 template <bool Defer, int MinBlocks, int Tree>
 __global__ void __launch_bounds__(kBlock, MinBlocks) k_path_tail(ShadeArgs a, int first, int last) {
     extern __shared__ uint32_t lds_stack[];
@@ -79,6 +79,7 @@ struct AdjointArgs {
     float scale;        // weight of this pass in the image (1 / passes for the mean)
     float *grad;        // 4 floats per material: d loss / d main_color (rgb, pad)
     float *tex_grad;    // optional: 4 floats per stored texel, d loss / d texel value (rgb, a unused)
+    uint32_t n_materials;
     float4 *rec;        // per-lane vertex records, max_vertices x lanes, lane-minor
     float4 *rec_tex;    // with tex_grad: 3 records per vertex (texels, weights + raw value, raw + sRGB flag)
     uint32_t lanes;
@@ -95,6 +96,63 @@ struct AdjointArgs {
     uint32_t nee_slots;
 };
 
+constexpr uint32_t kLdsGradMaterials = 1024;
+
+// The backward half of the replay for lane i (pixel adjoint dl) over its nv
+// vertex records: S_k = sum_{j >= k} C_j, added as adj . S_k / main_color to the
+// material of vertex k and, with texel gradients, to its texel footprint.
+__device__ __forceinline__ void adjoint_accumulate(const AdjointArgs &a, uint32_t i, int nv, float4 dl, bool lds_grad,
+                                                   float *s_grad) {
+    f3 S = f3{0.0f, 0.0f, 0.0f};
+    for (int k = nv - 1; k >= 0; k--) {
+        const float4 r = a.rec[(size_t)k * a.lanes + i];
+        S = S + f3{r.x, r.y, r.z};
+        const int m = __float_as_int(r.w);
+        if (m < 0) continue;
+        const float *mc = a.sc.materials[m].main_color;
+        const float g[3] = {dl.x * S.x, dl.y * S.y, dl.z * S.z};
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) {
+            if (mc[ch] == 0.0f || g[ch] == 0.0f) continue;
+            const float v = a.scale * (g[ch] / mc[ch]);
+            if (lds_grad) atomicAdd(&s_grad[4 * m + ch], v);
+            else atomicAdd(&a.grad[4 * m + ch], v);
+        }
+        if (a.tex_grad) {
+            // albedo = lin(raw) * main_color with raw the bilinear sum; S is
+            // linear in the albedo, so d pixel / d raw = S / lin(raw) * lin'(raw)
+            const float4 *rt = a.rec_tex + (size_t)k * 3 * a.lanes + i;
+            const float4 t0 = rt[0], t1 = rt[a.lanes], t2 = rt[2 * (size_t)a.lanes];
+            const float raw[3] = {t1.z, t1.w, t2.x};
+            const bool srgb = t2.y != 0.0f;
+            const float kx = t1.x, ky = t1.y;
+            const float w[4] = {(1.0f - kx) * (1.0f - ky), kx * (1.0f - ky), (1.0f - kx) * ky, kx * ky};
+            const uint32_t idx[4] = {__float_as_uint(t0.x), __float_as_uint(t0.y), __float_as_uint(t0.z),
+                                     __float_as_uint(t0.w)};
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++) {
+                float lin, dlin;
+                if (!srgb) {
+                    lin = raw[ch];
+                    dlin = 1.0f;
+                } else if (raw[ch] > 0.04045f) {
+                    const float b = (raw[ch] + 0.055f) / 1.055f;
+                    lin = rpow(b, 2.4f);
+                    dlin = 2.4f / 1.055f * rpow(b, 1.4f);
+                } else {
+                    lin = raw[ch] / 12.92f;
+                    dlin = 1.0f / 12.92f;
+                }
+                if (lin == 0.0f || g[ch] == 0.0f) continue;
+                const float gr = a.scale * (g[ch] / lin) * dlin;
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    if (w[q] != 0.0f) atomicAdd(&a.tex_grad[4 * (size_t)idx[q] + ch], gr * w[q]);
+            }
+        }
+    }
+}
+
 // Path replay for one pass: each lane regenerates its pixel's primary ray with
 // the pass's iteration and Halton table, follows the same path as the forward
 // pass (trace, ShadeSurface with visibility traced inline, next ray) and
@@ -105,10 +163,18 @@ struct AdjointArgs {
 //     d(pixel) / d main_color[c] = S_m[c] / main_color[c].
 // The suffix sums are formed backwards over the records; the gradient adds
 // adj[pixel] * scale * S_m / main_color with float atomics (their order is the
-// only run-to-run difference).
+// only run-to-run difference).  A scene of at most kLdsGradMaterials materials
+// sums the block's material gradients in LDS first: with few materials every
+// lane would otherwise add to the same few addresses.
 template <bool Defer, int MinBlocks, int Tree>
 __global__ void __launch_bounds__(kBlock, MinBlocks) k_adjoint(AdjointArgs a) {
     extern __shared__ uint32_t lds_stack[];
+    __shared__ float s_grad[4 * kLdsGradMaterials];
+    const bool lds_grad = a.n_materials <= kLdsGradMaterials;
+    if (lds_grad) {
+        for (uint32_t j = threadIdx.x; j < 4 * a.n_materials; j += blockDim.x) s_grad[j] = 0.0f;
+        __syncthreads();
+    }
     const int rw = a.pc.rect[2], rh_ = a.pc.rect[3];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = (int)i < rw * rh_;
@@ -149,69 +215,47 @@ __global__ void __launch_bounds__(kBlock, MinBlocks) k_adjoint(AdjointArgs a) {
                 f3 cj = f3{so.col.x, so.col.y, so.col.z};
                 if (Defer && so.nee_pending && sink.count > 0)
                     cj = cj + resolve_light_samples<Tree>(a.sc, a.halton, sink, so, ray.c, st, cnt);
-                a.rec[(size_t)nv * a.lanes + i] = make_float4(cj.x, cj.y, cj.z, __int_as_float(so.alb_mat));
-                if (a.tex_grad && so.alb_mat >= 0) {
-                    float4 *rt = a.rec_tex + (size_t)nv * 3 * a.lanes + i;
-                    rt[0] = make_float4(__uint_as_float(so.alb_foot.idx[0]), __uint_as_float(so.alb_foot.idx[1]),
-                                        __uint_as_float(so.alb_foot.idx[2]), __uint_as_float(so.alb_foot.idx[3]));
-                    rt[a.lanes] = make_float4(so.alb_foot.kx, so.alb_foot.ky, so.alb_raw.x, so.alb_raw.y);
-                    rt[2 * (size_t)a.lanes] = make_float4(so.alb_raw.z, so.alb_srgb ? 1.0f : 0.0f, 0.0f, 0.0f);
-                }
+                write_vertex_record(AdjRecords{a.rec, a.tex_grad ? a.rec_tex : nullptr, nullptr, a.lanes, 0, {0, 0, 0}},
+                                    i, nv, cj, so);
                 nv++;
                 b++;
                 if (!so.emit || skip_indirect || b >= last) break;
                 ray = so.next;
             }
-            f3 S = f3{0.0f, 0.0f, 0.0f};
-            for (int k = nv - 1; k >= 0; k--) {
-                const float4 r = a.rec[(size_t)k * a.lanes + i];
-                S = S + f3{r.x, r.y, r.z};
-                const int m = __float_as_int(r.w);
-                if (m < 0) continue;
-                const float *mc = a.sc.materials[m].main_color;
-                const float g[3] = {dl.x * S.x, dl.y * S.y, dl.z * S.z};
-#pragma unroll
-                for (int ch = 0; ch < 3; ch++) {
-                    if (mc[ch] != 0.0f && g[ch] != 0.0f) atomicAdd(&a.grad[4 * m + ch], a.scale * (g[ch] / mc[ch]));
-                }
-                if (a.tex_grad) {
-                    // albedo = lin(raw) * main_color with raw the bilinear sum; S is
-                    // linear in the albedo, so d pixel / d raw = S / lin(raw) * lin'(raw)
-                    const float4 *rt = a.rec_tex + (size_t)k * 3 * a.lanes + i;
-                    const float4 t0 = rt[0], t1 = rt[a.lanes], t2 = rt[2 * (size_t)a.lanes];
-                    const float raw[3] = {t1.z, t1.w, t2.x};
-                    const bool srgb = t2.y != 0.0f;
-                    const float kx = t1.x, ky = t1.y;
-                    const float w[4] = {(1.0f - kx) * (1.0f - ky), kx * (1.0f - ky), (1.0f - kx) * ky, kx * ky};
-                    const uint32_t idx[4] = {__float_as_uint(t0.x), __float_as_uint(t0.y), __float_as_uint(t0.z),
-                                             __float_as_uint(t0.w)};
-#pragma unroll
-                    for (int ch = 0; ch < 3; ch++) {
-                        float lin, dlin;
-                        if (!srgb) {
-                            lin = raw[ch];
-                            dlin = 1.0f;
-                        } else if (raw[ch] > 0.04045f) {
-                            const float b = (raw[ch] + 0.055f) / 1.055f;
-                            lin = rpow(b, 2.4f);
-                            dlin = 2.4f / 1.055f * rpow(b, 1.4f);
-                        } else {
-                            lin = raw[ch] / 12.92f;
-                            dlin = 1.0f / 12.92f;
-                        }
-                        if (lin == 0.0f || g[ch] == 0.0f) continue;
-                        const float gr = a.scale * (g[ch] / lin) * dlin;
-#pragma unroll
-                        for (int q = 0; q < 4; q++)
-                            if (w[q] != 0.0f) atomicAdd(&a.tex_grad[4 * (size_t)idx[q] + ch], gr * w[q]);
-                    }
-                }
-            }
+            adjoint_accumulate(a, i, nv, dl, lds_grad, s_grad);
         }
+    }
+    if (lds_grad) {
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < 4 * a.n_materials; j += blockDim.x)
+            if (s_grad[j] != 0.0f) atomicAdd(&a.grad[j], s_grad[j]);
     }
     if (a.ray_counter) wave_add_u64(a.ray_counter, traced);
     if (a.shadow_counter) wave_add_u64(a.shadow_counter, cnt.shadow_rays);
     if (st.overflow && a.overflow) atomicAdd(a.overflow, (unsigned long long)st.overflow);
+}
+
+// The backward half alone, after a staged replay (the forward stage kernels
+// wrote the vertex records and per-lane vertex counts).
+__global__ void __launch_bounds__(kBlock) k_adjoint_backward(AdjointArgs a, const int *nverts) {
+    __shared__ float s_grad[4 * kLdsGradMaterials];
+    const bool lds_grad = a.n_materials <= kLdsGradMaterials;
+    if (lds_grad) {
+        for (uint32_t j = threadIdx.x; j < 4 * a.n_materials; j += blockDim.x) s_grad[j] = 0.0f;
+        __syncthreads();
+    }
+    const int rw = a.pc.rect[2], rh_ = a.pc.rect[3];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int)i < rw * rh_) {
+        const int x = a.pc.rect[0] + (int)i % rw, y = a.pc.rect[1] + (int)i / rw;
+        const float4 dl = a.adj[y * a.pc.w + x];
+        if (dl.x != 0.0f || dl.y != 0.0f || dl.z != 0.0f) adjoint_accumulate(a, i, nverts[i], dl, lds_grad, s_grad);
+    }
+    if (lds_grad) {
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < 4 * a.n_materials; j += blockDim.x)
+            if (s_grad[j] != 0.0f) atomicAdd(&a.grad[j], s_grad[j]);
+    }
 }
 
 // main_color of materials [first, first + n) from packed rgb triples
@@ -472,6 +516,9 @@ struct rh_context {
     int refill = 0;            // closest-hit trace: persistent kernel with lane refill
     int trace_occupancy = 0;   // 8-wide trace/shadow kernels: 0 = as compiled (4 waves/SIMD), 5 = capped registers
     int max_bounces = RH_MAX_BOUNCES; // max_total_depth clamp (the reference's 10; up to RH_MAX_BOUNCES_EXT)
+    int adjoint_staged = 1;           // adjoint replay as staged wavefront passes (0: one lane per path)
+    int *adj_nverts = nullptr;        // staged replay: vertices recorded per lane
+    size_t adj_nverts_cap = 0;
     int cus = 0;               // compute units of the device
     int refill_blocks = 0, refill_blocks_lds = -1;
     int adjoint_occupancy = 2; // the same for the adjoint replay kernel
@@ -740,7 +787,7 @@ int launch_shade(rh_context *c, const ShadeArgs &a, int grid_rays) {
     s.spill = c->spill;
     s.spill_stride = (uint32_t)c->spill_cap;
     s.lds_depth = (uint32_t)c->lds_depth;
-    s.shadow_counter = c->stats_dev + 2;
+    s.shadow_counter = a.shadow_counter;
     s.overflow = c->stats_dev + 4;
     const size_t lanes = (size_t)grid_rays * a.nee_slots;
     {
@@ -763,6 +810,7 @@ int launch_shade(rh_context *c, const ShadeArgs &a, int grid_rays) {
     cb.nee_vis = c->nee_vis;
     cb.temp = a.temp;
     cb.out_rgba = a.out_rgba;
+    cb.rec = a.rec;
     hipLaunchKernelGGL(k_nee_combine, dim3(blocks), dim3(kBlock), 0, c->stream, cb);
     RH_CHECK(hipGetLastError());
     return 0;
@@ -887,6 +935,7 @@ void rh_destroy(rh_context *c) {
     free_ptr(c->visits);
     free_ptr(c->adj_rec);
     free_ptr(c->adj_rec_tex);
+    free_ptr(c->adj_nverts);
     free_ptr(c->color_stage);
     for (auto &e : c->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
@@ -943,7 +992,11 @@ int rh_set_halton(rh_context *c, const float *seq, size_t count) {
     return 0;
 }
 
-int rh_render_pass(rh_context *c, const rh_pass_desc *p) {
+// One pass in wavefront form.  rec == nullptr: RenderScene (pixels, path tail,
+// mix and tonemap).  Otherwise the staged adjoint replay of the same pass: the
+// stages write vertex records instead of pixels, every bounce is staged, and
+// rays are counted as adjoint rays.
+int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec) {
     if (!c->have_scene) return arg_error("no scene uploaded");
     const int rx = p->rect[0], ry = p->rect[1], rw = p->rect[2], rhh = p->rect[3];
     if (rw <= 0 || rhh <= 0 || rx < 0 || ry < 0 || rx + rw > c->w || ry + rhh > c->h)
@@ -968,7 +1021,9 @@ int rh_render_pass(rh_context *c, const rh_pass_desc *p) {
                        c->rays[0], c->aux[0], cnt[0]);
     RH_CHECK(hipGetLastError());
     tm.mark(); // 1
-    rc = launch_trace(c, trace_args(c, c->rays[0], cnt[0], n, c->hits, nullptr, c->stats_dev + 0), n, false);
+    unsigned long long *count_primary = rec ? c->stats_dev + 5 : c->stats_dev + 0;
+    unsigned long long *count_secondary = rec ? c->stats_dev + 5 : c->stats_dev + 1;
+    rc = launch_trace(c, trace_args(c, c->rays[0], cnt[0], n, c->hits, nullptr, count_primary), n, false);
     if (rc) return rc;
     tm.mark(); // 2
 
@@ -982,8 +1037,13 @@ int rh_render_pass(rh_context *c, const rh_pass_desc *p) {
     sa.pi = pi;
     sa.w = c->w;
     sa.halton = c->halton;
-    sa.temp = c->temp;
+    sa.temp = rec ? nullptr : c->temp;
     sa.primary = 1;
+    if (rec) {
+        sa.rec = *rec;
+        sa.rec.vertex = 0;
+        sa.shadow_counter = c->stats_dev + 6;
+    }
     sa.out_rays = c->rays[1];
     sa.out_aux = c->aux[1];
     sa.out_count = cnt[1];
@@ -996,7 +1056,7 @@ int rh_render_pass(rh_context *c, const rh_pass_desc *p) {
     const bool skip_indirect = (pi.settings.flags & RH_SKIP_INDIRECT_LIGHT) != 0;
     const int sort_on = c->sort_rays && p->sort_rays;
     for (int bounce = 0; bounce < pi.settings.max_total_depth && !skip_indirect; bounce++) {
-        if (c->path_tail >= 0 && bounce >= c->path_tail) {
+        if (!rec && c->path_tail >= 0 && bounce >= c->path_tail) {
             // the remaining bounces, one lane per path
             ShadeArgs tb = sa;
             tb.rays = c->rays[q];
@@ -1015,20 +1075,20 @@ int rh_render_pass(rh_context *c, const rh_pass_desc *p) {
                 if (tb.nee_ray && occ3)
                     hipLaunchKernelGGL((k_path_tail<true, 3, kTreeWide>), grid, blk, lds, c->stream, tb, bounce, last);
                 else if (tb.nee_ray)
-                    hipLaunchKernelGGL((k_path_tail<true, 1, kTreeWide>), grid, blk, lds, c->stream, tb, bounce, last);
+                    hipLaunchKernelGGL((k_path_tail<true, 2, kTreeWide>), grid, blk, lds, c->stream, tb, bounce, last);
                 else if (occ3)
                     hipLaunchKernelGGL((k_path_tail<false, 3, kTreeWide>), grid, blk, lds, c->stream, tb, bounce, last);
                 else
-                    hipLaunchKernelGGL((k_path_tail<false, 1, kTreeWide>), grid, blk, lds, c->stream, tb, bounce, last);
+                    hipLaunchKernelGGL((k_path_tail<false, 2, kTreeWide>), grid, blk, lds, c->stream, tb, bounce, last);
             } else {
                 if (tb.nee_ray && occ3)
                     hipLaunchKernelGGL((k_path_tail<true, 3, kTreeBinary>), grid, blk, lds, c->stream, tb, bounce, last);
                 else if (tb.nee_ray)
-                    hipLaunchKernelGGL((k_path_tail<true, 1, kTreeBinary>), grid, blk, lds, c->stream, tb, bounce, last);
+                    hipLaunchKernelGGL((k_path_tail<true, 2, kTreeBinary>), grid, blk, lds, c->stream, tb, bounce, last);
                 else if (occ3)
                     hipLaunchKernelGGL((k_path_tail<false, 3, kTreeBinary>), grid, blk, lds, c->stream, tb, bounce, last);
                 else
-                    hipLaunchKernelGGL((k_path_tail<false, 1, kTreeBinary>), grid, blk, lds, c->stream, tb, bounce, last);
+                    hipLaunchKernelGGL((k_path_tail<false, 2, kTreeBinary>), grid, blk, lds, c->stream, tb, bounce, last);
             }
             RH_CHECK(hipGetLastError());
             break;
@@ -1049,7 +1109,7 @@ int rh_render_pass(rh_context *c, const rh_pass_desc *p) {
             q ^= 1;
         }
         bt.mark();
-        rc = launch_trace(c, trace_args(c, c->rays[q], cnt[q], n, c->hits, nullptr, c->stats_dev + 1), n, false);
+        rc = launch_trace(c, trace_args(c, c->rays[q], cnt[q], n, c->hits, nullptr, count_secondary), n, false);
         if (rc) return rc;
         bt.mark();
         hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, cnt[q ^ 1], 0);
@@ -1059,6 +1119,7 @@ int rh_render_pass(rh_context *c, const rh_pass_desc *p) {
         sb.count = cnt[q];
         sb.pi.bounce = bounce + 3;
         sb.primary = 0;
+        sb.rec.vertex = bounce + 1;
         sb.out_rays = c->rays[q ^ 1];
         sb.out_aux = c->aux[q ^ 1];
         sb.out_count = cnt[q ^ 1];
@@ -1074,6 +1135,7 @@ int rh_render_pass(rh_context *c, const rh_pass_desc *p) {
         q ^= 1;
     }
 
+    if (rec) return 0; // the replay leaves the image alone
     hipLaunchKernelGGL(k_mix_tonemap, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->temp, c->clean, c->final_, rx,
                        ry, rw, rhh, c->w, pc.mix_k, p->cam.dtype, p->cam.gamma, p->cam.pass_settings.flags);
     RH_CHECK(hipGetLastError());
@@ -1088,6 +1150,8 @@ int rh_render_pass(rh_context *c, const rh_pass_desc *p) {
     }
     return 0;
 }
+
+int rh_render_pass(rh_context *c, const rh_pass_desc *p) { return run_pass(c, p, nullptr); }
 
 int rh_read_pixels(rh_context *c, float *rgba, size_t count) {
     const size_t n = (size_t)c->w * c->h * 4;
@@ -1147,6 +1211,8 @@ int rh_set_option(rh_context *c, const char *name, int value) {
         c->tail_occupancy = value;
     } else if (n == "refill") {
         c->refill = value != 0;
+    } else if (n == "adjoint_staged") {
+        c->adjoint_staged = value != 0;
     } else if (n == "max_bounces") {
         if (value < 1 || value > RH_MAX_BOUNCES_EXT) return arg_error("max_bounces must be 1..16");
         c->max_bounces = value;
@@ -1174,6 +1240,7 @@ int rh_get_option(rh_context *c, const char *name, int *value) {
     else if (n == "refill") *value = c->refill;
     else if (n == "trace_occupancy") *value = c->trace_occupancy;
     else if (n == "max_bounces") *value = c->max_bounces;
+    else if (n == "adjoint_staged") *value = c->adjoint_staged;
     else if (n == "adjoint_occupancy") *value = c->adjoint_occupancy;
     else if (n == "defer_nee") *value = c->defer_nee;
     else if (n == "stack_overflows") {
@@ -1466,6 +1533,7 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
     a.adj = reinterpret_cast<const float4 *>(d_adj_rgba);
     a.scale = scale;
     a.grad = d_grad;
+    a.n_materials = c->n_materials;
     a.rec = c->adj_rec;
     a.lanes = (uint32_t)lanes;
     a.tex_grad = d_texel_grad;
@@ -1482,20 +1550,41 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
         a.nee_term = c->nee_term;
         a.nee_slots = c->nee_slots;
     }
+    if (c->adjoint_staged) {
+        // the forward stage kernels replay the pass and write the vertex
+        // records (every bounce staged, as many rays in flight as the pass has);
+        // one kernel then forms the suffix sums
+        if (lanes > c->adj_nverts_cap || !c->adj_nverts) {
+            free_ptr(c->adj_nverts);
+            c->adj_nverts = nullptr;
+            c->adj_nverts_cap = lanes;
+            RH_CHECK(hipMalloc(&c->adj_nverts, lanes * sizeof(int)));
+        }
+        RH_CHECK(hipMemsetAsync(c->adj_nverts, 0, lanes * sizeof(int), c->stream));
+        AdjRecords rec{c->adj_rec, d_texel_grad ? c->adj_rec_tex : nullptr, c->adj_nverts, (uint32_t)lanes, 0,
+                       {rx, ry, rw}};
+        rc = run_pass(c, p, &rec);
+        if (rc) return rc;
+        KernelMark km(c, 4);
+        hipLaunchKernelGGL(k_adjoint_backward, dim3((unsigned)((lanes + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           c->stream, a, (const int *)c->adj_nverts);
+        RH_CHECK(hipGetLastError());
+        return 0;
+    }
     KernelMark km(c, 4);
     const dim3 grid((unsigned)((lanes + kBlock - 1) / kBlock)), blk(kBlock);
     const size_t lds = lds_bytes(c);
     const bool occ3 = c->adjoint_occupancy == 3;
     if (c->sc.wnodes) {
         if (defer && occ3) hipLaunchKernelGGL((k_adjoint<true, 3, kTreeWide>), grid, blk, lds, c->stream, a);
-        else if (defer) hipLaunchKernelGGL((k_adjoint<true, 1, kTreeWide>), grid, blk, lds, c->stream, a);
+        else if (defer) hipLaunchKernelGGL((k_adjoint<true, 2, kTreeWide>), grid, blk, lds, c->stream, a);
         else if (occ3) hipLaunchKernelGGL((k_adjoint<false, 3, kTreeWide>), grid, blk, lds, c->stream, a);
-        else hipLaunchKernelGGL((k_adjoint<false, 1, kTreeWide>), grid, blk, lds, c->stream, a);
+        else hipLaunchKernelGGL((k_adjoint<false, 2, kTreeWide>), grid, blk, lds, c->stream, a);
     } else {
         if (defer && occ3) hipLaunchKernelGGL((k_adjoint<true, 3, kTreeBinary>), grid, blk, lds, c->stream, a);
-        else if (defer) hipLaunchKernelGGL((k_adjoint<true, 1, kTreeBinary>), grid, blk, lds, c->stream, a);
+        else if (defer) hipLaunchKernelGGL((k_adjoint<true, 2, kTreeBinary>), grid, blk, lds, c->stream, a);
         else if (occ3) hipLaunchKernelGGL((k_adjoint<false, 3, kTreeBinary>), grid, blk, lds, c->stream, a);
-        else hipLaunchKernelGGL((k_adjoint<false, 1, kTreeBinary>), grid, blk, lds, c->stream, a);
+        else hipLaunchKernelGGL((k_adjoint<false, 2, kTreeBinary>), grid, blk, lds, c->stream, a);
     }
     RH_CHECK(hipGetLastError());
     return 0;

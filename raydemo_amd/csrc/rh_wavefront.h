@@ -438,6 +438,32 @@ __device__ __forceinline__ void trace_refill_body(const TraceArgs &a, int *next_
     }
 }
 
+// Per-lane vertex records of the path-replay adjoint (lane-minor: record k of
+// lane i at k * lanes + i): rec = (C_k, material whose main_color scales it);
+// rec_tex (when texel gradients are taken) = 3 records per vertex: the
+// footprint's 4 texel indices, (kx, ky, raw r, raw g), (raw b, sRGB flag);
+// nverts = vertices recorded per lane.
+struct AdjRecords {
+    float4 *rec;
+    float4 *rec_tex;
+    int *nverts;
+    uint32_t lanes;
+    int vertex;
+    int rect[3]; // x, y, w of the pass rectangle
+};
+
+__device__ __forceinline__ void write_vertex_record(const AdjRecords &r, uint32_t lane, int k, f3 cj,
+                                                    const ShadeOut &so) {
+    r.rec[(size_t)k * r.lanes + lane] = make_float4(cj.x, cj.y, cj.z, __int_as_float(so.alb_mat));
+    if (r.rec_tex && so.alb_mat >= 0) {
+        float4 *rt = r.rec_tex + (size_t)k * 3 * r.lanes + lane;
+        rt[0] = make_float4(__uint_as_float(so.alb_foot.idx[0]), __uint_as_float(so.alb_foot.idx[1]),
+                            __uint_as_float(so.alb_foot.idx[2]), __uint_as_float(so.alb_foot.idx[3]));
+        rt[r.lanes] = make_float4(so.alb_foot.kx, so.alb_foot.ky, so.alb_raw.x, so.alb_raw.y);
+        rt[2 * (size_t)r.lanes] = make_float4(so.alb_raw.z, so.alb_srgb ? 1.0f : 0.0f, 0.0f, 0.0f);
+    }
+}
+
 struct ShadeArgs {
     Scene sc;
     const rh_ray *rays;
@@ -472,6 +498,10 @@ struct ShadeArgs {
     int *pend_count;    // deferred terms of the ray (0: nothing pending)
     uint32_t *shadow_queue; // compacted slot indices of the recorded light samples
     int *shadow_count;      // entries appended to shadow_queue
+    // Vertex records of a staged adjoint replay (null: write pixels).  A ray
+    // of pixel (x, y) is lane (y - rec_rect[1]) * rec_rect[2] + (x - rec_rect[0])
+    // of the pass rectangle; this stage's vertex index is rec_vertex.
+    AdjRecords rec;
 };
 
 // Defer: direct light goes to the NEE stage (no traversal in this kernel).
@@ -534,7 +564,17 @@ __device__ __forceinline__ void shade_body(const ShadeArgs &a, uint32_t base, ui
             }
         }
 
-        if (a.temp) {
+        if (a.rec.rec) {
+            // staged adjoint replay: the vertex's colour goes to its record (the
+            // combine stage adds the visible direct light to it)
+            const uint32_t lane = (uint32_t)((py - a.rec.rect[1]) * a.rec.rect[2] + (px - a.rec.rect[0]));
+            write_vertex_record(a.rec, lane, a.rec.vertex, f3{so.col.x, so.col.y, so.col.z}, so);
+            a.rec.nverts[lane] = a.rec.vertex + 1;
+            if (Defer && nee_m) {
+                // the combine stage finds the record through pend_c.w
+                a.pend_c[i].w = __int_as_float((int)lane);
+            }
+        } else if (a.temp) {
             float4 &dst = a.temp[(size_t)py * a.w + px];
             if (a.primary) {
                 dst = make_float4(so.col.x, so.col.y, so.col.z, so.col.w);
@@ -704,7 +744,8 @@ struct CombineArgs {
     const float4 *nee_term;
     const uint8_t *nee_vis;
     float4 *temp;     // pixels (pend_c.w holds the pixel index), or
-    float4 *out_rgba; // per-ray output (parity entry point)
+    float4 *out_rgba; // per-ray output (parity entry point), or
+    AdjRecords rec;   // vertex records (pend_c.w holds the lane)
 };
 
 __device__ __forceinline__ void combine_body(const CombineArgs &a, uint32_t base) {
@@ -724,7 +765,9 @@ __device__ __forceinline__ void combine_body(const CombineArgs &a, uint32_t base
     const float4 pc = a.pend_c[i], pa = a.pend_alb[i];
     if (__float_as_int(pa.w) & 1) col = col * f3{pa.x, pa.y, pa.z};
     const f3 add = f3{pc.x * col.x, pc.y * col.y, pc.z * col.z};
-    float4 *dst = a.temp ? &a.temp[__float_as_int(pc.w)] : &a.out_rgba[i];
+    float4 *dst = a.rec.rec ? &a.rec.rec[(size_t)a.rec.vertex * a.rec.lanes + (uint32_t)__float_as_int(pc.w)]
+                  : a.temp  ? &a.temp[__float_as_int(pc.w)]
+                            : &a.out_rgba[i];
     float4 v = *dst;
     v.x += add.x;
     v.y += add.y;
