@@ -367,6 +367,25 @@ def run_large(args, lib, A, scenes, knobs, wide):
     reg = A.Region(lib, 0, 0, W, H)
     r.render(s, reg)  # upload + warm-up
     r.synchronize()
+    # a short sweep of the knobs that matter at this depth and size (2 passes each)
+    sweep = []
+    for cfg in ({"path_tail": -1, "lds_stack_depth": 16, "trace_occupancy": 0},
+                {"path_tail": -1, "lds_stack_depth": 8, "trace_occupancy": 5},
+                {"path_tail": 8, "lds_stack_depth": 16, "trace_occupancy": 0, "tail_occupancy": 3},
+                {"path_tail": knobs.get("path_tail", 2), "lds_stack_depth": knobs.get("lds_stack_depth", 8),
+                 "trace_occupancy": knobs.get("trace_occupancy", 0), "tail_occupancy": knobs.get("tail_occupancy", 2)}):
+        for k, v in cfg.items():
+            r.set_option(k, v)
+        r.reset_stats()
+        t = time.perf_counter()
+        for _ in range(2):
+            r.render(s, reg)
+        r.synchronize()
+        sweep.append((total(r.ray_counts()) / (time.perf_counter() - t) / 1e6, cfg))
+    best_cfg = max(sweep, key=lambda x: x[0])[1]
+    for k, v in best_cfg.items():
+        r.set_option(k, v)
+    log(f"large knobs: {[(round(m, 1), c) for m, c in sweep]} -> {best_cfg}")
     r.clear()
     r.reset_stats()
     reg = A.Region(lib, 0, 0, W, H)
@@ -380,7 +399,8 @@ def run_large(args, lib, A, scenes, knobs, wide):
            "config": {"workload": f"large_instanced: {spec.info['instances']} instances, "
                                   f"{spec.info['instanced_triangles']} instanced triangles, 32 sRGB textures "
                                   f"2048^2 with mips, {W}x{H}, {spp} spp, depth 12",
-                      "trees": "8-wide" if wide else "binary", "host_build_s": round(t_build, 1)}}
+                      "trees": "8-wide" if wide else "binary", "host_build_s": round(t_build, 1),
+                      "knobs": best_cfg}}
     log(f"large render: {out['mrays']} Mrays/s ({spp} passes in {dt:.3f}s)")
 
     # texel-gradient descent on 4 textures
@@ -431,7 +451,7 @@ def run_large(args, lib, A, scenes, knobs, wide):
         rs = []
         for sc, (build, w, h, n) in zip(ss, builds):
             rr = A.Renderer(lib, w, h, A.RENDERER_HIP, use_wide_bvh=wide)
-            for k, v in knobs.items():
+            for k, v in (best_cfg if sc is s else knobs).items():
                 rr.set_option(k, v)
             rr.set_option("max_bounces", 12)
             rr.render(sc, A.Region(lib, 0, 0, w, h))

@@ -121,6 +121,9 @@ int rh_shade_rays(rh_context *ctx, const rh_pass_info *pi, const rh_ray_packet *
 int rh_adjoint_pass(rh_context *ctx, const rh_pass_desc *pass, const float *d_adj_rgba, float *d_grad,
                     float *d_texel_grad, float scale);
 int rh_set_material_colors(rh_context *ctx, uint32_t first, uint32_t n, const float *rgb);
+/* limits texel gradients to the texels of the n given textures (n = 0: every
+ * texture, the default); the other textures' texels keep a zero gradient */
+int rh_set_gradient_textures(rh_context *ctx, const uint32_t *ids, uint32_t n);
 /* overwrites stored texels [first, first+n) of the uploaded scene (RGBA8 words) */
 int rh_set_texels(rh_context *ctx, uint32_t first, uint32_t n, const uint32_t *rgba);
 int rh_read_accum(rh_context *ctx, float *dst, size_t count);

@@ -157,6 +157,8 @@ int rhx_adjoint_scene(rhx_renderer *r, rhx_scene *s, rhx_region *reg, const floa
                       float *d_texel_grad, float scale);
 int rhx_set_material_colors(rhx_renderer *r, rhx_scene *s, uint32_t first, uint32_t n, const float *rgb);
 int rhx_set_texture_data(rhx_renderer *r, rhx_scene *s, uint32_t tex, const uint8_t *rgba);
+/* texel gradients only for the n given textures (n = 0: all); rh_set_gradient_textures */
+int rhx_set_gradient_textures(rhx_renderer *r, const uint32_t *ids, uint32_t n);
 int rhx_read_accum(rhx_renderer *r, float *dst, size_t count);
 
 /* Batching (HIP backend only; no counterpart in the reference, see

@@ -39,6 +39,7 @@ class DiffRenderer:
         self.rays_forward = 0
         self.rays_adjoint = 0
         self._layout = None
+        self._grad_textures = None
 
     def tex_layout(self) -> dict:
         """The scene's texture records, mip blocks and texel count (they do not
@@ -65,10 +66,15 @@ class DiffRenderer:
         self.r.read_accum(img.data_ptr())
         return img
 
-    def adjoint(self, grad_img: torch.Tensor, texels: bool = False):
+    def adjoint(self, grad_img: torch.Tensor, texels: bool = False, textures=None):
         """d loss / d main_color (n_materials, 3) for d loss / d image grad_img
         ((h, w, 3) or (h, w, 4)), for the colours of the last render; with
-        texels, also d loss / d stored texel (n_texels, 4) on the device."""
+        texels, also d loss / d stored texel (n_texels, 4) on the device, for
+        the texels of `textures` only when given (the others stay zero)."""
+        want = tuple(textures) if (texels and textures) else None
+        if want != self._grad_textures:
+            self.r.set_gradient_textures(want)
+            self._grad_textures = want
         adj = torch.zeros((self.h, self.w, 4), dtype=torch.float32, device=self.device)
         adj[..., :grad_img.shape[-1]] = grad_img.to(torch.float32)
         adj[..., 3] = 0.0
@@ -181,7 +187,7 @@ class TextureRenderFunction(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_out: torch.Tensor):
         dr = ctx.dr
-        _, tg = dr.adjoint(grad_out.contiguous(), texels=True)
+        _, tg = dr.adjoint(grad_out.contiguous(), texels=True, textures=ctx.tex_ids)
         layout = dr.tex_layout()
         g = torch.stack([fold_texel_grad_torch(layout, t, tg) for t in ctx.tex_ids])
         return g.to(ctx.param_device), None, None

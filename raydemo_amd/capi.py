@@ -225,6 +225,7 @@ class RayLib:
             "rhx_set_texture_data": (c_int, [P, P, c_uint32, P]),
             "rhx_set_material_colors": (c_int, [P, P, c_uint32, c_uint32, P]),
             "rhx_read_accum": (c_int, [P, P, c_size_t]),
+            "rhx_set_gradient_textures": (c_int, [P, P, c_uint32]),
         }
         optional.update({
             "rhx_batch_create": (c_int, [c_int, POINTER(P)]),
@@ -348,6 +349,12 @@ class Renderer:
     def set_texture_data(self, scene: "Scene", tex: int, rgba: np.ndarray):
         rgba = np.ascontiguousarray(rgba, np.uint8)
         self.lib.check(self.lib.lib.rhx_set_texture_data(self.h, scene.h, tex, _ptr(rgba)), "rhx_set_texture_data")
+
+    def set_gradient_textures(self, ids=None):
+        """Texel gradients only for these textures (None: every texture)."""
+        arr = np.ascontiguousarray(list(ids) if ids else [0], np.uint32)
+        self.lib.check(self.lib.lib.rhx_set_gradient_textures(self.h, _ptr(arr), len(ids) if ids else 0),
+                       "rhx_set_gradient_textures")
 
     def set_material_colors(self, scene: "Scene", first: int, rgb: np.ndarray):
         rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)

@@ -79,6 +79,7 @@ struct AdjointArgs {
     float scale;        // weight of this pass in the image (1 / passes for the mean)
     float *grad;        // 4 floats per material: d loss / d main_color (rgb, pad)
     float *tex_grad;    // optional: 4 floats per stored texel, d loss / d texel value (rgb, a unused)
+    const uint32_t *tex_mask; // optional: bit t set = texture t's texels get gradients (null: all)
     uint32_t n_materials;
     float4 *rec;        // per-lane vertex records, max_vertices x lanes, lane-minor
     float4 *rec_tex;    // with tex_grad: 3 records per vertex (texels, weights + raw value, raw + sRGB flag)
@@ -122,7 +123,10 @@ __device__ __forceinline__ void adjoint_accumulate(const AdjointArgs &a, uint32_
             // albedo = lin(raw) * main_color with raw the bilinear sum; S is
             // linear in the albedo, so d pixel / d raw = S / lin(raw) * lin'(raw)
             const float4 *rt = a.rec_tex + (size_t)k * 3 * a.lanes + i;
-            const float4 t0 = rt[0], t1 = rt[a.lanes], t2 = rt[2 * (size_t)a.lanes];
+            const float4 t2 = rt[2 * (size_t)a.lanes];
+            const uint32_t tex = __float_as_uint(t2.z);
+            if (a.tex_mask && !((a.tex_mask[tex >> 5] >> (tex & 31)) & 1u)) continue;
+            const float4 t0 = rt[0], t1 = rt[a.lanes];
             const float raw[3] = {t1.z, t1.w, t2.x};
             const bool srgb = t2.y != 0.0f;
             const float kx = t1.x, ky = t1.y;
@@ -517,6 +521,9 @@ struct rh_context {
     int trace_occupancy = 0;   // 8-wide trace/shadow kernels: 0 = as compiled (4 waves/SIMD), 5 = capped registers
     int max_bounces = RH_MAX_BOUNCES; // max_total_depth clamp (the reference's 10; up to RH_MAX_BOUNCES_EXT)
     int adjoint_staged = 1;           // adjoint replay as staged wavefront passes (0: one lane per path)
+    uint32_t n_textures = 0;
+    uint32_t *grad_tex_mask = nullptr; // textures whose texels get gradients (null: all)
+    size_t grad_tex_words = 0;
     int *adj_nverts = nullptr;        // staged replay: vertices recorded per lane
     size_t adj_nverts_cap = 0;
     int cus = 0;               // compute units of the device
@@ -936,6 +943,7 @@ void rh_destroy(rh_context *c) {
     free_ptr(c->adj_rec);
     free_ptr(c->adj_rec_tex);
     free_ptr(c->adj_nverts);
+    free_ptr(c->grad_tex_mask);
     free_ptr(c->color_stage);
     for (auto &e : c->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
@@ -973,6 +981,7 @@ int rh_upload_scene(rh_context *c, const rh_scene_view *v) {
     c->sc = s;
     c->n_materials = (uint32_t)v->materials_count;
     c->n_texels = (uint32_t)v->texels_count;
+    c->n_textures = (uint32_t)v->textures_count;
     c->have_scene = true;
     if (c->max_rays) {
         // spill slices sized for this scene's trees
@@ -1537,6 +1546,7 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
     a.rec = c->adj_rec;
     a.lanes = (uint32_t)lanes;
     a.tex_grad = d_texel_grad;
+    a.tex_mask = c->grad_tex_mask;
     a.rec_tex = d_texel_grad ? c->adj_rec_tex : nullptr;
     a.spill = c->spill;
     a.spill_stride = (uint32_t)c->spill_cap;
@@ -1587,6 +1597,32 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
         else hipLaunchKernelGGL((k_adjoint<false, 2, kTreeBinary>), grid, blk, lds, c->stream, a);
     }
     RH_CHECK(hipGetLastError());
+    return 0;
+}
+
+int rh_set_gradient_textures(rh_context *c, const uint32_t *ids, uint32_t n) {
+    if (!n) {
+        free_ptr(c->grad_tex_mask);
+        c->grad_tex_mask = nullptr;
+        c->grad_tex_words = 0;
+        return 0;
+    }
+    if (!ids) return arg_error("texture ids required");
+    uint32_t max_id = 0;
+    for (uint32_t k = 0; k < n; k++) max_id = std::max(max_id, ids[k]);
+    std::vector<uint32_t> mask(max_id / 32 + 1, 0u);
+    for (uint32_t k = 0; k < n; k++) mask[ids[k] >> 5] |= 1u << (ids[k] & 31);
+    // texture ids past the mask read as unselected: size it for every texture of the scene
+    mask.resize(std::max<size_t>(mask.size(), (c->n_textures + 31) / 32 + 1), 0u);
+    if (mask.size() > c->grad_tex_words || !c->grad_tex_mask) {
+        free_ptr(c->grad_tex_mask);
+        c->grad_tex_mask = nullptr;
+        c->grad_tex_words = mask.size();
+        RH_CHECK(hipMalloc(&c->grad_tex_mask, mask.size() * sizeof(uint32_t)));
+    }
+    RH_CHECK(hipMemcpyAsync(c->grad_tex_mask, mask.data(), mask.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                            c->stream));
+    RH_CHECK(hipStreamSynchronize(c->stream));
     return 0;
 }
 

@@ -245,6 +245,7 @@ struct ShadeOut {
     TexFoot alb_foot;
     f3 alb_raw;
     bool alb_srgb;
+    uint32_t alb_tex; // the albedo texture
 };
 
 // ShadeSurface (CoreRef.cpp:2055-2391) for one (ray, hit) pair.
@@ -366,6 +367,7 @@ __device__ __forceinline__ void shade_surface(const Scene &sc, const rh_pass_inf
     f4 albedo = sample_bilinear(sc, main_tex, uvs, (int)albedo_lod, &out.alb_foot);
     out.alb_raw = f3{albedo.x, albedo.y, albedo.z};
     out.alb_srgb = (mt.width & RH_TEXTURE_SRGB_BIT) != 0;
+    out.alb_tex = main_tex;
     if (mt.width & RH_TEXTURE_SRGB_BIT) albedo = srgb_to_rgb(albedo);
     albedo.x *= mat->main_color[0];
     albedo.y *= mat->main_color[1];

@@ -442,7 +442,7 @@ __device__ __forceinline__ void trace_refill_body(const TraceArgs &a, int *next_
 // lane i at k * lanes + i): rec = (C_k, material whose main_color scales it);
 // rec_tex (when texel gradients are taken) = 3 records per vertex: the
 // footprint's 4 texel indices, (kx, ky, raw r, raw g), (raw b, sRGB flag);
-// nverts = vertices recorded per lane.
+// (raw b, sRGB flag, texture id bits); nverts = vertices recorded per lane.
 struct AdjRecords {
     float4 *rec;
     float4 *rec_tex;
@@ -460,7 +460,7 @@ __device__ __forceinline__ void write_vertex_record(const AdjRecords &r, uint32_
         rt[0] = make_float4(__uint_as_float(so.alb_foot.idx[0]), __uint_as_float(so.alb_foot.idx[1]),
                             __uint_as_float(so.alb_foot.idx[2]), __uint_as_float(so.alb_foot.idx[3]));
         rt[r.lanes] = make_float4(so.alb_foot.kx, so.alb_foot.ky, so.alb_raw.x, so.alb_raw.y);
-        rt[2 * (size_t)r.lanes] = make_float4(so.alb_raw.z, so.alb_srgb ? 1.0f : 0.0f, 0.0f, 0.0f);
+        rt[2 * (size_t)r.lanes] = make_float4(so.alb_raw.z, so.alb_srgb ? 1.0f : 0.0f, __uint_as_float(so.alb_tex), 0.0f);
     }
 }
 

@@ -199,6 +199,12 @@ int rhx_set_material_colors(rhx_renderer *r, rhx_scene *s, uint32_t first, uint3
     });
 }
 
+int rhx_set_gradient_textures(rhx_renderer *r, const uint32_t *ids, uint32_t n) {
+    const int rc = rh_set_gradient_textures(r->r->context(), ids, n);
+    if (rc) g_err = rh_last_error();
+    return rc;
+}
+
 int rhx_set_texture_data(rhx_renderer *r, rhx_scene *s, uint32_t tex, const uint8_t *rgba) {
     return guarded([&] {
         if (!r->r->SetTextureData(s->s, tex, rgba)) {
