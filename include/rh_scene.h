@@ -185,9 +185,14 @@ int rhx_batch_restart(rhx_batch *b); /* back to pass 0 with cleared images */
 void rhx_set_environment(rhx_scene *s, const float env_col[3], float env_clamp, uint32_t env_map);
 uint32_t rhx_add_texture(rhx_scene *s, const uint8_t *rgba, int w, int h, int is_srgb, int generate_mipmaps);
 uint32_t rhx_add_material(rhx_scene *s, const rhx_mat_desc *m);
+/* bvh_flags: RHX_MESH_SPATIAL_SPLITS (mesh_desc_t::allow_spatial_splits),
+   RHX_MESH_FAST_BVH (mesh_desc_t::use_fast_bvh_build, SceneBase.h:94: the
+   HLBVH build of Core.cpp:532-680 in place of the SAH build) */
+#define RHX_MESH_SPATIAL_SPLITS 1
+#define RHX_MESH_FAST_BVH 2
 uint32_t rhx_add_mesh(rhx_scene *s, int layout, const float *attrs, size_t attrs_count,
                       const uint32_t *indices, size_t indices_count, int base_vertex,
-                      const rhx_shape_desc *shapes, size_t shapes_count, int allow_spatial_splits);
+                      const rhx_shape_desc *shapes, size_t shapes_count, int bvh_flags);
 void rhx_remove_mesh(rhx_scene *s, uint32_t i);
 uint32_t rhx_add_light(rhx_scene *s, const rhx_light_desc *l);
 uint32_t rhx_add_mesh_instance(rhx_scene *s, uint32_t mesh, const float xform[16]);

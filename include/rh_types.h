@@ -71,6 +71,13 @@ extern "C" {
 #define RH_HALTON_COUNT 24
 #define RH_HALTON_2D_COUNT 12
 #define RH_HALTON_SEQ_LEN 256
+/* Device (and oracle) Halton tables hold one more row, a copy of row 0.  A
+   pass reads dimensions 2*bounce and 2*bounce+1 of its row: bounce index 12
+   (the last bounce at depth 10, CoreRef.cpp ShadeSurface) and the bounces of
+   the max_bounces extension run past the row's 24 entries, into the next row,
+   and on the last row of an epoch past the 256-row table; the extra row keeps
+   those reads in bounds and defined (the wrap to row 0). */
+#define RH_HALTON_TABLE_FLOATS (RH_HALTON_COUNT * (RH_HALTON_SEQ_LEN + 1))
 
 #define RH_INVALID 0xffffffffu
 

@@ -121,6 +121,12 @@ void halton_table(const std::vector<uint16_t> &pm, int iteration, float *out) {
     }
 }
 
+// the table with its extra row (a copy of row 0), as the device holds it
+void halton_table_padded(const std::vector<uint16_t> &pm, int iteration, float *out) {
+    halton_table(pm, iteration, out);
+    std::copy(out, out + RH_HALTON_COUNT, out + RH_HALTON_COUNT * RH_HALTON_SEQ_LEN);
+}
+
 // ------------------------------------------------------------------ traversal
 struct Hit {
     int mask = 0, obj = -1, prim = -1;
@@ -1181,7 +1187,7 @@ int orc_set_max_bounces(int n) {
 int orc_render(const rh_scene_view *sc, const rh_camera *cam, int w, int h, const int32_t rect[4], int passes,
                int *iteration, float *clean, float *final_rgba, unsigned long long ray_counts[3]) {
     static const std::vector<uint16_t> pm = perms();
-    std::vector<float> halton(RH_HALTON_COUNT * RH_HALTON_SEQ_LEN);
+    std::vector<float> halton(RH_HALTON_TABLE_FLOATS);
     int have_halton = -1;
     float rmin[3] = {0, 0, 0}, cell[3] = {1, 1, 1};
     if (sc->macro_nodes_root != RH_INVALID && sc->mnodes) {
@@ -1212,7 +1218,7 @@ int orc_render(const rh_scene_view *sc, const rh_camera *cam, int w, int h, cons
     for (int pass = 0; pass < passes; pass++) {
         const int it = ++(*iteration);
         if (have_halton < 0 || it % RH_HALTON_SEQ_LEN == 0) {
-            halton_table(pm, it, halton.data());
+            halton_table_padded(pm, it, halton.data());
             have_halton = it;
         }
         rh_pass_info pi;
@@ -1291,7 +1297,7 @@ int orc_render(const rh_scene_view *sc, const rh_camera *cam, int w, int h, cons
 int orc_adjoint(const rh_scene_view *sc, const rh_camera *cam, int w, int h, const int32_t rect[4], int passes,
                 const float *adj_rgba, double scale, double *grad, double *texel_grad) {
     static const std::vector<uint16_t> pm = perms();
-    std::vector<float> halton(RH_HALTON_COUNT * RH_HALTON_SEQ_LEN);
+    std::vector<float> halton(RH_HALTON_TABLE_FLOATS);
     int have_halton = -1;
     const size_t n = (size_t)rect[2] * rect[3];
     std::vector<rh_ray_packet> rays(n), sec(n);
@@ -1304,7 +1310,7 @@ int orc_adjoint(const rh_scene_view *sc, const rh_camera *cam, int w, int h, con
     std::vector<std::vector<Vtx>> path((size_t)w * h);
     for (int it = 1; it <= passes; it++) {
         if (have_halton < 0 || it % RH_HALTON_SEQ_LEN == 0) {
-            halton_table(pm, it, halton.data());
+            halton_table_padded(pm, it, halton.data());
             have_halton = it;
         }
         rh_pass_info pi;

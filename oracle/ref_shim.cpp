@@ -208,7 +208,7 @@ uint32_t rhx_add_material(rhx_scene *s, const rhx_mat_desc *m) {
 
 uint32_t rhx_add_mesh(rhx_scene *s, int layout, const float *attrs, size_t attrs_count, const uint32_t *indices,
                       size_t indices_count, int base_vertex, const rhx_shape_desc *shapes, size_t shapes_count,
-                      int allow_spatial_splits) {
+                      int bvh_flags) {
     Ray::mesh_desc_t d;
     d.prim_type = Ray::TriangleList;
     d.layout = (Ray::eVertexLayout)layout;
@@ -217,7 +217,8 @@ uint32_t rhx_add_mesh(rhx_scene *s, int layout, const float *attrs, size_t attrs
     d.vtx_indices = indices;
     d.vtx_indices_count = indices_count;
     d.base_vertex = base_vertex;
-    d.allow_spatial_splits = allow_spatial_splits != 0;
+    d.allow_spatial_splits = (bvh_flags & 1) != 0;
+    d.use_fast_bvh_build = (bvh_flags & 2) != 0;
     for (size_t i = 0; i < shapes_count; i++) {
         d.shapes.emplace_back(shapes[i].mat_index, shapes[i].back_mat_index, (size_t)shapes[i].vtx_start,
                               (size_t)shapes[i].vtx_count);

@@ -519,7 +519,7 @@ class Scene:
         return self.lib.lib.rhx_add_material(self.h, C.byref(m))
 
     def add_mesh(self, layout, attrs: np.ndarray, indices: np.ndarray, shapes, base_vertex=0,
-                 allow_spatial_splits=False) -> int:
+                 allow_spatial_splits=False, use_fast_bvh_build=False) -> int:
         attrs = np.ascontiguousarray(attrs, dtype=np.float32).ravel()
         indices = np.ascontiguousarray(indices, dtype=np.uint32).ravel()
         sh = (ShapeDesc * len(shapes))()
@@ -530,7 +530,8 @@ class Scene:
                 sh[i] = ShapeDesc(s[0], s[1], s[2], s[3])
         n_vtx = attrs.size // ATTR_STRIDES[layout]
         return self.lib.lib.rhx_add_mesh(self.h, layout, _ptr(attrs), n_vtx, _ptr(indices), indices.size, base_vertex,
-                                         C.cast(sh, c_void_p), len(shapes), int(allow_spatial_splits))
+                                         C.cast(sh, c_void_p), len(shapes),
+                                         int(bool(allow_spatial_splits)) | (2 if use_fast_bvh_build else 0))
 
     def remove_mesh(self, i):
         self.lib.lib.rhx_remove_mesh(self.h, i)

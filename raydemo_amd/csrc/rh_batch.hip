@@ -24,6 +24,7 @@ using rh_rt::arg_error;
 using rh_rt::DevBuf;
 using rh_rt::fail;
 using rh_rt::free_ptr;
+using rh_rt::upload_halton;
 
 namespace {
 
@@ -563,7 +564,8 @@ int rh_batch_create(int device, rh_batch **out) {
     hipDeviceProp_t prop;
     RH_CHECK(hipGetDeviceProperties(&prop, device));
     b->cus = prop.multiProcessorCount;
-    RH_CHECK(hipMalloc(&b->halton, RH_HALTON_COUNT * RH_HALTON_SEQ_LEN * sizeof(float)));
+    RH_CHECK(hipMalloc(&b->halton, RH_HALTON_TABLE_FLOATS * sizeof(float)));
+    RH_CHECK(hipMemsetAsync(b->halton, 0, RH_HALTON_TABLE_FLOATS * sizeof(float), b->stream));
     RH_CHECK(hipMalloc(&b->stats, 8 * sizeof(unsigned long long)));
     RH_CHECK(hipMemsetAsync(b->stats, 0, 8 * sizeof(unsigned long long), b->stream));
     RH_CHECK(hipStreamSynchronize(b->stream));
@@ -628,8 +630,7 @@ int rh_batch_add_scene(rh_batch *b, const rh_scene_view *v, const rh_pass_desc *
 
 int rh_batch_set_halton(rh_batch *b, const float *seq, size_t count) {
     if (count < (size_t)RH_HALTON_COUNT * RH_HALTON_SEQ_LEN) return arg_error("Halton table needs 256 x 24 floats");
-    RH_CHECK(hipMemcpyAsync(b->halton, seq, RH_HALTON_COUNT * RH_HALTON_SEQ_LEN * sizeof(float),
-                            hipMemcpyHostToDevice, b->stream));
+    RH_CHECK(upload_halton(b->halton, seq, b->stream));
     RH_CHECK(hipStreamSynchronize(b->stream));
     return 0;
 }

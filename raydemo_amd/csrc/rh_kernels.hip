@@ -15,12 +15,14 @@
 #include <string>
 #include <vector>
 
+#include "rh_radix.h"
 #include "rh_runtime.h"
 
 using rh_rt::arg_error;
 using rh_rt::DevBuf;
 using rh_rt::fail;
 using rh_rt::free_ptr;
+using rh_rt::upload_halton;
 
 namespace {
 
@@ -99,61 +101,121 @@ struct AdjointArgs {
 
 constexpr uint32_t kLdsGradMaterials = 1024;
 
+constexpr int kAggRounds = 4;
+
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+constexpr uint32_t kNoKey = 0xffffffffu;
+
+// Adds v to dst[4 key + 0..2] (lanes with key kNoKey add nothing); every lane
+// of the wave calls it.  Lanes adding to the same key are summed in registers
+// first and one lane adds the sum: same-address atomics serialise, and a wave
+// often adds to one address (few materials, triangles a few pixels wide, the
+// texels of a coarse mip level under many pixels).  After kAggRounds distinct
+// keys the remaining lanes add their own values.
+__device__ __forceinline__ void wave_add3(float *dst, uint32_t key, f3 v) {
+    const uint32_t lane = __lane_id();
+    uint64_t pending = __ballot(key != kNoKey);
+    for (int round = 0; pending && round < kAggRounds; round++) {
+        const int leader = __builtin_ctzll(pending);
+        const uint32_t kk = __shfl(key, leader);
+        const bool mine = key == kk;
+        const float sx = wave_sum(mine ? v.x : 0.0f), sy = wave_sum(mine ? v.y : 0.0f),
+                    sz = wave_sum(mine ? v.z : 0.0f);
+        if ((int)lane == leader) {
+            float *d = dst + 4 * (size_t)kk;
+            if (sx != 0.0f) atomicAdd(d + 0, sx);
+            if (sy != 0.0f) atomicAdd(d + 1, sy);
+            if (sz != 0.0f) atomicAdd(d + 2, sz);
+        }
+        pending &= ~__ballot(mine);
+    }
+    if ((pending >> lane) & 1u) {
+        float *d = dst + 4 * (size_t)key;
+        if (v.x != 0.0f) atomicAdd(d + 0, v.x);
+        if (v.y != 0.0f) atomicAdd(d + 1, v.y);
+        if (v.z != 0.0f) atomicAdd(d + 2, v.z);
+    }
+}
+
 // The backward half of the replay for lane i (pixel adjoint dl) over its nv
 // vertex records: S_k = sum_{j >= k} C_j, added as adj . S_k / main_color to the
 // material of vertex k and, with texel gradients, to its texel footprint.
+// Every lane of the wave calls it (lanes with nothing to add pass nv = 0): the
+// vertex loop runs over the wave's longest path so that the sums can be formed
+// across the wave.
 __device__ __forceinline__ void adjoint_accumulate(const AdjointArgs &a, uint32_t i, int nv, float4 dl, bool lds_grad,
                                                    float *s_grad) {
+    int kmax = nv;
+    for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
+    float *mdst = lds_grad ? s_grad : a.grad;
     f3 S = f3{0.0f, 0.0f, 0.0f};
-    for (int k = nv - 1; k >= 0; k--) {
-        const float4 r = a.rec[(size_t)k * a.lanes + i];
-        S = S + f3{r.x, r.y, r.z};
-        const int m = __float_as_int(r.w);
-        if (m < 0) continue;
-        const float *mc = a.sc.materials[m].main_color;
-        const float g[3] = {dl.x * S.x, dl.y * S.y, dl.z * S.z};
-#pragma unroll
-        for (int ch = 0; ch < 3; ch++) {
-            if (mc[ch] == 0.0f || g[ch] == 0.0f) continue;
-            const float v = a.scale * (g[ch] / mc[ch]);
-            if (lds_grad) atomicAdd(&s_grad[4 * m + ch], v);
-            else atomicAdd(&a.grad[4 * m + ch], v);
+    for (int k = kmax - 1; k >= 0; k--) {
+        int m = -1;
+        f3 v = f3{0.0f, 0.0f, 0.0f};
+        f3 g = f3{0.0f, 0.0f, 0.0f};
+        if (k < nv) {
+            const float4 r = a.rec[(size_t)k * a.lanes + i];
+            S = S + f3{r.x, r.y, r.z};
+            m = __float_as_int(r.w);
+            if (m >= 0) {
+                const float *mc = a.sc.materials[m].main_color;
+                g = f3{dl.x * S.x, dl.y * S.y, dl.z * S.z};
+                if (mc[0] != 0.0f && g.x != 0.0f) v.x = a.scale * (g.x / mc[0]);
+                if (mc[1] != 0.0f && g.y != 0.0f) v.y = a.scale * (g.y / mc[1]);
+                if (mc[2] != 0.0f && g.z != 0.0f) v.z = a.scale * (g.z / mc[2]);
+            }
         }
-        if (a.tex_grad) {
-            // albedo = lin(raw) * main_color with raw the bilinear sum; S is
-            // linear in the albedo, so d pixel / d raw = S / lin(raw) * lin'(raw)
+        wave_add3(mdst, m >= 0 ? (uint32_t)m : kNoKey, v);
+        if (!a.tex_grad) continue;
+        // albedo = lin(raw) * main_color with raw the bilinear sum; S is linear
+        // in the albedo, so d pixel / d raw = S / lin(raw) * lin'(raw), and
+        // each footprint texel takes its bilinear weight's share
+        uint32_t idx[4] = {kNoKey, kNoKey, kNoKey, kNoKey};
+        float w[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        float gr[3] = {0.0f, 0.0f, 0.0f};
+        if (m >= 0) {
             const float4 *rt = a.rec_tex + (size_t)k * 3 * a.lanes + i;
             const float4 t2 = rt[2 * (size_t)a.lanes];
             const uint32_t tex = __float_as_uint(t2.z);
-            if (a.tex_mask && !((a.tex_mask[tex >> 5] >> (tex & 31)) & 1u)) continue;
-            const float4 t0 = rt[0], t1 = rt[a.lanes];
-            const float raw[3] = {t1.z, t1.w, t2.x};
-            const bool srgb = t2.y != 0.0f;
-            const float kx = t1.x, ky = t1.y;
-            const float w[4] = {(1.0f - kx) * (1.0f - ky), kx * (1.0f - ky), (1.0f - kx) * ky, kx * ky};
-            const uint32_t idx[4] = {__float_as_uint(t0.x), __float_as_uint(t0.y), __float_as_uint(t0.z),
-                                     __float_as_uint(t0.w)};
+            if (!a.tex_mask || ((a.tex_mask[tex >> 5] >> (tex & 31)) & 1u)) {
+                const float4 t0 = rt[0], t1 = rt[a.lanes];
+                const float raw[3] = {t1.z, t1.w, t2.x};
+                const float gch[3] = {g.x, g.y, g.z};
+                const bool srgb = t2.y != 0.0f;
+                const float kx = t1.x, ky = t1.y;
+                w[0] = (1.0f - kx) * (1.0f - ky);
+                w[1] = kx * (1.0f - ky);
+                w[2] = (1.0f - kx) * ky;
+                w[3] = kx * ky;
+                idx[0] = __float_as_uint(t0.x);
+                idx[1] = __float_as_uint(t0.y);
+                idx[2] = __float_as_uint(t0.z);
+                idx[3] = __float_as_uint(t0.w);
 #pragma unroll
-            for (int ch = 0; ch < 3; ch++) {
-                float lin, dlin;
-                if (!srgb) {
-                    lin = raw[ch];
-                    dlin = 1.0f;
-                } else if (raw[ch] > 0.04045f) {
-                    const float b = (raw[ch] + 0.055f) / 1.055f;
-                    lin = rpow(b, 2.4f);
-                    dlin = 2.4f / 1.055f * rpow(b, 1.4f);
-                } else {
-                    lin = raw[ch] / 12.92f;
-                    dlin = 1.0f / 12.92f;
+                for (int ch = 0; ch < 3; ch++) {
+                    float lin, dlin;
+                    if (!srgb) {
+                        lin = raw[ch];
+                        dlin = 1.0f;
+                    } else if (raw[ch] > 0.04045f) {
+                        const float b = (raw[ch] + 0.055f) / 1.055f;
+                        lin = rpow(b, 2.4f);
+                        dlin = 2.4f / 1.055f * rpow(b, 1.4f);
+                    } else {
+                        lin = raw[ch] / 12.92f;
+                        dlin = 1.0f / 12.92f;
+                    }
+                    if (lin != 0.0f && gch[ch] != 0.0f) gr[ch] = a.scale * (gch[ch] / lin) * dlin;
                 }
-                if (lin == 0.0f || g[ch] == 0.0f) continue;
-                const float gr = a.scale * (g[ch] / lin) * dlin;
-#pragma unroll
-                for (int q = 0; q < 4; q++)
-                    if (w[q] != 0.0f) atomicAdd(&a.tex_grad[4 * (size_t)idx[q] + ch], gr * w[q]);
             }
         }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            wave_add3(a.tex_grad, w[q] != 0.0f ? idx[q] : kNoKey, f3{gr[0] * w[q], gr[1] * w[q], gr[2] * w[q]});
     }
 }
 
@@ -187,10 +249,12 @@ __global__ void __launch_bounds__(kBlock, MinBlocks) k_adjoint(AdjointArgs a) {
     uint32_t traced = 0;
     Stack st;
     st.init(lds_stack, a.spill + i, a.spill_stride, a.lds_depth, kSpillDepth);
+    int nv = 0;
+    float4 dl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (active) {
         const int x = a.pc.rect[0] + (int)i % rw, y = a.pc.rect[1] + (int)i / rw;
         const int p = y * a.pc.w + x;
-        const float4 dl = a.adj[p];
+        dl = a.adj[p];
         if (dl.x != 0.0f || dl.y != 0.0f || dl.z != 0.0f) {
             RayFull ray = gen_primary_ray(a.pc, a.halton, x, y);
             rh_pass_info pi = a.pi;
@@ -199,7 +263,6 @@ __global__ void __launch_bounds__(kBlock, MinBlocks) k_adjoint(AdjointArgs a) {
             if (pi.settings.flags & RH_USE_COHERENT_SAMPLING) pi.rand_index = c_sampling_pattern[(y % 8) * 8 + (x % 8)];
             const int last = pi.settings.max_total_depth;
             const bool skip_indirect = (pi.settings.flags & RH_SKIP_INDIRECT_LIGHT) != 0;
-            int nv = 0;
             for (int b = -1;;) {
                 Hit hit;
                 hit.mask = 0;
@@ -226,9 +289,9 @@ __global__ void __launch_bounds__(kBlock, MinBlocks) k_adjoint(AdjointArgs a) {
                 if (!so.emit || skip_indirect || b >= last) break;
                 ray = so.next;
             }
-            adjoint_accumulate(a, i, nv, dl, lds_grad, s_grad);
         }
     }
+    adjoint_accumulate(a, i, nv, dl, lds_grad, s_grad);
     if (lds_grad) {
         __syncthreads();
         for (uint32_t j = threadIdx.x; j < 4 * a.n_materials; j += blockDim.x)
@@ -250,11 +313,14 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_backward(AdjointArgs a, cons
     }
     const int rw = a.pc.rect[2], rh_ = a.pc.rect[3];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    int nv = 0;
+    float4 dl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if ((int)i < rw * rh_) {
         const int x = a.pc.rect[0] + (int)i % rw, y = a.pc.rect[1] + (int)i / rw;
-        const float4 dl = a.adj[y * a.pc.w + x];
-        if (dl.x != 0.0f || dl.y != 0.0f || dl.z != 0.0f) adjoint_accumulate(a, i, nverts[i], dl, lds_grad, s_grad);
+        dl = a.adj[y * a.pc.w + x];
+        if (dl.x != 0.0f || dl.y != 0.0f || dl.z != 0.0f) nv = nverts[i];
     }
+    adjoint_accumulate(a, i, nv, dl, lds_grad, s_grad);
     if (lds_grad) {
         __syncthreads();
         for (uint32_t j = threadIdx.x; j < 4 * a.n_materials; j += blockDim.x)
@@ -312,135 +378,6 @@ __global__ void __launch_bounds__(kBlock) k_ray_keys(const rh_ray *__restrict__ 
     const float4 a = p[0], b = p[1];
     keys[i] = ray_hash(f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, pc.root_min, pc.cell_size);
     vals[i] = (uint32_t)i;
-}
-
-// Radix pass 1/3: per-tile digit histogram, stored digit-major.
-__global__ void __launch_bounds__(kBlock) k_radix_hist(const uint32_t *__restrict__ keys, const int *__restrict__ count,
-                                                       int n_static, int shift, uint32_t *__restrict__ hist,
-                                                       int num_tiles) {
-    __shared__ uint32_t h[256];
-    const int n = count ? *count : n_static;
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const int base = blockIdx.x * kSortTile;
-    if (base < n) {
-        for (int r = 0; r < kSortItems; r++) {
-            const int i = base + r * kBlock + threadIdx.x;
-            if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xffu], 1u);
-        }
-    }
-    __syncthreads();
-    hist[threadIdx.x * num_tiles + blockIdx.x] = h[threadIdx.x];
-}
-
-// Wave-level inclusive scan (64 lanes) with cross-lane shuffles.
-__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t u = __shfl_up(v, off);
-        if (lane >= (uint32_t)off) v += u;
-    }
-    return v;
-}
-
-// Radix pass 2a/3: one block per digit scans that digit's row of per-tile
-// counts (exclusive, in place) and records the digit's total.
-__global__ void __launch_bounds__(kBlock) k_radix_scan_rows(uint32_t *__restrict__ hist, int num_tiles,
-                                                            uint32_t *__restrict__ totals) {
-    __shared__ uint32_t wsum[kBlock / 64];
-    __shared__ uint32_t carry;
-    uint32_t *row = hist + (size_t)blockIdx.x * num_tiles;
-    const int t = threadIdx.x, wave = t >> 6;
-    if (t == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < num_tiles; base += kBlock) {
-        const int i = base + t;
-        const uint32_t v = i < num_tiles ? row[i] : 0u;
-        const uint32_t inc = wave_inclusive_scan(v);
-        if ((t & 63) == 63) wsum[wave] = inc;
-        __syncthreads();
-        uint32_t before = carry;
-        for (int w = 0; w < wave; w++) before += wsum[w];
-        if (i < num_tiles) row[i] = before + inc - v;
-        __syncthreads();
-        if (t == kBlock - 1) carry = before + inc;
-        __syncthreads();
-    }
-    if (t == 0) totals[blockIdx.x] = carry;
-}
-
-// Radix pass 2b/3: adds each digit's base (sum of the smaller digits' totals).
-__global__ void __launch_bounds__(kBlock) k_radix_scan_bases(uint32_t *__restrict__ hist, int num_tiles,
-                                                             const uint32_t *__restrict__ totals) {
-    __shared__ uint32_t base;
-    const int d = blockIdx.x;
-    if (threadIdx.x == 0) {
-        uint32_t s = 0;
-        for (int k = 0; k < d; k++) s += totals[k];
-        base = s;
-    }
-    __syncthreads();
-    uint32_t *row = hist + (size_t)d * num_tiles;
-    for (int i = threadIdx.x; i < num_tiles; i += kBlock) row[i] += base;
-}
-
-// Radix pass 3/3: stable scatter.  Within a round of 256 keys, each wave finds
-// the lanes sharing its digit with eight ballots (one per digit bit), ranks
-// them by lane order, and the per-wave digit counts are prefix-summed across
-// the block's four waves in LDS; rounds are processed in order, so equal keys
-// keep their input order.
-__global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t *__restrict__ keys_in,
-                                                          const uint32_t *__restrict__ vals_in,
-                                                          uint32_t *__restrict__ keys_out,
-                                                          uint32_t *__restrict__ vals_out, const int *__restrict__ count,
-                                                          int n_static, int shift, const uint32_t *__restrict__ hist,
-                                                          int num_tiles) {
-    __shared__ uint32_t run[256];          // running digit offsets of this tile
-    __shared__ uint32_t wcount[4][256];    // per-wave digit counts of this round
-    const int n = count ? *count : n_static;
-    const int base = blockIdx.x * kSortTile;
-    if (base >= n) return;
-    const int t = threadIdx.x, wave = t >> 6;
-    const uint32_t lane = lane_id();
-    run[t] = hist[t * num_tiles + blockIdx.x];
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-
-    for (int r = 0; r < kSortItems; r++) {
-        const int i = base + r * kBlock + t;
-        const bool active = i < n;
-        uint32_t key = 0, val = 0, digit = 0;
-        if (active) {
-            key = keys_in[i];
-            val = vals_in[i];
-            digit = (key >> shift) & 0xffu;
-        }
-        wcount[0][t] = wcount[1][t] = wcount[2][t] = wcount[3][t] = 0;
-        __syncthreads();
-        uint64_t peers = __ballot(active);
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            const uint64_t m = __ballot((digit >> b) & 1u);
-            peers &= ((digit >> b) & 1u) ? m : ~m;
-        }
-        const uint32_t rank = (uint32_t)__popcll(peers & lt);
-        if (active && rank == 0) wcount[wave][digit] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        uint32_t pos = 0;
-        if (active) {
-            uint32_t before = 0;
-            for (int w2 = 0; w2 < wave; w2++) before += wcount[w2][digit];
-            pos = run[digit] + before + rank;
-        }
-        __syncthreads();
-        // advance the running offsets by this round's totals
-        run[t] += wcount[0][t] + wcount[1][t] + wcount[2][t] + wcount[3][t];
-        if (active) {
-            keys_out[pos] = key;
-            vals_out[pos] = val;
-        }
-        __syncthreads();
-    }
 }
 
 __global__ void __launch_bounds__(kBlock) k_gather(const rh_ray *__restrict__ rays_in, const rh_ray_aux *__restrict__ aux_in,
@@ -905,8 +842,8 @@ int rh_create(int device, rh_context **out) {
     RH_CHECK(hipMemsetAsync(c->stats_dev, 0, 8 * sizeof(unsigned long long), c->stream));
     RH_CHECK(hipMalloc(&c->visits, 4 * sizeof(unsigned long long)));
     RH_CHECK(hipMemsetAsync(c->visits, 0, 4 * sizeof(unsigned long long), c->stream));
-    RH_CHECK(hipMalloc(&c->halton, RH_HALTON_COUNT * RH_HALTON_SEQ_LEN * sizeof(float)));
-    RH_CHECK(hipMemsetAsync(c->halton, 0, RH_HALTON_COUNT * RH_HALTON_SEQ_LEN * sizeof(float), c->stream));
+    RH_CHECK(hipMalloc(&c->halton, RH_HALTON_TABLE_FLOATS * sizeof(float)));
+    RH_CHECK(hipMemsetAsync(c->halton, 0, RH_HALTON_TABLE_FLOATS * sizeof(float), c->stream));
     *out = c;
     return 0;
 }
@@ -994,8 +931,7 @@ int rh_upload_scene(rh_context *c, const rh_scene_view *v) {
 
 int rh_set_halton(rh_context *c, const float *seq, size_t count) {
     if (count < (size_t)RH_HALTON_COUNT * RH_HALTON_SEQ_LEN) return arg_error("Halton table needs 256 x 24 floats");
-    RH_CHECK(hipMemcpyAsync(c->halton, seq, RH_HALTON_COUNT * RH_HALTON_SEQ_LEN * sizeof(float), hipMemcpyHostToDevice,
-                            c->stream));
+    RH_CHECK(upload_halton(c->halton, seq, c->stream));
     // the host table may change right after this call returns
     RH_CHECK(hipStreamSynchronize(c->stream));
     return 0;
@@ -1358,8 +1294,7 @@ int rh_gen_primary_rays(rh_context *c, const rh_camera *cam, const int32_t rect[
     PassConst pc = make_pass_const(c, &p);
     c->w = save_w;
     c->h = save_h;
-    RH_CHECK(hipMemcpyAsync(c->halton, halton, RH_HALTON_COUNT * RH_HALTON_SEQ_LEN * sizeof(float),
-                            hipMemcpyHostToDevice, c->stream));
+    RH_CHECK(upload_halton(c->halton, halton, c->stream));
     hipLaunchKernelGGL(k_gen_primary, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, pc, c->halton,
                        c->rays[0], c->aux[0], c->counters + 0);
     RH_CHECK(hipGetLastError());
@@ -1462,8 +1397,7 @@ int rh_shade_rays(rh_context *c, const rh_pass_info *pi, const rh_ray_packet *ra
     RH_CHECK(hipMalloc(&d_src, n * sizeof(int)));
     RH_CHECK(hipMemcpyAsync(d_pi, pi, n * sizeof(rh_pass_info), hipMemcpyHostToDevice, c->stream));
     RH_CHECK(hipMemcpyAsync(c->hits, hits, n * sizeof(rh_hit), hipMemcpyHostToDevice, c->stream));
-    RH_CHECK(hipMemcpyAsync(c->halton, halton, RH_HALTON_COUNT * RH_HALTON_SEQ_LEN * sizeof(float),
-                            hipMemcpyHostToDevice, c->stream));
+    RH_CHECK(upload_halton(c->halton, halton, c->stream));
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, c->counters + 1, 0);
     ShadeArgs a = shade_args(c);
     a.rays = c->rays[0];

@@ -35,6 +35,16 @@ inline void free_ptr(void *p) {
     if (p) (void)hipFree(p);
 }
 
+// Uploads a 256-row Halton table into a device table of RH_HALTON_TABLE_FLOATS
+// floats: the rows, then row 0 again as the extra row.
+inline hipError_t upload_halton(float *dev, const float *seq, hipStream_t stream) {
+    hipError_t e = hipMemcpyAsync(dev, seq, RH_HALTON_COUNT * RH_HALTON_SEQ_LEN * sizeof(float),
+                                  hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(dev + RH_HALTON_COUNT * RH_HALTON_SEQ_LEN, seq, RH_HALTON_COUNT * sizeof(float),
+                          hipMemcpyHostToDevice, stream);
+}
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;

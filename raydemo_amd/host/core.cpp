@@ -215,7 +215,10 @@ uint32_t PreprocessMesh(const float *attrs, const uint32_t *vtx_indices, size_t 
 
     const size_t indices_start = out_tri_indices.size();
     const uint32_t num_out_nodes =
-        PreprocessPrims_SAH(primitives.data(), primitives.size(), positions, attr_stride, s, out_nodes, out_tri_indices);
+        s.use_fast_bvh_build
+            ? PreprocessPrims_HLBVH(primitives.data(), primitives.size(), out_nodes, out_tri_indices)
+            : PreprocessPrims_SAH(primitives.data(), primitives.size(), positions, attr_stride, s, out_nodes,
+                                  out_tri_indices);
     for (size_t i = indices_start; i < out_tri_indices.size(); i++) out_tri_indices[i] += (uint32_t)tris_start;
     return num_out_nodes;
 }
@@ -292,6 +295,180 @@ uint32_t PreprocessPrims_SAH(const prim_t *prims, size_t prims_count, const floa
         }
     }
     return (uint32_t)(out_nodes.size() - root_node_index);
+}
+
+// ---- the fast build: HLBVH (mesh_desc_t::use_fast_bvh_build) ----
+
+uint32_t F2U32_x86(float f) {
+    // x86-64 converts float to uint32_t through a 64-bit truncation and keeps
+    // the low half; NaN and values outside the int64 range give 0x8000...0
+    if (!(f > -9.2233715e18f && f < 9.2233715e18f)) return 0u;
+    return (uint32_t)(int64_t)f;
+}
+
+static uint32_t spread_bits3(uint32_t x) {
+    // the 10 low bits of x moved to bits 0, 3, 6, ..., 27
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+uint32_t MortonCode3(uint32_t x, uint32_t y, uint32_t z) {
+    return (spread_bits3(z) << 2) + (spread_bits3(y) << 1) + spread_bits3(x);
+}
+
+void MortonCodes(const prim_t *prims, size_t n, uint32_t *codes) {
+    const float big = std::numeric_limits<float>::max();
+    vec3 lo = {{big, big, big}}, hi = {{-big, -big, -big}};
+    for (size_t j = 0; j < n; j++) {
+        lo = vmin(lo, prims[j].bbox_min);
+        hi = vmax(hi, prims[j].bbox_max);
+    }
+    float scale[3];
+    for (int a = 0; a < 3; a++) scale[a] = (float)(1 << kMortonBitsPerAxis) / (hi[a] - lo[a]);
+    for (size_t j = 0; j < n; j++) {
+        uint32_t c[3];
+        for (int a = 0; a < 3; a++) {
+            const float centre = 0.5f * (prims[j].bbox_min[a] + prims[j].bbox_max[a]);
+            c[a] = F2U32_x86((centre - lo[a]) * scale[a]);
+        }
+        codes[j] = MortonCode3(c[0], c[1], c[2]);
+    }
+}
+
+static void emit_lbvh_node(const prim_t *prims, const uint32_t *indices, const uint32_t *codes, uint32_t node,
+                           uint32_t first, uint32_t count, uint32_t index_offset, int bit,
+                           std::vector<rh_bvh_node> &out) {
+    const uint32_t last = first + count - 1;
+    while (bit >= 0 && count >= 8 && ((codes[first] >> bit) & 1u) == ((codes[last] >> bit) & 1u)) bit--;
+    if (bit < 0 || count < 8) {
+        const float big = std::numeric_limits<float>::max();
+        vec3 lo = {{big, big, big}}, hi = {{-big, -big, -big}};
+        for (uint32_t i = first; i <= last; i++) {
+            lo = vmin(lo, prims[indices[i]].bbox_min);
+            hi = vmax(hi, prims[indices[i]].bbox_max);
+        }
+        rh_bvh_node &n = out[node];
+        n.prim_index = RH_LEAF_NODE_BIT + first + index_offset;
+        n.prim_count = count;
+        memcpy(n.bbox_min, lo.v, 3 * sizeof(float));
+        memcpy(n.bbox_max, hi.v, 3 * sizeof(float));
+        return;
+    }
+    // codes are sorted and agree above `bit`: the range is 0...0 1...1 in it
+    const uint32_t split =
+        (uint32_t)(std::partition_point(codes + first, codes + last + 1,
+                                        [bit](uint32_t c) { return ((c >> bit) & 1u) == 0u; }) - codes);
+    uint32_t left = (uint32_t)out.size(), right = left + 1;
+    out.resize(out.size() + 2);
+    memset(&out[left], 0, 2 * sizeof(rh_bvh_node));
+    emit_lbvh_node(prims, indices, codes, left, first, split - first, index_offset, bit - 1, out);
+    emit_lbvh_node(prims, indices, codes, right, split, last + 1 - split, index_offset, bit - 1, out);
+    rh_bvh_node &n = out[node];
+    for (int a = 0; a < 3; a++) {
+        n.bbox_min[a] = fmin_std(out[left].bbox_min[a], out[right].bbox_min[a]);
+        n.bbox_max[a] = fmax_std(out[left].bbox_max[a], out[right].bbox_max[a]);
+    }
+    const uint32_t axis = (uint32_t)bit % 3;
+    if (out[left].bbox_min[axis] > out[right].bbox_min[axis]) std::swap(left, right);
+    n.prim_index = left;
+    n.prim_count = right + (axis << 30);
+}
+
+uint32_t EmitLBVH(const prim_t *prims, const uint32_t *indices, const uint32_t *codes, uint32_t first, uint32_t count,
+                  uint32_t index_offset, int bit, std::vector<rh_bvh_node> &out) {
+    // Depth first: a node splits its (code-sorted) range where the highest
+    // bit that differs inside it turns from 0 to 1; the two children are
+    // appended together, the lower half is built first.  Ranges of fewer
+    // than 8 primitives, or with identical codes, become leaves.
+    const uint32_t node = (uint32_t)out.size();
+    out.emplace_back();
+    memset(&out[node], 0, sizeof(rh_bvh_node));
+    emit_lbvh_node(prims, indices, codes, node, first, count, index_offset, bit, out);
+    return node;
+}
+
+uint32_t AssembleHLBVH(const std::vector<HlbvhTreelet> &treelets, const std::vector<rh_bvh_node> &bottom,
+                       const std::vector<rh_bvh_node> &top, const std::vector<uint32_t> &top_indices,
+                       std::vector<rh_bvh_node> &out_nodes) {
+    // The top tree's leaves (one treelet each) are dropped and its interior
+    // nodes point at the treelets' roots instead; the treelets' nodes follow
+    // the top tree's interior nodes.
+    const uint32_t base = (uint32_t)out_nodes.size();
+    std::vector<uint32_t> dropped_before(top.size() + 1, 0);
+    for (size_t i = 0; i < top.size(); i++)
+        dropped_before[i + 1] = dropped_before[i] + ((top[i].prim_index & RH_LEAF_NODE_BIT) ? 1u : 0u);
+    const uint32_t interior = (uint32_t)top.size() - dropped_before[top.size()];
+    const uint32_t bottom_base = base + interior;
+    auto child_ref = [&](uint32_t c) -> uint32_t {
+        const rh_bvh_node &ch = top[c];
+        if (ch.prim_index & RH_LEAF_NODE_BIT)
+            return bottom_base + treelets[top_indices[ch.prim_index & RH_PRIM_INDEX_BITS]].node_index;
+        return base + c - dropped_before[c];
+    };
+    for (size_t i = 0; i < top.size(); i++) {
+        rh_bvh_node n = top[i];
+        if (n.prim_index & RH_LEAF_NODE_BIT) continue;
+        n.prim_index = child_ref(n.prim_index);
+        n.prim_count = (n.prim_count & ~RH_RIGHT_CHILD_BITS) + child_ref(n.prim_count & RH_RIGHT_CHILD_BITS);
+        out_nodes.push_back(n);
+    }
+    for (rh_bvh_node n : bottom) {
+        if (!(n.prim_index & RH_LEAF_NODE_BIT)) {
+            n.prim_index += bottom_base;
+            n.prim_count += bottom_base;
+        }
+        out_nodes.push_back(n);
+    }
+    return (uint32_t)(out_nodes.size() - base);
+}
+
+void TopLevelSAH(const std::vector<HlbvhTreelet> &treelets, const std::vector<rh_bvh_node> &bottom,
+                 std::vector<rh_bvh_node> &top, std::vector<uint32_t> &top_indices) {
+    std::vector<prim_t> top_prims(treelets.size());
+    for (size_t i = 0; i < treelets.size(); i++) {
+        const rh_bvh_node &r = bottom[treelets[i].node_index];
+        top_prims[i] = prim_t{0, 0, 0, load3(r.bbox_min), load3(r.bbox_max)};
+    }
+    // every treelet its own leaf
+    bvh_settings_t s;
+    s.oversplit_threshold = std::numeric_limits<float>::max();
+    s.node_traversal_cost = 0.0f;
+    s.allow_spatial_splits = false;
+    PreprocessPrims_SAH(top_prims.data(), top_prims.size(), nullptr, 0, s, top, top_indices);
+}
+
+uint32_t PreprocessPrims_HLBVH(const prim_t *prims, size_t prims_count, std::vector<rh_bvh_node> &out_nodes,
+                               std::vector<uint32_t> &out_indices) {
+    if (prims_count == 0) return 0;
+    const uint32_t indices_start = (uint32_t)out_indices.size();
+    std::vector<uint32_t> codes(prims_count), order(prims_count);
+    MortonCodes(prims, prims_count, codes.data());
+    for (uint32_t j = 0; j < (uint32_t)prims_count; j++) order[j] = j;
+    // stable: equal codes keep their primitive order
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return codes[a] < codes[b]; });
+    std::vector<uint32_t> sorted(prims_count);
+    for (size_t j = 0; j < prims_count; j++) sorted[j] = codes[order[j]];
+    out_indices.insert(out_indices.end(), order.begin(), order.end());
+
+    std::vector<HlbvhTreelet> treelets;
+    std::vector<rh_bvh_node> bottom;
+    for (uint32_t start = 0; start < (uint32_t)prims_count;) {
+        uint32_t end = start + 1;
+        while (end < (uint32_t)prims_count && (sorted[end] & kTreeletMask) == (sorted[start] & kTreeletMask)) end++;
+        treelets.push_back({start, end - start, 0});
+        start = end;
+    }
+    for (HlbvhTreelet &t : treelets)
+        t.node_index = EmitLBVH(prims, &out_indices[indices_start], sorted.data(), t.first, t.count, indices_start,
+                                kTreeletBit, bottom);
+    std::vector<rh_bvh_node> top;
+    std::vector<uint32_t> top_indices;
+    TopLevelSAH(treelets, bottom, top, top_indices);
+    return AssembleHLBVH(treelets, bottom, top, top_indices, out_nodes);
 }
 
 void ConstructCamera(int type, int filter, int dtype, const float origin[3], const float fwd[3], const float up[3],

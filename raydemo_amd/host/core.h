@@ -36,6 +36,7 @@ struct bvh_settings_t {
     float oversplit_threshold = 0.95f;
     float node_traversal_cost = 0.025f;
     bool allow_spatial_splits = false;
+    bool use_fast_bvh_build = false;
 };
 
 // PreprocessTri (Core.cpp:162-214)
@@ -43,7 +44,7 @@ bool PreprocessTri(const float *p, int stride, rh_tri_accel *acc);
 // ExtractPlaneNormal (Core.cpp:216-228)
 void ExtractPlaneNormal(const rh_tri_accel &tri, float *out_normal);
 
-// PreprocessMesh (Core.cpp:230-275), SAH path only
+// PreprocessMesh (Core.cpp:230-275)
 uint32_t PreprocessMesh(const float *attrs, const uint32_t *vtx_indices, size_t vtx_indices_count, int layout,
                         int base_vertex, const bvh_settings_t &s, std::vector<rh_bvh_node> &out_nodes,
                         std::vector<rh_tri_accel> &out_tris, std::vector<uint32_t> &out_tri_indices);
@@ -52,6 +53,33 @@ uint32_t PreprocessMesh(const float *attrs, const uint32_t *vtx_indices, size_t 
 uint32_t PreprocessPrims_SAH(const prim_t *prims, size_t prims_count, const float *positions, size_t stride,
                              const bvh_settings_t &s, std::vector<rh_bvh_node> &out_nodes,
                              std::vector<uint32_t> &out_indices);
+
+// The fast build (PreprocessPrims_HLBVH, Core.cpp:532-680): primitives sorted
+// (stably) by the 30-bit Morton code of their centres in the mesh box
+// (10 bits per axis, Core.cpp:21-40, 549-562); runs sharing the top 12 bits
+// are treelets, each built by EmitLBVH (Core.cpp:342-429) from bit 17 down;
+// a SAH tree over the treelets' boxes (every treelet its own leaf) joins them.
+// The node array is the reference's: the top tree's interior nodes, then the
+// treelets' nodes in treelet order.
+constexpr int kMortonBitsPerAxis = 10;
+constexpr uint32_t kTreeletMask = 0x3ffc0000u; // Morton bits 18..29
+constexpr int kTreeletBit = 17;
+struct HlbvhTreelet {
+    uint32_t first, count; // range of the code-sorted primitives
+    uint32_t node_index;   // root within the treelets' node array
+};
+uint32_t F2U32_x86(float f);
+uint32_t MortonCode3(uint32_t x, uint32_t y, uint32_t z);
+void MortonCodes(const prim_t *prims, size_t n, uint32_t *codes);
+uint32_t EmitLBVH(const prim_t *prims, const uint32_t *indices, const uint32_t *codes, uint32_t first, uint32_t count,
+                  uint32_t index_offset, int bit, std::vector<rh_bvh_node> &out);
+void TopLevelSAH(const std::vector<HlbvhTreelet> &treelets, const std::vector<rh_bvh_node> &bottom,
+                 std::vector<rh_bvh_node> &top, std::vector<uint32_t> &top_indices);
+uint32_t AssembleHLBVH(const std::vector<HlbvhTreelet> &treelets, const std::vector<rh_bvh_node> &bottom,
+                       const std::vector<rh_bvh_node> &top, const std::vector<uint32_t> &top_indices,
+                       std::vector<rh_bvh_node> &out_nodes);
+uint32_t PreprocessPrims_HLBVH(const prim_t *prims, size_t prims_count, std::vector<rh_bvh_node> &out_nodes,
+                               std::vector<uint32_t> &out_indices);
 
 // FlattenBVH_Recursive (Core.cpp:683-819): appends the 8-wide form of the
 // binary tree rooted at nodes[root] to out (depth-first, a node before its

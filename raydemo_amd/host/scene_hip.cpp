@@ -268,6 +268,7 @@ uint32_t Scene::AddMesh(const mesh_desc_t &_m) {
     s.node_traversal_cost = 0.025f;
     s.oversplit_threshold = 0.95f;
     s.allow_spatial_splits = _m.allow_spatial_splits;
+    s.use_fast_bvh_build = _m.use_fast_bvh_build;
 
     m.node_index = (uint32_t)nodes_.size();
     m.node_count = RayHip::PreprocessMesh(_m.vtx_attrs, _m.vtx_indices, _m.vtx_indices_count, _m.layout,

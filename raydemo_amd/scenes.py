@@ -129,13 +129,15 @@ def cornell_geometry(seed: int = SEED_BASE + 1):
     return np.concatenate(attrs).astype(np.float32), np.concatenate(idx).astype(np.uint32), shapes, albedos
 
 
-def cornell_box(scene: "A.Scene", seed: int = SEED_BASE + 1, width=256, height=256, spp=16, depth=4) -> SceneSpec:
+def cornell_box(scene: "A.Scene", seed: int = SEED_BASE + 1, width=256, height=256, spp=16, depth=4,
+                fast_bvh=False) -> SceneSpec:
     attrs, idx, shapes, albedos = cornell_geometry(seed)
     tex = _white(scene)
     mats = [scene.add_material(A.DiffuseMaterial, tex, main_color=tuple(float(v) for v in albedos[k]))
             for k in range(7)]
     mats.append(scene.add_material(A.EmissiveMaterial, tex, main_color=(1.0, 1.0, 1.0), strength=10.0))
-    mesh = scene.add_mesh(A.PxyzNxyzTuv, attrs, idx, [(mats[k], s, c) for k, s, c in shapes])
+    mesh = scene.add_mesh(A.PxyzNxyzTuv, attrs, idx, [(mats[k], s, c) for k, s, c in shapes],
+                          use_fast_bvh_build=fast_bvh)
     scene.add_mesh_instance(mesh)
     scene.set_environment((0.0, 0.0, 0.0))
     cam = scene.add_camera(origin=(0.0, 5.0, -14.0), fwd=(0.0, 0.0, 1.0), fov=45.0, filter=A.Box,
@@ -296,7 +298,7 @@ def soup_geometry(seed: int = SEED_BASE + 2, n_spheres: int = 512, n_fine: int =
 
 
 def triangle_soup(scene: "A.Scene", seed: int = SEED_BASE + 2, width=1920, height=1080, spp=64, depth=8,
-                  n_spheres=512, n_fine=38) -> SceneSpec:
+                  n_spheres=512, n_fine=38, fast_bvh=False) -> SceneSpec:
     attrs, idx, spans, kind, albedo, rough = soup_geometry(seed, n_spheres, n_fine)
     tex = _white(scene)
     mats = []
@@ -308,7 +310,7 @@ def triangle_soup(scene: "A.Scene", seed: int = SEED_BASE + 2, width=1920, heigh
             mats.append(scene.add_material(A.GlossyMaterial, tex, main_color=col, roughness=float(rough[s])))
     ground = scene.add_material(A.DiffuseMaterial, tex, main_color=(0.6, 0.6, 0.6))
     shapes = [(mats[s], spans[s][0], spans[s][1]) for s in range(len(kind))] + [(ground, spans[-1][0], spans[-1][1])]
-    mesh = scene.add_mesh(A.PxyzNxyzTuv, attrs, idx, shapes)
+    mesh = scene.add_mesh(A.PxyzNxyzTuv, attrs, idx, shapes, use_fast_bvh_build=fast_bvh)
     scene.add_mesh_instance(mesh)
     scene.set_environment((0.5, 0.5, 0.5))
     scene.add_light(A.PointLight, position=(-10.0, 25.0, -10.0), radius=0.5, color=(50.0, 50.0, 50.0))
@@ -453,12 +455,12 @@ def _random_rotation(rng) -> np.ndarray:
 
 def large_instanced(scene: "A.Scene", seed: int = SEED_BASE + 5, width=3840, height=2160, spp=16, depth=12,
                     n_meshes=16, n_instances=4096, tri_range=(1200, 80000), n_textures=32, tex_size=2048,
-                    field_size=120.0) -> SceneSpec:
+                    field_size=120.0, fast_bvh=False) -> SceneSpec:
     """The large two-level workload (see the module docstring); the keyword
     arguments scale it down for tests.  info["textures"] are the texture ids,
     info["adjoint_textures"] the 4 differentiated ones (the ground's and those
     of the three most instanced meshes), info["adjoint_texels"] their level-0
-    RGB texels."""
+    RGB texels.  fast_bvh builds the meshes' trees with the fast (HLBVH) build."""
     rng = np.random.default_rng(seed)
     white = _white(scene)
     tex, texels = [], []
@@ -483,7 +485,7 @@ def large_instanced(scene: "A.Scene", seed: int = SEED_BASE + 5, width=3840, hei
             second = scene.add_material(A.MixMaterial, white, strength=0.5, mix_materials=(d1, gl), int_ior=1.5)
         half = (len(idx) // 6) * 3
         shapes = [(diff0, 0, half), (second, half, len(idx) - half)]
-        meshes.append(scene.add_mesh(A.PxyzNxyzTuv, attrs, idx, shapes))
+        meshes.append(scene.add_mesh(A.PxyzNxyzTuv, attrs, idx, shapes, use_fast_bvh_build=fast_bvh))
         mesh_tex.append((t0, t1))
         tris_per_mesh.append(len(idx) // 3)
     tris_per_mesh = np.array(tris_per_mesh, np.float64)
@@ -507,7 +509,7 @@ def large_instanced(scene: "A.Scene", seed: int = SEED_BASE + 5, width=3840, hei
     ga = ga.astype(np.float32)
     ga[:, 6:8] *= 24.0  # tile the ground texture
     ground_mat = scene.add_material(A.DiffuseMaterial, tex[0], main_color=(1.0, 1.0, 1.0))
-    ground = scene.add_mesh(A.PxyzNxyzTuv, ga, gi, [(ground_mat, 0, 6)])
+    ground = scene.add_mesh(A.PxyzNxyzTuv, ga, gi, [(ground_mat, 0, 6)], use_fast_bvh_build=fast_bvh)
     scene.add_mesh_instance(ground)
     scene.set_environment((0.3, 0.35, 0.4))
     for lx, lz in ((-0.3, -0.3), (0.3, -0.2), (-0.2, 0.3), (0.25, 0.3)):

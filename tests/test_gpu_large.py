@@ -79,6 +79,26 @@ def test_depth_extension_adds_light(host_lib, deep_oracle):
     assert imgs[12].tobytes() == ref.tobytes()
 
 
+def test_last_row_of_halton_epoch(host_lib, deep_oracle):
+    """Pass 255 of an epoch is the table's last row; at depth 12 its deep
+    bounces read past that row into the table's extra row (a copy of row 0).
+    256 passes, the next epoch's table included, equal the oracle's bit for bit."""
+    w, h, spp, depth = 8, 8, 257, 12
+    r = A.Renderer(host_lib, w, h, A.RENDERER_HIP, use_wide_bvh=True)
+    r.set_option("max_bounces", depth)
+    s = r.create_scene()
+    scenes.cornell_refined(s, k=1, width=w, height=h, spp=spp, depth=depth)
+    reg = A.Region(host_lib, 0, 0, w, h)
+    for _ in range(spp):
+        r.render(s, reg)
+    acc = np.zeros((h, w, 4), np.float32)
+    r.read_accum(acc.ctypes.data)
+    hs = standalone_scene(host_lib, wide=True)
+    scenes.cornell_refined(hs, k=1, width=w, height=h, spp=spp, depth=depth)
+    ref = deep_oracle.render_linear(hs.raw_view(), hs.camera_record(0), w, h, spp)
+    assert acc.tobytes() == ref.tobytes()
+
+
 def test_large_texel_gradients_match_oracle(host_lib, deep_oracle):
     from raydemo_amd import adjoint
     r = A.Renderer(host_lib, W, H, A.RENDERER_HIP, use_wide_bvh=True)
