@@ -510,6 +510,45 @@ def run_large(args, lib, A, scenes, knobs, wide):
     return out
 
 
+def run_build(args, lib, A, scenes):
+    """The fast (HLBVH) build, mesh_desc_t::use_fast_bvh_build, of the tree
+    over the triangle boxes of a soup of about 1M triangles: on the host, and
+    with the treelets built on the GPU (the SAH top tree over at most 4096
+    treelet roots stays on the host); best of 3 each, identical output
+    required.  Also the whole AddMesh of that soup (triangle records, vertices
+    and tangent frames included) for a standalone scene and a renderer's."""
+    import numpy as np
+    attrs, idx, spans, kind, albedo, rough = scenes.soup_geometry(scenes.SEED_BASE + 9, args.build_spheres, 38)
+    n_tris = len(idx) // 3
+    P = attrs[:, :3][idx].reshape(-1, 3, 3)
+    boxes = np.ascontiguousarray(np.concatenate([P.min(1), P.max(1)], axis=1), np.float32)
+    out = {"triangles": n_tris}
+    res = {}
+    for where, dev in (("host", -1), ("gpu", 0)):
+        ts = []
+        for _ in range(3):
+            t = time.perf_counter()
+            res[where] = A.build_fast_bvh(lib, boxes, device=dev)
+            ts.append(time.perf_counter() - t)
+        out[f"{where}_ms"] = round(min(ts) * 1e3, 2)
+    out["identical"] = all(a.tobytes() == b.tobytes() for a, b in zip(res["host"], res["gpu"]))
+    out["nodes"] = len(res["host"][0])
+    out["speedup"] = round(out["host_ms"] / out["gpu_ms"], 2)
+    out["mprims_per_s_gpu"] = round(n_tris / out["gpu_ms"] / 1e3, 1)
+    r = A.Renderer(lib, 16, 16, A.RENDERER_HIP, use_wide_bvh=False)
+    for where in ("host", "gpu"):
+        sc = A.standalone_scene(lib, False) if where == "host" else r.create_scene()
+        tex = sc.add_texture(np.full((1, 1, 4), 255, np.uint8), is_srgb=False, generate_mipmaps=False)
+        mat = sc.add_material(A.DiffuseMaterial, tex, main_color=(0.5, 0.5, 0.5))
+        t = time.perf_counter()
+        if sc.add_mesh(A.PxyzNxyzTuv, attrs, idx, [(mat, 0, len(idx))], use_fast_bvh_build=True) == A.INVALID:
+            raise RuntimeError("fast build failed")
+        out[f"add_mesh_{where}_ms"] = round((time.perf_counter() - t) * 1e3, 1)
+    r.close()
+    log(f"fast build: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -538,6 +577,8 @@ def main():
     ap.add_argument("--large-spp", type=int, default=16)
     ap.add_argument("--large-adj-steps", type=int, default=5, help="texel-gradient descent steps (0: skip)")
     ap.add_argument("--large-batch-scenes", type=int, default=16, help="small scenes batched with it (0: skip)")
+    ap.add_argument("--build-spheres", type=int, default=3000,
+                    help="icospheres of the fast-build mesh (about 390 triangles each; 0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -619,6 +660,7 @@ def main():
     if batch is not None:
         args.large_batch_knobs = {k: v for k, v in batch["knobs"].items() if k != "use_graph"}
     large = run_large(args, lib, A, scenes, plain_knobs, wide) if args.large else None
+    build = run_build(args, lib, A, scenes) if args.build_spheres > 0 else None
 
     line = {
         "metric": "Mrays/s at 1080p depth-8 (triangle soup, primary+secondary+shadow rays)",
@@ -652,6 +694,7 @@ def main():
         "adjoint": adj,
         "batch": batch,
         "large": large,
+        "fast_build": build,
     }
     print(json.dumps(line), flush=True)
 

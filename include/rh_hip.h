@@ -162,6 +162,27 @@ int rh_batch_set_option(rh_batch *batch, const char *name, int value);
  * rays/hits are device pointers in the split layout (rh_ray / rh_hit). */
 int rh_trace_device(rh_context *ctx, const rh_ray *d_rays, int n, int any_hit, rh_hit *d_hits, void *stream);
 
+/* ---- the fast BVH build on the device ----
+ * Replaces the bottom half of PreprocessPrims_HLBVH (Core.cpp:532-596) with
+ * EmitLBVH (Core.cpp:342-429): Morton codes of the n primitive boxes
+ * (prims: 6 floats each, bbox_min xyz then bbox_max xyz), their stable sort,
+ * the treelets (runs sharing the top 12 code bits) and each treelet's tree.
+ * Outputs, equal to the reference's: out_order[n], the primitives in sorted
+ * order; out_nodes, the treelets' nodes in the reference's order (interior
+ * child links relative to out_nodes[0], leaves at index_offset + sorted
+ * position), *out_node_count of them (nodes_cap >= 2 n); out_treelets, 3
+ * words per treelet (first sorted position, count, root node), at most 4096,
+ * *out_treelet_count of them.  The host builds the SAH top tree over the
+ * treelet roots (Core.cpp:598-680).  A builder owns a stream and scratch
+ * buffers that grow to the largest mesh built; calls on one builder
+ * serialise. */
+typedef struct rh_builder rh_builder;
+int rh_builder_create(int device, rh_builder **out);
+void rh_builder_destroy(rh_builder *b);
+int rh_build_lbvh(rh_builder *b, const float *prims, size_t n, uint32_t index_offset, uint32_t *out_order,
+                  rh_bvh_node *out_nodes, size_t nodes_cap, uint32_t *out_node_count, uint32_t *out_treelets,
+                  uint32_t *out_treelet_count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -136,6 +136,17 @@ int rhx_create_scene(rhx_renderer *r, rhx_scene **out);
 int rhx_create_scene_standalone(int use_wide_bvh, rhx_scene **out);
 void rhx_destroy_scene(rhx_scene *s);
 
+/* The fast (HLBVH) tree over n caller boxes (6 floats each: bbox_min xyz,
+ * bbox_max xyz), as PreprocessPrims_HLBVH (Core.cpp:532-680) builds a mesh's
+ * tree from its triangles' boxes: on the host (device < 0) or with the
+ * treelets built on GPU `device`.  out_nodes (capacity nodes_cap >= 2 n)
+ * receives *out_count nodes, root first, leaves indexing out_indices[n] (the
+ * boxes in Morton order).  Scenes use the same build for meshes with
+ * use_fast_bvh_build; this entry point has no counterpart in the reference's
+ * API (it exposes that build for any boxes). */
+int rhx_build_fast_bvh(const float *boxes, size_t n, int device, rh_bvh_node *out_nodes, size_t nodes_cap,
+                       uint32_t *out_count, uint32_t *out_indices);
+
 int rhx_region_create(int x, int y, int w, int h, rhx_region **out);
 void rhx_region_destroy(rhx_region *reg);
 int rhx_region_iteration(const rhx_region *reg);

@@ -46,7 +46,7 @@ def hipcc() -> str:
 
 
 def build_hip():
-    srcs = [CSRC / "rh_kernels.hip", CSRC / "rh_batch.hip"]
+    srcs = [CSRC / "rh_kernels.hip", CSRC / "rh_batch.hip", CSRC / "rh_bvh_build.hip"]
     tmp = HIP_LIB.with_suffix(".so.tmp")
     _run([hipcc(), *HIPCC_FLAGS, f"-I{INC}", f"-I{CSRC}", *srcs, "-o", tmp])
     os.replace(tmp, HIP_LIB)

@@ -226,6 +226,7 @@ class RayLib:
             "rhx_set_material_colors": (c_int, [P, P, c_uint32, c_uint32, P]),
             "rhx_read_accum": (c_int, [P, P, c_size_t]),
             "rhx_set_gradient_textures": (c_int, [P, P, c_uint32]),
+            "rhx_build_fast_bvh": (c_int, [P, c_size_t, c_int, P, c_size_t, POINTER(c_uint32), P]),
         }
         optional.update({
             "rhx_batch_create": (c_int, [c_int, POINTER(P)]),
@@ -378,6 +379,20 @@ class _NoRenderer:
         self.lib = lib
         self.h = None
         self.wide = wide
+
+
+def build_fast_bvh(lib: RayLib, boxes: np.ndarray, device: int = -1):
+    """The fast (HLBVH) tree over boxes (n x 6: bbox_min, bbox_max) on the
+    host (device < 0) or with its treelets built on GPU `device`: (nodes,
+    indices), nodes in NODE_DTYPE with the root first."""
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    n = len(boxes)
+    nodes = np.zeros(max(1, 2 * n), NODE_DTYPE)
+    idx = np.zeros(max(1, n), np.uint32)
+    count = c_uint32(0)
+    lib.check(lib.lib.rhx_build_fast_bvh(_ptr(boxes), n, int(device), _ptr(nodes), len(nodes), C.byref(count),
+                                         _ptr(idx)), "rhx_build_fast_bvh")
+    return nodes[:count.value], idx[:n]
 
 
 def standalone_scene(lib: RayLib, wide: bool = False) -> "Scene":

@@ -1,7 +1,9 @@
 // Flat C ABI (include/rh_scene.h) over Ray::Hip::Renderer / Ray::Hip::Scene.
 // Each function is a thin forwarder to the C++ API method named in the header.
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 
@@ -149,6 +151,40 @@ int rhx_create_scene_standalone(int use_wide_bvh, rhx_scene **out) {
 }
 void rhx_destroy_scene(rhx_scene *s) { delete s; }
 
+int rhx_build_fast_bvh(const float *boxes, size_t n, int device, rh_bvh_node *out_nodes, size_t nodes_cap,
+                       uint32_t *out_count, uint32_t *out_indices) {
+    return guarded([&] {
+        *out_count = 0;
+        if (nodes_cap < 2 * n) throw std::runtime_error("node output must hold 2 n nodes");
+        std::vector<RayHip::prim_t> prims(n);
+        for (size_t i = 0; i < n; i++) {
+            prims[i].i0 = prims[i].i1 = prims[i].i2 = 0;
+            memcpy(prims[i].bbox_min.v, boxes + 6 * i, 3 * sizeof(float));
+            memcpy(prims[i].bbox_max.v, boxes + 6 * i + 3, 3 * sizeof(float));
+        }
+        std::vector<rh_bvh_node> nodes;
+        std::vector<uint32_t> indices;
+        if (device < 0) {
+            *out_count = RayHip::PreprocessPrims_HLBVH(prims.data(), n, nodes, indices);
+        } else {
+            // one builder per device, kept for the process
+            static std::mutex mu;
+            static std::map<int, rh_builder *> builders;
+            std::lock_guard<std::mutex> lock(mu);
+            rh_builder *&b = builders[device];
+            if (!b && rh_builder_create(device, &b) != 0) {
+                b = nullptr;
+                throw std::runtime_error(std::string("fast BVH build: ") + rh_last_error());
+            }
+            const RayHip::LbvhBottom fn = Ray::Hip::DeviceLbvhBottom(&b, device);
+            *out_count = RayHip::PreprocessPrims_HLBVH(prims.data(), n, nodes, indices, &fn);
+        }
+        memcpy(out_nodes, nodes.data(), nodes.size() * sizeof(rh_bvh_node));
+        memcpy(out_indices, indices.data(), indices.size() * sizeof(uint32_t));
+        return 0;
+    });
+}
+
 int rhx_region_create(int x, int y, int w, int h, rhx_region **out) {
     *out = new rhx_region(Ray::rect_t{x, y, w, h});
     return 0;
@@ -276,7 +312,12 @@ uint32_t rhx_add_mesh(rhx_scene *s, int layout, const float *attrs, size_t attrs
         d.shapes.emplace_back(shapes[i].mat_index, shapes[i].back_mat_index, (size_t)shapes[i].vtx_start,
                               (size_t)shapes[i].vtx_count);
     }
-    return s->s->AddMesh(d);
+    try {
+        return s->s->AddMesh(d);
+    } catch (const std::exception &e) {
+        g_err = e.what(); // a device build that failed: the mesh is not added
+        return 0xffffffffu;
+    }
 }
 
 void rhx_remove_mesh(rhx_scene *s, uint32_t i) { s->s->RemoveMesh(i); }

@@ -187,7 +187,8 @@ void ExtractPlaneNormal(const rh_tri_accel &tri, float *out_normal) {
 
 uint32_t PreprocessMesh(const float *attrs, const uint32_t *vtx_indices, size_t vtx_indices_count, int layout,
                         int base_vertex, const bvh_settings_t &s, std::vector<rh_bvh_node> &out_nodes,
-                        std::vector<rh_tri_accel> &out_tris, std::vector<uint32_t> &out_tri_indices) {
+                        std::vector<rh_tri_accel> &out_tris, std::vector<uint32_t> &out_tri_indices,
+                        const LbvhBottom *device) {
     static const size_t attr_strides[] = {8, 10, 11, 13};
     std::vector<prim_t> primitives;
     primitives.reserve(vtx_indices_count / 3);
@@ -216,7 +217,7 @@ uint32_t PreprocessMesh(const float *attrs, const uint32_t *vtx_indices, size_t 
     const size_t indices_start = out_tri_indices.size();
     const uint32_t num_out_nodes =
         s.use_fast_bvh_build
-            ? PreprocessPrims_HLBVH(primitives.data(), primitives.size(), out_nodes, out_tri_indices)
+            ? PreprocessPrims_HLBVH(primitives.data(), primitives.size(), out_nodes, out_tri_indices, device)
             : PreprocessPrims_SAH(primitives.data(), primitives.size(), positions, attr_stride, s, out_nodes,
                                   out_tri_indices);
     for (size_t i = indices_start; i < out_tri_indices.size(); i++) out_tri_indices[i] += (uint32_t)tris_start;
@@ -442,9 +443,19 @@ void TopLevelSAH(const std::vector<HlbvhTreelet> &treelets, const std::vector<rh
 }
 
 uint32_t PreprocessPrims_HLBVH(const prim_t *prims, size_t prims_count, std::vector<rh_bvh_node> &out_nodes,
-                               std::vector<uint32_t> &out_indices) {
+                               std::vector<uint32_t> &out_indices, const LbvhBottom *device) {
     if (prims_count == 0) return 0;
     const uint32_t indices_start = (uint32_t)out_indices.size();
+    if (device) {
+        std::vector<uint32_t> order;
+        std::vector<HlbvhTreelet> treelets;
+        std::vector<rh_bvh_node> bottom, top;
+        std::vector<uint32_t> top_indices;
+        (*device)(prims, prims_count, indices_start, order, bottom, treelets);
+        out_indices.insert(out_indices.end(), order.begin(), order.end());
+        TopLevelSAH(treelets, bottom, top, top_indices);
+        return AssembleHLBVH(treelets, bottom, top, top_indices, out_nodes);
+    }
     std::vector<uint32_t> codes(prims_count), order(prims_count);
     MortonCodes(prims, prims_count, codes.data());
     for (uint32_t j = 0; j < (uint32_t)prims_count; j++) order[j] = j;

@@ -7,6 +7,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 #include "rh_types.h"
@@ -44,10 +45,21 @@ bool PreprocessTri(const float *p, int stride, rh_tri_accel *acc);
 // ExtractPlaneNormal (Core.cpp:216-228)
 void ExtractPlaneNormal(const rh_tri_accel &tri, float *out_normal);
 
-// PreprocessMesh (Core.cpp:230-275)
+struct HlbvhTreelet;
+// The bottom half of the fast build done elsewhere (on the device): given the
+// primitives and the index offset of their first sorted position, fills
+// order (the primitives in Morton order), bottom (the treelets' nodes) and
+// treelets exactly as the host's own steps would; throws on failure.
+using LbvhBottom = std::function<void(const prim_t *prims, size_t n, uint32_t index_offset,
+                                      std::vector<uint32_t> &order, std::vector<rh_bvh_node> &bottom,
+                                      std::vector<HlbvhTreelet> &treelets)>;
+
+// PreprocessMesh (Core.cpp:230-275); with s.use_fast_bvh_build the tree is
+// built by PreprocessPrims_HLBVH, its bottom half by `device` when given.
 uint32_t PreprocessMesh(const float *attrs, const uint32_t *vtx_indices, size_t vtx_indices_count, int layout,
                         int base_vertex, const bvh_settings_t &s, std::vector<rh_bvh_node> &out_nodes,
-                        std::vector<rh_tri_accel> &out_tris, std::vector<uint32_t> &out_tri_indices);
+                        std::vector<rh_tri_accel> &out_tris, std::vector<uint32_t> &out_tri_indices,
+                        const LbvhBottom *device = nullptr);
 
 // PreprocessPrims_SAH (Core.cpp:431-530) + SplitPrimitives_SAH (BVHSplit.cpp:126-390)
 uint32_t PreprocessPrims_SAH(const prim_t *prims, size_t prims_count, const float *positions, size_t stride,
@@ -79,7 +91,7 @@ uint32_t AssembleHLBVH(const std::vector<HlbvhTreelet> &treelets, const std::vec
                        const std::vector<rh_bvh_node> &top, const std::vector<uint32_t> &top_indices,
                        std::vector<rh_bvh_node> &out_nodes);
 uint32_t PreprocessPrims_HLBVH(const prim_t *prims, size_t prims_count, std::vector<rh_bvh_node> &out_nodes,
-                               std::vector<uint32_t> &out_indices);
+                               std::vector<uint32_t> &out_indices, const LbvhBottom *device = nullptr);
 
 // FlattenBVH_Recursive (Core.cpp:683-819): appends the 8-wide form of the
 // binary tree rooted at nodes[root] to out (depth-first, a node before its

@@ -64,6 +64,13 @@ class Scene : public SceneBase {
     uint32_t wide_macro_root_ = 0xffffffff, wide_light_root_ = 0xffffffff;
     void RebuildWide();
 
+    // fast (HLBVH) builds run on the device when the scene comes from a
+    // renderer; a standalone scene builds on the host
+    bool device_build_ = false;
+    int build_device_ = -1;
+    rh_builder *builder_ = nullptr;
+    RayHip::LbvhBottom device_bottom_;
+
     void RemoveNodes(uint32_t node_index, uint32_t node_count);
     void WriteBlock(const rh_tex_mip &m, const std::vector<uint32_t> &data, const int res[2]);
     void RebuildMacroBVH();
@@ -71,8 +78,12 @@ class Scene : public SceneBase {
     int AllocateInAtlas(const int res[2], int pos[2]);
 
 public:
-    explicit Scene(bool use_wide_bvh = false);
+    explicit Scene(bool use_wide_bvh = false, bool device_build = false, int device = -1);
+    ~Scene() override;
+    Scene(const Scene &) = delete;
+    Scene &operator=(const Scene &) = delete;
     bool wide() const { return wide_; }
+    bool device_build() const { return device_build_; }
 
     void GetEnvironment(environment_desc_t &env) override;
     void SetEnvironment(const environment_desc_t &env) override;
@@ -109,8 +120,13 @@ public:
     const camera_t &camera(uint32_t i) const { return cams_[i].cam; }
 };
 
+// The fast build's bottom half (Morton sort, treelets) on GPU `device`, with
+// the builder created at first use in *builder (kept by the caller).
+RayHip::LbvhBottom DeviceLbvhBottom(rh_builder **builder, int device);
+
 class Renderer : public RendererBase {
     rh_context *ctx_ = nullptr;
+    int device_ = -1;
     int w_ = 0, h_ = 0;
     bool sort_rays_ = true;
     bool use_wide_bvh_ = false;

@@ -19,6 +19,7 @@ Renderer::Renderer(const settings_t &s, int device_index) {
     if (rc != 0) {
         throw std::runtime_error(std::string("HIP backend: ") + rh_last_error());
     }
+    device_ = device_index;
     permutations_ = RayHip::ComputeRadicalInversePermutations();
     use_wide_bvh_ = s.use_wide_bvh;
     Resize(s.w, s.h);
@@ -52,7 +53,7 @@ const pixel_color_t *Renderer::get_pixels_ref() const {
     return host_pixels_.data();
 }
 
-std::shared_ptr<SceneBase> Renderer::CreateScene() { return std::make_shared<Scene>(use_wide_bvh_); }
+std::shared_ptr<SceneBase> Renderer::CreateScene() { return std::make_shared<Scene>(use_wide_bvh_, true, device_); }
 
 // Uploads the scene if it changed, advances the region's iteration and
 // refreshes/uploads its Halton table exactly as the Ref renderer does

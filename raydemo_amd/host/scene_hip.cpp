@@ -98,7 +98,9 @@ namespace Hip {
 
 static_assert(sizeof(camera_t) == sizeof(rh_camera), "camera_t layout");
 
-Scene::Scene(bool use_wide_bvh) : wide_(use_wide_bvh) {
+Scene::Scene(bool use_wide_bvh, bool device_build, int device)
+    : wide_(use_wide_bvh), device_build_(device_build), build_device_(device) {
+    device_bottom_ = DeviceLbvhBottom(&builder_, build_device_);
     const int atlas_res[2] = {RH_TEXTURE_ATLAS_SIZE, RH_TEXTURE_ATLAS_SIZE};
     splitters_.emplace_back(atlas_res);
 
@@ -258,6 +260,36 @@ uint32_t Scene::AddMaterial(const mat_desc_t &m) {
 }
 
 // SceneRef::AddMesh (SceneRef.cpp:156-263), binary BVH
+RayHip::LbvhBottom DeviceLbvhBottom(rh_builder **builder, int device) {
+    return [builder, device](const RayHip::prim_t *prims, size_t n, uint32_t index_offset,
+                             std::vector<uint32_t> &order, std::vector<rh_bvh_node> &bottom,
+                             std::vector<RayHip::HlbvhTreelet> &treelets) {
+        if (!*builder && rh_builder_create(device, builder) != 0) {
+            *builder = nullptr;
+            throw std::runtime_error(std::string("fast BVH build: ") + rh_last_error());
+        }
+        std::vector<float> boxes(6 * n);
+        for (size_t i = 0; i < n; i++) {
+            memcpy(&boxes[6 * i], prims[i].bbox_min.v, 3 * sizeof(float));
+            memcpy(&boxes[6 * i + 3], prims[i].bbox_max.v, 3 * sizeof(float));
+        }
+        order.resize(n);
+        bottom.resize(2 * n);
+        std::vector<uint32_t> tl(3 * 4096);
+        uint32_t nodes = 0, count = 0;
+        if (rh_build_lbvh(*builder, boxes.data(), n, index_offset, order.data(), bottom.data(), bottom.size(), &nodes,
+                          tl.data(), &count) != 0)
+            throw std::runtime_error(std::string("fast BVH build: ") + rh_last_error());
+        bottom.resize(nodes);
+        treelets.resize(count);
+        for (uint32_t t = 0; t < count; t++) treelets[t] = {tl[3 * t], tl[3 * t + 1], tl[3 * t + 2]};
+    };
+}
+
+Scene::~Scene() {
+    if (builder_) rh_builder_destroy(builder_);
+}
+
 uint32_t Scene::AddMesh(const mesh_desc_t &_m) {
     if (!_m.vtx_indices_count || _m.vtx_indices_count % 3) return 0xffffffff;
     meshes_.emplace_back();
@@ -271,8 +303,18 @@ uint32_t Scene::AddMesh(const mesh_desc_t &_m) {
     s.use_fast_bvh_build = _m.use_fast_bvh_build;
 
     m.node_index = (uint32_t)nodes_.size();
-    m.node_count = RayHip::PreprocessMesh(_m.vtx_attrs, _m.vtx_indices, _m.vtx_indices_count, _m.layout,
-                                          _m.base_vertex, s, nodes_, tris_, tri_indices_);
+    const size_t nodes_before = nodes_.size(), tri_indices_before = tri_indices_.size();
+    try {
+        m.node_count = RayHip::PreprocessMesh(_m.vtx_attrs, _m.vtx_indices, _m.vtx_indices_count, _m.layout,
+                                              _m.base_vertex, s, nodes_, tris_, tri_indices_,
+                                              device_build_ ? &device_bottom_ : nullptr);
+    } catch (...) {
+        nodes_.resize(nodes_before);
+        tris_.resize(tris_start);
+        tri_indices_.resize(tri_indices_before);
+        meshes_.pop_back();
+        throw;
+    }
 
     // triangle materials and the solid bit (no transparent material reachable)
     for (const shape_desc_t &sh : _m.shapes) {

@@ -129,3 +129,19 @@ def test_fast_build_invariants(host_lib):
     assert sorted(leaves) == list(range(int(mesh["tris_count"])))
     lo, hi = int(mesh["node_index"]), int(mesh["node_index"] + mesh["node_count"])
     assert seen[lo:hi].tolist() == [1] * (hi - lo)
+
+
+def test_fast_bvh_over_boxes_equals_mesh_build(host_lib):
+    """rhx_build_fast_bvh over a mesh's triangle boxes gives the mesh's tree
+    and index order (the mesh being the scene's first, both start at 0)."""
+    for name in ("many_treelets", "flat_y", "seven"):
+        attrs, idx = CASES[name]
+        hs = standalone_scene(host_lib)
+        _mesh_scene(hs, attrs, idx)
+        v = hs.view()
+        m = v["meshes"][0]
+        P = attrs[:, :3][idx].reshape(-1, 3, 3)
+        boxes = np.concatenate([P.min(1), P.max(1)], axis=1)
+        nodes, order = A.build_fast_bvh(host_lib, boxes)
+        assert nodes.tobytes() == v["nodes"][m["node_index"]:m["node_index"] + m["node_count"]].tobytes(), name
+        assert order.tobytes() == v["tri_indices"].tobytes(), name
