@@ -55,6 +55,12 @@ int rh_upload_scene(rh_context *ctx, const rh_scene_view *scene);
 int rh_set_halton(rh_context *ctx, const float *seq, size_t count);
 int rh_render_pass(rh_context *ctx, const rh_pass_desc *pass);
 int rh_read_pixels(rh_context *ctx, float *rgba, size_t count);
+/* The accumulated spherical-harmonic data of passes with OutputSH
+ * (RendererBase::get_sh_data_ref, RendererRef.h:64-66; FramebufferRef.cpp:
+ * 57-94): w*h records of 12 floats (coeff_r[4], coeff_g[4], coeff_b[4]).
+ * Returns 1 (and writes nothing) when no pass has produced SH data since the
+ * last resize. */
+int rh_read_sh(rh_context *ctx, float *out, size_t count);
 int rh_synchronize(rh_context *ctx);
 int rh_get_stats(rh_context *ctx, rh_stats *out);
 int rh_reset_stats(rh_context *ctx);

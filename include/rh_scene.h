@@ -118,6 +118,10 @@ uint32_t rhx_renderer_type(const rhx_renderer *r);
 int rhx_resize(rhx_renderer *r, int w, int h);
 int rhx_clear(rhx_renderer *r, const float rgba[4]);
 int rhx_get_pixels(rhx_renderer *r, float *out_rgba, size_t count);
+/* The accumulated SH data (RendererBase::get_sh_data_ref, RendererBase.h:69):
+ * w*h records of 12 floats (shl1_data_t); returns 1 when no pass with
+ * OutputSH has run since the last resize. */
+int rhx_get_sh_data(rhx_renderer *r, float *out, size_t count);
 int rhx_get_stats(rhx_renderer *r, rh_stats *out);
 int rhx_reset_stats(rhx_renderer *r);
 int rhx_get_ray_counts(rhx_renderer *r, rh_ray_counts *out);

@@ -1183,9 +1183,116 @@ int orc_set_max_bounces(int n) {
     return 0;
 }
 
+// float -> int as x86 cvttss2si: NaN and out-of-range values give INT_MIN
+static int f2i_x86(float f) {
+    if (!(f > -2147483904.0f && f < 2147483648.0f)) return INT32_MIN;
+    return (int)f;
+}
+
+// SampleMeshInTextureSpace (CoreRef.cpp:507-613): the region's rays and hits
+// for a geometry camera, triangle by triangle in order, the first triangle
+// whose texture-space footprint covers a pixel's jittered sample taking it
+static void sample_mesh_in_texture_space(const rh_scene_view *sc, const rh_camera *cam, int w, int h,
+                                         const int32_t r[4], int iteration, const float *halton,
+                                         rh_ray_packet *rays, rh_hit *hits) {
+    for (int y = r[1]; y < r[1] + r[3]; y++) {
+        for (int x = r[0]; x < r[0] + r[2]; x++) {
+            const int i = (y - r[1]) * r[2] + (x - r[0]);
+            memset(&rays[i], 0, sizeof(rays[i]));
+            rays[i].xy = (x << 16) | y;
+            rays[i].c[0] = rays[i].c[1] = rays[i].c[2] = 1.0f;
+            rays[i].ior = 1.0f;
+            memset(&hits[i], 0, sizeof(hits[i]));
+            hits[i].obj = hits[i].prim = -1;
+            hits[i].t = RH_MAX_DIST;
+            hits[i].xy = rays[i].xy;
+        }
+    }
+    const rh_mesh_instance &mi = sc->mesh_instances[cam->mi_index];
+    const rh_mesh &mesh = sc->meshes[mi.mesh_index];
+    const rh_transform &tr = sc->transforms[mi.tr_index];
+    const int uv = (int)cam->uv_index;
+    const float W = (float)w, H = (float)h;
+    const int rmin[2] = {r[0], r[1]}, rmax[2] = {r[0] + r[2] - 1, r[1] + r[3] - 1};
+    for (uint32_t tri = mesh.tris_index; tri < mesh.tris_index + mesh.tris_count; tri++) {
+        const rh_vertex &v0 = sc->vertices[sc->vtx_indices[tri * 3 + 0]];
+        const rh_vertex &v1 = sc->vertices[sc->vtx_indices[tri * 3 + 1]];
+        const rh_vertex &v2 = sc->vertices[sc->vtx_indices[tri * 3 + 2]];
+        const float t0[2] = {v0.t[uv][0] * W, (1.0f - v0.t[uv][1]) * H};
+        const float t1[2] = {v1.t[uv][0] * W, (1.0f - v1.t[uv][1]) * H};
+        const float t2[2] = {v2.t[uv][0] * W, (1.0f - v2.t[uv][1]) * H};
+        float bmin[2], bmax[2];
+        int ib0[2], ib1[2];
+        for (int k = 0; k < 2; k++) {
+            bmin[k] = std::min(std::min(t0[k], t1[k]), t2[k]);
+            bmax[k] = std::max(std::max(t0[k], t1[k]), t2[k]);
+            ib0[k] = f2i_x86(bmin[k]);
+            ib1[k] = f2i_x86(std::round(bmax[k]));
+        }
+        if (ib1[0] < rmin[0] || ib1[1] < rmin[1] || ib0[0] > rmax[0] || ib0[1] > rmax[1]) continue;
+        for (int k = 0; k < 2; k++) {
+            ib0[k] = std::max(ib0[k], rmin[k]);
+            ib1[k] = std::min(ib1[k], rmax[k]);
+        }
+        const float d01[2] = {t0[0] - t1[0], t0[1] - t1[1]}, d12[2] = {t1[0] - t2[0], t1[1] - t2[1]},
+                    d20[2] = {t2[0] - t0[0], t2[1] - t0[1]};
+        const float area = d01[0] * d20[1] - d20[0] * d01[1];
+        if (area < RH_FLT_EPS) continue;
+        const float inv_area = 1.0f / area;
+        for (int y = ib0[1]; y <= ib1[1]; y++) {
+            for (int x = ib0[0]; x <= ib1[0]; x++) {
+                const int i = (y - r[1]) * r[2] + (x - r[0]);
+                if (hits[i].mask) continue;
+                const int index = y * w + x;
+                const int hi = (iteration & (RH_HALTON_SEQ_LEN - 1)) * RH_HALTON_COUNT;
+                const int hv = hash(index);
+                const float sx = float(x) + fracpart(halton[hi + 0] + construct_float((uint32_t)hv));
+                const float sy = float(y) + fracpart(halton[hi + 1] + construct_float((uint32_t)hash(hv)));
+                float u = d01[0] * (sy - t0[1]) - d01[1] * (sx - t0[0]);
+                float v = d12[0] * (sy - t1[1]) - d12[1] * (sx - t1[0]);
+                float ww = d20[0] * (sy - t2[1]) - d20[1] * (sx - t2[0]);
+                if (u >= -RH_FLT_EPS && v >= -RH_FLT_EPS && ww >= -RH_FLT_EPS) {
+                    u *= inv_area;
+                    v *= inv_area;
+                    ww *= inv_area;
+                    float pl[3], nl[3];
+                    for (int k = 0; k < 3; k++) {
+                        pl[k] = v0.p[k] * v + v1.p[k] * ww + v2.p[k] * u;
+                        nl[k] = v0.n[k] * v + v1.n[k] * ww + v2.n[k] * u;
+                    }
+                    const float *X = tr.xform, *Y = tr.inv_xform;
+                    float p[3], n[3];
+                    for (int k = 0; k < 3; k++) {
+                        p[k] = X[k] * pl[0] + X[4 + k] * pl[1] + X[8 + k] * pl[2] + X[12 + k];
+                        n[k] = Y[4 * k] * nl[0] + Y[4 * k + 1] * nl[1] + Y[4 * k + 2] * nl[2];
+                    }
+                    for (int k = 0; k < 3; k++) {
+                        rays[i].o[k] = p[k] + n[k];
+                        rays[i].d[k] = -n[k];
+                    }
+                    hits[i].mask = -1;
+                    hits[i].prim = (int)tri;
+                    hits[i].obj = (int)cam->mi_index;
+                    hits[i].t = 1.0f;
+                    hits[i].u = ww;
+                    hits[i].v = u;
+                }
+            }
+        }
+    }
+}
+
 // RendererRef::RenderScene (RendererRef.cpp:19-325) over one region
 int orc_render(const rh_scene_view *sc, const rh_camera *cam, int w, int h, const int32_t rect[4], int passes,
                int *iteration, float *clean, float *final_rgba, unsigned long long ray_counts[3]) {
+    return orc_render_sh(sc, cam, w, h, rect, passes, iteration, clean, final_rgba, ray_counts, nullptr, nullptr);
+}
+
+// ... with the OutputSH data (FramebufferRef.h:58-90, FramebufferRef.cpp:48-94)
+// in sh_temp / sh_clean (12 floats per pixel, kept by the caller between calls)
+int orc_render_sh(const rh_scene_view *sc, const rh_camera *cam, int w, int h, const int32_t rect[4], int passes,
+                  int *iteration, float *clean, float *final_rgba, unsigned long long ray_counts[3], float *sh_temp,
+                  float *sh_clean) {
     static const std::vector<uint16_t> pm = perms();
     std::vector<float> halton(RH_HALTON_TABLE_FLOATS);
     int have_halton = -1;
@@ -1229,9 +1336,13 @@ int orc_render(const rh_scene_view *sc, const rh_camera *cam, int w, int h, cons
         pi.settings.max_total_depth = (uint8_t)std::min<int>(pi.settings.max_total_depth, g_max_bounces);
         Ctx c{*sc, halton.data()};
 
-        orc_gen_primary(cam, rect, w, h, it, halton.data(), rays.data());
-        orc_trace(sc, rays.data(), n, 0, nullptr, hits.data(), nullptr);
-        if (ray_counts) ray_counts[0] += n;
+        if (cam->type == RH_CAM_GEO) {
+            sample_mesh_in_texture_space(sc, cam, w, h, rect, it, halton.data(), rays.data(), hits.data());
+        } else {
+            orc_gen_primary(cam, rect, w, h, it, halton.data(), rays.data());
+            orc_trace(sc, rays.data(), n, 0, nullptr, hits.data(), nullptr);
+            if (ray_counts) ray_counts[0] += n;
+        }
         int count = 0;
         for (size_t i = 0; i < n; i++) {
             const int x = (rays[i].xy >> 16) & 0xffff, y = rays[i].xy & 0xffff;
@@ -1242,6 +1353,20 @@ int orc_render(const rh_scene_view *sc, const rh_camera *cam, int w, int h, cons
             shade(c, pi, hits[i], rays[i], col, &sec[count], e);
             if (e) count++;
             memcpy(&temp[(size_t)pi.index * 4], col, 16);
+        }
+        const bool sh = sh_temp && (cam->pass_settings.flags & RH_OUTPUT_SH);
+        if (sh) {
+            for (int y = rect[1]; y < rect[1] + rect[3]; y++)
+                for (int x = rect[0]; x < rect[0] + rect[2]; x++) sh_temp[12 * ((size_t)y * w + x) + 4] = 0.0f;
+            for (int i = 0; i < count; i++) {
+                const int x = (sec[i].xy >> 16) & 0xffff, y = sec[i].xy & 0xffff;
+                float *s = &sh_temp[12 * ((size_t)y * w + x)];
+                s[0] = 0.282094806f;
+                s[1] = 0.488602519f * sec[i].d[1];
+                s[2] = 0.488602519f * sec[i].d[2];
+                s[3] = 0.488602519f * sec[i].d[0];
+                s[4] = sec[i].c[0];
+            }
         }
         for (int b = 0; b < pi.settings.max_total_depth && count && !(pi.settings.flags & RH_SKIP_INDIRECT_LIGHT); b++) {
             orc_sort_rays(sec.data(), (size_t)count, rmin, cell);
@@ -1273,6 +1398,20 @@ int orc_render(const rh_scene_view *sc, const rh_camera *cam, int w, int h, cons
             for (int x = rect[0]; x < rect[0] + rect[2]; x++) {
                 const size_t p = ((size_t)y * w + x) * 4;
                 for (int j = 0; j < 4; j++) clean[p + j] += (temp[p + j] - clean[p + j]) * k;
+                if (sh) {
+                    // ComputeSHData then MixWith_SH
+                    float *s = &sh_temp[3 * p], *o = &sh_clean[3 * p];
+                    const float inv_weight = s[4] > RH_FLT_EPS ? (2.0f * RH_PI / s[4]) : 0.0f;
+                    const float pr = temp[p] * inv_weight, pg = temp[p + 1] * inv_weight,
+                                pb = temp[p + 2] * inv_weight;
+                    const float coeff[4] = {s[0], s[1], s[2], s[3]};
+                    for (int j = 0; j < 4; j++) {
+                        s[4 + j] = coeff[j] * pg;
+                        s[8 + j] = coeff[j] * pb;
+                        s[j] = coeff[j] * pr;
+                    }
+                    for (int j = 0; j < 12; j++) o[j] += (s[j] - o[j]) * k;
+                }
                 float v[4] = {clean[p], clean[p + 1], clean[p + 2], clean[p + 3]};
                 if (cam->dtype == RH_DEVICE_SRGB)
                     for (int j = 0; j < 3; j++) v[j] = v[j] > 0.0031308f ? std::pow(1.055f * v[j], (1.0f / 2.4f)) - 0.055f : 12.92f * v[j];

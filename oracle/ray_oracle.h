@@ -57,6 +57,11 @@ int orc_adjoint(const rh_scene_view *sc, const rh_camera *cam, int w, int h, con
 int orc_set_max_bounces(int n);
 int orc_render(const rh_scene_view *sc, const rh_camera *cam, int w, int h, const int32_t rect[4], int passes,
                int *iteration, float *clean, float *final_rgba, unsigned long long ray_counts[3]);
+/* orc_render with the OutputSH data: sh_temp and sh_clean hold 12 floats per
+ * pixel (zeroed by the caller before the first pass, kept between calls) */
+int orc_render_sh(const rh_scene_view *sc, const rh_camera *cam, int w, int h, const int32_t rect[4], int passes,
+                  int *iteration, float *clean, float *final_rgba, unsigned long long ray_counts[3], float *sh_temp,
+                  float *sh_clean);
 
 #ifdef __cplusplus
 }

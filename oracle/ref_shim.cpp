@@ -116,6 +116,17 @@ int rhx_get_pixels(rhx_renderer *r, float *out, size_t count) {
     memcpy(out, r->r->get_pixels_ref(), n * sizeof(float));
     return 0;
 }
+int rhx_get_sh_data(rhx_renderer *r, float *out, size_t count) {
+    auto sz = r->r->size();
+    size_t n = (size_t)sz.first * sz.second * 12;
+    if (count < n) {
+        g_err = "output buffer too small";
+        return -1;
+    }
+    // the caller renders with OutputSH first (the Ref framebuffer allocates it then)
+    memcpy(out, r->r->get_sh_data_ref(), n * sizeof(float));
+    return 0;
+}
 int rhx_get_stats(rhx_renderer *r, rh_stats *out) {
     Ray::RendererBase::stats_t st;
     r->r->GetStats(st);

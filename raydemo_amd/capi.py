@@ -214,6 +214,7 @@ class RayLib:
             "rhx_node_count": (c_uint32, [P]),
             "rhx_scene_view": (c_int, [P, POINTER(SceneView)]),
             "rhx_create_scene_standalone": (c_int, [c_int, POINTER(P)]),
+            "rhx_get_sh_data": (c_int, [P, P, c_size_t]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -294,6 +295,16 @@ class Renderer:
     def pixels(self) -> np.ndarray:
         out = np.zeros((self.hgt, self.w, 4), dtype=np.float32)
         self.lib.check(self.lib.lib.rhx_get_pixels(self.h, _ptr(out), out.size), "rhx_get_pixels")
+        return out
+
+    def sh_data(self):
+        """The accumulated SH data (h, w, 12: coeff_r[4], coeff_g[4],
+        coeff_b[4]), or None before any pass with OutputSH."""
+        out = np.zeros((self.hgt, self.w, 12), dtype=np.float32)
+        rc = self.lib.lib.rhx_get_sh_data(self.h, _ptr(out), out.size)
+        if rc == 1:
+            return None
+        self.lib.check(rc, "rhx_get_sh_data")
         return out
 
     def stats(self) -> dict:

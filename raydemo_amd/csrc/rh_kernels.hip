@@ -38,6 +38,227 @@ __global__ void __launch_bounds__(kBlock) k_gen_primary(PassConst pc, const floa
     store_ray(rays, aux, (uint32_t)i, gen_primary_ray(pc, halton, x, y));
 }
 
+// ---- geometry camera: SampleMeshInTextureSpace (CoreRef.cpp:507-613) ----
+// The lightmap pixel (x, y) of the region takes the first triangle of the
+// camera's mesh instance (in triangle order) whose texture-space footprint
+// covers the pixel's jittered sample; the ray starts at the surface point
+// plus its normal and points back along the normal, with a made-up hit at
+// t = 1 on that triangle.  Here every triangle marks the pixels it covers
+// with an atomic minimum of its index, and each pixel then builds its ray
+// from the winning triangle with the same arithmetic.
+struct GeoArgs {
+    const rh_vertex *vertices;
+    const uint32_t *vtx_indices;
+    const rh_mesh_instance *mesh_instances;
+    const rh_mesh *meshes;
+    const rh_transform *transforms;
+    const float *halton;
+    uint32_t *winner;
+    int rect[4];
+    int w, h, iteration;
+    uint32_t mi_index, uv;
+};
+
+struct GeoTri {
+    float t0x, t0y, t1x, t1y, t2x, t2y;
+    float d01x, d01y, d12x, d12y, d20x, d20y, inv_area;
+    int x0, y0, x1, y1; // clipped pixel box
+};
+
+// the triangle's texture-space setup; false when it misses the region or is
+// degenerate (or back-facing) in texture space
+__device__ __forceinline__ bool geo_setup(const GeoArgs &g, uint32_t tri, GeoTri &t) {
+    const float W = (float)g.w, H = (float)g.h;
+    const rh_vertex &v0 = g.vertices[g.vtx_indices[tri * 3 + 0]], &v1 = g.vertices[g.vtx_indices[tri * 3 + 1]],
+                    &v2 = g.vertices[g.vtx_indices[tri * 3 + 2]];
+    t.t0x = v0.t[g.uv][0] * W;
+    t.t0y = (1.0f - v0.t[g.uv][1]) * H;
+    t.t1x = v1.t[g.uv][0] * W;
+    t.t1y = (1.0f - v1.t[g.uv][1]) * H;
+    t.t2x = v2.t[g.uv][0] * W;
+    t.t2y = (1.0f - v2.t[g.uv][1]) * H;
+    const float bminx = smin(smin(t.t0x, t.t1x), t.t2x), bminy = smin(smin(t.t0y, t.t1y), t.t2y);
+    const float bmaxx = smax(smax(t.t0x, t.t1x), t.t2x), bmaxy = smax(smax(t.t0y, t.t1y), t.t2y);
+    int x0 = f2i_trunc(bminx), y0 = f2i_trunc(bminy);
+    int x1 = f2i_trunc(__builtin_roundf(bmaxx)), y1 = f2i_trunc(__builtin_roundf(bmaxy));
+    const int rx0 = g.rect[0], ry0 = g.rect[1], rx1 = g.rect[0] + g.rect[2] - 1, ry1 = g.rect[1] + g.rect[3] - 1;
+    if (x1 < rx0 || y1 < ry0 || x0 > rx1 || y0 > ry1) return false;
+    t.x0 = max(x0, rx0);
+    t.y0 = max(y0, ry0);
+    t.x1 = min(x1, rx1);
+    t.y1 = min(y1, ry1);
+    t.d01x = t.t0x - t.t1x;
+    t.d01y = t.t0y - t.t1y;
+    t.d12x = t.t1x - t.t2x;
+    t.d12y = t.t1y - t.t2y;
+    t.d20x = t.t2x - t.t0x;
+    t.d20y = t.t2y - t.t0y;
+    const float area = t.d01x * t.d20y - t.d20x * t.d01y;
+    if (area < RH_FLT_EPS) return false;
+    t.inv_area = 1.0f / area;
+    return true;
+}
+
+// the edge functions of pixel (x, y)'s jittered sample; true when inside
+__device__ __forceinline__ bool geo_sample(const GeoArgs &g, const GeoTri &t, int x, int y, float &u, float &v,
+                                           float &w) {
+    const int index = y * g.w + x;
+    const int hi = (g.iteration & (RH_HALTON_SEQ_LEN - 1)) * RH_HALTON_COUNT;
+    const int hash_val = hash(index);
+    const float sx = (float)x + fract_modf(g.halton[hi + 0] + construct_float((uint32_t)hash_val));
+    const float sy = (float)y + fract_modf(g.halton[hi + 1] + construct_float((uint32_t)hash(hash_val)));
+    u = t.d01x * (sy - t.t0y) - t.d01y * (sx - t.t0x);
+    v = t.d12x * (sy - t.t1y) - t.d12y * (sx - t.t1x);
+    w = t.d20x * (sy - t.t2y) - t.d20y * (sx - t.t2x);
+    return u >= -RH_FLT_EPS && v >= -RH_FLT_EPS && w >= -RH_FLT_EPS;
+}
+
+// every lane of the region: white throughput, no hit yet
+__global__ void __launch_bounds__(kBlock) k_geo_init(GeoArgs g, rh_ray *__restrict__ rays, rh_ray_aux *__restrict__ aux,
+                                                    rh_hit *__restrict__ hits, int *__restrict__ count) {
+    const int rw = g.rect[2], n = rw * g.rect[3];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *count = n;
+    if (i >= n) return;
+    const int x = g.rect[0] + i % rw, y = g.rect[1] + i / rw;
+    RayFull r;
+    memset(&r, 0, sizeof(r));
+    r.c = f3{1.0f, 1.0f, 1.0f};
+    r.ior = 1.0f;
+    r.xy = (x << 16) | y;
+    store_ray(rays, aux, (uint32_t)i, r);
+    rh_hit hh;
+    hh.mask = 0;
+    hh.obj = -1;
+    hh.prim = -1;
+    hh.t = RH_MAX_DIST;
+    hh.u = hh.v = 0.0f;
+    hh.xy = r.xy;
+    hh.pad = 0;
+    hits[i] = hh;
+    g.winner[i] = 0xffffffffu;
+}
+
+// one wave per triangle (a grid-stride loop over the mesh's triangles), its
+// lanes over the triangle's pixel box
+__global__ void __launch_bounds__(kBlock) k_geo_raster(GeoArgs g) {
+    const rh_mesh_instance &mi = g.mesh_instances[g.mi_index];
+    const rh_mesh &m = g.meshes[mi.mesh_index];
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t waves = gridDim.x * (kBlock / 64);
+    for (uint32_t k = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); k < m.tris_count; k += waves) {
+        const uint32_t tri = m.tris_index + k;
+        GeoTri t;
+        if (!geo_setup(g, tri, t)) continue;
+        const int bw = t.x1 - t.x0 + 1, bh = t.y1 - t.y0 + 1;
+        const int64_t npx = (int64_t)bw * bh;
+        for (int64_t q = lane; q < npx; q += 64) {
+            const int x = t.x0 + (int)(q % bw), y = t.y0 + (int)(q / bw);
+            float u, v, w;
+            if (geo_sample(g, t, x, y, u, v, w))
+                atomicMin(&g.winner[(y - g.rect[1]) * g.rect[2] + (x - g.rect[0])], tri);
+        }
+    }
+}
+
+// the ray and hit of each covered pixel, from its winning triangle
+__global__ void __launch_bounds__(kBlock) k_geo_finish(GeoArgs g, rh_ray *__restrict__ rays,
+                                                      rh_ray_aux *__restrict__ aux, rh_hit *__restrict__ hits) {
+    const int rw = g.rect[2], n = rw * g.rect[3];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t tri = g.winner[i];
+    if (tri == 0xffffffffu) return;
+    const int x = g.rect[0] + i % rw, y = g.rect[1] + i / rw;
+    GeoTri t;
+    float u, v, w;
+    geo_setup(g, tri, t);
+    geo_sample(g, t, x, y, u, v, w);
+    u *= t.inv_area;
+    v *= t.inv_area;
+    w *= t.inv_area;
+    const rh_vertex &v0 = g.vertices[g.vtx_indices[tri * 3 + 0]], &v1 = g.vertices[g.vtx_indices[tri * 3 + 1]],
+                    &v2 = g.vertices[g.vtx_indices[tri * 3 + 2]];
+    const rh_transform &tr = g.transforms[g.mesh_instances[g.mi_index].tr_index];
+    const f3 pl = f3{v0.p[0] * v + v1.p[0] * w + v2.p[0] * u, v0.p[1] * v + v1.p[1] * w + v2.p[1] * u,
+                     v0.p[2] * v + v1.p[2] * w + v2.p[2] * u};
+    const f3 nl = f3{v0.n[0] * v + v1.n[0] * w + v2.n[0] * u, v0.n[1] * v + v1.n[1] * w + v2.n[1] * u,
+                     v0.n[2] * v + v1.n[2] * w + v2.n[2] * u};
+    const float *X = tr.xform;
+    const f3 p = f3{X[0] * pl.x + X[4] * pl.y + X[8] * pl.z + X[12], X[1] * pl.x + X[5] * pl.y + X[9] * pl.z + X[13],
+                    X[2] * pl.x + X[6] * pl.y + X[10] * pl.z + X[14]};
+    const f3 nn = transform_normal(nl, tr.inv_xform);
+    RayFull r;
+    memset(&r, 0, sizeof(r));
+    r.o = p + nn;
+    r.d = f3{-nn.x, -nn.y, -nn.z};
+    r.c = f3{1.0f, 1.0f, 1.0f};
+    r.ior = 1.0f;
+    r.xy = (x << 16) | y;
+    store_ray(rays, aux, (uint32_t)i, r);
+    rh_hit hh;
+    hh.mask = -1;
+    hh.obj = (int)g.mi_index;
+    hh.prim = (int)tri;
+    hh.t = 1.0f;
+    hh.u = w;
+    hh.v = u;
+    hh.xy = r.xy;
+    hh.pad = 0;
+    hits[i] = hh;
+}
+
+// ---- spherical-harmonic output (OutputSH; FramebufferRef.h:58-90,
+// FramebufferRef.cpp:48-94) ----
+// Per pixel 12 floats (coeff_r[4], coeff_g[4], coeff_b[4]).  After the
+// primary bounce the pixel's secondary ray stores its direction's L1 basis in
+// coeff_r and its weight in coeff_g[0]; after the pass the pass image is
+// projected (scaled by 2 pi / weight) and mixed into the accumulated SH.
+constexpr float kShY0 = 0.282094806f, kShY1 = 0.488602519f;
+
+__global__ void __launch_bounds__(256) k_sh_reset_weights(float *__restrict__ sh, int x0, int y0, int rw, int rh_,
+                                                          int w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rw * rh_) return;
+    const size_t p = (size_t)(y0 + i / rw) * w + (x0 + i % rw);
+    sh[12 * p + 4] = 0.0f;
+}
+
+__global__ void __launch_bounds__(kBlock) k_sh_samples(const rh_ray *__restrict__ rays, const rh_ray_aux *__restrict__ aux,
+                                                      const int *__restrict__ count, int w, float *__restrict__ sh) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *count) return;
+    const RayFull r = load_ray(rays, aux, (uint32_t)i);
+    const int x = (r.xy >> 16) & 0xffff, y = r.xy & 0xffff;
+    float *s = sh + 12 * ((size_t)y * w + x);
+    s[0] = kShY0;
+    s[1] = kShY1 * r.d.y;
+    s[2] = kShY1 * r.d.z;
+    s[3] = kShY1 * r.d.x;
+    s[4] = r.c.x;
+}
+
+__global__ void __launch_bounds__(256) k_sh_finish(const float4 *__restrict__ temp, float *__restrict__ sh_temp,
+                                                   float *__restrict__ sh_clean, int x0, int y0, int rw, int rh_, int w,
+                                                   float k) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rw * rh_) return;
+    const size_t p = (size_t)(y0 + i / rw) * w + (x0 + i % rw);
+    float *s = sh_temp + 12 * p, *o = sh_clean + 12 * p;
+    const float inv_weight = s[4] > RH_FLT_EPS ? (2.0f * RH_PI / s[4]) : 0.0f;
+    float4 c = temp[p];
+    c.x *= inv_weight;
+    c.y *= inv_weight;
+    c.z *= inv_weight;
+    float coeff[4] = {s[0], s[1], s[2], s[3]};
+    for (int j = 0; j < 4; j++) {
+        s[4 + j] = coeff[j] * c.y;
+        s[8 + j] = coeff[j] * c.z;
+        s[j] = coeff[j] * c.x;
+    }
+    for (int j = 0; j < 12; j++) o[j] += (s[j] - o[j]) * k;
+}
+
 // Tree: the scene's tree kind (kTreeBinary / kTreeWide), one walk compiled per
 // kernel.  MinBlocks 5 caps an 8-wide walk's registers at 5 waves per SIMD
 // (some spills) instead of the 4 it would take (knob trace_occupancy).
@@ -437,6 +658,7 @@ struct rh_context {
     size_t color_stage_cap = 0; // floats
     uint32_t n_texels = 0;
     uint32_t n_materials = 0;
+    uint32_t n_mesh_instances = 0;
     int *counters = nullptr; // [0], [1]: ray queue counts, [2]: shadow queue count, [3]: refill fetch
     unsigned long long *stats_dev = nullptr; // [0] primary [1] secondary [2] shadow [3] sorted [4] overflow
     float *halton = nullptr;
@@ -458,6 +680,9 @@ struct rh_context {
     int trace_occupancy = 0;   // 8-wide trace/shadow kernels: 0 = as compiled (4 waves/SIMD), 5 = capped registers
     int max_bounces = RH_MAX_BOUNCES; // max_total_depth clamp (the reference's 10; up to RH_MAX_BOUNCES_EXT)
     int adjoint_staged = 1;           // adjoint replay as staged wavefront passes (0: one lane per path)
+    uint32_t *geo_winner = nullptr;   // geometry camera: winning triangle per lane
+    size_t geo_winner_cap = 0;
+    float *sh_temp = nullptr, *sh_clean = nullptr; // OutputSH: 12 floats per pixel, allocated on demand
     uint32_t n_textures = 0;
     uint32_t *grad_tex_mask = nullptr; // textures whose texels get gradients (null: all)
     size_t grad_tex_words = 0;
@@ -475,6 +700,10 @@ namespace {
 
 int alloc_frame(rh_context *c, int w, int h) {
     const size_t np = (size_t)w * h;
+    // spherical-harmonic data starts again (zeroed) at the next OutputSH pass
+    free_ptr(c->sh_temp);
+    free_ptr(c->sh_clean);
+    c->sh_temp = c->sh_clean = nullptr;
     free_ptr(c->temp);
     free_ptr(c->clean);
     free_ptr(c->final_);
@@ -853,6 +1082,9 @@ void rh_destroy(rh_context *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (auto &b : c->scene_bufs) free_ptr(b.p);
+    free_ptr(c->geo_winner);
+    free_ptr(c->sh_temp);
+    free_ptr(c->sh_clean);
     free_ptr(c->temp);
     free_ptr(c->clean);
     free_ptr(c->final_);
@@ -901,6 +1133,17 @@ int rh_clear(rh_context *c, const float rgba[4]) {
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->clean, n,
                        make_float4(rgba[0], rgba[1], rgba[2], rgba[3]));
     RH_CHECK(hipGetLastError());
+    // the accumulated SH data too (FramebufferRef.cpp Clear)
+    if (c->sh_clean) RH_CHECK(hipMemsetAsync(c->sh_clean, 0, n * 12 * sizeof(float), c->stream));
+    return 0;
+}
+
+int rh_read_sh(rh_context *c, float *out, size_t count) {
+    const size_t n = (size_t)c->w * c->h * 12;
+    if (!c->sh_clean) return 1;
+    if (count < n) return arg_error("output buffer too small");
+    RH_CHECK(hipMemcpyAsync(out, c->sh_clean, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    RH_CHECK(hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -917,6 +1160,7 @@ int rh_upload_scene(rh_context *c, const rh_scene_view *v) {
     c->nee_slots = (v->lights_count >= 1 && v->lights_count <= kMaxDeferredLights) ? v->lights_count : 0;
     c->sc = s;
     c->n_materials = (uint32_t)v->materials_count;
+    c->n_mesh_instances = (uint32_t)v->mesh_instances_count;
     c->n_texels = (uint32_t)v->texels_count;
     c->n_textures = (uint32_t)v->textures_count;
     c->have_scene = true;
@@ -959,18 +1203,63 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec) {
     pi.settings = p->cam.pass_settings;
     pi.settings.max_total_depth = (uint8_t)std::min<int>(pi.settings.max_total_depth, c->max_bounces);
 
+    const bool geo = p->cam.type == RH_CAM_GEO;
+    const bool out_sh = !rec && (p->cam.pass_settings.flags & RH_OUTPUT_SH);
+    if (geo) {
+        if (p->cam.mi_index >= c->n_mesh_instances) return arg_error("geometry camera: no such mesh instance");
+        if (p->cam.uv_index > 1) return arg_error("geometry camera: uv layer must be 0 or 1");
+        if ((size_t)n > c->geo_winner_cap) {
+            free_ptr(c->geo_winner);
+            c->geo_winner = nullptr;
+            c->geo_winner_cap = (size_t)n;
+            RH_CHECK(hipMalloc(&c->geo_winner, (size_t)n * sizeof(uint32_t)));
+        }
+    }
+    if (out_sh && !c->sh_temp) {
+        const size_t bytes = (size_t)c->w * c->h * 12 * sizeof(float);
+        RH_CHECK(hipMalloc(&c->sh_temp, bytes));
+        RH_CHECK(hipMalloc(&c->sh_clean, bytes));
+        RH_CHECK(hipMemsetAsync(c->sh_temp, 0, bytes, c->stream));
+        RH_CHECK(hipMemsetAsync(c->sh_clean, 0, bytes, c->stream));
+    }
+
     StageTimer tm(c);
     tm.mark(); // 0
     int *cnt[2] = {c->counters + 0, c->counters + 1};
-    hipLaunchKernelGGL(k_gen_primary, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, pc, c->halton,
-                       c->rays[0], c->aux[0], cnt[0]);
-    RH_CHECK(hipGetLastError());
-    tm.mark(); // 1
     unsigned long long *count_primary = rec ? c->stats_dev + 5 : c->stats_dev + 0;
     unsigned long long *count_secondary = rec ? c->stats_dev + 5 : c->stats_dev + 1;
-    rc = launch_trace(c, trace_args(c, c->rays[0], cnt[0], n, c->hits, nullptr, count_primary), n, false);
-    if (rc) return rc;
-    tm.mark(); // 2
+    if (geo) {
+        // rays and hits from the mesh's texture-space footprint, no trace
+        GeoArgs g;
+        g.vertices = c->sc.vertices;
+        g.vtx_indices = c->sc.vtx_indices;
+        g.mesh_instances = c->sc.mesh_instances;
+        g.meshes = c->sc.meshes;
+        g.transforms = c->sc.transforms;
+        g.halton = c->halton;
+        g.winner = c->geo_winner;
+        memcpy(g.rect, p->rect, sizeof(g.rect));
+        g.w = c->w;
+        g.h = c->h;
+        g.iteration = p->iteration;
+        g.mi_index = p->cam.mi_index;
+        g.uv = p->cam.uv_index;
+        const dim3 grid((n + kBlock - 1) / kBlock), blk(kBlock);
+        hipLaunchKernelGGL(k_geo_init, grid, blk, 0, c->stream, g, c->rays[0], c->aux[0], c->hits, cnt[0]);
+        hipLaunchKernelGGL(k_geo_raster, dim3(2 * c->cus * 4), blk, 0, c->stream, g);
+        hipLaunchKernelGGL(k_geo_finish, grid, blk, 0, c->stream, g, c->rays[0], c->aux[0], c->hits);
+        RH_CHECK(hipGetLastError());
+        tm.mark(); // 1
+        tm.mark(); // 2
+    } else {
+        hipLaunchKernelGGL(k_gen_primary, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, pc, c->halton,
+                           c->rays[0], c->aux[0], cnt[0]);
+        RH_CHECK(hipGetLastError());
+        tm.mark(); // 1
+        rc = launch_trace(c, trace_args(c, c->rays[0], cnt[0], n, c->hits, nullptr, count_primary), n, false);
+        if (rc) return rc;
+        tm.mark(); // 2
+    }
 
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, cnt[1], 0);
     ShadeArgs sa = shade_args(c);
@@ -994,6 +1283,14 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec) {
     sa.out_count = cnt[1];
     rc = launch_shade(c, sa, n);
     if (rc) return rc;
+    if (out_sh) {
+        // each pixel's secondary ray from the primary bounce (RendererRef.cpp:185-197)
+        hipLaunchKernelGGL(k_sh_reset_weights, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->sh_temp, rx, ry, rw,
+                           rhh, c->w);
+        hipLaunchKernelGGL(k_sh_samples, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->rays[1],
+                           c->aux[1], cnt[1], c->w, c->sh_temp);
+        RH_CHECK(hipGetLastError());
+    }
     tm.mark(); // 3
 
     float t_sort = 0, t_trace = 0, t_shade = 0;
@@ -1081,6 +1378,11 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec) {
     }
 
     if (rec) return 0; // the replay leaves the image alone
+    if (out_sh) {
+        hipLaunchKernelGGL(k_sh_finish, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->temp, c->sh_temp,
+                           c->sh_clean, rx, ry, rw, rhh, c->w, pc.mix_k);
+        RH_CHECK(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_mix_tonemap, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->temp, c->clean, c->final_, rx,
                        ry, rw, rhh, c->w, pc.mix_k, p->cam.dtype, p->cam.gamma, p->cam.pass_settings.flags);
     RH_CHECK(hipGetLastError());
@@ -1442,6 +1744,7 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
         return arg_error("region outside the framebuffer");
     if (p->iteration < 1) return arg_error("iteration must be >= 1");
     if (!d_adj_rgba || !d_grad) return arg_error("adjoint image and gradient buffers are required");
+    if (p->cam.type == RH_CAM_GEO) return arg_error("the adjoint replay takes perspective cameras only");
     const int depth = std::min<int>(p->cam.pass_settings.max_total_depth, c->max_bounces);
     const size_t lanes = (size_t)rw * rhh;
     // vertex records: at most depth + 1 <= RH_MAX_BOUNCES_EXT + 1 per lane

@@ -40,6 +40,10 @@ int BatchRenderer::AddScene(const std::shared_ptr<SceneBase> &_s, int w, int h, 
     rh_pass_desc pass;
     memset(&pass, 0, sizeof(pass));
     s->PassCamera(pass);
+    if (pass.cam.type == Geo || (pass.cam.pass_settings.flags & OutputSH)) {
+        last_error = "the batch renders perspective cameras without SH output";
+        return -1;
+    }
     pass.rect[2] = w;
     pass.rect[3] = h;
     pass.iteration = 1;

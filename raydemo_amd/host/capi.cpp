@@ -88,6 +88,18 @@ int rhx_get_pixels(rhx_renderer *r, float *out, size_t count) {
     memcpy(out, p, n * sizeof(float));
     return 0;
 }
+int rhx_get_sh_data(rhx_renderer *r, float *out, size_t count) {
+    const auto sz = r->r->size();
+    const size_t n = (size_t)sz.first * sz.second * 12;
+    if (count < n) {
+        g_err = "output buffer too small";
+        return -1;
+    }
+    const Ray::shl1_data_t *p = r->r->get_sh_data_ref();
+    if (!p) return 1;
+    memcpy(out, p, n * sizeof(float));
+    return 0;
+}
 
 int rhx_get_stats(rhx_renderer *r, rh_stats *out) {
     Ray::RendererBase::stats_t st;

@@ -133,6 +133,7 @@ class Renderer : public RendererBase {
 
     std::vector<uint16_t> permutations_;
     mutable std::vector<pixel_color_t> host_pixels_;
+    mutable std::vector<shl1_data_t> host_sh_;
     mutable bool host_dirty_ = true;
     const Scene *uploaded_scene_ = nullptr;
     uint64_t uploaded_version_ = 0;
@@ -149,7 +150,8 @@ public:
     eRendererType type() const override { return RendererHIP; }
     std::pair<int, int> size() const override { return std::make_pair(w_, h_); }
     const pixel_color_t *get_pixels_ref() const override;
-    const shl1_data_t *get_sh_data_ref() const override { return nullptr; }
+    // null until a pass with OutputSH has run (since the last Resize)
+    const shl1_data_t *get_sh_data_ref() const override;
     void Resize(int w, int h) override;
     void Clear(const pixel_color_t &c) override;
     std::shared_ptr<SceneBase> CreateScene() override;

@@ -53,6 +53,12 @@ const pixel_color_t *Renderer::get_pixels_ref() const {
     return host_pixels_.data();
 }
 
+const shl1_data_t *Renderer::get_sh_data_ref() const {
+    host_sh_.resize((size_t)w_ * h_);
+    const int rc = rh_read_sh(ctx_, reinterpret_cast<float *>(host_sh_.data()), host_sh_.size() * 12);
+    return rc == 0 ? host_sh_.data() : nullptr;
+}
+
 std::shared_ptr<SceneBase> Renderer::CreateScene() { return std::make_shared<Scene>(use_wide_bvh_, true, device_); }
 
 // Uploads the scene if it changed, advances the region's iteration and

@@ -234,6 +234,53 @@ def cornell_refined(scene: "A.Scene", k: int = 17, albedo_seed: int = SEED_BASE 
                      info={"materials": n_tris + 1, "diffuse": n_tris, "light_material": n_tris})
 
 
+def lightmap_box(scene: "A.Scene", k: int = 2, width=64, height=64, spp=4, depth=4, output_sh=True,
+                 perspective=False) -> SceneSpec:
+    """The Cornell box (faces as k x k grids, one material per face kind) with
+    a second UV layer that packs every triangle into its own cell of a square
+    grid (a lightmap atlas without overlaps), and a geometry camera on mesh
+    instance 0, UV layer 1 (camera_desc_t type Geo): each pass bakes one sample
+    per lightmap texel; with output_sh the SH projection of the pass's
+    secondary-ray directions too (OutputSH).  perspective: a perspective
+    camera instead (with the same SH flag)."""
+    attrs, idx, n_light = cornell_refined_geometry(k)
+    n_tris = len(idx) // 3
+    verts = attrs[idx]                           # one vertex per triangle corner
+    g = int(math.ceil(math.sqrt(n_tris)))
+    cell = 1.0 / g
+    corner = np.array([(0.15, 0.15), (0.85, 0.15), (0.15, 0.85)], np.float32)
+    uv1 = np.zeros((len(verts), 2), np.float32)
+    for t in range(n_tris):
+        cx, cy = (t % g) * cell, (t // g) * cell
+        uv1[3 * t:3 * t + 3] = np.array([cx, cy], np.float32) + corner * cell
+    a = np.zeros((len(verts), 10), np.float32)
+    a[:, :8] = verts
+    a[:, 8:10] = uv1
+    tex = _white(scene)
+    kinds = [scene.add_material(A.DiffuseMaterial, tex, main_color=c)
+             for c in ((0.75, 0.75, 0.75), (0.8, 0.2, 0.2), (0.2, 0.75, 0.25), (0.6, 0.6, 0.9))]
+    light = scene.add_material(A.EmissiveMaterial, tex, main_color=(1.0, 1.0, 1.0), strength=10.0)
+    n_face = 2 * k * k
+    shapes = []
+    for t in range(n_tris - n_light):
+        face = t // n_face
+        shapes.append((kinds[0 if face < 3 else (1 if face == 3 else (2 if face == 4 else 3))], 3 * t, 3))
+    shapes.append((light, 3 * (n_tris - n_light), 3 * n_light))
+    mesh = scene.add_mesh(A.PxyzNxyzTuvTuv, a, np.arange(len(verts), dtype=np.uint32), shapes)
+    scene.add_mesh_instance(mesh)
+    scene.set_environment((0.05, 0.05, 0.05))
+    scene.add_light(A.PointLight, position=(0.0, 8.5, 0.0), radius=0.3, color=(20.0, 20.0, 20.0))
+    common = dict(max_diff_depth=depth, max_glossy_depth=depth, max_refr_depth=depth, max_transp_depth=depth,
+                  max_total_depth=depth, termination_start_depth=16, output_sh=output_sh, dtype=A.DeviceNone)
+    if perspective:
+        cam = scene.add_camera(origin=(0.0, 5.0, -14.0), fwd=(0.0, 0.0, 1.0), fov=45.0, filter=A.Box, **common)
+    else:
+        cam = scene.add_camera(origin=(0.0, 0.0, 0.0), fwd=(0.0, 0.0, 1.0), type=A.Geo, mi_index=0, uv_index=1,
+                               **common)
+    scene.set_current_cam(cam)
+    return SceneSpec("lightmap_box", width, height, spp, depth, triangles=n_tris)
+
+
 def icosphere(subdiv: int):
     t = (1.0 + math.sqrt(5.0)) / 2.0
     verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),

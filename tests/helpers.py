@@ -149,6 +149,7 @@ class Oracle:
             ("orc_sort_rays", [P, C.c_size_t, P, P]),
             ("orc_shade", [P, P, P, P, C.c_size_t, P, P, P, P, P]),
             ("orc_render", [P, P, C.c_int, C.c_int, P, C.c_int, P, P, P, P]),
+            ("orc_render_sh", [P, P, C.c_int, C.c_int, P, C.c_int, P, P, P, P, P, P]),
             ("orc_adjoint", [P, P, C.c_int, C.c_int, P, C.c_int, P, C.c_double, P, P]),
             ("orc_set_max_bounces", [C.c_int]),
         ]:
@@ -226,6 +227,20 @@ class Oracle:
         self.lib.orc_adjoint(C.byref(view), C.byref(cam), w, h, _p(rect), passes, _p(adj), 1.0 / passes, _p(grad),
                              _p(tg) if texels else None)
         return (grad, tg) if texels else grad
+
+    def render_sh(self, view, cam, w, h, passes):
+        """Linear mean image, tonemapped image, ray counts and the accumulated
+        SH data (h, w, 12) of `passes` passes with OutputSH."""
+        rect = np.array((0, 0, w, h), np.int32)
+        clean = np.zeros((h, w, 4), np.float32)
+        final = np.zeros((h, w, 4), np.float32)
+        sh_temp = np.zeros((h, w, 12), np.float32)
+        sh = np.zeros((h, w, 12), np.float32)
+        it = np.zeros(1, np.int32)
+        counts = np.zeros(3, np.uint64)
+        self.lib.orc_render_sh(C.byref(view), C.byref(cam), w, h, _p(rect), passes, _p(it), _p(clean), _p(final),
+                               _p(counts), _p(sh_temp), _p(sh))
+        return clean, final, counts, sh
 
     def render(self, view, cam, w, h, passes, rect=None):
         rect = np.array(rect or (0, 0, w, h), np.int32)
