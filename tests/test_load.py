@@ -1,0 +1,174 @@
+"""The demo's scene formats (raydemo_amd/load.py, after src/DemoLib/load/
+Load.cpp): the .bin mesh format, Radiance RGBE images, the instance matrices
+of "pos"/"rot" against an independent FP64 composition, a JSON scene with
+every field the loader reads, and the reference's own assets when present."""
+import json
+import math
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from conftest import standalone_scene
+from raydemo_amd import capi as A, load, scenes
+
+ASSETS = "/root/reference/assets"
+
+
+def _write_hdr(path, rgbe):
+    """A run-length encoded Radiance file: each channel as one run of the first
+    half and literal values for the rest."""
+    h, w, _ = rgbe.shape
+    out = bytearray(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n" + f"-Y {h} +X {w}\n".encode())
+    for y in range(h):
+        out += bytes([2, 2, w >> 8, w & 255])
+        for c in range(4):
+            half = w // 2
+            out += bytes([128 + half, int(rgbe[y, 0, c])])
+            out += bytes([w - half]) + bytes(int(v) for v in rgbe[y, half:, c])
+    with open(path, "wb") as f:
+        f.write(bytes(out))
+
+
+def write_scene(d):
+    """A scene in the demo's format under directory d: two meshes (.bin), a
+    PNG texture, an HDR environment map, every material type, instances with
+    pos/rot, a sun and a targeted camera.  Returns the JSON path."""
+    from PIL import Image
+    os.makedirs(os.path.join(d, "assets"), exist_ok=True)
+    v, f = scenes.icosphere(2)
+    attrs = np.zeros((len(v), 8), np.float32)
+    attrs[:, :3] = v
+    attrs[:, 3:6] = v / np.linalg.norm(v, axis=1, keepdims=True)
+    attrs[:, 6] = 0.5 + np.arctan2(v[:, 2], v[:, 0]) / (2 * math.pi)
+    attrs[:, 7] = 0.5 + v[:, 1] * 0.5
+    idx = f.astype(np.uint32).ravel()
+    half = (len(idx) // 6) * 3
+    load.save_bin(os.path.join(d, "assets/ball.bin"), attrs, idx, [[0, half], [half, len(idx) - half]])
+    qa, qi = scenes._quad((-8, 0, -8), (-8, 0, 8), (8, 0, 8), (8, 0, -8), (0, 1, 0))
+    load.save_bin(os.path.join(d, "assets/floor.bin"), qa.astype(np.float32), qi, [[0, 6]])
+    rng = np.random.default_rng(3)
+    Image.fromarray(rng.integers(0, 255, (16, 16, 4), dtype=np.uint8)).save(os.path.join(d, "assets/tiles.png"))
+    env = np.zeros((4, 8, 4), np.uint8)
+    env[..., :3] = 120
+    env[..., 3] = 129
+    _write_hdr(os.path.join(d, "assets/sky.hdr"), env)
+    js = {
+        "environment": {"sun_dir": [0.3, -1.0, 0.2], "sun_col": [2.0, 2.0, 1.8], "sun_softness": 2.0,
+                        "env_col": [0.3, 0.35, 0.4], "env_map": "assets/sky.hdr"},
+        "camera": {"view_origin": [0.5, 4.0, -9.0], "view_target": [0.0, 1.0, 0.0], "fov": 50.0, "filter": "box",
+                   "srgb": False},
+        "materials": {
+            "floor": {"type": "diffuse", "main_texture": "assets/tiles.png"},
+            "red": {"type": "diffuse", "main_texture": "assets/tiles.png", "main_color": [0.9, 0.2, 0.2]},
+            "shiny": {"type": "glossy", "main_texture": "assets/tiles.png", "roughness": 0.3},
+            "mixed": {"type": "mix", "main_texture": "assets/tiles.png", "strength": 0.5, "int_ior": 1.5,
+                      "materials": ["red", "shiny"]},
+            "glass": {"type": "refractive", "main_texture": "assets/tiles.png", "int_ior": 1.4, "ext_ior": 1.0},
+            "lamp": {"type": "emissive", "main_texture": "assets/tiles.png", "strength": 4.0},
+            "see": {"type": "transparent", "main_texture": "assets/tiles.png"},
+        },
+        "meshes": {
+            "ball": {"vertex_data": "assets/ball.bin", "materials": ["mixed", "lamp"]},
+            "ball2": {"vertex_data": "assets/ball.bin", "materials": ["glass", "see"], "use_fast_bvh_build": True},
+            "floor": {"vertex_data": "assets/floor.bin", "materials": ["floor"]},
+        },
+        "mesh_instances": [
+            {"mesh": "floor"},
+            {"mesh": "ball", "pos": [-1.5, 1.0, 0.0], "rot": [10.0, 30.0, 45.0]},
+            {"mesh": "ball2", "pos": [1.5, 1.0, 0.5], "rot": [0.0, 90.0, 0.0]},
+        ],
+    }
+    path = os.path.join(d, "scene.json")
+    with open(path, "w") as fh:
+        json.dump(js, fh)
+    return path
+
+
+def test_bin_round_trip_and_errors(tmp_path):
+    rng = np.random.default_rng(0)
+    attrs = rng.standard_normal((10, 8)).astype(np.float32)
+    idx = rng.integers(0, 10, 12).astype(np.uint32)
+    p = str(tmp_path / "m.bin")
+    load.save_bin(p, attrs, idx, [[0, 6], [6, 6]])
+    a, i, g = load.load_bin(p)
+    assert a.tobytes() == attrs.tobytes() and i.tobytes() == idx.tobytes() and g.tolist() == [[0, 6], [6, 6]]
+    with open(p, "r+b") as f:
+        f.write(struct.pack("<I", 81))  # attribute count not a multiple of 8
+    with pytest.raises(ValueError):
+        load.load_bin(p)
+
+
+def test_hdr_reader(tmp_path):
+    rgbe = np.random.default_rng(1).integers(0, 255, (3, 6, 4), dtype=np.uint8)
+    rgbe[:, :3] = rgbe[:, :1]  # the first half of each row is one run
+    p = str(tmp_path / "x.hdr")
+    _write_hdr(p, rgbe)
+    assert load.read_hdr(p).tobytes() == rgbe.tobytes()
+
+
+def test_instance_transform_matches_fp64_composition():
+    """translate(pos) * Rz * Rx * Ry, columns first, against float64 matrices."""
+    pos, rot = (1.5, -2.0, 0.25), (10.0, 30.0, 45.0)
+
+    def R(axis, deg):
+        a = math.radians(deg)
+        c, s = math.cos(a), math.sin(a)
+        m = np.eye(4)
+        i, j = {"x": (1, 2), "y": (2, 0), "z": (0, 1)}[axis]
+        m[i, i], m[i, j], m[j, i], m[j, j] = c, -s, s, c
+        return m
+
+    T = np.eye(4)
+    T[:3, 3] = pos
+    ref = T @ R("z", rot[2]) @ R("x", rot[0]) @ R("y", rot[1])
+    got = load.instance_transform(pos, rot).reshape(4, 4).T.astype(np.float64)  # columns -> matrix
+    assert np.abs(got - ref).max() < 2e-6
+
+
+def test_load_scene_fields(tmp_path, host_lib):
+    path = write_scene(str(tmp_path))
+    s = standalone_scene(host_lib)
+    h = load.load_scene(s, path, base_dir=str(tmp_path))
+    v = s.view()
+    assert len(h["materials"]) == 7 and len(h["meshes"]) == 3 and len(h["instances"]) == 3
+    mats = v["materials"]
+    kinds = [int(mats[h["materials"][k]]["type"]) for k in ("floor", "shiny", "mixed", "glass", "lamp", "see")]
+    assert kinds == [A.DiffuseMaterial, A.GlossyMaterial, A.MixMaterial, A.RefractiveMaterial, A.EmissiveMaterial,
+                     A.TransparentMaterial]
+    mix = mats[h["materials"]["mixed"]]["textures"]
+    assert (int(mix[2]), int(mix[3])) == (h["materials"]["red"], h["materials"]["shiny"])
+    assert len(h["textures"]) == 2  # the PNG once, the HDR map
+    n_ball = len(scenes.icosphere(2)[1])
+    assert s.triangle_count() == 2 * n_ball + 2
+    # the sun: a directional light far along -sun_dir
+    lt = v["lights"][h["sun"]]
+    d = -np.array([0.3, -1.0, 0.2], np.float32)
+    assert np.allclose(lt["pos"] / 9999999.0, d, rtol=1e-6)
+    cam = s.camera_record(h["camera"])
+    fwd = np.array([0.0, 1.0, 0.0]) - np.array([0.5, 4.0, -9.0])
+    assert np.allclose(np.array(cam.fwd), fwd / np.linalg.norm(fwd), atol=1e-6)
+    assert cam.filter == A.Box and cam.dtype == A.DeviceNone
+
+
+def test_missing_texture_policy(tmp_path, host_lib):
+    path = write_scene(str(tmp_path))
+    os.remove(str(tmp_path / "assets/tiles.png"))
+    with pytest.raises(FileNotFoundError):
+        load.load_scene(standalone_scene(host_lib), path, base_dir=str(tmp_path))
+    load.load_scene(standalone_scene(host_lib), path, base_dir=str(tmp_path), missing_texture="white")
+
+
+@pytest.mark.skipif(not os.path.isdir(ASSETS), reason="the reference's assets are not present")
+def test_reference_assets(host_lib):
+    """The reference's sponza_simple.bin (13,776 triangles, per SceneRef
+    demo use) and its scene file (textures absent here: white stands in)."""
+    attrs, idx, groups = load.load_bin(os.path.join(ASSETS, "meshes/sponza_simple.bin"))
+    assert len(idx) // 3 == 13776
+    assert int(groups[:, 1].sum()) == len(idx) and groups[0, 0] == 0
+    assert (groups[1:, 0] == groups[:-1, 0] + groups[:-1, 1]).all()
+    s = standalone_scene(host_lib)
+    h = load.load_scene(s, os.path.join(ASSETS, "scenes/sponza_simple.json"), base_dir=os.path.dirname(ASSETS),
+                        missing_texture="white")
+    assert s.triangle_count() == 13776 and len(h["materials"]) == 9
