@@ -371,6 +371,8 @@ def run_large(args, lib, A, scenes, knobs, wide):
     sweep = []
     for cfg in ({"path_tail": -1, "lds_stack_depth": 16, "trace_occupancy": 0},
                 {"path_tail": -1, "lds_stack_depth": 8, "trace_occupancy": 5},
+                {"path_tail": -1, "lds_stack_depth": 16, "trace_occupancy": 5},
+                {"path_tail": -1, "lds_stack_depth": 24, "trace_occupancy": 0},
                 {"path_tail": 8, "lds_stack_depth": 16, "trace_occupancy": 0, "tail_occupancy": 3},
                 {"path_tail": knobs.get("path_tail", 2), "lds_stack_depth": knobs.get("lds_stack_depth", 8),
                  "trace_occupancy": knobs.get("trace_occupancy", 0), "tail_occupancy": knobs.get("tail_occupancy", 2)}):

@@ -837,6 +837,7 @@ __device__ __forceinline__ uint32_t atlas_get(const Scene &sc, uint32_t tex, int
 struct TexFoot {
     uint32_t idx[4];
     float kx, ky;
+    int lod;
 };
 
 __device__ __forceinline__ float texel_ch(uint32_t p, int c) { return (float)((p >> (8 * c)) & 0xffu); }
@@ -864,7 +865,7 @@ __device__ __forceinline__ f4 sample_bilinear(const Scene &sc, uint32_t tex, f2 
     const uint32_t i11 = atlas_index(sc, tex, lod, ipx, ipy, ix + 1, iy + 1);
     const uint32_t p00 = sc.texels[i00], p01 = sc.texels[i01], p10 = sc.texels[i10], p11 = sc.texels[i11];
     const float kx = rx - __builtin_floorf(rx), ky = ry - __builtin_floorf(ry);
-    if (foot) *foot = TexFoot{{i00, i01, i10, i11}, kx, ky};
+    if (foot) *foot = TexFoot{{i00, i01, i10, i11}, kx, ky, lod};
     float r[4];
     const float k = 1.0f / 255.0f;
 #pragma unroll

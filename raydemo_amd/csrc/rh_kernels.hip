@@ -318,11 +318,21 @@ struct AdjointArgs {
     int2 *nee_aux;
     float4 *nee_term;
     uint32_t nee_slots;
+    // texel blocks of at most kSmallBlockTexels (the coarse mip levels, which
+    // many pixels share) are summed in kTexelReplicas copies, a wave adding to
+    // one of them, and folded into tex_grad after the pass (k_fold_replicas):
+    // coarse_map[tex * RH_NUM_MIP_LEVELS + lod] = (block's first texel, its
+    // first slot in a copy, or kNoKey when the block is not replicated)
+    float *rep;
+    uint32_t rep_count; // slots per copy
+    const uint2 *coarse_map;
 };
 
 constexpr uint32_t kLdsGradMaterials = 1024;
+constexpr uint32_t kTexelReplicas = 32;
+constexpr uint32_t kSmallBlockTexels = 66 * 66;
 
-constexpr int kAggRounds = 4;
+constexpr int kAggRounds = 16;
 
 __device__ __forceinline__ float wave_sum(float v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -362,6 +372,31 @@ __device__ __forceinline__ void wave_add3(float *dst, uint32_t key, f3 v) {
     }
 }
 
+// wave_add3 with a per-lane target: lanes with the same key have the same
+// target (the key names the address within the lane's copy)
+__device__ __forceinline__ void wave_add3_at(float *target, uint32_t key, f3 v) {
+    const uint32_t lane = __lane_id();
+    uint64_t pending = __ballot(key != kNoKey);
+    for (int round = 0; pending && round < kAggRounds; round++) {
+        const int leader = __builtin_ctzll(pending);
+        const uint32_t kk = __shfl(key, leader);
+        const bool mine = key == kk;
+        const float sx = wave_sum(mine ? v.x : 0.0f), sy = wave_sum(mine ? v.y : 0.0f),
+                    sz = wave_sum(mine ? v.z : 0.0f);
+        if ((int)lane == leader) {
+            if (sx != 0.0f) atomicAdd(target + 0, sx);
+            if (sy != 0.0f) atomicAdd(target + 1, sy);
+            if (sz != 0.0f) atomicAdd(target + 2, sz);
+        }
+        pending &= ~__ballot(mine);
+    }
+    if ((pending >> lane) & 1u) {
+        if (v.x != 0.0f) atomicAdd(target + 0, v.x);
+        if (v.y != 0.0f) atomicAdd(target + 1, v.y);
+        if (v.z != 0.0f) atomicAdd(target + 2, v.z);
+    }
+}
+
 // The backward half of the replay for lane i (pixel adjoint dl) over its nv
 // vertex records: S_k = sum_{j >= k} C_j, added as adj . S_k / main_color to the
 // material of vertex k and, with texel gradients, to its texel footprint.
@@ -398,11 +433,13 @@ __device__ __forceinline__ void adjoint_accumulate(const AdjointArgs &a, uint32_
         uint32_t idx[4] = {kNoKey, kNoKey, kNoKey, kNoKey};
         float w[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         float gr[3] = {0.0f, 0.0f, 0.0f};
+        uint2 cm = make_uint2(0u, kNoKey); // small texel blocks go to this wave's replica
         if (m >= 0) {
             const float4 *rt = a.rec_tex + (size_t)k * 3 * a.lanes + i;
             const float4 t2 = rt[2 * (size_t)a.lanes];
             const uint32_t tex = __float_as_uint(t2.z);
             if (!a.tex_mask || ((a.tex_mask[tex >> 5] >> (tex & 31)) & 1u)) {
+                if (a.rep) cm = a.coarse_map[tex * RH_NUM_MIP_LEVELS + __float_as_int(t2.w)];
                 const float4 t0 = rt[0], t1 = rt[a.lanes];
                 const float raw[3] = {t1.z, t1.w, t2.x};
                 const float gch[3] = {g.x, g.y, g.z};
@@ -434,9 +471,15 @@ __device__ __forceinline__ void adjoint_accumulate(const AdjointArgs &a, uint32_
                 }
             }
         }
+        const uint32_t copy = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) % kTexelReplicas;
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-            wave_add3(a.tex_grad, w[q] != 0.0f ? idx[q] : kNoKey, f3{gr[0] * w[q], gr[1] * w[q], gr[2] * w[q]});
+        for (int q = 0; q < 4; q++) {
+            const uint32_t key = w[q] != 0.0f ? idx[q] : kNoKey;
+            float *target = cm.y != kNoKey && key != kNoKey
+                                ? a.rep + 4 * ((size_t)copy * a.rep_count + cm.y + (idx[q] - cm.x))
+                                : a.tex_grad + 4 * (size_t)(key == kNoKey ? 0u : key);
+            wave_add3_at(target, key, f3{gr[0] * w[q], gr[1] * w[q], gr[2] * w[q]});
+        }
     }
 }
 
@@ -547,6 +590,24 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_backward(AdjointArgs a, cons
         for (uint32_t j = threadIdx.x; j < 4 * a.n_materials; j += blockDim.x)
             if (s_grad[j] != 0.0f) atomicAdd(&a.grad[j], s_grad[j]);
     }
+}
+
+// Sums the replicas of the small texel blocks into the texel gradient and
+// clears them for the next pass (one lane per replicated texel).
+__global__ void __launch_bounds__(256) k_fold_replicas(float *__restrict__ rep, uint32_t count,
+                                                       const uint32_t *__restrict__ texel, float *__restrict__ tex_grad) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= count) return;
+    float s[3] = {0.0f, 0.0f, 0.0f};
+    for (uint32_t r = 0; r < kTexelReplicas; r++) {
+        float *p = rep + 4 * ((size_t)r * count + j);
+        for (int c = 0; c < 3; c++) {
+            s[c] += p[c];
+            p[c] = 0.0f;
+        }
+    }
+    float *g = tex_grad + 4 * (size_t)texel[j];
+    for (int c = 0; c < 3; c++) g[c] += s[c];
 }
 
 // main_color of materials [first, first + n) from packed rgb triples
@@ -683,6 +744,12 @@ struct rh_context {
     uint32_t *geo_winner = nullptr;   // geometry camera: winning triangle per lane
     size_t geo_winner_cap = 0;
     float *sh_temp = nullptr, *sh_clean = nullptr; // OutputSH: 12 floats per pixel, allocated on demand
+    std::vector<rh_tex_mip> host_tex_mips;  // the scene's mip table (small texel blocks are replicated)
+    uint2 *coarse_map = nullptr;            // per (texture, lod): block offset, replica slot or kNoKey
+    uint32_t *coarse_texel = nullptr;       // per replica slot: its texel
+    float *rep = nullptr;                   // kTexelReplicas copies of the slots' gradients
+    uint32_t rep_count = 0;
+    bool coarse_valid = false;
     uint32_t n_textures = 0;
     uint32_t *grad_tex_mask = nullptr; // textures whose texels get gradients (null: all)
     size_t grad_tex_words = 0;
@@ -1085,6 +1152,9 @@ void rh_destroy(rh_context *c) {
     free_ptr(c->geo_winner);
     free_ptr(c->sh_temp);
     free_ptr(c->sh_clean);
+    free_ptr(c->coarse_map);
+    free_ptr(c->coarse_texel);
+    free_ptr(c->rep);
     free_ptr(c->temp);
     free_ptr(c->clean);
     free_ptr(c->final_);
@@ -1161,6 +1231,8 @@ int rh_upload_scene(rh_context *c, const rh_scene_view *v) {
     c->sc = s;
     c->n_materials = (uint32_t)v->materials_count;
     c->n_mesh_instances = (uint32_t)v->mesh_instances_count;
+    c->host_tex_mips.assign(v->tex_mips, v->tex_mips + (size_t)v->textures_count * RH_NUM_MIP_LEVELS);
+    c->coarse_valid = false;
     c->n_texels = (uint32_t)v->texels_count;
     c->n_textures = (uint32_t)v->textures_count;
     c->have_scene = true;
@@ -1767,6 +1839,50 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
     if (rc) return rc;
     rc = ensure_spill(c, lanes);
     if (rc) return rc;
+    if (d_texel_grad && !c->coarse_valid) {
+        // replica slots for the small texel blocks (each block once, however
+        // many (texture, lod) entries name it)
+        free_ptr(c->coarse_map);
+        free_ptr(c->coarse_texel);
+        free_ptr(c->rep);
+        c->coarse_map = nullptr;
+        c->coarse_texel = nullptr;
+        c->rep = nullptr;
+        std::vector<uint2> map(c->host_tex_mips.size());
+        std::vector<uint32_t> texel;
+        std::vector<std::pair<uint32_t, uint32_t>> seen; // block offset -> first slot
+        for (size_t e = 0; e < map.size(); e++) {
+            const rh_tex_mip &m = c->host_tex_mips[e];
+            const uint32_t n = (uint32_t)m.bw * m.bh;
+            map[e] = make_uint2(m.offset, kNoKey);
+            if (!n || n > kSmallBlockTexels) continue;
+            uint32_t slot = kNoKey;
+            for (auto &b : seen)
+                if (b.first == m.offset) slot = b.second;
+            if (slot == kNoKey) {
+                slot = (uint32_t)texel.size();
+                seen.emplace_back(m.offset, slot);
+                for (uint32_t t = 0; t < n; t++) texel.push_back(m.offset + t);
+            }
+            map[e].y = slot;
+        }
+        c->rep_count = (uint32_t)texel.size();
+        if (!map.empty()) {
+            RH_CHECK(hipMalloc(&c->coarse_map, map.size() * sizeof(uint2)));
+            RH_CHECK(hipMemcpyAsync(c->coarse_map, map.data(), map.size() * sizeof(uint2), hipMemcpyHostToDevice,
+                                    c->stream));
+        }
+        if (c->rep_count) {
+            const size_t bytes = (size_t)kTexelReplicas * c->rep_count * 4 * sizeof(float);
+            RH_CHECK(hipMalloc(&c->coarse_texel, texel.size() * sizeof(uint32_t)));
+            RH_CHECK(hipMemcpyAsync(c->coarse_texel, texel.data(), texel.size() * sizeof(uint32_t),
+                                    hipMemcpyHostToDevice, c->stream));
+            RH_CHECK(hipMalloc(&c->rep, bytes));
+            RH_CHECK(hipMemsetAsync(c->rep, 0, bytes, c->stream));
+        }
+        RH_CHECK(hipStreamSynchronize(c->stream));
+        c->coarse_valid = true;
+    }
     AdjointArgs a;
     memset(&a, 0, sizeof(a));
     a.sc = c->sc;
@@ -1785,6 +1901,11 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
     a.tex_grad = d_texel_grad;
     a.tex_mask = c->grad_tex_mask;
     a.rec_tex = d_texel_grad ? c->adj_rec_tex : nullptr;
+    if (d_texel_grad && c->rep_count) {
+        a.rep = c->rep;
+        a.rep_count = c->rep_count;
+        a.coarse_map = c->coarse_map;
+    }
     a.spill = c->spill;
     a.spill_stride = (uint32_t)c->spill_cap;
     a.lds_depth = (uint32_t)c->lds_depth;
@@ -1815,6 +1936,9 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
         KernelMark km(c, 4);
         hipLaunchKernelGGL(k_adjoint_backward, dim3((unsigned)((lanes + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                            c->stream, a, (const int *)c->adj_nverts);
+        if (a.rep)
+            hipLaunchKernelGGL(k_fold_replicas, dim3((a.rep_count + 255) / 256), dim3(256), 0, c->stream, a.rep,
+                               a.rep_count, (const uint32_t *)c->coarse_texel, d_texel_grad);
         RH_CHECK(hipGetLastError());
         return 0;
     }
@@ -1833,6 +1957,9 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
         else if (occ3) hipLaunchKernelGGL((k_adjoint<false, 3, kTreeBinary>), grid, blk, lds, c->stream, a);
         else hipLaunchKernelGGL((k_adjoint<false, 2, kTreeBinary>), grid, blk, lds, c->stream, a);
     }
+    if (a.rep)
+        hipLaunchKernelGGL(k_fold_replicas, dim3((a.rep_count + 255) / 256), dim3(256), 0, c->stream, a.rep,
+                           a.rep_count, (const uint32_t *)c->coarse_texel, d_texel_grad);
     RH_CHECK(hipGetLastError());
     return 0;
 }

@@ -460,7 +460,8 @@ __device__ __forceinline__ void write_vertex_record(const AdjRecords &r, uint32_
         rt[0] = make_float4(__uint_as_float(so.alb_foot.idx[0]), __uint_as_float(so.alb_foot.idx[1]),
                             __uint_as_float(so.alb_foot.idx[2]), __uint_as_float(so.alb_foot.idx[3]));
         rt[r.lanes] = make_float4(so.alb_foot.kx, so.alb_foot.ky, so.alb_raw.x, so.alb_raw.y);
-        rt[2 * (size_t)r.lanes] = make_float4(so.alb_raw.z, so.alb_srgb ? 1.0f : 0.0f, __uint_as_float(so.alb_tex), 0.0f);
+        rt[2 * (size_t)r.lanes] = make_float4(so.alb_raw.z, so.alb_srgb ? 1.0f : 0.0f, __uint_as_float(so.alb_tex),
+                                              __int_as_float(so.alb_foot.lod));
     }
 }
 
