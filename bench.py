@@ -269,6 +269,20 @@ def run_batch(args, lib, A, scenes, knobs, wide):
         r.clear()
         r.reset_stats()
         rs.append(r)
+    # plain, scene after scene: each renderer's passes, then a wait before
+    # the next scene (every renderer has its own stream; without the wait the
+    # scenes' passes overlap on the GPU, reported apart as "overlapped")
+    t1 = time.perf_counter()
+    for r, s, it in zip(rs, ss, items):
+        reg = A.Region(lib, 0, 0, it.width, it.height)
+        for _ in range(it.spp):
+            r.render(s, reg)
+        r.synchronize()
+    t_plain = time.perf_counter() - t1
+    rays_plain = sum(total(r.ray_counts()) for r in rs)
+    for r in rs:
+        r.clear()
+        r.reset_stats()
     t1 = time.perf_counter()
     for r, s, it in zip(rs, ss, items):
         reg = A.Region(lib, 0, 0, it.width, it.height)
@@ -276,8 +290,8 @@ def run_batch(args, lib, A, scenes, knobs, wide):
             r.render(s, reg)
     for r in rs:
         r.synchronize()
-    t_plain = time.perf_counter() - t1
-    rays_plain = sum(total(r.ray_counts()) for r in rs)
+    t_overlap = time.perf_counter() - t1
+    rays_overlap = sum(total(r.ray_counts()) for r in rs)
     for r in rs:
         r.close()
     del rs
@@ -323,6 +337,8 @@ def run_batch(args, lib, A, scenes, knobs, wide):
         "batched_mrays": round(rays_batch / t_batch / 1e6, 2), "plain_mrays": round(rays_plain / t_plain / 1e6, 2),
         "speedup": round(t_plain / t_batch, 3), "batched_s": round(t_batch, 4), "plain_s": round(t_plain, 4),
         "rays": rays_batch, "rays_plain": rays_plain, "passes": b.passes,
+        "plain_overlapped_mrays": round(rays_overlap / t_overlap / 1e6, 2),
+        "speedup_vs_overlapped": round(rays_batch / t_batch / (rays_overlap / t_overlap), 3),
         "knobs": {"path_tail": best[0], "lds_stack_depth": best[1], "blocks_per_cu": best[2], "use_graph": 1},
         "plain_knobs": knobs,
         "autotune": {"columns": ["path_tail", "lds_stack_depth", "blocks_per_cu", "mrays_first_3_passes"],
@@ -462,11 +478,10 @@ def run_large(args, lib, A, scenes, knobs, wide):
             rr.reset_stats()
             rs.append(rr)
         t3 = time.perf_counter()
-        for rr, sc, (build, w, h, n) in zip(rs, ss, builds):
+        for rr, sc, (build, w, h, n) in zip(rs, ss, builds):  # scene after scene
             rg = A.Region(lib, 0, 0, w, h)
             for _ in range(n):
                 rr.render(sc, rg)
-        for rr in rs:
             rr.synchronize()
         t_plain = time.perf_counter() - t3
         rays_plain = sum(total(rr.ray_counts()) for rr in rs)
