@@ -242,6 +242,21 @@ class Oracle:
                                _p(counts), _p(sh_temp), _p(sh))
         return clean, final, counts, sh
 
+    def render_sh_regions(self, view, cam, w, h, passes, rects):
+        """render_sh over several regions of one framebuffer, each region with
+        its own iteration count (a RegionContext each), region after region."""
+        clean = np.zeros((h, w, 4), np.float32)
+        final = np.zeros((h, w, 4), np.float32)
+        sh_temp = np.zeros((h, w, 12), np.float32)
+        sh = np.zeros((h, w, 12), np.float32)
+        counts = np.zeros(3, np.uint64)
+        for rect in rects:
+            rc = np.array(rect, np.int32)
+            it = np.zeros(1, np.int32)
+            self.lib.orc_render_sh(C.byref(view), C.byref(cam), w, h, _p(rc), passes, _p(it), _p(clean), _p(final),
+                                   _p(counts), _p(sh_temp), _p(sh))
+        return clean, final, counts, sh
+
     def render(self, view, cam, w, h, passes, rect=None):
         rect = np.array(rect or (0, 0, w, h), np.int32)
         clean = np.zeros((h, w, 4), np.float32)

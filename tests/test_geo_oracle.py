@@ -32,3 +32,21 @@ def test_oracle_lightmap_and_sh_match_reference(ref_lib, host_lib, oracle, persp
     assert img_r.tobytes() == img_o.tobytes()
     assert sh_r.tobytes() == sh_o.tobytes()
     assert np.abs(sh_r).max() > 0 and covered == 0
+
+
+def test_oracle_lightmap_regions_match_reference(ref_lib, host_lib, oracle):
+    w, h, spp = 40, 36, 2
+    rects = [(0, 0, 40, 15), (5, 15, 30, 21)]
+    build = lambda s: scenes.lightmap_box(s, k=2, width=w, height=h, spp=spp, depth=3)
+    r = A.Renderer(ref_lib, w, h, A.RENDERER_REF)
+    s = r.create_scene()
+    build(s)
+    for rect in rects:
+        reg = A.Region(ref_lib, *rect)
+        for _ in range(spp):
+            r.render(s, reg)
+    hs = standalone_scene(host_lib)
+    build(hs)
+    _, img_o, _, sh_o = oracle.render_sh_regions(hs.raw_view(), hs.camera_record(0), w, h, spp, rects)
+    assert r.pixels().tobytes() == img_o.tobytes()
+    assert r.sh_data().tobytes() == sh_o.tobytes()

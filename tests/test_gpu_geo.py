@@ -70,3 +70,28 @@ def test_sh_cleared_and_batch_refuses_geometry_cameras(host_lib):
     b = A.Batch(host_lib)
     with pytest.raises(RuntimeError):
         b.add(s, w, h, 2)
+
+
+def test_lightmap_regions_match_oracle(host_lib, oracle):
+    """The lightmap baked in two regions (each its own RegionContext): triangle
+    boxes clipped to each region, as the reference clips them."""
+    w, h, spp = 40, 36, 3
+    rects = [(0, 0, 40, 15), (5, 15, 30, 21)]
+    build = lambda s: scenes.lightmap_box(s, k=2, width=w, height=h, spp=spp, depth=3)
+    r = A.Renderer(host_lib, w, h, A.RENDERER_HIP)
+    s = r.create_scene()
+    build(s)
+    for rect in rects:
+        reg = A.Region(host_lib, *rect)
+        for _ in range(spp):
+            r.render(s, reg)
+    acc = np.zeros((h, w, 4), np.float32)
+    r.read_accum(acc.ctypes.data)
+    hs = standalone_scene(host_lib)
+    build(hs)
+    clean, _, counts, sh_o = oracle.render_sh_regions(hs.raw_view(), hs.camera_record(0), w, h, spp, rects)
+    e = float(np.sqrt(np.mean((acc[..., :3].astype(np.float64) - clean[..., :3]) ** 2)))
+    se = float(np.abs(r.sh_data() - sh_o).max())
+    print(f"regions: RMSE {e:.3e}, SH max diff {se:.3e}")
+    assert e <= 1e-3 and se <= 1e-5 * np.abs(sh_o).max()
+    assert np.all(acc[:15, :, :3] >= 0) and np.abs(acc[15:, :5]).max() == 0  # outside both regions
