@@ -74,7 +74,8 @@ int rh_get_ray_counts(rh_context *ctx, rh_ray_counts *out);
  * "tail_occupancy" (2 or 3 waves per SIMD for the path-tail kernel: registers vs spills),
  * "refill" (0/1: closest-hit trace as a persistent kernel whose lanes take new rays as theirs finish;
  *   binary trees only),
- * "trace_occupancy" (0 or 5: the 8-wide trace and shadow kernels as compiled, or capped at 5 waves per SIMD),
+ * "trace_occupancy" (0, 5 or 6: the 8-wide trace and shadow kernels as compiled, or capped at 5 or 6 waves
+ *   per SIMD),
  * "max_bounces" (1..16: the clamp of max_total_depth; 10, the reference's MAX_BOUNCES, by default:
  *   larger values render deeper paths than the reference can, an extension),
  * "adjoint_staged" (1, the default: rh_adjoint_pass replays the pass as staged wavefront passes whose

@@ -31,7 +31,7 @@ SORT_MODES = ((0, 32), (1, 8), (1, 32))
 PATH_TAILS = (-1, 0, 1, 2, 3, 4)
 TAIL_OCCUPANCY = (2, 3)
 REFILL = (0, 1)
-TRACE_OCCUPANCY = (0, 5)
+TRACE_OCCUPANCY = (0, 5, 6)
 KNOBS = ("lds_stack_depth", "sort_rays", "sort_bits", "path_tail", "tail_occupancy", "refill", "trace_occupancy")
 
 

@@ -738,7 +738,7 @@ struct rh_context {
     int path_tail = 2;  // first bounce run by the path-tail kernel (-1: none)
     int tail_occupancy = 2;    // waves per SIMD the path-tail kernel is compiled for (2 or 3)
     int refill = 0;            // closest-hit trace: persistent kernel with lane refill
-    int trace_occupancy = 0;   // 8-wide trace/shadow kernels: 0 = as compiled (4 waves/SIMD), 5 = capped registers
+    int trace_occupancy = 0;   // 8-wide trace/shadow kernels: 0 = as compiled (4 waves/SIMD), 5 or 6 = capped registers
     int max_bounces = RH_MAX_BOUNCES; // max_total_depth clamp (the reference's 10; up to RH_MAX_BOUNCES_EXT)
     int adjoint_staged = 1;           // adjoint replay as staged wavefront passes (0: one lane per path)
     uint32_t *geo_winner = nullptr;   // geometry camera: winning triangle per lane
@@ -957,6 +957,14 @@ int launch_trace(rh_context *c, const TraceArgs &a, int grid_rays, bool any_hit)
             if (count) hipLaunchKernelGGL((k_trace<false, true, kTreeWide, 5>), grid, blk, lds, c->stream, a);
             else hipLaunchKernelGGL((k_trace<false, false, kTreeWide, 5>), grid, blk, lds, c->stream, a);
         }
+    } else if (c->sc.wnodes && c->trace_occupancy == 6) {
+        if (any_hit) {
+            if (count) hipLaunchKernelGGL((k_trace<true, true, kTreeWide, 6>), grid, blk, lds, c->stream, a);
+            else hipLaunchKernelGGL((k_trace<true, false, kTreeWide, 6>), grid, blk, lds, c->stream, a);
+        } else {
+            if (count) hipLaunchKernelGGL((k_trace<false, true, kTreeWide, 6>), grid, blk, lds, c->stream, a);
+            else hipLaunchKernelGGL((k_trace<false, false, kTreeWide, 6>), grid, blk, lds, c->stream, a);
+        }
     } else if (c->sc.wnodes) {
         if (any_hit) {
             if (count) hipLaunchKernelGGL((k_trace<true, true, kTreeWide>), grid, blk, lds, c->stream, a);
@@ -1035,6 +1043,8 @@ int launch_shade(rh_context *c, const ShadeArgs &a, int grid_rays) {
         const dim3 grid((unsigned)std::max<size_t>(1, (lanes + kBlock - 1) / kBlock)), blk(kBlock);
         if (c->sc.wnodes && c->trace_occupancy == 5)
             hipLaunchKernelGGL((k_shadow<kTreeWide, 5>), grid, blk, lds_bytes(c), c->stream, s);
+        else if (c->sc.wnodes && c->trace_occupancy == 6)
+            hipLaunchKernelGGL((k_shadow<kTreeWide, 6>), grid, blk, lds_bytes(c), c->stream, s);
         else if (c->sc.wnodes) hipLaunchKernelGGL(k_shadow<kTreeWide>, grid, blk, lds_bytes(c), c->stream, s);
         else hipLaunchKernelGGL(k_shadow<kTreeBinary>, grid, blk, lds_bytes(c), c->stream, s);
         RH_CHECK(hipGetLastError());
@@ -1536,7 +1546,7 @@ int rh_set_option(rh_context *c, const char *name, int value) {
         if (value < 1 || value > RH_MAX_BOUNCES_EXT) return arg_error("max_bounces must be 1..16");
         c->max_bounces = value;
     } else if (n == "trace_occupancy") {
-        if (value != 0 && value != 5) return arg_error("trace_occupancy must be 0 or 5");
+        if (value != 0 && value != 5 && value != 6) return arg_error("trace_occupancy must be 0, 5 or 6");
         c->trace_occupancy = value;
     } else if (n == "adjoint_occupancy") {
         if (value != 2 && value != 3) return arg_error("adjoint_occupancy must be 2 or 3");
