@@ -219,13 +219,14 @@ def run_adjoint(args, lib, A, scenes, knobs, wide):
     adj_rays = rc["adjoint"] + rc["adjoint_shadow"]
     res = adjoint.descend(dr, init, target, hidden, steps=args.adj_steps, lr=0.02, optimize=slice(0, n))
     out = {
-        "metric": "adjoint s/iter (forward + path-replay adjoint + Adam)",
+        "metric": "adjoint s/iter (forward + adjoint from the taped vertex records, or a path replay, + Adam)",
         "s_per_iter": round(res.s_per_iter, 5), "steps": res.steps, "seconds": round(res.seconds, 3),
         "forward_ms": round((t1 - t0) * 1e3, 3), "adjoint_ms": round((t2 - t1) * 1e3, 3),
         "forward_mrays": round(fwd_rays / (t1 - t0) / 1e6, 2), "adjoint_mrays": round(adj_rays / (t2 - t1) / 1e6, 2),
         "loss_first": res.loss_first, "loss_last": res.loss_last,
         "albedo_rmse_first": round(res.albedo_rmse_first, 5), "albedo_rmse_last": round(res.albedo_rmse_last, 5),
         "knobs": knobs,
+        "tape": dr.tape,
         "config": {"workload": f"cornell_refined {spec.triangles} tris, {n} albedo parameters x 3, "
                                f"{args.adj_size}x{args.adj_size}, {args.adj_spp} spp, depth 6, roulette off",
                    "trees": "8-wide" if wide else "binary",
@@ -423,7 +424,7 @@ def run_large(args, lib, A, scenes, knobs, wide):
 
     # texel-gradient descent on 4 textures
     if args.large_adj_steps > 0:
-        dr = adjoint.DiffRenderer(lib, r, s, W, H, spp, len(s.view()["materials"]))
+        dr = adjoint.DiffRenderer(lib, r, s, W, H, spp, len(s.view()["materials"]), tape_texels=True)
         ids = tuple(spec.info["adjoint_textures"])
         orig = np.stack(spec.info["adjoint_texels"]).astype(np.float32) / 255.0
         noise = np.random.default_rng(scenes.SEED_BASE + 55).normal(0.0, 0.1, orig.shape).astype(np.float32)
@@ -446,7 +447,7 @@ def run_large(args, lib, A, scenes, knobs, wide):
         torch.cuda.synchronize()
         dt2 = time.perf_counter() - t2
         out["texel_adjoint"] = {
-            "metric": "texel-gradient descent s/iter (forward + path-replay adjoint + fold + Adam)",
+            "metric": "texel-gradient descent s/iter (forward + adjoint from the taped vertex records, or a path replay, + fold + Adam)",
             "s_per_iter": round(dt2 / args.large_adj_steps, 4), "steps": args.large_adj_steps,
             "parameters": int(params.numel()), "loss_first": losses[0], "loss_last": losses[-1],
             "textures": list(ids)}

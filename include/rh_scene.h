@@ -126,6 +126,8 @@ int rhx_get_stats(rhx_renderer *r, rh_stats *out);
 int rhx_reset_stats(rhx_renderer *r);
 int rhx_get_ray_counts(rhx_renderer *r, rh_ray_counts *out);
 int rhx_set_option(rhx_renderer *r, const char *name, int value);
+/* the current value of a knob of rh_set_option (HIP backend only) */
+int rhx_get_option(rhx_renderer *r, const char *name, int *value);
 /* Device-wide synchronisation, and the per-kernel profile of the device layer
  * (HIP backend only; see rh_get_kernel_profile in rh_hip.h). */
 int rhx_synchronize(rhx_renderer *r);

@@ -29,7 +29,12 @@ class DiffRenderer:
     """A HIP renderer + scene pair whose material colours are the parameters."""
 
     def __init__(self, lib: A.RayLib, renderer: A.Renderer, scene: A.Scene, width: int, height: int, spp: int,
-                 n_materials: int, device: str = "cuda"):
+                 n_materials: int, device: str = "cuda", tape: bool | None = None, tape_texels: bool = False):
+        """tape: the forward passes keep their vertex records on the device
+        (the "adjoint_tape" option) so that adjoint() needs no replay; None
+        turns it on when the records of all passes take at most half of the
+        free device memory.  tape_texels: the records include texel
+        footprints (for texel gradients; 4x the memory)."""
         if not lib.has_adjoint:
             raise RuntimeError(f"{lib.path} has no adjoint entry points (HIP backend required)")
         self.lib, self.r, self.s = lib, renderer, scene
@@ -40,6 +45,13 @@ class DiffRenderer:
         self.rays_adjoint = 0
         self._layout = None
         self._grad_textures = None
+        if tape is None:
+            vmax = renderer.get_option("max_bounces") + 1
+            need = spp * width * height * (vmax * (64 if tape_texels else 16) + 4)
+            tape = need <= 0.5 * torch.cuda.mem_get_info()[0]
+        self.tape = bool(tape)
+        self.r.set_option("adjoint_tape", spp if self.tape else 0)
+        self.r.set_option("adjoint_tape_texels", 1 if (self.tape and tape_texels) else 0)
 
     def tex_layout(self) -> dict:
         """The scene's texture records, mip blocks and texel count (they do not
@@ -219,7 +231,7 @@ def textured_cornell(lib: A.RayLib, width=128, height=128, spp=8, depth=4, tex_s
     s = r.create_scene()
     spec = scenes.textured_cornell(s, tex_size=tex_size, width=width, height=height, spp=spp, depth=depth,
                                    mipmaps=mipmaps, textures=textures)
-    return DiffRenderer(lib, r, s, width, height, spp, len(s.view()["materials"])), spec
+    return DiffRenderer(lib, r, s, width, height, spp, len(s.view()["materials"]), tape_texels=True), spec
 
 
 @dataclass

@@ -228,6 +228,7 @@ class RayLib:
             "rhx_read_accum": (c_int, [P, P, c_size_t]),
             "rhx_set_gradient_textures": (c_int, [P, P, c_uint32]),
             "rhx_build_fast_bvh": (c_int, [P, c_size_t, c_int, P, c_size_t, POINTER(c_uint32), P]),
+            "rhx_get_option": (c_int, [P, C.c_char_p, POINTER(c_int)]),
         }
         optional.update({
             "rhx_batch_create": (c_int, [c_int, POINTER(P)]),
@@ -324,6 +325,11 @@ class Renderer:
 
     def set_option(self, name: str, value: int):
         self.lib.check(self.lib.lib.rhx_set_option(self.h, name.encode(), int(value)), f"rhx_set_option({name})")
+
+    def get_option(self, name: str) -> int:
+        v = c_int(0)
+        self.lib.check(self.lib.lib.rhx_get_option(self.h, name.encode(), C.byref(v)), f"rhx_get_option({name})")
+        return v.value
 
     def synchronize(self):
         self.lib.check(self.lib.lib.rhx_synchronize(self.h), "rhx_synchronize")

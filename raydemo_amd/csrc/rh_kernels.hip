@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -750,6 +751,14 @@ struct rh_context {
     float *rep = nullptr;                   // kTexelReplicas copies of the slots' gradients
     uint32_t rep_count = 0;
     bool coarse_valid = false;
+    // the adjoint tape: passes 1..tape_passes after a clear keep their vertex
+    // records (slot = iteration - 1) so that the adjoint needs no replay
+    int tape_passes = 0, tape_texels = 0;
+    size_t tape_lanes = 0, tape_vmax = 0;
+    int tape_texel_slots = 0;
+    float4 *tape_rec = nullptr, *tape_rec_tex = nullptr;
+    int *tape_nverts = nullptr;
+    std::vector<std::array<int, 4>> tape_rect; // recorded rect per slot (w = 0: empty)
     uint32_t n_textures = 0;
     uint32_t *grad_tex_mask = nullptr; // textures whose texels get gradients (null: all)
     size_t grad_tex_words = 0;
@@ -1061,6 +1070,7 @@ int launch_shade(rh_context *c, const ShadeArgs &a, int grid_rays) {
     cb.temp = a.temp;
     cb.out_rgba = a.out_rgba;
     cb.rec = a.rec;
+    cb.frame_w = a.w;
     hipLaunchKernelGGL(k_nee_combine, dim3(blocks), dim3(kBlock), 0, c->stream, cb);
     RH_CHECK(hipGetLastError());
     return 0;
@@ -1165,6 +1175,9 @@ void rh_destroy(rh_context *c) {
     free_ptr(c->coarse_map);
     free_ptr(c->coarse_texel);
     free_ptr(c->rep);
+    free_ptr(c->tape_rec);
+    free_ptr(c->tape_rec_tex);
+    free_ptr(c->tape_nverts);
     free_ptr(c->temp);
     free_ptr(c->clean);
     free_ptr(c->final_);
@@ -1207,12 +1220,15 @@ int rh_resize(rh_context *c, int w, int h) {
     return ensure_rays(c, (size_t)w * h);
 }
 
+static void tape_forget(rh_context *c);
+
 int rh_clear(rh_context *c, const float rgba[4]) {
     const size_t n = (size_t)c->w * c->h;
     if (!n) return 0;
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->clean, n,
                        make_float4(rgba[0], rgba[1], rgba[2], rgba[3]));
     RH_CHECK(hipGetLastError());
+    tape_forget(c);
     // the accumulated SH data too (FramebufferRef.cpp Clear)
     if (c->sh_clean) RH_CHECK(hipMemsetAsync(c->sh_clean, 0, n * 12 * sizeof(float), c->stream));
     return 0;
@@ -1243,6 +1259,7 @@ int rh_upload_scene(rh_context *c, const rh_scene_view *v) {
     c->n_mesh_instances = (uint32_t)v->mesh_instances_count;
     c->host_tex_mips.assign(v->tex_mips, v->tex_mips + (size_t)v->textures_count * RH_NUM_MIP_LEVELS);
     c->coarse_valid = false;
+    tape_forget(c);
     c->n_texels = (uint32_t)v->texels_count;
     c->n_textures = (uint32_t)v->textures_count;
     c->have_scene = true;
@@ -1267,7 +1284,10 @@ int rh_set_halton(rh_context *c, const float *seq, size_t count) {
 // mix and tonemap).  Otherwise the staged adjoint replay of the same pass: the
 // stages write vertex records instead of pixels, every bounce is staged, and
 // rays are counted as adjoint rays.
-int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec) {
+// rec: the pass writes vertex records (every bounce staged); keep_image: and
+// renders its image as usual (the adjoint tape), else the image is left alone
+// and the rays are counted as the adjoint's (the replay)
+int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool keep_image = false) {
     if (!c->have_scene) return arg_error("no scene uploaded");
     const int rx = p->rect[0], ry = p->rect[1], rw = p->rect[2], rhh = p->rect[3];
     if (rw <= 0 || rhh <= 0 || rx < 0 || ry < 0 || rx + rw > c->w || ry + rhh > c->h)
@@ -1308,8 +1328,9 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec) {
     StageTimer tm(c);
     tm.mark(); // 0
     int *cnt[2] = {c->counters + 0, c->counters + 1};
-    unsigned long long *count_primary = rec ? c->stats_dev + 5 : c->stats_dev + 0;
-    unsigned long long *count_secondary = rec ? c->stats_dev + 5 : c->stats_dev + 1;
+    const bool replay = rec && !keep_image;
+    unsigned long long *count_primary = replay ? c->stats_dev + 5 : c->stats_dev + 0;
+    unsigned long long *count_secondary = replay ? c->stats_dev + 5 : c->stats_dev + 1;
     if (geo) {
         // rays and hits from the mesh's texture-space footprint, no trace
         GeoArgs g;
@@ -1353,12 +1374,12 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec) {
     sa.pi = pi;
     sa.w = c->w;
     sa.halton = c->halton;
-    sa.temp = rec ? nullptr : c->temp;
+    sa.temp = replay ? nullptr : c->temp;
     sa.primary = 1;
     if (rec) {
         sa.rec = *rec;
         sa.rec.vertex = 0;
-        sa.shadow_counter = c->stats_dev + 6;
+        if (replay) sa.shadow_counter = c->stats_dev + 6;
     }
     sa.out_rays = c->rays[1];
     sa.out_aux = c->aux[1];
@@ -1459,7 +1480,7 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec) {
         q ^= 1;
     }
 
-    if (rec) return 0; // the replay leaves the image alone
+    if (replay) return 0; // the replay leaves the image alone
     if (out_sh) {
         hipLaunchKernelGGL(k_sh_finish, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->temp, c->sh_temp,
                            c->sh_clean, rx, ry, rw, rhh, c->w, pc.mix_k);
@@ -1480,7 +1501,62 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec) {
     return 0;
 }
 
-int rh_render_pass(rh_context *c, const rh_pass_desc *p) { return run_pass(c, p, nullptr); }
+static void tape_forget(rh_context *c) {
+    for (auto &r : c->tape_rect) r = {0, 0, 0, 0};
+}
+
+// Buffers of the tape for passes of `lanes` lanes; false when it is off or the
+// pass cannot be taped (geometry cameras, SH output).
+static int tape_slot(rh_context *c, const rh_pass_desc *p, size_t lanes) {
+    if (c->tape_passes <= 0 || p->iteration < 1 || p->iteration > c->tape_passes) return -1;
+    if (p->cam.type == RH_CAM_GEO || (p->cam.pass_settings.flags & RH_OUTPUT_SH)) return -1;
+    return p->iteration - 1;
+}
+
+static int tape_alloc(rh_context *c, size_t lanes) {
+    const size_t vmax = (size_t)std::min(c->max_bounces, RH_MAX_BOUNCES_EXT) + 1;
+    const int slots = c->tape_passes;
+    if (c->tape_rec && c->tape_lanes == lanes && c->tape_vmax == vmax && (int)c->tape_rect.size() == slots &&
+        c->tape_texel_slots == (c->tape_texels ? slots : 0))
+        return 0;
+    RH_CHECK(hipStreamSynchronize(c->stream));
+    free_ptr(c->tape_rec);
+    free_ptr(c->tape_rec_tex);
+    free_ptr(c->tape_nverts);
+    c->tape_rec = c->tape_rec_tex = nullptr;
+    c->tape_nverts = nullptr;
+    c->tape_rect.assign(slots, {0, 0, 0, 0});
+    c->tape_lanes = lanes;
+    c->tape_vmax = vmax;
+    c->tape_texel_slots = c->tape_texels ? slots : 0;
+    RH_CHECK(hipMalloc(&c->tape_rec, (size_t)slots * vmax * lanes * sizeof(float4)));
+    if (c->tape_texels) RH_CHECK(hipMalloc(&c->tape_rec_tex, (size_t)slots * vmax * 3 * lanes * sizeof(float4)));
+    RH_CHECK(hipMalloc(&c->tape_nverts, (size_t)slots * lanes * sizeof(int)));
+    return 0;
+}
+
+static AdjRecords tape_records(rh_context *c, int slot, const rh_pass_desc *p, bool texels) {
+    const size_t lanes = c->tape_lanes, vmax = c->tape_vmax;
+    AdjRecords r{c->tape_rec + (size_t)slot * vmax * lanes,
+                 texels ? c->tape_rec_tex + (size_t)slot * vmax * 3 * lanes : nullptr,
+                 c->tape_nverts + (size_t)slot * lanes, (uint32_t)lanes, 0, {p->rect[0], p->rect[1], p->rect[2]}};
+    return r;
+}
+
+int rh_render_pass(rh_context *c, const rh_pass_desc *p) {
+    const size_t lanes = (size_t)std::max(0, p->rect[2]) * std::max(0, p->rect[3]);
+    const int slot = tape_slot(c, p, lanes);
+    if (slot < 0) return run_pass(c, p, nullptr);
+    int rc = tape_alloc(c, lanes);
+    if (rc) return rc;
+    c->tape_rect[slot] = {0, 0, 0, 0};
+    AdjRecords rec = tape_records(c, slot, p, c->tape_texels != 0);
+    RH_CHECK(hipMemsetAsync(rec.nverts, 0, lanes * sizeof(int), c->stream));
+    rc = run_pass(c, p, &rec, true);
+    if (rc) return rc;
+    c->tape_rect[slot] = {p->rect[0], p->rect[1], p->rect[2], p->rect[3]};
+    return 0;
+}
 
 int rh_read_pixels(rh_context *c, float *rgba, size_t count) {
     const size_t n = (size_t)c->w * c->h * 4;
@@ -1540,6 +1616,13 @@ int rh_set_option(rh_context *c, const char *name, int value) {
         c->tail_occupancy = value;
     } else if (n == "refill") {
         c->refill = value != 0;
+    } else if (n == "adjoint_tape") {
+        if (value < 0 || value > 4096) return arg_error("adjoint_tape must be 0..4096 passes");
+        c->tape_passes = value;
+        tape_forget(c);
+    } else if (n == "adjoint_tape_texels") {
+        c->tape_texels = value != 0;
+        tape_forget(c);
     } else if (n == "adjoint_staged") {
         c->adjoint_staged = value != 0;
     } else if (n == "max_bounces") {
@@ -1570,6 +1653,8 @@ int rh_get_option(rh_context *c, const char *name, int *value) {
     else if (n == "trace_occupancy") *value = c->trace_occupancy;
     else if (n == "max_bounces") *value = c->max_bounces;
     else if (n == "adjoint_staged") *value = c->adjoint_staged;
+    else if (n == "adjoint_tape") *value = c->tape_passes;
+    else if (n == "adjoint_tape_texels") *value = c->tape_texels;
     else if (n == "adjoint_occupancy") *value = c->adjoint_occupancy;
     else if (n == "defer_nee") *value = c->defer_nee;
     else if (n == "stack_overflows") {
@@ -1927,6 +2012,22 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
         a.nee_aux = c->nee_aux;
         a.nee_term = c->nee_term;
         a.nee_slots = c->nee_slots;
+    }
+    const int tslot = (p->iteration >= 1 && p->iteration <= (int)c->tape_rect.size()) ? p->iteration - 1 : -1;
+    if (tslot >= 0 && c->tape_rect[tslot][0] == rx && c->tape_rect[tslot][1] == ry && c->tape_rect[tslot][2] == rw &&
+        c->tape_rect[tslot][3] == rhh && (!d_texel_grad || c->tape_texels)) {
+        // the forward pass kept this pass's vertex records: no replay
+        const AdjRecords tr = tape_records(c, tslot, p, d_texel_grad != nullptr);
+        a.rec = tr.rec;
+        a.rec_tex = tr.rec_tex;
+        KernelMark km(c, 4);
+        hipLaunchKernelGGL(k_adjoint_backward, dim3((unsigned)((lanes + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           c->stream, a, (const int *)tr.nverts);
+        if (a.rep)
+            hipLaunchKernelGGL(k_fold_replicas, dim3((a.rep_count + 255) / 256), dim3(256), 0, c->stream, a.rep,
+                               a.rep_count, (const uint32_t *)c->coarse_texel, d_texel_grad);
+        RH_CHECK(hipGetLastError());
+        return 0;
     }
     if (c->adjoint_staged) {
         // the forward stage kernels replay the pass and write the vertex

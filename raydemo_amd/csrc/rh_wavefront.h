@@ -575,7 +575,8 @@ __device__ __forceinline__ void shade_body(const ShadeArgs &a, uint32_t base, ui
                 // the combine stage finds the record through pend_c.w
                 a.pend_c[i].w = __int_as_float((int)lane);
             }
-        } else if (a.temp) {
+        }
+        if (a.temp) {
             float4 &dst = a.temp[(size_t)py * a.w + px];
             if (a.primary) {
                 dst = make_float4(so.col.x, so.col.y, so.col.z, so.col.w);
@@ -746,7 +747,9 @@ struct CombineArgs {
     const uint8_t *nee_vis;
     float4 *temp;     // pixels (pend_c.w holds the pixel index), or
     float4 *out_rgba; // per-ray output (parity entry point), or
-    AdjRecords rec;   // vertex records (pend_c.w holds the lane)
+    AdjRecords rec;   // vertex records (pend_c.w holds the lane), and the
+                      // pixels too when temp is set (the adjoint tape)
+    int frame_w;      // framebuffer width (pixel of a lane, with records and pixels)
 };
 
 __device__ __forceinline__ void combine_body(const CombineArgs &a, uint32_t base) {
@@ -774,6 +777,17 @@ __device__ __forceinline__ void combine_body(const CombineArgs &a, uint32_t base
     v.y += add.y;
     v.z += add.z;
     *dst = v;
+    if (a.rec.rec && a.temp) {
+        // the tape: the pixel too, found from the lane
+        const int lane = __float_as_int(pc.w);
+        float4 &px = a.temp[(size_t)(a.rec.rect[1] + lane / a.rec.rect[2]) * a.frame_w + a.rec.rect[0] +
+                            lane % a.rec.rect[2]];
+        float4 u = px;
+        u.x += add.x;
+        u.y += add.y;
+        u.z += add.z;
+        px = u;
+    }
 }
 
 // MixWith + tonemap of one pixel (FramebufferRef.cpp:80-86, RendererRef.cpp:301-324):

@@ -129,6 +129,12 @@ int rhx_set_option(rhx_renderer *r, const char *name, int value) {
     return rc;
 }
 
+int rhx_get_option(rhx_renderer *r, const char *name, int *value) {
+    const int rc = rh_get_option(r->r->context(), name, value);
+    if (rc) g_err = rh_last_error();
+    return rc;
+}
+
 int rhx_synchronize(rhx_renderer *r) {
     const int rc = rh_device_synchronize(r->r->context());
     if (rc) g_err = rh_last_error();
