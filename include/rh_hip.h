@@ -81,6 +81,9 @@ int rh_get_ray_counts(rh_context *ctx, rh_ray_counts *out);
  * "adjoint_staged" (1, the default: rh_adjoint_pass replays the pass as staged wavefront passes whose
  *   stages write the vertex records, then one kernel forms the gradient; 0: one lane per path),
  * "adjoint_occupancy" (2 or 3 waves per SIMD for the one-lane-per-path adjoint kernel),
+ * "adjoint_tape" (0..4096: passes 1..N after a clear keep their vertex records on the device, every bounce
+ *   staged, and rh_adjoint_pass of such a pass needs no replay; 0, the default: off),
+ * "adjoint_tape_texels" (0/1: the tape records texel footprints too, for texel gradients),
  * "defer_nee" (0/1: shadow rays of next-event estimation as their own stage). */
 int rh_set_option(rh_context *ctx, const char *name, int value);
 int rh_get_option(rh_context *ctx, const char *name, int *value);
