@@ -101,18 +101,32 @@ __global__ void __launch_bounds__(kBlock) k_lbvh_bounds(const float *__restrict_
 // 1024 / extent per axis (infinite for a flat axis, as in the reference).
 __global__ void __launch_bounds__(kBlock) k_lbvh_scale(const float *__restrict__ partial, int blocks,
                                                       float *__restrict__ box) {
-    if (threadIdx.x != 0) return;
+    __shared__ float s[6][kBlock];
     const float big = 3.402823466e+38f;
     float lo[3] = {big, big, big}, hi[3] = {-big, -big, -big};
-    for (int b = 0; b < blocks; b++)
+    for (int b = threadIdx.x; b < blocks; b += kBlock)
         for (int a = 0; a < 3; a++) {
             lo[a] = min_std(lo[a], partial[6 * b + a]);
             hi[a] = max_std(hi[a], partial[6 * b + 3 + a]);
         }
     for (int a = 0; a < 3; a++) {
-        box[a] = lo[a];
-        box[3 + a] = 1024.0f / (hi[a] - lo[a]);
+        s[a][threadIdx.x] = lo[a];
+        s[3 + a][threadIdx.x] = hi[a];
     }
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int a = 0; a < 3; a++) {
+                s[a][threadIdx.x] = min_std(s[a][threadIdx.x], s[a][threadIdx.x + w]);
+                s[3 + a][threadIdx.x] = max_std(s[3 + a][threadIdx.x], s[3 + a][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int a = 0; a < 3; a++) {
+            box[a] = s[a][0];
+            box[3 + a] = 1024.0f / (s[3 + a][0] - s[a][0]);
+        }
 }
 
 // Morton code of each primitive's box centre (Core.cpp:552-562), with its

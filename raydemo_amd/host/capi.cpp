@@ -174,15 +174,15 @@ int rhx_build_fast_bvh(const float *boxes, size_t n, int device, rh_bvh_node *ou
     return guarded([&] {
         *out_count = 0;
         if (nodes_cap < 2 * n) throw std::runtime_error("node output must hold 2 n nodes");
-        std::vector<RayHip::prim_t> prims(n);
-        for (size_t i = 0; i < n; i++) {
-            prims[i].i0 = prims[i].i1 = prims[i].i2 = 0;
-            memcpy(prims[i].bbox_min.v, boxes + 6 * i, 3 * sizeof(float));
-            memcpy(prims[i].bbox_max.v, boxes + 6 * i + 3, 3 * sizeof(float));
-        }
         std::vector<rh_bvh_node> nodes;
         std::vector<uint32_t> indices;
         if (device < 0) {
+            std::vector<RayHip::prim_t> prims(n);
+            for (size_t i = 0; i < n; i++) {
+                prims[i].i0 = prims[i].i1 = prims[i].i2 = 0;
+                memcpy(prims[i].bbox_min.v, boxes + 6 * i, 3 * sizeof(float));
+                memcpy(prims[i].bbox_max.v, boxes + 6 * i + 3, 3 * sizeof(float));
+            }
             *out_count = RayHip::PreprocessPrims_HLBVH(prims.data(), n, nodes, indices);
         } else {
             // one builder per device, kept for the process
@@ -194,8 +194,14 @@ int rhx_build_fast_bvh(const float *boxes, size_t n, int device, rh_bvh_node *ou
                 b = nullptr;
                 throw std::runtime_error(std::string("fast BVH build: ") + rh_last_error());
             }
-            const RayHip::LbvhBottom fn = Ray::Hip::DeviceLbvhBottom(&b, device);
-            *out_count = RayHip::PreprocessPrims_HLBVH(prims.data(), n, nodes, indices, &fn);
+            // the boxes go to the device as they are (the bottom half reads
+            // nothing else of the primitives)
+            const RayHip::LbvhBottom fn = [b, boxes](const RayHip::prim_t *, size_t m, uint32_t offset,
+                                                     std::vector<uint32_t> &order, std::vector<rh_bvh_node> &bottom,
+                                                     std::vector<RayHip::HlbvhTreelet> &treelets) {
+                Ray::Hip::DeviceLbvhBoxes(b, boxes, m, offset, order, bottom, treelets);
+            };
+            *out_count = RayHip::PreprocessPrims_HLBVH(nullptr, n, nodes, indices, &fn);
         }
         memcpy(out_nodes, nodes.data(), nodes.size() * sizeof(rh_bvh_node));
         memcpy(out_indices, indices.data(), indices.size() * sizeof(uint32_t));

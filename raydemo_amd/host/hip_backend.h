@@ -123,6 +123,10 @@ public:
 // The fast build's bottom half (Morton sort, treelets) on GPU `device`, with
 // the builder created at first use in *builder (kept by the caller).
 RayHip::LbvhBottom DeviceLbvhBottom(rh_builder **builder, int device);
+// the same for boxes already packed as 6 floats each
+void DeviceLbvhBoxes(rh_builder *builder, const float *boxes, size_t n, uint32_t index_offset,
+                     std::vector<uint32_t> &order, std::vector<rh_bvh_node> &bottom,
+                     std::vector<RayHip::HlbvhTreelet> &treelets);
 
 class Renderer : public RendererBase {
     rh_context *ctx_ = nullptr;

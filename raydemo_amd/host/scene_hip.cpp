@@ -260,6 +260,21 @@ uint32_t Scene::AddMaterial(const mat_desc_t &m) {
 }
 
 // SceneRef::AddMesh (SceneRef.cpp:156-263), binary BVH
+void DeviceLbvhBoxes(rh_builder *builder, const float *boxes, size_t n, uint32_t index_offset,
+                     std::vector<uint32_t> &order, std::vector<rh_bvh_node> &bottom,
+                     std::vector<RayHip::HlbvhTreelet> &treelets) {
+    order.resize(n);
+    bottom.resize(2 * n);
+    std::vector<uint32_t> tl(3 * 4096);
+    uint32_t nodes = 0, count = 0;
+    if (rh_build_lbvh(builder, boxes, n, index_offset, order.data(), bottom.data(), bottom.size(), &nodes, tl.data(),
+                      &count) != 0)
+        throw std::runtime_error(std::string("fast BVH build: ") + rh_last_error());
+    bottom.resize(nodes);
+    treelets.resize(count);
+    for (uint32_t t = 0; t < count; t++) treelets[t] = {tl[3 * t], tl[3 * t + 1], tl[3 * t + 2]};
+}
+
 RayHip::LbvhBottom DeviceLbvhBottom(rh_builder **builder, int device) {
     return [builder, device](const RayHip::prim_t *prims, size_t n, uint32_t index_offset,
                              std::vector<uint32_t> &order, std::vector<rh_bvh_node> &bottom,
@@ -273,16 +288,7 @@ RayHip::LbvhBottom DeviceLbvhBottom(rh_builder **builder, int device) {
             memcpy(&boxes[6 * i], prims[i].bbox_min.v, 3 * sizeof(float));
             memcpy(&boxes[6 * i + 3], prims[i].bbox_max.v, 3 * sizeof(float));
         }
-        order.resize(n);
-        bottom.resize(2 * n);
-        std::vector<uint32_t> tl(3 * 4096);
-        uint32_t nodes = 0, count = 0;
-        if (rh_build_lbvh(*builder, boxes.data(), n, index_offset, order.data(), bottom.data(), bottom.size(), &nodes,
-                          tl.data(), &count) != 0)
-            throw std::runtime_error(std::string("fast BVH build: ") + rh_last_error());
-        bottom.resize(nodes);
-        treelets.resize(count);
-        for (uint32_t t = 0; t < count; t++) treelets[t] = {tl[3 * t], tl[3 * t + 1], tl[3 * t + 2]};
+        DeviceLbvhBoxes(*builder, boxes.data(), n, index_offset, order, bottom, treelets);
     };
 }
 
