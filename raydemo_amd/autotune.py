@@ -32,7 +32,9 @@ PATH_TAILS = (-1, 0, 1, 2, 3, 4)
 TAIL_OCCUPANCY = (2, 3)
 REFILL = (0, 1)
 TRACE_OCCUPANCY = (0, 5, 6)
-KNOBS = ("lds_stack_depth", "sort_rays", "sort_bits", "path_tail", "tail_occupancy", "refill", "trace_occupancy")
+SHADOW_OCCUPANCY = (-1, 0, 5)
+KNOBS = ("lds_stack_depth", "sort_rays", "sort_bits", "path_tail", "tail_occupancy", "refill", "trace_occupancy",
+         "shadow_occupancy")
 
 
 @dataclass
@@ -50,7 +52,7 @@ def candidates(depths=LDS_DEPTHS, sort_modes=SORT_MODES, tails=PATH_TAILS, occup
         if pt < 0 and occ != occupancies[0]:
             continue  # no tail launch: occupancy is moot
         yield {"lds_stack_depth": d, "sort_rays": so, "sort_bits": sb, "path_tail": pt, "tail_occupancy": occ,
-               "refill": 0, "trace_occupancy": 0}
+               "refill": 0, "trace_occupancy": 0, "shadow_occupancy": -1}
 
 
 def apply(renderer, cfg: dict):
@@ -106,6 +108,14 @@ def tune(renderer, scene, region_factory, passes: int = 6, depths=LDS_DEPTHS, so
             cfg["trace_occupancy"] = occ
             cfg["lds_stack_depth"] = d
             table.append(measure(renderer, scene, region_factory, cfg, passes))
+    # the shadow kernel's own register budget, on the best so far
+    second = max(table, key=lambda r: r["mrays"])
+    for so in SHADOW_OCCUPANCY[1:]:
+        if so == second["trace_occupancy"]:
+            continue
+        cfg = {k: second[k] for k in KNOBS}
+        cfg["shadow_occupancy"] = so
+        table.append(measure(renderer, scene, region_factory, cfg, passes))
     best_row = max(table, key=lambda r: r["mrays"])
     best = {k: best_row[k] for k in KNOBS}
     apply(renderer, best)

@@ -740,6 +740,7 @@ struct rh_context {
     int tail_occupancy = 2;    // waves per SIMD the path-tail kernel is compiled for (2 or 3)
     int refill = 0;            // closest-hit trace: persistent kernel with lane refill
     int trace_occupancy = 0;   // 8-wide trace/shadow kernels: 0 = as compiled (4 waves/SIMD), 5 or 6 = capped registers
+    int shadow_occupancy = -1; // the shadow kernel's own setting (-1: as trace_occupancy)
     int max_bounces = RH_MAX_BOUNCES; // max_total_depth clamp (the reference's 10; up to RH_MAX_BOUNCES_EXT)
     int adjoint_staged = 1;           // adjoint replay as staged wavefront passes (0: one lane per path)
     uint32_t *geo_winner = nullptr;   // geometry camera: winning triangle per lane
@@ -1050,9 +1051,10 @@ int launch_shade(rh_context *c, const ShadeArgs &a, int grid_rays) {
     {
         KernelMark km(c, 3);
         const dim3 grid((unsigned)std::max<size_t>(1, (lanes + kBlock - 1) / kBlock)), blk(kBlock);
-        if (c->sc.wnodes && c->trace_occupancy == 5)
+        const int occ = c->shadow_occupancy >= 0 ? c->shadow_occupancy : c->trace_occupancy;
+        if (c->sc.wnodes && occ == 5)
             hipLaunchKernelGGL((k_shadow<kTreeWide, 5>), grid, blk, lds_bytes(c), c->stream, s);
-        else if (c->sc.wnodes && c->trace_occupancy == 6)
+        else if (c->sc.wnodes && occ == 6)
             hipLaunchKernelGGL((k_shadow<kTreeWide, 6>), grid, blk, lds_bytes(c), c->stream, s);
         else if (c->sc.wnodes) hipLaunchKernelGGL(k_shadow<kTreeWide>, grid, blk, lds_bytes(c), c->stream, s);
         else hipLaunchKernelGGL(k_shadow<kTreeBinary>, grid, blk, lds_bytes(c), c->stream, s);
@@ -1616,6 +1618,10 @@ int rh_set_option(rh_context *c, const char *name, int value) {
         c->tail_occupancy = value;
     } else if (n == "refill") {
         c->refill = value != 0;
+    } else if (n == "shadow_occupancy") {
+        if (value != -1 && value != 0 && value != 5 && value != 6)
+            return arg_error("shadow_occupancy must be -1, 0, 5 or 6");
+        c->shadow_occupancy = value;
     } else if (n == "adjoint_tape") {
         if (value < 0 || value > 4096) return arg_error("adjoint_tape must be 0..4096 passes");
         c->tape_passes = value;
@@ -1654,6 +1660,7 @@ int rh_get_option(rh_context *c, const char *name, int *value) {
     else if (n == "max_bounces") *value = c->max_bounces;
     else if (n == "adjoint_staged") *value = c->adjoint_staged;
     else if (n == "adjoint_tape") *value = c->tape_passes;
+    else if (n == "shadow_occupancy") *value = c->shadow_occupancy;
     else if (n == "adjoint_tape_texels") *value = c->tape_texels;
     else if (n == "adjoint_occupancy") *value = c->adjoint_occupancy;
     else if (n == "defer_nee") *value = c->defer_nee;
