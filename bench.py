@@ -567,6 +567,39 @@ def run_build(args, lib, A, scenes):
     return out
 
 
+def run_lightmap(args, lib, A, scenes, knobs, wide):
+    """Lightmap baking with a geometry camera and SH output: the Cornell box
+    (faces as 16 x 16 grids, 8,706 triangles, each in its own lightmap cell)
+    baked into a 1024^2 lightmap, 16 passes, depth 4; Mrays/s of the passes
+    (secondary + shadow rays: the geometry camera traces no primary rays)."""
+    w = h = args.lightmap_size
+    r = A.Renderer(lib, w, h, A.RENDERER_HIP, use_wide_bvh=wide)
+    for k, v in knobs.items():
+        r.set_option(k, v)
+    s = r.create_scene()
+    spec = scenes.lightmap_box(s, k=16, width=w, height=h, spp=16, depth=4)
+    reg = A.Region(lib, 0, 0, w, h)
+    r.render(s, reg)  # upload + warm-up
+    r.synchronize()
+    r.clear()
+    r.reset_stats()
+    reg = A.Region(lib, 0, 0, w, h)
+    t = time.perf_counter()
+    for _ in range(spec.spp):
+        r.render(s, reg)
+    r.synchronize()
+    dt = time.perf_counter() - t
+    rc = r.ray_counts()
+    rays = rc["primary"] + rc["secondary"] + rc["shadow"]
+    sh = r.sh_data()
+    out = {"metric": "lightmap bake Mrays/s (geometry camera + SH output)", "mrays": round(rays / dt / 1e6, 2),
+           "ms_per_pass": round(dt / spec.spp * 1e3, 3), "texels": w * h, "triangles": spec.triangles,
+           "spp": spec.spp, "sh_nonzero": bool(sh is not None and float(abs(sh).max()) > 0)}
+    r.close()
+    log(f"lightmap: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -595,6 +628,7 @@ def main():
     ap.add_argument("--large-spp", type=int, default=16)
     ap.add_argument("--large-adj-steps", type=int, default=5, help="texel-gradient descent steps (0: skip)")
     ap.add_argument("--large-batch-scenes", type=int, default=16, help="small scenes batched with it (0: skip)")
+    ap.add_argument("--lightmap-size", type=int, default=1024, help="lightmap bake leg (0: skip)")
     ap.add_argument("--build-spheres", type=int, default=3000,
                     help="icospheres of the fast-build mesh (about 390 triangles each; 0: skip)")
     args = ap.parse_args()
@@ -679,6 +713,7 @@ def main():
         args.large_batch_knobs = {k: v for k, v in batch["knobs"].items() if k != "use_graph"}
     large = run_large(args, lib, A, scenes, plain_knobs, wide) if args.large else None
     build = run_build(args, lib, A, scenes) if args.build_spheres > 0 else None
+    lightmap = run_lightmap(args, lib, A, scenes, plain_knobs, wide) if args.lightmap_size > 0 else None
 
     line = {
         "metric": "Mrays/s at 1080p depth-8 (triangle soup, primary+secondary+shadow rays)",
@@ -713,6 +748,7 @@ def main():
         "batch": batch,
         "large": large,
         "fast_build": build,
+        "lightmap": lightmap,
     }
     print(json.dumps(line), flush=True)
 
