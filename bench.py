@@ -117,6 +117,13 @@ def run_gpu(args, lib, A, scenes, wide, staged_modes=True, tuned=None):
         results[name] = res
         log(f"{name} {opts}: {args.steps} passes in {res['seconds']:.3f}s, {res['rays'] / 1e6:.1f} Mrays -> "
             f"{res['mrays']:.1f} Mrays/s")
+    if staged_modes and not args.only_tuned:
+        # the tuned knobs with UseCoherentSampling (the 8x8 pattern as the random index)
+        s.set_camera(0, **scenes.soup_camera(args.depth, coherent=True))
+        res = timed_passes(r, s, A, lib, W, H, args.steps, args.warmup, dict(tuned.best))
+        results["coherent"] = res
+        log(f"coherent sampling, tuned knobs: {res['mrays']:.1f} Mrays/s")
+        s.set_camera(0, **scenes.soup_camera(args.depth, coherent=False))
 
     # instrumented replay of the tuned passes for the traversal visit counts
     autotune.apply(r, tuned.best)

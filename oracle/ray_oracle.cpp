@@ -1183,6 +1183,16 @@ int orc_set_max_bounces(int n) {
     return 0;
 }
 
+// UseCoherentSampling: the random index of a pixel is its entry of the 8x8
+// sampling pattern (RendererRef.cpp:165-168, 262-265; pattern Core.cpp) instead
+// of the pixel index
+static int rand_index_of(const rh_pass_info &pi, int x, int y) {
+    static const int pattern[64] = {0, 1, 2, 3, 1, 2, 3, 0, 2, 3, 0, 1, 3, 0, 1, 2, 1, 2, 3, 0, 2, 1,
+                                    0, 3, 3, 0, 1, 2, 0, 3, 2, 1, 2, 3, 0, 1, 1, 0, 3, 2, 0, 1, 2, 3,
+                                    0, 3, 2, 1, 3, 2, 1, 0, 3, 2, 1, 0, 1, 0, 3, 2, 2, 1, 0, 3};
+    return (pi.settings.flags & RH_USE_COHERENT_SAMPLING) ? pattern[(y % 8) * 8 + (x % 8)] : pi.index;
+}
+
 // float -> int as x86 cvttss2si: NaN and out-of-range values give INT_MIN
 static int f2i_x86(float f) {
     if (!(f > -2147483904.0f && f < 2147483648.0f)) return INT32_MIN;
@@ -1347,7 +1357,7 @@ int orc_render_sh(const rh_scene_view *sc, const rh_camera *cam, int w, int h, c
         for (size_t i = 0; i < n; i++) {
             const int x = (rays[i].xy >> 16) & 0xffff, y = rays[i].xy & 0xffff;
             pi.index = y * w + x;
-            pi.rand_index = pi.index;
+            pi.rand_index = rand_index_of(pi, x, y);
             bool e;
             float col[4];
             shade(c, pi, hits[i], rays[i], col, &sec[count], e);
@@ -1379,7 +1389,7 @@ int orc_render_sh(const rh_scene_view *sc, const rh_camera *cam, int w, int h, c
             for (int i = 0; i < rc; i++) {
                 const int x = (rays[i].xy >> 16) & 0xffff, y = rays[i].xy & 0xffff;
                 pi.index = y * w + x;
-                pi.rand_index = pi.index;
+                pi.rand_index = rand_index_of(pi, x, y);
                 bool e;
                 float col[4];
                 shade(c, pi, hits[i], rays[i], col, &sec[count], e);
@@ -1469,7 +1479,7 @@ int orc_adjoint(const rh_scene_view *sc, const rh_camera *cam, int w, int h, con
             for (int i = 0; i < rc; i++) {
                 const int x = (rays[i].xy >> 16) & 0xffff, y = rays[i].xy & 0xffff;
                 pi.index = y * w + x;
-                pi.rand_index = pi.index;
+                pi.rand_index = rand_index_of(pi, x, y);
                 bool e;
                 float col[4];
                 Vtx vx;

@@ -123,8 +123,10 @@ int rhx_get_sh_data(rhx_renderer *r, float *out, size_t count) {
         g_err = "output buffer too small";
         return -1;
     }
-    // the caller renders with OutputSH first (the Ref framebuffer allocates it then)
-    memcpy(out, r->r->get_sh_data_ref(), n * sizeof(float));
+    // the Ref framebuffer allocates its SH data at the first pass with OutputSH
+    const Ray::shl1_data_t *sh = r->r->get_sh_data_ref();
+    if (!sh) return 1;
+    memcpy(out, sh, n * sizeof(float));
     return 0;
 }
 int rhx_get_stats(rhx_renderer *r, rh_stats *out) {

@@ -130,7 +130,7 @@ def cornell_geometry(seed: int = SEED_BASE + 1):
 
 
 def cornell_box(scene: "A.Scene", seed: int = SEED_BASE + 1, width=256, height=256, spp=16, depth=4,
-                fast_bvh=False) -> SceneSpec:
+                fast_bvh=False, coherent=False) -> SceneSpec:
     attrs, idx, shapes, albedos = cornell_geometry(seed)
     tex = _white(scene)
     mats = [scene.add_material(A.DiffuseMaterial, tex, main_color=tuple(float(v) for v in albedos[k]))
@@ -142,7 +142,7 @@ def cornell_box(scene: "A.Scene", seed: int = SEED_BASE + 1, width=256, height=2
     scene.set_environment((0.0, 0.0, 0.0))
     cam = scene.add_camera(origin=(0.0, 5.0, -14.0), fwd=(0.0, 0.0, 1.0), fov=45.0, filter=A.Box,
                            dtype=A.DeviceNone, max_diff_depth=depth, max_glossy_depth=depth, max_refr_depth=depth,
-                           max_transp_depth=depth, max_total_depth=depth)
+                           max_transp_depth=depth, max_total_depth=depth, use_coherent_sampling=coherent)
     scene.set_current_cam(cam)
     return SceneSpec("cornell_box", width, height, spp, depth, triangles=len(idx) // 3)
 
@@ -362,11 +362,16 @@ def triangle_soup(scene: "A.Scene", seed: int = SEED_BASE + 2, width=1920, heigh
     scene.set_environment((0.5, 0.5, 0.5))
     scene.add_light(A.PointLight, position=(-10.0, 25.0, -10.0), radius=0.5, color=(50.0, 50.0, 50.0))
     scene.add_light(A.PointLight, position=(12.0, 22.0, 8.0), radius=0.5, color=(50.0, 50.0, 50.0))
-    cam = scene.add_camera(origin=(0.0, 8.0, -48.0), fwd=(0.0, -0.15, 1.0), fov=50.0, filter=A.Box,
-                           dtype=A.DeviceNone, max_diff_depth=depth, max_glossy_depth=depth, max_refr_depth=depth,
-                           max_transp_depth=depth, max_total_depth=depth)
+    cam = scene.add_camera(**soup_camera(depth))
     scene.set_current_cam(cam)
     return SceneSpec("triangle_soup", width, height, spp, depth, triangles=len(idx) // 3)
+
+
+def soup_camera(depth=8, coherent=False) -> dict:
+    """The soup's camera (camera_desc_t fields); coherent: UseCoherentSampling."""
+    return dict(origin=(0.0, 8.0, -48.0), fwd=(0.0, -0.15, 1.0), fov=50.0, filter=A.Box, dtype=A.DeviceNone,
+                max_diff_depth=depth, max_glossy_depth=depth, max_refr_depth=depth, max_transp_depth=depth,
+                max_total_depth=depth, use_coherent_sampling=coherent)
 
 
 def mesh_lights(scene: "A.Scene", data) -> SceneSpec:

@@ -9,14 +9,14 @@ from conftest import standalone_scene
 from raydemo_amd import capi as A, scenes
 
 
-def _ref_render(ref_lib, build, w, h, spp):
+def _ref_render(ref_lib, build, w, h, spp, sh=True):
     r = A.Renderer(ref_lib, w, h, A.RENDERER_REF)
     s = r.create_scene()
     build(s)
     reg = A.Region(ref_lib, 0, 0, w, h)
     for _ in range(spp):
         r.render(s, reg)
-    return r.pixels(), r.sh_data()
+    return r.pixels(), (r.sh_data() if sh else None)
 
 
 @pytest.mark.parametrize("perspective", [False, True])
@@ -50,3 +50,19 @@ def test_oracle_lightmap_regions_match_reference(ref_lib, host_lib, oracle):
     _, img_o, _, sh_o = oracle.render_sh_regions(hs.raw_view(), hs.camera_record(0), w, h, spp, rects)
     assert r.pixels().tobytes() == img_o.tobytes()
     assert r.sh_data().tobytes() == sh_o.tobytes()
+
+
+def test_oracle_coherent_sampling_matches_reference(ref_lib, host_lib, oracle):
+    """UseCoherentSampling: the random index of a pixel is its entry of the
+    8x8 sampling pattern (RendererRef.cpp:165-168, 262-265)."""
+    w, h, spp = 48, 40, 3
+    imgs = {}
+    for coherent in (False, True):
+        build = lambda s: scenes.cornell_box(s, width=w, height=h, spp=spp, depth=3, coherent=coherent)
+        img_r, _ = _ref_render(ref_lib, build, w, h, spp, sh=False)
+        hs = standalone_scene(host_lib)
+        build(hs)
+        _, img_o, _, _ = oracle.render_sh(hs.raw_view(), hs.camera_record(0), w, h, spp)
+        assert img_r.tobytes() == img_o.tobytes(), coherent
+        imgs[coherent] = img_o
+    assert imgs[True].tobytes() != imgs[False].tobytes()

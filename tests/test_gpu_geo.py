@@ -95,3 +95,21 @@ def test_lightmap_regions_match_oracle(host_lib, oracle):
     print(f"regions: RMSE {e:.3e}, SH max diff {se:.3e}")
     assert e <= 1e-3 and se <= 1e-5 * np.abs(sh_o).max()
     assert np.all(acc[:15, :, :3] >= 0) and np.abs(acc[15:, :5]).max() == 0  # outside both regions
+
+
+def test_coherent_sampling_matches_oracle_and_reference(ref_lib, host_lib, oracle):
+    w, h, spp = 48, 40, 3
+    build = lambda s: scenes.cornell_box(s, width=w, height=h, spp=spp, depth=3, coherent=True)
+    r = A.Renderer(host_lib, w, h, A.RENDERER_HIP, use_wide_bvh=True)
+    s = r.create_scene()
+    build(s)
+    reg = A.Region(host_lib, 0, 0, w, h)
+    for _ in range(spp):
+        r.render(s, reg)
+    img = r.pixels()
+    hs = standalone_scene(host_lib, wide=True)
+    build(hs)
+    _, img_o, _, _ = oracle.render_sh(hs.raw_view(), hs.camera_record(0), w, h, spp)
+    img_r, _ = _ref_render(ref_lib, build, w, h, spp, sh=False)
+    for ref in (img_o, img_r):
+        assert float(np.sqrt(np.mean((img[..., :3].astype(np.float64) - ref[..., :3]) ** 2))) <= 1e-3
