@@ -64,6 +64,12 @@ def golden_scene1():
     return {k: d[k] for k in d.files}
 
 
+@pytest.fixture(scope="session")
+def golden_scene2():
+    d = np.load(GOLDEN / "scene2.npz")
+    return {k: d[k] for k in d.files}
+
+
 def standalone_scene(lib, wide=False):
     from raydemo_amd.capi import standalone_scene as make
     return make(lib, wide)
