@@ -20,8 +20,12 @@ index count per material group).
 Textures: JPEG/PNG/TGA are decoded with Pillow (the demo uses stb_image,
 whose JPEG decoder may differ from Pillow's in the last bit of some texels),
 Radiance .hdr (RGBE, new-style run-length encoding) by read_hdr, kept as RGBE
-texels as the demo's LoadHDR does.  The demo's OBJ reader is not restated:
-an .obj vertex_data raises.
+texels as the demo's LoadHDR does.
+
+load_obj follows the demo's OBJ reader (Load.cpp:445-614): triangles only,
+each corner v/vt/vn; a corner reuses one of the last 1000 vertices whose 8
+attributes all lie within 1e-7 of its own (searched from the newest), else it
+appends a vertex; "g" lines start material groups.
 """
 from __future__ import annotations
 
@@ -57,6 +61,63 @@ def load_bin(path: str):
     if len(idx) and int(idx.max()) >= len(attrs):
         raise ValueError(f"{path}: index out of range")
     return attrs, idx, groups
+
+
+def load_obj(path: str):
+    """(attrs float32 (n, 8), indices uint32, groups uint32 (k, 2)), as load_bin."""
+    f32 = np.float32
+    v, vn, vt = [], [], []
+    attrs = np.zeros((0, 8), f32)
+    rows, idx, groups = [], [], []
+    eps = f32(1e-7)
+
+    def flush():
+        nonlocal attrs, rows
+        if rows:
+            attrs = np.concatenate([attrs, np.array(rows, f32)])
+            rows = []
+
+    with open(path, "r") as fh:
+        for line in fh:
+            tok = line.split()
+            if not tok:
+                continue
+            if tok[0] == "v":
+                v.append([f32(float(x)) for x in tok[1:4]])
+            elif tok[0] == "vn":
+                vn.append([f32(float(x)) for x in tok[1:4]])
+            elif tok[0] == "vt":
+                vt.append([f32(float(x)) for x in tok[1:3]])
+            elif tok[0] == "f":
+                for corner in tok[1:4]:
+                    parts = (corner.split("/") + ["", ""])[:3]
+                    i1, i2, i3 = (int(p) - 1 if p else -1 for p in parts)
+                    a = np.array(v[i1] + vn[i3] + vt[i2], f32)
+                    flush()
+                    n = len(attrs)
+                    lo = max(0, n - 1000)
+                    hit = -1
+                    if n:
+                        win = attrs[lo:n]
+                        ok = np.all(np.abs(win - a) < eps, axis=1)
+                        where = np.nonzero(ok)[0]
+                        if len(where):
+                            hit = lo + int(where[-1])
+                    if hit >= 0:
+                        idx.append(hit)
+                    else:
+                        idx.append(n + len(rows))
+                        rows.append(a)
+                        flush()
+            elif tok[0] == "g":
+                if groups:
+                    groups.append(len(idx) - groups[-1])
+                groups.append(len(idx))
+    flush()
+    if not groups:
+        groups.append(0)
+    groups.append(len(idx) - groups[-1])
+    return attrs, np.array(idx, np.uint32), np.array(groups, np.uint32).reshape(-1, 2)
 
 
 def save_bin(path: str, attrs, indices, groups):
@@ -231,9 +292,12 @@ def load_scene(scene, js, base_dir: str = ".", missing_texture: str = "raise") -
         materials[name] = scene.add_material(_MAT_TYPES[kind], main_tex, **kw)
     for name, m in js["meshes"].items():
         vd = m["vertex_data"]
-        if ".bin" not in vd:
-            raise ValueError(f"{vd}: only the .bin mesh format is read")
-        attrs, idx, groups = load_bin(os.path.join(base_dir, vd))
+        if ".obj" in vd:
+            attrs, idx, groups = load_obj(os.path.join(base_dir, vd))
+        elif ".bin" in vd:
+            attrs, idx, groups = load_bin(os.path.join(base_dir, vd))
+        else:
+            raise ValueError(f"{vd}: unknown mesh format")
         shapes = []
         for g, (start, count) in enumerate(groups):
             mat = materials[m["materials"][g]]

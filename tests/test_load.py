@@ -172,3 +172,34 @@ def test_reference_assets(host_lib):
     h = load.load_scene(s, os.path.join(ASSETS, "scenes/sponza_simple.json"), base_dir=os.path.dirname(ASSETS),
                         missing_texture="white")
     assert s.triangle_count() == 13776 and len(h["materials"]) == 9
+
+
+def test_obj_reader(tmp_path):
+    """Two quads (4 triangles) sharing an edge, two groups: corners equal in
+    all 8 attributes reuse a vertex; a corner with another normal does not."""
+    p = tmp_path / "m.obj"
+    p.write_text("""v 0 0 0
+v 1 0 0
+v 1 1 0
+v 0 1 0
+v 2 0 0
+v 2 1 0
+vt 0 0
+vt 1 0
+vt 1 1
+vt 0 1
+vn 0 0 1
+vn 0 1 0
+g first
+f 1/1/1 2/2/1 3/3/1
+f 1/1/1 3/3/1 4/4/1
+g second
+f 2/2/1 5/1/1 6/4/1
+f 2/2/2 6/4/1 3/3/1
+""")
+    attrs, idx, groups = load.load_obj(str(p))
+    assert idx.tolist() == [0, 1, 2, 0, 2, 3, 1, 4, 5, 6, 5, 2]
+    assert groups.tolist() == [[0, 6], [6, 6]]
+    assert attrs.shape == (7, 8)
+    assert attrs[6].tolist() == [1, 0, 0, 0, 1, 0, 1, 0]  # vertex 2 with normal 2
+    assert attrs[4].tolist() == [2, 0, 0, 0, 0, 1, 0, 0]
