@@ -2112,6 +2112,8 @@ int rh_set_material_colors(rh_context *c, uint32_t first, uint32_t n, const floa
     if (!c->have_scene) return arg_error("no scene uploaded");
     if ((size_t)first + n > c->n_materials) return arg_error("material range outside the scene");
     if (!n) return 0;
+    // taped records hold colours of the old materials: the adjoint replays
+    tape_forget(c);
     const size_t floats = (size_t)3 * n;
     if (floats > c->color_stage_cap || !c->color_stage) {
         free_ptr(c->color_stage);

@@ -196,3 +196,20 @@ def test_tape_equals_replay(host_lib, texels):
         scale = np.abs(b).max()
         assert scale > 0
         assert np.abs(a - b).max() <= 1e-5 * scale
+
+
+def test_tape_dropped_when_colours_change(host_lib):
+    """Colours set after the render: the taped records would hold the old
+    colours, so the adjoint replays with the new ones (as without the tape)."""
+    dr, colors, spec = _small(host_lib)
+    wts = torch.tensor(np.random.default_rng(3).uniform(-1, 1, size=(H, W, 3)).astype(np.float32), device="cuda")
+    other = colors.copy()
+    other[:spec.info["diffuse"]] *= 0.5
+    out = {}
+    for tape in (1, 0):
+        dr.r.set_option("adjoint_tape", dr.spp if tape else 0)
+        dr.render(torch.tensor(colors))
+        dr.set_colors(torch.tensor(other))
+        out[tape] = dr.adjoint(wts).cpu().numpy().astype(np.float64)
+    scale = np.abs(out[0]).max()
+    assert np.abs(out[1] - out[0]).max() <= 1e-5 * scale
