@@ -62,10 +62,12 @@ int rh_read_pixels(rh_context *ctx, float *rgba, size_t count);
  * last resize. */
 int rh_read_sh(rh_context *ctx, float *out, size_t count);
 int rh_synchronize(rh_context *ctx);
+/* the stage timers of the passes rendered since the last reset (waits for the passes in flight) */
 int rh_get_stats(rh_context *ctx, rh_stats *out);
 int rh_reset_stats(rh_context *ctx);
 int rh_get_ray_counts(rh_context *ctx, rh_ray_counts *out);
-/* Knobs: "sort_rays" (0/1), "stage_timing" (0/1: hipEvent timers per stage),
+/* Knobs: "sort_rays" (0/1), "stage_timing" (0/1, default 1: the stage timers rh_get_stats reports, from HIP events
+ *   recorded at each pass's stage boundaries and summed once the pass has finished; no pass waits for them),
  * "kernel_timing" (0/1: events around each kernel launch, see rh_get_kernel_profile),
  * "count_visits" (0/1: instrumented trace kernels count node/triangle/instance visits),
  * "lds_stack_depth" (4..64 LDS entries of the per-lane traversal stack),

@@ -311,6 +311,8 @@ typedef struct rh_ray_counts {
     unsigned long long sorted;
     unsigned long long adjoint;        /* path segments traced by the adjoint replay */
     unsigned long long adjoint_shadow; /* visibility segments of the adjoint replay */
+    unsigned long long overflow;       /* traversal stack pushes dropped past the per-lane stack's capacity
+                                          (LDS part + global spill); non-zero means hits may have been missed */
 } rh_ray_counts;
 
 /* per-kernel profile: kinds 0 trace, 1 shade, 2 sort, 3 shadow (deferred NEE visibility),

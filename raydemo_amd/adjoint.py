@@ -12,6 +12,11 @@ backward.  FP32 atomics in the gradient sum are the only run-to-run
 difference.  Sampling decisions are held fixed (Russian roulette weights are
 not differentiated; the refined Cornell workload turns roulette off).
 
+Each autograd forward records the scene state it rendered (material colours,
+changed textures); a backward taken after the state changed (several renders
+before one backward, or new colours set in between) first puts that state back,
+so every image is differentiated at the parameters that made it.
+
 This module is only the gradient harness: the renderer never needs torch.
 """
 from __future__ import annotations
@@ -29,12 +34,14 @@ class DiffRenderer:
     """A HIP renderer + scene pair whose material colours are the parameters."""
 
     def __init__(self, lib: A.RayLib, renderer: A.Renderer, scene: A.Scene, width: int, height: int, spp: int,
-                 n_materials: int, device: str = "cuda", tape: bool | None = None, tape_texels: bool = False):
-        """tape: the forward passes keep their vertex records on the device
-        (the "adjoint_tape" option) so that adjoint() needs no replay; None
-        turns it on when the records of all passes take at most half of the
-        free device memory.  tape_texels: the records include texel
-        footprints (for texel gradients; 4x the memory)."""
+                 n_materials: int, device: str = "cuda", tape: bool | None = False, tape_texels: bool = False):
+        """tape: False (the default) differentiates by path replay, with
+        memory O(pixels); True keeps every forward pass's vertex records on the
+        device (the "adjoint_tape" option, O(pixels x passes x depth)) so that
+        adjoint() needs no replay; None turns the tape on when the records of
+        all passes take at most half of the free device memory.  tape_texels:
+        the records include texel footprints (for texel gradients; 4x the
+        memory)."""
         if not lib.has_adjoint:
             raise RuntimeError(f"{lib.path} has no adjoint entry points (HIP backend required)")
         self.lib, self.r, self.s = lib, renderer, scene
@@ -45,6 +52,12 @@ class DiffRenderer:
         self.rays_adjoint = 0
         self._layout = None
         self._grad_textures = None
+        # the parameter state now on the device: colours, textures changed
+        # through set_texture (with their original texels), and a version
+        self._colors = np.array(scene.view()["materials"]["main_color"], np.float32)
+        self._textures: dict = {}
+        self._tex_original: dict = {}
+        self._version = 0
         if tape is None:
             vmax = renderer.get_option("max_bounces") + 1
             need = spp * width * height * (vmax * (64 if tape_texels else 16) + 4)
@@ -61,9 +74,46 @@ class DiffRenderer:
             self._layout = {k: v[k] for k in ("textures", "tex_mips", "texels_count")}
         return self._layout
 
-    def set_colors(self, colors: torch.Tensor):
-        rgb = colors.detach().to(torch.float32).cpu().numpy()
+    def tape_bytes(self) -> int:
+        """Device memory the tape of all passes takes (0 without the tape)."""
+        if not self.tape:
+            return 0
+        vmax = self.r.get_option("max_bounces") + 1
+        tex = self.r.get_option("adjoint_tape_texels")
+        return self.spp * self.w * self.h * (vmax * (64 if tex else 16) + 4)
+
+    def replay_bytes(self) -> int:
+        """Device memory of the path replay's per-lane vertex records (held
+        for one pass at a time): O(pixels)."""
+        vmax = 17  # records per lane: at most max_bounces + 1 <= 17 vertices
+        return self.w * self.h * (vmax * (16 + 48) + 4)
+
+    def set_colors(self, colors: torch.Tensor | np.ndarray):
+        if isinstance(colors, torch.Tensor):
+            colors = colors.detach().to(torch.float32).cpu().numpy()
+        rgb = np.array(colors, np.float32).reshape(-1, 3)
         self.r.set_material_colors(self.s, 0, rgb)
+        if rgb.shape != self._colors.shape or not np.array_equal(rgb, self._colors):
+            self._colors = rgb
+            self._version += 1
+
+    def state(self):
+        """The parameter state now on the device, for restore()."""
+        return (self._version, self._colors, dict(self._textures))
+
+    def restore(self, state):
+        """Puts back a state() taken earlier: the colours, and every texture
+        changed since (to its texels then, or its original texels)."""
+        version, colors, textures = state
+        if version == self._version:
+            return
+        if colors.shape != self._colors.shape or not np.array_equal(colors, self._colors):
+            self.set_colors(colors)
+        for tex in set(textures) | set(self._textures):
+            want = textures.get(tex, self._tex_original.get(tex))
+            if want is not None and not np.array_equal(want, self._textures.get(tex)):
+                self._set_texture_rgba(tex, want)
+        self._version = version
 
     def render(self, colors: torch.Tensor | None = None) -> torch.Tensor:
         """Mean linear image (h, w, 4) of passes 1..spp, on the device."""
@@ -80,7 +130,9 @@ class DiffRenderer:
 
     def adjoint(self, grad_img: torch.Tensor, texels: bool = False, textures=None):
         """d loss / d main_color (n_materials, 3) for d loss / d image grad_img
-        ((h, w, 3) or (h, w, 4)), for the colours of the last render; with
+        ((h, w, 3) or (h, w, 4)), at the colours and texels now on the device
+        (the autograd functions put back the state of their own render
+        first); with
         texels, also d loss / d stored texel (n_texels, 4) on the device, for
         the texels of `textures` only when given (the others stay zero)."""
         want = tuple(textures) if (texels and textures) else None
@@ -109,7 +161,25 @@ class DiffRenderer:
         alpha 255), quantised to 8 bits as the texture stores them."""
         q = torch.clamp(torch.round(rgb.detach().to(torch.float32) * 255.0), 0, 255).to(torch.uint8).cpu().numpy()
         rgba = np.concatenate([q, np.full(q.shape[:2] + (1,), 255, np.uint8)], axis=2)
+        if not np.array_equal(rgba, self._textures.get(tex)):
+            self._set_texture_rgba(tex, rgba)
+            self._version += 1
+
+    def _set_texture_rgba(self, tex: int, rgba: np.ndarray):
+        if tex not in self._tex_original:
+            self._tex_original[tex] = self._level0(tex)
         self.r.set_texture_data(self.s, tex, rgba)
+        self._textures[tex] = rgba
+
+    def _level0(self, tex: int) -> np.ndarray:
+        """Texture tex's level-0 texels as stored (h, w, 4) uint8: the inside of
+        its bordered level-0 block."""
+        v = self.s.raw_view()
+        m = self.tex_layout()["tex_mips"][tex][0]
+        off, bw, bh = int(m["offset"]), int(m["bw"]), int(m["bh"])
+        words = np.ctypeslib.as_array((A.c_uint32 * (off + bw * bh)).from_address(v.texels))[off:]
+        block = words.reshape(bh, bw)[1:-1, 1:-1].copy()
+        return block.view(np.uint8).reshape(bh - 2, bw - 2, 4)
 
 
 def fold_texel_grad(view: dict, tex: int, tg: np.ndarray) -> np.ndarray:
@@ -173,10 +243,13 @@ class RenderFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, colors: torch.Tensor, dr: DiffRenderer):
         ctx.dr = dr
-        return dr.render(colors)[..., :3].contiguous()
+        img = dr.render(colors)[..., :3].contiguous()
+        ctx.state = dr.state()
+        return img
 
     @staticmethod
     def backward(ctx, grad_out: torch.Tensor):
+        ctx.dr.restore(ctx.state)
         return ctx.dr.adjoint(grad_out.contiguous()), None
 
 
@@ -194,11 +267,14 @@ class TextureRenderFunction(torch.autograd.Function):
         ctx.dr, ctx.tex_ids, ctx.param_device = dr, tex_ids, texels.device
         for k, t in enumerate(tex_ids):
             dr.set_texture(t, texels[k])
-        return dr.render()[..., :3].contiguous()
+        img = dr.render()[..., :3].contiguous()
+        ctx.state = dr.state()
+        return img
 
     @staticmethod
     def backward(ctx, grad_out: torch.Tensor):
         dr = ctx.dr
+        dr.restore(ctx.state)
         _, tg = dr.adjoint(grad_out.contiguous(), texels=True, textures=ctx.tex_ids)
         layout = dr.tex_layout()
         g = torch.stack([fold_texel_grad_torch(layout, t, tg) for t in ctx.tex_ids])
@@ -209,7 +285,8 @@ def render_textures(texels: torch.Tensor, dr: DiffRenderer, tex_ids: tuple) -> t
     return TextureRenderFunction.apply(texels, dr, tex_ids)
 
 
-def refined_cornell(lib: A.RayLib, k=17, width=512, height=512, spp=32, depth=6, albedo_seed=None, renderer=None):
+def refined_cornell(lib: A.RayLib, k=17, width=512, height=512, spp=32, depth=6, albedo_seed=None, renderer=None,
+                    tape=False):
     """DiffRenderer over the refined Cornell box: (dr, initial colours (n, 3) numpy)."""
     from . import scenes
     r = renderer or A.Renderer(lib, width, height, A.RENDERER_HIP)
@@ -219,11 +296,11 @@ def refined_cornell(lib: A.RayLib, k=17, width=512, height=512, spp=32, depth=6,
     n = spec.info["materials"]
     view = s.view()
     colors = np.array(view["materials"]["main_color"], np.float32)
-    return DiffRenderer(lib, r, s, width, height, spp, n), colors, spec
+    return DiffRenderer(lib, r, s, width, height, spp, n, tape=tape), colors, spec
 
 
 def textured_cornell(lib: A.RayLib, width=128, height=128, spp=8, depth=4, tex_size=64, mipmaps=True, textures=None,
-                     renderer=None):
+                     renderer=None, tape=False):
     """DiffRenderer over the textured Cornell box: (dr, spec); spec.info["textures"]
     are the four differentiable textures."""
     from . import scenes
@@ -231,7 +308,7 @@ def textured_cornell(lib: A.RayLib, width=128, height=128, spp=8, depth=4, tex_s
     s = r.create_scene()
     spec = scenes.textured_cornell(s, tex_size=tex_size, width=width, height=height, spp=spp, depth=depth,
                                    mipmaps=mipmaps, textures=textures)
-    return DiffRenderer(lib, r, s, width, height, spp, len(s.view()["materials"]), tape_texels=True), spec
+    return DiffRenderer(lib, r, s, width, height, spp, len(s.view()["materials"]), tape=tape, tape_texels=True), spec
 
 
 @dataclass

@@ -75,7 +75,7 @@ class Stats(Structure):
 
 class RayCounts(Structure):
     _fields_ = [("primary", c_uint64), ("secondary", c_uint64), ("shadow", c_uint64), ("sorted", c_uint64),
-                ("adjoint", c_uint64), ("adjoint_shadow", c_uint64)]
+                ("adjoint", c_uint64), ("adjoint_shadow", c_uint64), ("overflow", c_uint64)]
 
 
 class KernelProfile(Structure):

@@ -716,7 +716,7 @@ int rh_batch_get_ray_counts(rh_batch *b, rh_ray_counts *out) {
     out->primary = v[0];
     out->secondary = v[1];
     out->shadow = v[2];
-    if (v[4]) rh_rt::err() = "traversal stack overflow: " + std::to_string(v[4]) + " pushes dropped";
+    out->overflow = v[4];
     return 0;
 }
 

@@ -11,8 +11,10 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <string>
 #include <vector>
 
@@ -23,6 +25,7 @@ using rh_rt::arg_error;
 using rh_rt::DevBuf;
 using rh_rt::fail;
 using rh_rt::free_ptr;
+using rh_rt::release;
 using rh_rt::upload_halton;
 
 namespace {
@@ -760,6 +763,7 @@ struct rh_context {
     float4 *tape_rec = nullptr, *tape_rec_tex = nullptr;
     int *tape_nverts = nullptr;
     std::vector<std::array<int, 4>> tape_rect; // recorded rect per slot (w = 0: empty)
+    std::vector<uint64_t> tape_key;            // pass_key of the pass recorded in each slot
     uint32_t n_textures = 0;
     uint32_t *grad_tex_mask = nullptr; // textures whose texels get gradients (null: all)
     size_t grad_tex_words = 0;
@@ -768,9 +772,16 @@ struct rh_context {
     int cus = 0;               // compute units of the device
     int refill_blocks = 0, refill_blocks_lds = -1;
     int adjoint_occupancy = 2; // the same for the adjoint replay kernel
-    int stage_timing = 0;
     int lds_depth = 24;
-    rh_stats stats = {};
+    // stage timers (RendererBase::stats_t, in the order of rh_stats): every
+    // pass records events at its stage boundaries, each interval tagged with
+    // the timer it adds to (-1: none); a pass's intervals are summed once its
+    // last event has completed (at a later pass without waiting, or in
+    // rh_get_stats), so no pass waits for the device
+    int stage_timing = 1;
+    std::deque<std::vector<std::pair<hipEvent_t, int>>> stage_pending;
+    std::vector<hipEvent_t> stage_free;
+    double stage_us[6] = {0, 0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -812,26 +823,32 @@ int alloc_frame(rh_context *c, int w, int h) {
 
 int ensure_nee(rh_context *c);
 
+// The ray queues for n rays.  Capacities are recorded only once every buffer
+// is allocated: after a failed allocation the context holds null pointers and
+// no capacity, and the next call allocates again.
 int ensure_rays(rh_context *c, size_t n) {
     if (n <= c->max_rays && c->rays[0]) return ensure_nee(c);
-    c->max_rays = std::max(n, c->max_rays);
-    const size_t m = c->max_rays;
+    const size_t m = std::max(n, c->max_rays);
+    c->max_rays = 0;
     for (int k = 0; k < 2; k++) {
-        free_ptr(c->rays[k]);
-        free_ptr(c->aux[k]);
-        free_ptr(c->keys[k]);
-        free_ptr(c->vals[k]);
+        release(c->rays[k]);
+        release(c->aux[k]);
+        release(c->keys[k]);
+        release(c->vals[k]);
+    }
+    release(c->hits);
+    release(c->hist);
+    for (int k = 0; k < 2; k++) {
         RH_CHECK(hipMalloc(&c->rays[k], m * sizeof(rh_ray)));
         RH_CHECK(hipMalloc(&c->aux[k], m * sizeof(rh_ray_aux)));
         RH_CHECK(hipMalloc(&c->keys[k], m * sizeof(uint32_t)));
         RH_CHECK(hipMalloc(&c->vals[k], m * sizeof(uint32_t)));
     }
-    free_ptr(c->hits);
-    free_ptr(c->hist);
     RH_CHECK(hipMalloc(&c->hits, m * sizeof(rh_hit)));
     const size_t tiles = (m + kSortTile - 1) / kSortTile;
     // per-tile digit counts (digit-major) followed by the 256 digit totals
     RH_CHECK(hipMalloc(&c->hist, (256 * tiles + 256) * sizeof(uint32_t)));
+    c->max_rays = m;
     return ensure_nee(c);
 }
 
@@ -843,11 +860,13 @@ size_t spill_words(const rh_context *c) { return (size_t)kSpillDepth * (c->sc.wn
 int ensure_spill(rh_context *c, size_t lanes) {
     lanes = (lanes + kBlock - 1) / kBlock * kBlock; // whole blocks: persistent lanes index slices too
     if (lanes <= c->spill_cap && c->spill && c->spill_words >= spill_words(c)) return 0;
-    free_ptr(c->spill);
-    c->spill = nullptr;
-    c->spill_cap = std::max(lanes, c->spill_cap);
-    c->spill_words = spill_words(c);
-    RH_CHECK(hipMalloc(&c->spill, c->spill_words * c->spill_cap * sizeof(uint32_t)));
+    release(c->spill);
+    const size_t cap = std::max(lanes, c->spill_cap), words = spill_words(c);
+    c->spill_cap = 0;
+    c->spill_words = 0;
+    RH_CHECK(hipMalloc(&c->spill, words * cap * sizeof(uint32_t)));
+    c->spill_cap = cap;
+    c->spill_words = words;
     return 0;
 }
 
@@ -856,16 +875,16 @@ int ensure_nee(rh_context *c) {
     const size_t m = c->max_rays;
     if (!c->nee_slots) return ensure_spill(c, m);
     const size_t cap = m * c->nee_slots;
-    if (cap > c->nee_cap || !c->nee_ray) {
-        free_ptr(c->nee_ray);
-        free_ptr(c->nee_term);
-        free_ptr(c->nee_aux);
-        free_ptr(c->nee_vis);
-        free_ptr(c->pend_c);
-        free_ptr(c->pend_alb);
-        free_ptr(c->pend_count);
-        free_ptr(c->shadow_queue);
-        c->nee_cap = cap;
+    if (cap > c->nee_cap || !c->nee_ray || !c->shadow_queue) {
+        release(c->nee_ray);
+        release(c->nee_term);
+        release(c->nee_aux);
+        release(c->nee_vis);
+        release(c->pend_c);
+        release(c->pend_alb);
+        release(c->pend_count);
+        release(c->shadow_queue);
+        c->nee_cap = 0;
         RH_CHECK(hipMalloc(&c->nee_ray, cap * sizeof(float4)));
         RH_CHECK(hipMalloc(&c->nee_term, cap * sizeof(float4)));
         RH_CHECK(hipMalloc(&c->nee_aux, cap * sizeof(int2)));
@@ -874,6 +893,7 @@ int ensure_nee(rh_context *c) {
         RH_CHECK(hipMalloc(&c->pend_alb, m * sizeof(float4)));
         RH_CHECK(hipMalloc(&c->pend_count, m * sizeof(int)));
         RH_CHECK(hipMalloc(&c->shadow_queue, cap * sizeof(uint32_t)));
+        c->nee_cap = cap;
     }
     return ensure_spill(c, std::max(m, cap));
 }
@@ -1103,25 +1123,72 @@ int radix_sort(rh_context *c, const int *count, int max_n, int bits, int *result
 
 PassConst make_pass_const(rh_context *c, const rh_pass_desc *p) { return make_pass_const(c->w, c->h, p); }
 
-struct StageTimer {
+// stats_t timers a pass's intervals add to
+enum StageTimerIndex : int {
+    kNoTimer = -1,
+    kRayGen = 0,
+    kPrimaryTrace = 1,
+    kPrimaryShade = 2,
+    kSecondarySort = 3,
+    kSecondaryTrace = 4,
+    kSecondaryShade = 5
+};
+
+// Sums the intervals of finished passes into the stage timers; with wait, of
+// every recorded pass.
+int stage_harvest(rh_context *c, bool wait) {
+    while (!c->stage_pending.empty()) {
+        auto &marks = c->stage_pending.front();
+        const hipEvent_t last = marks.back().first;
+        if (wait) {
+            RH_CHECK(hipEventSynchronize(last));
+        } else {
+            const hipError_t q = hipEventQuery(last);
+            if (q == hipErrorNotReady) break;
+            RH_CHECK(q);
+        }
+        for (size_t k = 1; k < marks.size(); k++) {
+            float ms = 0.0f;
+            RH_CHECK(hipEventElapsedTime(&ms, marks[k - 1].first, marks[k].first));
+            if (marks[k].second >= 0) c->stage_us[marks[k].second] += 1000.0 * ms;
+        }
+        for (auto &m : marks) c->stage_free.push_back(m.first);
+        c->stage_pending.pop_front();
+    }
+    return 0;
+}
+
+// The stage boundaries of one pass: mark(t) records an event ending an
+// interval that adds to timer t.  commit() hands the pass to the harvest;
+// a pass that fails before it gives its events back.
+struct PassStages {
     rh_context *c;
-    hipEvent_t ev[8];
-    int n = 0;
-    explicit StageTimer(rh_context *ctx) : c(ctx) {
-        if (c->stage_timing)
-            for (auto &e : ev) (void)hipEventCreate(&e);
+    bool on;
+    std::vector<std::pair<hipEvent_t, int>> marks;
+    PassStages(rh_context *ctx, bool enabled) : c(ctx), on(enabled && ctx->stage_timing) {}
+    void mark(int timer) {
+        if (!on) return;
+        hipEvent_t e = nullptr;
+        if (!c->stage_free.empty()) {
+            e = c->stage_free.back();
+            c->stage_free.pop_back();
+        } else if (hipEventCreate(&e) != hipSuccess) {
+            on = false; // no timing for this pass
+            return;
+        }
+        if (hipEventRecord(e, c->stream) != hipSuccess) {
+            c->stage_free.push_back(e);
+            on = false;
+            return;
+        }
+        marks.emplace_back(e, timer);
     }
-    void mark() {
-        if (c->stage_timing && n < 8) (void)hipEventRecord(ev[n++], c->stream);
+    void commit() {
+        if (on && marks.size() > 1) c->stage_pending.push_back(std::move(marks));
+        marks.clear();
     }
-    float between(int a, int b) {
-        float ms = 0;
-        if (c->stage_timing && b < n) (void)hipEventElapsedTime(&ms, ev[a], ev[b]);
-        return ms;
-    }
-    ~StageTimer() {
-        if (c->stage_timing)
-            for (auto &e : ev) (void)hipEventDestroy(e);
+    ~PassStages() {
+        for (auto &m : marks) c->stage_free.push_back(m.first);
     }
 };
 
@@ -1210,6 +1277,9 @@ void rh_destroy(rh_context *c) {
     free_ptr(c->grad_tex_mask);
     free_ptr(c->color_stage);
     for (auto &e : c->ev_pool) (void)hipEventDestroy(e);
+    for (auto &p : c->stage_pending)
+        for (auto &m : p) (void)hipEventDestroy(m.first);
+    for (auto &e : c->stage_free) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1312,25 +1382,30 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool k
     if (geo) {
         if (p->cam.mi_index >= c->n_mesh_instances) return arg_error("geometry camera: no such mesh instance");
         if (p->cam.uv_index > 1) return arg_error("geometry camera: uv layer must be 0 or 1");
-        if ((size_t)n > c->geo_winner_cap) {
-            free_ptr(c->geo_winner);
-            c->geo_winner = nullptr;
-            c->geo_winner_cap = (size_t)n;
+        if ((size_t)n > c->geo_winner_cap || !c->geo_winner) {
+            release(c->geo_winner);
+            c->geo_winner_cap = 0;
             RH_CHECK(hipMalloc(&c->geo_winner, (size_t)n * sizeof(uint32_t)));
+            c->geo_winner_cap = (size_t)n;
         }
     }
-    if (out_sh && !c->sh_temp) {
+    if (out_sh && (!c->sh_temp || !c->sh_clean)) {
         const size_t bytes = (size_t)c->w * c->h * 12 * sizeof(float);
+        release(c->sh_temp);
+        release(c->sh_clean);
         RH_CHECK(hipMalloc(&c->sh_temp, bytes));
         RH_CHECK(hipMalloc(&c->sh_clean, bytes));
         RH_CHECK(hipMemsetAsync(c->sh_temp, 0, bytes, c->stream));
         RH_CHECK(hipMemsetAsync(c->sh_clean, 0, bytes, c->stream));
     }
 
-    StageTimer tm(c);
-    tm.mark(); // 0
-    int *cnt[2] = {c->counters + 0, c->counters + 1};
     const bool replay = rec && !keep_image;
+    // finished passes' stage times (no wait), then this pass's boundaries
+    rc = stage_harvest(c, false);
+    if (rc) return rc;
+    PassStages tm(c, !replay);
+    tm.mark(kNoTimer);
+    int *cnt[2] = {c->counters + 0, c->counters + 1};
     unsigned long long *count_primary = replay ? c->stats_dev + 5 : c->stats_dev + 0;
     unsigned long long *count_secondary = replay ? c->stats_dev + 5 : c->stats_dev + 1;
     if (geo) {
@@ -1354,16 +1429,15 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool k
         hipLaunchKernelGGL(k_geo_raster, dim3(2 * c->cus * 4), blk, 0, c->stream, g);
         hipLaunchKernelGGL(k_geo_finish, grid, blk, 0, c->stream, g, c->rays[0], c->aux[0], c->hits);
         RH_CHECK(hipGetLastError());
-        tm.mark(); // 1
-        tm.mark(); // 2
+        tm.mark(kRayGen); // the geometry camera makes its hits too: no primary trace
     } else {
         hipLaunchKernelGGL(k_gen_primary, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, pc, c->halton,
                            c->rays[0], c->aux[0], cnt[0]);
         RH_CHECK(hipGetLastError());
-        tm.mark(); // 1
+        tm.mark(kRayGen);
         rc = launch_trace(c, trace_args(c, c->rays[0], cnt[0], n, c->hits, nullptr, count_primary), n, false);
         if (rc) return rc;
-        tm.mark(); // 2
+        tm.mark(kPrimaryTrace);
     }
 
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, cnt[1], 0);
@@ -1396,9 +1470,8 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool k
                            c->aux[1], cnt[1], c->w, c->sh_temp);
         RH_CHECK(hipGetLastError());
     }
-    tm.mark(); // 3
+    tm.mark(kPrimaryShade);
 
-    float t_sort = 0, t_trace = 0, t_shade = 0;
     int q = 1; // queue holding the rays to trace next
     const bool skip_indirect = (pi.settings.flags & RH_SKIP_INDIRECT_LIGHT) != 0;
     const int sort_on = c->sort_rays && p->sort_rays;
@@ -1438,10 +1511,10 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool k
                     hipLaunchKernelGGL((k_path_tail<false, 2, kTreeBinary>), grid, blk, lds, c->stream, tb, bounce, last);
             }
             RH_CHECK(hipGetLastError());
+            // the path tail traces and shades in one kernel: its time counts as secondary tracing
+            tm.mark(kSecondaryTrace);
             break;
         }
-        StageTimer bt(c);
-        bt.mark();
         if (sort_on) {
             hipLaunchKernelGGL(k_ray_keys, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->rays[q],
                                cnt[q], n, pc, c->keys[0], c->vals[0]);
@@ -1454,11 +1527,11 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool k
             // the sorted rays now sit in the other buffer; move the count along
             hipLaunchKernelGGL(k_copy_int, dim3(1), dim3(1), 0, c->stream, cnt[q ^ 1], cnt[q]);
             q ^= 1;
+            tm.mark(kSecondarySort);
         }
-        bt.mark();
         rc = launch_trace(c, trace_args(c, c->rays[q], cnt[q], n, c->hits, nullptr, count_secondary), n, false);
         if (rc) return rc;
-        bt.mark();
+        tm.mark(kSecondaryTrace);
         hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, cnt[q ^ 1], 0);
         ShadeArgs sb = sa;
         sb.rays = c->rays[q];
@@ -1472,13 +1545,7 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool k
         sb.out_count = cnt[q ^ 1];
         rc = launch_shade(c, sb, n);
         if (rc) return rc;
-        bt.mark();
-        if (c->stage_timing) {
-            RH_CHECK(hipStreamSynchronize(c->stream));
-            t_sort += bt.between(0, 1);
-            t_trace += bt.between(1, 2);
-            t_shade += bt.between(2, 3);
-        }
+        tm.mark(kSecondaryShade);
         q ^= 1;
     }
 
@@ -1488,23 +1555,39 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool k
                            c->sh_clean, rx, ry, rw, rhh, c->w, pc.mix_k);
         RH_CHECK(hipGetLastError());
     }
+    tm.commit(); // mixing and tonemapping are outside the stage timers, as in RendererRef.cpp:280-290
     hipLaunchKernelGGL(k_mix_tonemap, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->temp, c->clean, c->final_, rx,
                        ry, rw, rhh, c->w, pc.mix_k, p->cam.dtype, p->cam.gamma, p->cam.pass_settings.flags);
     RH_CHECK(hipGetLastError());
-    if (c->stage_timing) {
-        RH_CHECK(hipStreamSynchronize(c->stream));
-        c->stats.time_primary_ray_gen_us += (unsigned long long)(tm.between(0, 1) * 1000.0f);
-        c->stats.time_primary_trace_us += (unsigned long long)(tm.between(1, 2) * 1000.0f);
-        c->stats.time_primary_shade_us += (unsigned long long)(tm.between(2, 3) * 1000.0f);
-        c->stats.time_secondary_sort_us += (unsigned long long)(t_sort * 1000.0f);
-        c->stats.time_secondary_trace_us += (unsigned long long)(t_trace * 1000.0f);
-        c->stats.time_secondary_shade_us += (unsigned long long)(t_shade * 1000.0f);
-    }
     return 0;
 }
 
 static void tape_forget(rh_context *c) {
     for (auto &r : c->tape_rect) r = {0, 0, 0, 0};
+    for (auto &k : c->tape_key) k = 0;
+}
+
+// FNV-1a over everything of a pass that decides its paths: the camera and its
+// pass settings (field by field: the settings record has padding), the rect,
+// the iteration and the depth clamp.  A taped pass is reused by the adjoint
+// only for a pass with the same key.
+static uint64_t pass_key(const rh_context *c, const rh_pass_desc *p) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void *data, size_t n) {
+        const unsigned char *b = static_cast<const unsigned char *>(data);
+        for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+    };
+    const rh_camera &cam = p->cam;
+    mix(&cam.type, offsetof(rh_camera, pass_settings) - offsetof(rh_camera, type));
+    const rh_pass_settings &ps = cam.pass_settings;
+    const uint8_t depths[6] = {ps.max_diff_depth,   ps.max_glossy_depth, ps.max_refr_depth,
+                               ps.max_transp_depth, ps.max_total_depth,  ps.termination_start_depth};
+    mix(depths, sizeof(depths));
+    mix(&ps.flags, sizeof(ps.flags));
+    mix(p->rect, sizeof(p->rect));
+    mix(&p->iteration, sizeof(p->iteration));
+    mix(&c->max_bounces, sizeof(c->max_bounces));
+    return h | 1ull; // never 0, the key of an empty slot
 }
 
 // Buffers of the tape for passes of `lanes` lanes; false when it is off or the
@@ -1522,18 +1605,19 @@ static int tape_alloc(rh_context *c, size_t lanes) {
         c->tape_texel_slots == (c->tape_texels ? slots : 0))
         return 0;
     RH_CHECK(hipStreamSynchronize(c->stream));
-    free_ptr(c->tape_rec);
-    free_ptr(c->tape_rec_tex);
-    free_ptr(c->tape_nverts);
-    c->tape_rec = c->tape_rec_tex = nullptr;
-    c->tape_nverts = nullptr;
+    release(c->tape_rec);
+    release(c->tape_rec_tex);
+    release(c->tape_nverts);
     c->tape_rect.assign(slots, {0, 0, 0, 0});
-    c->tape_lanes = lanes;
-    c->tape_vmax = vmax;
-    c->tape_texel_slots = c->tape_texels ? slots : 0;
+    c->tape_key.assign(slots, 0);
+    c->tape_lanes = c->tape_vmax = 0;
+    c->tape_texel_slots = 0;
     RH_CHECK(hipMalloc(&c->tape_rec, (size_t)slots * vmax * lanes * sizeof(float4)));
     if (c->tape_texels) RH_CHECK(hipMalloc(&c->tape_rec_tex, (size_t)slots * vmax * 3 * lanes * sizeof(float4)));
     RH_CHECK(hipMalloc(&c->tape_nverts, (size_t)slots * lanes * sizeof(int)));
+    c->tape_lanes = lanes;
+    c->tape_vmax = vmax;
+    c->tape_texel_slots = c->tape_texels ? slots : 0;
     return 0;
 }
 
@@ -1557,6 +1641,7 @@ int rh_render_pass(rh_context *c, const rh_pass_desc *p) {
     rc = run_pass(c, p, &rec, true);
     if (rc) return rc;
     c->tape_rect[slot] = {p->rect[0], p->rect[1], p->rect[2], p->rect[3]};
+    c->tape_key[slot] = pass_key(c, p);
     return 0;
 }
 
@@ -1574,12 +1659,17 @@ int rh_synchronize(rh_context *c) {
 }
 
 int rh_get_stats(rh_context *c, rh_stats *out) {
-    *out = c->stats;
+    const int rc = stage_harvest(c, true);
+    if (rc) return rc;
+    unsigned long long *f = &out->time_primary_ray_gen_us;
+    for (int k = 0; k < 6; k++) f[k] = (unsigned long long)c->stage_us[k];
     return 0;
 }
 
 int rh_reset_stats(rh_context *c) {
-    memset(&c->stats, 0, sizeof(c->stats));
+    const int rc = stage_harvest(c, true);
+    if (rc) return rc;
+    for (double &t : c->stage_us) t = 0.0;
     RH_CHECK(hipMemsetAsync(c->stats_dev, 0, 8 * sizeof(unsigned long long), c->stream));
     return 0;
 }
@@ -1594,9 +1684,7 @@ int rh_get_ray_counts(rh_context *c, rh_ray_counts *out) {
     out->sorted = v[3];
     out->adjoint = v[5];
     out->adjoint_shadow = v[6];
-    if (v[4]) {
-        rh_rt::err() = "traversal stack overflow: " + std::to_string(v[4]) + " pushes dropped";
-    }
+    out->overflow = v[4];
     return 0;
 }
 
@@ -1923,16 +2011,16 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
     const size_t lanes = (size_t)rw * rhh;
     // vertex records: at most depth + 1 <= RH_MAX_BOUNCES_EXT + 1 per lane
     if (lanes > c->adj_rec_cap || !c->adj_rec) {
-        free_ptr(c->adj_rec);
-        c->adj_rec = nullptr;
-        c->adj_rec_cap = lanes;
+        release(c->adj_rec);
+        c->adj_rec_cap = 0;
         RH_CHECK(hipMalloc(&c->adj_rec, (RH_MAX_BOUNCES_EXT + 1) * lanes * sizeof(float4)));
+        c->adj_rec_cap = lanes;
     }
     if (d_texel_grad && (lanes > c->adj_rec_tex_cap || !c->adj_rec_tex)) {
-        free_ptr(c->adj_rec_tex);
-        c->adj_rec_tex = nullptr;
-        c->adj_rec_tex_cap = lanes;
+        release(c->adj_rec_tex);
+        c->adj_rec_tex_cap = 0;
         RH_CHECK(hipMalloc(&c->adj_rec_tex, 3 * (RH_MAX_BOUNCES_EXT + 1) * lanes * sizeof(float4)));
+        c->adj_rec_tex_cap = lanes;
     }
     const bool defer = c->defer_nee && c->nee_slots;
     // NEE slots for every lane when deferred (this may grow the spill array,
@@ -2022,7 +2110,7 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
     }
     const int tslot = (p->iteration >= 1 && p->iteration <= (int)c->tape_rect.size()) ? p->iteration - 1 : -1;
     if (tslot >= 0 && c->tape_rect[tslot][0] == rx && c->tape_rect[tslot][1] == ry && c->tape_rect[tslot][2] == rw &&
-        c->tape_rect[tslot][3] == rhh && (!d_texel_grad || c->tape_texels)) {
+        c->tape_rect[tslot][3] == rhh && c->tape_key[tslot] == pass_key(c, p) && (!d_texel_grad || c->tape_texels)) {
         // the forward pass kept this pass's vertex records: no replay
         const AdjRecords tr = tape_records(c, tslot, p, d_texel_grad != nullptr);
         a.rec = tr.rec;
@@ -2041,10 +2129,10 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
         // records (every bounce staged, as many rays in flight as the pass has);
         // one kernel then forms the suffix sums
         if (lanes > c->adj_nverts_cap || !c->adj_nverts) {
-            free_ptr(c->adj_nverts);
-            c->adj_nverts = nullptr;
-            c->adj_nverts_cap = lanes;
+            release(c->adj_nverts);
+            c->adj_nverts_cap = 0;
             RH_CHECK(hipMalloc(&c->adj_nverts, lanes * sizeof(int)));
+            c->adj_nverts_cap = lanes;
         }
         RH_CHECK(hipMemsetAsync(c->adj_nverts, 0, lanes * sizeof(int), c->stream));
         AdjRecords rec{c->adj_rec, d_texel_grad ? c->adj_rec_tex : nullptr, c->adj_nverts, (uint32_t)lanes, 0,
@@ -2097,10 +2185,10 @@ int rh_set_gradient_textures(rh_context *c, const uint32_t *ids, uint32_t n) {
     // texture ids past the mask read as unselected: size it for every texture of the scene
     mask.resize(std::max<size_t>(mask.size(), (c->n_textures + 31) / 32 + 1), 0u);
     if (mask.size() > c->grad_tex_words || !c->grad_tex_mask) {
-        free_ptr(c->grad_tex_mask);
-        c->grad_tex_mask = nullptr;
-        c->grad_tex_words = mask.size();
+        release(c->grad_tex_mask);
+        c->grad_tex_words = 0;
         RH_CHECK(hipMalloc(&c->grad_tex_mask, mask.size() * sizeof(uint32_t)));
+        c->grad_tex_words = mask.size();
     }
     RH_CHECK(hipMemcpyAsync(c->grad_tex_mask, mask.data(), mask.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                             c->stream));
@@ -2116,10 +2204,10 @@ int rh_set_material_colors(rh_context *c, uint32_t first, uint32_t n, const floa
     tape_forget(c);
     const size_t floats = (size_t)3 * n;
     if (floats > c->color_stage_cap || !c->color_stage) {
-        free_ptr(c->color_stage);
-        c->color_stage = nullptr;
-        c->color_stage_cap = floats;
+        release(c->color_stage);
+        c->color_stage_cap = 0;
         RH_CHECK(hipMalloc(&c->color_stage, floats * sizeof(float)));
+        c->color_stage_cap = floats;
     }
     RH_CHECK(hipMemcpyAsync(c->color_stage, rgb, floats * sizeof(float), hipMemcpyDefault, c->stream));
     hipLaunchKernelGGL(k_set_material_colors, dim3((n + 255) / 256), dim3(256), 0, c->stream,
@@ -2134,6 +2222,8 @@ int rh_set_texels(rh_context *c, uint32_t first, uint32_t n, const uint32_t *rgb
     if (!c->have_scene) return arg_error("no scene uploaded");
     if ((size_t)first + n > c->n_texels) return arg_error("texel range outside the scene");
     if (!n) return 0;
+    // taped records hold path colours and raw values of the old texels: the adjoint replays
+    tape_forget(c);
     RH_CHECK(hipMemcpyAsync(const_cast<uint32_t *>(c->sc.texels) + first, rgba, (size_t)n * sizeof(uint32_t),
                             hipMemcpyDefault, c->stream));
     RH_CHECK(hipStreamSynchronize(c->stream));

@@ -34,6 +34,12 @@ inline int arg_error(const char *msg) {
 inline void free_ptr(void *p) {
     if (p) (void)hipFree(p);
 }
+// frees a device buffer and forgets it: a later failed allocation then leaves
+// a null pointer behind, never a freed one
+template <class T> inline void release(T *&p) {
+    free_ptr((void *)p);
+    p = nullptr;
+}
 
 // Uploads a 256-row Halton table into a device table of RH_HALTON_TABLE_FLOATS
 // floats: the rows, then row 0 again as the extra row.

@@ -55,10 +55,113 @@ struct split_data_t {
     bbox_t left_bounds, right_bounds;
 };
 
-// SplitPrimitives_SAH (BVHSplit.cpp:126-390), object splits: full sorted sweep
-// along each axis, primitives ordered by bbox max with std::sort.
+// ---- spatial splits (BVHSplit.cpp:6-124, 220-369) ----
+const float kSpatialSplitAlpha = 0.00001f; // BVHSplit.cpp:6
+const int kSpatialBins = 256;              // BVHSplit.cpp:7
+
+struct dvec3 {
+    double v[3];
+};
+
+// One Sutherland-Hodgman step (BVHSplit.cpp:26-70): the convex polygon in[0, n)
+// clipped to the half space axis >= pos (keep_above) or axis <= pos; the new
+// vertices on the plane get exactly pos on that axis.  Returns the new count.
+int clip_polygon(const dvec3 *in, int n, dvec3 *out, int axis, double pos, bool keep_above) {
+    if (n < 3) return 0;
+    const double sign = keep_above ? 1.0 : -1.0;
+    auto cut = [&](const dvec3 &a, const dvec3 &b) {
+        const double t = (pos - a.v[axis]) / (b.v[axis] - a.v[axis]);
+        dvec3 p;
+        for (int k = 0; k < 3; k++) p.v[k] = a.v[k] + (b.v[k] - a.v[k]) * t;
+        p.v[axis] = pos;
+        return p;
+    };
+    int m = 0;
+    dvec3 a = in[0];
+    bool a_in = sign * (a.v[axis] - pos) >= 0;
+    for (int i = 0; i < n; i++) {
+        const dvec3 &b = in[i + 1 == n ? 0 : i + 1];
+        const bool b_in = sign * (b.v[axis] - pos) >= 0;
+        if (a_in && b_in) {
+            out[m++] = b;
+        } else if (a_in) {
+            out[m++] = cut(a, b);
+        } else if (b_in) {
+            out[m++] = cut(a, b);
+            out[m++] = b;
+        }
+        a = b;
+        a_in = b_in;
+    }
+    return m;
+}
+
+// the float nearest below / above a double (castflt_down / castflt_up,
+// BVHSplit.cpp:72-96): one ulp step on the bit pattern when rounding went the
+// wrong way
+float float_below(double x) {
+    float f = (float)x;
+    if ((double)f > x) {
+        int32_t b;
+        memcpy(&b, &f, 4);
+        b += f > 0 ? -1 : 1;
+        memcpy(&f, &b, 4);
+    }
+    return f;
+}
+float float_above(double x) {
+    float f = (float)x;
+    if ((double)f < x) {
+        int32_t b;
+        memcpy(&b, &f, 4);
+        b += f < 0 ? -1 : 1;
+        memcpy(&f, &b, 4);
+    }
+    return f;
+}
+
+// The box of triangle (a, b, c) clipped to `limits` (GetClippedAABB,
+// BVHSplit.cpp:98-123): the polygon clipped by the six planes in double
+// precision, its vertices rounded outwards.
+bbox_t clipped_box(const vec3 &a, const vec3 &b, const vec3 &c, const bbox_t &limits) {
+    dvec3 p[9], q[9];
+    int n = 3;
+    for (int k = 0; k < 3; k++) {
+        p[0].v[k] = (double)a[k];
+        p[1].v[k] = (double)b[k];
+        p[2].v[k] = (double)c[k];
+    }
+    for (int axis = 0; axis < 3; axis++) {
+        n = clip_polygon(p, n, q, axis, limits.min[axis], true);
+        n = clip_polygon(q, n, p, axis, limits.max[axis], false);
+    }
+    bbox_t box;
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) {
+            box.min.v[k] = fmin_std(box.min[k], float_below(p[i].v[k]));
+            box.max.v[k] = fmax_std(box.max[k], float_above(p[i].v[k]));
+        }
+    return box;
+}
+
+// the three corners of primitive p
+struct TriCorners {
+    vec3 a, b, c;
+};
+inline TriCorners corners(const prim_t &p, const float *positions, size_t stride) {
+    return {load3(&positions[p.i0 * stride]), load3(&positions[p.i1 * stride]), load3(&positions[p.i2 * stride])};
+}
+
+// SplitPrimitives_SAH (BVHSplit.cpp:126-390): the full sorted sweep along each
+// axis (primitives ordered by bbox max with std::sort) for the best object
+// split; with s.allow_spatial_splits the primitives' boxes are first clipped to
+// the node's box, and where the best object split leaves the children
+// overlapping by more than a small fraction of the root's area, 256 split
+// planes per axis are tried too, a triangle crossing the winning plane going to
+// both children with its box clipped to each side.
 split_data_t SplitPrimitives_SAH(const prim_t *primitives, const std::vector<uint32_t> &prim_indices,
-                                 const vec3 &bbox_min, const vec3 &bbox_max, const bvh_settings_t &s) {
+                                 const float *positions, size_t stride, const vec3 &bbox_min, const vec3 &bbox_max,
+                                 const vec3 &root_min, const vec3 &root_max, const bvh_settings_t &s) {
     const size_t num_tris = prim_indices.size();
     const bbox_t whole_box(bbox_min, bbox_max);
 
@@ -67,6 +170,19 @@ split_data_t SplitPrimitives_SAH(const prim_t *primitives, const std::vector<uin
         axis_lists[axis].resize(num_tris);
         for (uint32_t i = 0; i < (uint32_t)num_tris; i++) axis_lists[axis][i] = i;
     }
+
+    // primitive boxes as the sweep sees them: clipped to the node with spatial splits
+    const bool clip = s.allow_spatial_splits && positions;
+    std::vector<bbox_t> clipped;
+    if (clip) {
+        clipped.resize(num_tris);
+        for (size_t i = 0; i < num_tris; i++) {
+            const TriCorners t = corners(primitives[prim_indices[i]], positions, stride);
+            clipped[i] = clipped_box(t.a, t.b, t.c, whole_box);
+        }
+    }
+    auto box_min = [&](uint32_t i) -> const vec3 & { return clip ? clipped[i].min : primitives[prim_indices[i]].bbox_min; };
+    auto box_max = [&](uint32_t i) -> const vec3 & { return clip ? clipped[i].max : primitives[prim_indices[i]].bbox_max; };
 
     std::vector<bbox_t> right_bounds(num_tris);
 
@@ -77,23 +193,27 @@ split_data_t SplitPrimitives_SAH(const prim_t *primitives, const std::vector<uin
 
     for (int axis = 0; axis < 3; axis++) {
         std::vector<uint32_t> &list = axis_lists[axis];
-        std::sort(list.begin(), list.end(), [axis, primitives, &prim_indices](uint32_t p1, uint32_t p2) -> bool {
-            return primitives[prim_indices[p1]].bbox_max[axis] < primitives[prim_indices[p2]].bbox_max[axis];
-        });
+        if (clip) {
+            std::sort(list.begin(), list.end(), [axis, &clipped](uint32_t p1, uint32_t p2) -> bool {
+                return clipped[p1].max[axis] < clipped[p2].max[axis];
+            });
+        } else {
+            std::sort(list.begin(), list.end(), [axis, primitives, &prim_indices](uint32_t p1, uint32_t p2) -> bool {
+                return primitives[prim_indices[p1]].bbox_max[axis] < primitives[prim_indices[p2]].bbox_max[axis];
+            });
+        }
 
         bbox_t cur_right_bounds;
         for (size_t i = list.size() - 1; i > 0; i--) {
-            const prim_t &p = primitives[prim_indices[list[i]]];
-            cur_right_bounds.min = vmin(cur_right_bounds.min, p.bbox_min);
-            cur_right_bounds.max = vmax(cur_right_bounds.max, p.bbox_max);
+            cur_right_bounds.min = vmin(cur_right_bounds.min, box_min(list[i]));
+            cur_right_bounds.max = vmax(cur_right_bounds.max, box_max(list[i]));
             right_bounds[i - 1] = cur_right_bounds;
         }
 
         bbox_t left_bounds;
         for (size_t i = 1; i < list.size(); i++) {
-            const prim_t &p = primitives[prim_indices[list[i - 1]]];
-            left_bounds.min = vmin(left_bounds.min, p.bbox_min);
-            left_bounds.max = vmax(left_bounds.max, p.bbox_max);
+            left_bounds.min = vmin(left_bounds.min, box_min(list[i - 1]));
+            left_bounds.max = vmax(left_bounds.max, box_max(list[i - 1]));
 
             const float sah = s.node_traversal_cost * whole_box.surface_area() + left_bounds.surface_area() * i +
                               right_bounds[i - 1].surface_area() * (list.size() - i);
@@ -104,6 +224,95 @@ split_data_t SplitPrimitives_SAH(const prim_t *primitives, const std::vector<uin
                 res_left_bounds = left_bounds;
                 res_right_bounds = right_bounds[i - 1];
             }
+        }
+    }
+
+    // The spatial split.  (The reference also enters it with allow_spatial_splits
+    // and no positions, where it would then read clipped boxes it never made;
+    // mesh builds always pass positions and the other trees never ask for it.)
+    const bbox_t overlap(vmax(res_left_bounds.min, res_right_bounds.min), vmin(res_left_bounds.max, res_right_bounds.max));
+    const bool disjoint_free = !(overlap.max[0] <= overlap.min[0]) && !(overlap.max[1] <= overlap.min[1]) &&
+                               !(overlap.max[2] <= overlap.min[2]);
+    if (clip && disjoint_free &&
+        overlap.surface_area() > kSpatialSplitAlpha * bbox_t::surface_area(root_min, root_max)) {
+        struct Bin {
+            bbox_t extends, limits;
+            uint32_t enter = 0, exit = 0, prims = 0;
+        };
+        int spatial_split = -1;
+        std::vector<Bin> bins(kSpatialBins);
+        for (int axis = 0; axis < 3; axis++) {
+            const double bin_size = double(whole_box.max[axis] - whole_box.min[axis]) / kSpatialBins;
+            if (bin_size < RH_FLT_EPS) continue; // flat along this axis
+            for (int i = 0; i < kSpatialBins; i++) {
+                Bin &b = bins[i];
+                b = Bin();
+                b.limits = whole_box;
+                b.limits.min.v[axis] = float_below(whole_box.min[axis] + i * bin_size);
+                b.limits.max.v[axis] = float_above(whole_box.min[axis] + (i + 1) * bin_size);
+            }
+            bins[kSpatialBins - 1].limits.max.v[axis] = whole_box.max[axis];
+
+            for (const uint32_t i : axis_lists[axis]) {
+                const float lo = clipped[i].min[axis], hi = clipped[i].max[axis];
+                int first = 0, last = 0;
+                for (int j = 0; j < kSpatialBins && lo >= bins[j].limits.min[axis]; j++) first = j;
+                for (int j = first; j < kSpatialBins && hi >= bins[j].limits.min[axis]; j++) last = j;
+                bins[first].enter++;
+                bins[last].exit++;
+                const TriCorners t = corners(primitives[prim_indices[i]], positions, stride);
+                for (int j = first; j <= last; j++) {
+                    const bbox_t box = clipped_box(t.a, t.b, t.c, bins[j].limits);
+                    bins[j].extends.min = vmin(bins[j].extends.min, box.min);
+                    bins[j].extends.max = vmax(bins[j].extends.max, box.max);
+                    bins[j].prims++;
+                }
+            }
+            for (Bin &b : bins) {
+                if (!b.prims) {
+                    b.extends = b.limits;
+                } else {
+                    b.extends.min = vmax(b.extends.min, b.limits.min);
+                    b.extends.max = vmin(b.extends.max, b.limits.max);
+                }
+            }
+            for (int split = 1; split < kSpatialBins; split++) {
+                bbox_t left, right;
+                int n_left = 0, n_right = 0;
+                for (int i = 0; i < split; i++) {
+                    if (!bins[i].prims) continue;
+                    left.min = vmin(left.min, bins[i].extends.min);
+                    left.max = vmax(left.max, bins[i].extends.max);
+                    n_left += (int)bins[i].enter;
+                }
+                for (int i = split; i < kSpatialBins; i++) {
+                    if (!bins[i].prims) continue;
+                    right.min = vmin(right.min, bins[i].extends.min);
+                    right.max = vmax(right.max, bins[i].extends.max);
+                    n_right += (int)bins[i].exit;
+                }
+                const float sah =
+                    s.node_traversal_cost + left.surface_area() * n_left + right.surface_area() * n_right;
+                if (sah < res_sah) {
+                    res_sah = sah;
+                    spatial_split = split;
+                    div_axis = axis;
+                    res_left_bounds = left;
+                    res_right_bounds = right;
+                }
+            }
+        }
+        if (spatial_split != -1) {
+            // a primitive reaching into both sides is referenced by both children
+            split_data_t out;
+            for (const uint32_t i : axis_lists[div_axis]) {
+                if (clipped[i].min[div_axis] <= res_left_bounds.max[div_axis]) out.left_indices.push_back(prim_indices[i]);
+                if (clipped[i].max[div_axis] >= res_right_bounds.min[div_axis])
+                    out.right_indices.push_back(prim_indices[i]);
+            }
+            out.left_bounds = res_left_bounds;
+            out.right_bounds = res_right_bounds;
+            return out;
         }
     }
 
@@ -224,7 +433,7 @@ uint32_t PreprocessMesh(const float *attrs, const uint32_t *vtx_indices, size_t 
     return num_out_nodes;
 }
 
-uint32_t PreprocessPrims_SAH(const prim_t *prims, size_t prims_count, const float * /*positions*/, size_t /*stride*/,
+uint32_t PreprocessPrims_SAH(const prim_t *prims, size_t prims_count, const float *positions, size_t stride,
                              const bvh_settings_t &s, std::vector<rh_bvh_node> &out_nodes,
                              std::vector<uint32_t> &out_indices) {
     struct prims_coll_t {
@@ -251,9 +460,10 @@ uint32_t PreprocessPrims_SAH(const prim_t *prims, size_t prims_count, const floa
         prim_lists.back().max = vmax(prim_lists.back().max, prims[j].bbox_max);
     }
 
+    const vec3 root_min = prim_lists.back().min, root_max = prim_lists.back().max;
     while (!prim_lists.empty()) {
-        split_data_t split =
-            SplitPrimitives_SAH(prims, prim_lists.back().indices, prim_lists.back().min, prim_lists.back().max, s);
+        split_data_t split = SplitPrimitives_SAH(prims, prim_lists.back().indices, positions, stride,
+                                                 prim_lists.back().min, prim_lists.back().max, root_min, root_max, s);
         prim_lists.pop_back();
 
         if (split.right_indices.empty()) {

@@ -130,14 +130,14 @@ def cornell_geometry(seed: int = SEED_BASE + 1):
 
 
 def cornell_box(scene: "A.Scene", seed: int = SEED_BASE + 1, width=256, height=256, spp=16, depth=4,
-                fast_bvh=False, coherent=False) -> SceneSpec:
+                fast_bvh=False, coherent=False, spatial_splits=False) -> SceneSpec:
     attrs, idx, shapes, albedos = cornell_geometry(seed)
     tex = _white(scene)
     mats = [scene.add_material(A.DiffuseMaterial, tex, main_color=tuple(float(v) for v in albedos[k]))
             for k in range(7)]
     mats.append(scene.add_material(A.EmissiveMaterial, tex, main_color=(1.0, 1.0, 1.0), strength=10.0))
     mesh = scene.add_mesh(A.PxyzNxyzTuv, attrs, idx, [(mats[k], s, c) for k, s, c in shapes],
-                          use_fast_bvh_build=fast_bvh)
+                          use_fast_bvh_build=fast_bvh, allow_spatial_splits=spatial_splits)
     scene.add_mesh_instance(mesh)
     scene.set_environment((0.0, 0.0, 0.0))
     cam = scene.add_camera(origin=(0.0, 5.0, -14.0), fwd=(0.0, 0.0, 1.0), fov=45.0, filter=A.Box,
@@ -345,7 +345,7 @@ def soup_geometry(seed: int = SEED_BASE + 2, n_spheres: int = 512, n_fine: int =
 
 
 def triangle_soup(scene: "A.Scene", seed: int = SEED_BASE + 2, width=1920, height=1080, spp=64, depth=8,
-                  n_spheres=512, n_fine=38, fast_bvh=False) -> SceneSpec:
+                  n_spheres=512, n_fine=38, fast_bvh=False, spatial_splits=False) -> SceneSpec:
     attrs, idx, spans, kind, albedo, rough = soup_geometry(seed, n_spheres, n_fine)
     tex = _white(scene)
     mats = []
@@ -357,7 +357,8 @@ def triangle_soup(scene: "A.Scene", seed: int = SEED_BASE + 2, width=1920, heigh
             mats.append(scene.add_material(A.GlossyMaterial, tex, main_color=col, roughness=float(rough[s])))
     ground = scene.add_material(A.DiffuseMaterial, tex, main_color=(0.6, 0.6, 0.6))
     shapes = [(mats[s], spans[s][0], spans[s][1]) for s in range(len(kind))] + [(ground, spans[-1][0], spans[-1][1])]
-    mesh = scene.add_mesh(A.PxyzNxyzTuv, attrs, idx, shapes, use_fast_bvh_build=fast_bvh)
+    mesh = scene.add_mesh(A.PxyzNxyzTuv, attrs, idx, shapes, use_fast_bvh_build=fast_bvh,
+                          allow_spatial_splits=spatial_splits)
     scene.add_mesh_instance(mesh)
     scene.set_environment((0.5, 0.5, 0.5))
     scene.add_light(A.PointLight, position=(-10.0, 25.0, -10.0), radius=0.5, color=(50.0, 50.0, 50.0))
@@ -365,6 +366,52 @@ def triangle_soup(scene: "A.Scene", seed: int = SEED_BASE + 2, width=1920, heigh
     cam = scene.add_camera(**soup_camera(depth))
     scene.set_current_cam(cam)
     return SceneSpec("triangle_soup", width, height, spp, depth, triangles=len(idx) // 3)
+
+
+def sliver_geometry(seed: int = SEED_BASE + 7, n: int = 400):
+    """Long thin triangles crossing a 20^3 box in random directions over a
+    ground quad: every split of their boxes leaves the halves overlapping, the
+    case spatial splits are for.  (attrs, indices) in the PxyzNxyzTuv layout."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-8.0, 8.0, size=(n, 3))
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    half = rng.uniform(4.0, 12.0, size=n)[:, None]
+    w = rng.standard_normal((n, 3))
+    w -= (w * d).sum(1, keepdims=True) * d
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    width = rng.uniform(0.05, 0.4, size=n)[:, None]
+    p0, p1, p2 = c - d * half, c + d * half, c + w * width
+    nrm = np.cross(p1 - p0, p2 - p0)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    verts = np.stack([p0, p1, p2], 1).reshape(-1, 3)
+    nv = np.repeat(nrm, 3, axis=0)
+    uv = np.tile(np.float64([[0, 0], [1, 0], [0, 1]]), (n, 1))
+    attrs = np.concatenate([verts, nv, uv], 1)
+    ga, gi = _quad((-30.0, -10.0, -30.0), (-30.0, -10.0, 30.0), (30.0, -10.0, 30.0), (30.0, -10.0, -30.0), (0, 1, 0))
+    idx = np.concatenate([np.arange(3 * n), gi + 3 * n])
+    return np.concatenate([attrs, ga]).astype(np.float32), idx.astype(np.uint32)
+
+
+def sliver_scene(scene: "A.Scene", seed: int = SEED_BASE + 7, n: int = 400, width=96, height=64, spp=4, depth=4,
+                 spatial_splits=True) -> SceneSpec:
+    """sliver_geometry lit by two sphere lights and a grey environment, one
+    diffuse material for the slivers and one for the ground."""
+    attrs, idx = sliver_geometry(seed, n)
+    tex = _white(scene)
+    m0 = scene.add_material(A.DiffuseMaterial, tex, main_color=(0.7, 0.5, 0.3))
+    m1 = scene.add_material(A.DiffuseMaterial, tex, main_color=(0.5, 0.5, 0.5))
+    mesh = scene.add_mesh(A.PxyzNxyzTuv, attrs, idx, [(m0, 0, 3 * n), (m1, 3 * n, 6)],
+                          allow_spatial_splits=spatial_splits)
+    scene.add_mesh_instance(mesh)
+    scene.set_environment((0.3, 0.3, 0.3))
+    scene.add_light(A.PointLight, position=(-6.0, 14.0, -8.0), radius=0.5, color=(40.0, 40.0, 40.0))
+    scene.add_light(A.PointLight, position=(9.0, 12.0, 4.0), radius=0.5, color=(40.0, 40.0, 40.0))
+    cam = scene.add_camera(origin=(0.0, 2.0, -30.0), fwd=(0.0, -0.05, 1.0), fov=50.0, filter=A.Box,
+                           dtype=A.DeviceNone, max_diff_depth=depth, max_glossy_depth=depth, max_refr_depth=depth,
+                           max_transp_depth=depth, max_total_depth=depth)
+    scene.set_current_cam(cam)
+    return SceneSpec("sliver_scene", width, height, spp, depth, triangles=len(idx) // 3)
 
 
 def soup_camera(depth=8, coherent=False) -> dict:

@@ -81,3 +81,9 @@ def oracle():
         pytest.skip("CPU oracle oracle/liboracle.so not built")
     from helpers import Oracle
     return Oracle(ORACLE_LIB)
+
+
+def no_overflow(renderer_or_batch) -> bool:
+    """True when no traversal stack push was dropped (a dropped push is a
+    possibly missed hit): every GPU image comparison checks it."""
+    return renderer_or_batch.ray_counts()["overflow"] == 0

@@ -6,7 +6,7 @@ whatever the launch knobs."""
 import numpy as np
 import pytest
 
-from conftest import standalone_scene
+from conftest import no_overflow, standalone_scene
 from raydemo_amd import capi as A, scenes
 
 pytestmark = pytest.mark.gpu
@@ -29,6 +29,7 @@ def _single(lib, build, w, h, spp):
     reg = A.Region(lib, 0, 0, w, h)
     for _ in range(spp):
         r.render(s, reg)
+    assert no_overflow(r)
     img = r.pixels()
     rc = r.ray_counts()
     return img, rc
@@ -53,6 +54,7 @@ def test_batch_matches_single_renders(host_lib, golden_scene1, opts):
     for i, (name, _, w, h, spp) in enumerate(cases):
         assert batch.pixels(i).tobytes() == singles[i][0].tobytes(), name
     rc = batch.ray_counts()
+    assert rc["overflow"] == 0
     for key in ("primary", "secondary", "shadow"):
         assert rc[key] == sum(s[1][key] for s in singles), key
 

@@ -7,7 +7,7 @@ oracle's and the reference Ref backend's."""
 import numpy as np
 import pytest
 
-from conftest import standalone_scene
+from conftest import no_overflow, standalone_scene
 from raydemo_amd import capi as A, scenes
 from test_hlbvh import CASES, _mesh_scene, _random_tris
 
@@ -65,6 +65,7 @@ def test_fast_built_soup_renders_as_reference(ref_lib, host_lib, oracle, wide):
     reg = A.Region(host_lib, 0, 0, w, h)
     for _ in range(spp):
         r.render(s, reg)
+    assert no_overflow(r)
     img = r.pixels()
     hs = standalone_scene(host_lib, wide=wide)
     build(hs)

@@ -6,7 +6,7 @@ from the same scene description."""
 import numpy as np
 import pytest
 
-from conftest import standalone_scene
+from conftest import no_overflow, standalone_scene
 from raydemo_amd import capi as A, scenes
 from test_geo_oracle import _ref_render
 
@@ -59,6 +59,7 @@ def test_feature_renders_as_reference(ref_lib, host_lib, oracle, name):
     reg = A.Region(host_lib, 0, 0, W, H)
     for _ in range(SPP):
         r.render(s, reg)
+    assert no_overflow(r)
     img = r.pixels()
     img_r, _ = _ref_render(ref_lib, build, W, H, SPP, sh=False)
     e = float(np.sqrt(np.mean((img[..., :3].astype(np.float64) - img_r[..., :3]) ** 2)))

@@ -6,7 +6,7 @@ SampleMeshInTextureSpace (CoreRef.cpp:507-613) and of the SH framebuffer
 import numpy as np
 import pytest
 
-from conftest import standalone_scene
+from conftest import no_overflow, standalone_scene
 from raydemo_amd import capi as A, scenes
 from test_geo_oracle import _ref_render
 
@@ -25,6 +25,7 @@ def _gpu(lib, build, w, h, spp, wide=False):
         r.render(s, reg)
     acc = np.zeros((h, w, 4), np.float32)
     r.read_accum(acc.ctypes.data)
+    assert no_overflow(r)
     return r, acc, r.pixels(), r.sh_data()
 
 
@@ -106,6 +107,7 @@ def test_coherent_sampling_matches_oracle_and_reference(ref_lib, host_lib, oracl
     reg = A.Region(host_lib, 0, 0, w, h)
     for _ in range(spp):
         r.render(s, reg)
+    assert no_overflow(r)
     img = r.pixels()
     hs = standalone_scene(host_lib, wide=True)
     build(hs)

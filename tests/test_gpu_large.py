@@ -7,7 +7,7 @@ batched together with small soups."""
 import numpy as np
 import pytest
 
-from conftest import standalone_scene
+from conftest import no_overflow, standalone_scene
 from raydemo_amd import capi as A, scenes
 
 pytestmark = pytest.mark.gpu
@@ -44,6 +44,7 @@ def test_large_render_matches_oracle(host_lib, deep_oracle, wide):
     reg = A.Region(host_lib, 0, 0, W, H)
     for _ in range(SPP):
         r.render(s, reg)
+    assert no_overflow(r)
     img = r.pixels()
     hs = standalone_scene(host_lib, wide=wide)
     _build(hs)
@@ -139,6 +140,7 @@ def test_large_scene_batched_with_small_scenes(host_lib):
         reg = A.Region(host_lib, 0, 0, w, h)
         for _ in range(spp):
             r.render(s, reg)
+        assert no_overflow(r)
         singles.append(r.pixels())
     batch = A.Batch(host_lib)
     batch.set_option("max_bounces", 12)
@@ -149,6 +151,7 @@ def test_large_scene_batched_with_small_scenes(host_lib):
         keep.append(s)
         batch.add(s, w, h, spp)
     batch.render_all()
+    assert no_overflow(batch)
     for i in range(len(cases)):
         assert batch.pixels(i).tobytes() == singles[i].tobytes(), i
 

@@ -5,7 +5,7 @@ fed by the same loader."""
 import numpy as np
 import pytest
 
-from conftest import standalone_scene
+from conftest import no_overflow, standalone_scene
 from raydemo_amd import capi as A, load
 from test_load import write_scene
 
@@ -24,6 +24,7 @@ def test_loaded_scene_renders_as_oracle_and_reference(tmp_path, ref_lib, host_li
     reg = A.Region(host_lib, 0, 0, w, h)
     for _ in range(spp):
         r.render(s, reg)
+    assert no_overflow(r)
     img = r.pixels()
     hs = standalone_scene(host_lib, wide=wide)
     load.load_scene(hs, path, base_dir=str(tmp_path))

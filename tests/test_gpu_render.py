@@ -7,7 +7,7 @@ import pytest
 
 from raydemo_amd import capi as A, scenes
 
-from conftest import standalone_scene
+from conftest import no_overflow, standalone_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -22,6 +22,7 @@ def render(lib, flags, build, w, h, spp, **opts):
     reg = A.Region(lib, 0, 0, w, h)
     for _ in range(spp):
         r.render(s, reg)
+    assert no_overflow(r)
     return r.pixels(), r
 
 
@@ -72,6 +73,33 @@ def test_soup_matches_reference_small(ref_lib, host_lib):
     assert e <= 1e-3
 
 
+@pytest.mark.parametrize("wide", [False, True])
+def test_spatial_split_scene_matches_reference(ref_lib, host_lib, wide):
+    """A mesh built with allow_spatial_splits (triangles referenced by several
+    leaves) renders as the reference renders it."""
+    build = lambda s: scenes.sliver_scene(s, n=1000, seed=scenes.SEED_BASE + 71)
+    w, h, spp = 96, 64, 4
+    r = A.Renderer(host_lib, w, h, A.RENDERER_HIP, use_wide_bvh=wide)
+    s = r.create_scene()
+    build(s)
+    v = s.view()
+    assert len(v["tri_indices"]) > len(v["tris"])
+    reg = A.Region(host_lib, 0, 0, w, h)
+    for _ in range(spp):
+        r.render(s, reg)
+    assert no_overflow(r)
+    img_g = r.pixels()
+    rr = A.Renderer(ref_lib, w, h, A.RENDERER_REF, use_wide_bvh=wide)
+    rs = rr.create_scene()
+    build(rs)
+    rreg = A.Region(ref_lib, 0, 0, w, h)
+    for _ in range(spp):
+        rr.render(rs, rreg)
+    e = rmse(img_g, rr.pixels())
+    print(f"spatial-split slivers ({'8-wide' if wide else 'binary'}): RMSE {e:.3e}")
+    assert e <= 1e-3
+
+
 @pytest.mark.parametrize("opts", [{}, {"sort_rays": 0, "lds_stack_depth": 8, "path_tail": -1}, {"defer_nee": 0},
                                   {"path_tail": 0}, {"path_tail": 1, "tail_occupancy": 3}, {"refill": 1}])
 def test_ray_counts_match_oracle(oracle, host_lib, opts):
@@ -86,3 +114,32 @@ def test_ray_counts_match_oracle(oracle, host_lib, opts):
     rc = r.ray_counts()
     assert (rc["primary"], rc["secondary"], rc["shadow"]) == tuple(int(c) for c in counts)
     assert rmse(img, ref_img) <= 1e-3
+
+
+def test_stage_stats_filled_by_default(host_lib):
+    """GetStats (RendererBase::stats_t, RendererBase.h:93-102) reports the stage
+    timers of every pass with no option set: the generation, trace and shade
+    stages are non-zero and their sum stays within the wall time of the
+    passes (the timers are HIP events at the stage boundaries; no pass waits
+    for them)."""
+    import time
+    w, h, spp = 256, 256, 8
+    r = A.Renderer(host_lib, w, h, A.RENDERER_HIP)
+    s = r.create_scene()
+    scenes.cornell_box(s)
+    reg = A.Region(host_lib, 0, 0, w, h)
+    r.render(s, reg)
+    r.synchronize()
+    r.reset_stats()
+    assert all(v == 0 for v in r.stats().values())
+    t0 = time.perf_counter()
+    for _ in range(spp):
+        r.render(s, reg)
+    r.synchronize()
+    wall_us = (time.perf_counter() - t0) * 1e6
+    st = r.stats()
+    print(f"stats over {spp} passes: {st}, wall {wall_us:.0f} us")
+    for k in ("time_primary_ray_gen_us", "time_primary_trace_us", "time_primary_shade_us",
+              "time_secondary_trace_us", "time_secondary_shade_us"):
+        assert st[k] > 0, k
+    assert sum(st.values()) <= wall_us

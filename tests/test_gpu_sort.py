@@ -5,7 +5,7 @@ the sort is invisible in the image."""
 import numpy as np
 import pytest
 
-from conftest import standalone_scene
+from conftest import no_overflow, standalone_scene
 from helpers import ref_ray_hashes, ref_sort_rays
 from raydemo_amd import capi as A, scenes
 from raydemo_amd.rh import Context
@@ -73,6 +73,7 @@ def test_sort_is_invisible_in_the_image(host_lib, lds_depth):
         reg = A.Region(host_lib, 0, 0, 160, 90)
         for _ in range(4):
             r.render(s, reg)
+        assert no_overflow(r)
         imgs.append(r.pixels())
     for im in imgs[1:]:
         assert im.tobytes() == imgs[0].tobytes()

@@ -7,7 +7,7 @@ on a wide scene against the oracle's FP64 gradient."""
 import numpy as np
 import pytest
 
-from conftest import standalone_scene
+from conftest import no_overflow, standalone_scene
 from helpers import ref_gen_primary, ref_halton, ref_trace, secondary_rays_from_hits
 from raydemo_amd import capi as A, scenes
 from raydemo_amd.rh import Context
@@ -61,6 +61,7 @@ def _render(lib, flags, build, w, h, spp, wide=True, **opts):
     reg = A.Region(lib, 0, 0, w, h)
     for _ in range(spp):
         r.render(s, reg)
+    assert no_overflow(r)
     return r.pixels(), r
 
 

@@ -1,0 +1,110 @@
+"""Spatial splits (mesh_desc_t::allow_spatial_splits, SceneBase.h:93): the
+host build's SplitPrimitives_SAH with clipped triangle boxes and 256-bin split
+planes (BVHSplit.cpp:6-124, 220-369) against the reference library's build of
+the same meshes, node for node, and against what a spatial split must
+preserve: every triangle is reachable, each triangle's clipped part lies in the
+leaves that reference it, and closest hits equal a brute-force search."""
+import numpy as np
+import pytest
+
+from conftest import standalone_scene
+from test_oracle import same_hits
+from raydemo_amd import capi as A, scenes
+from test_scene_build import canon_tree
+
+BUILDS = {
+    "slivers": lambda s: scenes.sliver_scene(s, n=300),
+    "slivers_1000_s1": lambda s: scenes.sliver_scene(s, seed=scenes.SEED_BASE + 71, n=1000),
+    "slivers_1000_s2": lambda s: scenes.sliver_scene(s, seed=scenes.SEED_BASE + 72, n=1000),
+    "cornell": lambda s: scenes.cornell_box(s, spatial_splits=True),
+    "soup": lambda s: scenes.triangle_soup(s, n_spheres=24, n_fine=2, spatial_splits=True),
+}
+
+
+def _walk_leaves(nodes, root):
+    st, out = [root], []
+    while st:
+        c = st.pop()
+        n = nodes[c]
+        if n["prim_index"] & 0x80000000:
+            out.append(c)
+        else:
+            st.append(n["prim_count"] & 0x3FFFFFFF)
+            st.append(n["prim_index"])
+    return out
+
+
+@pytest.mark.parametrize("name", list(BUILDS))
+def test_spatial_split_tree_matches_reference(ref_lib, host_lib, name):
+    rs, hs = standalone_scene(ref_lib), standalone_scene(host_lib)
+    BUILDS[name](rs)
+    BUILDS[name](hs)
+    r, h = rs.view(), hs.view()
+    for k in ("tris", "tri_indices", "meshes", "vertices", "vtx_indices"):
+        assert r[k].tobytes() == h[k].tobytes(), k
+    for m_r, m_h in zip(r["meshes"], h["meshes"]):
+        assert canon_tree(r["nodes"], m_r["node_index"]) == canon_tree(h["nodes"], m_h["node_index"])
+
+
+def test_spatial_splits_duplicate_references(host_lib):
+    """The sliver mesh really is split spatially: the leaves reference more
+    triangles than the mesh has, and without the flag they do not.  (Under the
+    reference's cost, 2 (dx + dy + dz) per box, a spatial split rarely beats
+    the best object split: a few per thousand triangles.)"""
+    on, off = standalone_scene(host_lib), standalone_scene(host_lib)
+    scenes.sliver_scene(on, n=300)
+    scenes.sliver_scene(off, n=300, spatial_splits=False)
+    v_on, v_off = on.view(), off.view()
+    n_tris = len(v_on["tris"])
+    assert len(v_off["tri_indices"]) == n_tris
+    assert len(v_on["tri_indices"]) > n_tris
+    assert np.array_equal(np.unique(v_on["tri_indices"]), np.arange(n_tris))
+
+
+def test_spatial_split_leaves_cover_their_triangles(host_lib):
+    """Each triangle's part inside a leaf box is covered: sample points of every
+    triangle lie in the box of at least one leaf that references it."""
+    hs = standalone_scene(host_lib)
+    scenes.sliver_scene(hs, n=300)
+    v = hs.view()
+    nodes, ti = v["nodes"], v["tri_indices"]
+    P = v["vertices"]["p"][v["vtx_indices"]].reshape(-1, 3, 3).astype(np.float64)
+    boxes_of = {}
+    for c in _walk_leaves(nodes, v["meshes"][0]["node_index"]):
+        first = nodes[c]["prim_index"] & 0x7FFFFFFF
+        for t in ti[first:first + nodes[c]["prim_count"]]:
+            boxes_of.setdefault(int(t), []).append((nodes[c]["bbox_min"], nodes[c]["bbox_max"]))
+    rng = np.random.default_rng(3)
+    for t in range(len(P)):
+        bc = rng.dirichlet((1, 1, 1), size=64)
+        pts = bc @ P[t]
+        inside = np.zeros(len(pts), bool)
+        for lo, hi in boxes_of[t]:
+            eps = 1e-5 * (1 + np.abs(hi))
+            inside |= ((pts >= lo - eps) & (pts <= hi + eps)).all(1)
+        assert inside.all(), t
+
+
+def test_spatial_split_trace(oracle, ref_lib, host_lib):
+    """Hits through the spatially split tree: equal to the reference library's
+    on the same rays, and the closest hit equals the one found with the
+    unsplit tree (the closest surface does not depend on the tree)."""
+    rs, hs, plain = standalone_scene(ref_lib), standalone_scene(host_lib), standalone_scene(host_lib)
+    scenes.sliver_scene(rs, n=300)
+    scenes.sliver_scene(hs, n=300)
+    scenes.sliver_scene(plain, n=300, spatial_splits=False)
+    from helpers import ref_trace
+    rng = np.random.default_rng(11)
+    n = 4000
+    rays = np.zeros(n, A.RAY_PACKET_DTYPE)
+    rays["o"] = rng.uniform(-14, 14, (n, 3)).astype(np.float32)
+    d = rng.standard_normal((n, 3))
+    rays["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    h_split = oracle.trace(hs.raw_view(), rays)
+    assert same_hits(h_split, ref_trace(ref_lib, rs, rays))
+    h_plain = oracle.trace(plain.raw_view(), rays)
+    hit = h_plain["mask"] != 0
+    assert ((h_split["mask"] != 0) == hit).all()
+    assert np.array_equal(h_split["t"][hit], h_plain["t"][hit])
+    assert (h_split["prim"][hit] == h_plain["prim"][hit]).mean() > 0.999
+    assert hit.mean() > 0.2
