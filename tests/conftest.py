@@ -85,5 +85,7 @@ def oracle():
 
 def no_overflow(renderer_or_batch) -> bool:
     """True when no traversal stack push was dropped (a dropped push is a
-    possibly missed hit): every GPU image comparison checks it."""
-    return renderer_or_batch.ray_counts()["overflow"] == 0
+    possibly missed hit): every GPU image comparison checks it.  The reference
+    library's renderers have no device counters (an empty dict) and pass."""
+    rc = renderer_or_batch.ray_counts()
+    return not rc or rc["overflow"] == 0

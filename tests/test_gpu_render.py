@@ -77,7 +77,7 @@ def test_soup_matches_reference_small(ref_lib, host_lib):
 def test_spatial_split_scene_matches_reference(ref_lib, host_lib, wide):
     """A mesh built with allow_spatial_splits (triangles referenced by several
     leaves) renders as the reference renders it."""
-    build = lambda s: scenes.sliver_scene(s, n=1000, seed=scenes.SEED_BASE + 71)
+    build = lambda s: scenes.sliver_scene(s, n=300)
     w, h, spp = 96, 64, 4
     r = A.Renderer(host_lib, w, h, A.RENDERER_HIP, use_wide_bvh=wide)
     s = r.create_scene()
@@ -98,6 +98,29 @@ def test_spatial_split_scene_matches_reference(ref_lib, host_lib, wide):
     e = rmse(img_g, rr.pixels())
     print(f"spatial-split slivers ({'8-wide' if wide else 'binary'}): RMSE {e:.3e}")
     assert e <= 1e-3
+
+
+@pytest.mark.parametrize("wide", [False, True])
+def test_dense_spatial_split_scene_matches_oracle(oracle, host_lib, wide):
+    """1000 overlapping slivers: rays cross hundreds of boxes, and the
+    reference's own 8-wide walk overflows its fixed 32-entry stack here
+    (TraversalStack::push asserts, CoreRef.cpp:177); the device stack (LDS part
+    plus global spill) holds, and the image equals the oracle's."""
+    build = lambda s: scenes.sliver_scene(s, n=1000, seed=scenes.SEED_BASE + 71)
+    w, h, spp = 96, 64, 4
+    r = A.Renderer(host_lib, w, h, A.RENDERER_HIP, use_wide_bvh=wide)
+    s = r.create_scene()
+    build(s)
+    reg = A.Region(host_lib, 0, 0, w, h)
+    for _ in range(spp):
+        r.render(s, reg)
+    assert no_overflow(r)
+    hs = standalone_scene(host_lib, wide=wide)
+    build(hs)
+    ref, counts = oracle.render(hs.raw_view(), hs.camera_record(0), w, h, spp)
+    rc = r.ray_counts()
+    assert (rc["primary"], rc["secondary"], rc["shadow"]) == tuple(int(c) for c in counts)
+    assert rmse(r.pixels(), ref) <= 1e-3
 
 
 @pytest.mark.parametrize("opts", [{}, {"sort_rays": 0, "lds_stack_depth": 8, "path_tail": -1}, {"defer_nee": 0},

@@ -34,16 +34,20 @@ def _walk_leaves(nodes, root):
     return out
 
 
+@pytest.mark.parametrize("wide", [False, True])
 @pytest.mark.parametrize("name", list(BUILDS))
-def test_spatial_split_tree_matches_reference(ref_lib, host_lib, name):
-    rs, hs = standalone_scene(ref_lib), standalone_scene(host_lib)
+def test_spatial_split_tree_matches_reference(ref_lib, host_lib, name, wide):
+    rs, hs = standalone_scene(ref_lib, wide), standalone_scene(host_lib, wide)
     BUILDS[name](rs)
     BUILDS[name](hs)
     r, h = rs.view(), hs.view()
     for k in ("tris", "tri_indices", "meshes", "vertices", "vtx_indices"):
         assert r[k].tobytes() == h[k].tobytes(), k
-    for m_r, m_h in zip(r["meshes"], h["meshes"]):
-        assert canon_tree(r["nodes"], m_r["node_index"]) == canon_tree(h["nodes"], m_h["node_index"])
+    if wide:
+        assert r["mnodes"].tobytes() == h["mnodes"].tobytes()
+    else:
+        for m_r, m_h in zip(r["meshes"], h["meshes"]):
+            assert canon_tree(r["nodes"], m_r["node_index"]) == canon_tree(h["nodes"], m_h["node_index"])
 
 
 def test_spatial_splits_duplicate_references(host_lib):
