@@ -108,6 +108,20 @@ int rh_trace_rays(rh_context *ctx, const rh_ray_packet *rays, size_t n, int any_
 int rh_ray_hashes(rh_context *ctx, const rh_ray_packet *rays, size_t n, const float root_min[3],
                   const float cell_size[3], uint32_t *out);
 int rh_sort_rays(rh_context *ctx, rh_ray_packet *rays, size_t n, const float root_min[3], const float cell_size[3]);
+/* Allocation-free form of rh_sort_rays on device buffers.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Sorts the n rays (d_rays, d_aux: the split layout of rh_types.h) stably by
+ * their get_ray_hash key over the hash grid (root_min, cell_size) into
+ * d_rays_out/d_aux_out, as rh_sort_rays does, on `stream` (null: the default
+ * stream) of the current device, asynchronously.  It allocates nothing: the
+ * keys, values and digit counts live in a fixed device array of the library
+ * (about 34 MB per GPU), so n is at most RH_NOALLOC_MAX_RAYS (larger n is an
+ * argument error) and calls that overlap on one GPU race on that array.  The
+ * output must not alias the input. */
+#define RH_NOALLOC_MAX_RAYS 2097152
+int rh_sort_rays_noalloc(const rh_ray *d_rays, const rh_ray_aux *d_aux, size_t n, const float root_min[3],
+                         const float cell_size[3], rh_ray *d_rays_out, rh_ray_aux *d_aux_out, void *stream);
 /* Shades n (ray, hit) pairs, each with its own pass info; secondary rays are
  * appended in arbitrary order, out_src[k] naming the input ray of out_secondary[k]. */
 int rh_shade_rays(rh_context *ctx, const rh_pass_info *pi, const rh_ray_packet *rays, const rh_hit *hits, size_t n,

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1099,26 +1100,32 @@ int launch_shade(rh_context *c, const ShadeArgs &a, int grid_rays) {
 }
 
 // Stable LSB radix sort of (key, val) pairs in keys[0]/vals[0] by the top
-// `bits` bits of the key (8, 16, 24 or 32: one 8-bit pass each).  Returns the
-// buffer index (0 or 1) that holds the sorted pairs.
-int radix_sort(rh_context *c, const int *count, int max_n, int bits, int *result) {
+// `bits` bits of the key (8, 16, 24 or 32: one 8-bit pass each), on `stream`;
+// hist holds 256 * ceil(max_n / kSortTile) + 256 words.  Returns the buffer
+// index (0 or 1) that holds the sorted pairs.
+int radix_passes(hipStream_t stream, uint32_t *const keys[2], uint32_t *const vals[2], uint32_t *hist,
+                 const int *count, int max_n, int bits, int *result) {
     const int tiles = std::max(1, (max_n + kSortTile - 1) / kSortTile);
-    KernelMark km(c, 2);
     int src = 0;
     for (int shift = 32 - bits; shift < 32; shift += 8) {
-        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kBlock), 0, c->stream, c->keys[src], count, max_n, shift,
-                           c->hist, tiles);
-        hipLaunchKernelGGL(k_radix_scan_rows, dim3(256), dim3(kBlock), 0, c->stream, c->hist, tiles,
-                           c->hist + (size_t)256 * tiles);
-        hipLaunchKernelGGL(k_radix_scan_bases, dim3(256), dim3(kBlock), 0, c->stream, c->hist, tiles,
-                           c->hist + (size_t)256 * tiles);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kBlock), 0, c->stream, c->keys[src], c->vals[src],
-                           c->keys[src ^ 1], c->vals[src ^ 1], count, max_n, shift, c->hist, tiles);
+        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kBlock), 0, stream, keys[src], count, max_n, shift, hist,
+                           tiles);
+        hipLaunchKernelGGL(k_radix_scan_rows, dim3(256), dim3(kBlock), 0, stream, hist, tiles,
+                           hist + (size_t)256 * tiles);
+        hipLaunchKernelGGL(k_radix_scan_bases, dim3(256), dim3(kBlock), 0, stream, hist, tiles,
+                           hist + (size_t)256 * tiles);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kBlock), 0, stream, keys[src], vals[src],
+                           keys[src ^ 1], vals[src ^ 1], count, max_n, shift, hist, tiles);
         src ^= 1;
     }
     RH_CHECK(hipGetLastError());
     *result = src;
     return 0;
+}
+
+int radix_sort(rh_context *c, const int *count, int max_n, int bits, int *result) {
+    KernelMark km(c, 2);
+    return radix_passes(c->stream, c->keys, c->vals, c->hist, count, max_n, bits, result);
 }
 
 PassConst make_pass_const(rh_context *c, const rh_pass_desc *p) { return make_pass_const(c->w, c->h, p); }
@@ -1192,6 +1199,37 @@ struct PassStages {
     }
 };
 
+} // namespace
+
+// ---- scratch of the allocation-free ray sort (rh_sort_rays_noalloc) ----
+// Keys and values twice (the ping-pong buffers of the radix passes), then the
+// per-tile digit counts and the 256 digit totals, for at most
+// RH_NOALLOC_MAX_RAYS rays: 34 MB on every GPU the library runs on, held until
+// the process ends.  Host code never uses the array's name as an address: the
+// device address of each GPU's copy comes from hipGetSymbolAddress (scratch_ptr).
+constexpr size_t kNoAllocTiles = ((size_t)RH_NOALLOC_MAX_RAYS + kSortTile - 1) / kSortTile;
+constexpr size_t kNoAllocWords = 4 * (size_t)RH_NOALLOC_MAX_RAYS + 256 * kNoAllocTiles + 256;
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t *g_scratch[kScratchDevices] = {};
+
+// The current device's copy of `scratch`, looked up once per device.
+int scratch_ptr(uint32_t **out) {
+    int dev = 0;
+    RH_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices) return arg_error("device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        RH_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = static_cast<uint32_t *>(p);
+    }
+    *out = g_scratch[dev];
+    return 0;
+}
 } // namespace
 
 extern "C" {
@@ -1942,6 +1980,36 @@ int rh_sort_rays(rh_context *c, rh_ray_packet *rays, size_t n, const float root_
                        c->aux[0], c->vals[sorted], nullptr, (int)n, c->rays[1], c->aux[1]);
     RH_CHECK(hipGetLastError());
     return download_packets(c, rays, n, 1);
+}
+
+int rh_sort_rays_noalloc(const rh_ray *d_rays, const rh_ray_aux *d_aux, size_t n, const float root_min[3],
+                         const float cell_size[3], rh_ray *d_rays_out, rh_ray_aux *d_aux_out, void *stream) {
+    if (!n) return 0;
+    if (n > (size_t)RH_NOALLOC_MAX_RAYS) return arg_error("rh_sort_rays_noalloc: more rays than RH_NOALLOC_MAX_RAYS");
+    if (!d_rays || !d_aux || !d_rays_out || !d_aux_out || !root_min || !cell_size)
+        return arg_error("rh_sort_rays_noalloc: null buffer");
+    if ((const void *)d_rays == (const void *)d_rays_out || (const void *)d_aux == (const void *)d_aux_out)
+        return arg_error("rh_sort_rays_noalloc: the output must not be the input");
+    uint32_t *base = nullptr;
+    const int rc0 = scratch_ptr(&base);
+    if (rc0) return rc0;
+    const size_t m = RH_NOALLOC_MAX_RAYS;
+    uint32_t *const keys[2] = {base, base + m}, *const vals[2] = {base + 2 * m, base + 3 * m};
+    uint32_t *hist = base + 4 * m;
+    PassConst pc;
+    memset(&pc, 0, sizeof(pc));
+    memcpy(pc.root_min, root_min, sizeof(pc.root_min));
+    memcpy(pc.cell_size, cell_size, sizeof(pc.cell_size));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const dim3 grid((unsigned)((n + kBlock - 1) / kBlock)), blk(kBlock);
+    hipLaunchKernelGGL(k_ray_keys, grid, blk, 0, s, d_rays, nullptr, (int)n, pc, keys[0], vals[0]);
+    RH_CHECK(hipGetLastError());
+    int sorted = 0;
+    const int rc = radix_passes(s, keys, vals, hist, nullptr, (int)n, 32, &sorted); // the full key, as SortRays_CPU
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_gather, grid, blk, 0, s, d_rays, d_aux, vals[sorted], nullptr, (int)n, d_rays_out, d_aux_out);
+    RH_CHECK(hipGetLastError());
+    return 0;
 }
 
 int rh_shade_rays(rh_context *c, const rh_pass_info *pi, const rh_ray_packet *rays, const rh_hit *hits, size_t n,

@@ -43,6 +43,7 @@ class DeviceLib:
             "rh_trace_rays": (c_int, [P, P, c_size_t, c_int, P, P]),
             "rh_ray_hashes": (c_int, [P, P, c_size_t, P, P, P]),
             "rh_sort_rays": (c_int, [P, P, c_size_t, P, P]),
+            "rh_sort_rays_noalloc": (c_int, [P, P, c_size_t, P, P, P, P, P]),
             "rh_shade_rays": (c_int, [P, P, P, P, c_size_t, P, P, P, POINTER(c_int), P]),
             "rh_get_kernel_profile": (c_int, [P, POINTER(KernelProfile)]),
             "rh_reset_kernel_profile": (c_int, [P]),
