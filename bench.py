@@ -21,10 +21,14 @@ Also reported on the same JSON line:
                 the same rays, tests/test_gpu_trace.py) over its HIP-event time
                 inside the timed region, against 8 TB/s; traffic = HBM bytes
                 per launch from the copy-calibrated FETCH_SIZE/WRITE_SIZE
-                counters committed in profiles/trace_traffic.json
-  cpu_baseline  the reference Ray library's Ref backend on the host cores,
-                128x128 buckets on a thread pool, over a bounded sample of the
-                same workload
+                counters committed in profiles/trace_traffic.json;
+                per_kernel adds the shadow kernel (per queue entry) and the
+                path tail (per shaded segment and light sample) from the same
+                replay's counters
+  cpu_baseline  the reference Ray library's Ref backend (and its AVX2 backend
+                beside it) on the host cores, 128x128 buckets on a thread pool,
+                over a bounded sample of the same workload; the CPU model and
+                core counts are recorded
   modes         Mrays/s and per-kernel ms of each timed mode
 
 Usage: python bench.py [--gpus 1] [--steps 64] [--warmup 2]
@@ -50,6 +54,24 @@ HBM_PEAK_GBS = 8000.0
 # per interior node visit of an 8-wide tree 224 (mbvh_node_t: 8 boxes + 8 children)
 B_RAY, B_HIT, B_NODE, B_INST, B_TRI = 32, 32, 32, 112, 52
 B_WNODE = 224
+# per shadow-queue entry: queue index 4 + shadow ray record 24 (nee_ray 16, nee_aux 8) + visibility byte 1
+B_SHADOW_ENTRY = 29
+# per shaded path segment, the reference's shading bytes (ray 112 + hit 32 + triangle 48 + vertex indices 12 +
+# 3 vertices 156 + material 48 + instance and transform 96 + 2 textures 148 + 2 bilinear footprints 32 +
+# secondary ray 112 + framebuffer read-modify-write 32); per deferred light sample its record 40
+B_SHADE, B_NEE_RECORD = 828, 40
+
+
+def traversal_bytes(v: dict, wide: bool) -> float:
+    """Bytes of the reference structures a traversal reads, from its visit counts."""
+    return v["nodes"] * (B_WNODE if wide else B_NODE) + v["instances"] * B_INST + v["tris"] * B_TRI
+
+
+def kernel_roofline(name: str, alg_bytes: float, ms: float, launches: int, traffic=None) -> dict:
+    achieved = alg_bytes / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+    return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "launches": launches,
+            "avg_launch_ms": round(ms / max(launches, 1), 4), "alg_bytes_per_launch": int(alg_bytes / max(launches, 1))}
 
 
 def log(*a):
@@ -141,13 +163,35 @@ def run_gpu(args, lib, A, scenes, wide, staged_modes=True, tuned=None):
     vis = r.kernel_profile()
     rc2 = r.ray_counts()
     r.set_option("count_visits", 0)
-    traced = vis["counted_rays"]  # rays of the counting k_trace launches (the path tail is not counted)
+    traced = vis["counted_rays"]  # rays of the counting k_trace launches (the tail and shadow kernels count apart)
     r.close()
     return spec, results, vis, traced, tuned
 
 
-def run_cpu_baseline(args, A, scenes, spp_sample):
-    """Reference Ref backend (or the CPU oracle) over spp_sample full-frame passes."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads() -> dict:
+    """Threads the CPU baseline uses: every core this process may run on, up
+    to 16 (the host share of one GPU on the machines this runs on)."""
+    nproc = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except Exception:
+        allowed = nproc
+    return {"nproc": nproc, "affinity": allowed, "threads": max(1, min(allowed, 16))}
+
+
+def run_cpu_baseline(args, A, scenes, spp_sample, flags):
+    """A reference CPU backend (flags: RENDERER_REF or RENDERER_AVX2) over
+    spp_sample full-frame passes."""
     ref_path = ROOT / "oracle" / "_ref" / "libray_ref.so"
     kind = "reference"
     if not ref_path.exists():
@@ -156,14 +200,12 @@ def run_cpu_baseline(args, A, scenes, spp_sample):
     W, H = args.width, args.height
     # the reference's default settings (settings_t::use_wide_bvh = true, RendererBase.h:29):
     # its 8-wide BVH is the faster of its two trees on the host
-    r = A.Renderer(lib, W, H, A.RENDERER_REF, use_wide_bvh=True)
+    r = A.Renderer(lib, W, H, flags, use_wide_bvh=True)
+    backend = {A.RENDERER_REF: "Ref", A.RENDERER_SSE2: "SSE2", A.RENDERER_AVX: "AVX",
+               A.RENDERER_AVX2: "AVX2"}.get(r.type(), str(r.type()))
     s = r.create_scene()
     scenes.triangle_soup(s, width=W, height=H, depth=args.depth)
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except Exception:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    cores = host_threads()["threads"]
     regions = [A.Region(lib, x, y, min(128, W - x), min(128, H - y)) for y in range(0, H, 128) for x in range(0, W, 128)]
 
     def one_pass():
@@ -189,14 +231,22 @@ def run_cpu_baseline(args, A, scenes, spp_sample):
     for _ in range(spp_sample):
         one_pass()
     dt = time.perf_counter() - t0
-    return dict(kind=kind, seconds=dt, cores=cores, spp=spp_sample)
+    return dict(kind=kind, seconds=dt, cores=cores, spp=spp_sample, backend=backend)
+
+
+def device_used_bytes() -> int:
+    import torch
+    free, total = torch.cuda.mem_get_info()
+    return int(total - free)
 
 
 def run_adjoint(args, lib, A, scenes, knobs, wide):
     """Refined Cornell box (9,828 triangles, one material per triangle), 512x512,
     32 spp, depth 6: Adam on the 9,826 albedos against a target rendered from a
     hidden albedo set (seed 0x5EED0005).  One iteration = forward (32 passes) +
-    adjoint replay (32 passes) + Adam update."""
+    adjoint by path replay (32 passes) + Adam update: the headline, with memory
+    O(pixels).  The same with the tape (the forward keeps every pass's vertex
+    records, memory O(pixels x passes x depth), no replay) is timed beside it."""
     import torch
     from raydemo_amd import adjoint
 
@@ -225,15 +275,28 @@ def run_adjoint(args, lib, A, scenes, knobs, wide):
     fwd_rays = rc["primary"] + rc["secondary"] + rc["shadow"]
     adj_rays = rc["adjoint"] + rc["adjoint_shadow"]
     res = adjoint.descend(dr, init, target, hidden, steps=args.adj_steps, lr=0.02, optimize=slice(0, n))
+    used_replay = device_used_bytes()
+    # the tape beside it: same workload, the forward keeps its vertex records
+    dr.tape = True
+    dr.r.set_option("adjoint_tape", dr.spp)
+    tape_steps = max(1, args.adj_steps // 4)
+    res_tape = adjoint.descend(dr, init, target, hidden, steps=tape_steps, lr=0.02, optimize=slice(0, n))
+    used_tape = device_used_bytes()
+    tape_bytes = dr.tape_bytes()
+    dr.tape = False
+    dr.r.set_option("adjoint_tape", 0)
     out = {
-        "metric": "adjoint s/iter (forward + adjoint from the taped vertex records, or a path replay, + Adam)",
+        "metric": "adjoint s/iter (forward + path-replay adjoint + Adam)",
         "s_per_iter": round(res.s_per_iter, 5), "steps": res.steps, "seconds": round(res.seconds, 3),
         "forward_ms": round((t1 - t0) * 1e3, 3), "adjoint_ms": round((t2 - t1) * 1e3, 3),
         "forward_mrays": round(fwd_rays / (t1 - t0) / 1e6, 2), "adjoint_mrays": round(adj_rays / (t2 - t1) / 1e6, 2),
         "loss_first": res.loss_first, "loss_last": res.loss_last,
         "albedo_rmse_first": round(res.albedo_rmse_first, 5), "albedo_rmse_last": round(res.albedo_rmse_last, 5),
         "knobs": knobs,
-        "tape": dr.tape,
+        "tape": False,
+        "replay_record_bytes": dr.replay_bytes(), "device_used_bytes_replay": used_replay,
+        "tape_s_per_iter": round(res_tape.s_per_iter, 5), "tape_steps": tape_steps, "tape_record_bytes": tape_bytes,
+        "device_used_bytes_tape": used_tape,
         "config": {"workload": f"cornell_refined {spec.triangles} tris, {n} albedo parameters x 3, "
                                f"{args.adj_size}x{args.adj_size}, {args.adj_spp} spp, depth 6, roulette off",
                    "trees": "8-wide" if wide else "binary",
@@ -437,27 +500,43 @@ def run_large(args, lib, A, scenes, knobs, wide):
         noise = np.random.default_rng(scenes.SEED_BASE + 55).normal(0.0, 0.1, orig.shape).astype(np.float32)
         target_tex = torch.tensor(np.clip(orig + noise, 0.0, 1.0), device="cuda")
         target = adjoint.render_textures(target_tex, dr, ids).detach().clone()
-        params = torch.tensor(orig, device="cuda").requires_grad_(True)
-        opt = torch.optim.Adam([params], lr=0.01)
-        losses = []
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        for _ in range(args.large_adj_steps):
-            opt.zero_grad(set_to_none=True)
-            img = adjoint.render_textures(params, dr, ids)
-            loss = torch.mean((img - target) ** 2)
-            loss.backward()
-            opt.step()
-            with torch.no_grad():
-                params.clamp_(0.0, 1.0)
-            losses.append(float(loss.detach()))
-        torch.cuda.synchronize()
-        dt2 = time.perf_counter() - t2
+        def descent(steps):
+            params = torch.tensor(orig, device="cuda").requires_grad_(True)
+            opt = torch.optim.Adam([params], lr=0.01)
+            losses = []
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            for _ in range(steps):
+                opt.zero_grad(set_to_none=True)
+                img = adjoint.render_textures(params, dr, ids)
+                loss = torch.mean((img - target) ** 2)
+                loss.backward()
+                opt.step()
+                with torch.no_grad():
+                    params.clamp_(0.0, 1.0)
+                losses.append(float(loss.detach()))
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t2) / steps, losses, int(params.numel())
+
+        s_iter, losses, n_params = descent(args.large_adj_steps)
+        used_replay = device_used_bytes()
+        # the tape beside it (the forward keeps every pass's vertex and texel records)
+        dr.tape = True
+        dr.r.set_option("adjoint_tape", dr.spp)
+        dr.r.set_option("adjoint_tape_texels", 1)
+        tape_steps = max(1, args.large_adj_steps // 2)
+        s_tape, _, _ = descent(tape_steps)
+        used_tape = device_used_bytes()
+        tape_bytes = dr.tape_bytes()
+        dr.tape = False
+        dr.r.set_option("adjoint_tape", 0)
         out["texel_adjoint"] = {
-            "metric": "texel-gradient descent s/iter (forward + adjoint from the taped vertex records, or a path replay, + fold + Adam)",
-            "s_per_iter": round(dt2 / args.large_adj_steps, 4), "steps": args.large_adj_steps,
-            "parameters": int(params.numel()), "loss_first": losses[0], "loss_last": losses[-1],
-            "textures": list(ids)}
+            "metric": "texel-gradient descent s/iter (forward + path-replay adjoint + fold + Adam)",
+            "s_per_iter": round(s_iter, 4), "steps": args.large_adj_steps,
+            "parameters": n_params, "loss_first": losses[0], "loss_last": losses[-1],
+            "textures": list(ids), "tape": False, "replay_record_bytes": dr.replay_bytes(),
+            "device_used_bytes_replay": used_replay, "tape_s_per_iter": round(s_tape, 4), "tape_steps": tape_steps,
+            "tape_record_bytes": tape_bytes, "device_used_bytes_tape": used_tape}
         log(f"large texel adjoint: {out['texel_adjoint']}")
         # back to the generator's textures for the batch leg
         adjoint.render_textures(torch.tensor(orig, device="cuda"), dr, ids)
@@ -690,27 +769,50 @@ def main():
         "avg_launch_ms": round(trace_ms / max(prof["trace_launches"], 1), 4),
         "alg_bytes_per_launch": int(bytes_alg / max(prof["trace_launches"], 1)),
     }
+    # the other traversal kernels from the same replay's counters: the shadow
+    # kernel per queue entry, the path tail per shaded segment and light sample
+    sh, tl = vis["shadow"], vis["tail"]
+    per_kernel = [dict(roofline)]
+    if prof["shadow_launches"]:
+        b_sh = sh["entries"] * B_SHADOW_ENTRY + traversal_bytes(sh, wide)
+        per_kernel.append(kernel_roofline("k_shadow", b_sh, prof["shadow_ms"], prof["shadow_launches"]))
+    if prof["tail_launches"]:
+        b_tl = tl["segments"] * B_SHADE + tl["shadow_segments"] * B_NEE_RECORD + traversal_bytes(tl, wide)
+        per_kernel.append(kernel_roofline("k_path_tail", b_tl, prof["tail_ms"], prof["tail_launches"]))
+    roofline["per_kernel"] = per_kernel
 
     cpu = None
     if not args.no_cpu:
-        cpu_run = run_cpu_baseline(args, A, scenes, args.cpu_spp)
-        if cpu_run is not None:
-            # rays of the identical sample, counted on the device (same paths, same seeds)
+        runs = {}
+        for flags in (A.RENDERER_REF, A.RENDERER_AVX2):
+            run = run_cpu_baseline(args, A, scenes, args.cpu_spp, flags)
+            if run is not None:
+                runs[run["backend"]] = run
+        if runs:
+            # rays of the identical sample, counted on the device (the Ref backend's paths and seeds;
+            # the AVX2 backend traces the same rays up to its packets' arithmetic)
             r = A.Renderer(lib, args.width, args.height, A.RENDERER_HIP, use_wide_bvh=True)
             s = r.create_scene()
             scenes.triangle_soup(s, width=args.width, height=args.height, depth=args.depth)
             reg = A.Region(lib, 0, 0, args.width, args.height)
             r.reset_stats()
-            for _ in range(cpu_run["spp"]):
+            for _ in range(args.cpu_spp):
                 r.render(s, reg)
             r.synchronize()
             rc = r.ray_counts()
             rays = rc["primary"] + rc["secondary"] + rc["shadow"]
-            cpu = {"value": round(rays / cpu_run["seconds"] / 1e6, 3), "unit": "Mrays/s", "cores": cpu_run["cores"],
-                   "kind": cpu_run["kind"],
-                   "sample": f"{cpu_run['spp']} full-frame 1920x1080 passes (spp 1..{cpu_run['spp']}) of the same scene, "
-                             f"128x128 buckets on {cpu_run['cores']} threads, 8-wide BVH (the reference default), "
-                             f"{cpu_run['seconds']:.1f}s"}
+            r.close()
+            per = {name: {"value": round(rays / run["seconds"] / 1e6, 3), "seconds": round(run["seconds"], 2)}
+                   for name, run in runs.items()}
+            head = runs.get("Ref") or next(iter(runs.values()))
+            th = host_threads()
+            cpu = {"value": per[head["backend"]]["value"], "unit": "Mrays/s", "cores": head["cores"],
+                   "kind": head["kind"], "backend": head["backend"], "backends": per,
+                   "cpu_model": cpu_model(), "nproc": th["nproc"], "affinity": th["affinity"],
+                   "sample": f"{args.cpu_spp} full-frame 1920x1080 passes (spp 1..{args.cpu_spp}) of the same scene, "
+                             f"128x128 buckets on {head['cores']} threads (every allowed core, at most 16: one GPU's "
+                             f"host share), 8-wide BVH (the reference default); the reference's Ref (scalar, the "
+                             f"parity target) and AVX2 (4x2 ray packets) backends"}
             log(f"cpu baseline: {cpu}")
 
     plain_knobs = {k: v for k, v in tuned.best.items() if k in autotune.KNOBS}
@@ -745,6 +847,7 @@ def main():
                          "kernel_ms": {k: round(v, 3) for k, v in m["prof"].items() if k.endswith("_ms")}}
                   for name, m in res.items()},
         "rays_per_pass": int(best["rays"] / args.steps), "ray_counts": best["counts"],
+        "overflow": best["counts"]["overflow"],
         "autotune": {"seconds": round(tuned.seconds, 2), "passes_per_candidate": args.tune_passes,
                      "columns": list(autotune.KNOBS) + ["mrays"],
                      "table": [[t[k] for k in autotune.KNOBS] + [t["mrays"]] for t in tuned.table]},

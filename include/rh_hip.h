@@ -74,6 +74,7 @@ int rh_get_ray_counts(rh_context *ctx, rh_ray_counts *out);
  * "sort_bits" (8/16/24/32: the in-pass ray sort orders by the top bits of the key),
  * "path_tail" (-1: every bounce staged; b >= 0: bounces b.. run as one path-per-lane launch),
  * "tail_occupancy" (2 or 3 waves per SIMD for the path-tail kernel: registers vs spills),
+ * "tail_refill" (0/1: the path tail as a persistent kernel whose lanes take the next path when theirs ends),
  * "refill" (0/1: closest-hit trace as a persistent kernel whose lanes take new rays as theirs finish;
  *   binary trees only),
  * "trace_occupancy" (0, 5 or 6: the 8-wide trace and shadow kernels as compiled, or capped at 5 or 6 waves

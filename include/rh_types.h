@@ -323,6 +323,11 @@ typedef struct rh_kernel_profile {
     unsigned long long launches[RH_KERNEL_KINDS];
     unsigned long long node_visits, tri_tests, instance_tests;
     unsigned long long counted_rays; /* rays traced by the counting trace kernels */
+    /* the counting shadow kernels: node visits, triangle tests, instance tests, queue entries, segments */
+    unsigned long long shadow_visits[5];
+    /* the counting path-tail kernels: node visits, triangle tests, instance tests (path and shadow
+     * traversals together), path segments, shadow segments */
+    unsigned long long tail_visits[5];
 } rh_kernel_profile;
 
 #ifdef __cplusplus

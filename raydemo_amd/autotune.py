@@ -6,7 +6,9 @@ global slice), whether bounce rays are sorted by their origin/direction hash
 before tracing, how many top bits of the 32-bit key the sort orders by, the
 bounce from which the remaining bounces run as one path-per-lane launch
 instead of one trace/shade/shadow stage each, the register budget (waves per
-SIMD) of the path-tail and 8-wide trace kernels, and the lane-refill trace.
+SIMD) of the path-tail and 8-wide trace kernels, the lane-refill trace, and
+the path tail as a persistent kernel whose lanes take a new path when theirs
+ends.
 The image is bit-identical for every setting (tests/test_gpu_sort.py), so the
 tuner may pick whichever is fastest on the scene at hand.
 
@@ -34,7 +36,7 @@ REFILL = (0, 1)
 TRACE_OCCUPANCY = (0, 5, 6)
 SHADOW_OCCUPANCY = (-1, 0, 5)
 KNOBS = ("lds_stack_depth", "sort_rays", "sort_bits", "path_tail", "tail_occupancy", "refill", "trace_occupancy",
-         "shadow_occupancy")
+         "shadow_occupancy", "tail_refill")
 
 
 @dataclass
@@ -52,7 +54,7 @@ def candidates(depths=LDS_DEPTHS, sort_modes=SORT_MODES, tails=PATH_TAILS, occup
         if pt < 0 and occ != occupancies[0]:
             continue  # no tail launch: occupancy is moot
         yield {"lds_stack_depth": d, "sort_rays": so, "sort_bits": sb, "path_tail": pt, "tail_occupancy": occ,
-               "refill": 0, "trace_occupancy": 0, "shadow_occupancy": -1}
+               "refill": 0, "trace_occupancy": 0, "shadow_occupancy": -1, "tail_refill": 0}
 
 
 def apply(renderer, cfg: dict):
@@ -95,8 +97,15 @@ def tune(renderer, scene, region_factory, passes: int = 6, depths=LDS_DEPTHS, so
                 return TuneResult(best=hit["best"], table=hit.get("table", []), seconds=0.0)
     t0 = time.perf_counter()
     table = [measure(renderer, scene, region_factory, cfg, passes) for cfg in candidates(depths, sort_modes, tails)]
-    # the refill trace and the capped-register trace only as second stages, on
-    # the best settings so far (both leave the image unchanged)
+    # the refill trace, the refilled path tail and the capped-register trace
+    # only as second stages, on the best settings so far (all leave the image
+    # unchanged)
+    first = max(table, key=lambda r: r["mrays"])
+    for pt in sorted({p for p in (first["path_tail"], 0, 1, 2) if p >= 0}):
+        for occ in TAIL_OCCUPANCY:
+            cfg = {k: first[k] for k in KNOBS}
+            cfg.update(path_tail=pt, tail_occupancy=occ, tail_refill=1)
+            table.append(measure(renderer, scene, region_factory, cfg, passes))
     first = max(table, key=lambda r: r["mrays"])
     for rf in REFILL[1:]:
         cfg = {k: first[k] for k in KNOBS}

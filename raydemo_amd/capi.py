@@ -80,7 +80,8 @@ class RayCounts(Structure):
 
 class KernelProfile(Structure):
     _fields_ = [("ms", C.c_double * 5), ("launches", c_uint64 * 5), ("node_visits", c_uint64),
-                ("tri_tests", c_uint64), ("instance_tests", c_uint64), ("counted_rays", c_uint64)]
+                ("tri_tests", c_uint64), ("instance_tests", c_uint64), ("counted_rays", c_uint64),
+                ("shadow_visits", c_uint64 * 5), ("tail_visits", c_uint64 * 5)]
 
 
 class PassSettings(Structure):
@@ -342,6 +343,9 @@ class Renderer:
         d.update({f"{k}_launches": p.launches[i] for i, k in enumerate(kinds)})
         d.update(node_visits=p.node_visits, tri_tests=p.tri_tests, instance_tests=p.instance_tests,
                  counted_rays=p.counted_rays)
+        names = ("nodes", "tris", "instances")
+        d["shadow"] = dict(zip(names + ("entries", "segments"), list(p.shadow_visits)))
+        d["tail"] = dict(zip(names + ("segments", "shadow_segments"), list(p.tail_visits)))
         return d
 
     def reset_kernel_profile(self):

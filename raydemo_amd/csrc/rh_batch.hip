@@ -268,6 +268,7 @@ template <int Tree> __global__ void __launch_bounds__(kBlock) k_b_shadow(BatchDe
                       a.lds_depth = b.lds_depth;
                       a.shadow_counter = b.stats + 2;
                       a.overflow = b.stats + 4;
+                      a.visits = nullptr;
                       shadow_body<Tree>(a, base, lds_stack);
                   });
 }

@@ -286,14 +286,17 @@ template <bool Defer> __global__ void __launch_bounds__(kBlock) k_shade(ShadeArg
     shade_body<Defer>(a, blockIdx.x * blockDim.x, lds_stack, s_app);
 }
 
-// the bounces first..last-1 of each queued ray in one lane (path_tail_body).
+// the bounces first..last-1 of each queued ray in one lane (path_tail_body);
+// with next_path a persistent grid whose lanes refill from the queue.
 // MinBlocks 3 asks the compiler for 3 resident blocks (waves per SIMD) per CU,
 // trading register spills for occupancy; 2 gives it 256 registers (2 waves per
 // SIMD).  Which is faster is measured (tail_occupancy).
-template <bool Defer, int MinBlocks, int Tree>
-__global__ void __launch_bounds__(kBlock, MinBlocks) k_path_tail(ShadeArgs a, int first, int last) {
+// Count: the instrumented form (traversal visits into a.visits), for the
+// roofline's algorithmic bytes.
+template <bool Defer, int MinBlocks, int Tree, bool Count = false>
+__global__ void __launch_bounds__(kBlock, MinBlocks) k_path_tail(ShadeArgs a, int first, int last, int *next_path) {
     extern __shared__ uint32_t lds_stack[];
-    path_tail_body<Defer, Tree>(a, first, last, blockIdx.x * blockDim.x, lds_stack);
+    path_tail_body<Defer, Tree, Count>(a, first, last, blockIdx.x * blockDim.x, lds_stack, next_path);
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -625,9 +628,10 @@ __global__ void k_set_material_colors(rh_material *mats, const float *__restrict
     m.main_color[2] = rgb[3 * k + 2];
 }
 
-template <int Tree, int MinBlocks = 1> __global__ void __launch_bounds__(kBlock, MinBlocks) k_shadow(ShadowArgs a) {
+template <int Tree, int MinBlocks = 1, bool Count = false>
+__global__ void __launch_bounds__(kBlock, MinBlocks) k_shadow(ShadowArgs a) {
     extern __shared__ uint32_t lds_stack[];
-    shadow_body<Tree>(a, blockIdx.x * blockDim.x, lds_stack);
+    shadow_body<Tree, Count>(a, blockIdx.x * blockDim.x, lds_stack);
 }
 
 __global__ void __launch_bounds__(kBlock) k_nee_combine(CombineArgs a) { combine_body(a, blockIdx.x * blockDim.x); }
@@ -733,7 +737,10 @@ struct rh_context {
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<std::pair<int, std::pair<size_t, size_t>>> ev_marks; // kind, (start, stop)
-    unsigned long long *visits = nullptr;                             // [nodes, tris, instances, rays]
+    // Count-mode traversal work: [0..3] trace (nodes, tris, instances, rays),
+    // [4..8] shadow (nodes, tris, instances, entries, segments), [9..13] path
+    // tail (nodes, tris, instances, segments, shadow segments)
+    unsigned long long *visits = nullptr;
 
     // options
     int kernel_timing = 0;
@@ -743,6 +750,8 @@ struct rh_context {
     int path_tail = 2;  // first bounce run by the path-tail kernel (-1: none)
     int tail_occupancy = 2;    // waves per SIMD the path-tail kernel is compiled for (2 or 3)
     int refill = 0;            // closest-hit trace: persistent kernel with lane refill
+    int tail_refill = 0;       // path tail: persistent kernel whose lanes take a new path when theirs ends
+    std::vector<std::pair<std::pair<const void *, size_t>, int>> resident; // (kernel, LDS bytes) -> blocks per CU
     int trace_occupancy = 0;   // 8-wide trace/shadow kernels: 0 = as compiled (4 waves/SIMD), 5 or 6 = capped registers
     int shadow_occupancy = -1; // the shadow kernel's own setting (-1: as trace_occupancy)
     int max_bounces = RH_MAX_BOUNCES; // max_total_depth clamp (the reference's 10; up to RH_MAX_BOUNCES_EXT)
@@ -953,6 +962,22 @@ struct KernelMark {
     }
 };
 
+// Blocks of `kern` (kBlock threads, `lds` bytes of dynamic LDS) resident on
+// one CU at once, from the occupancy calculator, remembered per (kernel, LDS).
+int resident_blocks(rh_context *c, const void *kern, size_t lds, int *out) {
+    for (const auto &r : c->resident)
+        if (r.first.first == kern && r.first.second == lds) {
+            *out = r.second;
+            return 0;
+        }
+    int per_cu = 0;
+    RH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, lds));
+    per_cu = std::max(1, per_cu);
+    c->resident.push_back({{kern, lds}, per_cu});
+    *out = per_cu;
+    return 0;
+}
+
 int launch_trace(rh_context *c, const TraceArgs &a, int grid_rays, bool any_hit) {
     const int blocks = std::max(1, (grid_rays + kBlock - 1) / kBlock);
     const bool count = c->count_visits && a.visits;
@@ -1068,12 +1093,18 @@ int launch_shade(rh_context *c, const ShadeArgs &a, int grid_rays) {
     s.lds_depth = (uint32_t)c->lds_depth;
     s.shadow_counter = a.shadow_counter;
     s.overflow = c->stats_dev + 4;
+    const bool count = c->count_visits && c->visits;
+    s.visits = count ? c->visits + 4 : nullptr;
     const size_t lanes = (size_t)grid_rays * a.nee_slots;
     {
         KernelMark km(c, 3);
         const dim3 grid((unsigned)std::max<size_t>(1, (lanes + kBlock - 1) / kBlock)), blk(kBlock);
         const int occ = c->shadow_occupancy >= 0 ? c->shadow_occupancy : c->trace_occupancy;
-        if (c->sc.wnodes && occ == 5)
+        if (count && c->sc.wnodes)
+            hipLaunchKernelGGL((k_shadow<kTreeWide, 1, true>), grid, blk, lds_bytes(c), c->stream, s);
+        else if (count)
+            hipLaunchKernelGGL((k_shadow<kTreeBinary, 1, true>), grid, blk, lds_bytes(c), c->stream, s);
+        else if (c->sc.wnodes && occ == 5)
             hipLaunchKernelGGL((k_shadow<kTreeWide, 5>), grid, blk, lds_bytes(c), c->stream, s);
         else if (c->sc.wnodes && occ == 6)
             hipLaunchKernelGGL((k_shadow<kTreeWide, 6>), grid, blk, lds_bytes(c), c->stream, s);
@@ -1129,6 +1160,8 @@ int radix_sort(rh_context *c, const int *count, int max_n, int bits, int *result
 }
 
 PassConst make_pass_const(rh_context *c, const rh_pass_desc *p) { return make_pass_const(c->w, c->h, p); }
+
+constexpr int kVisitWords = 16; // the Count-mode counters of rh_context::visits
 
 // stats_t timers a pass's intervals add to
 enum StageTimerIndex : int {
@@ -1263,8 +1296,8 @@ int rh_create(int device, rh_context **out) {
     RH_CHECK(hipMalloc(&c->stats_dev, 8 * sizeof(unsigned long long)));
     RH_CHECK(hipMemsetAsync(c->counters, 0, 16 * sizeof(int), c->stream));
     RH_CHECK(hipMemsetAsync(c->stats_dev, 0, 8 * sizeof(unsigned long long), c->stream));
-    RH_CHECK(hipMalloc(&c->visits, 4 * sizeof(unsigned long long)));
-    RH_CHECK(hipMemsetAsync(c->visits, 0, 4 * sizeof(unsigned long long), c->stream));
+    RH_CHECK(hipMalloc(&c->visits, kVisitWords * sizeof(unsigned long long)));
+    RH_CHECK(hipMemsetAsync(c->visits, 0, kVisitWords * sizeof(unsigned long long), c->stream));
     RH_CHECK(hipMalloc(&c->halton, RH_HALTON_TABLE_FLOATS * sizeof(float)));
     RH_CHECK(hipMemsetAsync(c->halton, 0, RH_HALTON_TABLE_FLOATS * sizeof(float), c->stream));
     *out = c;
@@ -1524,31 +1557,40 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool k
             tb.ray_counter = c->stats_dev + 1;
             KernelMark km(c, 4);
             // with deferred NEE on, the lane resolves its light samples itself
-            // after shading (the slots of ray i are free during the tail)
-            const dim3 grid((n + kBlock - 1) / kBlock), blk(kBlock);
+            // after shading (the slots of a path are free during the tail)
             const int last = pi.settings.max_total_depth;
             const size_t lds = lds_bytes(c);
             const bool occ3 = c->tail_occupancy == 3;
-            if (c->sc.wnodes) {
-                if (tb.nee_ray && occ3)
-                    hipLaunchKernelGGL((k_path_tail<true, 3, kTreeWide>), grid, blk, lds, c->stream, tb, bounce, last);
-                else if (tb.nee_ray)
-                    hipLaunchKernelGGL((k_path_tail<true, 2, kTreeWide>), grid, blk, lds, c->stream, tb, bounce, last);
-                else if (occ3)
-                    hipLaunchKernelGGL((k_path_tail<false, 3, kTreeWide>), grid, blk, lds, c->stream, tb, bounce, last);
+            const void *kern;
+            if (c->sc.wnodes)
+                kern = tb.nee_ray ? (occ3 ? (const void *)k_path_tail<true, 3, kTreeWide> : (const void *)k_path_tail<true, 2, kTreeWide>)
+                                  : (occ3 ? (const void *)k_path_tail<false, 3, kTreeWide> : (const void *)k_path_tail<false, 2, kTreeWide>);
+            else
+                kern = tb.nee_ray ? (occ3 ? (const void *)k_path_tail<true, 3, kTreeBinary> : (const void *)k_path_tail<true, 2, kTreeBinary>)
+                                  : (occ3 ? (const void *)k_path_tail<false, 3, kTreeBinary> : (const void *)k_path_tail<false, 2, kTreeBinary>);
+            if (c->count_visits && c->visits) {
+                // the instrumented tail (2 waves per SIMD: counts do not depend on the knob)
+                tb.visits = c->visits + 9;
+                if (c->sc.wnodes)
+                    kern = tb.nee_ray ? (const void *)k_path_tail<true, 2, kTreeWide, true>
+                                      : (const void *)k_path_tail<false, 2, kTreeWide, true>;
                 else
-                    hipLaunchKernelGGL((k_path_tail<false, 2, kTreeWide>), grid, blk, lds, c->stream, tb, bounce, last);
-            } else {
-                if (tb.nee_ray && occ3)
-                    hipLaunchKernelGGL((k_path_tail<true, 3, kTreeBinary>), grid, blk, lds, c->stream, tb, bounce, last);
-                else if (tb.nee_ray)
-                    hipLaunchKernelGGL((k_path_tail<true, 2, kTreeBinary>), grid, blk, lds, c->stream, tb, bounce, last);
-                else if (occ3)
-                    hipLaunchKernelGGL((k_path_tail<false, 3, kTreeBinary>), grid, blk, lds, c->stream, tb, bounce, last);
-                else
-                    hipLaunchKernelGGL((k_path_tail<false, 2, kTreeBinary>), grid, blk, lds, c->stream, tb, bounce, last);
+                    kern = tb.nee_ray ? (const void *)k_path_tail<true, 2, kTreeBinary, true>
+                                      : (const void *)k_path_tail<false, 2, kTreeBinary, true>;
             }
-            RH_CHECK(hipGetLastError());
+            int blocks = (n + kBlock - 1) / kBlock;
+            int *next_path = nullptr;
+            if (c->tail_refill) {
+                // persistent grid: the blocks that are resident at once, at most the queue's
+                int per_cu = 0;
+                rc = resident_blocks(c, kern, lds, &per_cu);
+                if (rc) return rc;
+                blocks = std::max(1, std::min(blocks, per_cu * c->cus));
+                next_path = c->counters + 3;
+                hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, next_path, 0);
+            }
+            void *args[] = {&tb, &bounce, (void *)&last, &next_path};
+            RH_CHECK(hipLaunchKernel(kern, dim3(blocks), dim3(kBlock), args, lds, c->stream));
             // the path tail traces and shades in one kernel: its time counts as secondary tracing
             tm.mark(kSecondaryTrace);
             break;
@@ -1744,6 +1786,8 @@ int rh_set_option(rh_context *c, const char *name, int value) {
         c->tail_occupancy = value;
     } else if (n == "refill") {
         c->refill = value != 0;
+    } else if (n == "tail_refill") {
+        c->tail_refill = value != 0;
     } else if (n == "shadow_occupancy") {
         if (value != -1 && value != 0 && value != 5 && value != 6)
             return arg_error("shadow_occupancy must be -1, 0, 5 or 6");
@@ -1782,6 +1826,7 @@ int rh_get_option(rh_context *c, const char *name, int *value) {
     else if (n == "path_tail") *value = c->path_tail;
     else if (n == "tail_occupancy") *value = c->tail_occupancy;
     else if (n == "refill") *value = c->refill;
+    else if (n == "tail_refill") *value = c->tail_refill;
     else if (n == "trace_occupancy") *value = c->trace_occupancy;
     else if (n == "max_bounces") *value = c->max_bounces;
     else if (n == "adjoint_staged") *value = c->adjoint_staged;
@@ -1814,12 +1859,16 @@ int rh_get_kernel_profile(rh_context *c, rh_kernel_profile *out) {
         out->ms[m.first] += ms;
         out->launches[m.first]++;
     }
-    unsigned long long v[4];
+    unsigned long long v[kVisitWords];
     RH_CHECK(hipMemcpy(v, c->visits, sizeof(v), hipMemcpyDeviceToHost));
     out->node_visits = v[0];
     out->counted_rays = v[3];
     out->tri_tests = v[1];
     out->instance_tests = v[2];
+    for (int k = 0; k < 5; k++) {
+        out->shadow_visits[k] = v[4 + k];
+        out->tail_visits[k] = v[9 + k];
+    }
     return 0;
 }
 
@@ -1827,7 +1876,7 @@ int rh_reset_kernel_profile(rh_context *c) {
     RH_CHECK(hipStreamSynchronize(c->stream));
     c->ev_marks.clear();
     c->ev_used = 0;
-    RH_CHECK(hipMemsetAsync(c->visits, 0, 4 * sizeof(unsigned long long), c->stream));
+    RH_CHECK(hipMemsetAsync(c->visits, 0, kVisitWords * sizeof(unsigned long long), c->stream));
     RH_CHECK(hipStreamSynchronize(c->stream));
     return 0;
 }

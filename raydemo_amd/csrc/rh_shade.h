@@ -20,10 +20,12 @@ struct ShadeCounters {
 };
 
 // ComputeVisibility (CoreRef.cpp:1801-1896): steps through transparent
-// occluders, resolving mix materials with the lane's random offset
+// occluders, resolving mix materials with the lane's random offset; tc (when
+// given) counts the traversal's node visits, triangle and instance tests
 template <int Tree = kTreeAny>
 __device__ __forceinline__ float compute_visibility(const Scene &sc, f3 p1, f3 p2, const float *halton, int hi,
-                                                    int rand_hash2, Stack &st, ShadeCounters &cnt) {
+                                                    int rand_hash2, Stack &st, ShadeCounters &cnt,
+                                                    TraceCounters *tc = nullptr) {
     f3 dir = p2 - p1;
     float dist = length(dir);
     dir = dir / dist;
@@ -39,7 +41,7 @@ __device__ __forceinline__ float compute_visibility(const Scene &sc, f3 p1, f3 p
         sh.u = sh.v = 0.0f;
         cnt.shadow_rays++;
         // all-solid scene: the first blocking triangle found decides (visibility 0)
-        if (sc.macro_root != RH_INVALID) traverse<true, Tree>(sc, ro, dir, sh, st, nullptr, sc.all_solid != 0);
+        if (sc.macro_root != RH_INVALID) traverse<true, Tree>(sc, ro, dir, sh, st, tc, sc.all_solid != 0);
         if (!sh.mask) break;
 
         const rh_tri_accel tri = load_tri(sc.tris, (uint32_t)sh.prim);
@@ -540,7 +542,7 @@ __device__ __forceinline__ void shade_surface(const Scene &sc, const rh_pass_inf
 template <int Tree = kTreeAny>
 __device__ __forceinline__ f3 resolve_light_samples(const Scene &sc, const float *halton, const NeeSink &sink,
                                                     const ShadeOut &out, f3 throughput, Stack &st,
-                                                    ShadeCounters &cnt) {
+                                                    ShadeCounters &cnt, TraceCounters *tc = nullptr) {
     f3 col = f3{0.0f, 0.0f, 0.0f};
     for (uint32_t j = 0; j < sink.count; j++) {
         const uint32_t k = sink.base + j;
@@ -548,7 +550,7 @@ __device__ __forceinline__ f3 resolve_light_samples(const Scene &sc, const float
         const int2 x = sink.aux[k];
         const rh_light &l = sc.lights[__float_as_uint(r.w)];
         st.sp = 0;
-        if (compute_visibility<Tree>(sc, f3{r.x, r.y, r.z}, ld3(l.pos), halton, x.x, x.y, st, cnt) != 0.0f) {
+        if (compute_visibility<Tree>(sc, f3{r.x, r.y, r.z}, ld3(l.pos), halton, x.x, x.y, st, cnt, tc) != 0.0f) {
             const float4 t = sink.term[k];
             col = col + f3{t.x, t.y, t.z};
         }

@@ -489,6 +489,8 @@ struct ShadeArgs {
     unsigned long long *shadow_counter;
     unsigned long long *overflow;
     unsigned long long *ray_counter; // path tail: secondary rays traced
+    unsigned long long *visits;      // path tail, Count mode: node visits, triangle tests, instance tests,
+                                     // segments, shadow rays
     // deferred next-event estimation (null nee_ray: visibility traced inline)
     float4 *nee_ray;
     int2 *nee_aux;
@@ -630,63 +632,111 @@ __device__ __forceinline__ void shade_body(const ShadeArgs &a, uint32_t base, ui
 // terms summed in light order, albedo, throughput: the combine stage's
 // arithmetic), so the shading state is dead while the shadow rays traverse
 // and fewer registers are live at the peak.
-template <bool Defer, int Tree = kTreeAny>
+//
+// Refill (next_path set): the grid is persistent and a lane whose path ends
+// takes the next unstarted path of the queue (a wave fetches for its idle
+// lanes with one atomic once a quarter of it is idle, or all of it), so a
+// wave no longer idles until its longest path ends.  Each path is still
+// followed by one lane in bounce order, so every pixel receives the same
+// values in the same order.  Without refill each lane follows the one path
+// of its index.  A lane's stack spill slice belongs to its grid position; the
+// NEE slots of a path belong to its queue index.
+template <bool Defer, int Tree = kTreeAny, bool Count = false>
 __device__ __forceinline__ void path_tail_body(const ShadeArgs &a, int first, int last, uint32_t base,
-                                               uint32_t *lds_stack) {
+                                               uint32_t *lds_stack, int *next_path = nullptr) {
     const int n = *a.count;
-    const uint32_t i = base + threadIdx.x;
-    if ((int)base >= n) return; // whole block idle
-    const bool active = (int)i < n;
+    const uint32_t gl = base + threadIdx.x;
+    if (first >= last || (!next_path && (int)base >= n)) return; // no bounce left, or the whole block idle
     ShadeCounters cnt;
     cnt.shadow_rays = 0;
     uint32_t traced = 0;
     Stack st;
-    st.init(lds_stack, a.spill + i, a.spill_stride, a.lds_depth, kSpillDepth);
-    if (active) {
-        RayFull ray = load_ray(a.rays, a.aux, i);
+    st.init(lds_stack, a.spill + gl, a.spill_stride, a.lds_depth, kSpillDepth);
+    TraceCounters tc = {0, 0, 0};
+    TraceCounters *tcp = Count ? &tc : nullptr;
+
+    int path = -1; // queue index of the lane's path (-1: none)
+    int b = first; // host bounce of the lane's next segment
+    RayFull ray;
+    rh_pass_info pi = a.pi;
+    size_t pixel = 0;
+    auto begin = [&](int p) {
+        path = p;
+        b = first;
+        ray = load_ray(a.rays, a.aux, (uint32_t)p);
         const int px = (ray.xy >> 16) & 0xffff, py = ray.xy & 0xffff;
-        rh_pass_info pi = a.pi;
         pi.index = py * a.w + px;
         pi.rand_index = pi.index;
         if (pi.settings.flags & RH_USE_COHERENT_SAMPLING) pi.rand_index = c_sampling_pattern[(py % 8) * 8 + (px % 8)];
-        float4 &dst = a.temp[(size_t)py * a.w + px];
-        for (int b = first; b < last; b++) {
-            Hit hit;
-            hit.mask = 0;
-            hit.obj = -1;
-            hit.prim = -1;
-            hit.t = RH_MAX_DIST;
-            hit.u = hit.v = 0.0f;
-            st.sp = 0;
-            if (a.sc.macro_root != RH_INVALID) traverse<false, Tree>(a.sc, ray.o, ray.d, hit, st, nullptr);
-            traced++;
-            pi.bounce = b + 3;
-            ShadeOut so;
-            so.emit = false;
-            NeeSink sink;
-            if (Defer) sink = NeeSink{a.nee_ray, a.nee_aux, a.nee_term, i * a.nee_slots, 0};
-            shade_surface<Tree>(a.sc, pi, hit, ray, a.halton, st, cnt, so, Defer ? &sink : nullptr);
-            float4 v = dst;
-            v.x += so.col.x;
-            v.y += so.col.y;
-            v.z += so.col.z;
-            v.w += 0.0f;
-            dst = v;
-            if (Defer && so.nee_pending && sink.count > 0) {
-                const f3 add = resolve_light_samples<Tree>(a.sc, a.halton, sink, so, ray.c, st, cnt);
-                float4 w = dst;
-                w.x += add.x;
-                w.y += add.y;
-                w.z += add.z;
-                dst = w;
+        pixel = (size_t)py * a.w + px;
+    };
+    bool exhausted = next_path == nullptr; // no more paths to take
+    if (!next_path && (int)gl < n) begin((int)gl);
+
+    for (;;) {
+        if (!exhausted) {
+            const uint64_t idle = __ballot(path < 0), live = __ballot(true);
+            if (idle && (__popcll(idle) >= 16 || idle == live)) {
+                const int want = __popcll(idle), leader = __ffsll((unsigned long long)idle) - 1;
+                int first_path = 0;
+                if ((int)lane_id() == leader) first_path = atomicAdd(next_path, want);
+                first_path = __shfl(first_path, leader);
+                if (first_path + want >= n) exhausted = true;
+                if (path < 0) {
+                    const int p = first_path + lanes_below(idle);
+                    if (p < n) begin(p);
+                }
             }
-            if (!so.emit) break;
-            ray = so.next;
         }
+        if (!__ballot(path >= 0)) {
+            if (exhausted) break;
+            continue;
+        }
+        if (path < 0) continue;
+        Hit hit;
+        hit.mask = 0;
+        hit.obj = -1;
+        hit.prim = -1;
+        hit.t = RH_MAX_DIST;
+        hit.u = hit.v = 0.0f;
+        st.sp = 0;
+        if (a.sc.macro_root != RH_INVALID) traverse<false, Tree>(a.sc, ray.o, ray.d, hit, st, tcp);
+        traced++;
+        pi.bounce = b + 3;
+        ShadeOut so;
+        so.emit = false;
+        NeeSink sink;
+        if (Defer) sink = NeeSink{a.nee_ray, a.nee_aux, a.nee_term, (uint32_t)path * a.nee_slots, 0};
+        shade_surface<Tree>(a.sc, pi, hit, ray, a.halton, st, cnt, so, Defer ? &sink : nullptr);
+        float4 &dst = a.temp[pixel];
+        float4 v = dst;
+        v.x += so.col.x;
+        v.y += so.col.y;
+        v.z += so.col.z;
+        v.w += 0.0f;
+        dst = v;
+        if (Defer && so.nee_pending && sink.count > 0) {
+            const f3 add = resolve_light_samples<Tree>(a.sc, a.halton, sink, so, ray.c, st, cnt, tcp);
+            float4 w = dst;
+            w.x += add.x;
+            w.y += add.y;
+            w.z += add.z;
+            dst = w;
+        }
+        b++;
+        if (!so.emit || b >= last) path = -1;
+        else ray = so.next;
     }
     if (a.ray_counter) wave_add_u64(a.ray_counter, traced);
     if (a.shadow_counter) wave_add_u64(a.shadow_counter, cnt.shadow_rays);
     if (st.overflow && a.overflow) atomicAdd(a.overflow, (unsigned long long)st.overflow);
+    if (Count && a.visits) {
+        wave_add_u64(a.visits + 0, tc.nodes);
+        wave_add_u64(a.visits + 1, tc.leaf_tris);
+        wave_add_u64(a.visits + 2, tc.instances);
+        wave_add_u64(a.visits + 3, traced);
+        wave_add_u64(a.visits + 4, cnt.shadow_rays);
+    }
 }
 
 struct ShadowArgs {
@@ -702,13 +752,14 @@ struct ShadowArgs {
     uint32_t lds_depth;
     unsigned long long *shadow_counter;
     unsigned long long *overflow;
+    unsigned long long *visits; // Count mode: node visits, triangle tests, instance tests, entries, segments
 };
 
 // Visibility of each deferred light sample: ComputeVisibility
 // (CoreRef.cpp:1801-1896), any-hit traversal stepping through transparent
 // surfaces.  One lane per queue entry; the grid covers every slot and lanes
 // past the device count exit at once.
-template <int Tree = kTreeAny>
+template <int Tree = kTreeAny, bool Count = false>
 __device__ __forceinline__ void shadow_body(const ShadowArgs &a, uint32_t base, uint32_t *lds_stack) {
     const int n = *a.count;
     const uint32_t q = base + threadIdx.x;
@@ -716,6 +767,7 @@ __device__ __forceinline__ void shadow_body(const ShadowArgs &a, uint32_t base, 
     if (q == 0 && a.shadow_counter) atomicAdd(a.shadow_counter, (unsigned long long)n);
     ShadeCounters cnt;
     cnt.shadow_rays = 0;
+    TraceCounters tc = {0, 0, 0};
     if ((int)q < n) {
         const uint32_t k = a.queue[q];
         const float4 r = a.nee_ray[k];
@@ -723,7 +775,8 @@ __device__ __forceinline__ void shadow_body(const ShadowArgs &a, uint32_t base, 
         const rh_light &l = a.sc.lights[__float_as_uint(r.w)];
         Stack st;
         st.init(lds_stack, a.spill + q, a.spill_stride, a.lds_depth, kSpillDepth);
-        const float vis = compute_visibility<Tree>(a.sc, f3{r.x, r.y, r.z}, ld3(l.pos), a.halton, x.x, x.y, st, cnt);
+        const float vis =
+            compute_visibility<Tree>(a.sc, f3{r.x, r.y, r.z}, ld3(l.pos), a.halton, x.x, x.y, st, cnt, Count ? &tc : nullptr);
         a.nee_vis[k] = vis != 0.0f ? 1 : 0;
         if (st.overflow && a.overflow) atomicAdd(a.overflow, (unsigned long long)st.overflow);
     }
@@ -732,6 +785,13 @@ __device__ __forceinline__ void shadow_body(const ShadowArgs &a, uint32_t base, 
     // light is within HIT_EPS)
     const int extra = (int)q < n ? (int)cnt.shadow_rays - 1 : 0;
     if (a.shadow_counter && __ballot(extra != 0)) wave_add_i64(a.shadow_counter, extra);
+    if (Count && a.visits) {
+        wave_add_u64(a.visits + 0, tc.nodes);
+        wave_add_u64(a.visits + 1, tc.leaf_tris);
+        wave_add_u64(a.visits + 2, tc.instances);
+        wave_add_u64(a.visits + 3, (int)q < n ? 1u : 0u);
+        wave_add_u64(a.visits + 4, cnt.shadow_rays);
+    }
 }
 
 // Adds each shaded ray's deferred direct light: ray.c * (sum of visible terms

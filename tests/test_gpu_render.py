@@ -124,7 +124,8 @@ def test_dense_spatial_split_scene_matches_oracle(oracle, host_lib, wide):
 
 
 @pytest.mark.parametrize("opts", [{}, {"sort_rays": 0, "lds_stack_depth": 8, "path_tail": -1}, {"defer_nee": 0},
-                                  {"path_tail": 0}, {"path_tail": 1, "tail_occupancy": 3}, {"refill": 1}])
+                                  {"path_tail": 0}, {"path_tail": 1, "tail_occupancy": 3}, {"refill": 1},
+                                  {"path_tail": 0, "tail_refill": 1}, {"path_tail": 2, "tail_refill": 1, "defer_nee": 0}])
 def test_ray_counts_match_oracle(oracle, host_lib, opts):
     """The device's primary/secondary/shadow counters (the numerator of Mrays/s)
     equal the CPU oracle's counts over the same passes."""

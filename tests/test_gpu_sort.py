@@ -63,7 +63,9 @@ def test_sort_is_invisible_in_the_image(host_lib, lds_depth):
                  {"sort_rays": 0, "defer_nee": 0, "path_tail": -1}, {"path_tail": 0}, {"path_tail": 1, "sort_rays": 1},
                  {"path_tail": 3, "defer_nee": 0}, {"path_tail": 0, "tail_occupancy": 3},
                  {"path_tail": 2, "tail_occupancy": 3, "defer_nee": 0}, {"refill": 1},
-                 {"refill": 1, "path_tail": -1, "sort_rays": 1, "lds_stack_depth": 4}):
+                 {"refill": 1, "path_tail": -1, "sort_rays": 1, "lds_stack_depth": 4},
+                 {"path_tail": 0, "tail_refill": 1}, {"path_tail": 2, "tail_refill": 1, "tail_occupancy": 3},
+                 {"path_tail": 1, "tail_refill": 1, "defer_nee": 0}):
         r = A.Renderer(host_lib, 160, 90, A.RENDERER_HIP)
         r.set_option("lds_stack_depth", lds_depth)
         for k, v in opts.items():
