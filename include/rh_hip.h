@@ -185,6 +185,13 @@ int rh_batch_read_pixels(rh_batch *batch, int scene, float *rgba, size_t count);
 int rh_batch_reset_stats(rh_batch *batch);
 int rh_batch_get_ray_counts(rh_batch *batch, rh_ray_counts *out);
 int rh_batch_set_option(rh_batch *batch, const char *name, int value);
+/* The rays entering stage `stage` (0: primary, 1 + k: bounce k) of scene
+ * `scene` in ray buffer `buffer` (0 or 1) after the last pass, for tests of
+ * the queue order: *count = the stage's ray count, min(count, cap) rays (the
+ * traversal half, rh_ray) copied to out.  With "sort_bits" set and every
+ * bounce staged, buffer 0 holds each sorted stage's rays in key order; the
+ * last stage's stay there after the pass. */
+int rh_batch_read_queue(rh_batch *batch, int scene, int stage, int buffer, rh_ray *out, size_t cap, int *count);
 
 /* ---- device-resident entry point for benchmarking the trace kernel ----
  * rays/hits are device pointers in the split layout (rh_ray / rh_hit). */

@@ -198,6 +198,8 @@ int rhx_batch_get_ray_counts(rhx_batch *b, rh_ray_counts *out);
 int rhx_batch_reset_stats(rhx_batch *b);
 int rhx_batch_synchronize(rhx_batch *b);
 int rhx_batch_restart(rhx_batch *b); /* back to pass 0 with cleared images */
+/* rh_batch_read_queue of the batch's device pipeline (queue-order tests) */
+int rhx_batch_read_queue(rhx_batch *b, int scene, int stage, int buffer, rh_ray *out, size_t cap, int *count);
 
 void rhx_set_environment(rhx_scene *s, const float env_col[3], float env_clamp, uint32_t env_map);
 uint32_t rhx_add_texture(rhx_scene *s, const uint8_t *rgba, int w, int h, int is_srgb, int generate_mipmaps);

@@ -239,6 +239,7 @@ class RayLib:
             "rhx_batch_render_pass": (c_int, [P]),
             "rhx_batch_get_pixels": (c_int, [P, c_int, P, c_size_t]),
             "rhx_batch_set_option": (c_int, [P, C.c_char_p, c_int]),
+            "rhx_batch_read_queue": (c_int, [P, c_int, c_int, c_int, P, c_size_t, POINTER(c_int)]),
             "rhx_batch_get_ray_counts": (c_int, [P, POINTER(RayCounts)]),
             "rhx_batch_reset_stats": (c_int, [P]),
             "rhx_batch_synchronize": (c_int, [P]),
@@ -480,6 +481,17 @@ class Batch:
 
     def reset_stats(self):
         self.lib.check(self.lib.lib.rhx_batch_reset_stats(self.h), "rhx_batch_reset_stats")
+
+    def read_queue(self, scene: int, stage: int, buffer: int) -> np.ndarray:
+        """The traversal halves of the rays entering `stage` of `scene` in ray
+        buffer `buffer` after the last pass, as (n, 8) float32 rows
+        (o xyz, xy bits, d xyz, depth bits)."""
+        w, h = self.sizes[scene]
+        out = np.zeros((w * h, 8), np.float32)
+        n = c_int(0)
+        self.lib.check(self.lib.lib.rhx_batch_read_queue(self.h, scene, stage, buffer, _ptr(out), w * h, C.byref(n)),
+                       "rhx_batch_read_queue")
+        return out[:n.value]
 
     def synchronize(self):
         self.lib.check(self.lib.lib.rhx_batch_synchronize(self.h), "rhx_batch_synchronize")

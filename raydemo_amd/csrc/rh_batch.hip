@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -29,8 +30,9 @@ using rh_rt::upload_halton;
 namespace {
 
 constexpr int kMaxBatchScenes = 256;
-// tile counters of one pass: generate, five per staged bounce, two tails, mix
-constexpr int kMaxStages = 5 * (RH_MAX_BOUNCES_EXT + 1) + 4;
+// tile counters of one pass: generate; per staged bounce five, and with the
+// sort two more plus two per radix pass; two tails; mix
+constexpr int kMaxStages = 15 * (RH_MAX_BOUNCES_EXT + 1) + 4;
 
 // one scene of the batch (device table)
 struct BatchEntry {
@@ -67,6 +69,8 @@ struct BatchDev {
     uint32_t lds_depth;
     const float *halton;
     unsigned long long *stats; // [0] primary [1] secondary [2] shadow [4] overflow
+    uint32_t *keys[2], *vals[2]; // ray sort: keys and ray indices (ping-pong), indexed as the rays
+    uint32_t *hist;              // ray sort: digit counts, 256 per tile of kSortTile keys
 };
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -83,20 +87,16 @@ __device__ __forceinline__ BatchEntry entry(const BatchDev &b, int s) {
 #endif
 }
 
-// Runs body(scene, tile base) for every 256-ray tile of every scene's queue,
-// count_s = counts[s] rays (0 for scenes that skip this stage).  Persistent:
-// blocks take tiles from *work until none is left.
-template <typename Active, typename Body>
-__device__ __forceinline__ void for_each_tile(const BatchDev &b, const int *counts, int *work, Active active,
-                                              Body body) {
-    __shared__ int s_pre[kMaxBatchScenes + 1];
-    __shared__ int s_t;
+// s_pre[s] = the first global tile of scene s, s_pre[S] = all tiles, for
+// tiles of TileRays rays of each active scene's counts[s] rays (S <= 256: one
+// scene per thread of the 256-thread block).  Every thread calls it.
+template <int TileRays, typename Active>
+__device__ __forceinline__ void scene_tile_prefix(const BatchDev &b, const int *counts, Active active, int *s_pre) {
     const int S = b.n_scenes;
-    // tiles per scene, then an exclusive scan (S <= 256: one scene per thread)
     int tiles = 0;
     if ((int)threadIdx.x < S) {
         const int s = threadIdx.x;
-        if (active(s)) tiles = (counts[s] + kBlock - 1) / kBlock;
+        if (active(s)) tiles = (counts[s] + TileRays - 1) / TileRays;
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int incl = tiles;
@@ -112,6 +112,19 @@ __device__ __forceinline__ void for_each_tile(const BatchDev &b, const int *coun
     if ((int)threadIdx.x < S) s_pre[threadIdx.x] = wave_base + incl - tiles;
     if ((int)threadIdx.x == S - 1) s_pre[S] = wave_base + incl;
     __syncthreads();
+}
+
+// Runs body(scene, tile base, global tile) for every TileRays-ray tile of
+// every scene's queue, count_s = counts[s] rays (0 for scenes that skip this
+// stage), the tiles numbered scene after scene.  Persistent: blocks take
+// tiles from *work until none is left.
+template <int TileRays = kBlock, typename Active, typename Body>
+__device__ __forceinline__ void for_each_tile(const BatchDev &b, const int *counts, int *work, Active active,
+                                              Body body) {
+    __shared__ int s_pre[kMaxBatchScenes + 1];
+    __shared__ int s_t;
+    const int S = b.n_scenes;
+    scene_tile_prefix<TileRays>(b, counts, active, s_pre);
     const int total = s_pre[S];
     for (;;) {
         if (threadIdx.x == 0) s_t = atomicAdd(work, 1);
@@ -126,7 +139,8 @@ __device__ __forceinline__ void for_each_tile(const BatchDev &b, const int *coun
             else hi = mid;
         }
         const int s = uniform(lo);
-        body(s, (uint32_t)(t - s_pre[s]) * kBlock);
+        if constexpr (std::is_invocable_v<Body, int, uint32_t, int>) body(s, (uint32_t)(t - s_pre[s]) * TileRays, t);
+        else body(s, (uint32_t)(t - s_pre[s]) * TileRays);
         __syncthreads(); // the body's LDS is reused by the next tile
     }
 }
@@ -297,6 +311,148 @@ __global__ void __launch_bounds__(kBlock) k_b_combine(BatchDev b, int stage, int
                   });
 }
 
+// ---- ray sort per scene (the single scene's sort on each scene's segment) ----
+// Before a staged bounce is traced, every scene's queue is sorted stably by
+// the get_ray_hash key of its rays (CoreRef.cpp:316-333, over the scene's own
+// hash grid) or by the key's top sort_bits bits: keys and identity indices,
+// then 8-bit LSB radix passes (per-tile digit counts, per-scene scan, stable
+// scatter), then the rays gathered into the other queue.  Tiles of kSortTile
+// keys never straddle scenes, and the digit counts are tile-major
+// (hist[tile * 256 + digit]).  The order of a queue changes no pixel.
+
+__device__ __forceinline__ bool sort_active(const BatchDev &b, int s, int stage) {
+    return stage >= 1 && stage_active(b.table[s], stage);
+}
+
+__global__ void __launch_bounds__(kBlock) k_b_sort_keys(BatchDev b, int stage, int q, int work) {
+    const int *counts = b.counts + stage * b.n_scenes;
+    for_each_tile(b, counts, b.work + work, [&](int s) { return sort_active(b, s, stage); },
+                  [&](int s, uint32_t base) {
+                      const BatchEntry e = entry(b, s);
+                      const int i = (int)(base + threadIdx.x);
+                      if (i >= counts[s]) return;
+                      const float4 *p = reinterpret_cast<const float4 *>(b.rays[q] + e.offset + i);
+                      const float4 r0 = p[0], r1 = p[1];
+                      b.keys[0][e.offset + i] =
+                          ray_hash(f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, e.pc.root_min, e.pc.cell_size);
+                      b.vals[0][e.offset + i] = (uint32_t)i;
+                  });
+}
+
+// digit counts of each kSortTile-key tile
+__global__ void __launch_bounds__(kBlock) k_b_sort_hist(BatchDev b, int stage, int src, int shift, int work) {
+    __shared__ uint32_t h[256];
+    const int *counts = b.counts + stage * b.n_scenes;
+    for_each_tile<kSortTile>(b, counts, b.work + work, [&](int s) { return sort_active(b, s, stage); },
+                             [&](int s, uint32_t base, int t) {
+                                 const BatchEntry e = entry(b, s);
+                                 const int n = counts[s];
+                                 const uint32_t *keys = b.keys[src] + e.offset;
+                                 h[threadIdx.x] = 0;
+                                 __syncthreads();
+                                 for (int r = 0; r < kSortItems; r++) {
+                                     const int i = (int)base + r * kBlock + (int)threadIdx.x;
+                                     if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xffu], 1u);
+                                 }
+                                 __syncthreads();
+                                 b.hist[(size_t)t * 256 + threadIdx.x] = h[threadIdx.x];
+                             });
+}
+
+// One block per scene: thread d turns digit d's counts over the scene's tiles
+// into each tile's first output position of the digit (the digit's base = the
+// scene's keys of smaller digits, plus the counts of earlier tiles).
+__global__ void __launch_bounds__(256) k_b_sort_scan(BatchDev b, int stage) {
+    __shared__ int s_pre[kMaxBatchScenes + 1];
+    __shared__ uint32_t s_tot[256];
+    const int *counts = b.counts + stage * b.n_scenes;
+    scene_tile_prefix<kSortTile>(b, counts, [&](int s) { return sort_active(b, s, stage); }, s_pre);
+    const int s = blockIdx.x;
+    const int first = s_pre[s], last = s_pre[s + 1];
+    if (first == last) return; // block-uniform: no tiles for this scene
+    const int d = threadIdx.x;
+    uint32_t tot = 0;
+    for (int t = first; t < last; t++) tot += b.hist[(size_t)t * 256 + d];
+    s_tot[d] = tot;
+    __syncthreads();
+    uint32_t run = 0;
+    for (int k = 0; k < d; k++) run += s_tot[k];
+    for (int t = first; t < last; t++) {
+        const uint32_t v = b.hist[(size_t)t * 256 + d];
+        b.hist[(size_t)t * 256 + d] = run;
+        run += v;
+    }
+}
+
+// Stable scatter of each tile (as k_radix_scatter: per round of 256 keys, each
+// wave ranks the lanes sharing its digit with eight ballots, the four waves'
+// counts are prefix-summed in LDS, rounds in order).
+__global__ void __launch_bounds__(kBlock) k_b_sort_scatter(BatchDev b, int stage, int src, int shift, int work) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcount[4][256];
+    const int *counts = b.counts + stage * b.n_scenes;
+    for_each_tile<kSortTile>(
+        b, counts, b.work + work, [&](int s) { return sort_active(b, s, stage); },
+        [&](int s, uint32_t base, int tile) {
+            const BatchEntry e = entry(b, s);
+            const int n = counts[s];
+            const uint32_t *keys_in = b.keys[src] + e.offset, *vals_in = b.vals[src] + e.offset;
+            uint32_t *keys_out = b.keys[src ^ 1] + e.offset, *vals_out = b.vals[src ^ 1] + e.offset;
+            const int t = threadIdx.x, wave = t >> 6;
+            const uint32_t lane = lane_id();
+            run[t] = b.hist[(size_t)tile * 256 + t];
+            const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+            for (int r = 0; r < kSortItems; r++) {
+                const int i = (int)base + r * kBlock + t;
+                const bool active = i < n;
+                uint32_t key = 0, val = 0, digit = 0;
+                if (active) {
+                    key = keys_in[i];
+                    val = vals_in[i];
+                    digit = (key >> shift) & 0xffu;
+                }
+                wcount[0][t] = wcount[1][t] = wcount[2][t] = wcount[3][t] = 0;
+                __syncthreads();
+                uint64_t peers = __ballot(active);
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const uint64_t m = __ballot((digit >> k) & 1u);
+                    peers &= ((digit >> k) & 1u) ? m : ~m;
+                }
+                const uint32_t rank = (uint32_t)__popcll(peers & lt);
+                if (active && rank == 0) wcount[wave][digit] = (uint32_t)__popcll(peers);
+                __syncthreads();
+                uint32_t pos = 0;
+                if (active) {
+                    uint32_t before = 0;
+                    for (int w2 = 0; w2 < wave; w2++) before += wcount[w2][digit];
+                    pos = run[digit] + before + rank;
+                }
+                __syncthreads();
+                run[t] += wcount[0][t] + wcount[1][t] + wcount[2][t] + wcount[3][t];
+                if (active) {
+                    keys_out[pos] = key;
+                    vals_out[pos] = val;
+                }
+                __syncthreads();
+            }
+        });
+}
+
+// the rays of queue q in sorted order into queue q ^ 1
+__global__ void __launch_bounds__(kBlock) k_b_sort_gather(BatchDev b, int stage, int q, int sorted, int work) {
+    const int *counts = b.counts + stage * b.n_scenes;
+    for_each_tile(b, counts, b.work + work, [&](int s) { return sort_active(b, s, stage); },
+                  [&](int s, uint32_t base) {
+                      const BatchEntry e = entry(b, s);
+                      const int i = (int)(base + threadIdx.x);
+                      if (i >= counts[s]) return;
+                      const uint32_t j = b.vals[sorted][e.offset + i];
+                      b.rays[q ^ 1][e.offset + i] = b.rays[q][e.offset + j];
+                      b.aux[q ^ 1][e.offset + i] = b.aux[q][e.offset + j];
+                  });
+}
+
 // the rays entering `stage` follow their paths to each scene's depth in one
 // lane; Defer: scenes whose light samples are resolved after shading (1..8 lights)
 template <bool Defer, int Tree>
@@ -358,6 +514,8 @@ struct rh_batch {
     size_t spill_cap = 0;
     float *halton = nullptr;
     unsigned long long *stats = nullptr;
+    uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
+    uint32_t *hist = nullptr;
     int passes_done = 0;
 
     // options
@@ -366,10 +524,11 @@ struct rh_batch {
     int use_graph = 1;
     int blocks_per_cu = 8;
     int max_bounces = RH_MAX_BOUNCES; // depth clamp of scenes added from now on
+    int sort_bits = 0;                 // staged bounces sorted by the top sort_bits of the ray key (0: no sort)
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    int graph_key[3] = {-1, -1, -1};
+    int graph_key[4] = {-1, -1, -1, -1};
 };
 
 namespace {
@@ -382,10 +541,12 @@ void drop_graph(rh_batch *b) {
 }
 
 void free_arrays(rh_batch *b) {
-    void *ps[] = {b->table, b->pass, b->counts, b->shadow_counts, b->work, b->rays[0], b->rays[1], b->aux[0],
-                  b->aux[1], b->hits, b->temp, b->clean, b->final_, b->nee_ray, b->nee_term, b->nee_aux, b->nee_vis,
-                  b->pend_c, b->pend_alb, b->pend_count, b->shadow_queue, b->spill};
+    void *ps[] = {b->table,   b->pass,    b->counts,  b->shadow_counts, b->work,    b->rays[0],   b->rays[1],
+                  b->aux[0],  b->aux[1],  b->hits,    b->temp,          b->clean,   b->final_,    b->nee_ray,
+                  b->nee_term, b->nee_aux, b->nee_vis, b->pend_c,       b->pend_alb, b->pend_count, b->shadow_queue,
+                  b->spill,   b->keys[0], b->keys[1], b->vals[0],       b->vals[1], b->hist};
     for (void *p : ps) free_ptr(p);
+    b->keys[0] = b->keys[1] = b->vals[0] = b->vals[1] = b->hist = nullptr;
     b->table = nullptr;
     b->pass = b->counts = b->shadow_counts = b->work = nullptr;
     b->rays[0] = b->rays[1] = nullptr;
@@ -420,7 +581,8 @@ int batch_tree(const rh_batch *b) {
 }
 
 // The per-bounce launches of a pass: stage 0 is the primary hit; stage k >= 1
-// traces bounce k - 1.  Bounces from path_tail on run in the tail kernel.
+// traces bounce k - 1 (its queue first sorted per scene when sort_bits is
+// set).  Bounces from path_tail on run in the tail kernel.
 template <int Tree>
 void launch_stages(rh_batch *b, const BatchDev &d, size_t lds, dim3 grid, dim3 blk, int tail, int &work) {
     int q = 0;
@@ -429,6 +591,18 @@ void launch_stages(rh_batch *b, const BatchDev &d, size_t lds, dim3 grid, dim3 b
             hipLaunchKernelGGL((k_b_tail<true, Tree>), grid, blk, lds, b->stream, d, stage, q, work++);
             hipLaunchKernelGGL((k_b_tail<false, Tree>), grid, blk, lds, b->stream, d, stage, q, work++);
             break;
+        }
+        if (stage >= 1 && b->sort_bits) {
+            hipLaunchKernelGGL(k_b_sort_keys, grid, blk, 0, b->stream, d, stage, q, work++);
+            int src = 0;
+            for (int shift = 32 - b->sort_bits; shift < 32; shift += 8) {
+                hipLaunchKernelGGL(k_b_sort_hist, grid, blk, 0, b->stream, d, stage, src, shift, work++);
+                hipLaunchKernelGGL(k_b_sort_scan, dim3((unsigned)b->entries.size()), dim3(256), 0, b->stream, d, stage);
+                hipLaunchKernelGGL(k_b_sort_scatter, grid, blk, 0, b->stream, d, stage, src, shift, work++);
+                src ^= 1;
+            }
+            hipLaunchKernelGGL(k_b_sort_gather, grid, blk, 0, b->stream, d, stage, q, src, work++);
+            q ^= 1; // the sorted rays
         }
         hipLaunchKernelGGL(k_b_trace<Tree>, grid, blk, lds, b->stream, d, stage, q, work++);
         hipLaunchKernelGGL(k_b_shade<true>, grid, blk, 0, b->stream, d, stage, q, work++);
@@ -516,7 +690,34 @@ BatchDev dev_view(rh_batch *b) {
     d.lds_depth = (uint32_t)b->lds_depth;
     d.halton = b->halton;
     d.stats = b->stats;
+    for (int k = 0; k < 2; k++) {
+        d.keys[k] = b->keys[k];
+        d.vals[k] = b->vals[k];
+    }
+    d.hist = b->hist;
     return d;
+}
+
+// The sort's buffers (once a pass sorts): keys and indices for every ray slot,
+// digit counts for the most tiles the scenes' queues can make.
+int ensure_sort_buffers(rh_batch *b) {
+    if (!b->sort_bits || b->hist) return 0;
+    const size_t P = b->total_pixels;
+    const size_t tiles = P / kSortTile + b->entries.size() + 1;
+    int rc = 0;
+    for (int k = 0; k < 2; k++) {
+        rc |= dev_alloc(&b->keys[k], P);
+        rc |= dev_alloc(&b->vals[k], P);
+    }
+    rc |= dev_alloc(&b->hist, tiles * 256);
+    if (rc) {
+        for (int k = 0; k < 2; k++) {
+            rh_rt::release(b->keys[k]);
+            rh_rt::release(b->vals[k]);
+        }
+        rh_rt::release(b->hist);
+    }
+    return rc;
 }
 
 // The launch sequence of one pass (fixed for given options): begin, generate,
@@ -524,7 +725,8 @@ BatchDev dev_view(rh_batch *b) {
 int enqueue_pass(rh_batch *b) {
     // first stage run by the tail kernel (bounce path_tail)
     const int tail = b->path_tail < 0 ? b->max_depth + 1 : std::min(b->path_tail + 1, b->max_depth + 1);
-    const int launches = 2 + 5 * tail + (tail <= b->max_depth ? 2 : 0);
+    const int sort_launches = b->sort_bits ? 2 + 2 * (b->sort_bits / 8) : 0;
+    const int launches = 2 + 5 * tail + sort_launches * std::max(0, tail - 1) + (tail <= b->max_depth ? 2 : 0);
     if (launches > kMaxStages) return arg_error("batch pass needs more stage counters than reserved");
     const BatchDev d = dev_view(b);
     const size_t lds = (size_t)b->lds_depth * kBlock * sizeof(uint32_t) * (any_wide(b) ? 2 : 1);
@@ -643,7 +845,9 @@ int rh_batch_render_pass(rh_batch *b) {
     int rc = build(b);
     if (rc) return rc;
     if (b->passes_done >= b->max_spp) return arg_error("every scene of the batch has rendered its passes");
-    const int key[3] = {b->lds_depth, b->path_tail, b->blocks_per_cu};
+    rc = ensure_sort_buffers(b);
+    if (rc) return rc;
+    const int key[4] = {b->lds_depth, b->path_tail, b->blocks_per_cu, b->sort_bits};
     if (b->use_graph) {
         if (!b->graph_exec || memcmp(key, b->graph_key, sizeof(key)) != 0) {
             drop_graph(b);
@@ -703,6 +907,20 @@ int rh_batch_read_pixels(rh_batch *b, int scene, float *rgba, size_t count) {
     return 0;
 }
 
+int rh_batch_read_queue(rh_batch *b, int scene, int stage, int buffer, rh_ray *out, size_t cap, int *count) {
+    if (scene < 0 || scene >= (int)b->entries.size()) return arg_error("scene index out of range");
+    if (stage < 0 || stage > RH_MAX_BOUNCES_EXT + 1 || buffer < 0 || buffer > 1)
+        return arg_error("stage or buffer out of range");
+    if (!b->built) return arg_error("batch has not rendered");
+    RH_CHECK(hipStreamSynchronize(b->stream));
+    int n = 0;
+    RH_CHECK(hipMemcpy(&n, b->counts + (size_t)stage * b->entries.size() + scene, sizeof(int), hipMemcpyDeviceToHost));
+    *count = n;
+    const size_t k = std::min<size_t>((size_t)std::max(n, 0), cap);
+    if (k) RH_CHECK(hipMemcpy(out, b->rays[buffer] + b->entries[scene].offset, k * sizeof(rh_ray), hipMemcpyDeviceToHost));
+    return 0;
+}
+
 int rh_batch_reset_stats(rh_batch *b) {
     RH_CHECK(hipMemsetAsync(b->stats, 0, 8 * sizeof(unsigned long long), b->stream));
     RH_CHECK(hipStreamSynchronize(b->stream));
@@ -737,6 +955,10 @@ int rh_batch_set_option(rh_batch *b, const char *name, int value) {
     } else if (n == "max_bounces") {
         if (value < 1 || value > RH_MAX_BOUNCES_EXT) return arg_error("max_bounces must be 1..16");
         b->max_bounces = value;
+    } else if (n == "sort_bits") {
+        if (value != 0 && value != 8 && value != 16 && value != 24 && value != 32)
+            return arg_error("sort_bits must be 0, 8, 16, 24 or 32");
+        b->sort_bits = value;
     } else {
         return arg_error("unknown option");
     }

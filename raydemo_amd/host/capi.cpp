@@ -476,6 +476,12 @@ int rhx_batch_get_ray_counts(rhx_batch *b, rh_ray_counts *out) {
 
 int rhx_batch_reset_stats(rhx_batch *b) { return rh_batch_reset_stats(b->b->handle()); }
 
+int rhx_batch_read_queue(rhx_batch *b, int scene, int stage, int buffer, rh_ray *out, size_t cap, int *count) {
+    const int rc = rh_batch_read_queue(b->b->handle(), scene, stage, buffer, out, cap, count);
+    if (rc != 0) g_err = rh_last_error();
+    return rc;
+}
+
 int rhx_batch_synchronize(rhx_batch *b) { return rh_batch_synchronize(b->b->handle()); }
 
 int rhx_batch_restart(rhx_batch *b) {

@@ -36,7 +36,8 @@ def _single(lib, build, w, h, spp):
 
 
 @pytest.mark.parametrize("opts", [{}, {"use_graph": 0}, {"path_tail": -1}, {"path_tail": 0, "lds_stack_depth": 8},
-                                  {"path_tail": 1, "blocks_per_cu": 1}])
+                                  {"path_tail": 1, "blocks_per_cu": 1}, {"sort_bits": 32, "path_tail": -1},
+                                  {"sort_bits": 24, "path_tail": 2, "blocks_per_cu": 1}])
 def test_batch_matches_single_renders(host_lib, golden_scene1, opts):
     cases = _cases(golden_scene1)
     singles = [_single(host_lib, b, w, h, spp) for _, b, w, h, spp in cases]
@@ -75,3 +76,35 @@ def test_batch_images_after_each_pass(host_lib):
         for i, (build, w, h, spp) in enumerate(cases):
             img, _ = _single(host_lib, build, w, h, min(k, spp))
             assert batch.pixels(i).tobytes() == img.tobytes(), (k, i)
+
+
+@pytest.mark.parametrize("bits", [32, 16])
+def test_batch_queues_sorted_per_scene(host_lib, oracle, bits):
+    """With sort_bits set and every bounce staged, each scene's queue of its
+    last bounce (bounce 0 at depth 1: ray buffer 0 after the pass) is in order of the top `bits`
+    bits of the get_ray_hash key over that scene's own hash grid
+    (CoreRef.cpp:316-333), and holds as many rays as the stage traced."""
+    builds = [lambda s: scenes.triangle_soup(s, n_spheres=32, n_fine=2, depth=1),
+              lambda s: scenes.cornell_box(s, depth=1)]
+    batch = A.Batch(host_lib)
+    batch.set_option("sort_bits", bits)
+    batch.set_option("path_tail", -1)
+    keep = []
+    for build in builds:
+        s = standalone_scene(host_lib)
+        build(s)
+        keep.append(s)
+        batch.add(s, 128, 96, 2)
+    batch.render_all()
+    for i, s in enumerate(keep):
+        q = batch.read_queue(i, 1, 0)
+        assert len(q) > 100
+        v = s.view()
+        root = v["nodes"][v["macro_nodes_root"]]
+        rmin = root["bbox_min"]
+        cell = ((root["bbox_max"] - root["bbox_min"]) / np.float32(255)).astype(np.float32)
+        p = np.zeros(len(q), A.RAY_PACKET_DTYPE)
+        p["o"], p["d"] = q[:, 0:3], q[:, 4:7]
+        keys = oracle.ray_hashes(p, rmin, cell).astype(np.int64) >> (32 - bits)
+        assert (np.diff(keys) >= 0).all(), i
+        assert len(np.unique(keys)) > 1
