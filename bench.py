@@ -370,29 +370,45 @@ def run_batch(args, lib, A, scenes, knobs, wide):
     b = A.Batch(lib)
     for it, s in zip(items, ss):
         b.add(s, it.width, it.height, it.spp)
-    # knobs of the batch: path tail, LDS stack share, persistent grid size (3 passes each)
-    table = []
-    for pt in (-1, 1, 2, 3):
-        for lds in (8, 16):
-            for bpc in (4, 8, 16):
-                b.set_option("path_tail", pt)
-                b.set_option("lds_stack_depth", lds)
-                b.set_option("blocks_per_cu", bpc)
-                b.restart()
-                b.render_pass()  # graph capture and warm-up
-                b.synchronize()
-                b.restart()
-                b.reset_stats()
-                t2 = time.perf_counter()
-                for _ in range(3):
-                    b.render_pass()
-                b.synchronize()
-                dt = time.perf_counter() - t2
-                table.append([pt, lds, bpc, round(total(b.ray_counts()) / dt / 1e6, 2)])
-    best = max(table, key=lambda r: r[-1])
-    b.set_option("path_tail", best[0])
-    b.set_option("lds_stack_depth", best[1])
-    b.set_option("blocks_per_cu", best[2])
+    # the batch's knobs by coordinate descent from the defaults, 3 passes per
+    # setting: path tail, LDS share of the traversal stack, persistent grid
+    # size, sort-key bits of the per-scene queue sort, and the queue size (rays
+    # in flight: scenes grouped to fit, 0 = every scene at once)
+    space = {"path_tail": (-1, 1, 2, 3), "lds_stack_depth": (8, 16, 24, 32), "blocks_per_cu": (4, 8, 16),
+             "sort_bits": (0, 8, 16, 24, 32), "queue_rays": (65536, 262144, 1048576, 4194304, 0)}
+    cur = {"path_tail": 2, "lds_stack_depth": 16, "blocks_per_cu": 8, "sort_bits": 0, "queue_rays": 0}
+    table, seen = [], {}
+
+    def measure(cfg):
+        key = tuple(cfg[k] for k in space)
+        if key in seen:
+            return seen[key]
+        for k, v in cfg.items():
+            b.set_option(k, v)
+        b.restart()
+        b.render_pass()  # (re)build, graph capture and warm-up
+        b.synchronize()
+        b.restart()
+        b.reset_stats()
+        t2 = time.perf_counter()
+        for _ in range(3):
+            b.render_pass()
+        b.synchronize()
+        mr = round(total(b.ray_counts()) / (time.perf_counter() - t2) / 1e6, 2)
+        seen[key] = mr
+        table.append(list(key) + [mr])
+        return mr
+
+    best_mr = measure(cur)
+    for _ in range(2):
+        for knob, values in space.items():
+            for v in values:
+                cfg = dict(cur, **{knob: v})
+                mr = measure(cfg)
+                if mr > best_mr:
+                    best_mr, cur = mr, cfg
+    for k, v in cur.items():
+        b.set_option(k, v)
     b.restart()
     b.render_pass()
     b.synchronize()
@@ -410,10 +426,10 @@ def run_batch(args, lib, A, scenes, knobs, wide):
         "rays": rays_batch, "rays_plain": rays_plain, "passes": b.passes,
         "plain_overlapped_mrays": round(rays_overlap / t_overlap / 1e6, 2),
         "speedup_vs_overlapped": round(rays_batch / t_batch / (rays_overlap / t_overlap), 3),
-        "knobs": {"path_tail": best[0], "lds_stack_depth": best[1], "blocks_per_cu": best[2], "use_graph": 1},
+        "knobs": dict(cur, use_graph=1),
         "plain_knobs": knobs,
-        "autotune": {"columns": ["path_tail", "lds_stack_depth", "blocks_per_cu", "mrays_first_3_passes"],
-                     "table": table},
+        "autotune": {"method": "coordinate descent from the defaults, two rounds, 3 passes per setting",
+                     "columns": list(space) + ["mrays_first_3_passes"], "table": table},
         "config": {"workload": f"{len(items)} icosphere soups, {sum(s.triangle_count() for s in ss)} triangles, "
                                f"{sum(i.width * i.height for i in items)} pixels, 4..32 spp, depth 4..8",
                    "scenes": len(items), "trees": "8-wide" if wide else "binary"},

@@ -39,10 +39,11 @@ struct BatchEntry {
     Scene sc;
     PassConst pc;      // camera and frame; iteration and mix_k are set per pass on the device
     rh_pass_info pi;   // pass settings (bounce set per stage)
-    uint32_t offset;   // first pixel / ray of the scene in the batch arrays
+    uint32_t offset;   // first pixel of the scene in the framebuffers
+    uint32_t q_offset; // first ray of the scene in the queue arrays (within its group)
     uint32_t pixels;   // w * h
     uint32_t nee_slots;  // light samples per shaded ray (0: visibility inline)
-    uint32_t nee_offset; // first slot of the scene in the NEE arrays
+    uint32_t nee_offset; // first slot of the scene in the NEE arrays (within its group)
     int spp;             // passes the scene takes part in
     int depth;           // max_total_depth (clamped)
 };
@@ -50,6 +51,7 @@ struct BatchEntry {
 struct BatchDev {
     const BatchEntry *__restrict__ table;
     int n_scenes;
+    int s_begin, s_end; // the scenes this launch works on (a group: their rays share the queues)
     int *pass;          // passes started so far (device)
     int *counts;        // [(RH_MAX_BOUNCES_EXT + 2) * n_scenes]: rays entering stage b (b = 0: primary)
     int *shadow_counts; // [(RH_MAX_BOUNCES_EXT + 2) * n_scenes]
@@ -96,7 +98,7 @@ __device__ __forceinline__ void scene_tile_prefix(const BatchDev &b, const int *
     int tiles = 0;
     if ((int)threadIdx.x < S) {
         const int s = threadIdx.x;
-        if (active(s)) tiles = (counts[s] + TileRays - 1) / TileRays;
+        if (s >= b.s_begin && s < b.s_end && active(s)) tiles = (counts[s] + TileRays - 1) / TileRays;
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int incl = tiles;
@@ -175,7 +177,7 @@ __global__ void __launch_bounds__(kBlock) k_b_gen(BatchDev b, int work) {
         if (i >= n) return;
         const int w = e.pc.w;
         if (i == 0) atomicAdd(b.stats + 0, (unsigned long long)n);
-        store_ray(b.rays[0] + e.offset, b.aux[0] + e.offset, (uint32_t)i, gen_primary_ray(e.pc, b.halton, i % w, i / w));
+        store_ray(b.rays[0] + e.q_offset, b.aux[0] + e.q_offset, (uint32_t)i, gen_primary_ray(e.pc, b.halton, i % w, i / w));
     });
 }
 
@@ -195,12 +197,12 @@ template <int Tree> __global__ void __launch_bounds__(kBlock) k_b_trace(BatchDev
                       const BatchEntry e = entry(b, s);
                       TraceArgs a;
                       a.sc = e.sc;
-                      a.rays = b.rays[q] + e.offset;
+                      a.rays = b.rays[q] + e.q_offset;
                       a.count = counts + s;
                       a.n = 0;
                       a.tmax = nullptr;
-                      a.hits = b.hits + e.offset;
-                      a.spill = b.spill + e.offset;
+                      a.hits = b.hits + e.q_offset;
+                      a.spill = b.spill + e.q_offset;
                       a.spill_stride = b.spill_stride;
                       a.lds_depth = b.lds_depth;
                       a.ray_counter = b.stats + (stage == 0 ? 0 : 1);
@@ -215,9 +217,9 @@ __device__ __forceinline__ ShadeArgs batch_shade_args(const BatchDev &b, const B
                                                       bool defer) {
     ShadeArgs a = {};
     a.sc = e.sc;
-    a.rays = b.rays[q] + e.offset;
-    a.aux = b.aux[q] + e.offset;
-    a.hits = b.hits + e.offset;
+    a.rays = b.rays[q] + e.q_offset;
+    a.aux = b.aux[q] + e.q_offset;
+    a.hits = b.hits + e.q_offset;
     a.count = b.counts + stage * b.n_scenes + s;
     a.pi = e.pi;
     a.pi.bounce = stage == 0 ? 2 : stage + 2;
@@ -225,10 +227,10 @@ __device__ __forceinline__ ShadeArgs batch_shade_args(const BatchDev &b, const B
     a.halton = b.halton;
     a.temp = b.temp + e.offset;
     a.primary = stage == 0 ? 1 : 0;
-    a.out_rays = b.rays[q ^ 1] + e.offset;
-    a.out_aux = b.aux[q ^ 1] + e.offset;
+    a.out_rays = b.rays[q ^ 1] + e.q_offset;
+    a.out_aux = b.aux[q ^ 1] + e.q_offset;
     a.out_count = b.counts + (stage + 1) * b.n_scenes + s;
-    a.spill = b.spill + e.offset;
+    a.spill = b.spill + e.q_offset;
     a.spill_stride = b.spill_stride;
     a.lds_depth = b.lds_depth;
     a.shadow_counter = b.stats + 2;
@@ -239,9 +241,9 @@ __device__ __forceinline__ ShadeArgs batch_shade_args(const BatchDev &b, const B
         a.nee_aux = b.nee_aux + e.nee_offset;
         a.nee_term = b.nee_term + e.nee_offset;
         a.nee_slots = e.nee_slots;
-        a.pend_c = b.pend_c + e.offset;
-        a.pend_alb = b.pend_alb + e.offset;
-        a.pend_count = b.pend_count + e.offset;
+        a.pend_c = b.pend_c + e.q_offset;
+        a.pend_alb = b.pend_alb + e.q_offset;
+        a.pend_count = b.pend_count + e.q_offset;
         a.shadow_queue = b.shadow_queue + e.nee_offset;
         a.shadow_count = b.shadow_counts + stage * b.n_scenes + s;
     }
@@ -297,9 +299,9 @@ __global__ void __launch_bounds__(kBlock) k_b_combine(BatchDev b, int stage, int
                       a.count = counts + s;
                       a.n = 0;
                       a.slots = e.nee_slots;
-                      a.pend_count = b.pend_count + e.offset;
-                      a.pend_c = b.pend_c + e.offset;
-                      a.pend_alb = b.pend_alb + e.offset;
+                      a.pend_count = b.pend_count + e.q_offset;
+                      a.pend_c = b.pend_c + e.q_offset;
+                      a.pend_alb = b.pend_alb + e.q_offset;
                       a.nee_term = b.nee_term + e.nee_offset;
                       a.nee_vis = b.nee_vis + e.nee_offset;
                       a.temp = b.temp + e.offset;
@@ -331,11 +333,11 @@ __global__ void __launch_bounds__(kBlock) k_b_sort_keys(BatchDev b, int stage, i
                       const BatchEntry e = entry(b, s);
                       const int i = (int)(base + threadIdx.x);
                       if (i >= counts[s]) return;
-                      const float4 *p = reinterpret_cast<const float4 *>(b.rays[q] + e.offset + i);
+                      const float4 *p = reinterpret_cast<const float4 *>(b.rays[q] + e.q_offset + i);
                       const float4 r0 = p[0], r1 = p[1];
-                      b.keys[0][e.offset + i] =
+                      b.keys[0][e.q_offset + i] =
                           ray_hash(f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, e.pc.root_min, e.pc.cell_size);
-                      b.vals[0][e.offset + i] = (uint32_t)i;
+                      b.vals[0][e.q_offset + i] = (uint32_t)i;
                   });
 }
 
@@ -347,7 +349,7 @@ __global__ void __launch_bounds__(kBlock) k_b_sort_hist(BatchDev b, int stage, i
                              [&](int s, uint32_t base, int t) {
                                  const BatchEntry e = entry(b, s);
                                  const int n = counts[s];
-                                 const uint32_t *keys = b.keys[src] + e.offset;
+                                 const uint32_t *keys = b.keys[src] + e.q_offset;
                                  h[threadIdx.x] = 0;
                                  __syncthreads();
                                  for (int r = 0; r < kSortItems; r++) {
@@ -396,8 +398,8 @@ __global__ void __launch_bounds__(kBlock) k_b_sort_scatter(BatchDev b, int stage
         [&](int s, uint32_t base, int tile) {
             const BatchEntry e = entry(b, s);
             const int n = counts[s];
-            const uint32_t *keys_in = b.keys[src] + e.offset, *vals_in = b.vals[src] + e.offset;
-            uint32_t *keys_out = b.keys[src ^ 1] + e.offset, *vals_out = b.vals[src ^ 1] + e.offset;
+            const uint32_t *keys_in = b.keys[src] + e.q_offset, *vals_in = b.vals[src] + e.q_offset;
+            uint32_t *keys_out = b.keys[src ^ 1] + e.q_offset, *vals_out = b.vals[src ^ 1] + e.q_offset;
             const int t = threadIdx.x, wave = t >> 6;
             const uint32_t lane = lane_id();
             run[t] = b.hist[(size_t)tile * 256 + t];
@@ -447,9 +449,9 @@ __global__ void __launch_bounds__(kBlock) k_b_sort_gather(BatchDev b, int stage,
                       const BatchEntry e = entry(b, s);
                       const int i = (int)(base + threadIdx.x);
                       if (i >= counts[s]) return;
-                      const uint32_t j = b.vals[sorted][e.offset + i];
-                      b.rays[q ^ 1][e.offset + i] = b.rays[q][e.offset + j];
-                      b.aux[q ^ 1][e.offset + i] = b.aux[q][e.offset + j];
+                      const uint32_t j = b.vals[sorted][e.q_offset + i];
+                      b.rays[q ^ 1][e.q_offset + i] = b.rays[q][e.q_offset + j];
+                      b.aux[q ^ 1][e.q_offset + i] = b.aux[q][e.q_offset + j];
                   });
 }
 
@@ -496,6 +498,10 @@ struct rh_batch {
     int max_depth = 0;
     int max_spp = 0;
     size_t total_pixels = 0, total_slots = 0;
+    // scenes in groups whose rays fit the queue limit, rendered one group after
+    // another within a pass: [first, end) per group, and the largest group's rays and NEE slots
+    std::vector<std::pair<int, int>> groups;
+    size_t queue_pixels = 0, queue_slots = 0;
 
     BatchEntry *table = nullptr;
     int *pass = nullptr;
@@ -525,6 +531,7 @@ struct rh_batch {
     int blocks_per_cu = 8;
     int max_bounces = RH_MAX_BOUNCES; // depth clamp of scenes added from now on
     int sort_bits = 0;                 // staged bounces sorted by the top sort_bits of the ray key (0: no sort)
+    long long queue_rays = 0;          // rays in flight at most (scenes grouped to fit; 0: every scene at once)
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -613,24 +620,53 @@ void launch_stages(rh_batch *b, const BatchDev &d, size_t lds, dim3 grid, dim3 b
     }
 }
 
+// Tile counters of the whole pass: kMaxStages per group, and the mix.
+size_t work_counters(const rh_batch *b) { return b->groups.size() * (size_t)kMaxStages + 4; }
+
+// Scenes in order into groups of at most queue_rays pixels (a larger scene
+// alone), each scene's queue and NEE offsets within its group.
+void group_scenes(rh_batch *b) {
+    b->groups.clear();
+    b->queue_pixels = b->queue_slots = 0;
+    size_t qp = 0, qs = 0;
+    int first = 0;
+    for (int s = 0; s < (int)b->entries.size(); s++) {
+        BatchEntry &e = b->entries[s];
+        if (b->queue_rays > 0 && qp > 0 && qp + e.pixels > (size_t)b->queue_rays) {
+            b->groups.push_back({first, s});
+            first = s;
+            qp = qs = 0;
+        }
+        e.q_offset = (uint32_t)qp;
+        e.nee_offset = (uint32_t)qs;
+        qp += e.pixels;
+        qs += (size_t)e.pixels * e.nee_slots;
+        b->queue_pixels = std::max(b->queue_pixels, qp);
+        b->queue_slots = std::max(b->queue_slots, qs);
+    }
+    b->groups.push_back({first, (int)b->entries.size()});
+}
+
 // Allocates the shared queues for the scenes added so far and uploads the table.
 int build(rh_batch *b) {
     if (b->built) return 0;
     if (b->entries.empty()) return arg_error("batch has no scenes");
     free_arrays(b);
     drop_graph(b);
-    const size_t P = b->total_pixels, NS = b->total_slots, S = b->entries.size();
+    group_scenes(b);
+    // framebuffers for every pixel of the batch; queues for the largest group
+    const size_t P = b->total_pixels, QP = b->queue_pixels, NS = b->queue_slots, S = b->entries.size();
     int rc = 0;
     rc |= dev_alloc(&b->table, S);
     rc |= dev_alloc(&b->pass, 1);
     rc |= dev_alloc(&b->counts, (RH_MAX_BOUNCES_EXT + 2) * S);
     rc |= dev_alloc(&b->shadow_counts, (RH_MAX_BOUNCES_EXT + 2) * S);
-    rc |= dev_alloc(&b->work, (size_t)kMaxStages);
+    rc |= dev_alloc(&b->work, work_counters(b));
     for (int k = 0; k < 2; k++) {
-        rc |= dev_alloc(&b->rays[k], P);
-        rc |= dev_alloc(&b->aux[k], P);
+        rc |= dev_alloc(&b->rays[k], QP);
+        rc |= dev_alloc(&b->aux[k], QP);
     }
-    rc |= dev_alloc(&b->hits, P);
+    rc |= dev_alloc(&b->hits, QP);
     rc |= dev_alloc(&b->temp, P);
     rc |= dev_alloc(&b->clean, P);
     rc |= dev_alloc(&b->final_, P);
@@ -639,10 +675,10 @@ int build(rh_batch *b) {
     rc |= dev_alloc(&b->nee_aux, NS);
     rc |= dev_alloc(&b->nee_vis, NS);
     rc |= dev_alloc(&b->shadow_queue, NS);
-    rc |= dev_alloc(&b->pend_c, P);
-    rc |= dev_alloc(&b->pend_alb, P);
-    rc |= dev_alloc(&b->pend_count, P);
-    b->spill_cap = std::max(P, NS);
+    rc |= dev_alloc(&b->pend_c, QP);
+    rc |= dev_alloc(&b->pend_alb, QP);
+    rc |= dev_alloc(&b->pend_count, QP);
+    b->spill_cap = std::max(QP, NS);
     // entries, and their distances when a scene has 8-wide trees
     rc |= dev_alloc(&b->spill, (size_t)kSpillDepth * (any_wide(b) ? 2 : 1) * b->spill_cap);
     if (rc) return rc;
@@ -665,6 +701,8 @@ BatchDev dev_view(rh_batch *b) {
     memset(&d, 0, sizeof(d));
     d.table = b->table;
     d.n_scenes = (int)b->entries.size();
+    d.s_begin = 0;
+    d.s_end = d.n_scenes;
     d.pass = b->pass;
     d.counts = b->counts;
     d.shadow_counts = b->shadow_counts;
@@ -702,7 +740,7 @@ BatchDev dev_view(rh_batch *b) {
 // digit counts for the most tiles the scenes' queues can make.
 int ensure_sort_buffers(rh_batch *b) {
     if (!b->sort_bits || b->hist) return 0;
-    const size_t P = b->total_pixels;
+    const size_t P = b->queue_pixels;
     const size_t tiles = P / kSortTile + b->entries.size() + 1;
     int rc = 0;
     for (int k = 0; k < 2; k++) {
@@ -726,19 +764,26 @@ int enqueue_pass(rh_batch *b) {
     // first stage run by the tail kernel (bounce path_tail)
     const int tail = b->path_tail < 0 ? b->max_depth + 1 : std::min(b->path_tail + 1, b->max_depth + 1);
     const int sort_launches = b->sort_bits ? 2 + 2 * (b->sort_bits / 8) : 0;
-    const int launches = 2 + 5 * tail + sort_launches * std::max(0, tail - 1) + (tail <= b->max_depth ? 2 : 0);
+    // tile counters of one group: generation, the staged bounces (with their sorts), the tails
+    const int launches = 1 + 5 * tail + sort_launches * std::max(0, tail - 1) + (tail <= b->max_depth ? 2 : 0);
     if (launches > kMaxStages) return arg_error("batch pass needs more stage counters than reserved");
-    const BatchDev d = dev_view(b);
+    const BatchDev all = dev_view(b);
     const size_t lds = (size_t)b->lds_depth * kBlock * sizeof(uint32_t) * (any_wide(b) ? 2 : 1);
     const dim3 grid((unsigned)(b->cus * b->blocks_per_cu)), blk(kBlock);
     int work = 0;
-    hipLaunchKernelGGL(k_b_begin_pass, dim3(1), dim3(1024), 0, b->stream, d, b->table, (int)kMaxStages);
-    hipLaunchKernelGGL(k_b_gen, grid, blk, 0, b->stream, d, work++);
+    hipLaunchKernelGGL(k_b_begin_pass, dim3(1), dim3(1024), 0, b->stream, all, b->table, (int)work_counters(b));
     const int tree = batch_tree(b);
-    if (tree == kTreeBinary) launch_stages<kTreeBinary>(b, d, lds, grid, blk, tail, work);
-    else if (tree == kTreeWide) launch_stages<kTreeWide>(b, d, lds, grid, blk, tail, work);
-    else launch_stages<kTreeAny>(b, d, lds, grid, blk, tail, work);
-    hipLaunchKernelGGL(k_b_mix, grid, blk, 0, b->stream, d, work++);
+    // the groups one after another: each fills the queues with its scenes' rays
+    for (const auto &g : b->groups) {
+        BatchDev d = all;
+        d.s_begin = g.first;
+        d.s_end = g.second;
+        hipLaunchKernelGGL(k_b_gen, grid, blk, 0, b->stream, d, work++);
+        if (tree == kTreeBinary) launch_stages<kTreeBinary>(b, d, lds, grid, blk, tail, work);
+        else if (tree == kTreeWide) launch_stages<kTreeWide>(b, d, lds, grid, blk, tail, work);
+        else launch_stages<kTreeAny>(b, d, lds, grid, blk, tail, work);
+    }
+    hipLaunchKernelGGL(k_b_mix, grid, blk, 0, b->stream, all, work++);
     RH_CHECK(hipGetLastError());
     return 0;
 }
@@ -917,7 +962,7 @@ int rh_batch_read_queue(rh_batch *b, int scene, int stage, int buffer, rh_ray *o
     RH_CHECK(hipMemcpy(&n, b->counts + (size_t)stage * b->entries.size() + scene, sizeof(int), hipMemcpyDeviceToHost));
     *count = n;
     const size_t k = std::min<size_t>((size_t)std::max(n, 0), cap);
-    if (k) RH_CHECK(hipMemcpy(out, b->rays[buffer] + b->entries[scene].offset, k * sizeof(rh_ray), hipMemcpyDeviceToHost));
+    if (k) RH_CHECK(hipMemcpy(out, b->rays[buffer] + b->entries[scene].q_offset, k * sizeof(rh_ray), hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -955,6 +1000,13 @@ int rh_batch_set_option(rh_batch *b, const char *name, int value) {
     } else if (n == "max_bounces") {
         if (value < 1 || value > RH_MAX_BOUNCES_EXT) return arg_error("max_bounces must be 1..16");
         b->max_bounces = value;
+    } else if (n == "queue_rays") {
+        if (value < 0) return arg_error("queue_rays must be >= 0 (0: no limit)");
+        if (value != b->queue_rays) {
+            b->queue_rays = value;
+            b->built = false; // queues and groups are laid out again at the next pass (which restarts the batch)
+            drop_graph(b);
+        }
     } else if (n == "sort_bits") {
         if (value != 0 && value != 8 && value != 16 && value != 24 && value != 32)
             return arg_error("sort_bits must be 0, 8, 16, 24 or 32");

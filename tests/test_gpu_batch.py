@@ -37,7 +37,9 @@ def _single(lib, build, w, h, spp):
 
 @pytest.mark.parametrize("opts", [{}, {"use_graph": 0}, {"path_tail": -1}, {"path_tail": 0, "lds_stack_depth": 8},
                                   {"path_tail": 1, "blocks_per_cu": 1}, {"sort_bits": 32, "path_tail": -1},
-                                  {"sort_bits": 24, "path_tail": 2, "blocks_per_cu": 1}])
+                                  {"sort_bits": 24, "path_tail": 2, "blocks_per_cu": 1}, {"queue_rays": 5000},
+                                  {"queue_rays": 1, "sort_bits": 16, "path_tail": -1},
+                                  {"queue_rays": 3000, "use_graph": 0, "path_tail": 1}])
 def test_batch_matches_single_renders(host_lib, golden_scene1, opts):
     cases = _cases(golden_scene1)
     singles = [_single(host_lib, b, w, h, spp) for _, b, w, h, spp in cases]

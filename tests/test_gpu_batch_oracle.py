@@ -32,7 +32,8 @@ def batch_cases(host_lib, oracle):
 
 @pytest.mark.parametrize("opts", [{}, {"path_tail": -1}, {"path_tail": 1, "lds_stack_depth": 8},
                                   {"sort_bits": 32, "path_tail": -1}, {"sort_bits": 16, "path_tail": 2},
-                                  {"sort_bits": 8, "path_tail": -1, "use_graph": 0}])
+                                  {"sort_bits": 8, "path_tail": -1, "use_graph": 0},
+                                  {"queue_rays": 20000, "sort_bits": 32, "path_tail": -1}, {"queue_rays": 40000}])
 def test_batch_matches_oracle(host_lib, batch_cases, opts):
     b = A.Batch(host_lib)
     for k, v in opts.items():
