@@ -22,7 +22,9 @@ enum eRendererType {
 
 struct settings_t {
     int w, h;
+#if !defined(DISABLE_OCL)
     int platform_index = -1, device_index = -1;
+#endif
     bool use_wide_bvh = true;
 };
 

@@ -51,7 +51,7 @@ int rhx_create_renderer(const rhx_settings *s, uint32_t flags, rhx_renderer **ou
         Ray::settings_t st;
         st.w = s->w;
         st.h = s->h;
-        st.device_index = s->device_index;
+        Ray::set_settings_device(st, s->device_index, 0);
         st.use_wide_bvh = s->use_wide_bvh != 0;
         std::ostringstream log;
         auto r = std::dynamic_pointer_cast<Ray::Hip::Renderer>(Ray::CreateRenderer(st, Ray::RendererHIP, log));

@@ -17,6 +17,18 @@
 #include "rh_hip.h"
 
 namespace Ray {
+// settings_t::device_index (the HIP device ordinal) where the build declares
+// it: the reference declares platform_index/device_index only when its OpenCL
+// backend is compiled in (RendererBase.h:24-29); otherwise device 0.
+template <class S> inline auto settings_device(const S &s, int) -> decltype(s.device_index, int()) {
+    return s.device_index;
+}
+template <class S> inline int settings_device(const S &, long) { return 0; }
+template <class S> inline auto set_settings_device(S &s, int d, int) -> decltype(s.device_index = d, void()) {
+    s.device_index = d;
+}
+template <class S> inline void set_settings_device(S &, int, long) {}
+
 namespace Hip {
 
 class Renderer;

@@ -203,7 +203,7 @@ void Renderer::ResetStats() { rh_reset_stats(ctx_); }
 std::shared_ptr<RendererBase> CreateRenderer(const settings_t &s, uint32_t flags, std::ostream &log_stream) {
     if (flags & RendererHIP) {
         log_stream << "Ray: Creating HIP renderer " << s.w << "x" << s.h << std::endl;
-        return std::make_shared<Hip::Renderer>(s, s.device_index);
+        return std::make_shared<Hip::Renderer>(s, settings_device(s, 0));
     }
     throw std::runtime_error("this library provides only the HIP backend (RendererHIP)");
 }
