@@ -414,6 +414,51 @@ def sliver_scene(scene: "A.Scene", seed: int = SEED_BASE + 7, n: int = 400, widt
     return SceneSpec("sliver_scene", width, height, spp, depth, triangles=len(idx) // 3)
 
 
+def furnace_box(scene: "A.Scene", p: float = 0.9, rho: float = 0.95, emission: float = 1.0, width=64, height=64,
+                spp=32, depth=12) -> SceneSpec:
+    """A closed cube seen from its centre, every wall the same mix material:
+    with probability 1 - p an emissive surface of strength `emission` (the
+    path ends), with probability p a Lambertian surface of albedo rho (the
+    path bounces).  The mix's interior IOR is tiny, so its Fresnel factor is 1
+    (total internal reflection at every angle but normal incidence) and the
+    choice probability is p at every hit.  Russian roulette is off
+    (termination_start_depth past the depth), so the expected pixel value is
+    (1 - p) emission sum_{k=0..depth} (p rho)^k (furnace_expected)."""
+    L = 5.0
+    faces = [((-L, -L, -L), (L, -L, -L), (L, -L, L), (-L, -L, L), (0, 1, 0)),
+             ((-L, L, L), (L, L, L), (L, L, -L), (-L, L, -L), (0, -1, 0)),
+             ((-L, -L, L), (L, -L, L), (L, L, L), (-L, L, L), (0, 0, -1)),
+             ((L, -L, -L), (-L, -L, -L), (-L, L, -L), (L, L, -L), (0, 0, 1)),
+             ((-L, -L, -L), (-L, -L, L), (-L, L, L), (-L, L, -L), (1, 0, 0)),
+             ((L, -L, L), (L, -L, -L), (L, L, -L), (L, L, L), (-1, 0, 0))]
+    attrs, idx = [], []
+    for f in faces:
+        a, i = _quad(*f)
+        idx.append(i + 4 * len(attrs))
+        attrs.append(a)
+    attrs, idx = np.concatenate(attrs), np.concatenate(idx)
+    tex = _white(scene)
+    emissive = scene.add_material(A.EmissiveMaterial, tex, main_color=(1.0, 1.0, 1.0), strength=emission)
+    diffuse = scene.add_material(A.DiffuseMaterial, tex, main_color=(rho, rho, rho), roughness=0.0)
+    mix = scene.add_material(A.MixMaterial, tex, mix_materials=(emissive, diffuse), strength=p, int_ior=1e-4,
+                             ext_ior=1e-4)
+    mesh = scene.add_mesh(A.PxyzNxyzTuv, attrs, idx, [(mix, 0, len(idx))])
+    scene.add_mesh_instance(mesh)
+    scene.set_environment((0.0, 0.0, 0.0))
+    cam = scene.add_camera(origin=(0.3, -0.2, 0.1), fwd=(0.2, 0.1, 1.0), fov=70.0, filter=A.Box, dtype=A.DeviceNone,
+                           max_diff_depth=depth, max_glossy_depth=depth, max_refr_depth=depth,
+                           max_transp_depth=depth, max_total_depth=depth, termination_start_depth=min(255, depth + 1))
+    scene.set_current_cam(cam)
+    return SceneSpec("furnace_box", width, height, spp, depth, triangles=len(idx) // 3,
+                     info={"p": p, "rho": rho, "emission": emission})
+
+
+def furnace_expected(p: float, rho: float, emission: float, depth: int) -> float:
+    """The closed form of furnace_box: sum over the hit k = 0..depth at which
+    the path ends in emission (probability (1 - p) p^k, throughput rho^k)."""
+    return (1.0 - p) * emission * sum((p * rho) ** k for k in range(depth + 1))
+
+
 def soup_camera(depth=8, coherent=False) -> dict:
     """The soup's camera (camera_desc_t fields); coherent: UseCoherentSampling."""
     return dict(origin=(0.0, 8.0, -48.0), fwd=(0.0, -0.15, 1.0), fov=50.0, filter=A.Box, dtype=A.DeviceNone,
