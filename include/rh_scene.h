@@ -201,12 +201,20 @@ int rhx_batch_restart(rhx_batch *b); /* back to pass 0 with cleared images */
 /* rh_batch_read_queue of the batch's device pipeline (queue-order tests) */
 int rhx_batch_read_queue(rhx_batch *b, int scene, int stage, int buffer, rh_ray *out, size_t cap, int *count);
 
-void rhx_set_environment(rhx_scene *s, const float env_col[3], float env_clamp, uint32_t env_map);
+/* The adders return 0xffffffff, and rhx_set_environment -1, with the reason in
+   rhx_last_error(), when an index names no existing texture or material
+   (main_texture and mix components required, normal_map and env_map optional
+   as 0xffffffff), a mesh index names no vertex, or a shape range passes the
+   mesh's indices; nothing is changed then.  (The reference stores such
+   indices unchecked and reads through them when rendering.) */
+int rhx_set_environment(rhx_scene *s, const float env_col[3], float env_clamp, uint32_t env_map);
 uint32_t rhx_add_texture(rhx_scene *s, const uint8_t *rgba, int w, int h, int is_srgb, int generate_mipmaps);
 uint32_t rhx_add_material(rhx_scene *s, const rhx_mat_desc *m);
 /* bvh_flags: RHX_MESH_SPATIAL_SPLITS (mesh_desc_t::allow_spatial_splits),
    RHX_MESH_FAST_BVH (mesh_desc_t::use_fast_bvh_build, SceneBase.h:94: the
-   HLBVH build of Core.cpp:532-680 in place of the SAH build) */
+   HLBVH build of Core.cpp:532-680 in place of the SAH build).
+   attrs_count is the number of vertices (mesh_desc_t::vtx_attrs_count,
+   SceneBase.h:88), each of the layout's stride in floats. */
 #define RHX_MESH_SPATIAL_SPLITS 1
 #define RHX_MESH_FAST_BVH 2
 uint32_t rhx_add_mesh(rhx_scene *s, int layout, const float *attrs, size_t attrs_count,

@@ -185,12 +185,13 @@ int rhx_render_scene(rhx_renderer *r, rhx_scene *s, rhx_region *reg) {
     return 0;
 }
 
-void rhx_set_environment(rhx_scene *s, const float env_col[3], float env_clamp, uint32_t env_map) {
+int rhx_set_environment(rhx_scene *s, const float env_col[3], float env_clamp, uint32_t env_map) {
     Ray::environment_desc_t e;
     memcpy(e.env_col, env_col, 3 * sizeof(float));
     e.env_clamp = env_clamp;
     e.env_map = env_map;
     s->s->SetEnvironment(e);
+    return 0;
 }
 
 uint32_t rhx_add_texture(rhx_scene *s, const uint8_t *rgba, int w, int h, int is_srgb, int gen_mips) {

@@ -198,7 +198,7 @@ class RayLib:
             "rhx_region_clear": (c_int, [P]),
             "rhx_region_halton": (c_int, [P, P, c_size_t]),
             "rhx_render_scene": (c_int, [P, P, P]),
-            "rhx_set_environment": (None, [P, POINTER(c_float), c_float, c_uint32]),
+            "rhx_set_environment": (c_int, [P, POINTER(c_float), c_float, c_uint32]),
             "rhx_add_texture": (c_uint32, [P, P, c_int, c_int, c_int, c_int]),
             "rhx_add_material": (c_uint32, [P, POINTER(MatDesc)]),
             "rhx_add_mesh": (c_uint32, [P, c_int, P, c_size_t, P, c_size_t, c_int, P, c_size_t, c_int]),
@@ -551,20 +551,29 @@ class Scene:
         except Exception:
             pass
 
+    def _added(self, index: int, what: str) -> int:
+        """The index an adder returned; RuntimeError with the library's reason
+        when it returned 0xffffffff (nothing was added)."""
+        if index == INVALID:
+            raise RuntimeError(f"{what} failed: {self.lib.error()}")
+        return index
+
     def set_environment(self, env_col=(0.0, 0.0, 0.0), env_clamp=0.0, env_map=INVALID):
-        self.lib.lib.rhx_set_environment(self.h, (c_float * 3)(*env_col), env_clamp, env_map)
+        self.lib.check(self.lib.lib.rhx_set_environment(self.h, (c_float * 3)(*env_col), env_clamp, env_map),
+                       "rhx_set_environment")
 
     def add_texture(self, rgba: np.ndarray, is_srgb=True, generate_mipmaps=False) -> int:
         rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
         h, w = rgba.shape[:2]
-        return self.lib.lib.rhx_add_texture(self.h, _ptr(rgba), w, h, int(is_srgb), int(generate_mipmaps))
+        return self._added(self.lib.lib.rhx_add_texture(self.h, _ptr(rgba), w, h, int(is_srgb), int(generate_mipmaps)),
+                           "rhx_add_texture")
 
     def add_material(self, type, main_texture, main_color=(1.0, 1.0, 1.0), normal_map=INVALID,
                      mix_materials=(INVALID, INVALID), roughness=0.0, strength=1.0, fresnel=1.0, int_ior=1.0,
                      ext_ior=1.0) -> int:
         m = MatDesc(type, (c_float * 3)(*main_color), main_texture, normal_map, (c_uint32 * 2)(*mix_materials),
                     roughness, strength, fresnel, int_ior, ext_ior)
-        return self.lib.lib.rhx_add_material(self.h, C.byref(m))
+        return self._added(self.lib.lib.rhx_add_material(self.h, C.byref(m)), "rhx_add_material")
 
     def add_mesh(self, layout, attrs: np.ndarray, indices: np.ndarray, shapes, base_vertex=0,
                  allow_spatial_splits=False, use_fast_bvh_build=False) -> int:
@@ -577,9 +586,10 @@ class Scene:
             else:
                 sh[i] = ShapeDesc(s[0], s[1], s[2], s[3])
         n_vtx = attrs.size // ATTR_STRIDES[layout]
-        return self.lib.lib.rhx_add_mesh(self.h, layout, _ptr(attrs), n_vtx, _ptr(indices), indices.size, base_vertex,
-                                         C.cast(sh, c_void_p), len(shapes),
-                                         int(bool(allow_spatial_splits)) | (2 if use_fast_bvh_build else 0))
+        return self._added(self.lib.lib.rhx_add_mesh(self.h, layout, _ptr(attrs), n_vtx, _ptr(indices), indices.size,
+                                                     base_vertex, C.cast(sh, c_void_p), len(shapes),
+                                                     int(bool(allow_spatial_splits)) | (2 if use_fast_bvh_build else 0)),
+                           "rhx_add_mesh")
 
     def remove_mesh(self, i):
         self.lib.lib.rhx_remove_mesh(self.h, i)

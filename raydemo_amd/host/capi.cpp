@@ -285,12 +285,15 @@ int rhx_read_accum(rhx_renderer *r, float *dst, size_t count) {
     });
 }
 
-void rhx_set_environment(rhx_scene *s, const float env_col[3], float env_clamp, uint32_t env_map) {
-    Ray::environment_desc_t e;
-    memcpy(e.env_col, env_col, 3 * sizeof(float));
-    e.env_clamp = env_clamp;
-    e.env_map = env_map;
-    s->s->SetEnvironment(e);
+int rhx_set_environment(rhx_scene *s, const float env_col[3], float env_clamp, uint32_t env_map) {
+    return guarded([&] {
+        Ray::environment_desc_t e;
+        memcpy(e.env_col, env_col, 3 * sizeof(float));
+        e.env_clamp = env_clamp;
+        e.env_map = env_map;
+        s->s->SetEnvironment(e);
+        return 0;
+    });
 }
 
 uint32_t rhx_add_texture(rhx_scene *s, const uint8_t *rgba, int w, int h, int is_srgb, int gen_mips) {
@@ -316,7 +319,12 @@ uint32_t rhx_add_material(rhx_scene *s, const rhx_mat_desc *m) {
     d.fresnel = m->fresnel;
     d.int_ior = m->int_ior;
     d.ext_ior = m->ext_ior;
-    return s->s->AddMaterial(d);
+    try {
+        return s->s->AddMaterial(d);
+    } catch (const std::exception &e) {
+        g_err = e.what(); // an index that names no texture or material: nothing is added
+        return 0xffffffffu;
+    }
 }
 
 uint32_t rhx_add_mesh(rhx_scene *s, int layout, const float *attrs, size_t attrs_count, const uint32_t *indices,

@@ -918,15 +918,27 @@ void ComputeTangentBasis(size_t vtx_offset, size_t vtx_start, std::vector<rh_ver
     }
 }
 
+// 2x2 box filter of TextureUtilsRef.cpp:9-37, over the row-major texels with
+// the reference's indexing: for an odd width the last pair of a row takes the
+// next row's first texel, and for an odd size the output holds ceil(w/2) x
+// ceil(h/2) texels while the next level is read as floor(w/2) x floor(h/2),
+// as in the reference.  Where those reads run past the end of the input (the
+// last row pair of an odd height) the reference reads outside its vector; here
+// such a read takes the texel one row up instead (the last row repeated).
 std::vector<uint32_t> DownsampleTexture(const std::vector<uint32_t> &tex, const int res[2]) {
     if (res[0] == 1 || res[1] == 1) return tex;
     std::vector<uint32_t> ret;
     ret.reserve((size_t)res[0] * res[1] / 4);
     auto ch = [](uint32_t p, int c) { return (int)((p >> (8 * c)) & 0xff); };
+    const size_t n = tex.size(), row = (size_t)res[0];
+    auto at = [&](size_t k) {
+        while (k >= n && k >= row) k -= row;
+        return k < n ? tex[k] : 0u;
+    };
     for (int j = 0; j < res[1]; j += 2) {
         for (int i = 0; i < res[0]; i += 2) {
-            const uint32_t a = tex[(size_t)(j + 0) * res[0] + i], b = tex[(size_t)(j + 0) * res[0] + i + 1],
-                           c = tex[(size_t)(j + 1) * res[0] + i], d = tex[(size_t)(j + 1) * res[0] + i + 1];
+            const uint32_t a = at((size_t)(j + 0) * res[0] + i), b = at((size_t)(j + 0) * res[0] + i + 1),
+                           c = at((size_t)(j + 1) * res[0] + i), d = at((size_t)(j + 1) * res[0] + i + 1);
             uint32_t out = 0;
             for (int k = 0; k < 4; k++) {
                 const int s = ch(a, k) + ch(b, k) + ch(c, k) + ch(d, k);

@@ -88,6 +88,10 @@ class Scene : public SceneBase {
     void RebuildMacroBVH();
     void RebuildLightBVH();
     int AllocateInAtlas(const int res[2], int pos[2]);
+    // std::invalid_argument unless tex (mat) names an existing texture
+    // (material), or is 0xffffffff where optional
+    void CheckTexture(uint32_t tex, bool optional, const char *what) const;
+    void CheckMaterial(uint32_t mat, bool optional, const char *what) const;
 
 public:
     explicit Scene(bool use_wide_bvh = false, bool device_build = false, int device = -1);

@@ -7,6 +7,8 @@
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
+#include <string>
+#include <vector>
 
 #include "hip_backend.h"
 
@@ -128,7 +130,29 @@ void Scene::GetEnvironment(environment_desc_t &env) {
     env.env_map = env_.env_map;
 }
 
+// The reference stores the indices a caller passes as they are and the
+// renderers read through them; an index past the scene's textures or
+// materials reads outside the device arrays.  They are checked here instead
+// (std::invalid_argument), with the sentinel 0xffffffff allowed where the
+// reference gives it a meaning.
+void Scene::CheckTexture(uint32_t tex, bool optional, const char *what) const {
+    if (optional && tex == 0xffffffff) return;
+    if (tex >= textures_.size()) {
+        throw std::invalid_argument(std::string(what) + ": texture " + std::to_string(tex) + " does not exist (" +
+                                    std::to_string(textures_.size()) + " textures)");
+    }
+}
+
+void Scene::CheckMaterial(uint32_t mat, bool optional, const char *what) const {
+    if (optional && mat == 0xffffffff) return;
+    if (mat >= materials_.size()) {
+        throw std::invalid_argument(std::string(what) + ": material " + std::to_string(mat) + " does not exist (" +
+                                    std::to_string(materials_.size()) + " materials)");
+    }
+}
+
 void Scene::SetEnvironment(const environment_desc_t &env) {
+    CheckTexture(env.env_map, true, "SetEnvironment env_map");
     memcpy(env_.env_col, env.env_col, 3 * sizeof(float));
     env_.env_clamp = env.env_clamp;
     env_.env_map = env.env_map == 0xffffffff ? default_env_texture_ : env.env_map;
@@ -167,7 +191,7 @@ void Scene::WriteBlock(const rh_tex_mip &m, const std::vector<uint32_t> &data, c
 bool Scene::SetTextureData(uint32_t tex, const uint32_t *rgba, uint32_t &first, uint32_t &count) {
     if (tex >= textures_.size()) return false;
     const rh_texture &t = textures_[tex];
-    int res[2] = {t.width & RH_TEXTURE_WIDTH_BITS, t.height};
+    int res[2] = {(int)(t.width & RH_TEXTURE_WIDTH_BITS), (int)t.height};
     std::vector<uint32_t> data(rgba, rgba + (size_t)res[0] * res[1]);
     const rh_tex_mip *mips = &tex_mips_[(size_t)tex * RH_NUM_MIP_LEVELS];
     first = mips[0].offset;
@@ -237,6 +261,15 @@ uint32_t Scene::AddTexture(const tex_desc_t &_t) {
 }
 
 uint32_t Scene::AddMaterial(const mat_desc_t &m) {
+    if ((int)m.type < (int)DiffuseMaterial || (int)m.type > (int)TransparentMaterial) {
+        throw std::invalid_argument("AddMaterial: unknown material type " + std::to_string((int)m.type));
+    }
+    CheckTexture(m.main_texture, false, "AddMaterial main_texture");
+    CheckTexture(m.normal_map, true, "AddMaterial normal_map");
+    if (m.type == MixMaterial) {
+        CheckMaterial(m.mix_materials[0], false, "AddMaterial mix_materials[0]");
+        CheckMaterial(m.mix_materials[1], false, "AddMaterial mix_materials[1]");
+    }
     rh_material mat;
     memset(&mat, 0, sizeof(mat));
     mat.type = m.type;
@@ -298,6 +331,25 @@ Scene::~Scene() {
 
 uint32_t Scene::AddMesh(const mesh_desc_t &_m) {
     if (!_m.vtx_indices_count || _m.vtx_indices_count % 3) return 0xffffffff;
+    if ((int)_m.layout < (int)PxyzNxyzTuv || (int)_m.layout > (int)PxyzNxyzBxyzTuvTuv) {
+        throw std::invalid_argument("AddMesh: unknown vertex layout " + std::to_string((int)_m.layout));
+    }
+    for (size_t i = 0; i < _m.vtx_indices_count; i++) {
+        const int64_t v = (int64_t)_m.vtx_indices[i] + _m.base_vertex;
+        if (v < 0 || v >= (int64_t)_m.vtx_attrs_count) {
+            throw std::invalid_argument("AddMesh: index " + std::to_string(i) + " names vertex " + std::to_string(v) +
+                                        " of " + std::to_string(_m.vtx_attrs_count));
+        }
+    }
+    for (const shape_desc_t &sh : _m.shapes) {
+        CheckMaterial(sh.mat_index, false, "AddMesh shape mat_index");
+        CheckMaterial(sh.back_mat_index, true, "AddMesh shape back_mat_index");
+        if (sh.vtx_start > _m.vtx_indices_count || sh.vtx_count > _m.vtx_indices_count - sh.vtx_start) {
+            throw std::invalid_argument("AddMesh: shape range [" + std::to_string(sh.vtx_start) + ", +" +
+                                        std::to_string(sh.vtx_count) + ") past the " +
+                                        std::to_string(_m.vtx_indices_count) + " indices");
+        }
+    }
     meshes_.emplace_back();
     rh_mesh &m = meshes_.back();
     const uint32_t tris_start = (uint32_t)tris_.size();
@@ -323,16 +375,23 @@ uint32_t Scene::AddMesh(const mesh_desc_t &_m) {
     }
 
     // triangle materials and the solid bit (no transparent material reachable)
+    // (SceneRef.cpp:185-203 walks the mix tree with a 32-entry stack; here the
+    // walk has no depth limit and visits each material once)
+    std::vector<uint32_t> material_stack;
+    std::vector<uint8_t> seen;
     for (const shape_desc_t &sh : _m.shapes) {
         bool is_solid = true;
-        uint32_t material_stack[32];
-        material_stack[0] = sh.mat_index;
-        uint32_t material_count = 1;
-        while (material_count) {
-            const rh_material &mat = materials_[material_stack[--material_count]];
+        material_stack.assign(1, sh.mat_index);
+        seen.assign(materials_.size(), 0);
+        while (!material_stack.empty()) {
+            const uint32_t mi = material_stack.back();
+            material_stack.pop_back();
+            if (seen[mi]) continue;
+            seen[mi] = 1;
+            const rh_material &mat = materials_[mi];
             if (mat.type == MixMaterial) {
-                material_stack[material_count++] = mat.textures[RH_MIX_MAT1];
-                material_stack[material_count++] = mat.textures[RH_MIX_MAT2];
+                material_stack.push_back(mat.textures[RH_MIX_MAT1]);
+                material_stack.push_back(mat.textures[RH_MIX_MAT2]);
             } else if (mat.type == TransparentMaterial) {
                 is_solid = false;
                 break;
