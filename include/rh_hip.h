@@ -76,7 +76,7 @@ int rh_get_ray_counts(rh_context *ctx, rh_ray_counts *out);
  * "tail_occupancy" (2 or 3 waves per SIMD for the path-tail kernel: registers vs spills),
  * "tail_refill" (0/1: the path tail as a persistent kernel whose lanes take the next path when theirs ends),
  * "refill" (0/1: closest-hit trace as a persistent kernel whose lanes take new rays as theirs finish;
- *   binary trees only),
+ *   binary and 8-wide trees, the latter with the trace_occupancy register cap),
  * "trace_occupancy" (0, 5 or 6: the 8-wide trace and shadow kernels as compiled, or capped at 5 or 6 waves
  *   per SIMD),
  * "shadow_occupancy" (-1, the default: as trace_occupancy; 0, 5 or 6: the 8-wide shadow kernel's own setting),

@@ -590,13 +590,16 @@ __device__ __forceinline__ bool wide_step(const float4 *wn, uint32_t node, f3 o,
                                           WideCur &next) {
     const float4 *p = wn + (size_t)node * 16;
     float dist[8];
-    uint32_t word[8], count[8];
+    uint32_t word[8];
     uint32_t mask = 0;
     static_for<8>([&](auto S) {
         constexpr int s = S;
-        const float4 a = p[2 * s], b = p[2 * s + 1];
+        // bbox_max without the count word: the count is read below for the
+        // entered slot only, when it is a leaf (8 registers fewer in flight)
+        const float4 a = p[2 * s];
+        const float3 bm = *reinterpret_cast<const float3 *>(p + 2 * s + 1);
+        const float4 b = make_float4(bm.x, bm.y, bm.z, 0.0f);
         word[s] = __float_as_uint(a.w);
-        count[s] = __float_as_uint(b.w);
         const bool h = slot_hit<Finite>(o, inv_d, t, a, b, dist[s]);
         if (h && word[s] != RH_MBVH_EMPTY) mask |= 1u << s;
     });
@@ -687,7 +690,8 @@ __device__ __forceinline__ bool wide_step(const float4 *wn, uint32_t node, f3 o,
         else st.sp = st.lds_depth + st.spill_depth;
     }
     next.word = pick8(word, top);
-    next.count = pick8(count, top);
+    next.count = (next.word & RH_LEAF_NODE_BIT) ? __float_as_uint(reinterpret_cast<const float *>(p + 2 * top + 1)[3])
+                                                : 0u;
     return true;
 }
 

@@ -279,6 +279,13 @@ template <bool Count> __global__ void __launch_bounds__(kBlock) k_trace_refill(T
     trace_refill_body<Count>(a, next_ray, lds_stack);
 }
 
+// the same for 8-wide trees (trace_refill_wide_body); MinBlocks as in k_trace
+template <bool Count, int MinBlocks>
+__global__ void __launch_bounds__(kBlock, MinBlocks) k_trace_refill_wide(TraceArgs a, int *next_ray) {
+    extern __shared__ uint32_t lds_stack[];
+    trace_refill_wide_body<Count>(a, next_ray, lds_stack);
+}
+
 // Defer: direct light goes to the NEE stage (no traversal in this kernel).
 template <bool Defer> __global__ void __launch_bounds__(kBlock) k_shade(ShadeArgs a) {
     extern __shared__ uint32_t lds_stack[];
@@ -780,7 +787,6 @@ struct rh_context {
     int *adj_nverts = nullptr;        // staged replay: vertices recorded per lane
     size_t adj_nverts_cap = 0;
     int cus = 0;               // compute units of the device
-    int refill_blocks = 0, refill_blocks_lds = -1;
     int adjoint_occupancy = 2; // the same for the adjoint replay kernel
     int lds_depth = 24;
     // stage timers (RendererBase::stats_t, in the order of rh_stats): every
@@ -981,24 +987,40 @@ int resident_blocks(rh_context *c, const void *kern, size_t lds, int *out) {
 int launch_trace(rh_context *c, const TraceArgs &a, int grid_rays, bool any_hit) {
     const int blocks = std::max(1, (grid_rays + kBlock - 1) / kBlock);
     const bool count = c->count_visits && a.visits;
-    if (!any_hit && c->refill && !c->sc.wnodes) {
+    if (!any_hit && c->refill) {
         // persistent grid: as many blocks as fit on the device at once (and no
         // more than the rays need); spill slices exist for every lane of them
-        if (c->refill_blocks_lds != c->lds_depth) {
-            int per_cu = 0;
-            RH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_refill<false>, kBlock, lds_bytes(c)));
-            c->refill_blocks = std::max(1, per_cu) * c->cus;
-            c->refill_blocks_lds = c->lds_depth;
-        }
-        const int grid = std::min(blocks, c->refill_blocks);
+        const bool wide = c->sc.wnodes != nullptr;
+        const int occ = wide ? c->trace_occupancy : 0;
+        const void *kern = !wide      ? (const void *)k_trace_refill<false>
+                           : occ == 5 ? (const void *)k_trace_refill_wide<false, 5>
+                           : occ == 6 ? (const void *)k_trace_refill_wide<false, 6>
+                                      : (const void *)k_trace_refill_wide<false, 1>;
+        int per_cu = 0;
+        if (const int rc = resident_blocks(c, kern, lds_bytes(c), &per_cu)) return rc;
+        const int grid = std::min(blocks, per_cu * c->cus);
         hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, c->counters + 3, 0);
         KernelMark km(c, 0);
-        if (count)
-            hipLaunchKernelGGL((k_trace_refill<true>), dim3(grid), dim3(kBlock), lds_bytes(c), c->stream, a,
-                               c->counters + 3);
-        else
-            hipLaunchKernelGGL((k_trace_refill<false>), dim3(grid), dim3(kBlock), lds_bytes(c), c->stream, a,
-                               c->counters + 3);
+        const dim3 g(grid), blk(kBlock);
+        const size_t lds = lds_bytes(c);
+        int *next = c->counters + 3;
+        if (!wide) {
+            if (count) hipLaunchKernelGGL((k_trace_refill<true>), g, blk, lds, c->stream, a, next);
+            else hipLaunchKernelGGL((k_trace_refill<false>), g, blk, lds, c->stream, a, next);
+        } else if (count) {
+            // the instrumented walk visits what the plain one does; its grid
+            // is sized by its own occupancy
+            int pc = 0;
+            if (const int rc = resident_blocks(c, (const void *)k_trace_refill_wide<true, 1>, lds, &pc)) return rc;
+            hipLaunchKernelGGL((k_trace_refill_wide<true, 1>), dim3(std::min(blocks, pc * c->cus)), blk, lds,
+                               c->stream, a, next);
+        } else if (occ == 5) {
+            hipLaunchKernelGGL((k_trace_refill_wide<false, 5>), g, blk, lds, c->stream, a, next);
+        } else if (occ == 6) {
+            hipLaunchKernelGGL((k_trace_refill_wide<false, 6>), g, blk, lds, c->stream, a, next);
+        } else {
+            hipLaunchKernelGGL((k_trace_refill_wide<false, 1>), g, blk, lds, c->stream, a, next);
+        }
         RH_CHECK(hipGetLastError());
         return 0;
     }

@@ -438,6 +438,169 @@ __device__ __forceinline__ void trace_refill_body(const TraceArgs &a, int *next_
     }
 }
 
+// trace_refill_body for 8-wide trees: traverse_wide's closest-hit walk cut
+// into single steps (one wide node, one instance, one leaf or one pop per loop
+// iteration), so each ray makes the same wide_step / wide_pop / intersect_leaf
+// calls in the same order as in traverse_wide and gets the same hit.  The
+// node step is shared by the two levels: (co, cd, cinv) is the ray of the
+// level the lane is in (the macro ray, or the instance's transformed one), so
+// lanes at different levels run one wide_step together.
+template <bool Count>
+__device__ __forceinline__ void trace_refill_wide_body(const TraceArgs &a, int *next_ray, uint32_t *lds_stack) {
+    const int n = a.count ? *a.count : a.n;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.ray_counter) atomicAdd(a.ray_counter, (unsigned long long)n);
+    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x; // persistent lane: owns spill slice gl
+    Stack st;
+    st.init(lds_stack, a.spill + gl, a.spill_stride, a.lds_depth, kSpillDepth);
+    TraceCounters tc = {0, 0, 0};
+    TraceCounters *cnt = Count ? &tc : nullptr;
+    const float4 *wn = a.sc.wnodes;
+
+    int r = -1;                   // ray of this lane
+    uint32_t step = kStepIdle;    // kStepMacro / kStepMicro: a node level; kStepInstances: a macro leaf's instances
+    f3 o = f3{0, 0, 0}, d = o, inv_d = o;  // the ray
+    f3 co = o, cd = o, cinv = o;           // the ray at the current level
+    float xy = 0.0f;
+    Hit hit;
+    hit.mask = 0;
+    hit.obj = hit.prim = -1;
+    hit.t = 0.0f;
+    hit.u = hit.v = 0.0f;
+    WideCur cur = {0, 0};         // node or leaf to process when `have`
+    bool have = false;            // else the next step pops
+    uint32_t leaf_cur = 0, leaf_end = 0, base = 0;
+    int mi_index = 0;
+    bool finite = true;
+    bool exhausted = false;       // the queue has no rays left for this wave
+
+    for (;;) {
+        // ---- refill
+        const uint64_t idle = __ballot(step == kStepIdle);
+        const uint64_t live = __ballot(true);
+        if (!exhausted && idle && (__popcll(idle) >= 16 || idle == live)) {
+            int first = 0;
+            const int want = __popcll(idle);
+            if (lane_id() == (uint32_t)(__ffsll((unsigned long long)idle) - 1)) first = atomicAdd(next_ray, want);
+            first = __shfl(first, __ffsll((unsigned long long)idle) - 1);
+            if (first + want >= n) exhausted = true;
+            if (step == kStepIdle) {
+                r = first + lanes_below(idle);
+                if (r < n) {
+                    const float4 *p = reinterpret_cast<const float4 *>(a.rays + r);
+                    const float4 r0 = p[0], r1 = p[1];
+                    o = f3{r0.x, r0.y, r0.z};
+                    d = f3{r1.x, r1.y, r1.z};
+                    xy = r0.w;
+                    inv_d = safe_invert(d);
+                    co = o;
+                    cd = d;
+                    cinv = inv_d;
+                    finite = true;
+                    base = 0;
+                    hit.mask = 0;
+                    hit.obj = -1;
+                    hit.prim = -1;
+                    hit.t = a.tmax ? a.tmax[r] : RH_MAX_DIST;
+                    hit.u = hit.v = 0.0f;
+                    st.sp = 0;
+                    if (a.sc.macro_root == RH_INVALID) {
+                        // empty scene: the miss is written now
+                        float4 *h = reinterpret_cast<float4 *>(a.hits + r);
+                        h[0] = make_float4(__int_as_float(hit.mask), __int_as_float(hit.obj), __int_as_float(hit.prim),
+                                           hit.t);
+                        h[1] = make_float4(hit.u, hit.v, xy, 0.0f);
+                        if (Count) atomicAdd(a.visits + 3, 1ull);
+                    } else {
+                        cur = wide_root(wn, a.sc.macro_root);
+                        have = true;
+                        step = kStepMacro;
+                    }
+                }
+            }
+        }
+        if (!__ballot(step != kStepIdle)) {
+            if (exhausted) break;
+            continue;
+        }
+        // ---- one traversal step, in three phases so that a lane can enter an
+        // instance, or pop, and process the node it arrives at in the same
+        // iteration (the wave runs each phase once for all its lanes)
+        bool finished = false;
+        // (1) the next instance of a macro leaf
+        if (step == kStepInstances) {
+            if (leaf_cur < leaf_end) {
+                mi_index = (int)a.sc.mi_indices[leaf_cur++];
+                const rh_mesh_instance mi = a.sc.mesh_instances[mi_index];
+                if (cnt) cnt->instances++;
+                if (box_hit<true>(o, inv_d, hit.t, mi.bbox_min, mi.bbox_max)) {
+                    const float *x = a.sc.transforms[mi.tr_index].inv_xform;
+                    co = f3{x[0] * o.x + x[4] * o.y + x[8] * o.z + x[12], x[1] * o.x + x[5] * o.y + x[9] * o.z + x[13],
+                            x[2] * o.x + x[6] * o.y + x[10] * o.z + x[14]};
+                    cd = f3{x[0] * d.x + x[4] * d.y + x[8] * d.z, x[1] * d.x + x[5] * d.y + x[9] * d.z,
+                            x[2] * d.x + x[6] * d.y + x[10] * d.z};
+                    cinv = f3{1.0f / cd.x, 1.0f / cd.y, 1.0f / cd.z};
+                    finite = __builtin_isfinite(cinv.x) && __builtin_isfinite(cinv.y) && __builtin_isfinite(cinv.z);
+                    base = st.sp;
+                    cur = wide_root(wn, a.sc.meshes[mi.mesh_index].node_index);
+                    have = true;
+                    step = kStepMicro;
+                }
+            } else {
+                step = kStepMacro; // the leaf is done: pop
+                have = false;
+            }
+        }
+        // (2) pop
+        if ((step == kStepMacro || step == kStepMicro) && !have) {
+            have = wide_pop(wn, base, hit.t, st, cur);
+            if (!have) {
+                if (step == kStepMicro) {
+                    // the instance is done: back to the macro leaf's next one
+                    co = o;
+                    cd = d;
+                    cinv = inv_d;
+                    finite = true;
+                    base = 0;
+                    step = kStepInstances;
+                } else {
+                    finished = true;
+                }
+            }
+        }
+        // (3) the node or leaf the lane is at
+        if ((step == kStepMacro || step == kStepMicro) && have) {
+            if (!(cur.word & RH_LEAF_NODE_BIT)) {
+                if (cnt) cnt->nodes++;
+                have = finite ? wide_step<true>(wn, cur.word, co, cinv, hit.t, st, cur)
+                              : wide_step<false>(wn, cur.word, co, cinv, hit.t, st, cur);
+            } else if (step == kStepMicro) {
+                NodeBox leaf;
+                leaf.a = cur.word;
+                leaf.b = cur.count;
+                intersect_leaf<false>(a.sc, leaf, co, cd, mi_index, hit, cnt);
+                have = false;
+            } else {
+                leaf_cur = cur.word & RH_PRIM_INDEX_BITS;
+                leaf_end = leaf_cur + cur.count;
+                step = kStepInstances;
+            }
+        }
+        if (finished) {
+            float4 *h = reinterpret_cast<float4 *>(a.hits + r);
+            h[0] = make_float4(__int_as_float(hit.mask), __int_as_float(hit.obj), __int_as_float(hit.prim), hit.t);
+            h[1] = make_float4(hit.u, hit.v, xy, 0.0f);
+            if (Count) atomicAdd(a.visits + 3, 1ull);
+            step = kStepIdle;
+        }
+    }
+    if (st.overflow && a.overflow) atomicAdd(a.overflow, (unsigned long long)st.overflow);
+    if (Count) {
+        atomicAdd(a.visits + 0, (unsigned long long)tc.nodes);
+        atomicAdd(a.visits + 1, (unsigned long long)tc.leaf_tris);
+        atomicAdd(a.visits + 2, (unsigned long long)tc.instances);
+    }
+}
+
 // Per-lane vertex records of the path-replay adjoint (lane-minor: record k of
 // lane i at k * lanes + i): rec = (C_k, material whose main_color scales it);
 // rec_tex (when texel gradients are taken) = 3 records per vertex: the

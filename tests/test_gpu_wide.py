@@ -26,8 +26,13 @@ BUILDS = {
 }
 
 
+@pytest.mark.parametrize("knobs", [{}, {"refill": 1}, {"refill": 1, "trace_occupancy": 5}],
+                         ids=["plain", "refill", "refill_occ5"])
 @pytest.mark.parametrize("name", sorted(BUILDS))
-def test_wide_trace_bit_exact_and_visits(ref_lib, host_lib, dev_lib, oracle, golden_scene1, name):
+def test_wide_trace_bit_exact_and_visits(ref_lib, host_lib, dev_lib, oracle, golden_scene1, name, knobs):
+    """Closest-hit and any-hit traces equal the reference library's, and the
+    visit counters the oracle's; with refill the closest-hit walk runs as the
+    persistent lane-refill kernel (the instrumented and the plain one)."""
     rs, hs = standalone_scene(ref_lib, wide=True), standalone_scene(host_lib, wide=True)
     BUILDS[name](rs, golden_scene1)
     BUILDS[name](hs, golden_scene1)
@@ -35,6 +40,8 @@ def test_wide_trace_bit_exact_and_visits(ref_lib, host_lib, dev_lib, oracle, gol
     rays = ref_gen_primary(ref_lib, rs.camera_record(0), (0, 0, w, h), w, h, 1, ref_halton(ref_lib, 1))
     view = hs.raw_view()
     ctx = Context(dev_lib, w=w, h=h)
+    for k, v in knobs.items():
+        ctx.set_option(k, v)
     ctx.set_option("count_visits", 1)
     ctx.upload(view)
     hr = ref_trace(ref_lib, rs, rays)
@@ -47,6 +54,9 @@ def test_wide_trace_bit_exact_and_visits(ref_lib, host_lib, dev_lib, oracle, gol
         _, cnt = oracle.trace(view, rs_, counters=True)
         assert counted == len(rs_)
         assert (nodes, tris, inst) == tuple(int(c) for c in cnt), f"{what}: {(nodes, tris, inst)} vs {cnt}"
+        ctx.set_option("count_visits", 0)
+        _check_hits(ctx.trace(rs_), ref_trace(ref_lib, rs, rs_), what + " (uninstrumented)")
+        ctx.set_option("count_visits", 1)
     tmax = np.full(len(sec), 6.0, np.float32)
     _check_hits(ctx.trace(sec, any_hit=True, tmax=tmax), ref_trace(ref_lib, rs, sec, True, tmax), "any-hit")
 
@@ -71,7 +81,9 @@ def _rmse(a, b):
 
 @pytest.mark.parametrize("opts", [{}, {"sort_rays": 1, "lds_stack_depth": 8, "path_tail": -1}, {"defer_nee": 0},
                                   {"path_tail": 0, "tail_occupancy": 3}, {"refill": 1},
-                                  {"trace_occupancy": 5, "lds_stack_depth": 8, "path_tail": 1}])
+                                  {"trace_occupancy": 5, "lds_stack_depth": 8, "path_tail": 1},
+                                  {"refill": 1, "trace_occupancy": 6, "lds_stack_depth": 4, "path_tail": -1,
+                                   "sort_rays": 1}])
 def test_wide_render_matches_oracle(oracle, host_lib, golden_scene1, opts):
     """Image and ray counts of the wide-tree renderer equal the oracle's wide
     render of the same passes."""
