@@ -218,6 +218,22 @@ int rh_build_lbvh(rh_builder *b, const float *prims, size_t n, uint32_t index_of
                   rh_bvh_node *out_nodes, size_t nodes_cap, uint32_t *out_node_count, uint32_t *out_treelets,
                   uint32_t *out_treelet_count);
 
+/* ---- the SAH build on the device ----
+ * PreprocessPrims_SAH (Core.cpp:431-530) with SplitPrimitives_SAH's object
+ * split (BVHSplit.cpp:126-218; no spatial splits) over n boxes (6 floats
+ * each, as above), with bvh_settings_t's oversplit_threshold and
+ * node_traversal_cost: out_nodes gets the tree, node for node the host's,
+ * root first, child links relative to out_nodes[0] (*out_node_count nodes,
+ * nodes_cap >= 2 n), and out_indices[n] the box indices its leaves cover
+ * (leaf prim_index = RH_LEAF_NODE_BIT | offset into out_indices).  Used for
+ * the instance tree (RebuildMacroBVH, SceneRef.cpp:492-516) and the fast
+ * build's top tree (Core.cpp:598-680).  Where two boxes share a bbox_max
+ * coordinate within a node, or a coordinate is not finite, the host's
+ * std::sort order cannot be reproduced: *host_needed = 1, nothing is written
+ * and the caller builds on the host. */
+int rh_build_sah(rh_builder *b, const float *boxes, size_t n, float oversplit, float cost, rh_bvh_node *out_nodes,
+                 size_t nodes_cap, uint32_t *out_node_count, uint32_t *out_indices, int *host_needed);
+
 #ifdef __cplusplus
 }
 #endif

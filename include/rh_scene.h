@@ -152,6 +152,15 @@ void rhx_destroy_scene(rhx_scene *s);
  * API (it exposes that build for any boxes). */
 int rhx_build_fast_bvh(const float *boxes, size_t n, int device, rh_bvh_node *out_nodes, size_t nodes_cap,
                        uint32_t *out_count, uint32_t *out_indices);
+/* PreprocessPrims_SAH (Core.cpp:431-530) without spatial splits over n boxes
+   (6 floats each: bbox_min, bbox_max) with bvh_settings_t's
+   oversplit_threshold and node_traversal_cost, as the instance tree
+   (RebuildMacroBVH, SceneRef.cpp:492-516) is built: on the host (device < 0)
+   or on GPU `device` (rh_build_sah), *on_device telling which built it (the
+   host builds where boxes share a bbox_max coordinate).  Same outputs as
+   rhx_build_fast_bvh. */
+int rhx_build_sah(const float *boxes, size_t n, int device, float oversplit, float cost, rh_bvh_node *out_nodes,
+                  size_t nodes_cap, uint32_t *out_count, uint32_t *out_indices, int *on_device);
 
 int rhx_region_create(int x, int y, int w, int h, rhx_region **out);
 void rhx_region_destroy(rhx_region *reg);

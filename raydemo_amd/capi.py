@@ -229,6 +229,8 @@ class RayLib:
             "rhx_read_accum": (c_int, [P, P, c_size_t]),
             "rhx_set_gradient_textures": (c_int, [P, P, c_uint32]),
             "rhx_build_fast_bvh": (c_int, [P, c_size_t, c_int, P, c_size_t, POINTER(c_uint32), P]),
+            "rhx_build_sah": (c_int, [P, c_size_t, c_int, c_float, c_float, P, c_size_t, POINTER(c_uint32), P,
+                                      POINTER(c_int)]),
             "rhx_get_option": (c_int, [P, C.c_char_p, POINTER(c_int)]),
         }
         optional.update({
@@ -415,6 +417,22 @@ def build_fast_bvh(lib: RayLib, boxes: np.ndarray, device: int = -1):
     lib.check(lib.lib.rhx_build_fast_bvh(_ptr(boxes), n, int(device), _ptr(nodes), len(nodes), C.byref(count),
                                          _ptr(idx)), "rhx_build_fast_bvh")
     return nodes[:count.value], idx[:n]
+
+
+def build_sah(lib: RayLib, boxes: np.ndarray, device: int = -1, oversplit: float = 0.95, cost: float = 0.025):
+    """The SAH tree (object splits) over boxes (n x 6), as the instance tree is
+    built, on the host (device < 0) or on GPU `device`: (nodes, indices,
+    on_device), nodes in NODE_DTYPE with the root first; on_device is False
+    where the device declined (tied keys) and the host built it."""
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    n = len(boxes)
+    nodes = np.zeros(max(1, 2 * n), NODE_DTYPE)
+    idx = np.zeros(max(1, n), np.uint32)
+    count = c_uint32(0)
+    on_device = c_int(0)
+    lib.check(lib.lib.rhx_build_sah(_ptr(boxes), n, int(device), float(oversplit), float(cost), _ptr(nodes), len(nodes),
+                                    C.byref(count), _ptr(idx), C.byref(on_device)), "rhx_build_sah")
+    return nodes[:count.value], idx[:n], bool(on_device.value)
 
 
 def standalone_scene(lib: RayLib, wide: bool = False) -> "Scene":

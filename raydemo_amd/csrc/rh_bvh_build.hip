@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -374,6 +375,366 @@ __global__ void __launch_bounds__(kBlock) k_lbvh_treelet_out(const Task *__restr
     out[3 * j + 2] = base[j];
 }
 
+// ------------------------------------------------------------------ SAH build
+// PreprocessPrims_SAH (Core.cpp:431-530) with SplitPrimitives_SAH's object
+// split (BVHSplit.cpp:126-218, no spatial splits) over n boxes, on the device,
+// node for node equal to the host's.  The host takes its lists from a deque in
+// creation order, so the nodes come out breadth first and a split node's
+// children take the next two slots: the k-th interior node in that order has
+// its children at 2k+1 and 2k+2, and the leaves' primitives follow one
+// another in node order.  Here each level of the tree is one launch with one
+// workgroup per node: per axis it sorts the node's primitives by bbox_max,
+// scans their boxes from both ends and evaluates every split position; a
+// second launch numbers the level's children and leaves in node order.
+//
+// The host orders each axis with std::sort, whose order of equal keys this
+// sort cannot reproduce: a node whose keys tie along any axis raises a flag
+// and the caller builds on the host instead.
+constexpr int kSahThreads = 1024;
+constexpr uint32_t kSahLdsKeys = 4096; // nodes of more primitives sort in global scratch
+
+struct SahTask {
+    uint32_t first, count; // range of the level's primitive order
+    uint32_t node;         // node index
+    float bmin[3], bmax[3]; // the node's box as the parent's split computed it
+};
+
+struct SahSplit {
+    int32_t axis;   // -1: a leaf
+    uint32_t index; // primitives going left
+    float lmin[3], lmax[3], rmin[3], rmax[3];
+};
+
+struct Box6 {
+    float v[6]; // min xyz, max xyz
+};
+
+__device__ __forceinline__ Box6 box_identity() {
+    Box6 b;
+    for (int k = 0; k < 3; k++) {
+        b.v[k] = 3.402823466e+38F;
+        b.v[3 + k] = -3.402823466e+38F;
+    }
+    return b;
+}
+// the running box `acc` grown by `x`, with std::min/std::max's choice between
+// equal values (the running value stays): associative, so scans may group
+__device__ __forceinline__ Box6 box_join(const Box6 &acc, const Box6 &x) {
+    Box6 r;
+    for (int k = 0; k < 3; k++) {
+        r.v[k] = min_std(acc.v[k], x.v[k]);
+        r.v[3 + k] = max_std(acc.v[3 + k], x.v[3 + k]);
+    }
+    return r;
+}
+__device__ __forceinline__ Box6 load_box(const float *boxes, uint32_t p) {
+    Box6 b;
+    for (int k = 0; k < 6; k++) b.v[k] = boxes[(size_t)p * 6 + k];
+    return b;
+}
+// bbox_t::surface_area (BVHSplit.cpp:8-23): 2 (dx + dy + dz)
+__device__ __forceinline__ float half_area(const float *mn, const float *mx) {
+    const float d0 = mx[0] - mn[0], d1 = mx[1] - mn[1], d2 = mx[2] - mn[2];
+    return 2 * (d0 + d1 + d2);
+}
+
+// ascending sort of m (key, value) pairs at key/val (room for the next power
+// of two), equal keys ordered by value; every thread of the block calls it
+__device__ void block_bitonic(float *key, uint32_t *val, uint32_t m) {
+    uint32_t np = 1;
+    while (np < m) np <<= 1;
+    for (uint32_t i = m + threadIdx.x; i < np; i += blockDim.x) {
+        key[i] = __builtin_inff();
+        val[i] = 0xffffffffu;
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= np; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const float ki = key[i], kl = key[l];
+                    const uint32_t vi = val[i], vl = val[l];
+                    const bool gt = ki > kl || (ki == kl && vi > vl);
+                    const bool up = (i & k) == 0;
+                    if (gt == up) {
+                        key[i] = kl;
+                        key[l] = ki;
+                        val[i] = vl;
+                        val[l] = vi;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Hillis-Steele scan of one Box6 per thread in LDS (6 floats each): forward,
+// each entry becomes the join of itself and all before it; backward, of
+// itself and all after it (the later entries' box being the running one)
+__device__ void block_scan_boxes(float *s, bool backward) {
+    const uint32_t t = threadIdx.x, T = blockDim.x;
+    for (uint32_t d = 1; d < T; d <<= 1) {
+        Box6 other, mine;
+        bool have = false;
+        const uint32_t o = backward ? t + d : t - d;
+        if (backward ? (t + d < T) : (t >= d)) {
+            for (int k = 0; k < 6; k++) other.v[k] = s[6 * o + k];
+            have = true;
+        }
+        for (int k = 0; k < 6; k++) mine.v[k] = s[6 * t + k];
+        __syncthreads();
+        if (have) {
+            // forward the earlier entries' box runs, backward the later ones'
+            const Box6 r = box_join(other, mine);
+            for (int k = 0; k < 6; k++) s[6 * t + k] = r.v[k];
+        }
+        __syncthreads();
+    }
+}
+
+// One workgroup per task of the level: the best object split of its range of
+// order_in (sorted orders, left/right bounds per axis in scratch), the chosen
+// axis's order (or, for a leaf, the incoming one) written to order_out.
+// flags[0] is set where keys tie.
+__global__ void __launch_bounds__(kSahThreads) k_sah_split(const float *__restrict__ boxes,
+                                                          const SahTask *__restrict__ tasks,
+                                                          const uint32_t *__restrict__ ntasks,
+                                                          const uint32_t *__restrict__ order_in,
+                                                          uint32_t *__restrict__ order_out, float *gkey,
+                                                          uint32_t *gval, uint32_t *sorted, float *lb, float *rb,
+                                                          size_t n, float oversplit, float cost,
+                                                          SahSplit *__restrict__ splits, uint32_t *flags) {
+    if (blockIdx.x >= *ntasks) return;
+    __shared__ float s_key[kSahLdsKeys];
+    __shared__ uint32_t s_val[kSahLdsKeys];
+    __shared__ float s_box[6 * kSahThreads];
+    __shared__ float s_sah[kSahThreads];
+    __shared__ uint32_t s_idx[kSahThreads];
+    __shared__ int s_tie;
+
+    const SahTask task = tasks[blockIdx.x];
+    const uint32_t first = task.first, m = task.count, t = threadIdx.x, T = blockDim.x;
+    const float sa_whole = half_area(task.bmin, task.bmax);
+    // a leaf unless a split beats this (the host's initial res_sah)
+    float best = oversplit * sa_whole * (float)m;
+    int best_axis = -1;
+    uint32_t best_index = 0;
+
+    const bool in_lds = m <= kSahLdsKeys;
+    float *key = in_lds ? s_key : gkey + 2 * (size_t)first;
+    uint32_t *val = in_lds ? s_val : gval + 2 * (size_t)first;
+    const uint32_t K = (m + T - 1) / T; // consecutive positions per thread
+    const uint32_t c0 = min(m, t * K), c1 = min(m, c0 + K);
+
+    for (int axis = 0; axis < 3 && m > 1; axis++) {
+        if (t == 0) s_tie = 0;
+        for (uint32_t i = t; i < m; i += T) {
+            const uint32_t p = order_in[first + i];
+            key[i] = boxes[(size_t)p * 6 + 3 + axis];
+            val[i] = p;
+        }
+        __syncthreads();
+        block_bitonic(key, val, m);
+        uint32_t *srt = sorted + (size_t)axis * n + first;
+        for (uint32_t i = t; i < m; i += T) {
+            srt[i] = val[i];
+            if (i + 1 < m && key[i] == key[i + 1]) s_tie = 1;
+        }
+        __syncthreads();
+        if (s_tie) {
+            // the host builds this tree; the node is written as a leaf so that
+            // the level's bookkeeping stays within its arrays
+            if (t == 0) atomicOr(flags, 1u);
+            best_axis = -1;
+            break;
+        }
+        // forward: left_i = join of sorted[0 .. i-1]
+        Box6 acc = box_identity();
+        for (uint32_t i = c0; i < c1; i++) acc = box_join(acc, load_box(boxes, srt[i]));
+        for (int k = 0; k < 6; k++) s_box[6 * t + k] = acc.v[k];
+        __syncthreads();
+        block_scan_boxes(s_box, false);
+        Box6 run = box_identity();
+        if (t > 0)
+            for (int k = 0; k < 6; k++) run.v[k] = s_box[6 * (t - 1) + k];
+        float *lba = lb + ((size_t)axis * n + first) * 6, *rba = rb + ((size_t)axis * n + first) * 6;
+        for (uint32_t i = c0; i < c1; i++) {
+            if (i >= 1)
+                for (int k = 0; k < 6; k++) lba[(size_t)i * 6 + k] = run.v[k];
+            run = box_join(run, load_box(boxes, srt[i]));
+        }
+        __syncthreads();
+        // backward: right(i) = join of sorted[m-1 .. i], the later ones running
+        acc = box_identity();
+        for (uint32_t i = c1; i > c0; i--) acc = box_join(acc, load_box(boxes, srt[i - 1]));
+        for (int k = 0; k < 6; k++) s_box[6 * t + k] = acc.v[k];
+        __syncthreads();
+        block_scan_boxes(s_box, true);
+        run = box_identity();
+        if (t + 1 < T)
+            for (int k = 0; k < 6; k++) run.v[k] = s_box[6 * (t + 1) + k];
+        for (uint32_t i = c1; i > c0; i--) {
+            run = box_join(run, load_box(boxes, srt[i - 1]));
+            if (i - 1 >= 1)
+                for (int k = 0; k < 6; k++) rba[(size_t)(i - 1) * 6 + k] = run.v[k];
+        }
+        // the split cost at every position: the first minimum wins
+        float my = __builtin_inff();
+        uint32_t my_i = 0xffffffffu;
+        for (uint32_t i = max(c0, 1u); i < c1; i++) {
+            const float *l = lba + (size_t)i * 6, *r = rba + (size_t)i * 6;
+            const float sah = cost * sa_whole + half_area(l, l + 3) * (float)i + half_area(r, r + 3) * (float)(m - i);
+            if (sah < my) {
+                my = sah;
+                my_i = i;
+            }
+        }
+        s_sah[t] = my;
+        s_idx[t] = my_i;
+        __syncthreads();
+        for (uint32_t d = T / 2; d > 0; d >>= 1) {
+            if (t < d) {
+                const float a = s_sah[t], b = s_sah[t + d];
+                const uint32_t ia = s_idx[t], ib = s_idx[t + d];
+                if (b < a || (b == a && ib < ia)) {
+                    s_sah[t] = b;
+                    s_idx[t] = ib;
+                }
+            }
+            __syncthreads();
+        }
+        if (s_idx[0] != 0xffffffffu && s_sah[0] < best) {
+            best = s_sah[0];
+            best_axis = axis;
+            best_index = s_idx[0];
+        }
+        __syncthreads();
+    }
+
+    // the children's order: the chosen axis's; a leaf keeps the incoming one
+    const uint32_t *src = best_axis >= 0 ? sorted + (size_t)best_axis * n + first : order_in + first;
+    for (uint32_t i = t; i < m; i += T) order_out[first + i] = src[i];
+    if (t == 0) {
+        SahSplit sp;
+        sp.axis = best_axis;
+        sp.index = best_index;
+        if (best_axis >= 0) {
+            const float *l = lb + ((size_t)best_axis * n + first + best_index) * 6;
+            const float *r = rb + ((size_t)best_axis * n + first + best_index) * 6;
+            for (int k = 0; k < 3; k++) {
+                sp.lmin[k] = l[k];
+                sp.lmax[k] = l[3 + k];
+                sp.rmin[k] = r[k];
+                sp.rmax[k] = r[3 + k];
+            }
+        } else {
+            for (int k = 0; k < 3; k++) {
+                sp.lmin[k] = task.bmin[k];
+                sp.lmax[k] = task.bmax[k];
+                sp.rmin[k] = sp.rmax[k] = 0.0f;
+            }
+        }
+        splits[blockIdx.x] = sp;
+    }
+}
+
+// One workgroup: the level's nodes in node order.  meta[0] = interior nodes
+// so far (children of the k-th at 2k+1, 2k+2), meta[1] = leaf primitives so
+// far, meta[2] = this level's task count, meta[3] = the next level's.
+__global__ void __launch_bounds__(kSahThreads) k_sah_emit(const SahTask *__restrict__ tasks,
+                                                         const SahSplit *__restrict__ splits,
+                                                         const uint32_t *__restrict__ order_in,
+                                                         SahTask *__restrict__ next, rh_bvh_node *__restrict__ nodes,
+                                                         uint32_t *__restrict__ out_indices, uint32_t *meta) {
+    __shared__ uint32_t s_int[kSahThreads], s_leaf[kSahThreads];
+    const uint32_t nt = meta[2], t = threadIdx.x, T = blockDim.x;
+    const uint32_t K = (nt + T - 1) / T;
+    const uint32_t c0 = min(nt, t * K), c1 = min(nt, c0 + K);
+    uint32_t ni = 0, nl = 0;
+    for (uint32_t i = c0; i < c1; i++) {
+        if (splits[i].axis >= 0) ni++;
+        else nl += tasks[i].count;
+    }
+    s_int[t] = ni;
+    s_leaf[t] = nl;
+    __syncthreads();
+    for (uint32_t d = 1; d < T; d <<= 1) {
+        const uint32_t a = t >= d ? s_int[t - d] : 0u, b = t >= d ? s_leaf[t - d] : 0u;
+        __syncthreads();
+        s_int[t] += a;
+        s_leaf[t] += b;
+        __syncthreads();
+    }
+    const uint32_t int_base = meta[0], leaf_base = meta[1];
+    uint32_t ri = (t ? s_int[t - 1] : 0u), rl = (t ? s_leaf[t - 1] : 0u);
+    for (uint32_t i = c0; i < c1; i++) {
+        const SahTask &tk = tasks[i];
+        const SahSplit &sp = splits[i];
+        rh_bvh_node nd;
+        if (sp.axis < 0) {
+            const uint32_t off = leaf_base + rl;
+            nd.prim_index = RH_LEAF_NODE_BIT + off;
+            nd.prim_count = tk.count;
+            for (int k = 0; k < 3; k++) {
+                nd.bbox_min[k] = tk.bmin[k];
+                nd.bbox_max[k] = tk.bmax[k];
+            }
+            for (uint32_t j = 0; j < tk.count; j++) out_indices[off + j] = order_in[tk.first + j];
+            rl += tk.count;
+        } else {
+            const uint32_t k2 = int_base + ri;
+            const uint32_t left = 2 * k2 + 1, right = 2 * k2 + 2;
+            float cl[3], cr[3], dist[3];
+            for (int k = 0; k < 3; k++) {
+                cl[k] = (sp.lmin[k] + sp.lmax[k]) / 2;
+                cr[k] = (sp.rmin[k] + sp.rmax[k]) / 2;
+                dist[k] = __builtin_fabsf(cl[k] - cr[k]);
+            }
+            uint32_t space_axis;
+            if (dist[0] > dist[1] && dist[0] > dist[2]) space_axis = 0;
+            else if (dist[1] > dist[0] && dist[1] > dist[2]) space_axis = 1;
+            else space_axis = 2;
+            nd.prim_index = left;
+            nd.prim_count = (space_axis << 30) + right;
+            for (int k = 0; k < 3; k++) {
+                nd.bbox_min[k] = min_std(sp.lmin[k], sp.rmin[k]);
+                nd.bbox_max[k] = max_std(sp.lmax[k], sp.rmax[k]);
+            }
+            SahTask a, b;
+            a.first = tk.first;
+            a.count = sp.index;
+            a.node = left;
+            b.first = tk.first + sp.index;
+            b.count = tk.count - sp.index;
+            b.node = right;
+            for (int k = 0; k < 3; k++) {
+                a.bmin[k] = sp.lmin[k];
+                a.bmax[k] = sp.lmax[k];
+                b.bmin[k] = sp.rmin[k];
+                b.bmax[k] = sp.rmax[k];
+            }
+            next[2 * ri] = a;
+            next[2 * ri + 1] = b;
+            ri++;
+        }
+        nodes[tk.node] = nd;
+    }
+    __syncthreads();
+    if (t == T - 1) {
+        meta[0] = int_base + s_int[T - 1];
+        meta[1] = leaf_base + s_leaf[T - 1];
+        meta[3] = 2 * s_int[T - 1];
+    }
+}
+
+// next level's task count becomes the current one
+__global__ void k_sah_advance(uint32_t *meta) {
+    meta[2] = meta[3];
+    meta[3] = 0;
+}
+
 } // namespace
 
 struct rh_builder {
@@ -385,7 +746,29 @@ struct rh_builder {
     uint32_t *head = nullptr, *meta = nullptr, *base = nullptr, *treelets = nullptr;
     Task *tasks = nullptr;
     rh_bvh_node *nodes = nullptr;
+    // the SAH build's buffers, for up to sah_cap primitives
+    size_t sah_cap = 0;
+    float *sah_boxes = nullptr, *sah_key = nullptr, *sah_lb = nullptr, *sah_rb = nullptr;
+    uint32_t *sah_order[2] = {nullptr, nullptr}, *sah_val = nullptr, *sah_sorted = nullptr, *sah_indices = nullptr;
+    uint32_t *sah_meta = nullptr; // 4 counters and the tie flag
+    SahTask *sah_tasks[2] = {nullptr, nullptr};
+    SahSplit *sah_splits = nullptr;
+    rh_bvh_node *sah_nodes = nullptr;
 };
+
+static void free_sah_buffers(rh_builder *b) {
+    for (void *p : {(void *)b->sah_boxes, (void *)b->sah_key, (void *)b->sah_lb, (void *)b->sah_rb,
+                    (void *)b->sah_order[0], (void *)b->sah_order[1], (void *)b->sah_val, (void *)b->sah_sorted,
+                    (void *)b->sah_indices, (void *)b->sah_meta, (void *)b->sah_tasks[0], (void *)b->sah_tasks[1],
+                    (void *)b->sah_splits, (void *)b->sah_nodes})
+        free_ptr(p);
+    b->sah_boxes = b->sah_key = b->sah_lb = b->sah_rb = nullptr;
+    b->sah_order[0] = b->sah_order[1] = b->sah_val = b->sah_sorted = b->sah_indices = b->sah_meta = nullptr;
+    b->sah_tasks[0] = b->sah_tasks[1] = nullptr;
+    b->sah_splits = nullptr;
+    b->sah_nodes = nullptr;
+    b->sah_cap = 0;
+}
 
 static void free_buffers(rh_builder *b) {
     for (void *p : {(void *)b->prims, (void *)b->keys[0], (void *)b->keys[1], (void *)b->vals[0], (void *)b->vals[1],
@@ -432,6 +815,7 @@ void rh_builder_destroy(rh_builder *b) {
     (void)hipSetDevice(b->device);
     (void)hipStreamSynchronize(b->stream);
     free_buffers(b);
+    free_sah_buffers(b);
     for (void *p : {(void *)b->partial, (void *)b->box, (void *)b->head, (void *)b->meta, (void *)b->base,
                     (void *)b->treelets})
         free_ptr(p);
@@ -515,6 +899,93 @@ int rh_build_lbvh(rh_builder *b, const float *prims, size_t n, uint32_t index_of
     RH_CHECK(hipStreamSynchronize(s));
     *out_node_count = total;
     *out_treelet_count = T;
+    return 0;
+}
+
+int rh_build_sah(rh_builder *b, const float *boxes, size_t n, float oversplit, float cost, rh_bvh_node *out_nodes,
+                 size_t nodes_cap, uint32_t *out_node_count, uint32_t *out_indices, int *host_needed) {
+    *out_node_count = 0;
+    *host_needed = 0;
+    if (!n) return 0;
+    if (n > 0x3fffffffu / 2) return arg_error("too many primitives for one tree");
+    if (nodes_cap < 2 * n) return arg_error("node output must hold 2 n nodes");
+    // the sort pads with +inf: boxes must be finite (else the host builds)
+    for (size_t i = 0; i < 6 * n; i++)
+        if (!std::isfinite(boxes[i])) {
+            *host_needed = 1;
+            return 0;
+        }
+    RH_CHECK(hipSetDevice(b->device));
+    hipStream_t s = b->stream;
+    if (n > b->sah_cap) {
+        free_sah_buffers(b);
+        const size_t cap = std::max(n, (size_t)1024);
+        RH_CHECK(hipMalloc(&b->sah_boxes, 6 * cap * sizeof(float)));
+        RH_CHECK(hipMalloc(&b->sah_key, 2 * cap * sizeof(float)));
+        RH_CHECK(hipMalloc(&b->sah_val, 2 * cap * sizeof(uint32_t)));
+        RH_CHECK(hipMalloc(&b->sah_lb, 18 * cap * sizeof(float)));
+        RH_CHECK(hipMalloc(&b->sah_rb, 18 * cap * sizeof(float)));
+        RH_CHECK(hipMalloc(&b->sah_order[0], cap * sizeof(uint32_t)));
+        RH_CHECK(hipMalloc(&b->sah_order[1], cap * sizeof(uint32_t)));
+        RH_CHECK(hipMalloc(&b->sah_sorted, 3 * cap * sizeof(uint32_t)));
+        RH_CHECK(hipMalloc(&b->sah_indices, cap * sizeof(uint32_t)));
+        RH_CHECK(hipMalloc(&b->sah_meta, 8 * sizeof(uint32_t)));
+        RH_CHECK(hipMalloc(&b->sah_tasks[0], cap * sizeof(SahTask)));
+        RH_CHECK(hipMalloc(&b->sah_tasks[1], cap * sizeof(SahTask)));
+        RH_CHECK(hipMalloc(&b->sah_splits, cap * sizeof(SahSplit)));
+        RH_CHECK(hipMalloc(&b->sah_nodes, 2 * cap * sizeof(rh_bvh_node)));
+        b->sah_cap = cap;
+    }
+    // the root: every primitive in index order, the box they span
+    std::vector<uint32_t> iota(n);
+    SahTask root;
+    root.first = 0;
+    root.count = (uint32_t)n;
+    root.node = 0;
+    for (int k = 0; k < 3; k++) {
+        root.bmin[k] = 3.402823466e+38F;
+        root.bmax[k] = -3.402823466e+38F;
+    }
+    for (size_t i = 0; i < n; i++) {
+        iota[i] = (uint32_t)i;
+        for (int k = 0; k < 3; k++) {
+            const float lo = boxes[6 * i + k], hi = boxes[6 * i + 3 + k];
+            root.bmin[k] = (lo < root.bmin[k]) ? lo : root.bmin[k];
+            root.bmax[k] = (root.bmax[k] < hi) ? hi : root.bmax[k];
+        }
+    }
+    const uint32_t meta0[8] = {0, 0, 1, 0, 0, 0, 0, 0};
+    RH_CHECK(hipMemcpyAsync(b->sah_boxes, boxes, 6 * n * sizeof(float), hipMemcpyHostToDevice, s));
+    RH_CHECK(hipMemcpyAsync(b->sah_order[0], iota.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    RH_CHECK(hipMemcpyAsync(b->sah_tasks[0], &root, sizeof(root), hipMemcpyHostToDevice, s));
+    RH_CHECK(hipMemcpyAsync(b->sah_meta, meta0, sizeof(meta0), hipMemcpyHostToDevice, s));
+    uint32_t ntasks = 1;
+    int cur = 0;
+    uint32_t meta[8];
+    while (ntasks) {
+        hipLaunchKernelGGL(k_sah_split, dim3(ntasks), dim3(kSahThreads), 0, s, b->sah_boxes, b->sah_tasks[cur],
+                           b->sah_meta + 2, b->sah_order[cur], b->sah_order[cur ^ 1], b->sah_key, b->sah_val,
+                           b->sah_sorted, b->sah_lb, b->sah_rb, n, oversplit, cost, b->sah_splits, b->sah_meta + 4);
+        hipLaunchKernelGGL(k_sah_emit, dim3(1), dim3(kSahThreads), 0, s, b->sah_tasks[cur], b->sah_splits,
+                           b->sah_order[cur], b->sah_tasks[cur ^ 1], b->sah_nodes, b->sah_indices, b->sah_meta);
+        RH_CHECK(hipGetLastError());
+        RH_CHECK(hipMemcpyAsync(meta, b->sah_meta, sizeof(meta), hipMemcpyDeviceToHost, s));
+        RH_CHECK(hipStreamSynchronize(s));
+        if (meta[4]) {
+            *host_needed = 1; // equal keys: the host's std::sort order decides
+            return 0;
+        }
+        if (2 * meta[0] + 1 > 2 * n || meta[1] > n) return arg_error("SAH build: inconsistent node count");
+        ntasks = meta[3];
+        if (ntasks) hipLaunchKernelGGL(k_sah_advance, dim3(1), dim3(1), 0, s, b->sah_meta);
+        cur ^= 1;
+    }
+    if (meta[1] != n) return arg_error("SAH build: primitives lost");
+    const uint32_t total = 2 * meta[0] + 1;
+    RH_CHECK(hipMemcpyAsync(out_nodes, b->sah_nodes, total * sizeof(rh_bvh_node), hipMemcpyDeviceToHost, s));
+    RH_CHECK(hipMemcpyAsync(out_indices, b->sah_indices, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    RH_CHECK(hipStreamSynchronize(s));
+    *out_node_count = total;
     return 0;
 }
 

@@ -169,6 +169,64 @@ int rhx_create_scene_standalone(int use_wide_bvh, rhx_scene **out) {
 }
 void rhx_destroy_scene(rhx_scene *s) { delete s; }
 
+namespace {
+// one builder per device, kept for the process (the standalone build entry points)
+rh_builder *process_builder(int device) {
+    static std::mutex mu;
+    static std::map<int, rh_builder *> builders;
+    std::lock_guard<std::mutex> lock(mu);
+    rh_builder *&b = builders[device];
+    if (!b && rh_builder_create(device, &b) != 0) {
+        b = nullptr;
+        throw std::runtime_error(std::string("BVH build: ") + rh_last_error());
+    }
+    return b;
+}
+} // namespace
+
+int rhx_build_sah(const float *boxes, size_t n, int device, float oversplit, float cost, rh_bvh_node *out_nodes,
+                  size_t nodes_cap, uint32_t *out_count, uint32_t *out_indices, int *on_device) {
+    return guarded([&] {
+        *out_count = 0;
+        *on_device = 0;
+        if (nodes_cap < 2 * n) throw std::runtime_error("node output must hold 2 n nodes");
+        std::vector<RayHip::prim_t> prims(n);
+        for (size_t i = 0; i < n; i++) {
+            prims[i].i0 = prims[i].i1 = prims[i].i2 = 0;
+            memcpy(prims[i].bbox_min.v, boxes + 6 * i, 3 * sizeof(float));
+            memcpy(prims[i].bbox_max.v, boxes + 6 * i + 3, 3 * sizeof(float));
+        }
+        RayHip::bvh_settings_t st;
+        st.oversplit_threshold = oversplit;
+        st.node_traversal_cost = cost;
+        std::vector<rh_bvh_node> nodes;
+        std::vector<uint32_t> indices;
+        RayHip::SahBoxes fn;
+        if (device >= 0) {
+            rh_builder *b = process_builder(device);
+            fn = [b, on_device](const float *bx, size_t m, const RayHip::bvh_settings_t &s,
+                                std::vector<rh_bvh_node> &nd, std::vector<uint32_t> &ix) {
+                nd.resize(2 * m);
+                ix.resize(m);
+                uint32_t count = 0;
+                int host_needed = 0;
+                if (rh_build_sah(b, bx, m, s.oversplit_threshold, s.node_traversal_cost, nd.data(), nd.size(), &count,
+                                 ix.data(), &host_needed) != 0)
+                    throw std::runtime_error(std::string("SAH build: ") + rh_last_error());
+                if (host_needed) return false;
+                nd.resize(count);
+                *on_device = 1;
+                return true;
+            };
+        }
+        *out_count =
+            RayHip::PreprocessPrims_SAH_Boxes(prims.data(), n, st, nodes, indices, device >= 0 ? &fn : nullptr);
+        memcpy(out_nodes, nodes.data(), nodes.size() * sizeof(rh_bvh_node));
+        memcpy(out_indices, indices.data(), indices.size() * sizeof(uint32_t));
+        return 0;
+    });
+}
+
 int rhx_build_fast_bvh(const float *boxes, size_t n, int device, rh_bvh_node *out_nodes, size_t nodes_cap,
                        uint32_t *out_count, uint32_t *out_indices) {
     return guarded([&] {
@@ -185,23 +243,28 @@ int rhx_build_fast_bvh(const float *boxes, size_t n, int device, rh_bvh_node *ou
             }
             *out_count = RayHip::PreprocessPrims_HLBVH(prims.data(), n, nodes, indices);
         } else {
-            // one builder per device, kept for the process
-            static std::mutex mu;
-            static std::map<int, rh_builder *> builders;
-            std::lock_guard<std::mutex> lock(mu);
-            rh_builder *&b = builders[device];
-            if (!b && rh_builder_create(device, &b) != 0) {
-                b = nullptr;
-                throw std::runtime_error(std::string("fast BVH build: ") + rh_last_error());
-            }
+            rh_builder *b = process_builder(device);
             // the boxes go to the device as they are (the bottom half reads
-            // nothing else of the primitives)
+            // nothing else of the primitives); the top tree is built there too
             const RayHip::LbvhBottom fn = [b, boxes](const RayHip::prim_t *, size_t m, uint32_t offset,
                                                      std::vector<uint32_t> &order, std::vector<rh_bvh_node> &bottom,
                                                      std::vector<RayHip::HlbvhTreelet> &treelets) {
                 Ray::Hip::DeviceLbvhBoxes(b, boxes, m, offset, order, bottom, treelets);
             };
-            *out_count = RayHip::PreprocessPrims_HLBVH(nullptr, n, nodes, indices, &fn);
+            const RayHip::SahBoxes top = [b](const float *bx, size_t m, const RayHip::bvh_settings_t &s,
+                                             std::vector<rh_bvh_node> &nd, std::vector<uint32_t> &ix) {
+                nd.resize(2 * m);
+                ix.resize(m);
+                uint32_t count = 0;
+                int host_needed = 0;
+                if (rh_build_sah(b, bx, m, s.oversplit_threshold, s.node_traversal_cost, nd.data(), nd.size(), &count,
+                                 ix.data(), &host_needed) != 0)
+                    throw std::runtime_error(std::string("SAH build: ") + rh_last_error());
+                if (host_needed) return false;
+                nd.resize(count);
+                return true;
+            };
+            *out_count = RayHip::PreprocessPrims_HLBVH(nullptr, n, nodes, indices, &fn, &top);
         }
         memcpy(out_nodes, nodes.data(), nodes.size() * sizeof(rh_bvh_node));
         memcpy(out_indices, indices.data(), indices.size() * sizeof(uint32_t));

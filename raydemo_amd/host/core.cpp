@@ -397,7 +397,7 @@ void ExtractPlaneNormal(const rh_tri_accel &tri, float *out_normal) {
 uint32_t PreprocessMesh(const float *attrs, const uint32_t *vtx_indices, size_t vtx_indices_count, int layout,
                         int base_vertex, const bvh_settings_t &s, std::vector<rh_bvh_node> &out_nodes,
                         std::vector<rh_tri_accel> &out_tris, std::vector<uint32_t> &out_tri_indices,
-                        const LbvhBottom *device) {
+                        const LbvhBottom *device, const SahBoxes *sah) {
     static const size_t attr_strides[] = {8, 10, 11, 13};
     std::vector<prim_t> primitives;
     primitives.reserve(vtx_indices_count / 3);
@@ -426,7 +426,7 @@ uint32_t PreprocessMesh(const float *attrs, const uint32_t *vtx_indices, size_t 
     const size_t indices_start = out_tri_indices.size();
     const uint32_t num_out_nodes =
         s.use_fast_bvh_build
-            ? PreprocessPrims_HLBVH(primitives.data(), primitives.size(), out_nodes, out_tri_indices, device)
+            ? PreprocessPrims_HLBVH(primitives.data(), primitives.size(), out_nodes, out_tri_indices, device, sah)
             : PreprocessPrims_SAH(primitives.data(), primitives.size(), positions, attr_stride, s, out_nodes,
                                   out_tri_indices);
     for (size_t i = indices_start; i < out_tri_indices.size(); i++) out_tri_indices[i] += (uint32_t)tris_start;
@@ -506,6 +506,39 @@ uint32_t PreprocessPrims_SAH(const prim_t *prims, size_t prims_count, const floa
         }
     }
     return (uint32_t)(out_nodes.size() - root_node_index);
+}
+
+uint32_t PreprocessPrims_SAH_Boxes(const prim_t *prims, size_t prims_count, const bvh_settings_t &s,
+                                   std::vector<rh_bvh_node> &out_nodes, std::vector<uint32_t> &out_indices,
+                                   const SahBoxes *sah) {
+    if (sah && prims_count) {
+        std::vector<float> boxes(6 * prims_count);
+        for (size_t i = 0; i < prims_count; i++) {
+            memcpy(&boxes[6 * i], prims[i].bbox_min.v, 3 * sizeof(float));
+            memcpy(&boxes[6 * i + 3], prims[i].bbox_max.v, 3 * sizeof(float));
+        }
+        std::vector<rh_bvh_node> nodes;
+        std::vector<uint32_t> indices;
+        bvh_settings_t plain = s;
+        plain.allow_spatial_splits = false;
+        if ((*sah)(boxes.data(), prims_count, plain, nodes, indices)) {
+            const uint32_t node_base = (uint32_t)out_nodes.size(), index_base = (uint32_t)out_indices.size();
+            for (rh_bvh_node n : nodes) {
+                if (n.prim_index & RH_LEAF_NODE_BIT) {
+                    n.prim_index = RH_LEAF_NODE_BIT + (n.prim_index & RH_PRIM_INDEX_BITS) + index_base;
+                } else {
+                    n.prim_index += node_base;
+                    n.prim_count = (n.prim_count & RH_SEP_AXIS_BITS) + (n.prim_count & RH_RIGHT_CHILD_BITS) + node_base;
+                }
+                out_nodes.push_back(n);
+            }
+            out_indices.insert(out_indices.end(), indices.begin(), indices.end());
+            return (uint32_t)nodes.size();
+        }
+    }
+    bvh_settings_t plain = s;
+    plain.allow_spatial_splits = false;
+    return PreprocessPrims_SAH(prims, prims_count, nullptr, 0, plain, out_nodes, out_indices);
 }
 
 // ---- the fast build: HLBVH (mesh_desc_t::use_fast_bvh_build) ----
@@ -638,7 +671,7 @@ uint32_t AssembleHLBVH(const std::vector<HlbvhTreelet> &treelets, const std::vec
 }
 
 void TopLevelSAH(const std::vector<HlbvhTreelet> &treelets, const std::vector<rh_bvh_node> &bottom,
-                 std::vector<rh_bvh_node> &top, std::vector<uint32_t> &top_indices) {
+                 std::vector<rh_bvh_node> &top, std::vector<uint32_t> &top_indices, const SahBoxes *sah) {
     std::vector<prim_t> top_prims(treelets.size());
     for (size_t i = 0; i < treelets.size(); i++) {
         const rh_bvh_node &r = bottom[treelets[i].node_index];
@@ -649,11 +682,11 @@ void TopLevelSAH(const std::vector<HlbvhTreelet> &treelets, const std::vector<rh
     s.oversplit_threshold = std::numeric_limits<float>::max();
     s.node_traversal_cost = 0.0f;
     s.allow_spatial_splits = false;
-    PreprocessPrims_SAH(top_prims.data(), top_prims.size(), nullptr, 0, s, top, top_indices);
+    PreprocessPrims_SAH_Boxes(top_prims.data(), top_prims.size(), s, top, top_indices, sah);
 }
 
 uint32_t PreprocessPrims_HLBVH(const prim_t *prims, size_t prims_count, std::vector<rh_bvh_node> &out_nodes,
-                               std::vector<uint32_t> &out_indices, const LbvhBottom *device) {
+                               std::vector<uint32_t> &out_indices, const LbvhBottom *device, const SahBoxes *sah) {
     if (prims_count == 0) return 0;
     const uint32_t indices_start = (uint32_t)out_indices.size();
     if (device) {
@@ -663,7 +696,7 @@ uint32_t PreprocessPrims_HLBVH(const prim_t *prims, size_t prims_count, std::vec
         std::vector<uint32_t> top_indices;
         (*device)(prims, prims_count, indices_start, order, bottom, treelets);
         out_indices.insert(out_indices.end(), order.begin(), order.end());
-        TopLevelSAH(treelets, bottom, top, top_indices);
+        TopLevelSAH(treelets, bottom, top, top_indices, sah);
         return AssembleHLBVH(treelets, bottom, top, top_indices, out_nodes);
     }
     std::vector<uint32_t> codes(prims_count), order(prims_count);

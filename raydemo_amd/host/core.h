@@ -54,17 +54,31 @@ using LbvhBottom = std::function<void(const prim_t *prims, size_t n, uint32_t in
                                       std::vector<uint32_t> &order, std::vector<rh_bvh_node> &bottom,
                                       std::vector<HlbvhTreelet> &treelets)>;
 
+// An SAH build of boxes (6 floats each: bbox_min, bbox_max) done elsewhere
+// (on the device): fills nodes (root first, child links relative to
+// nodes[0], leaves indexing `indices` from 0) and indices exactly as
+// PreprocessPrims_SAH without spatial splits would, or returns false when it
+// cannot (the caller then builds on the host); throws on failure.
+using SahBoxes = std::function<bool(const float *boxes, size_t n, const bvh_settings_t &s,
+                                    std::vector<rh_bvh_node> &nodes, std::vector<uint32_t> &indices)>;
+
 // PreprocessMesh (Core.cpp:230-275); with s.use_fast_bvh_build the tree is
-// built by PreprocessPrims_HLBVH, its bottom half by `device` when given.
+// built by PreprocessPrims_HLBVH, its bottom half by `device` and its top
+// tree by `sah` when given.
 uint32_t PreprocessMesh(const float *attrs, const uint32_t *vtx_indices, size_t vtx_indices_count, int layout,
                         int base_vertex, const bvh_settings_t &s, std::vector<rh_bvh_node> &out_nodes,
                         std::vector<rh_tri_accel> &out_tris, std::vector<uint32_t> &out_tri_indices,
-                        const LbvhBottom *device = nullptr);
+                        const LbvhBottom *device = nullptr, const SahBoxes *sah = nullptr);
 
 // PreprocessPrims_SAH (Core.cpp:431-530) + SplitPrimitives_SAH (BVHSplit.cpp:126-390)
 uint32_t PreprocessPrims_SAH(const prim_t *prims, size_t prims_count, const float *positions, size_t stride,
                              const bvh_settings_t &s, std::vector<rh_bvh_node> &out_nodes,
                              std::vector<uint32_t> &out_indices);
+// The same without spatial splits, built by `sah` where it can: its nodes are
+// appended as the host's would be (child links and leaf offsets rebased).
+uint32_t PreprocessPrims_SAH_Boxes(const prim_t *prims, size_t prims_count, const bvh_settings_t &s,
+                                   std::vector<rh_bvh_node> &out_nodes, std::vector<uint32_t> &out_indices,
+                                   const SahBoxes *sah);
 
 // The fast build (PreprocessPrims_HLBVH, Core.cpp:532-680): primitives sorted
 // (stably) by the 30-bit Morton code of their centres in the mesh box
@@ -86,12 +100,13 @@ void MortonCodes(const prim_t *prims, size_t n, uint32_t *codes);
 uint32_t EmitLBVH(const prim_t *prims, const uint32_t *indices, const uint32_t *codes, uint32_t first, uint32_t count,
                   uint32_t index_offset, int bit, std::vector<rh_bvh_node> &out);
 void TopLevelSAH(const std::vector<HlbvhTreelet> &treelets, const std::vector<rh_bvh_node> &bottom,
-                 std::vector<rh_bvh_node> &top, std::vector<uint32_t> &top_indices);
+                 std::vector<rh_bvh_node> &top, std::vector<uint32_t> &top_indices, const SahBoxes *sah = nullptr);
 uint32_t AssembleHLBVH(const std::vector<HlbvhTreelet> &treelets, const std::vector<rh_bvh_node> &bottom,
                        const std::vector<rh_bvh_node> &top, const std::vector<uint32_t> &top_indices,
                        std::vector<rh_bvh_node> &out_nodes);
 uint32_t PreprocessPrims_HLBVH(const prim_t *prims, size_t prims_count, std::vector<rh_bvh_node> &out_nodes,
-                               std::vector<uint32_t> &out_indices, const LbvhBottom *device = nullptr);
+                               std::vector<uint32_t> &out_indices, const LbvhBottom *device = nullptr,
+                               const SahBoxes *sah = nullptr);
 
 // FlattenBVH_Recursive (Core.cpp:683-819): appends the 8-wide form of the
 // binary tree rooted at nodes[root] to out (depth-first, a node before its

@@ -82,6 +82,7 @@ class Scene : public SceneBase {
     int build_device_ = -1;
     rh_builder *builder_ = nullptr;
     RayHip::LbvhBottom device_bottom_;
+    RayHip::SahBoxes device_sah_; // the instance tree and the fast build's top tree
 
     void RemoveNodes(uint32_t node_index, uint32_t node_count);
     void WriteBlock(const rh_tex_mip &m, const std::vector<uint32_t> &data, const int res[2]);
@@ -139,6 +140,9 @@ public:
 // The fast build's bottom half (Morton sort, treelets) on GPU `device`, with
 // the builder created at first use in *builder (kept by the caller).
 RayHip::LbvhBottom DeviceLbvhBottom(rh_builder **builder, int device);
+// The SAH build of boxes on GPU `device` (rh_build_sah), declining where the
+// host's order of equal keys decides.
+RayHip::SahBoxes DeviceSahBoxes(rh_builder **builder, int device);
 // the same for boxes already packed as 6 floats each
 void DeviceLbvhBoxes(rh_builder *builder, const float *boxes, size_t n, uint32_t index_offset,
                      std::vector<uint32_t> &order, std::vector<rh_bvh_node> &bottom,

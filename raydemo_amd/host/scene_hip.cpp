@@ -103,6 +103,7 @@ static_assert(sizeof(camera_t) == sizeof(rh_camera), "camera_t layout");
 Scene::Scene(bool use_wide_bvh, bool device_build, int device)
     : wide_(use_wide_bvh), device_build_(device_build), build_device_(device) {
     device_bottom_ = DeviceLbvhBottom(&builder_, build_device_);
+    device_sah_ = DeviceSahBoxes(&builder_, build_device_);
     const int atlas_res[2] = {RH_TEXTURE_ATLAS_SIZE, RH_TEXTURE_ATLAS_SIZE};
     splitters_.emplace_back(atlas_res);
 
@@ -325,6 +326,26 @@ RayHip::LbvhBottom DeviceLbvhBottom(rh_builder **builder, int device) {
     };
 }
 
+RayHip::SahBoxes DeviceSahBoxes(rh_builder **builder, int device) {
+    return [builder, device](const float *boxes, size_t n, const RayHip::bvh_settings_t &s,
+                             std::vector<rh_bvh_node> &nodes, std::vector<uint32_t> &indices) {
+        if (!*builder && rh_builder_create(device, builder) != 0) {
+            *builder = nullptr;
+            throw std::runtime_error(std::string("SAH build: ") + rh_last_error());
+        }
+        nodes.resize(2 * n);
+        indices.resize(n);
+        uint32_t count = 0;
+        int host_needed = 0;
+        if (rh_build_sah(*builder, boxes, n, s.oversplit_threshold, s.node_traversal_cost, nodes.data(), nodes.size(),
+                         &count, indices.data(), &host_needed) != 0)
+            throw std::runtime_error(std::string("SAH build: ") + rh_last_error());
+        if (host_needed) return false;
+        nodes.resize(count);
+        return true;
+    };
+}
+
 Scene::~Scene() {
     if (builder_) rh_builder_destroy(builder_);
 }
@@ -365,7 +386,8 @@ uint32_t Scene::AddMesh(const mesh_desc_t &_m) {
     try {
         m.node_count = RayHip::PreprocessMesh(_m.vtx_attrs, _m.vtx_indices, _m.vtx_indices_count, _m.layout,
                                               _m.base_vertex, s, nodes_, tris_, tri_indices_,
-                                              device_build_ ? &device_bottom_ : nullptr);
+                                              device_build_ ? &device_bottom_ : nullptr,
+                                              device_build_ ? &device_sah_ : nullptr);
     } catch (...) {
         nodes_.resize(nodes_before);
         tris_.resize(tris_start);
@@ -622,8 +644,9 @@ void Scene::RebuildMacroBVH() {
                               {{mi.bbox_max[0], mi.bbox_max[1], mi.bbox_max[2]}}});
     }
     macro_nodes_root_ = (uint32_t)nodes_.size();
-    macro_nodes_count_ =
-        RayHip::PreprocessPrims_SAH(primitives.data(), primitives.size(), nullptr, 0, {}, nodes_, mi_indices_);
+    // a renderer's scene builds the instance tree on the device (the same tree)
+    macro_nodes_count_ = RayHip::PreprocessPrims_SAH_Boxes(primitives.data(), primitives.size(), {}, nodes_,
+                                                           mi_indices_, device_build_ ? &device_sah_ : nullptr);
 }
 
 // SceneRef::RebuildLightBVH (SceneRef.cpp:518-587): each light's influence

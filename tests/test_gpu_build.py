@@ -102,3 +102,83 @@ def test_device_fast_bvh_over_boxes(host_lib):
     dn, di = A.build_fast_bvh(host_lib, boxes, device=0)
     print(f"{n} boxes: {len(hn)} nodes")
     assert hn.tobytes() == dn.tobytes() and hi.tobytes() == di.tobytes()
+
+
+# ---- the SAH build on the device (rh_build_sah): instance tree and fast-build top tree
+
+def _distinct_boxes(rng, n, spread=100.0):
+    """n random boxes whose bbox_max coordinates differ along every axis (the
+    device build's precondition; with equal keys the host builds)."""
+    while True:
+        lo = rng.uniform(-spread, spread, (n, 3)).astype(np.float32)
+        ext = rng.exponential(spread / max(1.0, np.cbrt(n)) / 2, (n, 3)).astype(np.float32)
+        hi = lo + ext
+        if all(len(np.unique(hi[:, a])) == n for a in range(3)):
+            return np.concatenate([lo, hi], axis=1)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 1000, 4096, 4097, 9000])
+def test_device_sah_equals_host(host_lib, n):
+    """Node for node and index for index the host's PreprocessPrims_SAH (the
+    LDS sort up to 4096 boxes a node, global scratch above)."""
+    boxes = _distinct_boxes(np.random.default_rng(1000 + n), n)
+    hn, hi, h_dev = A.build_sah(host_lib, boxes)
+    dn, di, d_dev = A.build_sah(host_lib, boxes, device=0)
+    assert not h_dev and d_dev
+    print(f"{n} boxes: {len(hn)} nodes")
+    assert hn.tobytes() == dn.tobytes() and hi.tobytes() == di.tobytes()
+
+
+def test_device_sah_top_tree_settings(host_lib):
+    """The fast build's top tree settings (oversplit FLT_MAX, traversal cost
+    0: every box its own leaf) and a clustered set of boxes."""
+    rng = np.random.default_rng(5)
+    boxes = _distinct_boxes(rng, 700)
+    boxes[:350, :3] *= 0.01
+    boxes[:350, 3:] = boxes[:350, :3] + np.abs(boxes[:350, 3:] - boxes[:350, :3]) * 0.01
+    assert all(len(np.unique(boxes[:, 3 + a])) == 700 for a in range(3))
+    kw = dict(oversplit=3.4028234e38, cost=0.0)
+    hn, hi, _ = A.build_sah(host_lib, boxes, **kw)
+    dn, di, d_dev = A.build_sah(host_lib, boxes, device=0, **kw)
+    assert d_dev and hn.tobytes() == dn.tobytes() and hi.tobytes() == di.tobytes()
+    leaves = hn[(hn["prim_index"] & 0x80000000) != 0]
+    assert len(leaves) == 700 and (leaves["prim_count"] == 1).all()
+
+
+def test_device_sah_ties_build_on_host(host_lib):
+    """Equal bbox_max keys: the device declines and the host's tree is used."""
+    boxes = _distinct_boxes(np.random.default_rng(9), 300)
+    boxes[10:20, 4] = boxes[10, 4]
+    hn, hi, _ = A.build_sah(host_lib, boxes)
+    dn, di, d_dev = A.build_sah(host_lib, boxes, device=0)
+    assert not d_dev
+    assert hn.tobytes() == dn.tobytes() and hi.tobytes() == di.tobytes()
+
+
+def _instanced_scene(s, rng, n_inst):
+    attrs, idx = _random_tris(rng, 400)
+    mesh = _mesh_scene(s, attrs, idx)
+    for _ in range(n_inst):
+        x = np.eye(4, dtype=np.float32)
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        x[:3, :3] = (q * rng.uniform(0.3, 2.0)).astype(np.float32)
+        x[3, :3] = rng.uniform(-200, 200, 3)
+        s.add_mesh_instance(mesh, x)
+
+
+@pytest.mark.parametrize("wide", [False, True])
+def test_instance_tree_built_on_device_equals_host(host_lib, wide):
+    """A renderer's scene builds its instance tree (RebuildMacroBVH) on the
+    device: 3000 instances give the standalone (host-built) scene's arrays
+    byte for byte, and the device does build this set (no tied keys)."""
+    r, ds = _device_scene(host_lib, wide)
+    hs = standalone_scene(host_lib, wide)
+    for s in (ds, hs):
+        _instanced_scene(s, np.random.default_rng(31), 3000)
+    dv, hv = ds.view(), hs.view()
+    for k in ("nodes", "mnodes", "mi_indices", "mesh_instances", "meshes"):
+        assert dv[k].tobytes() == hv[k].tobytes(), k
+    mi = hv["mesh_instances"]
+    boxes = np.concatenate([mi["bbox_min"], mi["bbox_max"]], axis=1)
+    _, _, on_device = A.build_sah(host_lib, boxes, device=0, oversplit=0.95, cost=0.025)
+    assert on_device
