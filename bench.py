@@ -456,8 +456,26 @@ def run_large(args, lib, A, scenes, knobs, wide):
         r.set_option(k, v)
     r.set_option("max_bounces", 12)
     s = r.create_scene()
+    # the library's own build time (scene calls and the lazy tree builds),
+    # apart from the generation of the synthetic data in Python
+    lib_time = [0.0]
+    timed = {}
+    for name in ("add_texture", "add_material", "add_mesh", "add_mesh_instance", "add_light", "add_camera"):
+        f = getattr(s, name)
+        def wrap(*a, _f=f, **k):
+            t = time.perf_counter()
+            try:
+                return _f(*a, **k)
+            finally:
+                lib_time[0] += time.perf_counter() - t
+        timed[name] = wrap
+        setattr(s, name, wrap)
     spec = scenes.large_instanced(s, width=args.large_width, height=args.large_height, spp=args.large_spp)
+    for name in timed:
+        delattr(s, name)
+    t_fin = time.perf_counter()
     s.triangle_count()
+    lib_time[0] += time.perf_counter() - t_fin
     s.view()
     t_build = time.time() - t0
     log(f"large: {spec.info['instances']} instances, {spec.info['instanced_triangles']} instanced triangles "
@@ -505,8 +523,48 @@ def run_large(args, lib, A, scenes, knobs, wide):
                                   f"{spec.info['instanced_triangles']} instanced triangles, 32 sRGB textures "
                                   f"2048^2 with mips, {W}x{H}, {spp} spp, depth 12",
                       "trees": "8-wide" if wide else "binary", "host_build_s": round(t_build, 1),
-                      "knobs": best_cfg}}
+                      "library_build_s": round(lib_time[0], 2),
+                      "synthetic_data_s": round(t_build - lib_time[0], 1), "knobs": best_cfg}}
     log(f"large render: {out['mrays']} Mrays/s ({spp} passes in {dt:.3f}s)")
+
+    # an animation step: every instance moved (SetMeshInstanceTransform), then
+    # one pass, which rebuilds the instance tree on the device and replaces
+    # only the instance arrays there; against one pass with nothing changed
+    # and one after an edit that re-uploads the whole scene
+    def one_pass():
+        r.synchronize()
+        t = time.perf_counter()
+        r.render(s, reg)
+        r.synchronize()
+        return time.perf_counter() - t
+
+    v = s.view()
+    mis, trs = v["mesh_instances"], v["transforms"]
+    orig_x = [trs[mis[i]["tr_index"]]["xform"].copy() for i in range(len(mis))]
+    def move(dy):
+        for i, x in enumerate(orig_x):
+            y = x.copy()
+            y[13] += dy
+            s.set_mesh_instance_transform(i, y)
+    base_s = min(one_pass() for _ in range(2))
+    updates0, uploads0 = r.get_option("instance_updates"), r.get_option("scene_uploads")
+    moved = []
+    for dy in (0.25, 0.0):
+        move(dy)
+        moved.append(one_pass())
+    counted = {"instance_updates": r.get_option("instance_updates") - updates0,
+               "full_uploads": r.get_option("scene_uploads") - uploads0}
+    s.set_environment((0.3, 0.35, 0.4))  # an edit outside the instances: the next pass uploads everything
+    full_s = one_pass()
+    mi_boxes = np.concatenate([mis["bbox_min"], mis["bbox_max"]], axis=1)
+    out["instance_update"] = {
+        "metric": "one pass after moving every instance, ms (instance tree rebuilt on the device, instance arrays "
+                  "replaced) vs after a full scene upload",
+        "instances": len(mis), "pass_ms_unchanged": round(base_s * 1e3, 2),
+        "pass_ms_after_move": round(min(moved) * 1e3, 2), "pass_ms_after_full_upload": round(full_s * 1e3, 2),
+        "update_ms": round((min(moved) - base_s) * 1e3, 2), "full_upload_ms": round((full_s - base_s) * 1e3, 2),
+        "moves_counted": counted, "instance_tree_on_device": A.build_sah(lib, mi_boxes, device=0)[2]}
+    log(f"large instance update: {out['instance_update']}")
 
     # texel-gradient descent on 4 textures
     if args.large_adj_steps > 0:

@@ -52,6 +52,15 @@ void rh_destroy(rh_context *ctx);
 int rh_resize(rh_context *ctx, int w, int h);
 int rh_clear(rh_context *ctx, const float rgba[4]);
 int rh_upload_scene(rh_context *ctx, const rh_scene_view *scene);
+/* After instance transforms changed (SceneBase::SetMeshInstanceTransform,
+ * SceneBase.h:265; the reference rebuilds its instance tree on the host and
+ * re-sends the whole scene): replaces only the arrays that follow the
+ * instances -- node arrays (with the rebuilt instance tree), meshes,
+ * instances, transforms, tree roots, light leaves -- and keeps the
+ * triangles, vertices, materials, textures and lights of the last
+ * rh_upload_scene resident.  An error (nothing replaced, the context needs
+ * rh_upload_scene) when any of those changed size. */
+int rh_update_instances(rh_context *ctx, const rh_scene_view *scene);
 int rh_set_halton(rh_context *ctx, const float *seq, size_t count);
 int rh_render_pass(rh_context *ctx, const rh_pass_desc *pass);
 int rh_read_pixels(rh_context *ctx, float *rgba, size_t count);
@@ -90,6 +99,8 @@ int rh_get_ray_counts(rh_context *ctx, rh_ray_counts *out);
  * "adjoint_tape_texels" (0/1: the tape records texel footprints too, for texel gradients),
  * "defer_nee" (0/1: shadow rays of next-event estimation as their own stage). */
 int rh_set_option(rh_context *ctx, const char *name, int value);
+/* Any option above, and two read-only counts: "scene_uploads" (rh_upload_scene
+ * calls) and "instance_updates" (rh_update_instances calls). */
 int rh_get_option(rh_context *ctx, const char *name, int *value);
 
 /* Per-kernel profile collected while "kernel_timing" is on (HIP events around

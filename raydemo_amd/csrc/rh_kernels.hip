@@ -701,7 +701,10 @@ struct rh_context {
     size_t max_rays = 0;
 
     // scene
-    std::vector<DevBuf> scene_bufs;
+    std::vector<DevBuf> scene_bufs;    // the arrays instance transforms leave alone
+    std::vector<DevBuf> instance_bufs; // the arrays rh_update_instances replaces
+    size_t static_counts[9] = {};      // their sizes at upload (rh_update_instances checks them)
+    int n_uploads = 0, n_instance_updates = 0; // options "scene_uploads", "instance_updates" (read-only)
     Scene sc = {};
     bool have_scene = false;
 
@@ -1331,6 +1334,7 @@ void rh_destroy(rh_context *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (auto &b : c->scene_bufs) free_ptr(b.p);
+    for (auto &b : c->instance_bufs) free_ptr(b.p);
     free_ptr(c->geo_winner);
     free_ptr(c->sh_temp);
     free_ptr(c->sh_clean);
@@ -1408,14 +1412,28 @@ int rh_read_sh(rh_context *c, float *out, size_t count) {
     return 0;
 }
 
+static void static_counts(const rh_scene_view *v, size_t out[9]) {
+    const size_t k[9] = {v->tris_count,      v->tri_indices_count, v->vertices_count,
+                         v->vtx_indices_count, v->materials_count,   v->textures_count,
+                         v->texels_count,     v->lights_count,      v->li_indices_count};
+    memcpy(out, k, sizeof(k));
+}
+
 int rh_upload_scene(rh_context *c, const rh_scene_view *v) {
     RH_CHECK(hipSetDevice(c->device));
     RH_CHECK(hipStreamSynchronize(c->stream));
+    c->have_scene = false;
     for (auto &b : c->scene_bufs) free_ptr(b.p);
+    for (auto &b : c->instance_bufs) free_ptr(b.p);
     c->scene_bufs.clear();
+    c->instance_bufs.clear();
     Scene s = {};
-    const int rc = build_device_scene(v, c->stream, c->scene_bufs, s);
+    int rc = build_device_static(v, c->stream, c->scene_bufs, s);
     if (rc) return rc;
+    rc = build_device_instances(v, c->stream, c->instance_bufs, s);
+    if (rc) return rc;
+    static_counts(v, c->static_counts);
+    c->n_uploads++;
     // deferred NEE keeps one slot per light for every shaded ray; scenes with
     // many lights trace visibility inline instead
     c->nee_slots = (v->lights_count >= 1 && v->lights_count <= kMaxDeferredLights) ? v->lights_count : 0;
@@ -1434,6 +1452,32 @@ int rh_upload_scene(rh_context *c, const rh_scene_view *v) {
         if (rc2) return rc2;
     }
     RH_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int rh_update_instances(rh_context *c, const rh_scene_view *v) {
+    if (!c->have_scene) return arg_error("rh_update_instances: no scene uploaded");
+    size_t k[9];
+    static_counts(v, k);
+    if (memcmp(k, c->static_counts, sizeof(k)) != 0)
+        return arg_error("rh_update_instances: the scene's triangles, materials, textures or lights changed");
+    RH_CHECK(hipSetDevice(c->device));
+    RH_CHECK(hipStreamSynchronize(c->stream));
+    c->have_scene = false; // until the new arrays are in place
+    for (auto &b : c->instance_bufs) free_ptr(b.p);
+    c->instance_bufs.clear();
+    Scene s = c->sc;
+    const int rc = build_device_instances(v, c->stream, c->instance_bufs, s);
+    if (rc) return rc;
+    c->sc = s;
+    c->n_mesh_instances = (uint32_t)v->mesh_instances_count;
+    tape_forget(c);
+    c->have_scene = true;
+    c->n_instance_updates++;
+    if (c->max_rays) {
+        const int rc2 = ensure_nee(c);
+        if (rc2) return rc2;
+    }
     return 0;
 }
 
@@ -1849,6 +1893,8 @@ int rh_get_option(rh_context *c, const char *name, int *value) {
     else if (n == "tail_occupancy") *value = c->tail_occupancy;
     else if (n == "refill") *value = c->refill;
     else if (n == "tail_refill") *value = c->tail_refill;
+    else if (n == "scene_uploads") *value = c->n_uploads;
+    else if (n == "instance_updates") *value = c->n_instance_updates;
     else if (n == "trace_occupancy") *value = c->trace_occupancy;
     else if (n == "max_bounces") *value = c->max_bounces;
     else if (n == "adjoint_staged") *value = c->adjoint_staged;

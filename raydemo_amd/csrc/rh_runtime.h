@@ -188,20 +188,14 @@ inline int build_wide(const rh_scene_view *v, hipStream_t stream, std::vector<rh
     return 0;
 }
 
-// Device copies of the scene arrays of v (appended to bufs) described by out.
-// The light tree's leaves are listed in the reference's depth-first order
-// (right child first) for the device's stackless light walk.
-inline int build_device_scene(const rh_scene_view *v, hipStream_t stream, std::vector<rh_rt::DevBuf> &bufs,
-                              Scene &out) {
-    Scene s = {};
+// Device copies of the scene arrays an instance transform leaves as they are:
+// triangles, vertices, materials, textures and lights (appended to bufs,
+// described in s).
+inline int build_device_static(const rh_scene_view *v, hipStream_t stream, std::vector<rh_rt::DevBuf> &bufs,
+                               Scene &s) {
     int rc = 0;
-    rc |= upload_array(bufs, stream, v->nodes, v->nodes_count, &s.nodes);
     rc |= upload_array(bufs, stream, v->tris, v->tris_count, &s.tris);
     rc |= upload_array(bufs, stream, v->tri_indices, v->tri_indices_count, &s.tri_indices);
-    if (!v->mnodes) rc |= upload_array(bufs, stream, v->meshes, v->meshes_count, &s.meshes); // wide: renumbered
-    rc |= upload_array(bufs, stream, v->mesh_instances, v->mesh_instances_count, &s.mesh_instances);
-    rc |= upload_array(bufs, stream, v->mi_indices, v->mi_indices_count, &s.mi_indices);
-    rc |= upload_array(bufs, stream, v->transforms, v->transforms_count, &s.transforms);
     rc |= upload_array(bufs, stream, v->vertices, v->vertices_count, &s.vertices);
     rc |= upload_array(bufs, stream, v->vtx_indices, v->vtx_indices_count, &s.vtx_indices);
     rc |= upload_array(bufs, stream, v->materials, v->materials_count, &s.materials);
@@ -215,6 +209,26 @@ inline int build_device_scene(const rh_scene_view *v, hipStream_t stream, std::v
     s.all_solid = 1;
     for (uint32_t i = 0; i < v->tris_count; i++)
         if (!((uint32_t)v->tris[i].ci & RH_TRI_SOLID_BIT)) s.all_solid = 0;
+    return 0;
+}
+
+// Device copies of the arrays that follow the instances: the node arrays (a
+// rebuilt instance tree renumbers them), meshes, instances, transforms, the
+// tree roots and the light tree's leaves, listed in the reference's
+// depth-first order (right child first) for the device's stackless light walk.
+inline int build_device_instances(const rh_scene_view *v, hipStream_t stream, std::vector<rh_rt::DevBuf> &bufs,
+                                  Scene &s) {
+    s.nodes = nullptr;
+    s.meshes = nullptr;
+    s.wnodes = nullptr;
+    s.light_untested = 0;
+    int rc = 0;
+    rc |= upload_array(bufs, stream, v->nodes, v->nodes_count, &s.nodes);
+    if (!v->mnodes) rc |= upload_array(bufs, stream, v->meshes, v->meshes_count, &s.meshes); // wide: renumbered
+    rc |= upload_array(bufs, stream, v->mesh_instances, v->mesh_instances_count, &s.mesh_instances);
+    rc |= upload_array(bufs, stream, v->mi_indices, v->mi_indices_count, &s.mi_indices);
+    rc |= upload_array(bufs, stream, v->transforms, v->transforms_count, &s.transforms);
+    if (rc) return rc;
     s.macro_root = v->macro_nodes_root;
     s.light_root = v->light_nodes_root;
     std::vector<rh_bvh_node> leaves;
@@ -241,6 +255,19 @@ inline int build_device_scene(const rh_scene_view *v, hipStream_t stream, std::v
     s.light_leaf_count = (uint32_t)leaves.size();
     if (!v->mnodes && s.macro_root != RH_INVALID && (!v->nodes || s.macro_root >= v->nodes_count))
         return rh_rt::arg_error("macro tree root outside the node array");
+    // the staging vectors above go out of scope when this returns
+    if (hipStreamSynchronize(stream) != hipSuccess) return rh_rt::arg_error("scene upload failed");
+    return 0;
+}
+
+// Device copies of all the scene arrays of v (appended to bufs) described by out.
+inline int build_device_scene(const rh_scene_view *v, hipStream_t stream, std::vector<rh_rt::DevBuf> &bufs,
+                              Scene &out) {
+    Scene s = {};
+    int rc = build_device_static(v, stream, bufs, s);
+    if (rc) return rc;
+    rc = build_device_instances(v, stream, bufs, s);
+    if (rc) return rc;
     out = s;
     return 0;
 }

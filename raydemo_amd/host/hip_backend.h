@@ -63,7 +63,13 @@ class Scene : public SceneBase {
     // the macro tree is rebuilt lazily (once before use) instead of after every
     // instance change; the light tree likewise
     bool macro_dirty_ = false, light_dirty_ = false;
-    uint64_t version_ = 1;
+    // version_ changes with every edit; static_version_ with every edit but
+    // an instance transform (then the device keeps triangles, textures, ...)
+    uint64_t version_ = 1, static_version_ = 1;
+    void Touch() {
+        version_++;
+        static_version_++;
+    }
 
     // 8-wide trees (settings use_wide_bvh, the reference's default): the
     // binary trees above stay the build's working form, and the wide arrays
@@ -74,6 +80,8 @@ class Scene : public SceneBase {
     std::vector<rh_mbvh_node> mnodes_;
     std::vector<rh_mesh> wide_meshes_;
     uint32_t wide_macro_root_ = 0xffffffff, wide_light_root_ = 0xffffffff;
+    bool wide_meshes_dirty_ = true; // a mesh came or went: every tree is flattened again
+    uint32_t wide_mesh_end_ = 0;    // the mesh trees' part of mnodes_
     void RebuildWide();
 
     // fast (HLBVH) builds run on the device when the scene comes from a
@@ -134,6 +142,7 @@ public:
     // [first, first + count) is the range of the texel array they occupy.
     bool SetTextureData(uint32_t tex, const uint32_t *rgba, uint32_t &first, uint32_t &count);
     uint64_t version() const { return version_; }
+    uint64_t static_version() const { return static_version_; }
     const camera_t &camera(uint32_t i) const { return cams_[i].cam; }
 };
 
@@ -160,7 +169,7 @@ class Renderer : public RendererBase {
     mutable std::vector<shl1_data_t> host_sh_;
     mutable bool host_dirty_ = true;
     const Scene *uploaded_scene_ = nullptr;
-    uint64_t uploaded_version_ = 0;
+    uint64_t uploaded_version_ = 0, uploaded_static_version_ = 0;
     const float *uploaded_halton_ = nullptr;
     int uploaded_halton_epoch_ = -1;
     stats_t stats_ = {};

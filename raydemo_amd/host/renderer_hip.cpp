@@ -70,12 +70,16 @@ bool Renderer::PreparePass(Scene &s, RegionContext &region, rh_pass_desc &pass) 
     s.Finalize();
     if (uploaded_scene_ != &s || uploaded_version_ != s.version()) {
         const rh_scene_view v = s.View();
-        if (rh_upload_scene(ctx_, &v) != 0) {
+        // only instance transforms moved: the device keeps the rest
+        const bool moved_only = uploaded_scene_ == &s && uploaded_static_version_ == s.static_version();
+        if (!(moved_only && rh_update_instances(ctx_, &v) == 0) && rh_upload_scene(ctx_, &v) != 0) {
             last_error = rh_last_error();
+            uploaded_scene_ = nullptr;
             return false;
         }
         uploaded_scene_ = &s;
         uploaded_version_ = s.version();
+        uploaded_static_version_ = s.static_version();
     }
 
     memset(&pass, 0, sizeof(pass));
@@ -150,9 +154,10 @@ bool Renderer::SetMaterialColors(const std::shared_ptr<SceneBase> &_s, uint32_t 
     }
     for (uint32_t k = 0; k < n; k++)
         for (int ch = 0; ch < 3; ch++) s->materials_[first + k].main_color[ch] = rgb[3 * k + ch];
-    // the device copy is patched in place when this scene is the uploaded one;
-    // otherwise the next pass uploads the updated host arrays
-    if (uploaded_scene_ == s.get() && uploaded_version_ == s->version()) {
+    // the device copy is patched in place when this scene's materials are the
+    // uploaded ones (moved instances aside); otherwise the next pass uploads
+    // the updated host arrays
+    if (uploaded_scene_ == s.get() && uploaded_static_version_ == s->static_version()) {
         if (rh_set_material_colors(ctx_, first, n, rgb) != 0) {
             last_error = rh_last_error();
             return false;
@@ -169,7 +174,7 @@ bool Renderer::SetTextureData(const std::shared_ptr<SceneBase> &_s, uint32_t tex
         last_error = "texture index out of range";
         return false;
     }
-    if (uploaded_scene_ == s.get() && uploaded_version_ == s->version()) {
+    if (uploaded_scene_ == s.get() && uploaded_static_version_ == s->static_version()) {
         if (rh_set_texels(ctx_, first, count, &s->texels_[first]) != 0) {
             last_error = rh_last_error();
             return false;

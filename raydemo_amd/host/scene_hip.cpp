@@ -157,7 +157,7 @@ void Scene::SetEnvironment(const environment_desc_t &env) {
     memcpy(env_.env_col, env.env_col, 3 * sizeof(float));
     env_.env_clamp = env.env_clamp;
     env_.env_map = env.env_map == 0xffffffff ? default_env_texture_ : env.env_map;
-    version_++;
+    Touch();
 }
 
 // TextureAtlasTiled::Allocate placement (TextureAtlasRef.cpp:192-231): first
@@ -257,7 +257,7 @@ uint32_t Scene::AddTexture(const tex_desc_t &_t) {
     }
     textures_.push_back(t);
     tex_mips_.insert(tex_mips_.end(), mips.begin(), mips.end());
-    version_++;
+    Touch();
     return tex_index;
 }
 
@@ -289,7 +289,7 @@ uint32_t Scene::AddMaterial(const mat_desc_t &m) {
     }
     mat.textures[RH_NORMALS_TEXTURE] = m.normal_map != 0xffffffff ? m.normal_map : default_normals_texture_;
     materials_.push_back(mat);
-    version_++;
+    Touch();
     return (uint32_t)(materials_.size() - 1);
 }
 
@@ -472,7 +472,8 @@ uint32_t Scene::AddMesh(const mesh_desc_t &_m) {
     }
     vtx_indices_.insert(vtx_indices_.end(), new_vtx_indices.begin(), new_vtx_indices.end());
     wide_dirty_ = true;
-    version_++;
+    wide_meshes_dirty_ = true;
+    Touch();
     return (uint32_t)(meshes_.size() - 1);
 }
 
@@ -483,6 +484,8 @@ uint32_t Scene::AddMesh(const mesh_desc_t &_m) {
 void Scene::RemoveMesh(uint32_t i) {
     if (i >= meshes_.size()) return;
     Finalize();
+    wide_dirty_ = true;
+    wide_meshes_dirty_ = true;
     const rh_mesh m = meshes_[i];
 
     // the mesh's triangle index range: leaves of its tree cover one contiguous range
@@ -524,7 +527,7 @@ void Scene::RemoveMesh(uint32_t i) {
     meshes_.erase(meshes_.begin() + i);
     RemoveNodes(m.node_index, m.node_count);
     macro_dirty_ = true;
-    version_++;
+    Touch();
 }
 
 uint32_t Scene::AddLight(const light_desc_t &_l) {
@@ -560,7 +563,7 @@ uint32_t Scene::AddLight(const light_desc_t &_l) {
     l.brightness = std::max(l.col[0], std::max(l.col[1], l.col[2]));
     lights_.push_back(l);
     light_dirty_ = true;
-    version_++;
+    Touch();
     return (uint32_t)(lights_.size() - 1);
 }
 
@@ -579,6 +582,7 @@ uint32_t Scene::AddMeshInstance(uint32_t mesh_index, const float *xform) {
     mi.tr_index = (uint32_t)transforms_.size();
     transforms_.emplace_back();
     SetMeshInstanceTransform(mi_index, xform);
+    Touch();
     return mi_index;
 }
 
@@ -606,7 +610,7 @@ void Scene::RemoveMeshInstance(uint32_t i) {
     if (i >= mesh_instances_.size()) return;
     mesh_instances_.erase(mesh_instances_.begin() + i);
     macro_dirty_ = true;
-    version_++;
+    Touch();
 }
 
 // SceneRef::RemoveNodes (SceneRef.cpp:437-490), binary BVH
@@ -714,12 +718,20 @@ void Scene::Finalize() {
 // reference edits in place as meshes, instances and lights come and go), the
 // trees themselves do not.
 void Scene::RebuildWide() {
-    mnodes_.clear();
-    wide_meshes_ = meshes_;
-    for (rh_mesh &m : wide_meshes_) {
-        const uint32_t before = (uint32_t)mnodes_.size();
-        m.node_index = RayHip::FlattenBVH(nodes_.data(), m.node_index, mnodes_);
-        m.node_count = (uint32_t)mnodes_.size() - before;
+    if (wide_meshes_dirty_ || wide_meshes_.size() != meshes_.size()) {
+        mnodes_.clear();
+        wide_meshes_ = meshes_;
+        for (rh_mesh &m : wide_meshes_) {
+            const uint32_t before = (uint32_t)mnodes_.size();
+            m.node_index = RayHip::FlattenBVH(nodes_.data(), m.node_index, mnodes_);
+            m.node_count = (uint32_t)mnodes_.size() - before;
+        }
+        wide_mesh_end_ = (uint32_t)mnodes_.size();
+        wide_meshes_dirty_ = false;
+    } else {
+        // only the instance or light tree changed: the meshes' wide nodes,
+        // which do not depend on where their binary nodes sit, stay
+        mnodes_.resize(wide_mesh_end_);
     }
     wide_macro_root_ = macro_nodes_root_ != 0xffffffff ? RayHip::FlattenBVH(nodes_.data(), macro_nodes_root_, mnodes_)
                                                        : 0xffffffff;
