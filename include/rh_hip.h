@@ -84,6 +84,8 @@ int rh_get_ray_counts(rh_context *ctx, rh_ray_counts *out);
  * "path_tail" (-1: every bounce staged; b >= 0: bounces b.. run as one path-per-lane launch),
  * "tail_occupancy" (2 or 3 waves per SIMD for the path-tail kernel: registers vs spills),
  * "tail_refill" (0/1: the path tail as a persistent kernel whose lanes take the next path when theirs ends),
+ * "tiled_primary" (1, the default: a pass queues its primary rays in 8x8 pixel tiles, one wave each; 0: in
+ *   row-major order; images and ray counts are the same),
  * "refill" (0/1: closest-hit trace as a persistent kernel whose lanes take new rays as theirs finish;
  *   binary and 8-wide trees, the latter with the trace_occupancy register cap),
  * "trace_occupancy" (0, 5 or 6: the 8-wide trace and shadow kernels as compiled, or capped at 5 or 6 waves

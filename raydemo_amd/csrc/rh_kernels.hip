@@ -31,7 +31,35 @@ using rh_rt::upload_halton;
 
 namespace {
 
-// one lane per pixel of the region
+// Queue position i -> pixel (x, y) of a rw x rh region in 8x8 tiles, one
+// wave each: the tiles cover the part whose sides are multiples of 8, a row
+// of tiles after another; the right and then the bottom remainder follow row
+// by row.  A bijection on [0, rw rh).  A wave's rays then start from a compact
+// block of pixels instead of a 64-pixel strip, and so do the bounce and
+// shadow rays that keep their queue order, which keeps a wave's traversals
+// on fewer nodes.
+__device__ __forceinline__ void tiled_pixel(int i, int rw, int rh, int &x, int &y) {
+    const int w8 = rw & ~7, h8 = rh & ~7, na = w8 * h8;
+    if (i < na) {
+        const int t = i >> 6, j = i & 63, tw = w8 >> 3;
+        x = ((t % tw) << 3) + (j & 7);
+        y = ((t / tw) << 3) + (j >> 3);
+        return;
+    }
+    i -= na;
+    const int rwr = rw - w8, nb = rwr * h8;
+    if (i < nb) {
+        x = w8 + i % rwr;
+        y = i / rwr;
+        return;
+    }
+    i -= nb;
+    x = i % rw;
+    y = h8 + i / rw;
+}
+
+// one lane per pixel of the region, in row-major order or (Tiled) in tiles
+template <bool Tiled>
 __global__ void __launch_bounds__(kBlock) k_gen_primary(PassConst pc, const float *__restrict__ halton,
                                                         rh_ray *__restrict__ rays, rh_ray_aux *__restrict__ aux,
                                                         int *__restrict__ count) {
@@ -39,8 +67,9 @@ __global__ void __launch_bounds__(kBlock) k_gen_primary(PassConst pc, const floa
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *count = rw * rh_;
     if (i >= rw * rh_) return;
-    const int x = pc.rect[0] + i % rw, y = pc.rect[1] + i / rw;
-    store_ray(rays, aux, (uint32_t)i, gen_primary_ray(pc, halton, x, y));
+    int x = i % rw, y = i / rw;
+    if (Tiled) tiled_pixel(i, rw, rh_, x, y);
+    store_ray(rays, aux, (uint32_t)i, gen_primary_ray(pc, halton, pc.rect[0] + x, pc.rect[1] + y));
 }
 
 // ---- geometry camera: SampleMeshInTextureSpace (CoreRef.cpp:507-613) ----
@@ -761,6 +790,7 @@ struct rh_context {
     int tail_occupancy = 2;    // waves per SIMD the path-tail kernel is compiled for (2 or 3)
     int refill = 0;            // closest-hit trace: persistent kernel with lane refill
     int tail_refill = 0;       // path tail: persistent kernel whose lanes take a new path when theirs ends
+    int tiled_primary = 1;     // primary rays queued in 8x8 pixel tiles (0: row-major)
     std::vector<std::pair<std::pair<const void *, size_t>, int>> resident; // (kernel, LDS bytes) -> blocks per CU
     int trace_occupancy = 0;   // 8-wide trace/shadow kernels: 0 = as compiled (4 waves/SIMD), 5 or 6 = capped registers
     int shadow_occupancy = -1; // the shadow kernel's own setting (-1: as trace_occupancy)
@@ -1568,8 +1598,12 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool k
         RH_CHECK(hipGetLastError());
         tm.mark(kRayGen); // the geometry camera makes its hits too: no primary trace
     } else {
-        hipLaunchKernelGGL(k_gen_primary, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, pc, c->halton,
-                           c->rays[0], c->aux[0], cnt[0]);
+        if (c->tiled_primary)
+            hipLaunchKernelGGL(k_gen_primary<true>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, pc,
+                               c->halton, c->rays[0], c->aux[0], cnt[0]);
+        else
+            hipLaunchKernelGGL(k_gen_primary<false>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, pc,
+                               c->halton, c->rays[0], c->aux[0], cnt[0]);
         RH_CHECK(hipGetLastError());
         tm.mark(kRayGen);
         rc = launch_trace(c, trace_args(c, c->rays[0], cnt[0], n, c->hits, nullptr, count_primary), n, false);
@@ -1854,6 +1888,8 @@ int rh_set_option(rh_context *c, const char *name, int value) {
         c->refill = value != 0;
     } else if (n == "tail_refill") {
         c->tail_refill = value != 0;
+    } else if (n == "tiled_primary") {
+        c->tiled_primary = value != 0;
     } else if (n == "shadow_occupancy") {
         if (value != -1 && value != 0 && value != 5 && value != 6)
             return arg_error("shadow_occupancy must be -1, 0, 5 or 6");
@@ -1893,6 +1929,7 @@ int rh_get_option(rh_context *c, const char *name, int *value) {
     else if (n == "tail_occupancy") *value = c->tail_occupancy;
     else if (n == "refill") *value = c->refill;
     else if (n == "tail_refill") *value = c->tail_refill;
+    else if (n == "tiled_primary") *value = c->tiled_primary;
     else if (n == "scene_uploads") *value = c->n_uploads;
     else if (n == "instance_updates") *value = c->n_instance_updates;
     else if (n == "trace_occupancy") *value = c->trace_occupancy;
@@ -2014,7 +2051,7 @@ int rh_gen_primary_rays(rh_context *c, const rh_camera *cam, const int32_t rect[
     c->w = save_w;
     c->h = save_h;
     RH_CHECK(upload_halton(c->halton, halton, c->stream));
-    hipLaunchKernelGGL(k_gen_primary, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, pc, c->halton,
+    hipLaunchKernelGGL(k_gen_primary<false>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, pc, c->halton,
                        c->rays[0], c->aux[0], c->counters + 0);
     RH_CHECK(hipGetLastError());
     return download_packets(c, out, (size_t)n, 0);

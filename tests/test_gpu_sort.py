@@ -55,8 +55,10 @@ def test_sort_matches_reference(ref_lib, host_lib, dev_lib, n, seed, deg):
 @pytest.mark.parametrize("lds_depth", [24, 8])
 def test_sort_is_invisible_in_the_image(host_lib, lds_depth):
     """Sorting (by the full key or only its top bits), the LDS share of the
-    traversal stack, deferred or inline shadow rays and the path-tail launch
-    change only the order of the work: the image is identical."""
+    traversal stack, deferred or inline shadow rays, the path-tail launch and
+    the primary queue's order (8x8 tiles by default, row-major; 90 rows leave
+    a remainder below the tiles) change only the order of the work: the image
+    is identical."""
     imgs = []
     for opts in ({"sort_rays": 0, "path_tail": -1}, {"sort_rays": 1, "path_tail": -1},
                  {"sort_rays": 1, "sort_bits": 8}, {"sort_rays": 1, "sort_bits": 16},
@@ -65,7 +67,8 @@ def test_sort_is_invisible_in_the_image(host_lib, lds_depth):
                  {"path_tail": 2, "tail_occupancy": 3, "defer_nee": 0}, {"refill": 1},
                  {"refill": 1, "path_tail": -1, "sort_rays": 1, "lds_stack_depth": 4},
                  {"path_tail": 0, "tail_refill": 1}, {"path_tail": 2, "tail_refill": 1, "tail_occupancy": 3},
-                 {"path_tail": 1, "tail_refill": 1, "defer_nee": 0}):
+                 {"path_tail": 1, "tail_refill": 1, "defer_nee": 0}, {"tiled_primary": 0},
+                 {"tiled_primary": 0, "sort_rays": 1, "path_tail": -1}, {"tiled_primary": 0, "refill": 1}):
         r = A.Renderer(host_lib, 160, 90, A.RENDERER_HIP)
         r.set_option("lds_stack_depth", lds_depth)
         for k, v in opts.items():
