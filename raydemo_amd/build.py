@@ -46,9 +46,23 @@ def hipcc() -> str:
 
 
 def build_hip():
+    """Each source compiled to its own object (in parallel), then linked."""
     srcs = [CSRC / "rh_kernels.hip", CSRC / "rh_batch.hip", CSRC / "rh_bvh_build.hip"]
+    obj_dir = ROOT / "_build"
+    obj_dir.mkdir(exist_ok=True)
+    flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    procs, objs = [], []
+    for src in srcs:
+        obj = obj_dir / (src.stem + ".o")
+        cmd = [hipcc(), *flags, "-c", f"-I{INC}", f"-I{CSRC}", src, "-o", obj]
+        print("+", " ".join(str(c) for c in cmd), flush=True)
+        procs.append(subprocess.Popen([str(c) for c in cmd]))
+        objs.append(obj)
+    failed = [p.wait() for p in procs]
+    if any(failed):
+        raise subprocess.CalledProcessError(max(failed), "hipcc -c")
     tmp = HIP_LIB.with_suffix(".so.tmp")
-    _run([hipcc(), *HIPCC_FLAGS, f"-I{INC}", f"-I{CSRC}", *srcs, "-o", tmp])
+    _run([hipcc(), "-shared", "--offload-arch=gfx950", *objs, "-o", tmp])
     os.replace(tmp, HIP_LIB)
 
 

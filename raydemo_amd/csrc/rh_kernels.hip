@@ -723,11 +723,52 @@ __global__ void __launch_bounds__(kBlock) k_gather(const rh_ray *__restrict__ ra
 
 // ====================================================================== host
 
+// The working set of one pass in flight: its stream, the pass's framebuffer,
+// the ray queues, hits, sort scratch, traversal spill slices, the deferred
+// NEE buffers and the queue counters.  Two passes may be in flight (option
+// pass_overlap): the context's fields of these names are the bound slot's,
+// and PassSlot holds the other one's (swap_slot exchanges them).
+struct PassSlot {
+    hipStream_t stream = nullptr;
+    float4 *temp = nullptr;
+    size_t temp_cap = 0;
+    size_t max_rays = 0;
+    rh_ray *rays[2] = {nullptr, nullptr};
+    rh_ray_aux *aux[2] = {nullptr, nullptr};
+    rh_hit *hits = nullptr;
+    uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
+    uint32_t *hist = nullptr;
+    uint32_t *spill = nullptr;
+    size_t spill_cap = 0, spill_words = 0;
+    size_t nee_cap = 0;
+    float4 *nee_ray = nullptr, *nee_term = nullptr;
+    int2 *nee_aux = nullptr;
+    uint8_t *nee_vis = nullptr;
+    float4 *pend_c = nullptr, *pend_alb = nullptr;
+    int *pend_count = nullptr;
+    uint32_t *shadow_queue = nullptr;
+    int *counters = nullptr;
+};
+#define RH_SLOT_FIELDS(X)                                                                                               \
+    X(stream) X(temp) X(temp_cap) X(max_rays) X(rays) X(aux) X(hits) X(keys) X(vals) X(hist) X(spill) X(spill_cap)     \
+        X(spill_words) X(nee_cap) X(nee_ray) X(nee_term) X(nee_aux) X(nee_vis) X(pend_c) X(pend_alb) X(pend_count)     \
+            X(shadow_queue) X(counters)
+
 struct rh_context {
     int device = 0;
     hipStream_t stream = nullptr;
     int w = 0, h = 0;
     size_t max_rays = 0;
+    // passes in flight: the other slot, which slot is bound, and the order
+    // between them (see begin_overlapped_pass)
+    PassSlot alt;
+    int bound = 0;
+    int pass_overlap = 1; // option: plain passes alternate between the two slots
+    bool dirty = true;    // device work other than an overlapped pass was queued since the last pass
+    hipEvent_t ev_mix = nullptr;  // the last pass's mix (passes mix into clean in pass order)
+    bool mix_recorded = false;
+    hipEvent_t ev_join = nullptr; // orders an overlapped pass after everything queued before it
+    size_t temp_cap = 0;          // pixels the bound slot's temp holds
 
     // scene
     std::vector<DevBuf> scene_bufs;    // the arrays instance transforms leave alone
@@ -835,39 +876,69 @@ struct rh_context {
 
 namespace {
 
+// Exchanges the bound slot's working set with the other one's.
+void swap_slot(rh_context *c) {
+#define RH_SWAP_FIELD(f) std::swap(c->f, c->alt.f);
+    RH_SLOT_FIELDS(RH_SWAP_FIELD)
+#undef RH_SWAP_FIELD
+    c->bound ^= 1;
+}
+
+// Waits for both slots' streams.
+int sync_slots(rh_context *c) {
+    RH_CHECK(hipStreamSynchronize(c->stream));
+    if (c->alt.stream) RH_CHECK(hipStreamSynchronize(c->alt.stream));
+    return 0;
+}
+
+// Frees the bound slot's frame-sized buffers (its queue counters and stream
+// stay); the next pass on the slot allocates them again.
+void free_slot_buffers(rh_context *c) {
+    release(c->temp);
+    for (int k = 0; k < 2; k++) {
+        release(c->rays[k]);
+        release(c->aux[k]);
+        release(c->keys[k]);
+        release(c->vals[k]);
+    }
+    release(c->hits);
+    release(c->hist);
+    release(c->spill);
+    release(c->nee_ray);
+    release(c->nee_term);
+    release(c->nee_aux);
+    release(c->nee_vis);
+    release(c->pend_c);
+    release(c->pend_alb);
+    release(c->pend_count);
+    release(c->shadow_queue);
+    c->temp_cap = c->max_rays = c->spill_cap = c->spill_words = c->nee_cap = 0;
+}
+
+int ensure_rays(rh_context *c, size_t n);
+
 int alloc_frame(rh_context *c, int w, int h) {
     const size_t np = (size_t)w * h;
+    int rc = sync_slots(c);
+    if (rc) return rc;
     // spherical-harmonic data starts again (zeroed) at the next OutputSH pass
-    free_ptr(c->sh_temp);
-    free_ptr(c->sh_clean);
-    c->sh_temp = c->sh_clean = nullptr;
-    free_ptr(c->temp);
-    free_ptr(c->clean);
-    free_ptr(c->final_);
-    for (int k = 0; k < 2; k++) {
-        free_ptr(c->rays[k]);
-        free_ptr(c->aux[k]);
-        free_ptr(c->keys[k]);
-        free_ptr(c->vals[k]);
-    }
-    free_ptr(c->hits);
-    free_ptr(c->hist);
-    free_ptr(c->spill);
-    c->temp = c->clean = c->final_ = nullptr;
-    c->hits = nullptr;
-    c->hist = nullptr;
-    c->spill = nullptr;
-    for (int k = 0; k < 2; k++) c->rays[k] = nullptr, c->aux[k] = nullptr, c->keys[k] = nullptr, c->vals[k] = nullptr;
+    release(c->sh_temp);
+    release(c->sh_clean);
+    release(c->clean);
+    release(c->final_);
+    // both slots' pass buffers are sized for the old frame: the bound slot's
+    // are allocated again below, the other's at its next pass
+    free_slot_buffers(c);
+    swap_slot(c);
+    free_slot_buffers(c);
+    swap_slot(c);
     c->w = w;
     c->h = h;
-    c->max_rays = std::max<size_t>(np, 1);
-    RH_CHECK(hipMalloc(&c->temp, np * sizeof(float4)));
     RH_CHECK(hipMalloc(&c->clean, np * sizeof(float4)));
     RH_CHECK(hipMalloc(&c->final_, np * sizeof(float4)));
-    RH_CHECK(hipMemsetAsync(c->temp, 0, np * sizeof(float4), c->stream));
     RH_CHECK(hipMemsetAsync(c->clean, 0, np * sizeof(float4), c->stream));
     RH_CHECK(hipMemsetAsync(c->final_, 0, np * sizeof(float4), c->stream));
-    return 0;
+    return ensure_rays(c, std::max<size_t>(np, 1));
 }
 
 int ensure_nee(rh_context *c);
@@ -876,8 +947,19 @@ int ensure_nee(rh_context *c);
 // is allocated: after a failed allocation the context holds null pointers and
 // no capacity, and the next call allocates again.
 int ensure_rays(rh_context *c, size_t n) {
+    const size_t np = (size_t)c->w * c->h;
+    if (np && (np > c->temp_cap || !c->temp)) {
+        // the bound slot's pass framebuffer (its last pass may still read the old one)
+        RH_CHECK(hipStreamSynchronize(c->stream));
+        release(c->temp);
+        c->temp_cap = 0;
+        RH_CHECK(hipMalloc(&c->temp, np * sizeof(float4)));
+        RH_CHECK(hipMemsetAsync(c->temp, 0, np * sizeof(float4), c->stream));
+        c->temp_cap = np;
+    }
     if (n <= c->max_rays && c->rays[0]) return ensure_nee(c);
     const size_t m = std::max(n, c->max_rays);
+    RH_CHECK(hipStreamSynchronize(c->stream)); // the slot's last pass may still use its queues
     c->max_rays = 0;
     for (int k = 0; k < 2; k++) {
         release(c->rays[k]);
@@ -909,6 +991,7 @@ size_t spill_words(const rh_context *c) { return (size_t)kSpillDepth * (c->sc.wn
 int ensure_spill(rh_context *c, size_t lanes) {
     lanes = (lanes + kBlock - 1) / kBlock * kBlock; // whole blocks: persistent lanes index slices too
     if (lanes <= c->spill_cap && c->spill && c->spill_words >= spill_words(c)) return 0;
+    RH_CHECK(hipStreamSynchronize(c->stream)); // the slot's last pass may still use its slices
     release(c->spill);
     const size_t cap = std::max(lanes, c->spill_cap), words = spill_words(c);
     c->spill_cap = 0;
@@ -925,6 +1008,7 @@ int ensure_nee(rh_context *c) {
     if (!c->nee_slots) return ensure_spill(c, m);
     const size_t cap = m * c->nee_slots;
     if (cap > c->nee_cap || !c->nee_ray || !c->shadow_queue) {
+        RH_CHECK(hipStreamSynchronize(c->stream)); // the slot's last pass may still use them
         release(c->nee_ray);
         release(c->nee_term);
         release(c->nee_aux);
@@ -1339,17 +1423,24 @@ int rh_create(int device, rh_context **out) {
     RH_CHECK(hipSetDevice(device));
     auto *c = new rh_context;
     c->device = device;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    // one stream per pass slot (both at the default priority)
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->alt.stream, hipStreamNonBlocking) != hipSuccess) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         rh_rt::err() = "hipStreamCreate failed";
         return -1003;
     }
+    RH_CHECK(hipEventCreateWithFlags(&c->ev_mix, hipEventDisableTiming));
+    RH_CHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     hipDeviceProp_t prop;
     RH_CHECK(hipGetDeviceProperties(&prop, device));
     c->cus = prop.multiProcessorCount;
     RH_CHECK(hipMalloc(&c->counters, 16 * sizeof(int)));
+    RH_CHECK(hipMalloc(&c->alt.counters, 16 * sizeof(int)));
     RH_CHECK(hipMalloc(&c->stats_dev, 8 * sizeof(unsigned long long)));
     RH_CHECK(hipMemsetAsync(c->counters, 0, 16 * sizeof(int), c->stream));
+    RH_CHECK(hipMemsetAsync(c->alt.counters, 0, 16 * sizeof(int), c->stream));
     RH_CHECK(hipMemsetAsync(c->stats_dev, 0, 8 * sizeof(unsigned long long), c->stream));
     RH_CHECK(hipMalloc(&c->visits, kVisitWords * sizeof(unsigned long long)));
     RH_CHECK(hipMemsetAsync(c->visits, 0, kVisitWords * sizeof(unsigned long long), c->stream));
@@ -1362,7 +1453,7 @@ int rh_create(int device, rh_context **out) {
 void rh_destroy(rh_context *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
+    (void)sync_slots(c);
     for (auto &b : c->scene_bufs) free_ptr(b.p);
     for (auto &b : c->instance_bufs) free_ptr(b.p);
     free_ptr(c->geo_winner);
@@ -1374,27 +1465,14 @@ void rh_destroy(rh_context *c) {
     free_ptr(c->tape_rec);
     free_ptr(c->tape_rec_tex);
     free_ptr(c->tape_nverts);
-    free_ptr(c->temp);
     free_ptr(c->clean);
     free_ptr(c->final_);
     for (int k = 0; k < 2; k++) {
-        free_ptr(c->rays[k]);
-        free_ptr(c->aux[k]);
-        free_ptr(c->keys[k]);
-        free_ptr(c->vals[k]);
+        free_slot_buffers(c);
+        free_ptr(c->counters);
+        c->counters = nullptr;
+        swap_slot(c);
     }
-    free_ptr(c->hits);
-    free_ptr(c->hist);
-    free_ptr(c->spill);
-    free_ptr(c->nee_ray);
-    free_ptr(c->nee_term);
-    free_ptr(c->nee_aux);
-    free_ptr(c->nee_vis);
-    free_ptr(c->pend_c);
-    free_ptr(c->pend_alb);
-    free_ptr(c->pend_count);
-    free_ptr(c->shadow_queue);
-    free_ptr(c->counters);
     free_ptr(c->stats_dev);
     free_ptr(c->halton);
     free_ptr(c->visits);
@@ -1407,11 +1485,15 @@ void rh_destroy(rh_context *c) {
     for (auto &p : c->stage_pending)
         for (auto &m : p) (void)hipEventDestroy(m.first);
     for (auto &e : c->stage_free) (void)hipEventDestroy(e);
+    if (c->ev_mix) (void)hipEventDestroy(c->ev_mix);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     (void)hipStreamDestroy(c->stream);
+    if (c->alt.stream) (void)hipStreamDestroy(c->alt.stream);
     delete c;
 }
 
 int rh_resize(rh_context *c, int w, int h) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     if (w <= 0 || h <= 0 || w > 65535 || h > 65535) return arg_error("framebuffer size must be 1..65535 per side");
     RH_CHECK(hipSetDevice(c->device));
     int rc = alloc_frame(c, w, h);
@@ -1422,6 +1504,7 @@ int rh_resize(rh_context *c, int w, int h) {
 static void tape_forget(rh_context *c);
 
 int rh_clear(rh_context *c, const float rgba[4]) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     const size_t n = (size_t)c->w * c->h;
     if (!n) return 0;
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->clean, n,
@@ -1434,6 +1517,7 @@ int rh_clear(rh_context *c, const float rgba[4]) {
 }
 
 int rh_read_sh(rh_context *c, float *out, size_t count) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     const size_t n = (size_t)c->w * c->h * 12;
     if (!c->sh_clean) return 1;
     if (count < n) return arg_error("output buffer too small");
@@ -1450,6 +1534,7 @@ static void static_counts(const rh_scene_view *v, size_t out[9]) {
 }
 
 int rh_upload_scene(rh_context *c, const rh_scene_view *v) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     RH_CHECK(hipSetDevice(c->device));
     RH_CHECK(hipStreamSynchronize(c->stream));
     c->have_scene = false;
@@ -1486,6 +1571,7 @@ int rh_upload_scene(rh_context *c, const rh_scene_view *v) {
 }
 
 int rh_update_instances(rh_context *c, const rh_scene_view *v) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     if (!c->have_scene) return arg_error("rh_update_instances: no scene uploaded");
     size_t k[9];
     static_counts(v, k);
@@ -1512,6 +1598,7 @@ int rh_update_instances(rh_context *c, const rh_scene_view *v) {
 }
 
 int rh_set_halton(rh_context *c, const float *seq, size_t count) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     if (count < (size_t)RH_HALTON_COUNT * RH_HALTON_SEQ_LEN) return arg_error("Halton table needs 256 x 24 floats");
     RH_CHECK(upload_halton(c->halton, seq, c->stream));
     // the host table may change right after this call returns
@@ -1730,6 +1817,9 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool k
     }
 
     if (replay) return 0; // the replay leaves the image alone
+    // passes mix into the accumulated image in pass order: this one after the
+    // last pass's mix, which may sit on the other slot's stream
+    if (c->mix_recorded) RH_CHECK(hipStreamWaitEvent(c->stream, c->ev_mix, 0));
     if (out_sh) {
         hipLaunchKernelGGL(k_sh_finish, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->temp, c->sh_temp,
                            c->sh_clean, rx, ry, rw, rhh, c->w, pc.mix_k);
@@ -1739,6 +1829,26 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool k
     hipLaunchKernelGGL(k_mix_tonemap, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->temp, c->clean, c->final_, rx,
                        ry, rw, rhh, c->w, pc.mix_k, p->cam.dtype, p->cam.gamma, p->cam.pass_settings.flags);
     RH_CHECK(hipGetLastError());
+    RH_CHECK(hipEventRecord(c->ev_mix, c->stream));
+    c->mix_recorded = true;
+    return 0;
+}
+
+// A plain pass goes to the slot the last pass did not use, so that its first
+// stages run while the last pass's path tail drains (a tail keeps few waves
+// resident once most of its paths have ended).  Passes share only the
+// accumulated image: each mixes after the last one's mix (run_pass), so the
+// image is the one passes run one after another give.  The bound stream is
+// then ordered after every earlier pass, and other entry points queue their
+// work there; when any did since the last pass (dirty), the new pass waits
+// for all of it.
+int begin_overlapped_pass(rh_context *c) {
+    if (c->dirty) RH_CHECK(hipEventRecord(c->ev_join, c->stream));
+    swap_slot(c);
+    if (c->dirty) {
+        RH_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+        c->dirty = false;
+    }
     return 0;
 }
 
@@ -1812,6 +1922,18 @@ static AdjRecords tape_records(rh_context *c, int slot, const rh_pass_desc *p, b
 int rh_render_pass(rh_context *c, const rh_pass_desc *p) {
     const size_t lanes = (size_t)std::max(0, p->rect[2]) * std::max(0, p->rect[3]);
     const int slot = tape_slot(c, p, lanes);
+    const bool plain = slot < 0 && p->cam.type != RH_CAM_GEO && !(p->cam.pass_settings.flags & RH_OUTPUT_SH);
+    if (plain && c->pass_overlap) {
+        int rc = begin_overlapped_pass(c);
+        if (rc == 0) rc = run_pass(c, p, nullptr);
+        if (rc) {
+            // a pass that failed part-way has no mix the next work can follow
+            (void)sync_slots(c);
+            c->dirty = true;
+        }
+        return rc;
+    }
+    c->dirty = true;
     if (slot < 0) return run_pass(c, p, nullptr);
     int rc = tape_alloc(c, lanes);
     if (rc) return rc;
@@ -1826,6 +1948,7 @@ int rh_render_pass(rh_context *c, const rh_pass_desc *p) {
 }
 
 int rh_read_pixels(rh_context *c, float *rgba, size_t count) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     const size_t n = (size_t)c->w * c->h * 4;
     if (count < n) return arg_error("output buffer too small");
     RH_CHECK(hipMemcpyAsync(rgba, c->final_, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
@@ -1834,8 +1957,8 @@ int rh_read_pixels(rh_context *c, float *rgba, size_t count) {
 }
 
 int rh_synchronize(rh_context *c) {
-    RH_CHECK(hipStreamSynchronize(c->stream));
-    return 0;
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
+    return sync_slots(c);
 }
 
 int rh_get_stats(rh_context *c, rh_stats *out) {
@@ -1847,6 +1970,7 @@ int rh_get_stats(rh_context *c, rh_stats *out) {
 }
 
 int rh_reset_stats(rh_context *c) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     const int rc = stage_harvest(c, true);
     if (rc) return rc;
     for (double &t : c->stage_us) t = 0.0;
@@ -1855,6 +1979,7 @@ int rh_reset_stats(rh_context *c) {
 }
 
 int rh_get_ray_counts(rh_context *c, rh_ray_counts *out) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     unsigned long long v[8];
     RH_CHECK(hipMemcpyAsync(v, c->stats_dev, sizeof(v), hipMemcpyDeviceToHost, c->stream));
     RH_CHECK(hipStreamSynchronize(c->stream));
@@ -1890,6 +2015,8 @@ int rh_set_option(rh_context *c, const char *name, int value) {
         c->tail_refill = value != 0;
     } else if (n == "tiled_primary") {
         c->tiled_primary = value != 0;
+    } else if (n == "pass_overlap") {
+        c->pass_overlap = value != 0;
     } else if (n == "shadow_occupancy") {
         if (value != -1 && value != 0 && value != 5 && value != 6)
             return arg_error("shadow_occupancy must be -1, 0, 5 or 6");
@@ -1940,7 +2067,10 @@ int rh_get_option(rh_context *c, const char *name, int *value) {
     else if (n == "adjoint_tape_texels") *value = c->tape_texels;
     else if (n == "adjoint_occupancy") *value = c->adjoint_occupancy;
     else if (n == "defer_nee") *value = c->defer_nee;
+    else if (n == "pass_overlap") *value = c->pass_overlap;
+    else if (n == "bound_slot") *value = c->bound; // read-only: the slot of the last pass
     else if (n == "stack_overflows") {
+        c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
         unsigned long long v[8];
         RH_CHECK(hipMemcpyAsync(v, c->stats_dev, sizeof(v), hipMemcpyDeviceToHost, c->stream));
         RH_CHECK(hipStreamSynchronize(c->stream));
@@ -1950,14 +2080,17 @@ int rh_get_option(rh_context *c, const char *name, int *value) {
 }
 
 int rh_device_synchronize(rh_context *c) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     RH_CHECK(hipSetDevice(c->device));
     RH_CHECK(hipDeviceSynchronize());
     return 0;
 }
 
 int rh_get_kernel_profile(rh_context *c, rh_kernel_profile *out) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     memset(out, 0, sizeof(*out));
-    RH_CHECK(hipStreamSynchronize(c->stream));
+    int rc = sync_slots(c);
+    if (rc) return rc;
     for (const auto &m : c->ev_marks) {
         float ms = 0;
         RH_CHECK(hipEventElapsedTime(&ms, c->ev_pool[m.second.first], c->ev_pool[m.second.second]));
@@ -1978,7 +2111,9 @@ int rh_get_kernel_profile(rh_context *c, rh_kernel_profile *out) {
 }
 
 int rh_reset_kernel_profile(rh_context *c) {
-    RH_CHECK(hipStreamSynchronize(c->stream));
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
+    int rc = sync_slots(c);
+    if (rc) return rc;
     c->ev_marks.clear();
     c->ev_used = 0;
     RH_CHECK(hipMemsetAsync(c->visits, 0, kVisitWords * sizeof(unsigned long long), c->stream));
@@ -2035,6 +2170,7 @@ static int download_packets(rh_context *c, rh_ray_packet *out, size_t n, int q) 
 
 int rh_gen_primary_rays(rh_context *c, const rh_camera *cam, const int32_t rect[4], int w, int h, int iteration,
                         const float *halton, rh_ray_packet *out) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     const int n = rect[2] * rect[3];
     if (n <= 0) return 0;
     int rc = ensure_rays(c, (size_t)n);
@@ -2058,6 +2194,7 @@ int rh_gen_primary_rays(rh_context *c, const rh_camera *cam, const int32_t rect[
 }
 
 int rh_trace_rays(rh_context *c, const rh_ray_packet *rays, size_t n, int any_hit, const float *tmax, rh_hit *out) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     if (!c->have_scene) return arg_error("no scene uploaded");
     if (!n) return 0;
     int rc = ensure_rays(c, n);
@@ -2078,6 +2215,7 @@ int rh_trace_rays(rh_context *c, const rh_ray_packet *rays, size_t n, int any_hi
 }
 
 int rh_trace_device(rh_context *c, const rh_ray *d_rays, int n, int any_hit, rh_hit *d_hits, void *stream) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     if (!c->have_scene) return arg_error("no scene uploaded");
     if (n <= 0) return 0;
     if ((size_t)n > c->max_rays) return arg_error("n exceeds the context's ray capacity (resize first)");
@@ -2098,6 +2236,7 @@ int rh_trace_device(rh_context *c, const rh_ray *d_rays, int n, int any_hit, rh_
 
 int rh_ray_hashes(rh_context *c, const rh_ray_packet *rays, size_t n, const float root_min[3], const float cell_size[3],
                   uint32_t *out) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     if (!n) return 0;
     int rc = ensure_rays(c, n);
     if (rc) return rc;
@@ -2116,6 +2255,7 @@ int rh_ray_hashes(rh_context *c, const rh_ray_packet *rays, size_t n, const floa
 }
 
 int rh_sort_rays(rh_context *c, rh_ray_packet *rays, size_t n, const float root_min[3], const float cell_size[3]) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     if (!n) return 0;
     int rc = ensure_rays(c, n);
     if (rc) return rc;
@@ -2168,6 +2308,7 @@ int rh_sort_rays_noalloc(const rh_ray *d_rays, const rh_ray_aux *d_aux, size_t n
 
 int rh_shade_rays(rh_context *c, const rh_pass_info *pi, const rh_ray_packet *rays, const rh_hit *hits, size_t n,
                   const float *halton, float *out_rgba, rh_ray_packet *out_secondary, int *out_count, int *out_src) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     if (!c->have_scene) return arg_error("no scene uploaded");
     *out_count = 0;
     if (!n) return 0;
@@ -2222,6 +2363,7 @@ int rh_shade_rays(rh_context *c, const rh_pass_info *pi, const rh_ray_packet *ra
 
 int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgba, float *d_grad, float *d_texel_grad,
                     float scale) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     if (!c->have_scene) return arg_error("no scene uploaded");
     const int rx = p->rect[0], ry = p->rect[1], rw = p->rect[2], rhh = p->rect[3];
     if (rw <= 0 || rhh <= 0 || rx < 0 || ry < 0 || rx + rw > c->w || ry + rhh > c->h)
@@ -2393,6 +2535,7 @@ int rh_adjoint_pass(rh_context *c, const rh_pass_desc *p, const float *d_adj_rgb
 }
 
 int rh_set_gradient_textures(rh_context *c, const uint32_t *ids, uint32_t n) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     if (!n) {
         free_ptr(c->grad_tex_mask);
         c->grad_tex_mask = nullptr;
@@ -2419,6 +2562,7 @@ int rh_set_gradient_textures(rh_context *c, const uint32_t *ids, uint32_t n) {
 }
 
 int rh_set_material_colors(rh_context *c, uint32_t first, uint32_t n, const float *rgb) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     if (!c->have_scene) return arg_error("no scene uploaded");
     if ((size_t)first + n > c->n_materials) return arg_error("material range outside the scene");
     if (!n) return 0;
@@ -2441,6 +2585,7 @@ int rh_set_material_colors(rh_context *c, uint32_t first, uint32_t n, const floa
 }
 
 int rh_set_texels(rh_context *c, uint32_t first, uint32_t n, const uint32_t *rgba) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     if (!c->have_scene) return arg_error("no scene uploaded");
     if ((size_t)first + n > c->n_texels) return arg_error("texel range outside the scene");
     if (!n) return 0;
@@ -2453,6 +2598,7 @@ int rh_set_texels(rh_context *c, uint32_t first, uint32_t n, const uint32_t *rgb
 }
 
 int rh_read_accum(rh_context *c, float *dst, size_t count) {
+    c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
     const size_t n = (size_t)c->w * c->h * 4;
     if (count < n) return arg_error("output buffer too small");
     RH_CHECK(hipMemcpyAsync(dst, c->clean, n * sizeof(float), hipMemcpyDefault, c->stream));
