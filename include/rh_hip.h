@@ -130,7 +130,7 @@ int rh_sort_rays(rh_context *ctx, rh_ray_packet *rays, size_t n, const float roo
  * d_rays_out/d_aux_out, as rh_sort_rays does, on `stream` (null: the default
  * stream) of the current device, asynchronously.  It allocates nothing: the
  * keys, values and digit counts live in a fixed device array of the library
- * (about 34 MB per GPU), so n is at most RH_NOALLOC_MAX_RAYS (larger n is an
+ * (about 42 MB per GPU), so n is at most RH_NOALLOC_MAX_RAYS (larger n is an
  * argument error) and calls that overlap on one GPU race on that array.  The
  * output must not alias the input. */
 #define RH_NOALLOC_MAX_RAYS 2097152

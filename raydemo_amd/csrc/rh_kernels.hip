@@ -689,9 +689,6 @@ __global__ void k_set_int(int *p, int v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
-__global__ void k_copy_int(int *dst, const int *src) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
-}
 
 // ------------------------------------------------------------------ sorting
 // Keys for the queue (get_ray_hash) and the identity permutation.
@@ -705,18 +702,6 @@ __global__ void __launch_bounds__(kBlock) k_ray_keys(const rh_ray *__restrict__ 
     const float4 a = p[0], b = p[1];
     keys[i] = ray_hash(f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, pc.root_min, pc.cell_size);
     vals[i] = (uint32_t)i;
-}
-
-__global__ void __launch_bounds__(kBlock) k_gather(const rh_ray *__restrict__ rays_in, const rh_ray_aux *__restrict__ aux_in,
-                                                   const uint32_t *__restrict__ perm, const int *__restrict__ count,
-                                                   int n_static, rh_ray *__restrict__ rays_out,
-                                                   rh_ray_aux *__restrict__ aux_out) {
-    const int n = count ? *count : n_static;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t j = perm[i];
-    rays_out[i] = rays_in[j];
-    aux_out[i] = aux_in[j];
 }
 
 } // namespace
@@ -976,9 +961,8 @@ int ensure_rays(rh_context *c, size_t n) {
         RH_CHECK(hipMalloc(&c->vals[k], m * sizeof(uint32_t)));
     }
     RH_CHECK(hipMalloc(&c->hits, m * sizeof(rh_hit)));
-    const size_t tiles = (m + kSortTile - 1) / kSortTile;
-    // per-tile digit counts (digit-major) followed by the 256 digit totals
-    RH_CHECK(hipMalloc(&c->hist, (256 * tiles + 256) * sizeof(uint32_t)));
+    // the ray sort's digit counts, tile tickets and per-tile status words
+    RH_CHECK(hipMalloc(&c->hist, sort_scratch_words(m) * sizeof(uint32_t)));
     c->max_rays = m;
     return ensure_nee(c);
 }
@@ -1269,33 +1253,40 @@ int launch_shade(rh_context *c, const ShadeArgs &a, int grid_rays) {
     return 0;
 }
 
-// Stable LSB radix sort of (key, val) pairs in keys[0]/vals[0] by the top
-// `bits` bits of the key (8, 16, 24 or 32: one 8-bit pass each), on `stream`;
-// hist holds 256 * ceil(max_n / kSortTile) + 256 words.  Returns the buffer
-// index (0 or 1) that holds the sorted pairs.
-int radix_passes(hipStream_t stream, uint32_t *const keys[2], uint32_t *const vals[2], uint32_t *hist,
-                 const int *count, int max_n, int bits, int *result) {
-    const int tiles = std::max(1, (max_n + kSortTile - 1) / kSortTile);
+// Sorts the queue rays_in/aux_in (`count` rays on the device, at most max_n)
+// by the top `bits` bits (8, 16, 24 or 32) of their hash into
+// rays_out/aux_out, stable, on `stream`: k_sort_clear, k_sort_keys, then
+// one k_sort_scatter per 8-bit digit, the last moving the rays and writing
+// the count to count_out (when given).  keys/vals
+// are two ping-pong buffers of max_n words each; scr holds
+// sort_scratch_words(max_n) words.
+int sort_queue(hipStream_t stream, const rh_ray *rays_in, const rh_ray_aux *aux_in, rh_ray *rays_out,
+               rh_ray_aux *aux_out, const int *count, int *count_out, int max_n, int bits, const PassConst &pc,
+               uint32_t *const keys[2], uint32_t *const vals[2], uint32_t *scr, unsigned long long *stalls) {
+    if (bits != 8 && bits != 16 && bits != 24 && bits != 32) return arg_error("sort bits must be 8, 16, 24 or 32");
+    if ((long long)max_n > kSortMaxRays) return arg_error("ray queue too long for the sort (2^30 rays)");
+    const int tiles = (int)std::max<size_t>(1, sort_tiles((size_t)max_n));
+    const int npass = bits / 8, shift0 = 32 - bits;
+    const size_t clear_words = kSortStatus + (size_t)npass * tiles * 256;
+    const unsigned clear_blocks = (unsigned)std::min<size_t>(1024, (clear_words + 255) / 256);
+    hipLaunchKernelGGL(k_sort_clear, dim3(clear_blocks), dim3(256), 0, stream, scr, count, max_n, npass, tiles);
+    hipLaunchKernelGGL(k_sort_keys, dim3(tiles), dim3(kBlock), 0, stream, rays_in, count, max_n, pc, keys[0], vals[0],
+                       scr, shift0, npass);
     int src = 0;
-    for (int shift = 32 - bits; shift < 32; shift += 8) {
-        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kBlock), 0, stream, keys[src], count, max_n, shift, hist,
-                           tiles);
-        hipLaunchKernelGGL(k_radix_scan_rows, dim3(256), dim3(kBlock), 0, stream, hist, tiles,
-                           hist + (size_t)256 * tiles);
-        hipLaunchKernelGGL(k_radix_scan_bases, dim3(256), dim3(kBlock), 0, stream, hist, tiles,
-                           hist + (size_t)256 * tiles);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kBlock), 0, stream, keys[src], vals[src],
-                           keys[src ^ 1], vals[src ^ 1], count, max_n, shift, hist, tiles);
+    for (int p = 0; p < npass; p++) {
+        const int shift = shift0 + 8 * p;
+        if (p + 1 < npass)
+            hipLaunchKernelGGL(k_sort_scatter<false>, dim3(tiles), dim3(kBlock), 0, stream, keys[src], vals[src],
+                               keys[src ^ 1], vals[src ^ 1], count, max_n, shift, p, scr, tiles, nullptr, nullptr,
+                               nullptr, nullptr, nullptr, stalls);
+        else
+            hipLaunchKernelGGL(k_sort_scatter<true>, dim3(tiles), dim3(kBlock), 0, stream, keys[src], vals[src],
+                               nullptr, nullptr, count, max_n, shift, p, scr, tiles, rays_in, aux_in, rays_out,
+                               aux_out, count_out, stalls);
         src ^= 1;
     }
     RH_CHECK(hipGetLastError());
-    *result = src;
     return 0;
-}
-
-int radix_sort(rh_context *c, const int *count, int max_n, int bits, int *result) {
-    KernelMark km(c, 2);
-    return radix_passes(c->stream, c->keys, c->vals, c->hist, count, max_n, bits, result);
 }
 
 PassConst make_pass_const(rh_context *c, const rh_pass_desc *p) { return make_pass_const(c->w, c->h, p); }
@@ -1375,12 +1366,11 @@ struct PassStages {
 
 // ---- scratch of the allocation-free ray sort (rh_sort_rays_noalloc) ----
 // Keys and values twice (the ping-pong buffers of the radix passes), then the
-// per-tile digit counts and the 256 digit totals, for at most
-// RH_NOALLOC_MAX_RAYS rays: 34 MB on every GPU the library runs on, held until
+// sort's digit counts, tile tickets and per-tile status words, for at most
+// RH_NOALLOC_MAX_RAYS rays: 42 MB on every GPU the library runs on, held until
 // the process ends.  Host code never uses the array's name as an address: the
 // device address of each GPU's copy comes from hipGetSymbolAddress (scratch_ptr).
-constexpr size_t kNoAllocTiles = ((size_t)RH_NOALLOC_MAX_RAYS + kSortTile - 1) / kSortTile;
-constexpr size_t kNoAllocWords = 4 * (size_t)RH_NOALLOC_MAX_RAYS + 256 * kNoAllocTiles + 256;
+constexpr size_t kNoAllocWords = 4 * (size_t)RH_NOALLOC_MAX_RAYS + sort_scratch_words(RH_NOALLOC_MAX_RAYS);
 static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
 
 namespace {
@@ -1783,16 +1773,13 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool k
             break;
         }
         if (sort_on) {
-            hipLaunchKernelGGL(k_ray_keys, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->rays[q],
-                               cnt[q], n, pc, c->keys[0], c->vals[0]);
-            int sorted = 0;
-            rc = radix_sort(c, cnt[q], n, c->sort_bits, &sorted);
+            {
+                KernelMark km(c, 2);
+                // the sorted rays go to the other buffer, their count with them
+                rc = sort_queue(c->stream, c->rays[q], c->aux[q], c->rays[q ^ 1], c->aux[q ^ 1], cnt[q], cnt[q ^ 1],
+                                n, c->sort_bits, pc, c->keys, c->vals, c->hist, c->stats_dev + 7);
+            }
             if (rc) return rc;
-            hipLaunchKernelGGL(k_gather, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->rays[q],
-                               c->aux[q], c->vals[sorted], cnt[q], n, c->rays[q ^ 1], c->aux[q ^ 1]);
-            RH_CHECK(hipGetLastError());
-            // the sorted rays now sit in the other buffer; move the count along
-            hipLaunchKernelGGL(k_copy_int, dim3(1), dim3(1), 0, c->stream, cnt[q ^ 1], cnt[q]);
             q ^= 1;
             tm.mark(kSecondarySort);
         }
@@ -2069,12 +2056,13 @@ int rh_get_option(rh_context *c, const char *name, int *value) {
     else if (n == "defer_nee") *value = c->defer_nee;
     else if (n == "pass_overlap") *value = c->pass_overlap;
     else if (n == "bound_slot") *value = c->bound; // read-only: the slot of the last pass
-    else if (n == "stack_overflows") {
+    else if (n == "stack_overflows" || n == "sort_stalls") {
+        // read-only: dropped traversal stack entries; sort look-backs that gave up
         c->dirty = true; // work other than an overlapped pass (begin_overlapped_pass)
         unsigned long long v[8];
         RH_CHECK(hipMemcpyAsync(v, c->stats_dev, sizeof(v), hipMemcpyDeviceToHost, c->stream));
         RH_CHECK(hipStreamSynchronize(c->stream));
-        *value = (int)v[4];
+        *value = (int)v[n == "sort_stalls" ? 7 : 4];
     } else return arg_error("unknown option");
     return 0;
 }
@@ -2265,14 +2253,10 @@ int rh_sort_rays(rh_context *c, rh_ray_packet *rays, size_t n, const float root_
     memset(&pc, 0, sizeof(pc));
     memcpy(pc.root_min, root_min, sizeof(pc.root_min));
     memcpy(pc.cell_size, cell_size, sizeof(pc.cell_size));
-    hipLaunchKernelGGL(k_ray_keys, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, c->rays[0],
-                       nullptr, (int)n, pc, c->keys[0], c->vals[0]);
-    int sorted = 0;
-    rc = radix_sort(c, nullptr, (int)n, 32, &sorted); // the full key, as SortRays_CPU
+    // the full key, as SortRays_CPU
+    rc = sort_queue(c->stream, c->rays[0], c->aux[0], c->rays[1], c->aux[1], nullptr, nullptr, (int)n, 32, pc, c->keys,
+                    c->vals, c->hist, c->stats_dev + 7);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_gather, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, c->rays[0],
-                       c->aux[0], c->vals[sorted], nullptr, (int)n, c->rays[1], c->aux[1]);
-    RH_CHECK(hipGetLastError());
     return download_packets(c, rays, n, 1);
 }
 
@@ -2295,15 +2279,9 @@ int rh_sort_rays_noalloc(const rh_ray *d_rays, const rh_ray_aux *d_aux, size_t n
     memcpy(pc.root_min, root_min, sizeof(pc.root_min));
     memcpy(pc.cell_size, cell_size, sizeof(pc.cell_size));
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const dim3 grid((unsigned)((n + kBlock - 1) / kBlock)), blk(kBlock);
-    hipLaunchKernelGGL(k_ray_keys, grid, blk, 0, s, d_rays, nullptr, (int)n, pc, keys[0], vals[0]);
-    RH_CHECK(hipGetLastError());
-    int sorted = 0;
-    const int rc = radix_passes(s, keys, vals, hist, nullptr, (int)n, 32, &sorted); // the full key, as SortRays_CPU
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_gather, grid, blk, 0, s, d_rays, d_aux, vals[sorted], nullptr, (int)n, d_rays_out, d_aux_out);
-    RH_CHECK(hipGetLastError());
-    return 0;
+    // the full key, as SortRays_CPU; a look-back that gives up is not reported here (no device counter)
+    return sort_queue(s, d_rays, d_aux, d_rays_out, d_aux_out, nullptr, nullptr, (int)n, 32, pc, keys, vals, hist,
+                      nullptr);
 }
 
 int rh_shade_rays(rh_context *c, const rh_pass_info *pi, const rh_ray_packet *rays, const rh_hit *hits, size_t n,

@@ -139,4 +139,240 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t *__rest
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// One-sweep radix sort of the ray queue by its hash (get_ray_hash,
+// CoreRef.cpp:316-333; the result is the stable sort by key that
+// SortRays_CPU, CoreRef.cpp:615-657, produces).  Per sort: one clear, one
+// kernel that writes the keys and counts the digits of every pass over the
+// whole queue, and one scatter per 8-bit digit, the last of which moves the
+// rays themselves.  A scatter block takes a tile ticket in start order,
+// counts its tile's digits, publishes them, and sums the counts of the
+// tiles before it from their published words (decoupled look-back: a tile
+// whose prefix is known publishes it, so a walk stops there).  A block only
+// ever waits for tiles whose tickets were taken before its own, by blocks
+// already running, so every wait ends; the walk is also bounded
+// (kLookbackSpins polls of one word) and counts a give-up in `stalls`.
+//
+// Scratch layout (32-bit words), sort_scratch_words(max_n) in all:
+//   [kSortTotals, +4 x 256)  digit counts of the whole queue per digit position
+//   [kSortTickets, +4)        tile tickets of the scatter passes
+//   [kSortStatus, ...)        per pass, per tile, per digit: the tile's count
+//                             (flag kTileAggregate) or the count of every
+//                             tile up to and including it (kTilePrefix)
+constexpr int kSweepItems = 4;                   // keys per thread of a one-sweep tile
+constexpr int kSweepTile = kBlock * kSweepItems; // keys per tile (4 waves x 256)
+constexpr uint32_t kSortTotals = 0, kSortTickets = 1024, kSortStatus = 1280;
+constexpr uint32_t kTileAggregate = 1u << 30, kTilePrefix = 2u << 30, kTileValue = kTileAggregate - 1u;
+constexpr int kLookbackSpins = 1 << 20;
+// largest queue the 30-bit counts of the status words hold
+constexpr long long kSortMaxRays = (1ll << 30) - 1;
+
+__host__ __device__ constexpr size_t sort_tiles(size_t max_n) { return (max_n + kSweepTile - 1) / kSweepTile; }
+__host__ __device__ constexpr size_t sort_scratch_words(size_t max_n) {
+    return kSortStatus + 4 * 256 * (sort_tiles(max_n) > 0 ? sort_tiles(max_n) : 1);
+}
+
+// Zeroes the totals, the tickets and the status rows of this sort's passes
+// for the tiles of the device count.
+__global__ void __launch_bounds__(256) k_sort_clear(uint32_t *__restrict__ scr, const int *__restrict__ count,
+                                                    int n_static, int npass, int tiles_static) {
+    const int n = count ? *count : n_static;
+    const size_t tiles = sort_tiles((size_t)(n > 0 ? n : 0));
+    const size_t plane = tiles * 256, total = kSortStatus + (size_t)npass * plane;
+    for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (size_t)gridDim.x * blockDim.x) {
+        if (w < kSortStatus) {
+            scr[w] = 0u;
+        } else {
+            const size_t r = w - kSortStatus, pass = r / plane, o = r % plane;
+            scr[kSortStatus + pass * (size_t)tiles_static * 256 + o] = 0u;
+        }
+    }
+}
+
+// Keys (the ray hash) and the identity permutation of the queue, and the
+// digit counts of every pass (shift0 + 8 p, p < npass) over the whole queue.
+__global__ void __launch_bounds__(kBlock) k_sort_keys(const rh_ray *__restrict__ rays, const int *__restrict__ count,
+                                                      int n_static, PassConst pc, uint32_t *__restrict__ keys,
+                                                      uint32_t *__restrict__ vals, uint32_t *__restrict__ scr,
+                                                      int shift0, int npass) {
+    __shared__ uint32_t h[4][256];
+    const int n = count ? *count : n_static;
+    const int base = blockIdx.x * kSweepTile;
+    if (base >= n) return;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int p = 0; p < 4; p++) h[p][t] = 0u;
+    float4 a[kSweepItems], b[kSweepItems];
+#pragma unroll
+    for (int r = 0; r < kSweepItems; r++) { // the loads first, all in flight together
+        const int i = base + r * kBlock + t;
+        if (i < n) {
+            const float4 *q = reinterpret_cast<const float4 *>(rays + i);
+            a[r] = q[0];
+            b[r] = q[1];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSweepItems; r++) {
+        const int i = base + r * kBlock + t;
+        if (i < n) {
+            const uint32_t k = ray_hash(f3{a[r].x, a[r].y, a[r].z}, f3{b[r].x, b[r].y, b[r].z}, pc.root_min, pc.cell_size);
+            keys[i] = k;
+            vals[i] = (uint32_t)i;
+            for (int p = 0; p < npass; p++) atomicAdd(&h[p][(k >> (shift0 + 8 * p)) & 0xffu], 1u);
+        }
+    }
+    __syncthreads();
+    for (int p = 0; p < npass; p++) {
+        const uint32_t v = h[p][t];
+        if (v) atomicAdd(scr + kSortTotals + 256 * p + t, v);
+    }
+}
+
+// One stable scatter pass by the digit at `shift` (pass index `pass`).
+// Gather: the last pass, which writes the rays (rays_in[value]) at the
+// sorted positions instead of the keys and values.  Wave w of a block owns
+// the w-th quarter of the tile, 16 rounds of 64 keys in index order; it ranks
+// each key among its wave's earlier keys of the same digit (peers found with
+// eight ballots, one per digit bit; its own running digit counts in LDS), so
+// no block-wide barrier is needed per round.  The waves' counts then give
+// the tile's digit counts and each wave's offset within a digit.
+template <bool Gather>
+__global__ void __launch_bounds__(kBlock) k_sort_scatter(const uint32_t *__restrict__ keys_in,
+                                                         const uint32_t *__restrict__ vals_in,
+                                                         uint32_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out,
+                                                         const int *__restrict__ count, int n_static, int shift, int pass,
+                                                         uint32_t *__restrict__ scr, int tiles_static,
+                                                         const rh_ray *__restrict__ rays_in,
+                                                         const rh_ray_aux *__restrict__ aux_in,
+                                                         rh_ray *__restrict__ rays_out, rh_ray_aux *__restrict__ aux_out,
+                                                         int *__restrict__ count_out,
+                                                         unsigned long long *__restrict__ stalls) {
+    constexpr int kWaves = kBlock / 64, kWaveKeys = kSweepTile / kWaves;
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t run[256];       // the digit's first position in the output, for this tile
+    __shared__ uint32_t wc[kWaves][256]; // per-wave digit counts, then each wave's offset within the digit
+    __shared__ uint32_t wsum[kWaves];
+    const int n = count ? *count : n_static;
+    const int tiles = (int)sort_tiles((size_t)(n > 0 ? n : 0));
+    if (Gather && count_out && blockIdx.x == 0 && threadIdx.x == 0) *count_out = n; // the sorted queue's count
+    if ((int)blockIdx.x >= tiles) return; // exactly `tiles` blocks take tickets
+    const int t = threadIdx.x, wave = t >> 6;
+    const uint32_t lane = lane_id();
+    if (t == 0) s_tile = atomicAdd(scr + kSortTickets + pass, 1u);
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) wc[w][t] = 0u;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const int first = (int)tile * kSweepTile + wave * kWaveKeys; // this wave's first key
+
+    uint32_t k[kSweepItems], v[kSweepItems], local[kSweepItems];
+#pragma unroll
+    for (int r = 0; r < kSweepItems; r++) {
+        const int i = first + r * 64 + (int)lane;
+        k[r] = 0u;
+        v[r] = 0u;
+        if (i < n) {
+            k[r] = keys_in[i];
+            v[r] = vals_in[i];
+        }
+    }
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int r = 0; r < kSweepItems; r++) {
+        const bool active = first + r * 64 + (int)lane < n;
+        const uint32_t digit = (k[r] >> shift) & 0xffu;
+        uint64_t peers = __ballot(active);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const uint64_t m = __ballot((digit >> b) & 1u);
+            peers &= ((digit >> b) & 1u) ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        const uint32_t before = active ? wc[wave][digit] : 0u;
+        local[r] = before + rank;
+        // the wave's lanes read the count before its first lane of the digit adds to it
+        if (active && rank == 0) wc[wave][digit] = before + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    // digit t: the tile's count, and each wave's offset among the tile's keys of the digit
+    uint32_t mine = 0u;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) {
+        const uint32_t c = wc[w][t];
+        wc[w][t] = mine;
+        mine += c;
+    }
+
+    // publish, then sum the tiles before this one (thread t: digit t)
+    uint32_t *status = scr + kSortStatus + (size_t)pass * tiles_static * 256;
+    __hip_atomic_store(status + (size_t)tile * 256 + t, (tile == 0 ? kTilePrefix : kTileAggregate) | mine,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t excl = 0u;
+    if (tile > 0) {
+        int j = (int)tile - 1;
+        int spins = 0;
+        for (;;) {
+            const uint32_t w = __hip_atomic_load(status + (size_t)j * 256 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t flag = w & ~kTileValue;
+            if (flag == 0u) {
+                if (++spins > kLookbackSpins) {
+                    if (stalls) atomicAdd(stalls, 1ull);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += w & kTileValue;
+            if (flag == kTilePrefix) break;
+            --j; // tile 0 publishes a prefix, so j stays >= 0
+        }
+        __hip_atomic_store(status + (size_t)tile * 256 + t, kTilePrefix | (excl + mine), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+
+    // the digit's first position: counts of the smaller digits over the whole
+    // queue, plus this digit's count in the earlier tiles
+    const uint32_t tot = scr[kSortTotals + 256 * pass + t];
+    const uint32_t inc = wave_inclusive_scan(tot);
+    if ((t & 63) == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t below = 0u;
+    for (int w = 0; w < wave; w++) below += wsum[w];
+    run[t] = below + inc - tot + excl;
+    __syncthreads();
+
+    if (Gather) {
+        // every ray of the thread loaded before any is stored
+        rh_ray rr[kSweepItems];
+        rh_ray_aux ra[kSweepItems];
+#pragma unroll
+        for (int r = 0; r < kSweepItems; r++) {
+            if (first + r * 64 + (int)lane < n) {
+                rr[r] = rays_in[v[r]];
+                ra[r] = aux_in[v[r]];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kSweepItems; r++) {
+            if (first + r * 64 + (int)lane >= n) continue;
+            const uint32_t digit = (k[r] >> shift) & 0xffu;
+            const uint32_t pos = run[digit] + wc[wave][digit] + local[r];
+            rays_out[pos] = rr[r];
+            aux_out[pos] = ra[r];
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < kSweepItems; r++) {
+            if (first + r * 64 + (int)lane >= n) continue;
+            const uint32_t digit = (k[r] >> shift) & 0xffu;
+            const uint32_t pos = run[digit] + wc[wave][digit] + local[r];
+            keys_out[pos] = k[r];
+            vals_out[pos] = v[r];
+        }
+    }
+}
+
 } // namespace

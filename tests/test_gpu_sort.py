@@ -26,7 +26,8 @@ def _rays(n, seed, lo=-6.0, hi=12.0, degenerate=False):
     return r
 
 
-@pytest.mark.parametrize("n,seed,deg", [(1, 0, False), (1000, 1, False), (70001, 2, True), (300000, 3, False)])
+@pytest.mark.parametrize("n,seed,deg", [(1, 0, False), (1000, 1, False), (4096, 5, False), (70001, 2, True),
+                                        (300000, 3, False), (1048577, 4, True)])
 def test_sort_matches_reference(ref_lib, host_lib, dev_lib, n, seed, deg):
     rs, hs = standalone_scene(ref_lib), standalone_scene(host_lib)
     scenes.cornell_box(rs)
@@ -50,6 +51,7 @@ def test_sort_matches_reference(ref_lib, host_lib, dev_lib, n, seed, deg):
     assert (np.diff(ks.astype(np.int64)) >= 0).all()
     eq = np.diff(ks.astype(np.int64)) == 0
     assert (np.diff(s_gpu["xy"])[eq] > 0).all()
+    assert ctx.get_option("sort_stalls") == 0  # no tile look-back gave up
 
 
 @pytest.mark.parametrize("lds_depth", [24, 8])
@@ -62,6 +64,7 @@ def test_sort_is_invisible_in_the_image(host_lib, lds_depth):
     imgs = []
     for opts in ({"sort_rays": 0, "path_tail": -1}, {"sort_rays": 1, "path_tail": -1},
                  {"sort_rays": 1, "sort_bits": 8}, {"sort_rays": 1, "sort_bits": 16},
+                 {"sort_rays": 1, "sort_bits": 24, "path_tail": -1}, {"sort_rays": 1, "sort_bits": 32, "path_tail": 2},
                  {"sort_rays": 0, "defer_nee": 0, "path_tail": -1}, {"path_tail": 0}, {"path_tail": 1, "sort_rays": 1},
                  {"path_tail": 3, "defer_nee": 0}, {"path_tail": 0, "tail_occupancy": 3},
                  {"path_tail": 2, "tail_occupancy": 3, "defer_nee": 0}, {"refill": 1},
@@ -79,6 +82,7 @@ def test_sort_is_invisible_in_the_image(host_lib, lds_depth):
         for _ in range(4):
             r.render(s, reg)
         assert no_overflow(r)
+        assert r.get_option("sort_stalls") == 0
         imgs.append(r.pixels())
     for im in imgs[1:]:
         assert im.tobytes() == imgs[0].tobytes()
