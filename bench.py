@@ -663,7 +663,9 @@ def run_large(args, lib, A, scenes, knobs, wide):
             b.add(sc, w, h, n)
         # batch knobs: the 64-scene workload's, or every bounce staged (deep paths)
         bsweep = []
-        for cfg in (dict(args.large_batch_knobs or {}), dict(args.large_batch_knobs or {}, path_tail=-1)):
+        base_b = dict(args.large_batch_knobs or {})
+        for cfg in (base_b, dict(base_b, path_tail=-1), dict(base_b, path_tail=-1, blocks_per_cu=8),
+                    dict(base_b, path_tail=8, blocks_per_cu=8), dict(base_b, path_tail=-1, lds_stack_depth=8)):
             for k, v in cfg.items():
                 b.set_option(k, v)
             b.restart()

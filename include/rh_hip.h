@@ -182,14 +182,20 @@ int rh_read_accum(rh_context *ctx, float *dst, size_t count);
  * A pass's launches are captured once as a hipGraph and replayed ("use_graph",
  * default 1).  Knobs: "lds_stack_depth", "path_tail", "max_bounces" (as
  * rh_set_option; max_bounces applies to scenes added after it is set),
- * "use_graph", "blocks_per_cu" (persistent grid size).  Stats: rays counted
- * as by rh_get_ray_counts (primary, secondary, shadow). */
+ * "use_graph", "blocks_per_cu" (persistent grid size), "sort_bits" (staged
+ * bounces sorted by their ray hash), "queue_rays" (rays in flight at most),
+ * "pass_overlap" (default 1: two passes in flight on two streams, each
+ * mixing after the last).  Changing queue_rays or pass_overlap lays the batch
+ * out again and restarts it at the next pass.  rh_batch_passes_done gives the
+ * passes rendered since the last restart (0 once such a change is pending).
+ * Stats: rays counted as by rh_get_ray_counts (primary, secondary, shadow). */
 typedef struct rh_batch rh_batch;
 int rh_batch_create(int device, rh_batch **out);
 void rh_batch_destroy(rh_batch *batch);
 int rh_batch_add_scene(rh_batch *batch, const rh_scene_view *scene, const rh_pass_desc *desc, int w, int h, int spp);
 int rh_batch_set_halton(rh_batch *batch, const float *seq, size_t count);
 int rh_batch_passes(const rh_batch *batch);
+int rh_batch_passes_done(const rh_batch *batch);
 int rh_batch_render_pass(rh_batch *batch);
 int rh_batch_synchronize(rh_batch *batch);
 /* back to pass 0 with cleared images (the scenes stay uploaded) */

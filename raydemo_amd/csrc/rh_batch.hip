@@ -150,8 +150,10 @@ __device__ __forceinline__ void for_each_tile(const BatchDev &b, const int *coun
 // First kernel of a pass: advances the pass counter, sets every scene's
 // iteration and mixing weight, clears the queue and tile counters, and
 // activates the primary queue of scenes still rendering.
-__global__ void k_b_begin_pass(BatchDev b, BatchEntry *table, int n_stages) {
-    const int pass = *b.pass + 1; // every thread reads the old value
+// stride: passes between two passes of this slot's pass counter (2 with two
+// passes in flight, each slot taking every other pass).
+__global__ void k_b_begin_pass(BatchDev b, BatchEntry *table, int n_stages, int stride) {
+    const int pass = *b.pass + stride; // every thread reads the old value
     __syncthreads();
     const int S = b.n_scenes;
     const int nc = (RH_MAX_BOUNCES_EXT + 2) * S;
@@ -487,10 +489,49 @@ __global__ void k_b_fill(float4 *buf, size_t n, float4 v) {
 
 } // namespace
 
+// The working set of one batch pass in flight: its stream, its copy of the
+// scene table (the pass writes iteration and mixing weight into it), its
+// pass counter, queue and tile counters, the queues, the pass framebuffer,
+// the NEE buffers, spill slices, sort scratch and captured graph.  With two
+// passes in flight (option pass_overlap) rh_batch's fields of these names are
+// the bound slot's and BatchSlot holds the other's (swap_slot).
+struct BatchSlot {
+    hipStream_t stream = nullptr;
+    BatchEntry *table = nullptr;
+    int *pass = nullptr;
+    int *counts = nullptr, *shadow_counts = nullptr, *work = nullptr;
+    rh_ray *rays[2] = {nullptr, nullptr};
+    rh_ray_aux *aux[2] = {nullptr, nullptr};
+    rh_hit *hits = nullptr;
+    float4 *temp = nullptr;
+    float4 *nee_ray = nullptr, *nee_term = nullptr;
+    int2 *nee_aux = nullptr;
+    uint8_t *nee_vis = nullptr;
+    float4 *pend_c = nullptr, *pend_alb = nullptr;
+    int *pend_count = nullptr;
+    uint32_t *shadow_queue = nullptr;
+    uint32_t *spill = nullptr;
+    uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
+    uint32_t *hist = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_key[5] = {-1, -1, -1, -1, -1};
+};
+#define RH_BATCH_SLOT_FIELDS(X)                                                                                        \
+    X(stream) X(table) X(pass) X(counts) X(shadow_counts) X(work) X(rays) X(aux) X(hits) X(temp) X(nee_ray)            \
+        X(nee_term) X(nee_aux) X(nee_vis) X(pend_c) X(pend_alb) X(pend_count) X(shadow_queue) X(spill) X(keys)        \
+            X(vals) X(hist) X(graph) X(graph_exec) X(graph_key)
+
 struct rh_batch {
     int device = 0;
     hipStream_t stream = nullptr;
     int cus = 0;
+    // passes in flight: the other slot, which slot is bound, the mix order
+    BatchSlot alt;
+    int bound = 0;
+    int pass_overlap = 1;
+    hipEvent_t ev_mix = nullptr; // the last pass's mix (passes mix in pass order)
+    bool mix_recorded = false;
 
     std::vector<std::vector<DevBuf>> scene_bufs;
     std::vector<BatchEntry> entries; // host copy of the table
@@ -523,6 +564,7 @@ struct rh_batch {
     uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
     uint32_t *hist = nullptr;
     int passes_done = 0;
+    int mix_work = 0; // tile counter of the mix (the last of a pass; the same for both slots' graphs)
 
     // options
     int lds_depth = 16;
@@ -535,10 +577,25 @@ struct rh_batch {
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    int graph_key[4] = {-1, -1, -1, -1};
+    int graph_key[5] = {-1, -1, -1, -1, -1};
 };
 
 namespace {
+
+// Exchanges the bound slot's working set with the other one's.
+void swap_slot(rh_batch *b) {
+#define RH_SWAP_FIELD(f) std::swap(b->f, b->alt.f);
+    RH_BATCH_SLOT_FIELDS(RH_SWAP_FIELD)
+#undef RH_SWAP_FIELD
+    b->bound ^= 1;
+}
+
+// Waits for both slots' streams.
+int sync_all(rh_batch *b) {
+    RH_CHECK(hipStreamSynchronize(b->stream));
+    if (b->alt.stream) RH_CHECK(hipStreamSynchronize(b->alt.stream));
+    return 0;
+}
 
 void drop_graph(rh_batch *b) {
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
@@ -547,11 +604,13 @@ void drop_graph(rh_batch *b) {
     b->graph = nullptr;
 }
 
-void free_arrays(rh_batch *b) {
-    void *ps[] = {b->table,   b->pass,    b->counts,  b->shadow_counts, b->work,    b->rays[0],   b->rays[1],
-                  b->aux[0],  b->aux[1],  b->hits,    b->temp,          b->clean,   b->final_,    b->nee_ray,
-                  b->nee_term, b->nee_aux, b->nee_vis, b->pend_c,       b->pend_alb, b->pend_count, b->shadow_queue,
-                  b->spill,   b->keys[0], b->keys[1], b->vals[0],       b->vals[1], b->hist};
+// The bound slot's arrays and graph.
+void free_slot_arrays(rh_batch *b) {
+    drop_graph(b);
+    void *ps[] = {b->table,    b->pass,    b->counts,  b->shadow_counts, b->work,       b->rays[0],   b->rays[1],
+                  b->aux[0],   b->aux[1],  b->hits,    b->temp,          b->nee_ray,    b->nee_term,  b->nee_aux,
+                  b->nee_vis,  b->pend_c,  b->pend_alb, b->pend_count,   b->shadow_queue, b->spill,   b->keys[0],
+                  b->keys[1],  b->vals[0], b->vals[1], b->hist};
     for (void *p : ps) free_ptr(p);
     b->keys[0] = b->keys[1] = b->vals[0] = b->vals[1] = b->hist = nullptr;
     b->table = nullptr;
@@ -559,7 +618,7 @@ void free_arrays(rh_batch *b) {
     b->rays[0] = b->rays[1] = nullptr;
     b->aux[0] = b->aux[1] = nullptr;
     b->hits = nullptr;
-    b->temp = b->clean = b->final_ = nullptr;
+    b->temp = nullptr;
     b->nee_ray = b->nee_term = nullptr;
     b->nee_aux = nullptr;
     b->nee_vis = nullptr;
@@ -567,7 +626,23 @@ void free_arrays(rh_batch *b) {
     b->pend_count = nullptr;
     b->shadow_queue = nullptr;
     b->spill = nullptr;
+}
+
+// Both slots' arrays and the shared framebuffers.
+void free_arrays(rh_batch *b) {
+    free_slot_arrays(b);
+    swap_slot(b);
+    free_slot_arrays(b);
+    swap_slot(b);
+    free_ptr(b->clean);
+    free_ptr(b->final_);
+    b->clean = b->final_ = nullptr;
     b->built = false;
+}
+
+// Binds slot `slot` (0 or 1).
+void bind_slot(rh_batch *b, int slot) {
+    if (b->bound != slot) swap_slot(b);
 }
 
 template <typename T> int dev_alloc(T **p, size_t n) {
@@ -647,14 +722,9 @@ void group_scenes(rh_batch *b) {
     b->groups.push_back({first, (int)b->entries.size()});
 }
 
-// Allocates the shared queues for the scenes added so far and uploads the table.
-int build(rh_batch *b) {
-    if (b->built) return 0;
-    if (b->entries.empty()) return arg_error("batch has no scenes");
-    free_arrays(b);
-    drop_graph(b);
-    group_scenes(b);
-    // framebuffers for every pixel of the batch; queues for the largest group
+// The bound slot's arrays: its table copy, counters, queues for the largest
+// group and the pass framebuffer for every pixel of the batch.
+int build_slot(rh_batch *b) {
     const size_t P = b->total_pixels, QP = b->queue_pixels, NS = b->queue_slots, S = b->entries.size();
     int rc = 0;
     rc |= dev_alloc(&b->table, S);
@@ -668,8 +738,6 @@ int build(rh_batch *b) {
     }
     rc |= dev_alloc(&b->hits, QP);
     rc |= dev_alloc(&b->temp, P);
-    rc |= dev_alloc(&b->clean, P);
-    rc |= dev_alloc(&b->final_, P);
     rc |= dev_alloc(&b->nee_ray, NS);
     rc |= dev_alloc(&b->nee_term, NS);
     rc |= dev_alloc(&b->nee_aux, NS);
@@ -678,20 +746,63 @@ int build(rh_batch *b) {
     rc |= dev_alloc(&b->pend_c, QP);
     rc |= dev_alloc(&b->pend_alb, QP);
     rc |= dev_alloc(&b->pend_count, QP);
-    b->spill_cap = std::max(QP, NS);
     // entries, and their distances when a scene has 8-wide trees
     rc |= dev_alloc(&b->spill, (size_t)kSpillDepth * (any_wide(b) ? 2 : 1) * b->spill_cap);
+    return rc;
+}
+
+// Every scene back to pass 0: each slot's table copy and pass counter (slot 0
+// takes passes 1, 3, 5, ... and slot 1 passes 2, 4, ... with two passes in
+// flight; slot 0 every pass otherwise), the accumulated images cleared.
+int reset_passes(rh_batch *b) {
+    int rc = sync_all(b);
     if (rc) return rc;
-    RH_CHECK(hipMemcpyAsync(b->table, b->entries.data(), S * sizeof(BatchEntry), hipMemcpyHostToDevice, b->stream));
-    RH_CHECK(hipMemsetAsync(b->pass, 0, sizeof(int), b->stream));
-    const size_t n4 = P;
+    const int slots = b->pass_overlap ? 2 : 1;
+    int init[2] = {b->pass_overlap ? -1 : 0, 0};
+    bind_slot(b, 0);
+    for (int k = 0; k < slots; k++) {
+        bind_slot(b, k);
+        RH_CHECK(hipMemcpyAsync(b->table, b->entries.data(), b->entries.size() * sizeof(BatchEntry),
+                                hipMemcpyHostToDevice, b->stream));
+        RH_CHECK(hipMemcpyAsync(b->pass, &init[k], sizeof(int), hipMemcpyHostToDevice, b->stream));
+    }
+    bind_slot(b, 0);
+    const size_t n4 = b->total_pixels;
     hipLaunchKernelGGL(k_b_fill, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, b->stream, b->clean, n4,
                        make_float4(0, 0, 0, 0));
     hipLaunchKernelGGL(k_b_fill, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, b->stream, b->final_, n4,
                        make_float4(0, 0, 0, 0));
     RH_CHECK(hipGetLastError());
-    RH_CHECK(hipStreamSynchronize(b->stream));
+    rc = sync_all(b); // init[] is read by the copies above
+    if (rc) return rc;
     b->passes_done = 0;
+    b->mix_recorded = false;
+    return 0;
+}
+
+// Lays out the queues for the scenes added so far and allocates the shared
+// framebuffers and each slot's arrays.
+int build(rh_batch *b) {
+    if (b->built) return 0;
+    if (b->entries.empty()) return arg_error("batch has no scenes");
+    int rc = sync_all(b);
+    if (rc) return rc;
+    free_arrays(b);
+    group_scenes(b);
+    const size_t P = b->total_pixels;
+    b->spill_cap = std::max(b->queue_pixels, b->queue_slots);
+    rc |= dev_alloc(&b->clean, P);
+    rc |= dev_alloc(&b->final_, P);
+    bind_slot(b, 0);
+    rc |= build_slot(b);
+    if (b->pass_overlap) {
+        swap_slot(b);
+        rc |= build_slot(b);
+        swap_slot(b);
+    }
+    if (rc) return rc;
+    rc = reset_passes(b);
+    if (rc) return rc;
     b->built = true;
     return 0;
 }
@@ -759,8 +870,9 @@ int ensure_sort_buffers(rh_batch *b) {
 }
 
 // The launch sequence of one pass (fixed for given options): begin, generate,
-// then per stage trace / shade / shadow / combine, the path tail, mix.
-int enqueue_pass(rh_batch *b) {
+// then per stage trace / shade / shadow / combine, the path tail.  The mix
+// follows outside (rh_batch_render_pass) at tile counter *mix_work.
+int enqueue_pass(rh_batch *b, int stride, int *mix_work) {
     // first stage run by the tail kernel (bounce path_tail)
     const int tail = b->path_tail < 0 ? b->max_depth + 1 : std::min(b->path_tail + 1, b->max_depth + 1);
     const int sort_launches = b->sort_bits ? 2 + 2 * (b->sort_bits / 8) : 0;
@@ -771,7 +883,7 @@ int enqueue_pass(rh_batch *b) {
     const size_t lds = (size_t)b->lds_depth * kBlock * sizeof(uint32_t) * (any_wide(b) ? 2 : 1);
     const dim3 grid((unsigned)(b->cus * b->blocks_per_cu)), blk(kBlock);
     int work = 0;
-    hipLaunchKernelGGL(k_b_begin_pass, dim3(1), dim3(1024), 0, b->stream, all, b->table, (int)work_counters(b));
+    hipLaunchKernelGGL(k_b_begin_pass, dim3(1), dim3(1024), 0, b->stream, all, b->table, (int)work_counters(b), stride);
     const int tree = batch_tree(b);
     // the groups one after another: each fills the queues with its scenes' rays
     for (const auto &g : b->groups) {
@@ -783,7 +895,7 @@ int enqueue_pass(rh_batch *b) {
         else if (tree == kTreeWide) launch_stages<kTreeWide>(b, d, lds, grid, blk, tail, work);
         else launch_stages<kTreeAny>(b, d, lds, grid, blk, tail, work);
     }
-    hipLaunchKernelGGL(k_b_mix, grid, blk, 0, b->stream, all, work++);
+    *mix_work = work++;
     RH_CHECK(hipGetLastError());
     return 0;
 }
@@ -804,11 +916,15 @@ int rh_batch_create(int device, rh_batch **out) {
     RH_CHECK(hipSetDevice(device));
     auto *b = new rh_batch;
     b->device = device;
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    // one stream per pass slot (both at the default priority)
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->alt.stream, hipStreamNonBlocking) != hipSuccess) {
+        if (b->stream) (void)hipStreamDestroy(b->stream);
         delete b;
         rh_rt::err() = "hipStreamCreate failed";
         return -1003;
     }
+    RH_CHECK(hipEventCreateWithFlags(&b->ev_mix, hipEventDisableTiming));
     hipDeviceProp_t prop;
     RH_CHECK(hipGetDeviceProperties(&prop, device));
     b->cus = prop.multiProcessorCount;
@@ -824,14 +940,15 @@ int rh_batch_create(int device, rh_batch **out) {
 void rh_batch_destroy(rh_batch *b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
-    (void)hipStreamSynchronize(b->stream);
-    drop_graph(b);
+    (void)sync_all(b);
     free_arrays(b);
     for (auto &v : b->scene_bufs)
         for (auto &x : v) free_ptr(x.p);
     free_ptr(b->halton);
     free_ptr(b->stats);
+    if (b->ev_mix) (void)hipEventDestroy(b->ev_mix);
     (void)hipStreamDestroy(b->stream);
+    if (b->alt.stream) (void)hipStreamDestroy(b->alt.stream);
     delete b;
 }
 
@@ -843,6 +960,8 @@ int rh_batch_add_scene(rh_batch *b, const rh_scene_view *v, const rh_pass_desc *
         return arg_error("batched scenes render their whole frame");
     if (b->total_pixels + (size_t)w * h > 0x7fffffffu) return arg_error("batch exceeds 2^31 pixels");
     RH_CHECK(hipSetDevice(b->device));
+    int rc0 = sync_all(b);
+    if (rc0) return rc0;
     std::vector<DevBuf> bufs;
     BatchEntry e;
     memset(&e, 0, sizeof(e));
@@ -878,6 +997,8 @@ int rh_batch_add_scene(rh_batch *b, const rh_scene_view *v, const rh_pass_desc *
 
 int rh_batch_set_halton(rh_batch *b, const float *seq, size_t count) {
     if (count < (size_t)RH_HALTON_COUNT * RH_HALTON_SEQ_LEN) return arg_error("Halton table needs 256 x 24 floats");
+    const int rc = sync_all(b); // passes in flight read the table
+    if (rc) return rc;
     RH_CHECK(upload_halton(b->halton, seq, b->stream));
     RH_CHECK(hipStreamSynchronize(b->stream));
     return 0;
@@ -885,19 +1006,25 @@ int rh_batch_set_halton(rh_batch *b, const float *seq, size_t count) {
 
 int rh_batch_passes(const rh_batch *b) { return b->max_spp; }
 
+int rh_batch_passes_done(const rh_batch *b) { return b->built ? b->passes_done : 0; }
+
 int rh_batch_render_pass(rh_batch *b) {
     RH_CHECK(hipSetDevice(b->device));
     int rc = build(b);
     if (rc) return rc;
     if (b->passes_done >= b->max_spp) return arg_error("every scene of the batch has rendered its passes");
+    // with two passes in flight, pass p (from 1) on slot (p - 1) & 1; its
+    // stages run while the other slot's pass finishes
+    bind_slot(b, b->pass_overlap ? (b->passes_done & 1) : 0);
     rc = ensure_sort_buffers(b);
     if (rc) return rc;
-    const int key[4] = {b->lds_depth, b->path_tail, b->blocks_per_cu, b->sort_bits};
+    const int stride = b->pass_overlap ? 2 : 1;
+    const int key[5] = {b->lds_depth, b->path_tail, b->blocks_per_cu, b->sort_bits, stride};
     if (b->use_graph) {
         if (!b->graph_exec || memcmp(key, b->graph_key, sizeof(key)) != 0) {
             drop_graph(b);
             RH_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            rc = enqueue_pass(b);
+            rc = enqueue_pass(b, stride, &b->mix_work);
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(b->stream, &g);
             if (rc) {
@@ -911,9 +1038,16 @@ int rh_batch_render_pass(rh_batch *b) {
         }
         RH_CHECK(hipGraphLaunch(b->graph_exec, b->stream));
     } else {
-        rc = enqueue_pass(b);
+        rc = enqueue_pass(b, stride, &b->mix_work);
         if (rc) return rc;
     }
+    // passes mix into the accumulated images in pass order
+    if (b->mix_recorded) RH_CHECK(hipStreamWaitEvent(b->stream, b->ev_mix, 0));
+    hipLaunchKernelGGL(k_b_mix, dim3((unsigned)(b->cus * b->blocks_per_cu)), dim3(kBlock), 0, b->stream, dev_view(b),
+                       b->mix_work);
+    RH_CHECK(hipGetLastError());
+    RH_CHECK(hipEventRecord(b->ev_mix, b->stream));
+    b->mix_recorded = true;
     b->passes_done++;
     return 0;
 }
@@ -921,31 +1055,18 @@ int rh_batch_render_pass(rh_batch *b) {
 int rh_batch_restart(rh_batch *b) {
     RH_CHECK(hipSetDevice(b->device));
     if (!b->built) return 0;
-    RH_CHECK(hipStreamSynchronize(b->stream));
-    RH_CHECK(hipMemcpyAsync(b->table, b->entries.data(), b->entries.size() * sizeof(BatchEntry), hipMemcpyHostToDevice,
-                            b->stream));
-    RH_CHECK(hipMemsetAsync(b->pass, 0, sizeof(int), b->stream));
-    const size_t n4 = b->total_pixels;
-    hipLaunchKernelGGL(k_b_fill, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, b->stream, b->clean, n4,
-                       make_float4(0, 0, 0, 0));
-    hipLaunchKernelGGL(k_b_fill, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, b->stream, b->final_, n4,
-                       make_float4(0, 0, 0, 0));
-    RH_CHECK(hipGetLastError());
-    RH_CHECK(hipStreamSynchronize(b->stream));
-    b->passes_done = 0;
-    return 0;
+    return reset_passes(b);
 }
 
-int rh_batch_synchronize(rh_batch *b) {
-    RH_CHECK(hipStreamSynchronize(b->stream));
-    return 0;
-}
+int rh_batch_synchronize(rh_batch *b) { return sync_all(b); }
 
 int rh_batch_read_pixels(rh_batch *b, int scene, float *rgba, size_t count) {
     if (scene < 0 || scene >= (int)b->entries.size()) return arg_error("scene index out of range");
     if (!b->built) return arg_error("batch has not rendered");
     const BatchEntry &e = b->entries[scene];
     if (count < (size_t)e.pixels * 4) return arg_error("output buffer too small");
+    const int rc = sync_all(b);
+    if (rc) return rc;
     RH_CHECK(hipMemcpyAsync(rgba, b->final_ + e.offset, (size_t)e.pixels * sizeof(float4), hipMemcpyDefault,
                             b->stream));
     RH_CHECK(hipStreamSynchronize(b->stream));
@@ -957,7 +1078,9 @@ int rh_batch_read_queue(rh_batch *b, int scene, int stage, int buffer, rh_ray *o
     if (stage < 0 || stage > RH_MAX_BOUNCES_EXT + 1 || buffer < 0 || buffer > 1)
         return arg_error("stage or buffer out of range");
     if (!b->built) return arg_error("batch has not rendered");
-    RH_CHECK(hipStreamSynchronize(b->stream));
+    // the queues of the last pass (the bound slot's)
+    const int rc = sync_all(b);
+    if (rc) return rc;
     int n = 0;
     RH_CHECK(hipMemcpy(&n, b->counts + (size_t)stage * b->entries.size() + scene, sizeof(int), hipMemcpyDeviceToHost));
     *count = n;
@@ -967,12 +1090,16 @@ int rh_batch_read_queue(rh_batch *b, int scene, int stage, int buffer, rh_ray *o
 }
 
 int rh_batch_reset_stats(rh_batch *b) {
+    const int rc = sync_all(b);
+    if (rc) return rc;
     RH_CHECK(hipMemsetAsync(b->stats, 0, 8 * sizeof(unsigned long long), b->stream));
     RH_CHECK(hipStreamSynchronize(b->stream));
     return 0;
 }
 
 int rh_batch_get_ray_counts(rh_batch *b, rh_ray_counts *out) {
+    const int rc = sync_all(b);
+    if (rc) return rc;
     unsigned long long v[8];
     RH_CHECK(hipMemcpyAsync(v, b->stats, sizeof(v), hipMemcpyDeviceToHost, b->stream));
     RH_CHECK(hipStreamSynchronize(b->stream));
@@ -1005,12 +1132,18 @@ int rh_batch_set_option(rh_batch *b, const char *name, int value) {
         if (value != b->queue_rays) {
             b->queue_rays = value;
             b->built = false; // queues and groups are laid out again at the next pass (which restarts the batch)
-            drop_graph(b);
         }
     } else if (n == "sort_bits") {
         if (value != 0 && value != 8 && value != 16 && value != 24 && value != 32)
             return arg_error("sort_bits must be 0, 8, 16, 24 or 32");
         b->sort_bits = value;
+    } else if (n == "pass_overlap") {
+        if ((value != 0) != (b->pass_overlap != 0)) {
+            const int rc = sync_all(b);
+            if (rc) return rc;
+            b->pass_overlap = value != 0;
+            b->built = false; // the slots are laid out again at the next pass (which restarts the batch)
+        }
     } else {
         return arg_error("unknown option");
     }

@@ -32,7 +32,7 @@ int BatchRenderer::AddScene(const std::shared_ptr<SceneBase> &_s, int w, int h, 
         last_error = "scene has no camera";
         return -1;
     }
-    if (pass_ > 0) {
+    if (passes_done() > 0) {
         last_error = "scenes are added before the first pass";
         return -1;
     }
@@ -58,11 +58,13 @@ int BatchRenderer::AddScene(const std::shared_ptr<SceneBase> &_s, int w, int h, 
 }
 
 bool BatchRenderer::RenderPass() {
-    if (pass_ >= passes()) {
+    // the device's count: a knob that lays the batch out again restarts it
+    const int done = passes_done();
+    if (done >= passes()) {
         last_error = "every scene has rendered its passes";
         return false;
     }
-    const int it = pass_ + 1;
+    const int it = done + 1;
     if (it == 1 || it % RH_HALTON_SEQ_LEN == 0) {
         RayHip::UpdateHaltonSequence(permutations_, it, halton_.data());
         if (rh_batch_set_halton(b_, halton_.data(), halton_.size()) != 0) {
@@ -74,7 +76,6 @@ bool BatchRenderer::RenderPass() {
         last_error = rh_last_error();
         return false;
     }
-    pass_ = it;
     return true;
 }
 
@@ -83,7 +84,6 @@ bool BatchRenderer::Restart() {
         last_error = rh_last_error();
         return false;
     }
-    pass_ = 0;
     return true;
 }
 

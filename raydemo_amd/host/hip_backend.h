@@ -219,7 +219,6 @@ class BatchRenderer {
     std::vector<float> halton_;
     std::vector<std::pair<int, int>> sizes_;
     std::vector<std::vector<pixel_color_t>> pixels_;
-    int pass_ = 0;
 
 public:
     explicit BatchRenderer(int device_index);
@@ -231,7 +230,7 @@ public:
     // passes; returns its index, or -1 (see last_error).
     int AddScene(const std::shared_ptr<SceneBase> &s, int w, int h, int spp);
     int passes() const { return rh_batch_passes(b_); }
-    int passes_done() const { return pass_; }
+    int passes_done() const { return rh_batch_passes_done(b_); }
     int scenes() const { return (int)sizes_.size(); }
     // One pass over the batch; false on error or when every scene is done.
     bool RenderPass();

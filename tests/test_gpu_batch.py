@@ -39,7 +39,8 @@ def _single(lib, build, w, h, spp):
                                   {"path_tail": 1, "blocks_per_cu": 1}, {"sort_bits": 32, "path_tail": -1},
                                   {"sort_bits": 24, "path_tail": 2, "blocks_per_cu": 1}, {"queue_rays": 5000},
                                   {"queue_rays": 1, "sort_bits": 16, "path_tail": -1},
-                                  {"queue_rays": 3000, "use_graph": 0, "path_tail": 1}])
+                                  {"queue_rays": 3000, "use_graph": 0, "path_tail": 1}, {"pass_overlap": 0},
+                                  {"pass_overlap": 0, "use_graph": 0, "sort_bits": 8}])
 def test_batch_matches_single_renders(host_lib, golden_scene1, opts):
     cases = _cases(golden_scene1)
     singles = [_single(host_lib, b, w, h, spp) for _, b, w, h, spp in cases]
@@ -110,3 +111,30 @@ def test_batch_queues_sorted_per_scene(host_lib, oracle, bits):
         keys = oracle.ray_hashes(p, rmin, cell).astype(np.int64) >> (32 - bits)
         assert (np.diff(keys) >= 0).all(), i
         assert len(np.unique(keys)) > 1
+
+
+def test_batch_restart_and_overlap_toggle(host_lib, golden_scene1):
+    """Two passes in flight (the default): a restart after some passes, and
+    switching to one pass in flight (the batch is laid out again and
+    restarted), each render the single renders' images bit for bit."""
+    cases = _cases(golden_scene1)[:3]
+    singles = [_single(host_lib, b, w, h, spp) for _, b, w, h, spp in cases]
+    batch = A.Batch(host_lib)
+    keep = []
+    for _, build, w, h, spp in cases:
+        s = standalone_scene(host_lib)
+        build(s)
+        keep.append(s)
+        batch.add(s, w, h, spp)
+    for overlap in (1, 0, 1):
+        batch.set_option("pass_overlap", overlap)
+        batch.render_pass()
+        batch.render_pass()
+        batch.restart()
+        batch.reset_stats()
+        batch.render_all()
+        for i, (name, _, w, h, spp) in enumerate(cases):
+            assert batch.pixels(i).tobytes() == singles[i][0].tobytes(), (overlap, name)
+        rc = batch.ray_counts()
+        for key in ("primary", "secondary", "shadow"):
+            assert rc[key] == sum(s[1][key] for s in singles), (overlap, key)
