@@ -10,9 +10,9 @@ Mrays/s.  Scene build and upload happen before the timed region.
 The launch knobs (LDS share of the traversal stack, ray sort on/off and the
 number of key bits it orders by) are chosen first by raydemo_amd.autotune on
 a few passes per candidate; every candidate renders the identical image.  The
-headline value is the tuned setting (by default two passes in flight: the
-next pass's first stages run on a second stream while the last pass's path
-tail drains); the same knobs with one pass in flight (serial), and the staged
+headline value is the tuned setting (by default four passes in flight, each
+on its own stream: a pass's first stages run while the earlier passes' late
+bounces drain); the same knobs with one pass in flight (serial), and the staged
 sort-off (coherent order) and full 32-bit sort modes are timed the same way
 and reported beside it.
 
@@ -136,9 +136,9 @@ def run_gpu(args, lib, A, scenes, wide, staged_modes=True, tuned=None):
              # the tuned knobs with one pass in flight (each pass waits for the last one's path tail)
              "serial": dict(tuned.best, pass_overlap=0),
              "sort_off": {"lds_stack_depth": depth, "sort_rays": 0, "sort_bits": 32, "path_tail": -1,
-                          "pass_overlap": 1},
+                          "pass_overlap": tuned.best.get("pass_overlap", 3)},
              "sort_on": {"lds_stack_depth": depth, "sort_rays": 1, "sort_bits": 32, "path_tail": -1,
-                         "pass_overlap": 1}}
+                         "pass_overlap": tuned.best.get("pass_overlap", 3)}}
     if args.only_tuned or not staged_modes:
         modes = {"tuned": modes["tuned"], "serial": modes["serial"]}
     if not tuned.best.get("pass_overlap", 1):

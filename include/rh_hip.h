@@ -99,10 +99,17 @@ int rh_get_ray_counts(rh_context *ctx, rh_ray_counts *out);
  * "adjoint_tape" (0..4096: passes 1..N after a clear keep their vertex records on the device, every bounce
  *   staged, and rh_adjoint_pass of such a pass needs no replay; 0, the default: off),
  * "adjoint_tape_texels" (0/1: the tape records texel footprints too, for texel gradients),
- * "defer_nee" (0/1: shadow rays of next-event estimation as their own stage). */
+ * "defer_nee" (0/1: shadow rays of next-event estimation as their own stage),
+ * "pass_overlap" (0..3, default 3: passes in flight besides the newest; plain rh_render_pass passes
+ *   (no tape, geometry camera or SH output) rotate over pass_overlap + 1 slots, each with its own stream
+ *   and working set, and mix into the image after the pass before them, so the image is the serial one;
+ *   any other entry point's work is waited for by the next pass). */
 int rh_set_option(rh_context *ctx, const char *name, int value);
-/* Any option above, and two read-only counts: "scene_uploads" (rh_upload_scene
- * calls) and "instance_updates" (rh_update_instances calls). */
+/* Any option above, and read-only values: "scene_uploads" (rh_upload_scene
+ * calls), "instance_updates" (rh_update_instances calls), "bound_slot" (the
+ * pass slot of the last pass), "stack_overflows" (traversal stack entries
+ * dropped) and "sort_stalls" (ray-sort tile look-backs that gave up waiting;
+ * 0 unless something is wrong). */
 int rh_get_option(rh_context *ctx, const char *name, int *value);
 
 /* Per-kernel profile collected while "kernel_timing" is on (HIP events around
@@ -184,8 +191,9 @@ int rh_read_accum(rh_context *ctx, float *dst, size_t count);
  * rh_set_option; max_bounces applies to scenes added after it is set),
  * "use_graph", "blocks_per_cu" (persistent grid size), "sort_bits" (staged
  * bounces sorted by their ray hash), "queue_rays" (rays in flight at most),
- * "pass_overlap" (default 1: two passes in flight on two streams, each
- * mixing after the last).  Changing queue_rays or pass_overlap lays the batch
+ * "pass_overlap" (passes in flight besides the newest, 0..3, default 3: each
+ * on its own stream, mixing after the one before).  Changing queue_rays or
+ * pass_overlap lays the batch
  * out again and restarts it at the next pass.  rh_batch_passes_done gives the
  * passes rendered since the last restart (0 once such a change is pending).
  * Stats: rays counted as by rh_get_ray_counts (primary, secondary, shadow). */

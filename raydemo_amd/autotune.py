@@ -8,8 +8,8 @@ bounce from which the remaining bounces run as one path-per-lane launch
 instead of one trace/shade/shadow stage each, the register budget (waves per
 SIMD) of the path-tail and 8-wide trace kernels, the lane-refill trace, and
 the path tail as a persistent kernel whose lanes take a new path when theirs
-ends, and whether two passes are in flight at once (the next pass's first
-stages on a second stream while the last pass's path tail drains).
+ends, and how many passes are in flight at once (each on its own stream:
+a pass's first stages run while the earlier passes' late bounces drain).
 The image is bit-identical for every setting (tests/test_gpu_sort.py), so the
 tuner may pick whichever is fastest on the scene at hand.
 
@@ -36,7 +36,7 @@ TAIL_OCCUPANCY = (2, 3)
 REFILL = (0, 1)
 TRACE_OCCUPANCY = (0, 5, 6)
 SHADOW_OCCUPANCY = (-1, 0, 5)
-PASS_OVERLAP = (1, 0)
+PASS_OVERLAP = (3, 2, 1, 0)  # passes in flight besides the newest
 KNOBS = ("lds_stack_depth", "sort_rays", "sort_bits", "path_tail", "tail_occupancy", "refill", "trace_occupancy",
          "shadow_occupancy", "tail_refill", "pass_overlap")
 
@@ -56,7 +56,7 @@ def candidates(depths=LDS_DEPTHS, sort_modes=SORT_MODES, tails=PATH_TAILS, occup
         if pt < 0 and occ != occupancies[0]:
             continue  # no tail launch: occupancy is moot
         yield {"lds_stack_depth": d, "sort_rays": so, "sort_bits": sb, "path_tail": pt, "tail_occupancy": occ,
-               "refill": 0, "trace_occupancy": 0, "shadow_occupancy": -1, "tail_refill": 0, "pass_overlap": 1}
+               "refill": 0, "trace_occupancy": 0, "shadow_occupancy": -1, "tail_refill": 0, "pass_overlap": PASS_OVERLAP[0]}
 
 
 def apply(renderer, cfg: dict):
@@ -127,7 +127,7 @@ def tune(renderer, scene, region_factory, passes: int = 6, depths=LDS_DEPTHS, so
         cfg = {k: second[k] for k in KNOBS}
         cfg["shadow_occupancy"] = so
         table.append(measure(renderer, scene, region_factory, cfg, passes))
-    # one pass in flight instead of two, on the best so far
+    # fewer passes in flight, on the best so far
     third = max(table, key=lambda r: r["mrays"])
     for ov in PASS_OVERLAP[1:]:
         cfg = {k: third[k] for k in KNOBS}

@@ -492,9 +492,11 @@ __global__ void k_b_fill(float4 *buf, size_t n, float4 v) {
 // The working set of one batch pass in flight: its stream, its copy of the
 // scene table (the pass writes iteration and mixing weight into it), its
 // pass counter, queue and tile counters, the queues, the pass framebuffer,
-// the NEE buffers, spill slices, sort scratch and captured graph.  With two
-// passes in flight (option pass_overlap) rh_batch's fields of these names are
-// the bound slot's and BatchSlot holds the other's (swap_slot).
+// the NEE buffers, spill slices, sort scratch and captured graph.  With
+// several passes in flight (option pass_overlap: passes in flight besides the
+// newest) rh_batch's fields of these names are the bound slot's and store[]
+// holds the other slots' (bind_slot).
+constexpr int kMaxBatchSlots = 4;
 struct BatchSlot {
     hipStream_t stream = nullptr;
     BatchEntry *table = nullptr;
@@ -526,10 +528,11 @@ struct rh_batch {
     int device = 0;
     hipStream_t stream = nullptr;
     int cus = 0;
-    // passes in flight: the other slot, which slot is bound, the mix order
-    BatchSlot alt;
+    // passes in flight: the other slots (store[bound] unused: the bound slot
+    // lives in the fields above and below), which slot is bound, the mix order
+    BatchSlot store[kMaxBatchSlots];
     int bound = 0;
-    int pass_overlap = 1;
+    int pass_overlap = kMaxBatchSlots - 1;
     hipEvent_t ev_mix = nullptr; // the last pass's mix (passes mix in pass order)
     bool mix_recorded = false;
 
@@ -582,18 +585,23 @@ struct rh_batch {
 
 namespace {
 
-// Exchanges the bound slot's working set with the other one's.
-void swap_slot(rh_batch *b) {
-#define RH_SWAP_FIELD(f) std::swap(b->f, b->alt.f);
-    RH_BATCH_SLOT_FIELDS(RH_SWAP_FIELD)
-#undef RH_SWAP_FIELD
-    b->bound ^= 1;
+// Makes slot j the bound one: its working set into the batch's fields, the
+// bound one's back to the store.
+void bind_slot(rh_batch *b, int j) {
+    if (j == b->bound) return;
+#define RH_BIND_FIELD(f)                                                                                                \
+    std::swap(b->f, b->store[j].f);                                                                                    \
+    std::swap(b->store[j].f, b->store[b->bound].f);
+    RH_BATCH_SLOT_FIELDS(RH_BIND_FIELD)
+#undef RH_BIND_FIELD
+    b->bound = j;
 }
 
-// Waits for both slots' streams.
+// Waits for every slot's stream.
 int sync_all(rh_batch *b) {
     RH_CHECK(hipStreamSynchronize(b->stream));
-    if (b->alt.stream) RH_CHECK(hipStreamSynchronize(b->alt.stream));
+    for (int j = 0; j < kMaxBatchSlots; j++)
+        if (j != b->bound && b->store[j].stream) RH_CHECK(hipStreamSynchronize(b->store[j].stream));
     return 0;
 }
 
@@ -628,22 +636,21 @@ void free_slot_arrays(rh_batch *b) {
     b->spill = nullptr;
 }
 
-// Both slots' arrays and the shared framebuffers.
+// Every slot's arrays and the shared framebuffers.
 void free_arrays(rh_batch *b) {
-    free_slot_arrays(b);
-    swap_slot(b);
-    free_slot_arrays(b);
-    swap_slot(b);
+    const int keep = b->bound;
+    for (int j = 0; j < kMaxBatchSlots; j++) {
+        bind_slot(b, j);
+        free_slot_arrays(b);
+    }
+    bind_slot(b, keep);
     free_ptr(b->clean);
     free_ptr(b->final_);
     b->clean = b->final_ = nullptr;
     b->built = false;
 }
 
-// Binds slot `slot` (0 or 1).
-void bind_slot(rh_batch *b, int slot) {
-    if (b->bound != slot) swap_slot(b);
-}
+
 
 template <typename T> int dev_alloc(T **p, size_t n) {
     RH_CHECK(hipMalloc(p, std::max<size_t>(n, 1) * sizeof(T)));
@@ -751,15 +758,15 @@ int build_slot(rh_batch *b) {
     return rc;
 }
 
-// Every scene back to pass 0: each slot's table copy and pass counter (slot 0
-// takes passes 1, 3, 5, ... and slot 1 passes 2, 4, ... with two passes in
-// flight; slot 0 every pass otherwise), the accumulated images cleared.
+// Every scene back to pass 0: each slot's table copy and pass counter (with
+// S slots, slot k takes passes k + 1, k + 1 + S, ...: its counter starts at
+// k + 1 - S and steps by S), the accumulated images cleared.
 int reset_passes(rh_batch *b) {
     int rc = sync_all(b);
     if (rc) return rc;
-    const int slots = b->pass_overlap ? 2 : 1;
-    int init[2] = {b->pass_overlap ? -1 : 0, 0};
-    bind_slot(b, 0);
+    const int slots = b->pass_overlap + 1;
+    int init[kMaxBatchSlots];
+    for (int k = 0; k < slots; k++) init[k] = k + 1 - slots;
     for (int k = 0; k < slots; k++) {
         bind_slot(b, k);
         RH_CHECK(hipMemcpyAsync(b->table, b->entries.data(), b->entries.size() * sizeof(BatchEntry),
@@ -793,13 +800,11 @@ int build(rh_batch *b) {
     b->spill_cap = std::max(b->queue_pixels, b->queue_slots);
     rc |= dev_alloc(&b->clean, P);
     rc |= dev_alloc(&b->final_, P);
-    bind_slot(b, 0);
-    rc |= build_slot(b);
-    if (b->pass_overlap) {
-        swap_slot(b);
+    for (int k = 0; k <= b->pass_overlap; k++) {
+        bind_slot(b, k);
         rc |= build_slot(b);
-        swap_slot(b);
     }
+    bind_slot(b, 0);
     if (rc) return rc;
     rc = reset_passes(b);
     if (rc) return rc;
@@ -916,10 +921,14 @@ int rh_batch_create(int device, rh_batch **out) {
     RH_CHECK(hipSetDevice(device));
     auto *b = new rh_batch;
     b->device = device;
-    // one stream per pass slot (both at the default priority)
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&b->alt.stream, hipStreamNonBlocking) != hipSuccess) {
+    // one stream per pass slot (all at the default priority)
+    bool streams_ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
+    for (int j = 1; j < kMaxBatchSlots && streams_ok; j++)
+        streams_ok = hipStreamCreateWithFlags(&b->store[j].stream, hipStreamNonBlocking) == hipSuccess;
+    if (!streams_ok) {
         if (b->stream) (void)hipStreamDestroy(b->stream);
+        for (int j = 1; j < kMaxBatchSlots; j++)
+            if (b->store[j].stream) (void)hipStreamDestroy(b->store[j].stream);
         delete b;
         rh_rt::err() = "hipStreamCreate failed";
         return -1003;
@@ -948,7 +957,8 @@ void rh_batch_destroy(rh_batch *b) {
     free_ptr(b->stats);
     if (b->ev_mix) (void)hipEventDestroy(b->ev_mix);
     (void)hipStreamDestroy(b->stream);
-    if (b->alt.stream) (void)hipStreamDestroy(b->alt.stream);
+    for (int j = 0; j < kMaxBatchSlots; j++)
+        if (j != b->bound && b->store[j].stream) (void)hipStreamDestroy(b->store[j].stream);
     delete b;
 }
 
@@ -1013,12 +1023,12 @@ int rh_batch_render_pass(rh_batch *b) {
     int rc = build(b);
     if (rc) return rc;
     if (b->passes_done >= b->max_spp) return arg_error("every scene of the batch has rendered its passes");
-    // with two passes in flight, pass p (from 1) on slot (p - 1) & 1; its
-    // stages run while the other slot's pass finishes
-    bind_slot(b, b->pass_overlap ? (b->passes_done & 1) : 0);
+    // with S slots, pass p (from 1) on slot (p - 1) % S; its stages run while
+    // the other slots' passes finish
+    const int stride = b->pass_overlap + 1;
+    bind_slot(b, b->passes_done % stride);
     rc = ensure_sort_buffers(b);
     if (rc) return rc;
-    const int stride = b->pass_overlap ? 2 : 1;
     const int key[5] = {b->lds_depth, b->path_tail, b->blocks_per_cu, b->sort_bits, stride};
     if (b->use_graph) {
         if (!b->graph_exec || memcmp(key, b->graph_key, sizeof(key)) != 0) {
@@ -1138,10 +1148,11 @@ int rh_batch_set_option(rh_batch *b, const char *name, int value) {
             return arg_error("sort_bits must be 0, 8, 16, 24 or 32");
         b->sort_bits = value;
     } else if (n == "pass_overlap") {
-        if ((value != 0) != (b->pass_overlap != 0)) {
+        if (value < 0 || value >= kMaxBatchSlots) return arg_error("pass_overlap must be 0..3");
+        if (value != b->pass_overlap) {
             const int rc = sync_all(b);
             if (rc) return rc;
-            b->pass_overlap = value != 0;
+            b->pass_overlap = value;
             b->built = false; // the slots are laid out again at the next pass (which restarts the batch)
         }
     } else {

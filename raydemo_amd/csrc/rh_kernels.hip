@@ -710,9 +710,11 @@ __global__ void __launch_bounds__(kBlock) k_ray_keys(const rh_ray *__restrict__ 
 
 // The working set of one pass in flight: its stream, the pass's framebuffer,
 // the ray queues, hits, sort scratch, traversal spill slices, the deferred
-// NEE buffers and the queue counters.  Two passes may be in flight (option
-// pass_overlap): the context's fields of these names are the bound slot's,
-// and PassSlot holds the other one's (swap_slot exchanges them).
+// NEE buffers and the queue counters.  Up to kMaxPassSlots passes may be in
+// flight (option pass_overlap: passes in flight besides the newest): the
+// context's fields of these names are the bound slot's, and store[] holds
+// the other slots' (bind_slot moves them).
+constexpr int kMaxPassSlots = 4;
 struct PassSlot {
     hipStream_t stream = nullptr;
     float4 *temp = nullptr;
@@ -744,11 +746,12 @@ struct rh_context {
     hipStream_t stream = nullptr;
     int w = 0, h = 0;
     size_t max_rays = 0;
-    // passes in flight: the other slot, which slot is bound, and the order
-    // between them (see begin_overlapped_pass)
-    PassSlot alt;
+    // passes in flight: the other slots (store[bound] is not used: the bound
+    // slot lives in the fields above and below), which slot is bound, and the
+    // order between them (see begin_overlapped_pass)
+    PassSlot store[kMaxPassSlots];
     int bound = 0;
-    int pass_overlap = 1; // option: plain passes alternate between the two slots
+    int pass_overlap = kMaxPassSlots - 1; // option: plain passes rotate over pass_overlap + 1 slots
     bool dirty = true;    // device work other than an overlapped pass was queued since the last pass
     hipEvent_t ev_mix = nullptr;  // the last pass's mix (passes mix into clean in pass order)
     bool mix_recorded = false;
@@ -861,18 +864,23 @@ struct rh_context {
 
 namespace {
 
-// Exchanges the bound slot's working set with the other one's.
-void swap_slot(rh_context *c) {
-#define RH_SWAP_FIELD(f) std::swap(c->f, c->alt.f);
-    RH_SLOT_FIELDS(RH_SWAP_FIELD)
-#undef RH_SWAP_FIELD
-    c->bound ^= 1;
+// Makes slot j the bound one: its working set into the context's fields,
+// the bound one's back to the store.
+void bind_slot(rh_context *c, int j) {
+    if (j == c->bound) return;
+#define RH_BIND_FIELD(f)                                                                                                \
+    std::swap(c->f, c->store[j].f);                                                                                    \
+    std::swap(c->store[j].f, c->store[c->bound].f);
+    RH_SLOT_FIELDS(RH_BIND_FIELD)
+#undef RH_BIND_FIELD
+    c->bound = j;
 }
 
-// Waits for both slots' streams.
+// Waits for every slot's stream.
 int sync_slots(rh_context *c) {
     RH_CHECK(hipStreamSynchronize(c->stream));
-    if (c->alt.stream) RH_CHECK(hipStreamSynchronize(c->alt.stream));
+    for (int j = 0; j < kMaxPassSlots; j++)
+        if (j != c->bound && c->store[j].stream) RH_CHECK(hipStreamSynchronize(c->store[j].stream));
     return 0;
 }
 
@@ -911,12 +919,14 @@ int alloc_frame(rh_context *c, int w, int h) {
     release(c->sh_clean);
     release(c->clean);
     release(c->final_);
-    // both slots' pass buffers are sized for the old frame: the bound slot's
-    // are allocated again below, the other's at its next pass
-    free_slot_buffers(c);
-    swap_slot(c);
-    free_slot_buffers(c);
-    swap_slot(c);
+    // every slot's pass buffers are sized for the old frame: the bound slot's
+    // are allocated again below, the others' at their next pass
+    const int keep = c->bound;
+    for (int j = 0; j < kMaxPassSlots; j++) {
+        bind_slot(c, j);
+        free_slot_buffers(c);
+    }
+    bind_slot(c, keep);
     c->w = w;
     c->h = h;
     RH_CHECK(hipMalloc(&c->clean, np * sizeof(float4)));
@@ -1413,10 +1423,14 @@ int rh_create(int device, rh_context **out) {
     RH_CHECK(hipSetDevice(device));
     auto *c = new rh_context;
     c->device = device;
-    // one stream per pass slot (both at the default priority)
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->alt.stream, hipStreamNonBlocking) != hipSuccess) {
+    // one stream per pass slot (all at the default priority)
+    bool streams_ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+    for (int j = 1; j < kMaxPassSlots && streams_ok; j++)
+        streams_ok = hipStreamCreateWithFlags(&c->store[j].stream, hipStreamNonBlocking) == hipSuccess;
+    if (!streams_ok) {
         if (c->stream) (void)hipStreamDestroy(c->stream);
+        for (int j = 1; j < kMaxPassSlots; j++)
+            if (c->store[j].stream) (void)hipStreamDestroy(c->store[j].stream);
         delete c;
         rh_rt::err() = "hipStreamCreate failed";
         return -1003;
@@ -1427,10 +1441,12 @@ int rh_create(int device, rh_context **out) {
     RH_CHECK(hipGetDeviceProperties(&prop, device));
     c->cus = prop.multiProcessorCount;
     RH_CHECK(hipMalloc(&c->counters, 16 * sizeof(int)));
-    RH_CHECK(hipMalloc(&c->alt.counters, 16 * sizeof(int)));
+    for (int j = 1; j < kMaxPassSlots; j++) {
+        RH_CHECK(hipMalloc(&c->store[j].counters, 16 * sizeof(int)));
+        RH_CHECK(hipMemsetAsync(c->store[j].counters, 0, 16 * sizeof(int), c->stream));
+    }
     RH_CHECK(hipMalloc(&c->stats_dev, 8 * sizeof(unsigned long long)));
     RH_CHECK(hipMemsetAsync(c->counters, 0, 16 * sizeof(int), c->stream));
-    RH_CHECK(hipMemsetAsync(c->alt.counters, 0, 16 * sizeof(int), c->stream));
     RH_CHECK(hipMemsetAsync(c->stats_dev, 0, 8 * sizeof(unsigned long long), c->stream));
     RH_CHECK(hipMalloc(&c->visits, kVisitWords * sizeof(unsigned long long)));
     RH_CHECK(hipMemsetAsync(c->visits, 0, kVisitWords * sizeof(unsigned long long), c->stream));
@@ -1457,11 +1473,11 @@ void rh_destroy(rh_context *c) {
     free_ptr(c->tape_nverts);
     free_ptr(c->clean);
     free_ptr(c->final_);
-    for (int k = 0; k < 2; k++) {
+    for (int j = 0; j < kMaxPassSlots; j++) {
+        bind_slot(c, j);
         free_slot_buffers(c);
         free_ptr(c->counters);
         c->counters = nullptr;
-        swap_slot(c);
     }
     free_ptr(c->stats_dev);
     free_ptr(c->halton);
@@ -1478,7 +1494,8 @@ void rh_destroy(rh_context *c) {
     if (c->ev_mix) (void)hipEventDestroy(c->ev_mix);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     (void)hipStreamDestroy(c->stream);
-    if (c->alt.stream) (void)hipStreamDestroy(c->alt.stream);
+    for (int j = 0; j < kMaxPassSlots; j++)
+        if (j != c->bound && c->store[j].stream) (void)hipStreamDestroy(c->store[j].stream);
     delete c;
 }
 
@@ -1831,7 +1848,7 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool k
 // for all of it.
 int begin_overlapped_pass(rh_context *c) {
     if (c->dirty) RH_CHECK(hipEventRecord(c->ev_join, c->stream));
-    swap_slot(c);
+    bind_slot(c, (c->bound + 1) % (c->pass_overlap + 1));
     if (c->dirty) {
         RH_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
         c->dirty = false;
@@ -2003,7 +2020,8 @@ int rh_set_option(rh_context *c, const char *name, int value) {
     } else if (n == "tiled_primary") {
         c->tiled_primary = value != 0;
     } else if (n == "pass_overlap") {
-        c->pass_overlap = value != 0;
+        if (value < 0 || value >= kMaxPassSlots) return arg_error("pass_overlap must be 0..3");
+        c->pass_overlap = value;
     } else if (n == "shadow_occupancy") {
         if (value != -1 && value != 0 && value != 5 && value != 6)
             return arg_error("shadow_occupancy must be -1, 0, 5 or 6");

@@ -114,9 +114,9 @@ def test_batch_queues_sorted_per_scene(host_lib, oracle, bits):
 
 
 def test_batch_restart_and_overlap_toggle(host_lib, golden_scene1):
-    """Two passes in flight (the default): a restart after some passes, and
-    switching to one pass in flight (the batch is laid out again and
-    restarted), each render the single renders' images bit for bit."""
+    """Four passes in flight (the default), and three, two and one (each
+    switch lays the batch out again and restarts it): after some passes, a
+    restart; each renders the single renders' images bit for bit."""
     cases = _cases(golden_scene1)[:3]
     singles = [_single(host_lib, b, w, h, spp) for _, b, w, h, spp in cases]
     batch = A.Batch(host_lib)
@@ -126,7 +126,7 @@ def test_batch_restart_and_overlap_toggle(host_lib, golden_scene1):
         build(s)
         keep.append(s)
         batch.add(s, w, h, spp)
-    for overlap in (1, 0, 1):
+    for overlap in (3, 1, 0, 2):
         batch.set_option("pass_overlap", overlap)
         batch.render_pass()
         batch.render_pass()

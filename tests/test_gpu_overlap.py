@@ -1,6 +1,7 @@
-"""Passes in flight on two streams (option pass_overlap, on by default): a
-plain RenderScene pass runs on the pass slot the last pass did not use, so
-that its first stages run while the last pass's path tail drains.  The
+"""Passes in flight on several streams (option pass_overlap: passes in flight
+besides the newest, 3 by default): plain RenderScene passes rotate over that
+many pass slots plus one, so that a pass's first stages run while the earlier
+passes' late bounces drain.  The
 accumulated image must be the one passes run one after another give, bit for
 bit, whatever entry points are called between the passes (clears, colour and
 texel uploads, reads, region passes, resizes, option changes)."""
@@ -45,12 +46,14 @@ def run(lib, overlap, wide, knobs, passes=8, between=None, build=soup, w=W, h=H)
 
 @pytest.mark.parametrize("wide", [False, True], ids=["binary", "wide"])
 @pytest.mark.parametrize("knobs", [TAIL, STAGED_SORT], ids=["tail", "staged_sort"])
-def test_overlapped_passes_equal_serial_passes(host_lib, wide, knobs):
+@pytest.mark.parametrize("overlap", [1, 2, 3])
+def test_overlapped_passes_equal_serial_passes(host_lib, wide, knobs, overlap):
     img0, rc0, slots0, _ = run(host_lib, 0, wide, knobs)
-    img1, rc1, slots1, _ = run(host_lib, 1, wide, knobs)
+    img1, rc1, slots1, _ = run(host_lib, overlap, wide, knobs)
     print(f"slots serial {slots0}, overlapped {slots1}; rays {rc1}")
     assert len(set(slots0)) == 1, "serial passes stay on one slot"
     assert all(a != b for a, b in zip(slots1, slots1[1:])), "overlapped passes alternate slots"
+    assert len(set(slots1)) == overlap + 1
     assert np.array_equal(img0.view(np.uint32), img1.view(np.uint32))
     assert rc0 == rc1
 
@@ -79,7 +82,7 @@ def test_calls_between_overlapped_passes(host_lib):
     img0, rc0, _, _ = run(host_lib, 0, True, TAIL, passes=10, between=between)
     serial_reads = list(reads)
     reads.clear()
-    img1, rc1, slots1, _ = run(host_lib, 1, True, TAIL, passes=10, between=between)
+    img1, rc1, slots1, _ = run(host_lib, 2, True, TAIL, passes=10, between=between)
     assert all(a != b for a, b in zip(slots1, slots1[1:]))
     for a, b in zip(serial_reads, reads):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
@@ -113,7 +116,7 @@ def test_region_passes_and_resize(host_lib):
         return out
 
     a0, b0 = go(0)
-    a1, b1 = go(1)
+    a1, b1 = go(3)
     assert np.array_equal(a0.view(np.uint32), a1.view(np.uint32))
     assert np.array_equal(b0.view(np.uint32), b1.view(np.uint32))
 
@@ -123,7 +126,7 @@ def test_overlapped_passes_match_oracle(host_lib, oracle):
     same scene, passes, seeds and 8-wide trees."""
     w, h, spp = 64, 48, 4
     build = lambda s: scenes.triangle_soup(s, width=w, height=h, spp=spp, depth=8, n_spheres=64, n_fine=4)
-    img, rc, slots, _ = run(host_lib, 1, True, TAIL, passes=spp, w=w, h=h, build=build)
+    img, rc, slots, _ = run(host_lib, 3, True, TAIL, passes=spp, w=w, h=h, build=build)
     assert all(a != b for a, b in zip(slots, slots[1:]))
     hs = standalone_scene(host_lib, wide=True)
     build(hs)

@@ -140,15 +140,16 @@ def test_ray_counts_match_oracle(oracle, host_lib, opts):
     assert rmse(img, ref_img) <= 1e-3
 
 
-@pytest.mark.parametrize("overlap", [0, 1])
+@pytest.mark.parametrize("overlap", [0, 3])
 def test_stage_stats_filled_by_default(host_lib, overlap):
     """GetStats (RendererBase::stats_t, RendererBase.h:93-102) reports the stage
     timers of every pass with no option set: the generation, trace and shade
     stages are non-zero and their sum stays within the wall time of the
     passes (the timers are HIP events at the stage boundaries; no pass waits
-    for them).  With two passes in flight (pass_overlap) the stages of two
-    passes may run at once, so their sum stays within twice the wall time
-    (as the Ref backend's stage sums over region threads exceed it)."""
+    for them).  With several passes in flight (pass_overlap: that many
+    besides the newest) the stages of that many passes may run at once, so
+    their sum stays within that many times the wall time (as the Ref
+    backend's stage sums over region threads exceed it)."""
     import time
     w, h, spp = 256, 256, 8
     r = A.Renderer(host_lib, w, h, A.RENDERER_HIP)
@@ -170,4 +171,4 @@ def test_stage_stats_filled_by_default(host_lib, overlap):
     for k in ("time_primary_ray_gen_us", "time_primary_trace_us", "time_primary_shade_us",
               "time_secondary_trace_us", "time_secondary_shade_us"):
         assert st[k] > 0, k
-    assert sum(st.values()) <= wall_us * (2 if overlap else 1)
+    assert sum(st.values()) <= wall_us * (overlap + 1)
