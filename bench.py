@@ -781,7 +781,9 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--cpu-spp", type=int, default=64, help="full-frame passes of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tune-passes", type=int, default=4, help="passes per autotuner candidate")
+    ap.add_argument("--tune-passes", type=int, default=24,
+                    help="passes per autotuner candidate (enough that the passes in flight fill and drain "
+                         "only at its ends)")
     ap.add_argument("--knobs", default="",
                     help="skip the autotuner: lds_stack_depth,sort_rays,sort_bits,path_tail"
                          "[,tail_occupancy[,refill[,trace_occupancy[,shadow_occupancy[,tail_refill"
@@ -942,8 +944,8 @@ def main():
         "rays_per_pass": int(best["rays"] / args.steps), "ray_counts": best["counts"],
         "overflow": best["counts"]["overflow"],
         "autotune": {"seconds": round(tuned.seconds, 2), "passes_per_candidate": args.tune_passes,
-                     "columns": list(autotune.KNOBS) + ["mrays"],
-                     "table": [[t[k] for k in autotune.KNOBS] + [t["mrays"]] for t in tuned.table]},
+                     "columns": list(autotune.KNOBS) + ["passes", "mrays"],
+                     "table": [[t[k] for k in autotune.KNOBS] + [t.get("passes"), t["mrays"]] for t in tuned.table]},
         "kernel_ms": {k: round(v, 3) for k, v in prof.items() if k.endswith("_ms")},
         "roofline": roofline,
         "cpu_baseline": cpu,

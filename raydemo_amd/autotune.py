@@ -64,8 +64,10 @@ def apply(renderer, cfg: dict):
         renderer.set_option(k, int(v))
 
 
-def measure(renderer, scene, region_factory, cfg: dict, passes: int, warmup: int = 1) -> dict:
-    """Mrays/s of `passes` progressive passes under cfg (after `warmup` passes)."""
+def measure(renderer, scene, region_factory, cfg: dict, passes: int, warmup: int = 2) -> dict:
+    """Mrays/s of `passes` progressive passes under cfg (after `warmup` passes).
+    With several passes in flight the first and last few of a timed run
+    overlap less, so a candidate needs enough passes to show its rate."""
     apply(renderer, cfg)
     renderer.clear()
     reg = region_factory()
@@ -80,7 +82,7 @@ def measure(renderer, scene, region_factory, cfg: dict, passes: int, warmup: int
     dt = time.perf_counter() - t0
     rc = renderer.ray_counts()
     rays = rc["primary"] + rc["secondary"] + rc["shadow"]
-    return dict(cfg, ms_per_pass=round(dt / passes * 1e3, 4), mrays=round(rays / dt / 1e6, 2))
+    return dict(cfg, passes=passes, ms_per_pass=round(dt / passes * 1e3, 4), mrays=round(rays / dt / 1e6, 2))
 
 
 def tune(renderer, scene, region_factory, passes: int = 6, depths=LDS_DEPTHS, sort_modes=SORT_MODES,
@@ -98,43 +100,35 @@ def tune(renderer, scene, region_factory, passes: int = 6, depths=LDS_DEPTHS, so
                 renderer.clear()
                 return TuneResult(best=hit["best"], table=hit.get("table", []), seconds=0.0)
     t0 = time.perf_counter()
+    mr = lambda r: r["mrays"]
+    knobs_of = lambda r: {k: r[k] for k in KNOBS}
     table = [measure(renderer, scene, region_factory, cfg, passes) for cfg in candidates(depths, sort_modes, tails)]
-    # the refill trace, the refilled path tail and the capped-register trace
-    # only as second stages, on the best settings so far (all leave the image
-    # unchanged)
-    first = max(table, key=lambda r: r["mrays"])
-    for pt in sorted({p for p in (first["path_tail"], 0, 1, 2) if p >= 0}):
-        for occ in TAIL_OCCUPANCY:
-            cfg = {k: first[k] for k in KNOBS}
-            cfg.update(path_tail=pt, tail_occupancy=occ, tail_refill=1)
-            table.append(measure(renderer, scene, region_factory, cfg, passes))
-    first = max(table, key=lambda r: r["mrays"])
-    for rf in REFILL[1:]:
-        cfg = {k: first[k] for k in KNOBS}
-        cfg["refill"] = rf
-        table.append(measure(renderer, scene, region_factory, cfg, passes))
-    for occ in TRACE_OCCUPANCY[1:]:
-        for d in sorted({first["lds_stack_depth"], LDS_DEPTHS[0]}):
-            cfg = {k: first[k] for k in KNOBS}
-            cfg["trace_occupancy"] = occ
-            cfg["lds_stack_depth"] = d
-            table.append(measure(renderer, scene, region_factory, cfg, passes))
-    # the shadow kernel's own register budget, on the best so far
-    second = max(table, key=lambda r: r["mrays"])
-    for so in SHADOW_OCCUPANCY[1:]:
-        if so == second["trace_occupancy"]:
-            continue
-        cfg = {k: second[k] for k in KNOBS}
-        cfg["shadow_occupancy"] = so
-        table.append(measure(renderer, scene, region_factory, cfg, passes))
-    # fewer passes in flight, on the best so far
-    third = max(table, key=lambda r: r["mrays"])
-    for ov in PASS_OVERLAP[1:]:
-        cfg = {k: third[k] for k in KNOBS}
-        cfg["pass_overlap"] = ov
-        table.append(measure(renderer, scene, region_factory, cfg, passes))
-    best_row = max(table, key=lambda r: r["mrays"])
-    best = {k: best_row[k] for k in KNOBS}
+    # The leaders are often within a few per cent of each other: the best five
+    # are timed again over four times the passes, and every later stage times
+    # its candidates that long and compares them only with the best of those.
+    long_run = 4 * passes
+
+    def trial(cfg):
+        row = measure(renderer, scene, region_factory, cfg, long_run)
+        table.append(row)
+        return row
+
+    best = max((trial(knobs_of(r)) for r in sorted(table, key=mr, reverse=True)[:5]), key=mr)
+    # the refilled path tail, the refill trace, the capped-register trace and
+    # shadow kernel, then how many passes are in flight, each on the best so
+    # far (all leave the image unchanged)
+    stages = [
+        lambda b: [dict(knobs_of(b), path_tail=pt, tail_occupancy=occ, tail_refill=1)
+                   for pt in sorted({p for p in (b["path_tail"], 0, 1, 2) if p >= 0}) for occ in TAIL_OCCUPANCY],
+        lambda b: [dict(knobs_of(b), refill=rf) for rf in REFILL[1:]],
+        lambda b: [dict(knobs_of(b), trace_occupancy=occ, lds_stack_depth=d)
+                   for occ in TRACE_OCCUPANCY[1:] for d in sorted({b["lds_stack_depth"], LDS_DEPTHS[0]})],
+        lambda b: [dict(knobs_of(b), shadow_occupancy=so) for so in SHADOW_OCCUPANCY[1:] if so != b["trace_occupancy"]],
+        lambda b: [dict(knobs_of(b), pass_overlap=ov) for ov in PASS_OVERLAP if ov != b["pass_overlap"]],
+    ]
+    for stage in stages:
+        best = max([best] + [trial(cfg) for cfg in stage(best)], key=mr)
+    best = knobs_of(best)
     apply(renderer, best)
     renderer.clear()
     res = TuneResult(best=best, table=table, seconds=time.perf_counter() - t0)

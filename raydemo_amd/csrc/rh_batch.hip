@@ -921,18 +921,16 @@ int rh_batch_create(int device, rh_batch **out) {
     RH_CHECK(hipSetDevice(device));
     auto *b = new rh_batch;
     b->device = device;
-    // one stream per pass slot (all at the default priority)
-    bool streams_ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
-    for (int j = 1; j < kMaxBatchSlots && streams_ok; j++)
-        streams_ok = hipStreamCreateWithFlags(&b->store[j].stream, hipStreamNonBlocking) == hipSuccess;
-    if (!streams_ok) {
-        if (b->stream) (void)hipStreamDestroy(b->stream);
-        for (int j = 1; j < kMaxBatchSlots; j++)
-            if (b->store[j].stream) (void)hipStreamDestroy(b->store[j].stream);
+    // one stream per pass slot, from the device's shared pool
+    static_assert(kMaxBatchSlots <= rh_rt::kPassStreams, "a stream per pass slot");
+    hipStream_t streams[rh_rt::kPassStreams];
+    const int src = rh_rt::pass_streams(device, streams);
+    if (src) {
         delete b;
-        rh_rt::err() = "hipStreamCreate failed";
-        return -1003;
+        return src;
     }
+    b->stream = streams[0];
+    for (int j = 1; j < kMaxBatchSlots; j++) b->store[j].stream = streams[j];
     RH_CHECK(hipEventCreateWithFlags(&b->ev_mix, hipEventDisableTiming));
     hipDeviceProp_t prop;
     RH_CHECK(hipGetDeviceProperties(&prop, device));
@@ -956,10 +954,7 @@ void rh_batch_destroy(rh_batch *b) {
     free_ptr(b->halton);
     free_ptr(b->stats);
     if (b->ev_mix) (void)hipEventDestroy(b->ev_mix);
-    (void)hipStreamDestroy(b->stream);
-    for (int j = 0; j < kMaxBatchSlots; j++)
-        if (j != b->bound && b->store[j].stream) (void)hipStreamDestroy(b->store[j].stream);
-    delete b;
+    delete b; // the streams stay in the device's pool
 }
 
 int rh_batch_add_scene(rh_batch *b, const rh_scene_view *v, const rh_pass_desc *desc, int w, int h, int spp) {

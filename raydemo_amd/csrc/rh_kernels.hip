@@ -1423,18 +1423,16 @@ int rh_create(int device, rh_context **out) {
     RH_CHECK(hipSetDevice(device));
     auto *c = new rh_context;
     c->device = device;
-    // one stream per pass slot (all at the default priority)
-    bool streams_ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
-    for (int j = 1; j < kMaxPassSlots && streams_ok; j++)
-        streams_ok = hipStreamCreateWithFlags(&c->store[j].stream, hipStreamNonBlocking) == hipSuccess;
-    if (!streams_ok) {
-        if (c->stream) (void)hipStreamDestroy(c->stream);
-        for (int j = 1; j < kMaxPassSlots; j++)
-            if (c->store[j].stream) (void)hipStreamDestroy(c->store[j].stream);
+    // one stream per pass slot, from the device's shared pool
+    static_assert(kMaxPassSlots <= rh_rt::kPassStreams, "a stream per pass slot");
+    hipStream_t streams[rh_rt::kPassStreams];
+    const int src = rh_rt::pass_streams(device, streams);
+    if (src) {
         delete c;
-        rh_rt::err() = "hipStreamCreate failed";
-        return -1003;
+        return src;
     }
+    c->stream = streams[0];
+    for (int j = 1; j < kMaxPassSlots; j++) c->store[j].stream = streams[j];
     RH_CHECK(hipEventCreateWithFlags(&c->ev_mix, hipEventDisableTiming));
     RH_CHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     hipDeviceProp_t prop;
@@ -1493,10 +1491,7 @@ void rh_destroy(rh_context *c) {
     for (auto &e : c->stage_free) (void)hipEventDestroy(e);
     if (c->ev_mix) (void)hipEventDestroy(c->ev_mix);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
-    (void)hipStreamDestroy(c->stream);
-    for (int j = 0; j < kMaxPassSlots; j++)
-        if (j != c->bound && c->store[j].stream) (void)hipStreamDestroy(c->store[j].stream);
-    delete c;
+    delete c; // the streams stay in the device's pool
 }
 
 int rh_resize(rh_context *c, int w, int h) {

@@ -5,9 +5,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -55,6 +57,35 @@ struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
 };
+
+// The streams of the pass slots: kPassStreams per device (the device current
+// when a device's first context asks), created once and shared by every
+// context and batch of the process until it ends (all at the default
+// priority).  A process gets few hardware queues per device; streams created
+// and destroyed by context after context can come to share one queue, which
+// serialises passes meant to overlap.  Contexts sharing a stream only add
+// order between their work: each orders its own with events it recorded
+// before waiting on them.
+constexpr int kPassStreams = 4;
+inline int pass_streams(int device, hipStream_t out[kPassStreams]) {
+    static std::mutex mu;
+    static std::vector<std::array<hipStream_t, kPassStreams>> pool;
+    std::lock_guard<std::mutex> lock(mu);
+    if (device < 0) return arg_error("pass_streams: no device");
+    if ((int)pool.size() <= device) pool.resize((size_t)device + 1, std::array<hipStream_t, kPassStreams>{});
+    auto &p = pool[(size_t)device];
+    for (int j = 0; j < kPassStreams; j++) {
+        if (!p[j]) {
+            const hipError_t e = hipStreamCreateWithFlags(&p[j], hipStreamNonBlocking);
+            if (e != hipSuccess) {
+                p[j] = nullptr;
+                return fail(e, "hipStreamCreateWithFlags");
+            }
+        }
+        out[j] = p[j];
+    }
+    return 0;
+}
 
 } // namespace rh_rt
 
