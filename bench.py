@@ -864,10 +864,26 @@ def main():
         "measured_in": "serial" if ser is not best else "tuned",
         "overlapped_avg_launch_ms": round(best["prof"]["trace_ms"] / max(best["prof"]["trace_launches"], 1), 4),
     }
+    # the trace launches by stage (the primary rays, then each staged bounce):
+    # the same byte model over each stage's counted visits and launch time
+    vis_st = {st["stage"]: st for st in vis.get("trace_stages", [])}
+    per_stage = []
+    for st in prof.get("trace_stages", []):
+        cv = vis_st.get(st["stage"])
+        if not cv or not st["launches"] or st["ms"] <= 0:
+            continue
+        b = (cv["rays"] * (B_RAY + B_HIT) + cv["nodes"] * b_node + cv["instances"] * B_INST + cv["tris"] * B_TRI)
+        per_stage.append({"stage": "primary" if st["stage"] == 0 else f"bounce {st['stage'] - 1}",
+                          "launches": st["launches"], "avg_launch_ms": round(st["ms"] / st["launches"], 4),
+                          "rays_per_launch": int(cv["rays"] / st["launches"]),
+                          "alg_bytes_per_launch": int(b / st["launches"]),
+                          "achieved": round(b / (st["ms"] / 1e3) / 1e9, 1),
+                          "frac": round(b / (st["ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)})
+    roofline["per_stage"] = per_stage
     # the other traversal kernels from the same replay's counters: the shadow
     # kernel per queue entry, the path tail per shaded segment and light sample
     sh, tl = vis["shadow"], vis["tail"]
-    per_kernel = [dict(roofline)]
+    per_kernel = [{k: v for k, v in roofline.items() if k != "per_stage"}]
     if prof["shadow_launches"]:
         b_sh = sh["entries"] * B_SHADOW_ENTRY + traversal_bytes(sh, wide)
         per_kernel.append(kernel_roofline("k_shadow", b_sh, prof["shadow_ms"], prof["shadow_launches"]))

@@ -318,6 +318,8 @@ typedef struct rh_ray_counts {
 /* per-kernel profile: kinds 0 trace, 1 shade, 2 sort, 3 shadow (deferred NEE visibility),
  * 4 path tail (the last bounces traced and shaded in one launch) */
 #define RH_KERNEL_KINDS 5
+/* stages of a pass's closest-hit trace launches: 0 the primary rays, b + 1 bounce b */
+#define RH_TRACE_STAGES (RH_MAX_BOUNCES_EXT + 1)
 typedef struct rh_kernel_profile {
     double ms[RH_KERNEL_KINDS];
     unsigned long long launches[RH_KERNEL_KINDS];
@@ -328,6 +330,11 @@ typedef struct rh_kernel_profile {
     /* the counting path-tail kernels: node visits, triangle tests, instance tests (path and shadow
      * traversals together), path segments, shadow segments */
     unsigned long long tail_visits[5];
+    /* the closest-hit trace launches of rh_render_pass by stage: time and launches (with kernel_timing),
+     * node visits, triangle tests, instance tests and rays (with count_visits; also in the totals above) */
+    double trace_stage_ms[RH_TRACE_STAGES];
+    unsigned long long trace_stage_launches[RH_TRACE_STAGES];
+    unsigned long long trace_stage_visits[RH_TRACE_STAGES][4];
 } rh_kernel_profile;
 
 #ifdef __cplusplus

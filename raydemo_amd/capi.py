@@ -78,10 +78,15 @@ class RayCounts(Structure):
                 ("adjoint", c_uint64), ("adjoint_shadow", c_uint64), ("overflow", c_uint64)]
 
 
+TRACE_STAGES = 17  # RH_TRACE_STAGES: the primary rays, then bounces 0..15
+
+
 class KernelProfile(Structure):
     _fields_ = [("ms", C.c_double * 5), ("launches", c_uint64 * 5), ("node_visits", c_uint64),
                 ("tri_tests", c_uint64), ("instance_tests", c_uint64), ("counted_rays", c_uint64),
-                ("shadow_visits", c_uint64 * 5), ("tail_visits", c_uint64 * 5)]
+                ("shadow_visits", c_uint64 * 5), ("tail_visits", c_uint64 * 5),
+                ("trace_stage_ms", C.c_double * TRACE_STAGES), ("trace_stage_launches", c_uint64 * TRACE_STAGES),
+                ("trace_stage_visits", (c_uint64 * 4) * TRACE_STAGES)]
 
 
 class PassSettings(Structure):
@@ -349,6 +354,11 @@ class Renderer:
         names = ("nodes", "tris", "instances")
         d["shadow"] = dict(zip(names + ("entries", "segments"), list(p.shadow_visits)))
         d["tail"] = dict(zip(names + ("segments", "shadow_segments"), list(p.tail_visits)))
+        # closest-hit trace launches of the passes by stage (0: primary rays, b + 1: bounce b)
+        d["trace_stages"] = [
+            {"stage": k, "ms": p.trace_stage_ms[k], "launches": p.trace_stage_launches[k],
+             **dict(zip(names + ("rays",), list(p.trace_stage_visits[k])))}
+            for k in range(TRACE_STAGES) if p.trace_stage_launches[k] or p.trace_stage_visits[k][3]]
         return d
 
     def reset_kernel_profile(self):

@@ -804,7 +804,8 @@ struct rh_context {
     // profiling: event pairs around trace / shade / sort launches, read back on demand
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
-    std::vector<std::pair<int, std::pair<size_t, size_t>>> ev_marks; // kind, (start, stop)
+    std::vector<std::pair<int, std::pair<size_t, size_t>>> ev_marks; // kind + 8 (stage + 1), (start, stop)
+    int mark_stage = -1; // trace stage KernelMark tags a trace launch with (-1: none)
     // Count-mode traversal work: [0..3] trace (nodes, tris, instances, rays),
     // [4..8] shadow (nodes, tris, instances, entries, segments), [9..13] path
     // tail (nodes, tris, instances, segments, shadow segments)
@@ -1074,7 +1075,7 @@ struct KernelMark {
             const size_t b = ev_next(c);
             if (b == (size_t)-1) return;
             (void)hipEventRecord(c->ev_pool[b], c->stream);
-            c->ev_marks.push_back({kind, {a, b}});
+            c->ev_marks.push_back({kind + (kind == 0 ? 8 * (c->mark_stage + 1) : 0), {a, b}});
         }
     }
 };
@@ -1301,7 +1302,10 @@ int sort_queue(hipStream_t stream, const rh_ray *rays_in, const rh_ray_aux *aux_
 
 PassConst make_pass_const(rh_context *c, const rh_pass_desc *p) { return make_pass_const(c->w, c->h, p); }
 
-constexpr int kVisitWords = 16; // the Count-mode counters of rh_context::visits
+// the Count-mode counters of rh_context::visits: kVisitWords words, the pass
+// trace launches' per stage from kStageVisits (4 per stage)
+constexpr int kStageVisits = 16;
+constexpr int kVisitWords = kStageVisits + 4 * RH_TRACE_STAGES;
 
 // stats_t timers a pass's intervals add to
 enum StageTimerIndex : int {
@@ -1695,7 +1699,11 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool k
                                c->halton, c->rays[0], c->aux[0], cnt[0]);
         RH_CHECK(hipGetLastError());
         tm.mark(kRayGen);
-        rc = launch_trace(c, trace_args(c, c->rays[0], cnt[0], n, c->hits, nullptr, count_primary), n, false);
+        TraceArgs ta = trace_args(c, c->rays[0], cnt[0], n, c->hits, nullptr, count_primary);
+        ta.visits = c->visits + kStageVisits; // stage 0
+        c->mark_stage = 0;
+        rc = launch_trace(c, ta, n, false);
+        c->mark_stage = -1;
         if (rc) return rc;
         tm.mark(kPrimaryTrace);
     }
@@ -1795,7 +1803,11 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool k
             q ^= 1;
             tm.mark(kSecondarySort);
         }
-        rc = launch_trace(c, trace_args(c, c->rays[q], cnt[q], n, c->hits, nullptr, count_secondary), n, false);
+        TraceArgs ta = trace_args(c, c->rays[q], cnt[q], n, c->hits, nullptr, count_secondary);
+        ta.visits = c->visits + kStageVisits + 4 * (bounce + 1); // stage bounce + 1
+        c->mark_stage = bounce + 1;
+        rc = launch_trace(c, ta, n, false);
+        c->mark_stage = -1;
         if (rc) return rc;
         tm.mark(kSecondaryTrace);
         hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, c->stream, cnt[q ^ 1], 0);
@@ -2095,15 +2107,27 @@ int rh_get_kernel_profile(rh_context *c, rh_kernel_profile *out) {
     for (const auto &m : c->ev_marks) {
         float ms = 0;
         RH_CHECK(hipEventElapsedTime(&ms, c->ev_pool[m.second.first], c->ev_pool[m.second.second]));
-        out->ms[m.first] += ms;
-        out->launches[m.first]++;
+        const int kind = m.first & 7, stage = (m.first >> 3) - 1;
+        out->ms[kind] += ms;
+        out->launches[kind]++;
+        if (stage >= 0 && stage < RH_TRACE_STAGES) {
+            out->trace_stage_ms[stage] += ms;
+            out->trace_stage_launches[stage]++;
+        }
     }
     unsigned long long v[kVisitWords];
     RH_CHECK(hipMemcpy(v, c->visits, sizeof(v), hipMemcpyDeviceToHost));
-    out->node_visits = v[0];
-    out->counted_rays = v[3];
-    out->tri_tests = v[1];
-    out->instance_tests = v[2];
+    // the trace totals: the parity entry points' launches and every stage's
+    unsigned long long tot[4] = {v[0], v[1], v[2], v[3]};
+    for (int k = 0; k < RH_TRACE_STAGES; k++)
+        for (int j = 0; j < 4; j++) {
+            out->trace_stage_visits[k][j] = v[kStageVisits + 4 * k + j];
+            tot[j] += v[kStageVisits + 4 * k + j];
+        }
+    out->node_visits = tot[0];
+    out->counted_rays = tot[3];
+    out->tri_tests = tot[1];
+    out->instance_tests = tot[2];
     for (int k = 0; k < 5; k++) {
         out->shadow_visits[k] = v[4 + k];
         out->tail_visits[k] = v[9 + k];
