@@ -383,9 +383,11 @@ def run_batch(args, lib, A, scenes, knobs, wide):
     # setting: path tail, LDS share of the traversal stack, persistent grid
     # size, sort-key bits of the per-scene queue sort, and the queue size (rays
     # in flight: scenes grouped to fit, 0 = every scene at once)
-    space = {"path_tail": (-1, 1, 2, 3), "lds_stack_depth": (8, 16, 24, 32), "blocks_per_cu": (4, 8, 16),
-             "sort_bits": (0, 8, 16, 24, 32), "queue_rays": (65536, 262144, 1048576, 4194304, 0)}
-    cur = {"path_tail": 2, "lds_stack_depth": 16, "blocks_per_cu": 8, "sort_bits": 0, "queue_rays": 0}
+    space = {"path_tail": (-1, 1, 2, 3), "lds_stack_depth": (8, 16, 24, 32), "blocks_per_cu": (4, 5, 8, 16),
+             "sort_bits": (0, 8, 16, 24, 32), "queue_rays": (65536, 262144, 1048576, 4194304, 0),
+             "pass_overlap": (3, 2, 1, 0)}
+    cur = {"path_tail": 2, "lds_stack_depth": 16, "blocks_per_cu": 8, "sort_bits": 0, "queue_rays": 0, "pass_overlap": 3}
+    tune_passes = 6
     table, seen = [], {}
 
     def measure(cfg):
@@ -400,7 +402,7 @@ def run_batch(args, lib, A, scenes, knobs, wide):
         b.restart()
         b.reset_stats()
         t2 = time.perf_counter()
-        for _ in range(3):
+        for _ in range(tune_passes):
             b.render_pass()
         b.synchronize()
         mr = round(total(b.ray_counts()) / (time.perf_counter() - t2) / 1e6, 2)
@@ -437,8 +439,8 @@ def run_batch(args, lib, A, scenes, knobs, wide):
         "speedup_vs_overlapped": round(rays_batch / t_batch / (rays_overlap / t_overlap), 3),
         "knobs": dict(cur, use_graph=1),
         "plain_knobs": knobs,
-        "autotune": {"method": "coordinate descent from the defaults, two rounds, 3 passes per setting",
-                     "columns": list(space) + ["mrays_first_3_passes"], "table": table},
+        "autotune": {"method": f"coordinate descent from the defaults, two rounds, {tune_passes} passes per setting",
+                     "columns": list(space) + [f"mrays_first_{tune_passes}_passes"], "table": table},
         "config": {"workload": f"{len(items)} icosphere soups, {sum(s.triangle_count() for s in ss)} triangles, "
                                f"{sum(i.width * i.height for i in items)} pixels, 4..32 spp, depth 4..8",
                    "scenes": len(items), "trees": "8-wide" if wide else "binary"},
@@ -664,8 +666,9 @@ def run_large(args, lib, A, scenes, knobs, wide):
         # batch knobs: the 64-scene workload's, or every bounce staged (deep paths)
         bsweep = []
         base_b = dict(args.large_batch_knobs or {})
-        for cfg in (base_b, dict(base_b, path_tail=-1), dict(base_b, path_tail=-1, blocks_per_cu=8),
-                    dict(base_b, path_tail=8, blocks_per_cu=8), dict(base_b, path_tail=-1, lds_stack_depth=8)):
+        for cfg in (base_b, dict(base_b, path_tail=-1), dict(base_b, path_tail=-1, blocks_per_cu=5, lds_stack_depth=16),
+                    dict(base_b, path_tail=-1, blocks_per_cu=8), dict(base_b, path_tail=8, blocks_per_cu=5),
+                    dict(base_b, path_tail=-1, lds_stack_depth=8)):
             for k, v in cfg.items():
                 b.set_option(k, v)
             b.restart()

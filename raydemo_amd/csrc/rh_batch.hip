@@ -119,7 +119,10 @@ __device__ __forceinline__ void scene_tile_prefix(const BatchDev &b, const int *
 // Runs body(scene, tile base, global tile) for every TileRays-ray tile of
 // every scene's queue, count_s = counts[s] rays (0 for scenes that skip this
 // stage), the tiles numbered scene after scene.  Persistent: blocks take
-// tiles from *work until none is left.
+// runs of consecutive tiles from *work until none is left.  One counter
+// hands them out and same-address atomics serialise (a 4K frame is 32k
+// tiles a stage), so a block takes about a quarter of its fair share per
+// grab, and one tile at a time once there are few tiles per block.
 template <int TileRays = kBlock, typename Active, typename Body>
 __device__ __forceinline__ void for_each_tile(const BatchDev &b, const int *counts, int *work, Active active,
                                               Body body) {
@@ -128,22 +131,62 @@ __device__ __forceinline__ void for_each_tile(const BatchDev &b, const int *coun
     const int S = b.n_scenes;
     scene_tile_prefix<TileRays>(b, counts, active, s_pre);
     const int total = s_pre[S];
+    const int chunk = max(1, total / (int)(4 * gridDim.x));
     for (;;) {
-        if (threadIdx.x == 0) s_t = atomicAdd(work, 1);
+        if (threadIdx.x == 0) s_t = atomicAdd(work, chunk);
         __syncthreads();
-        const int t = uniform(s_t);
+        const int t0 = uniform(s_t);
         __syncthreads();
-        if (t >= total) break;
-        int lo = 0, hi = S; // s_pre[lo] <= t < s_pre[hi]
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (s_pre[mid] <= t) lo = mid;
-            else hi = mid;
+        if (t0 >= total) break;
+        const int t1 = min(t0 + chunk, total);
+        for (int t = t0; t < t1; t++) {
+            int lo = 0, hi = S; // s_pre[lo] <= t < s_pre[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_pre[mid] <= t) lo = mid;
+                else hi = mid;
+            }
+            const int s = uniform(lo);
+            if constexpr (std::is_invocable_v<Body, int, uint32_t, int>) body(s, (uint32_t)(t - s_pre[s]) * TileRays, t);
+            else body(s, (uint32_t)(t - s_pre[s]) * TileRays);
+            __syncthreads(); // the body's LDS is reused by the next tile
         }
-        const int s = uniform(lo);
-        if constexpr (std::is_invocable_v<Body, int, uint32_t, int>) body(s, (uint32_t)(t - s_pre[s]) * TileRays, t);
-        else body(s, (uint32_t)(t - s_pre[s]) * TileRays);
-        __syncthreads(); // the body's LDS is reused by the next tile
+    }
+}
+
+// for_each_tile for bodies whose lanes work alone (the traversal kernels:
+// per-lane LDS stack columns and wave-level counters, no block-wide LDS or
+// barrier): each wave takes runs of 64-ray tiles by itself, so a wave whose
+// rays are done takes new ones while its block's other waves still trace.
+// The body gets base = the tile's first ray minus 64 x the wave's index
+// within the block (unsigned arithmetic), so that base + threadIdx.x is the
+// lane's ray.
+template <typename Active, typename Body>
+__device__ __forceinline__ void for_each_wave_tile(const BatchDev &b, const int *counts, int *work, Active active,
+                                                   Body body) {
+    __shared__ int s_pre[kMaxBatchScenes + 1];
+    const int S = b.n_scenes;
+    scene_tile_prefix<64>(b, counts, active, s_pre);
+    const int total = s_pre[S];
+    const int waves = (int)(gridDim.x * (blockDim.x >> 6));
+    const int chunk = max(1, total / (4 * waves));
+    const uint32_t wave_first = threadIdx.x & ~63u;
+    for (;;) {
+        int t0 = 0;
+        if (lane_id() == 0) t0 = atomicAdd(work, chunk);
+        t0 = uniform(__shfl(t0, 0));
+        if (t0 >= total) break;
+        const int t1 = min(t0 + chunk, total);
+        for (int t = t0; t < t1; t++) {
+            int lo = 0, hi = S; // s_pre[lo] <= t < s_pre[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_pre[mid] <= t) lo = mid;
+                else hi = mid;
+            }
+            const int s = uniform(lo);
+            body(s, (uint32_t)(t - s_pre[s]) * 64u - wave_first);
+        }
     }
 }
 
@@ -194,7 +237,7 @@ __device__ __forceinline__ bool stage_active(const BatchEntry &e, int stage) {
 template <int Tree> __global__ void __launch_bounds__(kBlock) k_b_trace(BatchDev b, int stage, int q, int work) {
     extern __shared__ uint32_t lds_stack[];
     const int *counts = b.counts + stage * b.n_scenes;
-    for_each_tile(b, counts, b.work + work, [&](int s) { return stage_active(b.table[s], stage); },
+    for_each_wave_tile(b, counts, b.work + work, [&](int s) { return stage_active(b.table[s], stage); },
                   [&](int s, uint32_t base) {
                       const BatchEntry e = entry(b, s);
                       TraceArgs a;
@@ -270,7 +313,7 @@ template <bool Defer> __global__ void __launch_bounds__(kBlock) k_b_shade(BatchD
 template <int Tree> __global__ void __launch_bounds__(kBlock) k_b_shadow(BatchDev b, int stage, int work) {
     extern __shared__ uint32_t lds_stack[];
     const int *counts = b.shadow_counts + stage * b.n_scenes;
-    for_each_tile(b, counts, b.work + work, [&](int s) { return b.table[s].nee_slots != 0; },
+    for_each_wave_tile(b, counts, b.work + work, [&](int s) { return b.table[s].nee_slots != 0; },
                   [&](int s, uint32_t base) {
                       const BatchEntry e = entry(b, s);
                       ShadowArgs a;
@@ -463,7 +506,7 @@ template <bool Defer, int Tree>
 __global__ void __launch_bounds__(kBlock) k_b_tail(BatchDev b, int stage, int q, int work) {
     extern __shared__ uint32_t lds_stack[];
     const int *counts = b.counts + stage * b.n_scenes;
-    for_each_tile(b, counts, b.work + work,
+    for_each_wave_tile(b, counts, b.work + work,
                   [&](int s) { return stage_active(b.table[s], stage) && ((b.table[s].nee_slots != 0) == Defer); },
                   [&](int s, uint32_t base) {
                       const BatchEntry e = entry(b, s);
