@@ -140,6 +140,12 @@ inline int build_wide(const rh_scene_view *v, hipStream_t stream, std::vector<rh
         if (!is_leaf(i) || root[i]) dev[i] = m++;
     if (m >= (1u << 28)) return rh_rt::arg_error("too many wide nodes for the device form (2^28)");
 
+    // kWideHalf on the words naming an interior node with at most 4 used slots
+    auto half_flag = [&](uint32_t i) {
+        int used = 0;
+        for (int k = 0; k < 8; k++) used += v->mnodes[i].child[k] != RH_MBVH_EMPTY;
+        return used <= 4 ? kWideHalf : 0u;
+    };
     std::vector<float4> w((size_t)m * 16);
     const float nan = std::numeric_limits<float>::quiet_NaN();
     auto as_float = [](uint32_t u) {
@@ -166,7 +172,7 @@ inline int build_wide(const rh_scene_view *v, hipStream_t stream, std::vector<rh
                     word = v->mnodes[c].child[0];
                     count = v->mnodes[c].child[1];
                 } else {
-                    word = dev[c];
+                    word = dev[c] | half_flag(c);
                 }
             }
             out[2 * j] = make_float4(nd.bbox_min[0][k], nd.bbox_min[1][k], nd.bbox_min[2][k], as_float(word));
@@ -181,7 +187,7 @@ inline int build_wide(const rh_scene_view *v, hipStream_t stream, std::vector<rh
     int rc = upload_array(bufs, stream, w.data(), w.size(), &s.wnodes);
     if (rc) return rc;
     // a tree root is named by its node's index, flagged when it is a leaf
-    auto root_word = [&](uint32_t i) { return dev[i] | (is_leaf(i) ? kWideLeafRoot : 0u); };
+    auto root_word = [&](uint32_t i) { return dev[i] | (is_leaf(i) ? kWideLeafRoot : half_flag(i)); };
     std::vector<rh_mesh> meshes(v->meshes, v->meshes + v->meshes_count);
     for (rh_mesh &mesh : meshes) mesh.node_index = root_word(mesh.node_index);
     rc = upload_array(bufs, stream, meshes.data(), meshes.size(), &s.meshes);
