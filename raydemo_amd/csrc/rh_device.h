@@ -94,8 +94,14 @@ struct Scene {
     const rh_bvh_node *nodes;
     const rh_tri_accel *tris;
     const uint32_t *tri_indices;
+    // tris in tri_indices order (leaf_tris[k] = tris[tri_indices[k]]): a
+    // leaf's triangle is loaded beside its index instead of after it
+    const rh_tri_accel *leaf_tris;
     const rh_mesh *meshes;
     const rh_mesh_instance *mesh_instances;
+    // one record of 5 float4 per mi_indices entry, the instance that entry
+    // names with its tree root and inverse transform resolved (load_instance)
+    const float4 *inst_recs;
     const uint32_t *mi_indices;
     const rh_transform *transforms;
     const rh_vertex *vertices;
@@ -110,6 +116,9 @@ struct Scene {
     uint32_t light_leaf_count;
     rh_environment env;
     uint32_t macro_root;
+    // 8-wide scenes: the macro tree's root as wide_root reads it (a leaf
+    // root's range without the load)
+    uint32_t macro_word, macro_count;
     uint32_t light_root;
     // 8-wide trees (see the wide traversal below); null for binary scenes
     const float4 *wnodes;
@@ -262,6 +271,40 @@ __device__ __forceinline__ NodeBox load_node(const rh_bvh_node *nodes, uint32_t 
     return n;
 }
 
+// Instance record k (the instance mi_indices[k] names): r0 = (bbox_min,
+// tree root), r1 = (bbox_max, instance index), r2..r4 = the rows of its
+// inverse transform's 3x4 part, so that the box test and TransformRay need
+// one round of independent loads instead of three dependent ones.
+struct InstanceRec {
+    float bmin[3], bmax[3];
+    uint32_t root, mi_index;
+    float4 x0, x1, x2;
+};
+__device__ __forceinline__ InstanceRec load_instance(const Scene &sc, uint32_t k) {
+    const float4 *p = sc.inst_recs + (size_t)k * 5;
+    const float4 a = p[0], b = p[1];
+    InstanceRec r;
+    r.bmin[0] = a.x;
+    r.bmin[1] = a.y;
+    r.bmin[2] = a.z;
+    r.root = __float_as_uint(a.w);
+    r.bmax[0] = b.x;
+    r.bmax[1] = b.y;
+    r.bmax[2] = b.z;
+    r.mi_index = __float_as_uint(b.w);
+    r.x0 = p[2];
+    r.x1 = p[3];
+    r.x2 = p[4];
+    return r;
+}
+// TransformRay (CoreRef.cpp:1605-1617), the same products and sums in the same order
+__device__ __forceinline__ void transform_ray(const InstanceRec &r, f3 o, f3 d, f3 &mo, f3 &md) {
+    mo = f3{r.x0.x * o.x + r.x0.y * o.y + r.x0.z * o.z + r.x0.w, r.x1.x * o.x + r.x1.y * o.y + r.x1.z * o.z + r.x1.w,
+            r.x2.x * o.x + r.x2.y * o.y + r.x2.z * o.z + r.x2.w};
+    md = f3{r.x0.x * d.x + r.x0.y * d.y + r.x0.z * d.z, r.x1.x * d.x + r.x1.y * d.y + r.x1.z * d.z,
+            r.x2.x * d.x + r.x2.y * d.y + r.x2.z * d.z};
+}
+
 // Per-lane traversal stack: the first LDS_DEPTH entries live in LDS (lane-minor,
 // so the 64 lanes of a wave hit distinct banks), deeper entries spill to a
 // per-thread slice of global memory; pushes past LDS_DEPTH + SPILL_DEPTH are
@@ -356,9 +399,10 @@ __device__ __forceinline__ bool intersect_leaf(const Scene &sc, const NodeBox &n
                                                TraceCounters *cnt) {
     const uint32_t first = n.a & RH_PRIM_INDEX_BITS, count = n.b;
     bool leaf_hit = false;
+    int hit_ci = 0; // flags of the leaf's latest hit triangle
     for (uint32_t k = first; k < first + count; k++) {
         const uint32_t ti = sc.tri_indices[k];
-        const rh_tri_accel tri = load_tri(sc.tris, ti);
+        const rh_tri_accel tri = load_tri(sc.leaf_tris, k);
         if (cnt) cnt->leaf_tris++;
         float t, u, v;
         if (intersect_tri(o, d, tri, hit.t, t, u, v)) {
@@ -369,14 +413,11 @@ __device__ __forceinline__ bool intersect_leaf(const Scene &sc, const NodeBox &n
             hit.u = u;
             hit.v = v;
             leaf_hit = true;
+            hit_ci = tri.ci;
         }
         if (AnyHit && leaf_hit && (tri.ci & RH_TRI_SOLID_BIT)) break;
     }
-    if (AnyHit && leaf_hit) {
-        const rh_tri_accel ht = load_tri(sc.tris, (uint32_t)hit.prim);
-        return (ht.ci & RH_TRI_SOLID_BIT) != 0;
-    }
-    return false;
+    return AnyHit && leaf_hit && (hit_ci & RH_TRI_SOLID_BIT);
 }
 
 // Interior-node step: both children's boxes are tested here (siblings are
@@ -472,18 +513,14 @@ __device__ __forceinline__ void traverse_fast(const Scene &sc, f3 o, f3 d, Hit &
         }
         // instances of the current macro leaf, in index order
         while (leaf_cur < leaf_end) {
-            const uint32_t mi_index = sc.mi_indices[leaf_cur++];
-            const rh_mesh_instance mi = sc.mesh_instances[mi_index];
+            const InstanceRec mi = load_instance(sc, leaf_cur++);
+            const uint32_t mi_index = mi.mi_index;
             if (cnt) cnt->instances++;
-            if (!box_hit<true>(o, inv_d, hit.t, mi.bbox_min, mi.bbox_max)) continue;
-            const float *x = sc.transforms[mi.tr_index].inv_xform;
-            // TransformRay (CoreRef.cpp:1605-1617)
-            const f3 mo = f3{x[0] * o.x + x[4] * o.y + x[8] * o.z + x[12], x[1] * o.x + x[5] * o.y + x[9] * o.z + x[13],
-                             x[2] * o.x + x[6] * o.y + x[10] * o.z + x[14]};
-            const f3 md = f3{x[0] * d.x + x[4] * d.y + x[8] * d.z, x[1] * d.x + x[5] * d.y + x[9] * d.z,
-                             x[2] * d.x + x[6] * d.y + x[10] * d.z};
+            if (!box_hit<true>(o, inv_d, hit.t, mi.bmin, mi.bmax)) continue;
+            f3 mo, md;
+            transform_ray(mi, o, d, mo, md);
             const f3 minv = f3{1.0f / md.x, 1.0f / md.y, 1.0f / md.z};
-            const NodeBox mc = load_node(sc.nodes, sc.meshes[mi.mesh_index].node_index);
+            const NodeBox mc = load_node(sc.nodes, mi.root);
             if (cnt) cnt->nodes++;
             const bool finite = __builtin_isfinite(minv.x) && __builtin_isfinite(minv.y) && __builtin_isfinite(minv.z);
             const bool solid = finite ? walk_micro<AnyHit, true>(sc, mc, mo, md, minv, (int)mi_index, hit, st, cnt)
@@ -779,24 +816,19 @@ __device__ __forceinline__ void traverse_wide(const Scene &sc, f3 o, f3 d, Hit &
                                               bool first_hit = false) {
     const f3 inv_d = safe_invert(d);
     st.sp = 0;
-    WideCur cur = wide_root(sc.wnodes, sc.macro_root);
+    WideCur cur = WideCur{sc.macro_word, sc.macro_count};
     while (true) {
         if (cur.word & RH_LEAF_NODE_BIT) {
             const uint32_t first = cur.word & RH_PRIM_INDEX_BITS, end = first + cur.count;
             for (uint32_t i = first; i < end; i++) {
-                const uint32_t mi_index = sc.mi_indices[i];
-                const rh_mesh_instance mi = sc.mesh_instances[mi_index];
+                const InstanceRec mi = load_instance(sc, i);
+                const uint32_t mi_index = mi.mi_index;
                 if (cnt) cnt->instances++;
-                if (!box_hit<true>(o, inv_d, hit.t, mi.bbox_min, mi.bbox_max)) continue;
-                const float *x = sc.transforms[mi.tr_index].inv_xform;
-                // TransformRay (CoreRef.cpp:1605-1617)
-                const f3 mo = f3{x[0] * o.x + x[4] * o.y + x[8] * o.z + x[12],
-                                 x[1] * o.x + x[5] * o.y + x[9] * o.z + x[13],
-                                 x[2] * o.x + x[6] * o.y + x[10] * o.z + x[14]};
-                const f3 md = f3{x[0] * d.x + x[4] * d.y + x[8] * d.z, x[1] * d.x + x[5] * d.y + x[9] * d.z,
-                                 x[2] * d.x + x[6] * d.y + x[10] * d.z};
+                if (!box_hit<true>(o, inv_d, hit.t, mi.bmin, mi.bmax)) continue;
+                f3 mo, md;
+                transform_ray(mi, o, d, mo, md);
                 const f3 minv = f3{1.0f / md.x, 1.0f / md.y, 1.0f / md.z};
-                const uint32_t root = sc.meshes[mi.mesh_index].node_index;
+                const uint32_t root = mi.root;
                 const bool finite =
                     __builtin_isfinite(minv.x) && __builtin_isfinite(minv.y) && __builtin_isfinite(minv.z);
                 const bool fh = AnyHit && first_hit;

@@ -121,7 +121,7 @@ int upload_array(std::vector<rh_rt::DevBuf> &bufs, hipStream_t stream, const T *
 // it), so testing the leaf boxes alone, in this order, takes the same lights
 // in the same order; a leaf root has no box test at all.
 inline int build_wide(const rh_scene_view *v, hipStream_t stream, std::vector<rh_rt::DevBuf> &bufs, Scene &s,
-                      std::vector<rh_bvh_node> &leaves) {
+                      std::vector<rh_bvh_node> &leaves, std::vector<rh_mesh> &meshes) {
     const uint32_t n = v->mnodes_count;
     auto is_leaf = [&](uint32_t i) { return (v->mnodes[i].child[0] & RH_LEAF_NODE_BIT) != 0; };
     // device nodes: the interior nodes and the tree roots (a leaf child lives in its parent's slot)
@@ -188,11 +188,16 @@ inline int build_wide(const rh_scene_view *v, hipStream_t stream, std::vector<rh
     if (rc) return rc;
     // a tree root is named by its node's index, flagged when it is a leaf
     auto root_word = [&](uint32_t i) { return dev[i] | (is_leaf(i) ? kWideLeafRoot : half_flag(i)); };
-    std::vector<rh_mesh> meshes(v->meshes, v->meshes + v->meshes_count);
+    meshes.assign(v->meshes, v->meshes + v->meshes_count);
     for (rh_mesh &mesh : meshes) mesh.node_index = root_word(mesh.node_index);
     rc = upload_array(bufs, stream, meshes.data(), meshes.size(), &s.meshes);
     if (rc) return rc;
-    if (s.macro_root != RH_INVALID) s.macro_root = root_word(s.macro_root);
+    if (s.macro_root != RH_INVALID) {
+        const rh_mbvh_node &mr = v->mnodes[s.macro_root];
+        s.macro_word = is_leaf(s.macro_root) ? mr.child[0] : root_word(s.macro_root);
+        s.macro_count = is_leaf(s.macro_root) ? mr.child[1] : 0u;
+        s.macro_root = root_word(s.macro_root);
+    }
 
     if (s.light_root == RH_INVALID) return 0;
     if (s.light_root >= n) return rh_rt::arg_error("light tree root outside the node array");
@@ -233,6 +238,16 @@ inline int build_device_static(const rh_scene_view *v, hipStream_t stream, std::
     int rc = 0;
     rc |= upload_array(bufs, stream, v->tris, v->tris_count, &s.tris);
     rc |= upload_array(bufs, stream, v->tri_indices, v->tri_indices_count, &s.tri_indices);
+    if (v->tri_indices_count) {
+        std::vector<rh_tri_accel> leaf_tris(v->tri_indices_count);
+        for (size_t k = 0; k < v->tri_indices_count; k++) {
+            if (v->tri_indices[k] >= v->tris_count) return rh_rt::arg_error("triangle index outside the triangle array");
+            leaf_tris[k] = v->tris[v->tri_indices[k]];
+        }
+        rc |= upload_array(bufs, stream, leaf_tris.data(), leaf_tris.size(), &s.leaf_tris);
+        // the upload is asynchronous: the staging vector must outlive it
+        if (!rc) RH_CHECK(hipStreamSynchronize(stream));
+    }
     rc |= upload_array(bufs, stream, v->vertices, v->vertices_count, &s.vertices);
     rc |= upload_array(bufs, stream, v->vtx_indices, v->vtx_indices_count, &s.vtx_indices);
     rc |= upload_array(bufs, stream, v->materials, v->materials_count, &s.materials);
@@ -269,10 +284,35 @@ inline int build_device_instances(const rh_scene_view *v, hipStream_t stream, st
     s.macro_root = v->macro_nodes_root;
     s.light_root = v->light_nodes_root;
     std::vector<rh_bvh_node> leaves;
+    std::vector<rh_mesh> meshes; // as the device walks name their roots
     if (v->mnodes) {
-        rc = build_wide(v, stream, bufs, s, leaves);
+        rc = build_wide(v, stream, bufs, s, leaves, meshes);
         if (rc) return rc;
-    } else if (s.light_root != RH_INVALID && v->nodes && s.light_root < v->nodes_count) {
+    } else {
+        meshes.assign(v->meshes, v->meshes + v->meshes_count);
+    }
+    // the instance records (load_instance), in mi_indices order
+    std::vector<float4> recs((size_t)v->mi_indices_count * 5);
+    for (size_t k = 0; k < v->mi_indices_count; k++) {
+        const uint32_t mi = v->mi_indices[k];
+        if (mi >= v->mesh_instances_count) return rh_rt::arg_error("instance index outside the instance array");
+        const rh_mesh_instance &in = v->mesh_instances[mi];
+        if (in.tr_index >= v->transforms_count || in.mesh_index >= meshes.size())
+            return rh_rt::arg_error("instance names a transform or mesh outside its array");
+        const float *x = v->transforms[in.tr_index].inv_xform;
+        uint32_t root = meshes[in.mesh_index].node_index;
+        float4 *r = &recs[k * 5];
+        r[0] = make_float4(in.bbox_min[0], in.bbox_min[1], in.bbox_min[2], 0.0f);
+        r[1] = make_float4(in.bbox_max[0], in.bbox_max[1], in.bbox_max[2], 0.0f);
+        memcpy(&r[0].w, &root, 4);
+        memcpy(&r[1].w, &mi, 4);
+        r[2] = make_float4(x[0], x[4], x[8], x[12]);
+        r[3] = make_float4(x[1], x[5], x[9], x[13]);
+        r[4] = make_float4(x[2], x[6], x[10], x[14]);
+    }
+    rc = upload_array(bufs, stream, recs.data(), recs.size(), &s.inst_recs);
+    if (rc) return rc;
+    if (!v->mnodes && s.light_root != RH_INVALID && v->nodes && s.light_root < v->nodes_count) {
         std::vector<uint32_t> st{s.light_root};
         while (!st.empty()) {
             const uint32_t cur = st.back();

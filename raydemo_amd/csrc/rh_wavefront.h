@@ -384,19 +384,15 @@ __device__ __forceinline__ void trace_refill_body(const TraceArgs &a, int *next_
             }
         } else if (step == kStepInstances) {
             if (leaf_cur < leaf_end) {
-                mi_index = (int)a.sc.mi_indices[leaf_cur++];
-                const rh_mesh_instance mi = a.sc.mesh_instances[mi_index];
+                const InstanceRec mi = load_instance(a.sc, leaf_cur++);
+                mi_index = (int)mi.mi_index;
                 if (cnt) cnt->instances++;
-                if (box_hit<true>(o, inv_d, hit.t, mi.bbox_min, mi.bbox_max)) {
-                    const float *x = a.sc.transforms[mi.tr_index].inv_xform;
-                    mo = f3{x[0] * o.x + x[4] * o.y + x[8] * o.z + x[12], x[1] * o.x + x[5] * o.y + x[9] * o.z + x[13],
-                            x[2] * o.x + x[6] * o.y + x[10] * o.z + x[14]};
-                    md = f3{x[0] * d.x + x[4] * d.y + x[8] * d.z, x[1] * d.x + x[5] * d.y + x[9] * d.z,
-                            x[2] * d.x + x[6] * d.y + x[10] * d.z};
+                if (box_hit<true>(o, inv_d, hit.t, mi.bmin, mi.bmax)) {
+                    transform_ray(mi, o, d, mo, md);
                     minv = f3{1.0f / md.x, 1.0f / md.y, 1.0f / md.z};
                     finite = __builtin_isfinite(minv.x) && __builtin_isfinite(minv.y) && __builtin_isfinite(minv.z);
                     base = st.sp;
-                    node = load_node(a.sc.nodes, a.sc.meshes[mi.mesh_index].node_index);
+                    node = load_node(a.sc.nodes, mi.root);
                     if (cnt) cnt->nodes++;
                     have = finite ? box_hit<true>(mo, minv, hit.t, node.bmin, node.bmax)
                                   : box_hit<false>(mo, minv, hit.t, node.bmin, node.bmax);
@@ -511,7 +507,7 @@ __device__ __forceinline__ void trace_refill_wide_body(const TraceArgs &a, int *
                         h[1] = make_float4(hit.u, hit.v, xy, 0.0f);
                         if (Count) atomicAdd(a.visits + 3, 1ull);
                     } else {
-                        cur = wide_root(wn, a.sc.macro_root);
+                        cur = WideCur{a.sc.macro_word, a.sc.macro_count};
                         have = true;
                         step = kStepMacro;
                     }
@@ -529,19 +525,15 @@ __device__ __forceinline__ void trace_refill_wide_body(const TraceArgs &a, int *
         // (1) the next instance of a macro leaf
         if (step == kStepInstances) {
             if (leaf_cur < leaf_end) {
-                mi_index = (int)a.sc.mi_indices[leaf_cur++];
-                const rh_mesh_instance mi = a.sc.mesh_instances[mi_index];
+                const InstanceRec mi = load_instance(a.sc, leaf_cur++);
+                mi_index = (int)mi.mi_index;
                 if (cnt) cnt->instances++;
-                if (box_hit<true>(o, inv_d, hit.t, mi.bbox_min, mi.bbox_max)) {
-                    const float *x = a.sc.transforms[mi.tr_index].inv_xform;
-                    co = f3{x[0] * o.x + x[4] * o.y + x[8] * o.z + x[12], x[1] * o.x + x[5] * o.y + x[9] * o.z + x[13],
-                            x[2] * o.x + x[6] * o.y + x[10] * o.z + x[14]};
-                    cd = f3{x[0] * d.x + x[4] * d.y + x[8] * d.z, x[1] * d.x + x[5] * d.y + x[9] * d.z,
-                            x[2] * d.x + x[6] * d.y + x[10] * d.z};
+                if (box_hit<true>(o, inv_d, hit.t, mi.bmin, mi.bmax)) {
+                    transform_ray(mi, o, d, co, cd);
                     cinv = f3{1.0f / cd.x, 1.0f / cd.y, 1.0f / cd.z};
                     finite = __builtin_isfinite(cinv.x) && __builtin_isfinite(cinv.y) && __builtin_isfinite(cinv.z);
                     base = st.sp;
-                    cur = wide_root(wn, a.sc.meshes[mi.mesh_index].node_index);
+                    cur = wide_root(wn, mi.root);
                     have = true;
                     step = kStepMicro;
                 }
