@@ -790,7 +790,7 @@ def main():
     ap.add_argument("--knobs", default="",
                     help="skip the autotuner: lds_stack_depth,sort_rays,sort_bits,path_tail"
                          "[,tail_occupancy[,refill[,trace_occupancy[,shadow_occupancy[,tail_refill"
-                         "[,pass_overlap]]]]]]")
+                         "[,pass_overlap[,trace_group]]]]]]]")
     ap.add_argument("--only-tuned", action="store_true", help="time only the tuned (or given) knobs")
     ap.add_argument("--trees", default="both", choices=("both", "binary", "wide"),
                     help="BVH kind(s) to time: binary, the reference's default 8-wide, or both (the faster is the "

@@ -37,8 +37,9 @@ REFILL = (0, 1)
 TRACE_OCCUPANCY = (0, 5, 6)
 SHADOW_OCCUPANCY = (-1, 0, 5)
 PASS_OVERLAP = (3, 2, 1, 0)  # passes in flight besides the newest
+TRACE_GROUP = (0, 4, 8)      # lanes per ray of the late staged bounces' traces (0: one)
 KNOBS = ("lds_stack_depth", "sort_rays", "sort_bits", "path_tail", "tail_occupancy", "refill", "trace_occupancy",
-         "shadow_occupancy", "tail_refill", "pass_overlap")
+         "shadow_occupancy", "tail_refill", "pass_overlap", "trace_group")
 
 
 @dataclass
@@ -56,7 +57,8 @@ def candidates(depths=LDS_DEPTHS, sort_modes=SORT_MODES, tails=PATH_TAILS, occup
         if pt < 0 and occ != occupancies[0]:
             continue  # no tail launch: occupancy is moot
         yield {"lds_stack_depth": d, "sort_rays": so, "sort_bits": sb, "path_tail": pt, "tail_occupancy": occ,
-               "refill": 0, "trace_occupancy": 0, "shadow_occupancy": -1, "tail_refill": 0, "pass_overlap": PASS_OVERLAP[0]}
+               "refill": 0, "trace_occupancy": 0, "shadow_occupancy": -1, "tail_refill": 0, "pass_overlap": PASS_OVERLAP[0],
+               "trace_group": 0}
 
 
 def apply(renderer, cfg: dict):
@@ -115,8 +117,9 @@ def tune(renderer, scene, region_factory, passes: int = 6, depths=LDS_DEPTHS, so
 
     best = max((trial(knobs_of(r)) for r in sorted(table, key=mr, reverse=True)[:5]), key=mr)
     # the refilled path tail, the refill trace, the capped-register trace and
-    # shadow kernel, then how many passes are in flight, each on the best so
-    # far (all leave the image unchanged)
+    # shadow kernel, lane groups for the late staged bounces, then how many
+    # passes are in flight, each on the best so far (all leave the image
+    # unchanged)
     stages = [
         lambda b: [dict(knobs_of(b), path_tail=pt, tail_occupancy=occ, tail_refill=1)
                    for pt in sorted({p for p in (b["path_tail"], 0, 1, 2) if p >= 0}) for occ in TAIL_OCCUPANCY],
@@ -124,6 +127,7 @@ def tune(renderer, scene, region_factory, passes: int = 6, depths=LDS_DEPTHS, so
         lambda b: [dict(knobs_of(b), trace_occupancy=occ, lds_stack_depth=d)
                    for occ in TRACE_OCCUPANCY[1:] for d in sorted({b["lds_stack_depth"], LDS_DEPTHS[0]})],
         lambda b: [dict(knobs_of(b), shadow_occupancy=so) for so in SHADOW_OCCUPANCY[1:] if so != b["trace_occupancy"]],
+        lambda b: [dict(knobs_of(b), trace_group=g) for g in TRACE_GROUP[1:] if b["path_tail"] < 0 or b["path_tail"] > 2],
         lambda b: [dict(knobs_of(b), pass_overlap=ov) for ov in PASS_OVERLAP if ov != b["pass_overlap"]],
     ]
     for stage in stages:

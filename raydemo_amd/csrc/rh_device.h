@@ -339,6 +339,15 @@ struct Stack {
         lds_d = reinterpret_cast<float *>(lds_base + (size_t)depth * blockDim.x);
         spill_d = reinterpret_cast<float *>(spill_base + (size_t)spill_entries * stride);
     }
+    // the stack of ray `ray` of the `rays` a block walks with lane groups
+    // (every lane of a group uses the same entries)
+    __device__ __forceinline__ void init_ray(uint32_t *lds_base, uint32_t *spill_base, uint32_t stride, uint32_t depth,
+                                             uint32_t spill_entries, uint32_t ray, uint32_t rays) {
+        init(lds_base, spill_base, stride, depth, spill_entries);
+        lane = ray;
+        block = rays;
+        lds_d = reinterpret_cast<float *>(lds_base + (size_t)depth * rays);
+    }
     // entry k (any position below capacity) with its distance
     __device__ __forceinline__ void put(uint32_t k, uint32_t v, float d) {
         if (k < lds_depth) {
@@ -628,31 +637,10 @@ struct WideCur {
 // handling of equal distances included) and the nearest popped; of five or
 // more, the same with the stable insertion sort of sort_topN, done here by
 // writing each entry straight to its sorted place.  false: no slot hit.
-template <bool Finite>
-__device__ __forceinline__ bool wide_step(const float4 *wn, uint32_t node, f3 o, f3 inv_d, float t, Stack &st,
-                                          WideCur &next) {
-    const bool full = !(node & kWideHalf);
-    node &= kWideIndex;
-    const float4 *p = wn + (size_t)node * 16;
-    float dist[8];
-    uint32_t word[8];
-    uint32_t mask = 0;
-    static_for<8>([&](auto S) {
-        constexpr int s = S;
-        // bbox_max without the count word: the count is read below for the
-        // entered slot only, when it is a leaf (8 registers fewer in flight)
-        float4 a = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(RH_MBVH_EMPTY));
-        float3 bm = make_float3(0.0f, 0.0f, 0.0f);
-        if (s < 4 || full) {
-            a = p[2 * s];
-            bm = *reinterpret_cast<const float3 *>(p + 2 * s + 1);
-        }
-        const float4 b = make_float4(bm.x, bm.y, bm.z, 0.0f);
-        word[s] = __float_as_uint(a.w);
-        const bool h = slot_hit<Finite>(o, inv_d, t, a, b, dist[s]);
-        if (h && word[s] != RH_MBVH_EMPTY) mask |= 1u << s;
-    });
-    if (!mask) return false;
+// (wide_order: the ordering part, given the hit slots' distances and words;
+// returns the slot entered)
+__device__ __forceinline__ uint32_t wide_order(const float (&dist)[8], const uint32_t (&word)[8], uint32_t mask,
+                                               uint32_t node, Stack &st) {
     const uint32_t k = __builtin_popcount(mask);
     uint32_t top; // slot entered next
     if (k == 1) {
@@ -738,10 +726,98 @@ __device__ __forceinline__ bool wide_step(const float4 *wn, uint32_t node, f3 o,
         if (st.sp + k - 1 <= st.lds_depth + st.spill_depth) st.sp += k - 1;
         else st.sp = st.lds_depth + st.spill_depth;
     }
+    return top;
+}
+
+template <bool Finite>
+__device__ __forceinline__ bool wide_step(const float4 *wn, uint32_t node, f3 o, f3 inv_d, float t, Stack &st,
+                                          WideCur &next) {
+    const bool full = !(node & kWideHalf);
+    node &= kWideIndex;
+    const float4 *p = wn + (size_t)node * 16;
+    float dist[8];
+    uint32_t word[8];
+    uint32_t mask = 0;
+    static_for<8>([&](auto S) {
+        constexpr int s = S;
+        // bbox_max without the count word: the count is read below for the
+        // entered slot only, when it is a leaf (8 registers fewer in flight)
+        float4 a = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(RH_MBVH_EMPTY));
+        float3 bm = make_float3(0.0f, 0.0f, 0.0f);
+        if (s < 4 || full) {
+            a = p[2 * s];
+            bm = *reinterpret_cast<const float3 *>(p + 2 * s + 1);
+        }
+        const float4 b = make_float4(bm.x, bm.y, bm.z, 0.0f);
+        word[s] = __float_as_uint(a.w);
+        const bool h = slot_hit<Finite>(o, inv_d, t, a, b, dist[s]);
+        if (h && word[s] != RH_MBVH_EMPTY) mask |= 1u << s;
+    });
+    if (!mask) return false;
+    const uint32_t top = wide_order(dist, word, mask, node, st);
     next.word = pick8(word, top);
     next.count = (next.word & RH_LEAF_NODE_BIT) ? __float_as_uint(reinterpret_cast<const float *>(p + 2 * top + 1)[3])
                                                 : 0u;
     return true;
+}
+
+// wide_step for a ray walked by G lanes (G = 4 or 8, aligned lane groups of
+// one wave) that hold identical copies of the ray's state: lane j of the
+// group loads and tests slots j, j + G, ... (the group's loads of a node's
+// first 128-byte line are one cache line), the hit mask comes from ballots
+// and the distances and words from the group's lanes, and every lane then
+// orders and stacks the same entries as wide_step, so that the group's
+// copies stay identical (its lanes write the same stack entries).  All G
+// lanes must be active.
+template <bool Finite, int G>
+__device__ __forceinline__ bool wide_step_group(const float4 *wn, uint32_t node, f3 o, f3 inv_d, float t, Stack &st,
+                                                WideCur &next) {
+    static_assert(G == 4 || G == 8, "groups of 4 or 8 lanes");
+    constexpr int S = 8 / G; // slots per lane
+    const uint32_t lane = __lane_id(), gb = lane & ~(uint32_t)(G - 1), j = lane & (G - 1);
+    const bool full = !(node & kWideHalf);
+    node &= kWideIndex;
+    const float4 *p = wn + (size_t)node * 16;
+    float dl[S];
+    uint32_t wl[S], cl[S];
+    uint32_t mask = 0;
+    static_for<S>([&](auto I) {
+        constexpr int i = I;
+        const uint32_t s = j + i * G;
+        float4 a = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(RH_MBVH_EMPTY));
+        float4 b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (s < 4 || full) {
+            a = p[2 * s];
+            b = p[2 * s + 1];
+        }
+        wl[i] = __float_as_uint(a.w);
+        cl[i] = __float_as_uint(b.w);
+        const bool h = slot_hit<Finite>(o, inv_d, t, a, b, dl[i]) && wl[i] != RH_MBVH_EMPTY;
+        mask |= (uint32_t)((__ballot(h) >> gb) & ((1ull << G) - 1)) << (i * G);
+    });
+    if (!mask) return false;
+    float dist[8];
+    uint32_t word[8];
+    static_for<8>([&](auto SS) {
+        constexpr int s = SS;
+        dist[s] = __shfl(dl[s / G], (int)(gb + s % G));
+        word[s] = __shfl(wl[s / G], (int)(gb + s % G));
+    });
+    const uint32_t top = wide_order(dist, word, mask, node, st);
+    next.word = pick8(word, top);
+    uint32_t c = cl[0];
+    if constexpr (S == 2) c = top >= (uint32_t)G ? cl[1] : cl[0];
+    c = __shfl(c, (int)(gb + (top & (G - 1))));
+    next.count = (next.word & RH_LEAF_NODE_BIT) ? c : 0u;
+    return true;
+}
+
+// wide_step of one lane (G = 1) or of a lane group
+template <bool Finite, int G>
+__device__ __forceinline__ bool wide_step_g(const float4 *wn, uint32_t node, f3 o, f3 inv_d, float t, Stack &st,
+                                            WideCur &next) {
+    if constexpr (G == 1) return wide_step<Finite>(wn, node, o, inv_d, t, st, next);
+    else return wide_step_group<Finite, G>(wn, node, o, inv_d, t, st, next);
 }
 
 // Pops until an entry nearer than t: false when the stack is back at base.
@@ -778,7 +854,7 @@ __device__ __forceinline__ WideCur wide_root(const float4 *wn, uint32_t root_wor
 // first_hit the walk ends at the first solid triangle it hits instead: for a
 // scene whose triangles are all solid, whether a segment is blocked does not
 // depend on which blocking triangle is found (visibility queries only).
-template <bool Finite>
+template <bool Finite, int G = 1>
 __device__ __forceinline__ bool walk_micro_wide(const Scene &sc, uint32_t root, f3 mo, f3 md, f3 minv, int mi_index,
                                                 Hit &hit, Stack &st, TraceCounters *cnt, bool first_hit) {
     const uint32_t base = st.sp;
@@ -802,7 +878,7 @@ __device__ __forceinline__ bool walk_micro_wide(const Scene &sc, uint32_t root, 
             res |= hit.t != t_before || hit.prim != before;
         } else {
             if (cnt) cnt->nodes++;
-            if (wide_step<Finite>(sc.wnodes, cur.word, mo, minv, hit.t, st, cur)) continue;
+            if (wide_step_g<Finite, G>(sc.wnodes, cur.word, mo, minv, hit.t, st, cur)) continue;
         }
         if (!wide_pop(sc.wnodes, base, hit.t, st, cur)) return res;
     }
@@ -810,8 +886,9 @@ __device__ __forceinline__ bool walk_micro_wide(const Scene &sc, uint32_t root, 
 
 // Traverse_MacroTree_WithStack_{ClosestHit,AnyHit}, 8-wide (CoreRef.cpp:
 // 1133-1222, 1270-1368): an any-hit walk ends at the first instance whose
-// closest hit is on a solid triangle.
-template <bool AnyHit>
+// closest hit is on a solid triangle.  G > 1: the ray is walked by a lane
+// group (wide_step_group), every other step done alike by all its lanes.
+template <bool AnyHit, int G = 1>
 __device__ __forceinline__ void traverse_wide(const Scene &sc, f3 o, f3 d, Hit &hit, Stack &st, TraceCounters *cnt,
                                               bool first_hit = false) {
     const f3 inv_d = safe_invert(d);
@@ -833,8 +910,8 @@ __device__ __forceinline__ void traverse_wide(const Scene &sc, f3 o, f3 d, Hit &
                     __builtin_isfinite(minv.x) && __builtin_isfinite(minv.y) && __builtin_isfinite(minv.z);
                 const bool fh = AnyHit && first_hit;
                 const bool found =
-                    finite ? walk_micro_wide<true>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt, fh)
-                           : walk_micro_wide<false>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt, fh);
+                    finite ? walk_micro_wide<true, G>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt, fh)
+                           : walk_micro_wide<false, G>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt, fh);
                 if (AnyHit && found) {
                     const rh_tri_accel ht = load_tri(sc.tris, (uint32_t)hit.prim);
                     if (ht.ci & RH_TRI_SOLID_BIT) {
@@ -845,7 +922,7 @@ __device__ __forceinline__ void traverse_wide(const Scene &sc, f3 o, f3 d, Hit &
             }
         } else {
             if (cnt) cnt->nodes++;
-            if (wide_step<true>(sc.wnodes, cur.word, o, inv_d, hit.t, st, cur)) continue;
+            if (wide_step_g<true, G>(sc.wnodes, cur.word, o, inv_d, hit.t, st, cur)) continue;
         }
         if (!wide_pop(sc.wnodes, 0, hit.t, st, cur)) return;
     }

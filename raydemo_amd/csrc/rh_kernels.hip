@@ -302,6 +302,13 @@ __global__ void __launch_bounds__(kBlock, MinBlocks) k_trace(TraceArgs a) {
     trace_body<AnyHit, Count, Tree>(a, blockIdx.x * blockDim.x, lds_stack);
 }
 
+// 8-wide trees, G lanes per ray (trace_group_body)
+template <bool AnyHit, bool Count, int G, int MinBlocks>
+__global__ void __launch_bounds__(kBlock, MinBlocks) k_trace_group(TraceArgs a) {
+    extern __shared__ uint32_t lds_stack[];
+    trace_group_body<AnyHit, Count, G>(a, lds_stack);
+}
+
 // persistent closest-hit trace with lane refill (trace_refill_body)
 template <bool Count> __global__ void __launch_bounds__(kBlock) k_trace_refill(TraceArgs a, int *next_ray) {
     extern __shared__ uint32_t lds_stack[];
@@ -824,6 +831,7 @@ struct rh_context {
     std::vector<std::pair<std::pair<const void *, size_t>, int>> resident; // (kernel, LDS bytes) -> blocks per CU
     int trace_occupancy = 0;   // 8-wide trace/shadow kernels: 0 = as compiled (4 waves/SIMD), 5 or 6 = capped registers
     int shadow_occupancy = -1; // the shadow kernel's own setting (-1: as trace_occupancy)
+    int trace_group = 0;       // 8-wide trees: lanes per ray (k_trace_group) of rh_trace and late staged bounces; 0 = one
     int max_bounces = RH_MAX_BOUNCES; // max_total_depth clamp (the reference's 10; up to RH_MAX_BOUNCES_EXT)
     int adjoint_staged = 1;           // adjoint replay as staged wavefront passes (0: one lane per path)
     uint32_t *geo_winner = nullptr;   // geometry camera: winning triangle per lane
@@ -1096,7 +1104,14 @@ int resident_blocks(rh_context *c, const void *kern, size_t lds, int *out) {
     return 0;
 }
 
-int launch_trace(rh_context *c, const TraceArgs &a, int grid_rays, bool any_hit) {
+// Staged bounces from this trace stage on (bounce 2) may be walked by lane
+// groups (option trace_group): their queues hold few rays, which one lane per
+// ray leaves too few to keep the device busy.
+constexpr int kGroupFromStage = 3;
+
+// stage: the pass's trace stage (0 primary rays, b + 1 bounce b), or -1 for a
+// trace of the caller's rays (rh_trace)
+int launch_trace(rh_context *c, const TraceArgs &a, int grid_rays, bool any_hit, int stage = -1) {
     const int blocks = std::max(1, (grid_rays + kBlock - 1) / kBlock);
     const bool count = c->count_visits && a.visits;
     if (!any_hit && c->refill) {
@@ -1139,7 +1154,26 @@ int launch_trace(rh_context *c, const TraceArgs &a, int grid_rays, bool any_hit)
     KernelMark km(c, 0);
     const dim3 grid(blocks), blk(kBlock);
     const size_t lds = lds_bytes(c);
-    if (c->sc.wnodes && c->trace_occupancy == 5) {
+    if (c->sc.wnodes && c->trace_group && (stage < 0 || stage >= kGroupFromStage)) {
+        // G lanes per ray: G times the threads, one LDS stack per ray
+        const int g = c->trace_group;
+        const dim3 gg((unsigned)std::max<size_t>(1, ((size_t)grid_rays * g + kBlock - 1) / kBlock));
+        const size_t glds = (size_t)c->lds_depth * (kBlock / g) * sizeof(uint32_t) * 2;
+#define RH_TRACE_GROUP(G)                                                                                              \
+    if (any_hit) {                                                                                                     \
+        if (count) hipLaunchKernelGGL((k_trace_group<true, true, G, 1>), gg, blk, glds, c->stream, a);                \
+        else hipLaunchKernelGGL((k_trace_group<true, false, G, 1>), gg, blk, glds, c->stream, a);                     \
+    } else {                                                                                                           \
+        if (count) hipLaunchKernelGGL((k_trace_group<false, true, G, 1>), gg, blk, glds, c->stream, a);               \
+        else hipLaunchKernelGGL((k_trace_group<false, false, G, 1>), gg, blk, glds, c->stream, a);                    \
+    }
+        if (g == 4) {
+            RH_TRACE_GROUP(4)
+        } else {
+            RH_TRACE_GROUP(8)
+        }
+#undef RH_TRACE_GROUP
+    } else if (c->sc.wnodes && c->trace_occupancy == 5) {
         if (any_hit) {
             if (count) hipLaunchKernelGGL((k_trace<true, true, kTreeWide, 5>), grid, blk, lds, c->stream, a);
             else hipLaunchKernelGGL((k_trace<true, false, kTreeWide, 5>), grid, blk, lds, c->stream, a);
@@ -1702,7 +1736,7 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool k
         TraceArgs ta = trace_args(c, c->rays[0], cnt[0], n, c->hits, nullptr, count_primary);
         ta.visits = c->visits + kStageVisits; // stage 0
         c->mark_stage = 0;
-        rc = launch_trace(c, ta, n, false);
+        rc = launch_trace(c, ta, n, false, 0);
         c->mark_stage = -1;
         if (rc) return rc;
         tm.mark(kPrimaryTrace);
@@ -1806,7 +1840,7 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool k
         TraceArgs ta = trace_args(c, c->rays[q], cnt[q], n, c->hits, nullptr, count_secondary);
         ta.visits = c->visits + kStageVisits + 4 * (bounce + 1); // stage bounce + 1
         c->mark_stage = bounce + 1;
-        rc = launch_trace(c, ta, n, false);
+        rc = launch_trace(c, ta, n, false, bounce + 1);
         c->mark_stage = -1;
         if (rc) return rc;
         tm.mark(kSecondaryTrace);
@@ -2048,6 +2082,9 @@ int rh_set_option(rh_context *c, const char *name, int value) {
     } else if (n == "trace_occupancy") {
         if (value != 0 && value != 5 && value != 6) return arg_error("trace_occupancy must be 0, 5 or 6");
         c->trace_occupancy = value;
+    } else if (n == "trace_group") {
+        if (value != 0 && value != 4 && value != 8) return arg_error("trace_group must be 0, 4 or 8");
+        c->trace_group = value;
     } else if (n == "adjoint_occupancy") {
         if (value != 2 && value != 3) return arg_error("adjoint_occupancy must be 2 or 3");
         c->adjoint_occupancy = value;
@@ -2072,6 +2109,7 @@ int rh_get_option(rh_context *c, const char *name, int *value) {
     else if (n == "scene_uploads") *value = c->n_uploads;
     else if (n == "instance_updates") *value = c->n_instance_updates;
     else if (n == "trace_occupancy") *value = c->trace_occupancy;
+    else if (n == "trace_group") *value = c->trace_group;
     else if (n == "max_bounces") *value = c->max_bounces;
     else if (n == "adjoint_staged") *value = c->adjoint_staged;
     else if (n == "adjoint_tape") *value = c->tape_passes;

@@ -286,6 +286,45 @@ __device__ __forceinline__ void trace_body(const TraceArgs &a, uint32_t base, ui
     h[1] = make_float4(hit.u, hit.v, r0.w, 0.0f);
 }
 
+// trace_body for 8-wide trees with G lanes per ray (traverse_wide's group
+// form): thread t walks ray t / G together with the other lanes of its
+// group, which hold identical copies of the ray's state; the group's first
+// lane writes the hit and the counts.  The block's LDS holds one stack per
+// ray (blockDim.x / G of them).
+template <bool AnyHit, bool Count, int G>
+__device__ __forceinline__ void trace_group_body(const TraceArgs &a, uint32_t *lds_stack) {
+    const int n = a.count ? *a.count : a.n;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x, i = t / G;
+    if (t == 0 && a.ray_counter) atomicAdd(a.ray_counter, (unsigned long long)n);
+    if ((int)i >= n) return; // the whole group
+
+    const float4 *p = reinterpret_cast<const float4 *>(a.rays + i);
+    const float4 r0 = p[0], r1 = p[1];
+    Hit hit;
+    hit.mask = 0;
+    hit.obj = -1;
+    hit.prim = -1;
+    hit.t = a.tmax ? a.tmax[i] : RH_MAX_DIST;
+    hit.u = hit.v = 0.0f;
+
+    Stack st;
+    st.init_ray(lds_stack, a.spill + i, a.spill_stride, a.lds_depth, kSpillDepth, threadIdx.x / G, blockDim.x / G);
+    TraceCounters tc = {0, 0, 0};
+    if (a.sc.macro_root != RH_INVALID)
+        traverse_wide<AnyHit, G>(a.sc, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, hit, st, Count ? &tc : nullptr);
+    if (threadIdx.x & (G - 1)) return;
+    if (st.overflow && a.overflow) atomicAdd(a.overflow, (unsigned long long)st.overflow);
+    if (Count) {
+        atomicAdd(a.visits + 0, (unsigned long long)tc.nodes);
+        atomicAdd(a.visits + 1, (unsigned long long)tc.leaf_tris);
+        atomicAdd(a.visits + 2, (unsigned long long)tc.instances);
+        atomicAdd(a.visits + 3, 1ull);
+    }
+    float4 *h = reinterpret_cast<float4 *>(a.hits + i);
+    h[0] = make_float4(__int_as_float(hit.mask), __int_as_float(hit.obj), __int_as_float(hit.prim), hit.t);
+    h[1] = make_float4(hit.u, hit.v, r0.w, 0.0f);
+}
+
 // Closest-hit trace with lane refill: a persistent grid walks the queue and a
 // lane whose ray is finished takes the next one while the rest of its wave is
 // still traversing, so the wave does not idle until its slowest ray ends.

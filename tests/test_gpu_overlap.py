@@ -58,6 +58,19 @@ def test_overlapped_passes_equal_serial_passes(host_lib, wide, knobs, overlap):
     assert rc0 == rc1
 
 
+@pytest.mark.parametrize("group", [4, 8])
+def test_lane_group_trace_equals_lane_trace(host_lib, group):
+    """The staged traces of bounce 2 on walked by groups of 4 or 8 lanes per
+    ray (trace_group): the image and ray counts of one lane per ray, bit for
+    bit, with passes in flight and with a shallow LDS stack whose entries
+    spill."""
+    for knobs in (STAGED_SORT, dict(TAIL, path_tail=5, lds_stack_depth=4)):
+        img0, rc0, _, _ = run(host_lib, 3, True, knobs)
+        img1, rc1, _, _ = run(host_lib, 3, True, dict(knobs, trace_group=group))
+        assert np.array_equal(img0.view(np.uint32), img1.view(np.uint32))
+        assert rc0 == rc1
+
+
 def test_calls_between_overlapped_passes(host_lib):
     """Every entry point a caller may use between passes: the pass after it
     waits for it, and the image is the serial one."""
