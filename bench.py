@@ -501,13 +501,16 @@ def run_large(args, lib, A, scenes, knobs, wide):
     r.synchronize()
     # a short sweep of the knobs that matter at this depth and size (2 passes each)
     sweep = []
-    for cfg in ({"path_tail": -1, "lds_stack_depth": 16, "trace_occupancy": 0},
-                {"path_tail": -1, "lds_stack_depth": 8, "trace_occupancy": 5},
-                {"path_tail": -1, "lds_stack_depth": 16, "trace_occupancy": 5},
-                {"path_tail": -1, "lds_stack_depth": 24, "trace_occupancy": 0},
-                {"path_tail": 8, "lds_stack_depth": 16, "trace_occupancy": 0, "tail_occupancy": 3},
+    for cfg in ({"path_tail": -1, "lds_stack_depth": 16, "trace_occupancy": 0, "trace_group": 0},
+                {"path_tail": -1, "lds_stack_depth": 8, "trace_occupancy": 5, "trace_group": 0},
+                {"path_tail": -1, "lds_stack_depth": 16, "trace_occupancy": 5, "trace_group": 0},
+                {"path_tail": -1, "lds_stack_depth": 24, "trace_occupancy": 0, "trace_group": 0},
+                {"path_tail": -1, "lds_stack_depth": 16, "trace_occupancy": 0, "trace_group": 4},
+                {"path_tail": -1, "lds_stack_depth": 16, "trace_occupancy": 0, "trace_group": 8},
+                {"path_tail": 8, "lds_stack_depth": 16, "trace_occupancy": 0, "tail_occupancy": 3, "trace_group": 0},
                 {"path_tail": knobs.get("path_tail", 2), "lds_stack_depth": knobs.get("lds_stack_depth", 8),
-                 "trace_occupancy": knobs.get("trace_occupancy", 0), "tail_occupancy": knobs.get("tail_occupancy", 2)}):
+                 "trace_occupancy": knobs.get("trace_occupancy", 0), "tail_occupancy": knobs.get("tail_occupancy", 2),
+                 "trace_group": knobs.get("trace_group", 0)}):
         for k, v in cfg.items():
             r.set_option(k, v)
         r.reset_stats()
