@@ -91,7 +91,7 @@ int rh_get_ray_counts(rh_context *ctx, rh_ray_counts *out);
  * "trace_occupancy" (0, 5 or 6: the 8-wide trace and shadow kernels as compiled, or capped at 5 or 6 waves
  *   per SIMD),
  * "shadow_occupancy" (-1, the default: as trace_occupancy; 0, 5 or 6: the 8-wide shadow kernel's own setting),
- * "trace_group" (0, the default, or 4 or 8: 8-wide closest-hit and any-hit traces of rh_trace, and the staged
+ * "trace_group" (0, the default, or 2, 4 or 8: 8-wide closest-hit and any-hit traces of rh_trace, and the staged
  *   traces of bounce 2 and later, walk each ray with a group of that many lanes, each testing 8 / group of a
  *   node's slots; faster for the few rays of late bounces, slower for full queues; results are the same),
  * "max_bounces" (1..16: the clamp of max_total_depth; 10, the reference's MAX_BOUNCES, by default:

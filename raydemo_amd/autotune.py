@@ -37,7 +37,7 @@ REFILL = (0, 1)
 TRACE_OCCUPANCY = (0, 5, 6)
 SHADOW_OCCUPANCY = (-1, 0, 5)
 PASS_OVERLAP = (3, 2, 1, 0)  # passes in flight besides the newest
-TRACE_GROUP = (0, 4, 8)      # lanes per ray of the late staged bounces' traces (0: one)
+TRACE_GROUP = (0, 2, 4, 8)   # lanes per ray of the late staged bounces' traces (0: one)
 KNOBS = ("lds_stack_depth", "sort_rays", "sort_bits", "path_tail", "tail_occupancy", "refill", "trace_occupancy",
          "shadow_occupancy", "tail_refill", "pass_overlap", "trace_group")
 

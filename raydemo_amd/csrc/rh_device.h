@@ -761,7 +761,7 @@ __device__ __forceinline__ bool wide_step(const float4 *wn, uint32_t node, f3 o,
     return true;
 }
 
-// wide_step for a ray walked by G lanes (G = 4 or 8, aligned lane groups of
+// wide_step for a ray walked by G lanes (G = 2, 4 or 8, aligned lane groups of
 // one wave) that hold identical copies of the ray's state: lane j of the
 // group loads and tests slots j, j + G, ... (the group's loads of a node's
 // first 128-byte line are one cache line), the hit mask comes from ballots
@@ -772,7 +772,7 @@ __device__ __forceinline__ bool wide_step(const float4 *wn, uint32_t node, f3 o,
 template <bool Finite, int G>
 __device__ __forceinline__ bool wide_step_group(const float4 *wn, uint32_t node, f3 o, f3 inv_d, float t, Stack &st,
                                                 WideCur &next) {
-    static_assert(G == 4 || G == 8, "groups of 4 or 8 lanes");
+    static_assert(G == 2 || G == 4 || G == 8, "groups of 2, 4 or 8 lanes");
     constexpr int S = 8 / G; // slots per lane
     const uint32_t lane = __lane_id(), gb = lane & ~(uint32_t)(G - 1), j = lane & (G - 1);
     const bool full = !(node & kWideHalf);
@@ -806,7 +806,9 @@ __device__ __forceinline__ bool wide_step_group(const float4 *wn, uint32_t node,
     const uint32_t top = wide_order(dist, word, mask, node, st);
     next.word = pick8(word, top);
     uint32_t c = cl[0];
-    if constexpr (S == 2) c = top >= (uint32_t)G ? cl[1] : cl[0];
+    static_for<S>([&](auto I) {
+        if (top / G == (uint32_t)I) c = cl[I];
+    });
     c = __shfl(c, (int)(gb + (top & (G - 1))));
     next.count = (next.word & RH_LEAF_NODE_BIT) ? c : 0u;
     return true;

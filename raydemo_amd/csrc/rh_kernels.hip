@@ -302,7 +302,7 @@ __global__ void __launch_bounds__(kBlock, MinBlocks) k_trace(TraceArgs a) {
     trace_body<AnyHit, Count, Tree>(a, blockIdx.x * blockDim.x, lds_stack);
 }
 
-// 8-wide trees, G lanes per ray (trace_group_body)
+// 8-wide trees, G = 2, 4 or 8 lanes per ray (trace_group_body)
 template <bool AnyHit, bool Count, int G, int MinBlocks>
 __global__ void __launch_bounds__(kBlock, MinBlocks) k_trace_group(TraceArgs a) {
     extern __shared__ uint32_t lds_stack[];
@@ -1167,7 +1167,9 @@ int launch_trace(rh_context *c, const TraceArgs &a, int grid_rays, bool any_hit,
         if (count) hipLaunchKernelGGL((k_trace_group<false, true, G, 1>), gg, blk, glds, c->stream, a);               \
         else hipLaunchKernelGGL((k_trace_group<false, false, G, 1>), gg, blk, glds, c->stream, a);                    \
     }
-        if (g == 4) {
+        if (g == 2) {
+            RH_TRACE_GROUP(2)
+        } else if (g == 4) {
             RH_TRACE_GROUP(4)
         } else {
             RH_TRACE_GROUP(8)
@@ -2083,7 +2085,7 @@ int rh_set_option(rh_context *c, const char *name, int value) {
         if (value != 0 && value != 5 && value != 6) return arg_error("trace_occupancy must be 0, 5 or 6");
         c->trace_occupancy = value;
     } else if (n == "trace_group") {
-        if (value != 0 && value != 4 && value != 8) return arg_error("trace_group must be 0, 4 or 8");
+        if (value != 0 && value != 2 && value != 4 && value != 8) return arg_error("trace_group must be 0, 2, 4 or 8");
         c->trace_group = value;
     } else if (n == "adjoint_occupancy") {
         if (value != 2 && value != 3) return arg_error("adjoint_occupancy must be 2 or 3");

@@ -58,9 +58,9 @@ def test_overlapped_passes_equal_serial_passes(host_lib, wide, knobs, overlap):
     assert rc0 == rc1
 
 
-@pytest.mark.parametrize("group", [4, 8])
+@pytest.mark.parametrize("group", [2, 4, 8])
 def test_lane_group_trace_equals_lane_trace(host_lib, group):
-    """The staged traces of bounce 2 on walked by groups of 4 or 8 lanes per
+    """The staged traces of bounce 2 on walked by groups of 2, 4 or 8 lanes per
     ray (trace_group): the image and ray counts of one lane per ray, bit for
     bit, with passes in flight and with a shallow LDS stack whose entries
     spill."""

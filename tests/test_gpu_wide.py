@@ -26,15 +26,15 @@ BUILDS = {
 }
 
 
-@pytest.mark.parametrize("knobs", [{}, {"refill": 1}, {"refill": 1, "trace_occupancy": 5}, {"trace_group": 4},
-                                   {"trace_group": 8, "lds_stack_depth": 4}],
-                         ids=["plain", "refill", "refill_occ5", "group4", "group8_lds4"])
+@pytest.mark.parametrize("knobs", [{}, {"refill": 1}, {"refill": 1, "trace_occupancy": 5}, {"trace_group": 2},
+                                   {"trace_group": 4}, {"trace_group": 8, "lds_stack_depth": 4}],
+                         ids=["plain", "refill", "refill_occ5", "group2", "group4", "group8_lds4"])
 @pytest.mark.parametrize("name", sorted(BUILDS))
 def test_wide_trace_bit_exact_and_visits(ref_lib, host_lib, dev_lib, oracle, golden_scene1, name, knobs):
     """Closest-hit and any-hit traces equal the reference library's, and the
     visit counters the oracle's; with refill the closest-hit walk runs as the
     persistent lane-refill kernel (the instrumented and the plain one); with
-    trace_group each ray is walked by a group of 4 or 8 lanes."""
+    trace_group each ray is walked by a group of 2, 4 or 8 lanes."""
     rs, hs = standalone_scene(ref_lib, wide=True), standalone_scene(host_lib, wide=True)
     BUILDS[name](rs, golden_scene1)
     BUILDS[name](hs, golden_scene1)
