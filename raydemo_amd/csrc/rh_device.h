@@ -565,6 +565,14 @@ constexpr uint32_t kWideSlotRef = RH_LEAF_NODE_BIT;
 constexpr uint32_t kWideLeafRoot = 0x40000000u;
 constexpr uint32_t kWideHalf = 0x20000000u;
 constexpr uint32_t kWideIndex = kWideHalf - 1u; // node indices are below 2^28
+// A leaf child's slot word may carry its triangle count (1..15) in bits 27-30
+// when its first index is below 2^27 (0 there: the count is only in the
+// slot's count word), so entering it needs no further load.
+constexpr uint32_t kLeafCountShift = 27, kLeafCountBits = 15u << kLeafCountShift;
+// the walk's form of a leaf child's slot word: prim index | leaf bit
+__device__ __forceinline__ uint32_t leaf_word(uint32_t w) {
+    return (w & kLeafCountBits) ? (w & ~kLeafCountBits) : w;
+}
 
 // slab test of one slot with its entry distance (bbox_test_oct, CoreRef.cpp:133-161)
 template <bool Finite>
@@ -755,9 +763,14 @@ __device__ __forceinline__ bool wide_step(const float4 *wn, uint32_t node, f3 o,
     });
     if (!mask) return false;
     const uint32_t top = wide_order(dist, word, mask, node, st);
-    next.word = pick8(word, top);
-    next.count = (next.word & RH_LEAF_NODE_BIT) ? __float_as_uint(reinterpret_cast<const float *>(p + 2 * top + 1)[3])
-                                                : 0u;
+    const uint32_t w = pick8(word, top);
+    next.word = w;
+    next.count = 0;
+    if (w & RH_LEAF_NODE_BIT) {
+        const uint32_t c = (w & kLeafCountBits) >> kLeafCountShift;
+        next.word = leaf_word(w);
+        next.count = c ? c : __float_as_uint(reinterpret_cast<const float *>(p + 2 * top + 1)[3]);
+    }
     return true;
 }
 
@@ -805,6 +818,7 @@ __device__ __forceinline__ bool wide_step_group(const float4 *wn, uint32_t node,
     });
     const uint32_t top = wide_order(dist, word, mask, node, st);
     next.word = pick8(word, top);
+    if (next.word & RH_LEAF_NODE_BIT) next.word = leaf_word(next.word);
     uint32_t c = cl[0];
     static_for<S>([&](auto I) {
         if (top / G == (uint32_t)I) c = cl[I];
@@ -831,7 +845,7 @@ __device__ __forceinline__ bool wide_pop(const float4 *wn, uint32_t base, float 
         if (d > t) continue;
         if (w & kWideSlotRef) {
             const float *slot = reinterpret_cast<const float *>(wn + (size_t)(w & RH_PRIM_INDEX_BITS) * 2);
-            next.word = __float_as_uint(slot[3]);
+            next.word = leaf_word(__float_as_uint(slot[3]));
             next.count = __float_as_uint(slot[7]);
         } else {
             next.word = w;

@@ -171,6 +171,9 @@ inline int build_wide(const rh_scene_view *v, hipStream_t stream, std::vector<rh
                 if (is_leaf(c)) {
                     word = v->mnodes[c].child[0];
                     count = v->mnodes[c].child[1];
+                    // small counts travel in the word too (kLeafCountBits)
+                    if (count >= 1 && count <= 15 && (word & RH_PRIM_INDEX_BITS) < (1u << kLeafCountShift))
+                        word |= count << kLeafCountShift;
                 } else {
                     word = dev[c] | half_flag(c);
                 }
