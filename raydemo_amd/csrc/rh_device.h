@@ -122,6 +122,9 @@ struct Scene {
     uint32_t light_root;
     // 8-wide trees (see the wide traversal below); null for binary scenes
     const float4 *wnodes;
+    // leaf children's words carry small prim counts (kLeafCountBits) and are
+    // stacked as they are (a device tree of fewer than 2^24 nodes)
+    uint32_t wide_leaf_counts;
     // the wide light tree is a single leaf: its lights are taken for every
     // point, with no bounds test (the wide walk tests only child slots)
     uint32_t light_untested;
@@ -567,7 +570,10 @@ constexpr uint32_t kWideHalf = 0x20000000u;
 constexpr uint32_t kWideIndex = kWideHalf - 1u; // node indices are below 2^28
 // A leaf child's slot word may carry its triangle count (1..15) in bits 27-30
 // when its first index is below 2^27 (0 there: the count is only in the
-// slot's count word), so entering it needs no further load.
+// slot's count word), so entering it needs no further load, and such a word
+// is stacked as it is (a stacked slot reference, node x 8 + slot below 2^27,
+// has those bits clear).  Only trees of fewer than 2^24 device nodes do this
+// (Scene::wide_leaf_counts).
 constexpr uint32_t kLeafCountShift = 27, kLeafCountBits = 15u << kLeafCountShift;
 // the walk's form of a leaf child's slot word: prim index | leaf bit
 __device__ __forceinline__ uint32_t leaf_word(uint32_t w) {
@@ -708,7 +714,8 @@ __device__ __forceinline__ uint32_t wide_order(const float (&dist)[8], const uin
             constexpr uint32_t c = C;
             if (c + 1 < k) {
                 const uint32_t w = pick8(word, e[c]);
-                st.push2((w & RH_LEAF_NODE_BIT) ? (kWideSlotRef | (node * 8 + e[c])) : w, d[c]);
+                st.push2(((w & RH_LEAF_NODE_BIT) && !(w & kLeafCountBits)) ? (kWideSlotRef | (node * 8 + e[c])) : w,
+                         d[c]);
             }
         });
         top = k == 2 ? e[1] : (k == 3 ? e[2] : e[3]);
@@ -728,7 +735,10 @@ __device__ __forceinline__ uint32_t wide_order(const float (&dist)[8], const uin
             if (pos == k - 1) {
                 top = s;
             } else {
-                st.put(st.sp + pos, (word[s] & RH_LEAF_NODE_BIT) ? (kWideSlotRef | (node * 8 + s)) : word[s], dist[s]);
+                st.put(st.sp + pos,
+                       ((word[s] & RH_LEAF_NODE_BIT) && !(word[s] & kLeafCountBits)) ? (kWideSlotRef | (node * 8 + s))
+                                                                                   : word[s],
+                       dist[s]);
             }
         });
         if (st.sp + k - 1 <= st.lds_depth + st.spill_depth) st.sp += k - 1;
@@ -837,13 +847,18 @@ __device__ __forceinline__ bool wide_step_g(const float4 *wn, uint32_t node, f3 
 }
 
 // Pops until an entry nearer than t: false when the stack is back at base.
-__device__ __forceinline__ bool wide_pop(const float4 *wn, uint32_t base, float t, Stack &st, WideCur &next) {
+__device__ __forceinline__ bool wide_pop(const float4 *wn, uint32_t base, float t, Stack &st, WideCur &next,
+                                         uint32_t leaf_counts) {
     while (st.sp > base) {
         uint32_t w;
         float d;
         st.pop2(w, d);
         if (d > t) continue;
-        if (w & kWideSlotRef) {
+        if (leaf_counts && (w & RH_LEAF_NODE_BIT) && (w & kLeafCountBits)) {
+            // a leaf word with its count: no load
+            next.word = w & ~kLeafCountBits;
+            next.count = (w & kLeafCountBits) >> kLeafCountShift;
+        } else if (w & kWideSlotRef) {
             const float *slot = reinterpret_cast<const float *>(wn + (size_t)(w & RH_PRIM_INDEX_BITS) * 2);
             next.word = leaf_word(__float_as_uint(slot[3]));
             next.count = __float_as_uint(slot[7]);
@@ -896,7 +911,7 @@ __device__ __forceinline__ bool walk_micro_wide(const Scene &sc, uint32_t root, 
             if (cnt) cnt->nodes++;
             if (wide_step_g<Finite, G>(sc.wnodes, cur.word, mo, minv, hit.t, st, cur)) continue;
         }
-        if (!wide_pop(sc.wnodes, base, hit.t, st, cur)) return res;
+        if (!wide_pop(sc.wnodes, base, hit.t, st, cur, sc.wide_leaf_counts)) return res;
     }
 }
 
@@ -940,7 +955,7 @@ __device__ __forceinline__ void traverse_wide(const Scene &sc, f3 o, f3 d, Hit &
             if (cnt) cnt->nodes++;
             if (wide_step_g<true, G>(sc.wnodes, cur.word, o, inv_d, hit.t, st, cur)) continue;
         }
-        if (!wide_pop(sc.wnodes, 0, hit.t, st, cur)) return;
+        if (!wide_pop(sc.wnodes, 0, hit.t, st, cur, sc.wide_leaf_counts)) return;
     }
 }
 

@@ -146,6 +146,10 @@ inline int build_wide(const rh_scene_view *v, hipStream_t stream, std::vector<rh
         for (int k = 0; k < 8; k++) used += v->mnodes[i].child[k] != RH_MBVH_EMPTY;
         return used <= 4 ? kWideHalf : 0u;
     };
+    // leaf counts in the words (Scene::wide_leaf_counts) while every stacked
+    // slot reference (node x 8 + slot) stays below 2^27
+    const bool fold_counts = m < (1u << 24);
+    s.wide_leaf_counts = fold_counts ? 1u : 0u;
     std::vector<float4> w((size_t)m * 16);
     const float nan = std::numeric_limits<float>::quiet_NaN();
     auto as_float = [](uint32_t u) {
@@ -172,7 +176,8 @@ inline int build_wide(const rh_scene_view *v, hipStream_t stream, std::vector<rh
                     word = v->mnodes[c].child[0];
                     count = v->mnodes[c].child[1];
                     // small counts travel in the word too (kLeafCountBits)
-                    if (count >= 1 && count <= 15 && (word & RH_PRIM_INDEX_BITS) < (1u << kLeafCountShift))
+                    if (fold_counts && count >= 1 && count <= 15 &&
+                        (word & RH_PRIM_INDEX_BITS) < (1u << kLeafCountShift))
                         word |= count << kLeafCountShift;
                 } else {
                     word = dev[c] | half_flag(c);

@@ -583,7 +583,7 @@ __device__ __forceinline__ void trace_refill_wide_body(const TraceArgs &a, int *
         }
         // (2) pop
         if ((step == kStepMacro || step == kStepMicro) && !have) {
-            have = wide_pop(wn, base, hit.t, st, cur);
+            have = wide_pop(wn, base, hit.t, st, cur, a.sc.wide_leaf_counts);
             if (!have) {
                 if (step == kStepMicro) {
                     // the instance is done: back to the macro leaf's next one
