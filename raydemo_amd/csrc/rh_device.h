@@ -119,6 +119,11 @@ struct Scene {
     // 8-wide scenes: the macro tree's root as wide_root reads it (a leaf
     // root's range without the load)
     uint32_t macro_word, macro_count;
+    // 8-wide scenes whose macro root is a leaf of one instance (a scene of one
+    // mesh instance): that instance's record (inst_recs), kept here so that
+    // every ray tests it without reading memory
+    uint32_t macro_single;
+    float4 macro_inst[5];
     uint32_t light_root;
     // 8-wide trees (see the wide traversal below); null for binary scenes
     const float4 *wnodes;
@@ -283,8 +288,7 @@ struct InstanceRec {
     uint32_t root, mi_index;
     float4 x0, x1, x2;
 };
-__device__ __forceinline__ InstanceRec load_instance(const Scene &sc, uint32_t k) {
-    const float4 *p = sc.inst_recs + (size_t)k * 5;
+__device__ __forceinline__ InstanceRec instance_rec(const float4 *p) {
     const float4 a = p[0], b = p[1];
     InstanceRec r;
     r.bmin[0] = a.x;
@@ -299,6 +303,9 @@ __device__ __forceinline__ InstanceRec load_instance(const Scene &sc, uint32_t k
     r.x1 = p[3];
     r.x2 = p[4];
     return r;
+}
+__device__ __forceinline__ InstanceRec load_instance(const Scene &sc, uint32_t k) {
+    return instance_rec(sc.inst_recs + (size_t)k * 5);
 }
 // TransformRay (CoreRef.cpp:1605-1617), the same products and sums in the same order
 __device__ __forceinline__ void transform_ray(const InstanceRec &r, f3 o, f3 d, f3 &mo, f3 &md) {
@@ -924,33 +931,42 @@ __device__ __forceinline__ void traverse_wide(const Scene &sc, f3 o, f3 d, Hit &
                                               bool first_hit = false) {
     const f3 inv_d = safe_invert(d);
     st.sp = 0;
+    // an instance of a macro leaf: true when the walk ends (an any-hit walk
+    // found a solid triangle)
+    auto instance = [&](const InstanceRec &mi) -> bool {
+        const uint32_t mi_index = mi.mi_index;
+        if (cnt) cnt->instances++;
+        if (!box_hit<true>(o, inv_d, hit.t, mi.bmin, mi.bmax)) return false;
+        f3 mo, md;
+        transform_ray(mi, o, d, mo, md);
+        const f3 minv = f3{1.0f / md.x, 1.0f / md.y, 1.0f / md.z};
+        const uint32_t root = mi.root;
+        const bool finite = __builtin_isfinite(minv.x) && __builtin_isfinite(minv.y) && __builtin_isfinite(minv.z);
+        const bool fh = AnyHit && first_hit;
+        const bool found = finite ? walk_micro_wide<true, G>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt, fh)
+                                  : walk_micro_wide<false, G>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt, fh);
+        if (AnyHit && found) {
+            const rh_tri_accel ht = load_tri(sc.tris, (uint32_t)hit.prim);
+            if (ht.ci & RH_TRI_SOLID_BIT) {
+                st.sp = 0;
+                return true;
+            }
+        }
+        return false;
+    };
+    if (!AnyHit && sc.macro_single) {
+        // the leaf root's one instance, then nothing is stacked (closest-hit
+        // walks only: in the any-hit walk the second copy of the instance
+        // walk costs the shadow kernel more registers than it saves)
+        instance(instance_rec(sc.macro_inst));
+        return;
+    }
     WideCur cur = WideCur{sc.macro_word, sc.macro_count};
     while (true) {
         if (cur.word & RH_LEAF_NODE_BIT) {
             const uint32_t first = cur.word & RH_PRIM_INDEX_BITS, end = first + cur.count;
-            for (uint32_t i = first; i < end; i++) {
-                const InstanceRec mi = load_instance(sc, i);
-                const uint32_t mi_index = mi.mi_index;
-                if (cnt) cnt->instances++;
-                if (!box_hit<true>(o, inv_d, hit.t, mi.bmin, mi.bmax)) continue;
-                f3 mo, md;
-                transform_ray(mi, o, d, mo, md);
-                const f3 minv = f3{1.0f / md.x, 1.0f / md.y, 1.0f / md.z};
-                const uint32_t root = mi.root;
-                const bool finite =
-                    __builtin_isfinite(minv.x) && __builtin_isfinite(minv.y) && __builtin_isfinite(minv.z);
-                const bool fh = AnyHit && first_hit;
-                const bool found =
-                    finite ? walk_micro_wide<true, G>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt, fh)
-                           : walk_micro_wide<false, G>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt, fh);
-                if (AnyHit && found) {
-                    const rh_tri_accel ht = load_tri(sc.tris, (uint32_t)hit.prim);
-                    if (ht.ci & RH_TRI_SOLID_BIT) {
-                        st.sp = 0;
-                        return;
-                    }
-                }
-            }
+            for (uint32_t i = first; i < end; i++)
+                if (instance(load_instance(sc, i))) return;
         } else {
             if (cnt) cnt->nodes++;
             if (wide_step_g<true, G>(sc.wnodes, cur.word, o, inv_d, hit.t, st, cur)) continue;

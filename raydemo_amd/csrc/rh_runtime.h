@@ -320,6 +320,14 @@ inline int build_device_instances(const rh_scene_view *v, hipStream_t stream, st
     }
     rc = upload_array(bufs, stream, recs.data(), recs.size(), &s.inst_recs);
     if (rc) return rc;
+    // a leaf macro root of one instance: its record in the scene record
+    s.macro_single = 0;
+    if (v->mnodes && s.macro_root != RH_INVALID && (s.macro_word & RH_LEAF_NODE_BIT) && s.macro_count == 1) {
+        const size_t first = s.macro_word & RH_PRIM_INDEX_BITS;
+        if (first >= v->mi_indices_count) return rh_rt::arg_error("macro leaf outside the instance list");
+        for (int k = 0; k < 5; k++) s.macro_inst[k] = recs[first * 5 + k];
+        s.macro_single = 1;
+    }
     if (!v->mnodes && s.light_root != RH_INVALID && v->nodes && s.light_root < v->nodes_count) {
         std::vector<uint32_t> st{s.light_root};
         while (!st.empty()) {
