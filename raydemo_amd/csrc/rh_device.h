@@ -124,6 +124,10 @@ struct Scene {
     // every ray tests it without reading memory
     uint32_t macro_single;
     float4 macro_inst[5];
+    // and when that instance's tree root is an interior node: the root's
+    // device node (its first step of every ray reads no memory)
+    uint32_t root_cached;
+    float4 root_node[16];
     uint32_t light_root;
     // 8-wide trees (see the wide traversal below); null for binary scenes
     const float4 *wnodes;
@@ -754,12 +758,15 @@ __device__ __forceinline__ uint32_t wide_order(const float (&dist)[8], const uin
     return top;
 }
 
-template <bool Finite>
-__device__ __forceinline__ bool wide_step(const float4 *wn, uint32_t node, f3 o, f3 inv_d, float t, Stack &st,
-                                          WideCur &next) {
+// wide_step on the node at p in the node array (node: the word naming it,
+// for its flags and the slot references it stacks), its slots read by
+// slot(S, a, bm) for a constant slot index S: from p, or from a copy the
+// caller holds
+template <bool Finite, class SlotRead>
+__device__ __forceinline__ bool wide_step_at(const float4 *p, SlotRead slot, uint32_t node, f3 o, f3 inv_d, float t,
+                                             Stack &st, WideCur &next) {
     const bool full = !(node & kWideHalf);
     node &= kWideIndex;
-    const float4 *p = wn + (size_t)node * 16;
     float dist[8];
     uint32_t word[8];
     uint32_t mask = 0;
@@ -769,10 +776,7 @@ __device__ __forceinline__ bool wide_step(const float4 *wn, uint32_t node, f3 o,
         // entered slot only, when it is a leaf (8 registers fewer in flight)
         float4 a = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(RH_MBVH_EMPTY));
         float3 bm = make_float3(0.0f, 0.0f, 0.0f);
-        if (s < 4 || full) {
-            a = p[2 * s];
-            bm = *reinterpret_cast<const float3 *>(p + 2 * s + 1);
-        }
+        if (s < 4 || full) slot(S, a, bm);
         const float4 b = make_float4(bm.x, bm.y, bm.z, 0.0f);
         word[s] = __float_as_uint(a.w);
         const bool h = slot_hit<Finite>(o, inv_d, t, a, b, dist[s]);
@@ -789,6 +793,18 @@ __device__ __forceinline__ bool wide_step(const float4 *wn, uint32_t node, f3 o,
         next.count = c ? c : __float_as_uint(reinterpret_cast<const float *>(p + 2 * top + 1)[3]);
     }
     return true;
+}
+
+template <bool Finite>
+__device__ __forceinline__ bool wide_step(const float4 *wn, uint32_t node, f3 o, f3 inv_d, float t, Stack &st,
+                                          WideCur &next) {
+    const float4 *p = wn + (size_t)(node & kWideIndex) * 16;
+    auto slot = [&](auto S, float4 &a, float3 &bm) {
+        constexpr int s = S;
+        a = p[2 * s];
+        bm = *reinterpret_cast<const float3 *>(p + 2 * s + 1);
+    };
+    return wide_step_at<Finite>(p, slot, node, o, inv_d, t, st, next);
 }
 
 // wide_step for a ray walked by G lanes (G = 2, 4 or 8, aligned lane groups of
@@ -894,10 +910,26 @@ __device__ __forceinline__ WideCur wide_root(const float4 *wn, uint32_t root_wor
 // depend on which blocking triangle is found (visibility queries only).
 template <bool Finite, int G = 1>
 __device__ __forceinline__ bool walk_micro_wide(const Scene &sc, uint32_t root, f3 mo, f3 md, f3 minv, int mi_index,
-                                                Hit &hit, Stack &st, TraceCounters *cnt, bool first_hit) {
+                                                Hit &hit, Stack &st, TraceCounters *cnt, bool first_hit,
+                                                const float4 *root_node = nullptr) {
     const uint32_t base = st.sp;
     WideCur cur = wide_root(sc.wnodes, root);
     bool res = false;
+    if (root_node && !(cur.word & RH_LEAF_NODE_BIT)) {
+        // the root's step from the copy of its node the caller holds (the
+        // same slots as the node array's: its index is read by constants)
+        if (cnt) cnt->nodes++;
+        auto slot = [&](auto S, float4 &a, float3 &bm) {
+            constexpr int s = S;
+            a = root_node[2 * s];
+            const float4 b = root_node[2 * s + 1];
+            bm = make_float3(b.x, b.y, b.z);
+        };
+        if (!wide_step_at<Finite>(sc.wnodes + (size_t)(cur.word & kWideIndex) * 16, slot, cur.word, mo, minv, hit.t,
+                                  st, cur) &&
+            !wide_pop(sc.wnodes, base, hit.t, st, cur, sc.wide_leaf_counts))
+            return false;
+    }
     while (true) {
         if (cur.word & RH_LEAF_NODE_BIT) {
             NodeBox leaf;
@@ -933,7 +965,7 @@ __device__ __forceinline__ void traverse_wide(const Scene &sc, f3 o, f3 d, Hit &
     st.sp = 0;
     // an instance of a macro leaf: true when the walk ends (an any-hit walk
     // found a solid triangle)
-    auto instance = [&](const InstanceRec &mi) -> bool {
+    auto instance = [&](const InstanceRec &mi, const float4 *root_node) -> bool {
         const uint32_t mi_index = mi.mi_index;
         if (cnt) cnt->instances++;
         if (!box_hit<true>(o, inv_d, hit.t, mi.bmin, mi.bmax)) return false;
@@ -943,8 +975,9 @@ __device__ __forceinline__ void traverse_wide(const Scene &sc, f3 o, f3 d, Hit &
         const uint32_t root = mi.root;
         const bool finite = __builtin_isfinite(minv.x) && __builtin_isfinite(minv.y) && __builtin_isfinite(minv.z);
         const bool fh = AnyHit && first_hit;
-        const bool found = finite ? walk_micro_wide<true, G>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt, fh)
-                                  : walk_micro_wide<false, G>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt, fh);
+        const bool found =
+            finite ? walk_micro_wide<true, G>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt, fh, root_node)
+                   : walk_micro_wide<false, G>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt, fh);
         if (AnyHit && found) {
             const rh_tri_accel ht = load_tri(sc.tris, (uint32_t)hit.prim);
             if (ht.ci & RH_TRI_SOLID_BIT) {
@@ -958,7 +991,7 @@ __device__ __forceinline__ void traverse_wide(const Scene &sc, f3 o, f3 d, Hit &
         // the leaf root's one instance, then nothing is stacked (closest-hit
         // walks only: in the any-hit walk the second copy of the instance
         // walk costs the shadow kernel more registers than it saves)
-        instance(instance_rec(sc.macro_inst));
+        instance(instance_rec(sc.macro_inst), (G == 1 && sc.root_cached) ? sc.root_node : nullptr);
         return;
     }
     WideCur cur = WideCur{sc.macro_word, sc.macro_count};
@@ -966,7 +999,7 @@ __device__ __forceinline__ void traverse_wide(const Scene &sc, f3 o, f3 d, Hit &
         if (cur.word & RH_LEAF_NODE_BIT) {
             const uint32_t first = cur.word & RH_PRIM_INDEX_BITS, end = first + cur.count;
             for (uint32_t i = first; i < end; i++)
-                if (instance(load_instance(sc, i))) return;
+                if (instance(load_instance(sc, i), nullptr)) return;
         } else {
             if (cnt) cnt->nodes++;
             if (wide_step_g<true, G>(sc.wnodes, cur.word, o, inv_d, hit.t, st, cur)) continue;

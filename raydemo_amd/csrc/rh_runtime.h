@@ -121,7 +121,7 @@ int upload_array(std::vector<rh_rt::DevBuf> &bufs, hipStream_t stream, const T *
 // it), so testing the leaf boxes alone, in this order, takes the same lights
 // in the same order; a leaf root has no box test at all.
 inline int build_wide(const rh_scene_view *v, hipStream_t stream, std::vector<rh_rt::DevBuf> &bufs, Scene &s,
-                      std::vector<rh_bvh_node> &leaves, std::vector<rh_mesh> &meshes) {
+                      std::vector<rh_bvh_node> &leaves, std::vector<rh_mesh> &meshes, std::vector<float4> &nodes_out) {
     const uint32_t n = v->mnodes_count;
     auto is_leaf = [&](uint32_t i) { return (v->mnodes[i].child[0] & RH_LEAF_NODE_BIT) != 0; };
     // device nodes: the interior nodes and the tree roots (a leaf child lives in its parent's slot)
@@ -194,6 +194,7 @@ inline int build_wide(const rh_scene_view *v, hipStream_t stream, std::vector<rh
     }
     int rc = upload_array(bufs, stream, w.data(), w.size(), &s.wnodes);
     if (rc) return rc;
+    nodes_out.swap(w); // the same buffer: it outlives the upload (the caller synchronises)
     // a tree root is named by its node's index, flagged when it is a leaf
     auto root_word = [&](uint32_t i) { return dev[i] | (is_leaf(i) ? kWideLeafRoot : half_flag(i)); };
     meshes.assign(v->meshes, v->meshes + v->meshes_count);
@@ -293,8 +294,9 @@ inline int build_device_instances(const rh_scene_view *v, hipStream_t stream, st
     s.light_root = v->light_nodes_root;
     std::vector<rh_bvh_node> leaves;
     std::vector<rh_mesh> meshes; // as the device walks name their roots
+    std::vector<float4> wide_nodes; // host copy of the device wide nodes
     if (v->mnodes) {
-        rc = build_wide(v, stream, bufs, s, leaves, meshes);
+        rc = build_wide(v, stream, bufs, s, leaves, meshes, wide_nodes);
         if (rc) return rc;
     } else {
         meshes.assign(v->meshes, v->meshes + v->meshes_count);
@@ -327,6 +329,19 @@ inline int build_device_instances(const rh_scene_view *v, hipStream_t stream, st
         if (first >= v->mi_indices_count) return rh_rt::arg_error("macro leaf outside the instance list");
         for (int k = 0; k < 5; k++) s.macro_inst[k] = recs[first * 5 + k];
         s.macro_single = 1;
+    }
+    // and its tree root's device node, when the root is interior
+    auto root_of = [](float f) {
+        uint32_t u;
+        memcpy(&u, &f, 4);
+        return u;
+    };
+    s.root_cached = 0;
+    if (s.macro_single && !(root_of(s.macro_inst[0].w) & kWideLeafRoot)) {
+        const uint32_t idx = root_of(s.macro_inst[0].w) & kWideIndex;
+        if ((size_t)idx >= wide_nodes.size() / 16) return rh_rt::arg_error("instance tree root outside the wide nodes");
+        for (int k = 0; k < 16; k++) s.root_node[k] = wide_nodes[(size_t)idx * 16 + k];
+        s.root_cached = 1;
     }
     if (!v->mnodes && s.light_root != RH_INVALID && v->nodes && s.light_root < v->nodes_count) {
         std::vector<uint32_t> st{s.light_root};
