@@ -94,8 +94,9 @@ struct Scene {
     const rh_bvh_node *nodes;
     const rh_tri_accel *tris;
     const uint32_t *tri_indices;
-    // tris in tri_indices order (leaf_tris[k] = tris[tri_indices[k]]): a
-    // leaf's triangle is loaded beside its index instead of after it
+    // tris in tri_indices order (leaf_tris[k] = tris[tri_indices[k]], with
+    // the material field mi holding the triangle index tri_indices[k] instead:
+    // the leaf test needs no material): one load round per triangle test
     const rh_tri_accel *leaf_tris;
     const rh_mesh *meshes;
     const rh_mesh_instance *mesh_instances;
@@ -424,8 +425,8 @@ __device__ __forceinline__ bool intersect_leaf(const Scene &sc, const NodeBox &n
     bool leaf_hit = false;
     int hit_ci = 0; // flags of the leaf's latest hit triangle
     for (uint32_t k = first; k < first + count; k++) {
-        const uint32_t ti = sc.tri_indices[k];
         const rh_tri_accel tri = load_tri(sc.leaf_tris, k);
+        const uint32_t ti = tri.mi; // the triangle's index (leaf_tris)
         if (cnt) cnt->leaf_tris++;
         float t, u, v;
         if (intersect_tri(o, d, tri, hit.t, t, u, v)) {

@@ -252,6 +252,7 @@ inline int build_device_static(const rh_scene_view *v, hipStream_t stream, std::
         for (size_t k = 0; k < v->tri_indices_count; k++) {
             if (v->tri_indices[k] >= v->tris_count) return rh_rt::arg_error("triangle index outside the triangle array");
             leaf_tris[k] = v->tris[v->tri_indices[k]];
+            leaf_tris[k].mi = v->tri_indices[k]; // the leaf test's hit index in place of the material
         }
         rc |= upload_array(bufs, stream, leaf_tris.data(), leaf_tris.size(), &s.leaf_tris);
         // the upload is asynchronous: the staging vector must outlive it
