@@ -980,6 +980,11 @@ __device__ __forceinline__ void traverse_wide(const Scene &sc, f3 o, f3 d, Hit &
             finite ? walk_micro_wide<true, G>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt, fh, root_node)
                    : walk_micro_wide<false, G>(sc, root, mo, md, minv, (int)mi_index, hit, st, cnt, fh);
         if (AnyHit && found) {
+            // first_hit (every triangle solid): the walk found a solid one
+            if (fh) {
+                st.sp = 0;
+                return true;
+            }
             const rh_tri_accel ht = load_tri(sc.tris, (uint32_t)hit.prim);
             if (ht.ci & RH_TRI_SOLID_BIT) {
                 st.sp = 0;

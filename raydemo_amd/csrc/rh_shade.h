@@ -43,6 +43,11 @@ __device__ __forceinline__ float compute_visibility(const Scene &sc, f3 p1, f3 p
         // all-solid scene: the first blocking triangle found decides (visibility 0)
         if (sc.macro_root != RH_INVALID) traverse<true, Tree>(sc, ro, dir, sh, st, tc, sc.all_solid != 0);
         if (!sh.mask) break;
+        if (sc.all_solid) {
+            // every triangle is solid: the one found blocks (no load to see it)
+            visibility = 0;
+            break;
+        }
 
         const rh_tri_accel tri = load_tri(sc.tris, (uint32_t)sh.prim);
         if (tri.ci & RH_TRI_SOLID_BIT) {
