@@ -671,10 +671,11 @@ __global__ void k_set_material_colors(rh_material *mats, const float *__restrict
     m.main_color[2] = rgb[3 * k + 2];
 }
 
-template <int Tree, int MinBlocks = 1, bool Count = false>
+// Solid: for scenes whose triangles are all solid (compute_visibility)
+template <int Tree, int MinBlocks = 1, bool Count = false, bool Solid = false>
 __global__ void __launch_bounds__(kBlock, MinBlocks) k_shadow(ShadowArgs a) {
     extern __shared__ uint32_t lds_stack[];
-    shadow_body<Tree, Count>(a, blockIdx.x * blockDim.x, lds_stack);
+    shadow_body<Tree, Count, Solid>(a, blockIdx.x * blockDim.x, lds_stack);
 }
 
 __global__ void __launch_bounds__(kBlock) k_nee_combine(CombineArgs a) { combine_body(a, blockIdx.x * blockDim.x); }
@@ -1274,11 +1275,19 @@ int launch_shade(rh_context *c, const ShadeArgs &a, int grid_rays) {
             hipLaunchKernelGGL((k_shadow<kTreeWide, 1, true>), grid, blk, lds_bytes(c), c->stream, s);
         else if (count)
             hipLaunchKernelGGL((k_shadow<kTreeBinary, 1, true>), grid, blk, lds_bytes(c), c->stream, s);
+        else if (c->sc.wnodes && c->sc.all_solid && occ == 5)
+            hipLaunchKernelGGL((k_shadow<kTreeWide, 5, false, true>), grid, blk, lds_bytes(c), c->stream, s);
+        else if (c->sc.wnodes && c->sc.all_solid && occ == 6)
+            hipLaunchKernelGGL((k_shadow<kTreeWide, 6, false, true>), grid, blk, lds_bytes(c), c->stream, s);
+        else if (c->sc.wnodes && c->sc.all_solid)
+            hipLaunchKernelGGL((k_shadow<kTreeWide, 1, false, true>), grid, blk, lds_bytes(c), c->stream, s);
         else if (c->sc.wnodes && occ == 5)
             hipLaunchKernelGGL((k_shadow<kTreeWide, 5>), grid, blk, lds_bytes(c), c->stream, s);
         else if (c->sc.wnodes && occ == 6)
             hipLaunchKernelGGL((k_shadow<kTreeWide, 6>), grid, blk, lds_bytes(c), c->stream, s);
         else if (c->sc.wnodes) hipLaunchKernelGGL(k_shadow<kTreeWide>, grid, blk, lds_bytes(c), c->stream, s);
+        else if (c->sc.all_solid)
+            hipLaunchKernelGGL((k_shadow<kTreeBinary, 1, false, true>), grid, blk, lds_bytes(c), c->stream, s);
         else hipLaunchKernelGGL(k_shadow<kTreeBinary>, grid, blk, lds_bytes(c), c->stream, s);
         RH_CHECK(hipGetLastError());
     }

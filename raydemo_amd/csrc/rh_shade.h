@@ -22,7 +22,10 @@ struct ShadeCounters {
 // ComputeVisibility (CoreRef.cpp:1801-1896): steps through transparent
 // occluders, resolving mix materials with the lane's random offset; tc (when
 // given) counts the traversal's node visits, triangle and instance tests
-template <int Tree = kTreeAny>
+// Solid: the scene's triangles are all solid (Scene::all_solid), so the
+// first triangle found decides and no transparent surface is stepped through
+// (the same visibility; the kernel holds none of that path's registers).
+template <int Tree = kTreeAny, bool Solid = false>
 __device__ __forceinline__ float compute_visibility(const Scene &sc, f3 p1, f3 p2, const float *halton, int hi,
                                                     int rand_hash2, Stack &st, ShadeCounters &cnt,
                                                     TraceCounters *tc = nullptr) {
@@ -31,6 +34,20 @@ __device__ __forceinline__ float compute_visibility(const Scene &sc, f3 p1, f3 p
     dir = dir / dist;
     f3 ro = p1;
     float visibility = 1.0f;
+    if constexpr (Solid) {
+        if (dist > RH_HIT_EPS) {
+            Hit sh;
+            sh.mask = 0;
+            sh.obj = -1;
+            sh.prim = -1;
+            sh.t = dist;
+            sh.u = sh.v = 0.0f;
+            cnt.shadow_rays++;
+            if (sc.macro_root != RH_INVALID) traverse<true, Tree>(sc, ro, dir, sh, st, tc, true);
+            if (sh.mask) visibility = 0;
+        }
+        return visibility;
+    }
 
     while (dist > RH_HIT_EPS) {
         Hit sh;

@@ -953,7 +953,7 @@ struct ShadowArgs {
 // (CoreRef.cpp:1801-1896), any-hit traversal stepping through transparent
 // surfaces.  One lane per queue entry; the grid covers every slot and lanes
 // past the device count exit at once.
-template <int Tree = kTreeAny, bool Count = false>
+template <int Tree = kTreeAny, bool Count = false, bool Solid = false>
 __device__ __forceinline__ void shadow_body(const ShadowArgs &a, uint32_t base, uint32_t *lds_stack) {
     const int n = *a.count;
     const uint32_t q = base + threadIdx.x;
@@ -970,7 +970,8 @@ __device__ __forceinline__ void shadow_body(const ShadowArgs &a, uint32_t base, 
         Stack st;
         st.init(lds_stack, a.spill + q, a.spill_stride, a.lds_depth, kSpillDepth);
         const float vis =
-            compute_visibility<Tree>(a.sc, f3{r.x, r.y, r.z}, ld3(l.pos), a.halton, x.x, x.y, st, cnt, Count ? &tc : nullptr);
+            compute_visibility<Tree, Solid>(a.sc, f3{r.x, r.y, r.z}, ld3(l.pos), a.halton, x.x, x.y, st, cnt,
+                                            Count ? &tc : nullptr);
         a.nee_vis[k] = vis != 0.0f ? 1 : 0;
         if (st.overflow && a.overflow) atomicAdd(a.overflow, (unsigned long long)st.overflow);
     }
