@@ -336,10 +336,11 @@ template <bool Defer> __global__ void __launch_bounds__(kBlock) k_shade(ShadeArg
 // SIMD).  Which is faster is measured (tail_occupancy).
 // Count: the instrumented form (traversal visits into a.visits), for the
 // roofline's algorithmic bytes.
-template <bool Defer, int MinBlocks, int Tree, bool Count = false>
+// Solid: for scenes whose triangles are all solid (compute_visibility).
+template <bool Defer, int MinBlocks, int Tree, bool Count = false, bool Solid = false>
 __global__ void __launch_bounds__(kBlock, MinBlocks) k_path_tail(ShadeArgs a, int first, int last, int *next_path) {
     extern __shared__ uint32_t lds_stack[];
-    path_tail_body<Defer, Tree, Count>(a, first, last, blockIdx.x * blockDim.x, lds_stack, next_path);
+    path_tail_body<Defer, Tree, Count, Solid>(a, first, last, blockIdx.x * blockDim.x, lds_stack, next_path);
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -1804,9 +1805,16 @@ int run_pass(rh_context *c, const rh_pass_desc *p, const AdjRecords *rec, bool k
             const size_t lds = lds_bytes(c);
             const bool occ3 = c->tail_occupancy == 3;
             const void *kern;
-            if (c->sc.wnodes)
+            const bool solid = c->sc.all_solid && tb.nee_ray; // the lane's light samples resolved by first hit
+            if (c->sc.wnodes && solid)
+                kern = occ3 ? (const void *)k_path_tail<true, 3, kTreeWide, false, true>
+                            : (const void *)k_path_tail<true, 2, kTreeWide, false, true>;
+            else if (c->sc.wnodes)
                 kern = tb.nee_ray ? (occ3 ? (const void *)k_path_tail<true, 3, kTreeWide> : (const void *)k_path_tail<true, 2, kTreeWide>)
                                   : (occ3 ? (const void *)k_path_tail<false, 3, kTreeWide> : (const void *)k_path_tail<false, 2, kTreeWide>);
+            else if (solid)
+                kern = occ3 ? (const void *)k_path_tail<true, 3, kTreeBinary, false, true>
+                            : (const void *)k_path_tail<true, 2, kTreeBinary, false, true>;
             else
                 kern = tb.nee_ray ? (occ3 ? (const void *)k_path_tail<true, 3, kTreeBinary> : (const void *)k_path_tail<true, 2, kTreeBinary>)
                                   : (occ3 ? (const void *)k_path_tail<false, 3, kTreeBinary> : (const void *)k_path_tail<false, 2, kTreeBinary>);

@@ -561,7 +561,7 @@ __device__ __forceinline__ void shade_surface(const Scene &sc, const rh_pass_inf
 // terms summed in light order, times the albedo when considered, times the
 // ray's throughput -- the arithmetic of the separate shadow and combine
 // stages, so adding the result after out.col gives the same pixel value.
-template <int Tree = kTreeAny>
+template <int Tree = kTreeAny, bool Solid = false>
 __device__ __forceinline__ f3 resolve_light_samples(const Scene &sc, const float *halton, const NeeSink &sink,
                                                     const ShadeOut &out, f3 throughput, Stack &st,
                                                     ShadeCounters &cnt, TraceCounters *tc = nullptr) {
@@ -572,7 +572,7 @@ __device__ __forceinline__ f3 resolve_light_samples(const Scene &sc, const float
         const int2 x = sink.aux[k];
         const rh_light &l = sc.lights[__float_as_uint(r.w)];
         st.sp = 0;
-        if (compute_visibility<Tree>(sc, f3{r.x, r.y, r.z}, ld3(l.pos), halton, x.x, x.y, st, cnt, tc) != 0.0f) {
+        if (compute_visibility<Tree, Solid>(sc, f3{r.x, r.y, r.z}, ld3(l.pos), halton, x.x, x.y, st, cnt, tc) != 0.0f) {
             const float4 t = sink.term[k];
             col = col + f3{t.x, t.y, t.z};
         }

@@ -835,7 +835,7 @@ __device__ __forceinline__ void shade_body(const ShadeArgs &a, uint32_t base, ui
 // values in the same order.  Without refill each lane follows the one path
 // of its index.  A lane's stack spill slice belongs to its grid position; the
 // NEE slots of a path belong to its queue index.
-template <bool Defer, int Tree = kTreeAny, bool Count = false>
+template <bool Defer, int Tree = kTreeAny, bool Count = false, bool Solid = false>
 __device__ __forceinline__ void path_tail_body(const ShadeArgs &a, int first, int last, uint32_t base,
                                                uint32_t *lds_stack, int *next_path = nullptr) {
     const int n = *a.count;
@@ -910,7 +910,7 @@ __device__ __forceinline__ void path_tail_body(const ShadeArgs &a, int first, in
         v.w += 0.0f;
         dst = v;
         if (Defer && so.nee_pending && sink.count > 0) {
-            const f3 add = resolve_light_samples<Tree>(a.sc, a.halton, sink, so, ray.c, st, cnt, tcp);
+            const f3 add = resolve_light_samples<Tree, Solid>(a.sc, a.halton, sink, so, ray.c, st, cnt, tcp);
             float4 w = dst;
             w.x += add.x;
             w.y += add.y;
