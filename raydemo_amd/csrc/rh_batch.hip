@@ -310,7 +310,8 @@ template <bool Defer> __global__ void __launch_bounds__(kBlock) k_b_shade(BatchD
                   });
 }
 
-template <int Tree> __global__ void __launch_bounds__(kBlock) k_b_shadow(BatchDev b, int stage, int work) {
+// Solid: every scene of the batch has solid triangles only (Scene::all_solid)
+template <int Tree, bool Solid> __global__ void __launch_bounds__(kBlock) k_b_shadow(BatchDev b, int stage, int work) {
     extern __shared__ uint32_t lds_stack[];
     const int *counts = b.shadow_counts + stage * b.n_scenes;
     for_each_wave_tile(b, counts, b.work + work, [&](int s) { return b.table[s].nee_slots != 0; },
@@ -330,7 +331,7 @@ template <int Tree> __global__ void __launch_bounds__(kBlock) k_b_shadow(BatchDe
                       a.shadow_counter = b.stats + 2;
                       a.overflow = b.stats + 4;
                       a.visits = nullptr;
-                      shadow_body<Tree>(a, base, lds_stack);
+                      shadow_body<Tree, false, Solid>(a, base, lds_stack);
                   });
 }
 
@@ -501,8 +502,9 @@ __global__ void __launch_bounds__(kBlock) k_b_sort_gather(BatchDev b, int stage,
 }
 
 // the rays entering `stage` follow their paths to each scene's depth in one
-// lane; Defer: scenes whose light samples are resolved after shading (1..8 lights)
-template <bool Defer, int Tree>
+// lane; Defer: scenes whose light samples are resolved after shading (1..8 lights);
+// Solid: every scene of the batch has solid triangles only
+template <bool Defer, int Tree, bool Solid>
 __global__ void __launch_bounds__(kBlock) k_b_tail(BatchDev b, int stage, int q, int work) {
     extern __shared__ uint32_t lds_stack[];
     const int *counts = b.counts + stage * b.n_scenes;
@@ -511,7 +513,7 @@ __global__ void __launch_bounds__(kBlock) k_b_tail(BatchDev b, int stage, int q,
                   [&](int s, uint32_t base) {
                       const BatchEntry e = entry(b, s);
                       const ShadeArgs a = batch_shade_args(b, e, s, stage, q, Defer);
-                      path_tail_body<Defer, Tree>(a, stage - 1, e.depth, base, lds_stack);
+                      path_tail_body<Defer, Tree, false, Solid>(a, stage - 1, e.depth, base, lds_stack);
                   });
 }
 
@@ -706,6 +708,12 @@ bool any_wide(const rh_batch *b) {
     return false;
 }
 
+bool batch_all_solid(const rh_batch *b) {
+    for (const BatchEntry &e : b->entries)
+        if (!e.sc.all_solid) return false;
+    return true;
+}
+
 int batch_tree(const rh_batch *b) {
     int wide = 0;
     for (const BatchEntry &e : b->entries) wide += e.sc.wnodes ? 1 : 0;
@@ -715,13 +723,13 @@ int batch_tree(const rh_batch *b) {
 // The per-bounce launches of a pass: stage 0 is the primary hit; stage k >= 1
 // traces bounce k - 1 (its queue first sorted per scene when sort_bits is
 // set).  Bounces from path_tail on run in the tail kernel.
-template <int Tree>
+template <int Tree, bool Solid>
 void launch_stages(rh_batch *b, const BatchDev &d, size_t lds, dim3 grid, dim3 blk, int tail, int &work) {
     int q = 0;
     for (int stage = 0; stage <= b->max_depth; stage++) {
         if (stage >= tail) {
-            hipLaunchKernelGGL((k_b_tail<true, Tree>), grid, blk, lds, b->stream, d, stage, q, work++);
-            hipLaunchKernelGGL((k_b_tail<false, Tree>), grid, blk, lds, b->stream, d, stage, q, work++);
+            hipLaunchKernelGGL((k_b_tail<true, Tree, Solid>), grid, blk, lds, b->stream, d, stage, q, work++);
+            hipLaunchKernelGGL((k_b_tail<false, Tree, false>), grid, blk, lds, b->stream, d, stage, q, work++);
             break;
         }
         if (stage >= 1 && b->sort_bits) {
@@ -739,7 +747,7 @@ void launch_stages(rh_batch *b, const BatchDev &d, size_t lds, dim3 grid, dim3 b
         hipLaunchKernelGGL(k_b_trace<Tree>, grid, blk, lds, b->stream, d, stage, q, work++);
         hipLaunchKernelGGL(k_b_shade<true>, grid, blk, 0, b->stream, d, stage, q, work++);
         hipLaunchKernelGGL(k_b_shade<false>, grid, blk, lds, b->stream, d, stage, q, work++);
-        hipLaunchKernelGGL(k_b_shadow<Tree>, grid, blk, lds, b->stream, d, stage, work++);
+        hipLaunchKernelGGL((k_b_shadow<Tree, Solid>), grid, blk, lds, b->stream, d, stage, work++);
         hipLaunchKernelGGL(k_b_combine, grid, blk, 0, b->stream, d, stage, work++);
         q ^= 1;
     }
@@ -933,15 +941,17 @@ int enqueue_pass(rh_batch *b, int stride, int *mix_work) {
     int work = 0;
     hipLaunchKernelGGL(k_b_begin_pass, dim3(1), dim3(1024), 0, b->stream, all, b->table, (int)work_counters(b), stride);
     const int tree = batch_tree(b);
+    const bool solid = batch_all_solid(b);
     // the groups one after another: each fills the queues with its scenes' rays
     for (const auto &g : b->groups) {
         BatchDev d = all;
         d.s_begin = g.first;
         d.s_end = g.second;
         hipLaunchKernelGGL(k_b_gen, grid, blk, 0, b->stream, d, work++);
-        if (tree == kTreeBinary) launch_stages<kTreeBinary>(b, d, lds, grid, blk, tail, work);
-        else if (tree == kTreeWide) launch_stages<kTreeWide>(b, d, lds, grid, blk, tail, work);
-        else launch_stages<kTreeAny>(b, d, lds, grid, blk, tail, work);
+        if (tree == kTreeWide && solid) launch_stages<kTreeWide, true>(b, d, lds, grid, blk, tail, work);
+        else if (tree == kTreeBinary) launch_stages<kTreeBinary, false>(b, d, lds, grid, blk, tail, work);
+        else if (tree == kTreeWide) launch_stages<kTreeWide, false>(b, d, lds, grid, blk, tail, work);
+        else launch_stages<kTreeAny, false>(b, d, lds, grid, blk, tail, work);
     }
     *mix_work = work++;
     RH_CHECK(hipGetLastError());
