@@ -671,7 +671,11 @@ def run_large(args, lib, A, scenes, knobs, wide):
         base_b = dict(args.large_batch_knobs or {})
         for cfg in (base_b, dict(base_b, path_tail=-1), dict(base_b, path_tail=-1, blocks_per_cu=5, lds_stack_depth=16),
                     dict(base_b, path_tail=-1, blocks_per_cu=8), dict(base_b, path_tail=8, blocks_per_cu=5),
-                    dict(base_b, path_tail=-1, lds_stack_depth=8)):
+                    dict(base_b, path_tail=-1, lds_stack_depth=8),
+                    # per-scene queue sort (the 4K scene's bounces gain from coherent rays)
+                    dict(base_b, path_tail=-1, lds_stack_depth=16, sort_bits=24),
+                    dict(base_b, path_tail=-1, lds_stack_depth=16, sort_bits=24, blocks_per_cu=5),
+                    dict(base_b, path_tail=-1, lds_stack_depth=16, sort_bits=16)):
             for k, v in cfg.items():
                 b.set_option(k, v)
             b.restart()
