@@ -675,7 +675,8 @@ def run_large(args, lib, A, scenes, knobs, wide):
                     # per-scene queue sort (the 4K scene's bounces gain from coherent rays)
                     dict(base_b, path_tail=-1, lds_stack_depth=16, sort_bits=24),
                     dict(base_b, path_tail=-1, lds_stack_depth=16, sort_bits=24, blocks_per_cu=5),
-                    dict(base_b, path_tail=-1, lds_stack_depth=16, sort_bits=16)):
+                    dict(base_b, path_tail=-1, lds_stack_depth=16, sort_bits=16),
+                    dict(base_b, path_tail=-1, lds_stack_depth=16, sort_bits=24, blocks_per_cu=8, pass_overlap=3)):
             for k, v in cfg.items():
                 b.set_option(k, v)
             b.restart()
